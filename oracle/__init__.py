@@ -1,0 +1,175 @@
+"""ctypes binding of the CPU oracle (test infrastructure only).
+
+The oracle is a scalar C++ restatement of the reference's CPU mesh stage
+(see oracle.hpp). Only the tests, ``__graft_entry__.smoke`` and the CPU
+baseline leg of ``bench.py`` use it, as the checker; the product package
+``sesmesh_amd`` never imports it.
+"""
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+_c_p = ctypes.c_void_p
+_lib = None
+
+
+def build(force=False):
+    """Compiles liboracle.so with the Makefile beside this file."""
+    if force or not LIB.exists():
+        subprocess.run(["make", "-s", "-B" if force else "-s", "-C", str(HERE)], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = ctypes.CDLL(str(LIB))
+        L.orc_surface_new.restype = _c_p
+        L.orc_surface_new.argtypes = [_c_p, ctypes.c_size_t, ctypes.c_double, ctypes.c_double, ctypes.c_size_t,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
+        L.orc_surface_free.argtypes = [_c_p]
+        L.orc_field_sample.restype = ctypes.c_double
+        L.orc_time_mesh_sample.restype = ctypes.c_double
+        L.orc_time_mesh_sample.argtypes = [_c_p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, _c_p]
+        L.orc_csr.restype = ctypes.c_size_t
+        for name in ("orc_surface_counts", "orc_surface_params", "orc_surface_times", "orc_surface_bricks",
+                     "orc_surface_mesh", "orc_surface_measure", "orc_brick_values", "orc_sub_states",
+                     "orc_uniform_sign", "orc_arrangement", "orc_circles", "orc_sas_vertices"):
+            getattr(L, name).restype = None
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_c_p) if a is not None else None
+
+
+class Surface:
+    """The oracle's pipeline on one sphere set (`stage` 1: field and engine,
+    2: plus brick classification, 3: plus meshing and assembly)."""
+
+    def __init__(self, xyzr, probe=1.4, spacing=0.5, brick=10, threads=None, stage=3):
+        L = lib()
+        a = np.ascontiguousarray(xyzr, dtype=np.float64).reshape(-1, 4)
+        self.xyzr = a
+        if threads is None:
+            threads = min(8, os.cpu_count() or 1)
+        err = ctypes.create_string_buffer(512)
+        h = L.orc_surface_new(_ptr(a), len(a), probe, spacing, brick, threads, stage, err, 512)
+        if not h:
+            raise ValueError(err.value.decode())
+        self._h = h
+        self.probe, self.spacing, self.brick = probe, spacing, brick
+        c = (ctypes.c_int64 * 14)()
+        L.orc_surface_counts(_c_p(h), c)
+        (self.touched, self.n_bricks, self.n_vertices, self.n_triangles, self.evals, self.n_spheres,
+         self.n_circles, self.n_sas_vertices, self.n_contained, self.n_exposed, f32_ok, fx, fy, fz) = list(c)
+        self.f32_ok = bool(f32_ok)
+        self.frame = (fx, fy, fz)
+        p = (ctypes.c_double * 6)()
+        L.orc_surface_params(_c_p(h), p)
+        self.band, self.coord_max, self.edge_clamp, self.rho_max, self.max_expanded, self.extent = list(p)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().orc_surface_free(_c_p(h))
+            self._h = None
+
+    @property
+    def handle(self):
+        return _c_p(self._h)
+
+    def bricks(self):
+        out = np.zeros((self.n_bricks, 3), dtype=np.int32)
+        lib().orc_surface_bricks(self.handle, _ptr(out))
+        return out
+
+    def mesh(self):
+        nv, nt = self.n_vertices, self.n_triangles
+        pos = np.zeros((nv, 3), np.float32)
+        nrm = np.zeros((nv, 3), np.float32)
+        atoms = np.zeros(nv, np.uint32)
+        patches = np.zeros(nv, np.uint8)
+        tris = np.zeros((nt, 3), np.uint32)
+        t = np.zeros(nv, np.float64)
+        ev = np.zeros(nv, np.uint32)
+        lib().orc_surface_mesh(self.handle, _ptr(pos), _ptr(nrm), _ptr(atoms), _ptr(patches), _ptr(tris), _ptr(t),
+                               _ptr(ev))
+        return {"positions": pos, "normals": nrm, "atoms": atoms, "patches": patches, "triangles": tris, "t": t,
+                "evals": ev}
+
+    def measure(self):
+        m = (ctypes.c_double * 4)()
+        lib().orc_surface_measure(self.handle, m)
+        return {"components": int(m[0]), "cavities": int(m[1]), "area": m[2], "volume": m[3]}
+
+    def times(self):
+        m = (ctypes.c_double * 4)()
+        lib().orc_surface_times(self.handle, m)
+        return dict(zip(("geometry", "classify", "mesh", "assemble"), list(m)))
+
+    def brick_values(self, keys):
+        keys = np.ascontiguousarray(keys, dtype=np.int32).reshape(-1, 3)
+        side = self.brick + 1
+        out = np.zeros((len(keys), side ** 3), np.float32)
+        lib().orc_brick_values(self.handle, _ptr(keys), ctypes.c_size_t(len(keys)), _ptr(out))
+        return out
+
+    def sub_states(self, keys):
+        keys = np.ascontiguousarray(keys, dtype=np.int32).reshape(-1, 3)
+        out = np.zeros((len(keys), 8), np.uint8)
+        lib().orc_sub_states(self.handle, _ptr(keys), ctypes.c_size_t(len(keys)), _ptr(out))
+        return out
+
+    def uniform_sign(self, centers, rho):
+        c = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(len(c), np.uint8)
+        lib().orc_uniform_sign(self.handle, _ptr(c), ctypes.c_size_t(len(c)), ctypes.c_double(rho), _ptr(out))
+        return out
+
+    def sample(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        closest = np.zeros(3, np.float64)
+        feat = np.zeros(2, np.uint32)
+        v = lib().orc_field_sample(self.handle, _ptr(x), _ptr(closest), _ptr(feat))
+        return v, closest, (int(feat[0]), int(feat[1]))
+
+    def arrangement(self):
+        n = self.n_spheres
+        c = np.zeros((n, 3)); e = np.zeros(n); r = np.zeros(n)
+        st = np.zeros(n, np.uint8); orig = np.zeros(n, np.uint32)
+        lib().orc_arrangement(self.handle, _ptr(c), _ptr(e), _ptr(r), _ptr(st), _ptr(orig))
+        out = {"centers": c, "expanded": e, "radii": r, "status": st, "orig": orig}
+        for which, name in ((0, "neighbors"), (1, "circle_blockers")):
+            rows = n if which == 0 else self.n_circles
+            m = lib().orc_csr(self.handle, which, None, None)
+            off = np.zeros(rows + 1, np.uint32); items = np.zeros(m, np.uint32)
+            lib().orc_csr(self.handle, which, _ptr(off), _ptr(items))
+            out[name] = (off, items)
+        ij = np.zeros((self.n_circles, 2), np.uint32); geo = np.zeros((self.n_circles, 10))
+        lib().orc_circles(self.handle, _ptr(ij), _ptr(geo))
+        out["circles_ij"], out["circles"] = ij, geo
+        vp = np.zeros((self.n_sas_vertices, 3)); va = np.zeros((self.n_sas_vertices, 3), np.uint32)
+        lib().orc_sas_vertices(self.handle, _ptr(vp), _ptr(va))
+        out["vertices"], out["vertex_atoms"] = vp, va
+        return out
+
+    def time_mesh_sample(self, first, count, threads):
+        v = ctypes.c_uint64()
+        s = lib().orc_time_mesh_sample(self.handle, first, count, threads, ctypes.byref(v))
+        return s, v.value
+
+
+def mc_table():
+    off = np.zeros(257, np.uint16)
+    tris = np.zeros(3 * 1024, np.uint8)
+    lib().orc_mc_table(_ptr(off), _ptr(tris), ctypes.c_size_t(1024))
+    return off, tris[:3 * int(off[256])].reshape(-1, 3)

@@ -1,0 +1,255 @@
+// Oracle C API for the Python tests and the CPU baseline of bench.py (test
+// infrastructure only; the product never links this library).
+#include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <thread>
+
+#include "oracle.hpp"
+
+using namespace orc;
+
+namespace {
+void set_err(char* err, size_t len, const std::string& s) {
+    if (err && len) {
+        std::snprintf(err, len, "%s", s.c_str());
+    }
+}
+}  // namespace
+
+extern "C" {
+
+// Builds the field and engine, then (stage >= 2) classifies bricks and
+// (stage >= 3) meshes them with `threads` threads. Returns NULL on error.
+void* orc_surface_new(const double* xyzr, size_t n, double probe, double spacing, size_t brick, int threads,
+                      int stage, char* err, size_t errlen) {
+    std::vector<Sphere> spheres(n);
+    for (size_t i = 0; i < n; i++) {
+        spheres[i].c[0] = xyzr[4 * i];
+        spheres[i].c[1] = xyzr[4 * i + 1];
+        spheres[i].c[2] = xyzr[4 * i + 2];
+        spheres[i].r = xyzr[4 * i + 3];
+    }
+    Surface* S = new Surface();
+    std::string e;
+    if (stage >= 3) {
+        e = compute_surface(spheres, probe, spacing, brick, threads, *S);
+    } else {
+        e = S->field.build(spheres, probe);
+        if (e.empty()) e = S->engine.init(&S->field, spacing, brick);
+        if (e.empty() && stage >= 2) e = S->engine.surface_bricks(S->touched, S->bricks);
+    }
+    if (!e.empty()) {
+        set_err(err, errlen, e);
+        delete S;
+        return nullptr;
+    }
+    return S;
+}
+
+void orc_surface_free(void* s) { delete static_cast<Surface*>(s); }
+
+// [touched, bricks, vertices, triangles, evals, spheres, circles, sas_vertices,
+//  contained, exposed, f32_ok, frame x, y, z]
+void orc_surface_counts(void* p, int64_t out[14]) {
+    Surface* S = static_cast<Surface*>(p);
+    const Arrangement& a = S->field.arr;
+    out[0] = (int64_t)S->touched;
+    out[1] = (int64_t)S->bricks.size();
+    out[2] = (int64_t)S->mesh.positions.size();
+    out[3] = (int64_t)S->mesh.triangles.size();
+    out[4] = (int64_t)S->mesh.evals;
+    out[5] = (int64_t)a.centers.size();
+    out[6] = (int64_t)a.circles.size();
+    out[7] = (int64_t)a.vertices.size();
+    out[8] = (int64_t)a.n_contained;
+    out[9] = (int64_t)a.n_exposed;
+    out[10] = S->engine.f32_ok ? 1 : 0;
+    out[11] = S->engine.lattice.frame[0];
+    out[12] = S->engine.lattice.frame[1];
+    out[13] = S->engine.lattice.frame[2];
+}
+
+// [band, coord_max, edge_clamp, rho_max, max_expanded, extent]
+void orc_surface_params(void* p, double out[6]) {
+    Surface* S = static_cast<Surface*>(p);
+    out[0] = S->engine.band;
+    out[1] = S->engine.coord_max;
+    out[2] = edge_clamp(S->engine);
+    out[3] = S->field.rho_max;
+    out[4] = S->field.arr.max_expanded;
+    out[5] = S->field.extent;
+}
+
+void orc_surface_times(void* p, double out[4]) {
+    Surface* S = static_cast<Surface*>(p);
+    out[0] = S->t_geometry;
+    out[1] = S->t_classify;
+    out[2] = S->t_mesh;
+    out[3] = S->t_assemble;
+}
+
+void orc_surface_bricks(void* p, int32_t* keys) {
+    Surface* S = static_cast<Surface*>(p);
+    for (size_t i = 0; i < S->bricks.size(); i++)
+        for (int a = 0; a < 3; a++) keys[3 * i + a] = S->bricks[i][a];
+}
+
+void orc_surface_mesh(void* p, float* pos, float* nrm, uint32_t* atoms, uint8_t* patches, uint32_t* tris,
+                      double* t, uint32_t* vevals) {
+    Surface* S = static_cast<Surface*>(p);
+    const Mesh& m = S->mesh;
+    for (size_t i = 0; i < m.positions.size(); i++)
+        for (int k = 0; k < 3; k++) {
+            if (pos) pos[3 * i + k] = m.positions[i][k];
+            if (nrm) nrm[3 * i + k] = m.normals[i][k];
+        }
+    for (size_t i = 0; i < m.positions.size(); i++) {
+        if (atoms) atoms[i] = m.vertex_atom[i];
+        if (patches) patches[i] = m.vertex_patch[i];
+        if (t) t[i] = m.t[i];
+        if (vevals) vevals[i] = m.vevals[i];
+    }
+    if (tris)
+        for (size_t i = 0; i < m.triangles.size(); i++)
+            for (int k = 0; k < 3; k++) tris[3 * i + k] = m.triangles[i][k];
+}
+
+// [components, cavities, area, volume]
+void orc_surface_measure(void* p, double out[4]) {
+    ComponentStats s = measure(static_cast<Surface*>(p)->mesh);
+    out[0] = (double)s.components;
+    out[1] = (double)s.cavities;
+    out[2] = s.area;
+    out[3] = s.volume;
+}
+
+// Samples (field/kernel.rs Engine::evaluate) of `count` bricks.
+void orc_brick_values(void* p, const int32_t* keys, size_t count, float* values) {
+    Surface* S = static_cast<Surface*>(p);
+    BrickScratch s = S->engine.scratch();
+    size_t spb = S->engine.samples_per_brick();
+    for (size_t i = 0; i < count; i++) {
+        S->engine.evaluate(keys + 3 * i, s);
+        std::copy(s.values.begin(), s.values.begin() + spb, values + i * spb);
+    }
+}
+
+void orc_sub_states(void* p, const int32_t* keys, size_t count, uint8_t* states) {
+    Surface* S = static_cast<Surface*>(p);
+    for (size_t i = 0; i < count; i++) S->engine.sub_states(keys + 3 * i, states + 8 * i);
+}
+
+// Brick classification of given centres with ball radius rho.
+void orc_uniform_sign(void* p, const double* centers, size_t count, double rho, uint8_t* out) {
+    Surface* S = static_cast<Surface*>(p);
+    for (size_t i = 0; i < count; i++) {
+        V3 c(centers[3 * i], centers[3 * i + 1], centers[3 * i + 2]);
+        out[i] = uniform_sign(S->field, c, rho);
+    }
+}
+
+// SesField::sample: value; closest point; feature kind and id.
+double orc_field_sample(void* p, const double x[3], double closest[3], uint32_t feat[2]) {
+    Surface* S = static_cast<Surface*>(p);
+    V3 c;
+    Feature f;
+    double v = S->field.sample(V3(x[0], x[1], x[2]), c, f);
+    closest[0] = c.x;
+    closest[1] = c.y;
+    closest[2] = c.z;
+    feat[0] = f.kind;
+    feat[1] = f.id;
+    return v;
+}
+
+// Arrangement arrays in internal order: centres (n*3), expanded (n), radii
+// (n), status (n), orig (n).
+void orc_arrangement(void* p, double* centers, double* expanded, double* radii, uint8_t* status, uint32_t* orig) {
+    const Arrangement& a = static_cast<Surface*>(p)->field.arr;
+    for (size_t i = 0; i < a.centers.size(); i++) {
+        centers[3 * i] = a.centers[i].x;
+        centers[3 * i + 1] = a.centers[i].y;
+        centers[3 * i + 2] = a.centers[i].z;
+        expanded[i] = a.expanded[i];
+        radii[i] = a.radii[i];
+        status[i] = a.status[i];
+        orig[i] = a.orig[i];
+    }
+}
+
+// Neighbour and circle-blocker CSR sizes and contents.
+size_t orc_csr(void* p, int which, uint32_t* off, uint32_t* items) {
+    const Arrangement& a = static_cast<Surface*>(p)->field.arr;
+    const Csr& c = which == 0 ? a.neighbors : a.circle_blockers;
+    if (off) std::copy(c.off.begin(), c.off.end(), off);
+    if (items) std::copy(c.items.begin(), c.items.end(), items);
+    return c.items.size();
+}
+
+// Circles: i, j (2 u32) and center, axis, e1, radius (10 f64) each.
+void orc_circles(void* p, uint32_t* ij, double* geo) {
+    const Arrangement& a = static_cast<Surface*>(p)->field.arr;
+    for (size_t k = 0; k < a.circles.size(); k++) {
+        const Circle& c = a.circles[k];
+        ij[2 * k] = c.i;
+        ij[2 * k + 1] = c.j;
+        double g[10] = {c.center.x, c.center.y, c.center.z, c.axis.x, c.axis.y,
+                        c.axis.z,   c.e1.x,     c.e1.y,     c.e1.z,   c.radius};
+        for (int q = 0; q < 10; q++) geo[10 * k + q] = g[q];
+    }
+}
+
+void orc_sas_vertices(void* p, double* pos, uint32_t* atoms) {
+    const Arrangement& a = static_cast<Surface*>(p)->field.arr;
+    for (size_t k = 0; k < a.vertices.size(); k++) {
+        for (int q = 0; q < 3; q++) {
+            pos[3 * k + q] = a.vertices[k].position[q];
+            atoms[3 * k + q] = a.vertices[k].atoms[q];
+        }
+    }
+}
+
+void orc_mc_table(uint16_t offset[257], uint8_t* tris, size_t cap) {
+    const McTable& t = mc_table();
+    for (int i = 0; i < 257; i++) offset[i] = t.offset[i];
+    for (size_t i = 0; i < t.tris.size() && i < cap; i++)
+        for (int k = 0; k < 3; k++) tris[3 * i + k] = t.tris[i][k];
+}
+
+// CPU baseline: classify + sample + mesh bricks [first, first + count) of a
+// stage >= 2 surface with `threads` threads over chunks of `chunk` bricks;
+// returns wall seconds and writes the number of vertices made.
+double orc_time_mesh_sample(void* p, size_t first, size_t count, int threads, uint64_t* vertices) {
+    Surface* S = static_cast<Surface*>(p);
+    size_t end = std::min(S->bricks.size(), first + count);
+    if (threads < 1) threads = 1;
+    size_t n = end > first ? end - first : 0;
+    size_t chunk = chunk_len(n);
+    size_t nchunks = (n + chunk - 1) / chunk;
+    std::atomic<size_t> next{0};
+    std::atomic<uint64_t> verts{0};
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<std::thread> pool;
+    for (int w = 0; w < threads; w++)
+        pool.emplace_back([&] {
+            BrickScratch s = S->engine.scratch();
+            for (;;) {
+                size_t c = next.fetch_add(1);
+                if (c >= nchunks) break;
+                BrickMesh part;
+                size_t e = std::min(end, first + (c + 1) * chunk);
+                for (size_t i = first + c * chunk; i < e; i++) {
+                    S->engine.evaluate(S->bricks[i].data(), s);
+                    mesh_brick(S->engine, S->bricks[i].data(), s.values.data(), part);
+                }
+                verts += part.edges.size();
+            }
+        });
+    for (auto& t : pool) t.join();
+    auto t1 = std::chrono::steady_clock::now();
+    if (vertices) *vertices = verts.load();
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+
+}  // extern "C"
