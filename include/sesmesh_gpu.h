@@ -1,0 +1,182 @@
+/* sesmesh_gpu.h: C ABI of the MI355X (gfx950) mesh stage of sesmesh.
+ *
+ * The drop-in boundary for sesmesh's `--backend hip` mesh stage. The
+ * reference crosses it in Rust through the `sesmesh_gpu::batch` traits
+ * (`DeviceResident`, `DeviceBatch`) implemented by `hip::Resident` /
+ * `hip::Batch`; each entry point below cites the reference interface it
+ * replaces (paths relative to the reference repository root).
+ *
+ * Conventions: every function returns 0 on success or a negative SMG_E_*
+ * code, never throws and never aborts; `smg_last_error()` gives a
+ * thread-local message for the last failure. Pointers in *_ref structs are
+ * host pointers read during the call only. Output views stay valid until the
+ * next call that writes the same object. Thread safety: one thread drives a
+ * resident at a time; a device handle may be shared.
+ */
+#ifndef SESMESH_GPU_H
+#define SESMESH_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes, mirroring crates/sesmesh-gpu/src/error.rs:10-66. */
+#define SMG_OK 0
+#define SMG_E_NO_RUNTIME (-1)  /* Error::NoRuntime: HIP runtime unusable */
+#define SMG_E_NO_DEVICE (-2)   /* Error::NoDevice: no such device ordinal */
+#define SMG_E_UNSUPPORTED (-3) /* Error::Unsupported: not gfx950, or a
+                                  parameter the device path does not take */
+#define SMG_E_CALL (-4)        /* Error::Call: a HIP call failed */
+#define SMG_E_SELFTEST (-5)    /* Error::SelfTest: device arithmetic differs
+                                  from IEEE on the host */
+#define SMG_E_LENGTH (-6)      /* Error::Length: array sizes inconsistent */
+#define SMG_E_INVALID (-7)     /* invalid input (sesmesh Error::InvalidInput) */
+#define SMG_E_INTERNAL (-8)    /* internal inconsistency (Error::Internal) */
+
+typedef struct smg_device smg_device;
+typedef struct smg_arrangement smg_arrangement;
+typedef struct smg_resident smg_resident;
+typedef struct smg_mesh smg_mesh;
+typedef struct smg_batch smg_batch;
+
+/* Thread-local message of the last failure (never NULL). */
+const char* smg_last_error(void);
+
+/* ------------------------------------------------------------------ device
+ * Replaces hip::Runtime::load/open (crates/sesmesh-gpu/src/hip/mod.rs:92-234)
+ * and its open-time self-test (hip/selftest.rs:21-70): opens device
+ * `ordinal`, checks it is gfx950, and runs the f32 self-test of the
+ * reference (selftest.rs:25-38) plus an f64 one (the f64 exact refinement
+ * runs on this device); a mismatch returns SMG_E_SELFTEST. */
+typedef struct {
+    char name[256];
+    char arch[64];
+    int compute_units;
+    int clock_khz;
+    size_t total_memory;
+    size_t lds_per_block;
+    int wavefront;
+} smg_device_info;
+
+int smg_device_open(int ordinal, smg_device** out, smg_device_info* info);
+void smg_device_close(smg_device* dev);
+
+/* Self-test on `n` seeded input pairs (selftest.rs:64-80 input generator):
+ * counts of f32 and f64 results whose bits differ from the host's. */
+int smg_selftest(smg_device* dev, size_t n, uint64_t seed, uint64_t* f32_mismatch, uint64_t* f64_mismatch);
+
+/* ------------------------------------------------------------ arrangement
+ * Host-side SAS arrangement of a sphere set for one probe radius, f64, as
+ * crates/sesmesh/src/geom/arrangement.rs:437-645 builds it (with circles and
+ * triple points, without arcs) plus the field's circle and vertex cell grids
+ * (crates/sesmesh/src/field/point.rs:213-274). `xyzr` holds n rows of centre
+ * x, y, z and radius; radius 0 is ignored. `threads` <= 0 means all cores.
+ * The caller of the reference builds this on the CPU before the mesh stage;
+ * it is provided here so a user of this library has the whole path. */
+int smg_arrangement_build(const double* xyzr, size_t n, double probe, int threads, smg_arrangement** out);
+void smg_arrangement_free(smg_arrangement* a);
+
+typedef struct {
+    const double* origin; /* 3 */
+    double cell;
+    uint32_t dims[3];
+    const uint32_t* starts; /* dims[0]*dims[1]*dims[2] + 1 */
+    const uint32_t* items;
+    size_t n_items;
+} smg_grid_ref;
+
+/* The arrangement as the mesh stage reads it. Replaces ArrangementRef /
+ * GridRef (crates/sesmesh-gpu/src/batch.rs:44-82), which carried f32
+ * frame-local copies; this boundary carries the f64 arrangement itself and
+ * the device derives the f32 copies, so its decisions are the CPU's. */
+typedef struct {
+    size_t n_spheres;
+    const double* center;   /* 3 per sphere, internal (Morton) order */
+    const double* radius;   /* vdW radius */
+    const double* expanded; /* radius + probe */
+    const uint8_t* status;  /* 0 exposed, 1 buried, 2 contained */
+    const uint32_t* orig;   /* internal -> input index */
+    const uint32_t* nb_off; /* n_spheres + 1: neighbours (exposed rows) */
+    const uint32_t* nb_items;
+    size_t n_circles;
+    const uint32_t* circle_ij; /* 2 per circle */
+    const double* circle;      /* 10 per circle: centre, axis, e1, radius */
+    const uint32_t* cb_off;    /* n_circles + 1: circle blockers */
+    const uint32_t* cb_items;
+    size_t n_vertices;
+    const double* vertex;        /* 3 per SAS vertex */
+    const uint32_t* vertex_atoms; /* 3 per SAS vertex */
+    smg_grid_ref sphere_grid, circle_grid, vertex_grid;
+    double probe, max_expanded, rho_max;
+} smg_arrangement_ref;
+
+/* View of a built arrangement (valid while it lives). */
+int smg_arrangement_view(const smg_arrangement* a, smg_arrangement_ref* out);
+
+/* --------------------------------------------------------------- resident
+ * Replaces hip::Resident::new (crates/sesmesh-gpu/src/hip/resident.rs:343-371)
+ * and DeviceResident::set_refinement (batch.rs:165-207): uploads the
+ * arrangement once into HBM (SoA, f64 plus f32 copies). */
+int smg_resident_new(smg_device* dev, const smg_arrangement_ref* arr, smg_resident** out);
+void smg_resident_free(smg_resident* r);
+/* DeviceResident::resident_bytes (batch.rs:165-207). */
+size_t smg_resident_bytes(const smg_resident* r);
+
+/* ------------------------------------------------------------ mesh stage
+ * Parameters of a mesh stage: SurfaceParams' spacing, brick size and
+ * refinement mode (crates/sesmesh/src/surface.rs:195-300). */
+#define SMG_REFINE_EXACT 0 /* f64 refinement, CPU-identical bits */
+#define SMG_REFINE_FAST 1  /* f32 compensated refinement (not identical) */
+typedef struct {
+    double spacing;
+    uint32_t brick;       /* even, 4..=32 (2x2x2 sub-blocks) */
+    int32_t refine;       /* SMG_REFINE_* */
+    int32_t sizing;       /* 0 count-scan-fill lists; 1 learned/LDS sizing */
+    int32_t reserved;
+} smg_mesh_params;
+
+/* Per-pass device times (ms) of the last mesh stage, from HIP events; the
+ * reference's BatchTimes/RefineTimes (batch.rs:118-161). */
+typedef struct {
+    double classify_ms;  /* touched bricks + brick and sub-block classification */
+    double sample_ms;    /* candidate gather + field sampling */
+    double edges_ms;     /* crossing edges, vertex numbering */
+    double refine_ms;    /* vertex refinement + attributes */
+    double cells_ms;     /* cell pass + assembly */
+    double total_ms;
+} smg_mesh_times;
+
+/* The whole mesh stage on the device: the work of surface.rs:1047-1079
+ * (classification, mesh_stage, assemble) for the CPU back end, whose result
+ * it reproduces bit for bit in SMG_REFINE_EXACT mode: surface bricks,
+ * samples, vertices (position, normal, atom, patch) in brick order and
+ * triangles as global vertex indices. The mesh stays in HBM; copy it out
+ * with smg_mesh_copy. */
+int smg_mesh_new(smg_resident* r, const smg_mesh_params* p, smg_mesh** out, smg_mesh_times* times);
+void smg_mesh_free(smg_mesh* m);
+
+typedef struct {
+    size_t touched_bricks, surface_bricks, vertices, triangles;
+    uint64_t refine_evaluations;
+    size_t sampled_subblocks, overflow_subblocks;
+    int64_t frame[3];
+    double band, edge_clamp;
+} smg_mesh_info;
+int smg_mesh_info_get(const smg_mesh* m, smg_mesh_info* info);
+
+/* Copies the mesh to host arrays (any may be NULL): positions and normals
+ * 3 f32 per vertex, atom (input index) u32 and patch u8 (0 convex, 1 saddle,
+ * 2 concave) per vertex, triangles 3 u32 each, surface brick keys 3 i32
+ * each, brick samples ((b+1)^3 f32 per brick, x fastest), per-vertex edge
+ * parameter t (f64, unclamped) and evaluations (| 1<<31 if Newton
+ * converged). */
+int smg_mesh_copy(const smg_mesh* m, float* positions, float* normals, uint32_t* atoms, uint8_t* patches,
+                  uint32_t* triangles, int32_t* bricks, float* values, double* t, uint32_t* evals);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

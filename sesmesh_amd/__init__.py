@@ -1,0 +1,330 @@
+"""sesmesh_amd: sesmesh's SES mesh stage on AMD MI355X (gfx950).
+
+The product is ``libsesmesh_hip.so``: hand-written gfx950 HIP kernels behind
+the C ABI of ``include/sesmesh_gpu.h``. This module is the thin Python host
+over that ABI (ctypes, no torch types), mirroring the reference's library
+API for the path: ``compute_surface(spheres, params)`` with
+``SurfaceParams`` (reference crates/sesmesh/src/surface.rs:195-300, 599-685)
+and the device/resident objects of ``sesmesh_gpu::hip`` (reference
+crates/sesmesh-gpu/src/hip/mod.rs, resident.rs).
+
+There is no CPU fallback: without the built library, or without a gfx950
+device, every call raises.
+"""
+import ctypes
+import os
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libsesmesh_hip.so"
+
+SMG_ERRORS = {
+    -1: "NoRuntime", -2: "NoDevice", -3: "Unsupported", -4: "Call", -5: "SelfTest", -6: "Length",
+    -7: "InvalidInput", -8: "Internal",
+}
+
+
+class BackendError(RuntimeError):
+    """A failed C-ABI call (reference sesmesh_gpu::Error wrapped as
+    sesmesh::Error::Backend, crates/sesmesh-gpu/src/error.rs:10-66)."""
+
+    def __init__(self, code, message):
+        super().__init__(f"hip: {SMG_ERRORS.get(code, code)}: {message}")
+        self.code = code
+        self.kind = SMG_ERRORS.get(code, str(code))
+
+
+_vp = ctypes.c_void_p
+
+
+class _GridRef(ctypes.Structure):
+    _fields_ = [("origin", _vp), ("cell", ctypes.c_double), ("dims", ctypes.c_uint32 * 3), ("starts", _vp),
+                ("items", _vp), ("n_items", ctypes.c_size_t)]
+
+
+class ArrangementRef(ctypes.Structure):
+    """smg_arrangement_ref of include/sesmesh_gpu.h."""
+    _fields_ = [("n_spheres", ctypes.c_size_t), ("center", _vp), ("radius", _vp), ("expanded", _vp),
+                ("status", _vp), ("orig", _vp), ("nb_off", _vp), ("nb_items", _vp), ("n_circles", ctypes.c_size_t),
+                ("circle_ij", _vp), ("circle", _vp), ("cb_off", _vp), ("cb_items", _vp),
+                ("n_vertices", ctypes.c_size_t), ("vertex", _vp), ("vertex_atoms", _vp), ("sphere_grid", _GridRef),
+                ("circle_grid", _GridRef), ("vertex_grid", _GridRef), ("probe", ctypes.c_double),
+                ("max_expanded", ctypes.c_double), ("rho_max", ctypes.c_double)]
+
+
+class DeviceInfo(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 256), ("arch", ctypes.c_char * 64), ("compute_units", ctypes.c_int),
+                ("clock_khz", ctypes.c_int), ("total_memory", ctypes.c_size_t), ("lds_per_block", ctypes.c_size_t),
+                ("wavefront", ctypes.c_int)]
+
+
+class MeshParams(ctypes.Structure):
+    _fields_ = [("spacing", ctypes.c_double), ("brick", ctypes.c_uint32), ("refine", ctypes.c_int32),
+                ("sizing", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class MeshTimes(ctypes.Structure):
+    _fields_ = [("classify_ms", ctypes.c_double), ("sample_ms", ctypes.c_double), ("edges_ms", ctypes.c_double),
+                ("refine_ms", ctypes.c_double), ("cells_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class MeshInfo(ctypes.Structure):
+    _fields_ = [("touched_bricks", ctypes.c_size_t), ("surface_bricks", ctypes.c_size_t),
+                ("vertices", ctypes.c_size_t), ("triangles", ctypes.c_size_t),
+                ("refine_evaluations", ctypes.c_uint64), ("sampled_subblocks", ctypes.c_size_t),
+                ("overflow_subblocks", ctypes.c_size_t), ("frame", ctypes.c_int64 * 3), ("band", ctypes.c_double),
+                ("edge_clamp", ctypes.c_double)]
+
+
+# Every function include/sesmesh_gpu.h declares, plus the helpers the
+# library exports for tests and synthetic inputs.
+ABI_FUNCTIONS = [
+    "smg_last_error", "smg_device_open", "smg_device_close", "smg_selftest", "smg_arrangement_build",
+    "smg_arrangement_free", "smg_arrangement_view", "smg_resident_new", "smg_resident_free", "smg_resident_bytes",
+    "smg_mesh_new", "smg_mesh_free", "smg_mesh_info_get", "smg_mesh_copy",
+]
+
+_lib = None
+
+
+def build(force=False):
+    """Builds libsesmesh_hip.so (hipcc --offload-arch=gfx950) in place."""
+    import subprocess
+    args = ["make", "-C", str(HERE / "csrc")]
+    if force:
+        args.insert(1, "-B")
+    subprocess.run(args, check=True, stdout=subprocess.DEVNULL)
+
+
+def lib():
+    """The loaded C-ABI library; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with sesmesh_amd.build() (hipcc, gfx950)")
+        L = ctypes.CDLL(str(LIB_PATH))
+        L.smg_last_error.restype = ctypes.c_char_p
+        L.smg_resident_bytes.restype = ctypes.c_size_t
+        for name in ("smg_device_close", "smg_arrangement_free", "smg_resident_free", "smg_mesh_free"):
+            getattr(L, name).restype = None
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise BackendError(rc, lib().smg_last_error().decode())
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_vp) if a is not None else None
+
+
+class Device:
+    """An open gfx950 device (reference hip::Device, crates/sesmesh-gpu/src/hip/mod.rs:170-234):
+    opening runs the f32 and f64 arithmetic self-test."""
+
+    def __init__(self, ordinal=0):
+        self._h = _vp()
+        info = DeviceInfo()
+        _check(lib().smg_device_open(ctypes.c_int(ordinal), ctypes.byref(self._h), ctypes.byref(info)))
+        self.ordinal = ordinal
+        self.name = info.name.decode()
+        self.arch = info.arch.decode()
+        self.compute_units = info.compute_units
+        self.clock_khz = info.clock_khz
+        self.total_memory = info.total_memory
+        self.lds_per_block = info.lds_per_block
+        self.wavefront = info.wavefront
+
+    def selftest(self, n=1 << 16, seed=1):
+        m32, m64 = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().smg_selftest(self._h, ctypes.c_size_t(n), ctypes.c_uint64(seed), ctypes.byref(m32),
+                                  ctypes.byref(m64)))
+        return m32.value, m64.value
+
+    def close(self):
+        if self._h:
+            lib().smg_device_close(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Arrangement:
+    """The host-built SAS arrangement (reference geom/arrangement.rs:437-645 and
+    field/point.rs:213-274)."""
+
+    def __init__(self, xyzr, probe=1.4, threads=0):
+        a = np.ascontiguousarray(xyzr, dtype=np.float64).reshape(-1, 4)
+        self._h = _vp()
+        _check(lib().smg_arrangement_build(_ptr(a), ctypes.c_size_t(len(a)), ctypes.c_double(probe),
+                                           ctypes.c_int(threads), ctypes.byref(self._h)))
+        self.ref = ArrangementRef()
+        _check(lib().smg_arrangement_view(self._h, ctypes.byref(self.ref)))
+        self.probe = probe
+
+    def arrays(self):
+        """Copies of the arrangement arrays (internal order)."""
+        r = self.ref
+        n, nc, nv = r.n_spheres, r.n_circles, r.n_vertices
+
+        def arr(p, count, dtype):
+            if count == 0:
+                return np.zeros(0, dtype)
+            ct = np.ctypeslib.as_ctypes_type(np.dtype(dtype))
+            return np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ct)), shape=(count,)).copy()
+
+        nb = arr(r.nb_off, n + 1, np.uint32)
+        cb = arr(r.cb_off, nc + 1, np.uint32)
+        return {
+            "centers": arr(r.center, 3 * n, np.float64).reshape(-1, 3), "radii": arr(r.radius, n, np.float64),
+            "expanded": arr(r.expanded, n, np.float64), "status": arr(r.status, n, np.uint8),
+            "orig": arr(r.orig, n, np.uint32), "neighbors": (nb, arr(r.nb_items, int(nb[-1]), np.uint32)),
+            "circles_ij": arr(r.circle_ij, 2 * nc, np.uint32).reshape(-1, 2),
+            "circles": arr(r.circle, 10 * nc, np.float64).reshape(-1, 10),
+            "circle_blockers": (cb, arr(r.cb_items, int(cb[-1]), np.uint32)),
+            "vertices": arr(r.vertex, 3 * nv, np.float64).reshape(-1, 3),
+            "vertex_atoms": arr(r.vertex_atoms, 3 * nv, np.uint32).reshape(-1, 3),
+            "max_expanded": r.max_expanded, "rho_max": r.rho_max,
+        }
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().smg_arrangement_free(h)
+            self._h = _vp()
+
+
+class Resident:
+    """The arrangement resident in HBM (reference hip::Resident::new,
+    crates/sesmesh-gpu/src/hip/resident.rs:343-371)."""
+
+    def __init__(self, device, arrangement):
+        self.device = device
+        self._h = _vp()
+        _check(lib().smg_resident_new(device._h, ctypes.byref(arrangement.ref), ctypes.byref(self._h)))
+
+    @property
+    def bytes(self):
+        return lib().smg_resident_bytes(self._h)
+
+    def mesh(self, spacing=0.5, brick=10, refine="exact", sizing=0):
+        return Mesh(self, spacing, brick, refine, sizing)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().smg_resident_free(h)
+            self._h = _vp()
+
+
+class Mesh:
+    """One mesh stage on the device; the mesh stays in HBM until copied."""
+
+    def __init__(self, resident, spacing, brick, refine, sizing):
+        p = MeshParams(spacing, brick, 0 if refine == "exact" else 1, sizing, 0)
+        self._h = _vp()
+        self.times = MeshTimes()
+        _check(lib().smg_mesh_new(resident._h, ctypes.byref(p), ctypes.byref(self._h), ctypes.byref(self.times)))
+        self.resident = resident
+        info = MeshInfo()
+        _check(lib().smg_mesh_info_get(self._h, ctypes.byref(info)))
+        self.info = info
+        self.brick = brick
+        self.n_vertices, self.n_triangles, self.n_bricks = info.vertices, info.triangles, info.surface_bricks
+
+    def arrays(self, values=False):
+        nv, nt, nb = self.n_vertices, self.n_triangles, self.n_bricks
+        out = {"positions": np.zeros((nv, 3), np.float32), "normals": np.zeros((nv, 3), np.float32),
+               "atoms": np.zeros(nv, np.uint32), "patches": np.zeros(nv, np.uint8),
+               "triangles": np.zeros((nt, 3), np.uint32), "bricks": np.zeros((nb, 3), np.int32),
+               "t": np.zeros(nv, np.float64), "evals": np.zeros(nv, np.uint32)}
+        vals = np.zeros((nb, (self.brick + 1) ** 3), np.float32) if values else None
+        _check(lib().smg_mesh_copy(self._h, _ptr(out["positions"]), _ptr(out["normals"]), _ptr(out["atoms"]),
+                                   _ptr(out["patches"]), _ptr(out["triangles"]), _ptr(out["bricks"]), _ptr(vals),
+                                   _ptr(out["t"]), _ptr(out["evals"])))
+        if values:
+            out["values"] = vals
+        return out
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().smg_mesh_free(h)
+            self._h = _vp()
+
+
+@dataclass
+class SurfaceParams:
+    """The parameters of the mesh stage (reference SurfaceParams,
+    crates/sesmesh/src/surface.rs:195-300): SES with probe 1.4 A, spacing
+    0.5 A, brick 10, exact refinement."""
+    probe_radius: float = 1.4
+    spacing: float = 0.5
+    brick_size: int = 10
+    refine: str = "exact"
+    sizing: int = 0
+
+
+def compute_surface(xyzr, params=None, device=None):
+    """The SES mesh of spheres `xyzr` (n x 4: centre, radius) on the GPU:
+    returns a dict of the mesh arrays (positions, normals, atoms, patches,
+    triangles) and stats, equal to the reference CPU back end's mesh."""
+    params = params or SurfaceParams()
+    dev = device or Device(0)
+    arr = Arrangement(xyzr, params.probe_radius)
+    res = Resident(dev, arr)
+    m = res.mesh(params.spacing, params.brick_size, params.refine, params.sizing)
+    out = m.arrays()
+    out["stats"] = {"touched_bricks": m.info.touched_bricks, "surface_bricks": m.info.surface_bricks,
+                    "vertices": m.n_vertices, "triangles": m.n_triangles,
+                    "refine_evaluations": m.info.refine_evaluations, "timings_ms": m.times.as_dict()}
+    return out
+
+
+# ------------------------------------------------------------ synthetic inputs
+def synth_globule(seed, n_atoms, radius=None):
+    """Seeded random-walk globule (n x 4 xyzr); ball radius 14 A at 330 atoms,
+    scaled as n^(1/3) unless given."""
+    if radius is None:
+        radius = 14.0 * (n_atoms / 330.0) ** (1.0 / 3.0)
+    out = np.zeros((n_atoms, 4), np.float64)
+    lib().smg_synth_globule(ctypes.c_uint64(seed), ctypes.c_size_t(n_atoms), ctypes.c_double(radius), _ptr(out))
+    return out
+
+
+def synth_rings(seed=2, n_globule=4100, per_ring=7, ring_radius=55.0, separation=40.0):
+    """Two stacked 7-fold rings of one 4.1k-atom globule (the 58k-atom assembly)."""
+    radius = 14.0 * (n_globule / 330.0) ** (1.0 / 3.0)
+    out = np.zeros((2 * per_ring * n_globule, 4), np.float64)
+    lib().smg_synth_rings(ctypes.c_uint64(seed), ctypes.c_size_t(n_globule), ctypes.c_double(radius),
+                          ctypes.c_int(per_ring), ctypes.c_double(ring_radius), ctypes.c_double(separation), _ptr(out))
+    return out
+
+
+def synth_batch_spec(seed, count, lo=20, hi=5000):
+    atoms = np.zeros(count, np.uint32)
+    probe = np.zeros(count, np.float64)
+    spacing = np.zeros(count, np.float64)
+    seeds = np.zeros(count, np.uint64)
+    lib().smg_synth_batch_spec(ctypes.c_uint64(seed), ctypes.c_size_t(count), ctypes.c_uint32(lo),
+                               ctypes.c_uint32(hi), _ptr(atoms), _ptr(probe), _ptr(spacing), _ptr(seeds))
+    return atoms, probe, spacing, seeds
+
+
+def mc_table():
+    off = np.zeros(257, np.uint16)
+    edges = np.zeros(3 * 1024, np.uint8)
+    _check(lib().smg_mc_table(_ptr(off), _ptr(edges), ctypes.c_size_t(len(edges))))
+    return off, edges[:3 * int(off[256])].reshape(-1, 3)

@@ -1,0 +1,129 @@
+// Cell pass and assembly on the device: Marching Cubes cases from sample
+// signs, triangles as edge references resolved to global vertex indices
+// (reference mesh/extract.rs:226-265 and :606-691). Vertex indices are the
+// brick's vertex base plus the edge's rank among the target brick's crossing
+// edges, so triangles come out in the CPU's brick / cell / table order.
+#pragma once
+#include "dev_common.hpp"
+
+namespace smg {
+
+__constant__ uint16_t c_tri_offset[257];
+__constant__ uint8_t c_tri_edges[820 * 3];
+__constant__ uint8_t c_edge_lower[12];
+
+__device__ inline uint32_t cell_case(const float* val, uint32_t side, uint32_t x, uint32_t y, uint32_t z) {
+    uint32_t i0 = x + side * (y + side * z), cs = 0;
+    for (uint32_t c = 0; c < 8; c++) {
+        uint32_t st = (c & 1) + ((c >> 1) & 1) * side + ((c >> 2) & 1) * side * side;
+        if (val[i0 + st] < 0.0f) cs |= 1u << c;
+    }
+    return cs;
+}
+
+// Block-wide exclusive scan of one value per thread (256 threads).
+__device__ inline uint32_t block_scan_256(uint32_t v, uint32_t* tmp, uint32_t& total) {
+    uint32_t lane = lane_id(), wv = threadIdx.x / WAVE;
+    uint32_t inc = v;
+    for (int o = 1; o < WAVE; o <<= 1) {
+        uint32_t y = __shfl_up(inc, o, WAVE);
+        if ((int)lane >= o) inc += y;
+    }
+    if (lane == WAVE - 1) tmp[wv] = inc;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (uint32_t q = 0; q < blockDim.x / WAVE; q++) {
+        if (q < wv) before += tmp[q];
+        all += tmp[q];
+    }
+    __syncthreads();
+    total = all;
+    return before + inc - v;
+}
+
+// Triangles per brick: thread t takes the contiguous cells [t*per, (t+1)*per)
+// in (z, y, x) order.
+__global__ void __launch_bounds__(256) k_cell_count(DRun P, const float* values, uint32_t n_bricks,
+                                                    uint32_t* tcount) {
+    __shared__ uint32_t tmp[4];
+    uint32_t brick = blockIdx.x;
+    if (brick >= n_bricks) return;
+    const float* val = values + (size_t)brick * P.spb;
+    uint32_t b = P.b, ncell = b * b * b, per = (ncell + blockDim.x - 1) / blockDim.x;
+    uint32_t mine = 0;
+    for (uint32_t c = threadIdx.x * per; c < min(ncell, (threadIdx.x + 1) * per); c++) {
+        uint32_t x = c % b, y = (c / b) % b, z = c / (b * b);
+        uint32_t cs = cell_case(val, P.side, x, y, z);
+        mine += c_tri_offset[cs + 1] - c_tri_offset[cs];
+    }
+    uint32_t total;
+    block_scan_256(mine, tmp, total);
+    if (threadIdx.x == 0) tcount[brick] = total;
+}
+
+__device__ inline int64_t dense_index(const DRun& P, int64_t kx, int64_t ky, int64_t kz) {
+    int64_t x = kx - P.blo[0], y = ky - P.blo[1], z = kz - P.blo[2];
+    if (x < 0 || y < 0 || z < 0 || x >= P.bdim[0] || y >= P.bdim[1] || z >= P.bdim[2]) return -1;
+    return (z * (int64_t)P.bdim[1] + y) * (int64_t)P.bdim[0] + x;
+}
+
+// Emits the triangles of every brick at tbase[brick] (mesh/extract.rs:245-260
+// and :629-667). A reference that does not resolve sets *err.
+__global__ void __launch_bounds__(256) k_cell_emit(DRun P, const float* values, const int32_t* keys,
+                                                   uint32_t n_bricks, const uint32_t* tbase,
+                                                   const uint32_t* dense, const uint32_t* vbase,
+                                                   const unsigned long long* mask, const uint32_t* wpre,
+                                                   uint32_t words, uint32_t* tris, uint32_t* err) {
+    __shared__ uint32_t tmp[4];
+    uint32_t brick = blockIdx.x;
+    if (brick >= n_bricks) return;
+    const float* val = values + (size_t)brick * P.spb;
+    uint32_t b = P.b, ncell = b * b * b, per = (ncell + blockDim.x - 1) / blockDim.x;
+    uint32_t c_lo = threadIdx.x * per, c_hi = min(ncell, (threadIdx.x + 1) * per);
+    uint32_t mine = 0;
+    for (uint32_t c = c_lo; c < c_hi; c++) {
+        uint32_t cs = cell_case(val, P.side, c % b, (c / b) % b, c / (b * b));
+        mine += c_tri_offset[cs + 1] - c_tri_offset[cs];
+    }
+    uint32_t total;
+    uint32_t at = tbase[brick] + block_scan_256(mine, tmp, total);
+    int64_t key[3] = {keys[3 * brick], keys[3 * brick + 1], keys[3 * brick + 2]};
+    int64_t target[8];
+    for (int q = 0; q < 8; q++) target[q] = -2;
+    target[0] = brick;
+    for (uint32_t c = c_lo; c < c_hi; c++) {
+        uint32_t x = c % b, y = (c / b) % b, z = c / (b * b);
+        uint32_t cs = cell_case(val, P.side, x, y, z);
+        for (uint32_t t = c_tri_offset[cs]; t < c_tri_offset[cs + 1]; t++) {
+            for (int q = 0; q < 3; q++) {
+                uint32_t e = c_tri_edges[3 * t + q];
+                uint32_t axis = e / 4, lo = c_edge_lower[e];
+                uint32_t l[3] = {x + (lo & 1), y + ((lo >> 1) & 1), z + ((lo >> 2) & 1)};
+                uint32_t code = 0, lc[3] = {0, 0, 0};
+                for (int a = 0; a < 3; a++) {
+                    if (l[a] == b)
+                        code |= 1u << a;
+                    else
+                        lc[a] = l[a];
+                }
+                uint32_t id = ((lc[2] * b + lc[1]) * b + lc[0]) * 3 + axis;
+                if (target[code] == -2) {
+                    int64_t di = dense_index(P, key[0] + (code & 1), key[1] + ((code >> 1) & 1),
+                                             key[2] + ((code >> 2) & 1));
+                    target[code] = (di < 0 || dense[di] == 0xffffffffu) ? -1 : (int64_t)dense[di];
+                }
+                int64_t tb = target[code];
+                uint32_t vi = 0;
+                if (tb < 0 || !edge_is_crossing(mask, words, (uint32_t)tb, id)) {
+                    atomicOr(err, 1u);
+                } else {
+                    vi = vbase[tb] + edge_rank(mask, wpre, words, (uint32_t)tb, id);
+                }
+                tris[3 * (size_t)at + q] = vi;
+            }
+            at++;
+        }
+    }
+}
+
+}  // namespace smg

@@ -1,0 +1,189 @@
+// Narrow-band brick classification on the device, in f64 with the CPU's
+// operations: touched bricks (reference field/lattice.rs:105-163), the
+// conservative uniform-sign test (field/lattice.rs:190-227, with the exact
+// ses_eval of field/point.rs:384-489) for bricks and their 2x2x2 sub-blocks
+// (field/kernel.rs:467-482). One wavefront evaluates one ball; its lanes
+// split the cell-grid scans and the minima are reduced across the wave (a
+// minimum does not depend on the order its terms are visited in).
+#pragma once
+#include "dev_common.hpp"
+
+namespace smg {
+
+// SesField::try_sphere_with with the full neighbour row (point.rs:419-442):
+// updates the lane's running `best`.
+__device__ inline void try_sphere_full(const DArr& A, double x, double y, double z, uint32_t i, double& best) {
+    double vx = x - A.cx[i], vy = y - A.cy[i], vz = z - A.cz[i];
+    double len = sqrt(vx * vx + vy * vy + vz * vz);
+    double Ri = A.R[i];
+    double lb = fabs(len - Ri);
+    if (lb >= best || len < 1e-12) return;
+    double s = Ri / len;
+    double qx = A.cx[i] + vx * s, qy = A.cy[i] + vy * s, qz = A.cz[i] + vz * s;
+    for (uint32_t t = A.nb_off[i], e = A.nb_off[i + 1]; t < e; t++) {
+        uint32_t k = A.nb_items[t];
+        if (!(ddist(qx, qy, qz, A.cx[k], A.cy[k], A.cz[k]) >= A.R[k] - EPS)) return;
+    }
+    best = lb;
+}
+
+// SesField::try_circle_with with the full blocker row (point.rs:450-475).
+__device__ inline void try_circle_full(const DArr& A, double x, double y, double z, uint32_t c, double& best) {
+    double yx = x - A.mx[c], yy = y - A.my[c], yz = z - A.mz[c];
+    double ax = A.ax[c], ay = A.ay[c], az = A.az[c];
+    double along = yx * ax + yy * ay + yz * az;
+    double wx = yx - ax * along, wy = yy - ay * along, wz = yz - az * along;
+    double wl = sqrt(wx * wx + wy * wy + wz * wz);
+    double rho = A.crho[c];
+    double dist = sqrt(along * along + (wl - rho) * (wl - rho));
+    if (dist >= best) return;
+    double dx, dy, dz;
+    if (wl > 1e-12) {
+        dx = wx / wl;
+        dy = wy / wl;
+        dz = wz / wl;
+    } else {
+        dx = A.e1x[c];
+        dy = A.e1y[c];
+        dz = A.e1z[c];
+    }
+    double qx = A.mx[c] + dx * rho, qy = A.my[c] + dy * rho, qz = A.mz[c] + dz * rho;
+    for (uint32_t t = A.cb_off[c], e = A.cb_off[c + 1]; t < e; t++) {
+        uint32_t k = A.cb_items[t];
+        if (!(ddist(qx, qy, qz, A.cx[k], A.cy[k], A.cz[k]) >= A.R[k] - EPS)) return;
+    }
+    best = dist;
+}
+
+// lattice::uniform_sign (SES), one wavefront. Returns ST_NONE/OUT/IN in
+// every lane.
+__device__ inline uint8_t uniform_sign_wave(const DArr& A, double x, double y, double z, double rho) {
+    const uint32_t lane = lane_id();
+    double margin = rho + CLASSIFY_EPS;
+    double s_sas = INFINITY, s_vdw = INFINITY;
+    for_near(A.gs, x, y, z, margin + A.max_expanded, lane, WAVE, [&](uint32_t k) {
+        if (A.status[k] != S_CONTAINED) {
+            double d = ddist(x, y, z, A.cx[k], A.cy[k], A.cz[k]);
+            s_sas = fmin(s_sas, d - A.R[k]);
+            s_vdw = fmin(s_vdw, d - A.r[k]);
+        }
+    });
+    s_sas = wave_min(s_sas);
+    s_vdw = wave_min(s_vdw);
+    if (s_sas >= margin) return ST_OUT;
+    if (s_vdw <= -margin) return ST_IN;
+    double cutoff = A.probe + margin + CLASSIFY_EPS;
+    // ses_eval(center, cutoff): inside the SAS at all?
+    bool inside = false;
+    for_near(A.gs, x, y, z, A.max_expanded, lane, WAVE, [&](uint32_t k) {
+        if (!inside && A.status[k] != S_CONTAINED && ddist(x, y, z, A.cx[k], A.cy[k], A.cz[k]) < A.R[k])
+            inside = true;
+    });
+    double dist = 0.0;
+    if (wave_any(inside)) {
+        double best = cutoff;
+        for_near(A.gs, x, y, z, cutoff + A.max_expanded, lane, WAVE, [&](uint32_t i) {
+            if (A.status[i] == S_EXPOSED) try_sphere_full(A, x, y, z, i, best);
+        });
+        for_near(A.gc, x, y, z, cutoff + A.rho_max, lane, WAVE,
+                 [&](uint32_t c) { try_circle_full(A, x, y, z, c, best); });
+        for_near(A.gv, x, y, z, cutoff, lane, WAVE, [&](uint32_t v) {
+            double d = ddist(x, y, z, A.vx[v], A.vy[v], A.vz[v]);
+            if (d < best) best = d;
+        });
+        dist = wave_min(best);
+    }
+    double v = A.probe - dist;
+    if (v > margin) return ST_OUT;
+    if (v < -margin) return ST_IN;
+    return ST_NONE;
+}
+
+// Lattice::brick_of (lattice.rs:88-90).
+__device__ inline int64_t brick_of(double v, double bh) { return (int64_t)floor(v / bh); }
+
+// Brick key range of every non-contained sphere's ball (lattice.rs:119-132):
+// global min/max per axis, as biased unsigned values for atomics.
+__global__ void k_brick_range(DArr A, double bh, unsigned long long* lohi) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n_spheres || A.status[i] == S_CONTAINED) return;
+    double r = A.R[i] + CLASSIFY_EPS;
+    double c[3] = {A.cx[i], A.cy[i], A.cz[i]};
+    for (int a = 0; a < 3; a++) {
+        int64_t lo = brick_of(c[a] - r, bh), hi = brick_of(c[a] + r, bh);
+        // Keys beyond +-2^40 saturate: the host rejects anything past the key range.
+        lo = lo < -(int64_t(1) << 40) ? -(int64_t(1) << 40) : (lo > (int64_t(1) << 40) ? (int64_t(1) << 40) : lo);
+        hi = hi < -(int64_t(1) << 40) ? -(int64_t(1) << 40) : (hi > (int64_t(1) << 40) ? (int64_t(1) << 40) : hi);
+        atomicMin(&lohi[a], (unsigned long long)(lo + (int64_t(1) << 41)));
+        atomicMax(&lohi[3 + a], (unsigned long long)(hi + (int64_t(1) << 41)));
+    }
+}
+
+// Marks the touched bricks of every non-contained sphere in a dense bitmap
+// over the brick box (lattice.rs:119-139); set bits in ascending linear
+// order are the sorted packed keys.
+__global__ void k_mark_touched(DArr A, DRun P, double bh, uint32_t* bits) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n_spheres || A.status[i] == S_CONTAINED) return;
+    double r = A.R[i] + CLASSIFY_EPS;
+    double c[3] = {A.cx[i], A.cy[i], A.cz[i]};
+    int64_t lo[3], hi[3];
+    for (int a = 0; a < 3; a++) {
+        lo[a] = brick_of(c[a] - r, bh) - P.blo[a];
+        hi[a] = brick_of(c[a] + r, bh) - P.blo[a];
+    }
+    for (int64_t z = lo[2]; z <= hi[2]; z++)
+        for (int64_t y = lo[1]; y <= hi[1]; y++)
+            for (int64_t x = lo[0]; x <= hi[0]; x++) {
+                uint64_t lin = ((uint64_t)z * P.bdim[1] + (uint64_t)y) * P.bdim[0] + (uint64_t)x;
+                atomicOr(&bits[lin >> 5], 1u << (lin & 31));
+            }
+}
+
+__device__ inline void key_of_lin(const DRun& P, uint64_t lin, int32_t key[3]) {
+    uint64_t x = lin % P.bdim[0];
+    uint64_t t = lin / P.bdim[0];
+    key[0] = (int32_t)((int64_t)x + P.blo[0]);
+    key[1] = (int32_t)((int64_t)(t % P.bdim[1]) + P.blo[1]);
+    key[2] = (int32_t)((int64_t)(t / P.bdim[1]) + P.blo[2]);
+}
+
+// Lattice::brick_center (lattice.rs:73-76).
+__device__ inline void brick_center(const DRun& P, const int32_t key[3], double c[3]) {
+    for (int a = 0; a < 3; a++) c[a] = ((double)key[a] + 0.5) * (double)P.b * P.h;
+}
+
+// Classification of touched bricks (lattice.rs:168-183): one wavefront per
+// brick; keep[t] = 1 where the surface may pass.
+__global__ void k_classify_bricks(DArr A, DRun P, const uint32_t* touched_lin, uint32_t n, uint32_t* keep) {
+    uint32_t w = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    if (w >= n) return;
+    int32_t key[3];
+    key_of_lin(P, touched_lin[w], key);
+    double c[3];
+    brick_center(P, key, c);
+    double rho = 0.5 * sqrt(3.0) * (double)P.b * P.h;
+    uint8_t s = uniform_sign_wave(A, c[0], c[1], c[2], rho);
+    if (lane_id() == 0) keep[w] = s == ST_NONE ? 1u : 0u;
+}
+
+// Sub-block states of surface bricks (kernel.rs:467-482): one wavefront per
+// (brick, sub-block); st[8 * brick + sb].
+__global__ void k_sub_states(DArr A, DRun P, const int32_t* keys, uint32_t n_bricks, uint8_t* st) {
+    uint32_t w = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    if (w >= 8 * n_bricks) return;
+    uint32_t brick = w >> 3, i = w & 7;
+    double sub = (double)P.sub, h = P.h;
+    double rho = 0.5 * sqrt(3.0) * sub * h;
+    double half = sub * 0.5;
+    double c[3];
+    for (int a = 0; a < 3; a++) {
+        double base = (double)((int64_t)keys[3 * brick + a] * (int64_t)P.b) * h;
+        double off = (double)((i >> a) & 1) * sub + half;
+        c[a] = base + off * h;
+    }
+    uint8_t s = uniform_sign_wave(A, c[0], c[1], c[2], rho);
+    if (lane_id() == 0) st[w] = s;
+}
+
+}  // namespace smg
