@@ -1,14 +1,19 @@
-"""Benchmark of the MI355X mesh stage.
+"""Benchmark of the MI355X mesh stage (one GPU).
 
-Workload: the synthetic 58k-atom assembly (14 copies of a seeded 4.1k-atom
-random-walk globule in two stacked 7-fold rings, ring radius 55 A, rings
-40 A apart), SES probe 1.4 A, grid spacing 0.25 A, brick 10, exact mode.
-A step is one whole mesh stage on the device (classification, gather,
-sampling, crossing edges, f64 refinement, cell pass and assembly) with the
-arrangement resident in HBM; the arrangement build is excluded and
-reported separately.
+Workload (the configuration the metric is quoted on): a synthetic 58k-atom
+assembly, 14 copies of one seeded 4.1k-atom random-walk globule in two
+stacked 7-fold rings (ring radius 55 A, rings 40 A apart), SES probe 1.4 A,
+grid spacing 0.25 A, brick 10, exact refinement. A step is one whole mesh
+stage on the device: classification, candidate gather and sampling,
+crossing edges, f64 vertex refinement, cell pass and assembly, with the
+arrangement resident in HBM (its host build is reported, not timed).
 
-Prints one JSON line: metric "s per SES mesh (58k atoms, 0.25 A, exact)".
+Also measured: batched throughput on seeded molecules of 20-5000 atoms
+(molecules/s), and, unless --no-cpu-baseline, the CPU oracle (a scalar C++
+restatement of the reference CPU back end) on a bounded sample of the same
+workload with all host threads.
+
+Prints one JSON line.
 """
 import argparse
 import json
@@ -24,6 +29,87 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "s per SES mesh (58k atoms, 0.25 Å, exact) + batched molecules/s vs plain GPU"
 
+# MI355X peaks (dense, published): FP64 vector 78.6 TFLOP/s, FP32 157.3 TFLOP/s.
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
+# Algorithmic operations per unit on this workload, counted by the oracle
+# over every 8th surface brick (one op per IEEE add/sub/mul/div/sqrt/min/
+# compare): used when the CPU leg is skipped; the CPU leg recounts them.
+OPS_PER_EVAL_F64 = 3161.5
+OPS_PER_SAMPLE_F32 = 2360.7
+
+
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def host_threads():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(xyzr, probe, spacing, brick, threads, sample_stride):
+    """The oracle's mesh stage on the host: full brick classification (timed)
+    plus sampling and meshing of every `sample_stride`-th surface brick
+    (timed, scaled to all bricks)."""
+    import oracle
+
+    t0 = time.perf_counter()
+    s = oracle.Surface(xyzr, probe, spacing, brick, threads=threads, stage=1)
+    t1 = time.perf_counter()
+    del s
+    t2 = time.perf_counter()
+    s = oracle.Surface(xyzr, probe, spacing, brick, threads=threads, stage=2)
+    t3 = time.perf_counter()
+    t_classify = (t3 - t2) - (t1 - t0)  # stage 2 minus the field build
+    n = s.n_bricks
+    count = (n + sample_stride - 1) // sample_stride
+    t_mesh, verts = s.time_mesh_sample(0, count, sample_stride, threads)
+    ops = s.count_ops(3, max(1, n // 64), 64)
+    scaled = t_mesh * n / count
+    return {
+        "value": t_classify + scaled, "unit": "s/mesh", "cores": threads, "kind": "port",
+        "sample": (f"oracle (C++ restatement of the reference CPU back end) on the same workload: brick "
+                   f"classification of all {s.touched} touched bricks ({t_classify:.2f} s) plus sampling and "
+                   f"meshing of every {sample_stride}th of the {n} surface bricks ({count} bricks, "
+                   f"{t_mesh:.2f} s, scaled x{n / count:.2f}); {threads} threads on {cpu_model()}"),
+        "classify_s": t_classify, "mesh_sample_s": t_mesh, "sample_bricks": count, "sample_vertices": verts,
+        "ops": ops,
+    }
+
+
+def batched_plain(dev, count, seed):
+    """Per-molecule launches of the mesh stage on `count` seeded molecules
+    (atoms log-uniform in 20-5000, probe 1.2-1.4 A, spacing 0.2/0.25/0.5 A):
+    arrangements built and resident first; the timed loop is the mesh stages."""
+    import sesmesh_amd
+
+    atoms, probe, spacing, seeds = sesmesh_amd.synth_batch_spec(seed, count)
+    residents = []
+    for n, p, sd in zip(atoms, probe, seeds):
+        xyzr = sesmesh_amd.synth_globule(int(sd), int(n))
+        residents.append(sesmesh_amd.Resident(dev, sesmesh_amd.Arrangement(xyzr, float(p))))
+    t = time.perf_counter()
+    verts = 0
+    for r, h in zip(residents, spacing):
+        m = r.mesh(float(h), 10)
+        verts += m.n_vertices
+        del m
+    dt = time.perf_counter() - t
+    return {"molecules": count, "atoms": int(atoms.sum()), "vertices": int(verts), "seconds": dt,
+            "molecules_per_s": count / dt}
+
 
 def main():
     ap = argparse.ArgumentParser()
@@ -31,15 +117,19 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch", type=int, default=128, help="molecules in the batched measurement (0: skip)")
+    ap.add_argument("--cpu-stride", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the mesh stage shards by "
-              "brick ranges across GPUs only in design)", file=sys.stderr)
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported: the mesh stage would shard by "
+              "contiguous surface-brick ranges with the arrangement replicated on every GPU (design only).",
+              file=sys.stderr)
         sys.exit(2)
     import sesmesh_amd
 
     xyzr = sesmesh_amd.synth_rings(seed=2)
     probe, spacing, brick = 1.4, 0.25, 10
+    progress(f"workload: {len(xyzr)} atoms")
     t0 = time.perf_counter()
     arr = sesmesh_amd.Arrangement(xyzr, probe)
     t_arr = time.perf_counter() - t0
@@ -48,8 +138,7 @@ def main():
     for _ in range(args.warmup):
         m = res.mesh(spacing, brick)
         del m
-    times = []
-    phase = []
+    times, phase = [], []
     for _ in range(args.steps):
         t = time.perf_counter()
         m = res.mesh(spacing, brick)
@@ -57,20 +146,62 @@ def main():
         phase.append(m.times.as_dict())
         info = m.info
         del m
-    total = sum(times)
-    s_per_mesh = total / args.steps
+    s_per_mesh = sum(times) / args.steps
+    progress(f"GPU: {s_per_mesh:.4f} s/mesh over {args.steps} steps")
+    ph = {k: float(np.median([p[k] for p in phase])) for k in phase[0]}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        progress(f"CPU baseline on {host_threads()} threads")
+        cpu = cpu_baseline(xyzr, probe, spacing, brick, host_threads(), args.cpu_stride)
+        progress(f"CPU baseline: {cpu['value']:.2f} s/mesh")
+    ops = cpu["ops"] if cpu else None
+    f64_per_eval = ops["f64_ops"] / ops["evals"] if ops else OPS_PER_EVAL_F64
+    f32_per_sample = ops["f32_ops"] / ops["samples"] if ops else OPS_PER_SAMPLE_F32
+
+    kernels = {
+        "k_refine": ("mfma" if False else "valu-fp64", ph["k_refine_ms"], f64_per_eval * info.refine_evaluations,
+                     "fp64"),
+        "k_sample": ("valu-fp32", ph["k_sample_ms"], f32_per_sample * info.evaluated_samples, "fp32"),
+    }
+    dom = max(kernels, key=lambda k: kernels[k][1])
+    _, ms, flops, prec = kernels[dom]
+    achieved = flops / (ms * 1e-3) / 1e12
+    roofline = {"kernel": dom, "bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS[prec],
+                "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[prec], "traffic": None,
+                "precision": prec, "ms": ms,
+                "units": (f"{f64_per_eval:.1f} algorithmic fp64 ops per evaluation x {info.refine_evaluations} "
+                          f"evaluations" if dom == "k_refine" else
+                          f"{f32_per_sample:.1f} algorithmic fp32 ops per sample x {info.evaluated_samples} samples")}
+    others = {}
+    for k, (_, kms, kfl, kp) in kernels.items():
+        a = kfl / (kms * 1e-3) / 1e12
+        others[k] = {"ms": kms, "achieved_tflops": a, "frac": a / PEAK_TFLOPS[kp], "precision": kp}
+
+    batch = None
+    if args.batch:
+        progress(f"batched: {args.batch} molecules")
+        batch = {"plain": batched_plain(dev, args.batch, seed=4)}
+        progress(f"batched: {batch['plain']['molecules_per_s']:.1f} molecules/s")
+
     out = {
         "metric": METRIC, "value": s_per_mesh, "unit": "s/mesh", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1000.0 * s_per_mesh, "higher_is_better": False,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32 sampling + f64 refinement",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 sampling, f64 classification and refinement",
         "data": "synthetic (seeded random-walk globules)",
         "config": {"workload": "58k-atom two-ring assembly, SES probe 1.4 A, spacing 0.25 A, brick 10, exact",
-                   "atoms": int(len(xyzr)), "surface_bricks": int(info.surface_bricks),
-                   "vertices": int(info.vertices), "triangles": int(info.triangles)},
+                   "atoms": int(len(xyzr)), "touched_bricks": int(info.touched_bricks),
+                   "surface_bricks": int(info.surface_bricks), "vertices": int(info.vertices),
+                   "triangles": int(info.triangles), "evaluated_samples": int(info.evaluated_samples),
+                   "refine_evaluations": int(info.refine_evaluations)},
+        "roofline": roofline,
+        "cpu_baseline": ({k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")} if cpu else None),
         "best_s": min(times), "median_s": float(np.median(times)),
-        "phases_ms": {k: float(np.median([p[k] for p in phase])) for k in phase[0]},
-        "arrangement_s": t_arr,
+        "phases_ms": ph, "kernels": others, "arrangement_s": t_arr,
+        "batched": batch, "device": dev.name,
     }
+    if cpu:
+        out["speedup_vs_cpu_baseline"] = cpu["value"] / s_per_mesh
     print(json.dumps(out))
 
 

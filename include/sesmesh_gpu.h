@@ -147,6 +147,9 @@ typedef struct {
     double refine_ms;    /* vertex refinement + attributes */
     double cells_ms;     /* cell pass + assembly */
     double total_ms;
+    /* Durations of the main kernels alone (HIP events around their
+       launches on the library's stream). */
+    double k_classify_ms, k_substates_ms, k_sample_ms, k_refine_ms;
 } smg_mesh_times;
 
 /* The whole mesh stage on the device: the work of surface.rs:1047-1079
@@ -162,6 +165,7 @@ typedef struct {
     size_t touched_bricks, surface_bricks, vertices, triangles;
     uint64_t refine_evaluations;
     size_t sampled_subblocks, overflow_subblocks;
+    uint64_t evaluated_samples; /* samples the SES lane kernel evaluated */
     int64_t frame[3];
     double band, edge_clamp;
 } smg_mesh_info;

@@ -37,7 +37,10 @@ def lib():
         L.orc_surface_free.argtypes = [_c_p]
         L.orc_field_sample.restype = ctypes.c_double
         L.orc_time_mesh_sample.restype = ctypes.c_double
-        L.orc_time_mesh_sample.argtypes = [_c_p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, _c_p]
+        L.orc_time_mesh_sample.argtypes = [_c_p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int,
+                                           _c_p]
+        L.orc_count_ops.restype = None
+        L.orc_count_ops.argtypes = [_c_p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, _c_p]
         L.orc_csr.restype = ctypes.c_size_t
         for name in ("orc_surface_counts", "orc_surface_params", "orc_surface_times", "orc_surface_bricks",
                      "orc_surface_mesh", "orc_surface_measure", "orc_brick_values", "orc_sub_states",
@@ -162,10 +165,20 @@ class Surface:
         out["vertices"], out["vertex_atoms"] = vp, va
         return out
 
-    def time_mesh_sample(self, first, count, threads):
+    def time_mesh_sample(self, first, count, stride, threads):
+        """Wall seconds to sample and mesh bricks first + k*stride (k < count)
+        with `threads` threads, and the vertices made."""
         v = ctypes.c_uint64()
-        s = lib().orc_time_mesh_sample(self.handle, first, count, threads, ctypes.byref(v))
+        s = lib().orc_time_mesh_sample(self.handle, first, count, stride, threads, ctypes.byref(v))
         return s, v.value
+
+    def count_ops(self, first, count, stride):
+        """Algorithmic op counts over bricks first + k*stride: f32 ops and
+        evaluated samples of the lane kernel, f64 ops and evaluations of the
+        refinement, vertices."""
+        out = (ctypes.c_uint64 * 5)()
+        lib().orc_count_ops(self.handle, first, count, stride, out)
+        return dict(zip(("f32_ops", "samples", "f64_ops", "evals", "vertices"), list(out)))
 
 
 def mc_table():

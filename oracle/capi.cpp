@@ -37,7 +37,7 @@ void* orc_surface_new(const double* xyzr, size_t n, double probe, double spacing
     } else {
         e = S->field.build(spheres, probe);
         if (e.empty()) e = S->engine.init(&S->field, spacing, brick);
-        if (e.empty() && stage >= 2) e = S->engine.surface_bricks(S->touched, S->bricks);
+        if (e.empty() && stage >= 2) e = classify_bricks(*S, threads);
     }
     if (!e.empty()) {
         set_err(err, errlen, e);
@@ -217,16 +217,15 @@ void orc_mc_table(uint16_t offset[257], uint8_t* tris, size_t cap) {
         for (int k = 0; k < 3; k++) tris[3 * i + k] = t.tris[i][k];
 }
 
-// CPU baseline: classify + sample + mesh bricks [first, first + count) of a
-// stage >= 2 surface with `threads` threads over chunks of `chunk` bricks;
-// returns wall seconds and writes the number of vertices made.
-double orc_time_mesh_sample(void* p, size_t first, size_t count, int threads, uint64_t* vertices) {
+// CPU baseline: sample and mesh the bricks first + k * stride (k < count)
+// of a stage >= 2 surface with `threads` threads over chunks of
+// par::chunk_len bricks; returns wall seconds and the vertices made.
+double orc_time_mesh_sample(void* p, size_t first, size_t count, size_t stride, int threads, uint64_t* vertices) {
     Surface* S = static_cast<Surface*>(p);
-    size_t end = std::min(S->bricks.size(), first + count);
+    std::vector<size_t> idx;
+    for (size_t k = 0; k < count && first + k * stride < S->bricks.size(); k++) idx.push_back(first + k * stride);
     if (threads < 1) threads = 1;
-    size_t n = end > first ? end - first : 0;
-    size_t chunk = chunk_len(n);
-    size_t nchunks = (n + chunk - 1) / chunk;
+    size_t n = idx.size(), chunk = chunk_len(n), nchunks = (n + chunk - 1) / chunk;
     std::atomic<size_t> next{0};
     std::atomic<uint64_t> verts{0};
     auto t0 = std::chrono::steady_clock::now();
@@ -238,10 +237,9 @@ double orc_time_mesh_sample(void* p, size_t first, size_t count, int threads, ui
                 size_t c = next.fetch_add(1);
                 if (c >= nchunks) break;
                 BrickMesh part;
-                size_t e = std::min(end, first + (c + 1) * chunk);
-                for (size_t i = first + c * chunk; i < e; i++) {
-                    S->engine.evaluate(S->bricks[i].data(), s);
-                    mesh_brick(S->engine, S->bricks[i].data(), s.values.data(), part);
+                for (size_t q = c * chunk; q < std::min(n, (c + 1) * chunk); q++) {
+                    S->engine.evaluate(S->bricks[idx[q]].data(), s);
+                    mesh_brick(S->engine, S->bricks[idx[q]].data(), s.values.data(), part);
                 }
                 verts += part.edges.size();
             }
@@ -250,6 +248,28 @@ double orc_time_mesh_sample(void* p, size_t first, size_t count, int threads, ui
     auto t1 = std::chrono::steady_clock::now();
     if (vertices) *vertices = verts.load();
     return std::chrono::duration<double>(t1 - t0).count();
+}
+
+// Algorithmic op counts of sampling and refining the bricks first + k *
+// stride (k < count), one thread: [f32 ops, evaluated samples, f64 ops of
+// eval_local, evaluations, vertices].
+void orc_count_ops(void* p, size_t first, size_t count, size_t stride, uint64_t out[5]) {
+    Surface* S = static_cast<Surface*>(p);
+    BrickScratch s = S->engine.scratch();
+    BrickMesh part;
+    g_ops = OpCounts();
+    g_ops.on = true;
+    for (size_t k = 0; k < count && first + k * stride < S->bricks.size(); k++) {
+        size_t i = first + k * stride;
+        S->engine.evaluate(S->bricks[i].data(), s);
+        mesh_brick(S->engine, S->bricks[i].data(), s.values.data(), part);
+    }
+    g_ops.on = false;
+    out[0] = g_ops.f32_ops;
+    out[1] = g_ops.samples;
+    out[2] = g_ops.f64_ops;
+    out[3] = g_ops.evals;
+    out[4] = part.edges.size();
 }
 
 }  // extern "C"

@@ -376,60 +376,77 @@ void Engine::gather_ses(const V3& b, double rho, BrickScratch& s) const {
 // field/kernel.rs:837-952 for one lane. Lane operations are IEEE f32
 // add/sub/mul/div/sqrt/min/compare in the order of the vector code.
 float ses_lane(const Gathered& g, float x, float y, float z, float probe, float cutoff2) {
+    uint64_t ops = 0;
     bool inside = false;
     for (size_t k = 0; k < g.n_inside; k++) {
         float dx = x - g.nx[k], dy = y - g.ny[k], dz = z - g.nz[k];
         inside |= (dx * dx + dy * dy + dz * dz) < g.nr2[k];
+        ops += 9;
     }
-    if (!inside) return probe;
-    float best2 = cutoff2;
-    for (size_t v = 0; v < g.vx.size(); v++) {
-        float dx = x - g.vx[v], dy = y - g.vy[v], dz = z - g.vz[v];
-        float d2 = dx * dx + dy * dy + dz * dz;
-        best2 = d2 < best2 ? d2 : best2;
-    }
-    for (size_t si = 0; si < g.sx.size(); si++) {
-        float cx = g.sx[si], cy = g.sy[si], cz = g.sz[si], r = g.sr[si];
-        float dx = x - cx, dy = y - cy, dz = z - cz;
-        float dist = std::sqrt(dx * dx + dy * dy + dz * dz);
-        float lb = std::fabs(dist - r);
-        float lb2 = lb * lb;
-        bool valid = lb2 < best2 && 1e-6f < dist;
-        if (!valid) continue;
-        float scale = r / dist;
-        float qx = cx + dx * scale, qy = cy + dy * scale, qz = cz + dz * scale;
-        for (u32 t = g.s_off[si]; t < g.s_off[si + 1] && valid; t++) {
-            u32 k = g.s_blk[t];
-            float ex = qx - g.nx[k], ey = qy - g.ny[k], ez = qz - g.nz[k];
-            valid = (ex * ex + ey * ey + ez * ez) >= g.nr2tol[k];
+    float result = probe;
+    if (inside) {
+        float best2 = cutoff2;
+        for (size_t v = 0; v < g.vx.size(); v++) {
+            float dx = x - g.vx[v], dy = y - g.vy[v], dz = z - g.vz[v];
+            float d2 = dx * dx + dy * dy + dz * dz;
+            best2 = d2 < best2 ? d2 : best2;
+            ops += 9;
         }
-        if (valid) best2 = lb2;
-    }
-    for (size_t ci = 0; ci < g.cx.size(); ci++) {
-        float mx = g.cx[ci], my = g.cy[ci], mz = g.cz[ci];
-        float ux = g.ux[ci], uy = g.uy[ci], uz = g.uz[ci];
-        float rho = g.rho[ci];
-        float yx = x - mx, yy = y - my, yz = z - mz;
-        float along = yx * ux + yy * uy + yz * uz;
-        float wx = yx - ux * along, wy = yy - uy * along, wz = yz - uz * along;
-        float wl = std::sqrt(wx * wx + wy * wy + wz * wz);
-        float t = wl - rho;
-        float d2 = along * along + t * t;
-        bool valid = d2 < best2;
-        if (!valid) continue;
-        bool degenerate = wl < 1e-6f;
-        float sc = rho / wl;
-        float qx = degenerate ? mx + g.ex[ci] * rho : mx + wx * sc;
-        float qy = degenerate ? my + g.ey[ci] * rho : my + wy * sc;
-        float qz = degenerate ? mz + g.ez[ci] * rho : mz + wz * sc;
-        for (u32 tt = g.c_off[ci]; tt < g.c_off[ci + 1] && valid; tt++) {
-            u32 k = g.c_blk[tt];
-            float ex = qx - g.nx[k], ey = qy - g.ny[k], ez = qz - g.nz[k];
-            valid = (ex * ex + ey * ey + ez * ez) >= g.nr2tol[k];
+        for (size_t si = 0; si < g.sx.size(); si++) {
+            float cx = g.sx[si], cy = g.sy[si], cz = g.sz[si], r = g.sr[si];
+            float dx = x - cx, dy = y - cy, dz = z - cz;
+            float dist = std::sqrt(dx * dx + dy * dy + dz * dz);
+            float lb = std::fabs(dist - r);
+            float lb2 = lb * lb;
+            ops += 13;
+            bool valid = lb2 < best2 && 1e-6f < dist;
+            if (!valid) continue;
+            float scale = r / dist;
+            float qx = cx + dx * scale, qy = cy + dy * scale, qz = cz + dz * scale;
+            ops += 7;
+            for (u32 t = g.s_off[si]; t < g.s_off[si + 1] && valid; t++) {
+                u32 k = g.s_blk[t];
+                float ex = qx - g.nx[k], ey = qy - g.ny[k], ez = qz - g.nz[k];
+                valid = (ex * ex + ey * ey + ez * ez) >= g.nr2tol[k];
+                ops += 9;
+            }
+            if (valid) best2 = lb2;
         }
-        if (valid) best2 = d2;
+        for (size_t ci = 0; ci < g.cx.size(); ci++) {
+            float mx = g.cx[ci], my = g.cy[ci], mz = g.cz[ci];
+            float ux = g.ux[ci], uy = g.uy[ci], uz = g.uz[ci];
+            float rho = g.rho[ci];
+            float yx = x - mx, yy = y - my, yz = z - mz;
+            float along = yx * ux + yy * uy + yz * uz;
+            float wx = yx - ux * along, wy = yy - uy * along, wz = yz - uz * along;
+            float wl = std::sqrt(wx * wx + wy * wy + wz * wz);
+            float t = wl - rho;
+            float d2 = along * along + t * t;
+            ops += 25;
+            bool valid = d2 < best2;
+            if (!valid) continue;
+            bool degenerate = wl < 1e-6f;
+            float sc = rho / wl;
+            float qx = degenerate ? mx + g.ex[ci] * rho : mx + wx * sc;
+            float qy = degenerate ? my + g.ey[ci] * rho : my + wy * sc;
+            float qz = degenerate ? mz + g.ez[ci] * rho : mz + wz * sc;
+            ops += 7;
+            for (u32 tt = g.c_off[ci]; tt < g.c_off[ci + 1] && valid; tt++) {
+                u32 k = g.c_blk[tt];
+                float ex = qx - g.nx[k], ey = qy - g.ny[k], ez = qz - g.nz[k];
+                valid = (ex * ex + ey * ey + ez * ez) >= g.nr2tol[k];
+                ops += 9;
+            }
+            if (valid) best2 = d2;
+        }
+        result = probe - std::sqrt(best2);
+        ops += 2;
     }
-    return probe - std::sqrt(best2);
+    if (g_ops.on) {
+        g_ops.f32_ops += ops;
+        g_ops.samples++;
+    }
+    return result;
 }
 
 // field/kernel.rs:415-449

@@ -6,7 +6,13 @@
 
 namespace orc {
 
+thread_local OpCounts g_ops;
+thread_local bool g_in_eval_local = false;
+
 namespace {
+inline void count64(uint64_t n) {
+    if (g_ops.on && g_in_eval_local) g_ops.f64_ops += n;
+}
 constexpr double EPS = 1e-9;  // geom/arrangement.rs:26
 
 // field/point.rs:158-161
@@ -119,10 +125,13 @@ void SesField::try_sphere_with(const V3& x, u32 i, const u32* blk, size_t nb, Po
     V3 v = x - a.centers[i];
     double len = v.norm();
     double lb = std::fabs(len - a.expanded[i]);
+    count64(12);
     if (lb >= best.distance || len < 1e-12) return;
     V3 q = a.centers[i] + v * (a.expanded[i] / len);
+    count64(7);
     for (size_t t = 0; t < nb; t++) {
         u32 k = blk[t];
+        count64(11);
         if (!(q.dist(a.centers[k]) >= a.expanded[k] - EPS)) return;
     }
     best = PointEval{lb, q, Feature{F_SPHERE, i}};
@@ -137,11 +146,14 @@ void SesField::try_circle_with(const V3& x, u32 ci, const u32* blk, size_t nb, P
     V3 w = y - c.axis * along;
     double wl = w.norm();
     double dist = std::sqrt(along * along + (wl - c.radius) * (wl - c.radius));
+    count64(27);
     if (dist >= best.distance) return;
     V3 dir = wl > 1e-12 ? w / wl : c.e1;
     V3 q = c.center + dir * c.radius;
+    count64(wl > 1e-12 ? 10 : 4);
     for (size_t t = 0; t < nb; t++) {
         u32 k = blk[t];
+        count64(11);
         if (!(q.dist(a.centers[k]) >= a.expanded[k] - EPS)) return;
     }
     best = PointEval{dist, q, Feature{F_CIRCLE, ci}};
@@ -151,6 +163,7 @@ void SesField::try_circle_with(const V3& x, u32 ci, const u32* blk, size_t nb, P
 void SesField::try_vertex(const V3& x, u32 vi, PointEval& best) const {
     const V3& p = arr.vertices[vi].position;
     double d = x.dist(p);
+    count64(10);
     if (d < best.distance) best = PointEval{d, p, Feature{F_VERTEX, vi}};
 }
 
@@ -206,12 +219,19 @@ void SesField::gather_local(const V3& b, double rho, double band, LocalCandidate
 // field/point.rs:686-722 (SES)
 NearEval SesField::eval_local(const V3& x, double band, const LocalCandidates& l) const {
     const Arrangement& a = arr;
+    struct Scope {
+        Scope() { g_in_eval_local = true; }
+        ~Scope() { g_in_eval_local = false; }
+    } scope;
+    if (g_ops.on) g_ops.evals++;
     bool inside = false;
-    for (u32 k : l.inside)
+    for (u32 k : l.inside) {
+        count64(10);
         if (x.dist(a.centers[k]) < a.expanded[k]) {
             inside = true;
             break;
         }
+    }
     if (!inside) return NearEval{probe, Feature{F_SOLVENT, 0}, x};
     PointEval best{probe + band, x, Feature{F_BEYOND, 0}};
     for (size_t k = 0; k < l.spheres.size(); k++)

@@ -408,6 +408,42 @@ ComponentStats measure(const Mesh& m) {
     return s;
 }
 
+// Brick classification (field/kernel.rs:301-305, lattice.rs:105-183) with
+// `threads` threads over the touched bricks; the order of the result is the
+// sorted key order whatever the thread count.
+std::string classify_bricks(Surface& S, int threads) {
+    std::vector<u64> keys;
+    std::string err = touched_bricks(S.field, S.engine.lattice, keys);
+    if (!err.empty()) return err;
+    S.touched = keys.size();
+    if (threads < 1) threads = 1;
+    std::vector<uint8_t> keep(keys.size());
+    double rho = S.engine.lattice.brick_radius();
+    std::atomic<size_t> next{0};
+    std::vector<std::thread> pool;
+    for (int w = 0; w < threads; w++)
+        pool.emplace_back([&] {
+            for (;;) {
+                size_t c = next.fetch_add(1024);
+                if (c >= keys.size()) break;
+                for (size_t i = c; i < std::min(keys.size(), c + 1024); i++) {
+                    int32_t key[3];
+                    unpack(keys[i], key);
+                    keep[i] = uniform_sign(S.field, S.engine.lattice.brick_center(key), rho) == SS_NONE;
+                }
+            }
+        });
+    for (auto& t : pool) t.join();
+    S.bricks.clear();
+    for (size_t i = 0; i < keys.size(); i++)
+        if (keep[i]) {
+            int32_t key[3];
+            unpack(keys[i], key);
+            S.bricks.push_back({key[0], key[1], key[2]});
+        }
+    return "";
+}
+
 // surface.rs:1041-1079 (CPU back end, no memory guard, exact refinement)
 std::string compute_surface(const std::vector<Sphere>& spheres, double probe, double spacing, size_t brick,
                             int threads, Surface& S) {
@@ -418,33 +454,9 @@ std::string compute_surface(const std::vector<Sphere>& spheres, double probe, do
     err = S.engine.init(&S.field, spacing, brick);
     if (!err.empty()) return err;
     auto t1 = clk::now();
-    // Brick classification, parallel over touched bricks (order preserved).
-    std::vector<u64> keys;
-    err = touched_bricks(S.field, S.engine.lattice, keys);
+    err = classify_bricks(S, threads);
     if (!err.empty()) return err;
-    S.touched = keys.size();
     if (threads < 1) threads = 1;
-    std::vector<uint8_t> keep(keys.size());
-    {
-        double rho = S.engine.lattice.brick_radius();
-        std::vector<std::thread> pool;
-        for (int w = 0; w < threads; w++)
-            pool.emplace_back([&, w] {
-                for (size_t i = w; i < keys.size(); i += threads) {
-                    int32_t key[3];
-                    unpack(keys[i], key);
-                    keep[i] = uniform_sign(S.field, S.engine.lattice.brick_center(key), rho) == SS_NONE;
-                }
-            });
-        for (auto& t : pool) t.join();
-    }
-    S.bricks.clear();
-    for (size_t i = 0; i < keys.size(); i++)
-        if (keep[i]) {
-            int32_t key[3];
-            unpack(keys[i], key);
-            S.bricks.push_back({key[0], key[1], key[2]});
-        }
     auto t2 = clk::now();
     size_t chunk = chunk_len(S.bricks.size());
     size_t nchunks = (S.bricks.size() + chunk - 1) / chunk;

@@ -201,6 +201,17 @@ struct SesField {
     double sample(const V3& x, V3& closest, Feature& f) const;
 };
 
+// Algorithmic operation counts (one per IEEE add, sub, mul, div, sqrt, min
+// or compare; abs not counted) of the f32 lane kernel and of the f64
+// refinement's eval_local, for roofline accounting. Counting is per thread
+// and off unless `on` is set.
+struct OpCounts {
+    bool on = false;
+    uint64_t f32_ops = 0, samples = 0, f64_ops = 0, evals = 0;
+};
+extern thread_local OpCounts g_ops;
+extern thread_local bool g_in_eval_local;
+
 // ---------------------------------------------------------------- lattice
 constexpr int64_t KEY_BIAS = int64_t(1) << 20;  // field/lattice.rs:15
 constexpr double CLASSIFY_EPS = 2e-3;           // field/lattice.rs:20
@@ -373,5 +384,6 @@ std::string compute_surface(const std::vector<Sphere>& spheres, double probe, do
                             int threads, Surface& out);
 
 size_t chunk_len(size_t n);  // par.rs:66-68
+std::string classify_bricks(Surface& S, int threads);
 
 }  // namespace orc

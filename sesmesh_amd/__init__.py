@@ -68,7 +68,9 @@ class MeshParams(ctypes.Structure):
 
 class MeshTimes(ctypes.Structure):
     _fields_ = [("classify_ms", ctypes.c_double), ("sample_ms", ctypes.c_double), ("edges_ms", ctypes.c_double),
-                ("refine_ms", ctypes.c_double), ("cells_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
+                ("refine_ms", ctypes.c_double), ("cells_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+                ("k_classify_ms", ctypes.c_double), ("k_substates_ms", ctypes.c_double),
+                ("k_sample_ms", ctypes.c_double), ("k_refine_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -78,7 +80,8 @@ class MeshInfo(ctypes.Structure):
     _fields_ = [("touched_bricks", ctypes.c_size_t), ("surface_bricks", ctypes.c_size_t),
                 ("vertices", ctypes.c_size_t), ("triangles", ctypes.c_size_t),
                 ("refine_evaluations", ctypes.c_uint64), ("sampled_subblocks", ctypes.c_size_t),
-                ("overflow_subblocks", ctypes.c_size_t), ("frame", ctypes.c_int64 * 3), ("band", ctypes.c_double),
+                ("overflow_subblocks", ctypes.c_size_t), ("evaluated_samples", ctypes.c_uint64),
+                ("frame", ctypes.c_int64 * 3), ("band", ctypes.c_double),
                 ("edge_clamp", ctypes.c_double)]
 
 

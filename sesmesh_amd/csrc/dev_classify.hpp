@@ -55,8 +55,144 @@ __device__ inline void try_circle_full(const DArr& A, double x, double y, double
     best = dist;
 }
 
-// lattice::uniform_sign (SES), one wavefront. Returns ST_NONE/OUT/IN in
-// every lane.
+// Candidate list of one wavefront in LDS (the kernels below run one
+// wavefront per workgroup).
+constexpr uint32_t CL_CAP = 512;
+struct CandList {
+    double d[CL_CAP];
+    uint32_t ref[CL_CAP];  // id | kind << 30 (0 sphere, 1 circle, 2 vertex)
+    uint32_t n;
+};
+
+// SesField::ses_eval's distance (field/point.rs:384-411) of a point inside
+// the SAS, clamped at `cutoff`, by one wavefront: every candidate's distance
+// first, then blockers of the nearest unchecked candidate until one is
+// free (the minimum over free candidates below the cutoff, as the
+// sequential loop finds it). Falls back to each lane scanning its own
+// candidates when the list outgrows the LDS.
+__device__ inline double ses_distance_wave(const DArr& A, double x, double y, double z, double cutoff) {
+    __shared__ CandList L;
+    const uint32_t lane = lane_id();
+    if (lane == 0) L.n = 0;
+    __syncthreads();
+    for_near(A.gs, x, y, z, cutoff + A.max_expanded, lane, WAVE, [&](uint32_t i) {
+        if (A.status[i] != S_EXPOSED) return;
+        double vx = x - A.cx[i], vy = y - A.cy[i], vz = z - A.cz[i];
+        double len = sqrt(vx * vx + vy * vy + vz * vz);
+        double lb = fabs(len - A.R[i]);
+        if (lb >= cutoff || len < 1e-12) return;
+        uint32_t at = atomicAdd(&L.n, 1u);
+        if (at < CL_CAP) {
+            L.d[at] = lb;
+            L.ref[at] = i;
+        }
+    });
+    for_near(A.gc, x, y, z, cutoff + A.rho_max, lane, WAVE, [&](uint32_t c) {
+        double yx = x - A.mx[c], yy = y - A.my[c], yz = z - A.mz[c];
+        double ax = A.ax[c], ay = A.ay[c], az = A.az[c];
+        double along = yx * ax + yy * ay + yz * az;
+        double wx = yx - ax * along, wy = yy - ay * along, wz = yz - az * along;
+        double wl = sqrt(wx * wx + wy * wy + wz * wz);
+        double rho = A.crho[c];
+        double dist = sqrt(along * along + (wl - rho) * (wl - rho));
+        if (dist >= cutoff) return;
+        uint32_t at = atomicAdd(&L.n, 1u);
+        if (at < CL_CAP) {
+            L.d[at] = dist;
+            L.ref[at] = c | (1u << 30);
+        }
+    });
+    for_near(A.gv, x, y, z, cutoff, lane, WAVE, [&](uint32_t v) {
+        double d = ddist(x, y, z, A.vx[v], A.vy[v], A.vz[v]);
+        if (!(d < cutoff)) return;
+        uint32_t at = atomicAdd(&L.n, 1u);
+        if (at < CL_CAP) {
+            L.d[at] = d;
+            L.ref[at] = v | (2u << 30);
+        }
+    });
+    __syncthreads();
+    const uint32_t count = L.n;
+    if (count > CL_CAP) {
+        double best = cutoff;
+        for_near(A.gs, x, y, z, cutoff + A.max_expanded, lane, WAVE, [&](uint32_t i) {
+            if (A.status[i] == S_EXPOSED) try_sphere_full(A, x, y, z, i, best);
+        });
+        for_near(A.gc, x, y, z, cutoff + A.rho_max, lane, WAVE,
+                 [&](uint32_t c) { try_circle_full(A, x, y, z, c, best); });
+        for_near(A.gv, x, y, z, cutoff, lane, WAVE, [&](uint32_t v) {
+            double d = ddist(x, y, z, A.vx[v], A.vy[v], A.vz[v]);
+            if (d < best) best = d;
+        });
+        return wave_min(best);
+    }
+    for (;;) {
+        double bd = INFINITY;
+        uint32_t bo = 0xffffffffu;
+        for (uint32_t o = lane; o < count; o += WAVE)
+            if (L.d[o] < bd) {
+                bd = L.d[o];
+                bo = o;
+            }
+        for (int m = 32; m > 0; m >>= 1) {
+            double od = __shfl_xor(bd, m, WAVE);
+            uint32_t oo = __shfl_xor(bo, m, WAVE);
+            if (od < bd || (od == bd && oo < bo)) {
+                bd = od;
+                bo = oo;
+            }
+        }
+        if (bo == 0xffffffffu) return cutoff;
+        uint32_t ref = L.ref[bo], kind = ref >> 30, id = ref & 0x3fffffffu;
+        __syncthreads();
+        if (lane == bo % WAVE) L.d[bo] = INFINITY;
+        __syncthreads();
+        if (kind == 2) return bd;
+        double qx, qy, qz;
+        const uint32_t *row, *end;
+        if (kind == 0) {
+            double vx = x - A.cx[id], vy = y - A.cy[id], vz = z - A.cz[id];
+            double len = sqrt(vx * vx + vy * vy + vz * vz);
+            double s = A.R[id] / len;
+            qx = A.cx[id] + vx * s;
+            qy = A.cy[id] + vy * s;
+            qz = A.cz[id] + vz * s;
+            row = A.nb_items + A.nb_off[id];
+            end = A.nb_items + A.nb_off[id + 1];
+        } else {
+            double yx = x - A.mx[id], yy = y - A.my[id], yz = z - A.mz[id];
+            double ax = A.ax[id], ay = A.ay[id], az = A.az[id];
+            double along = yx * ax + yy * ay + yz * az;
+            double wx = yx - ax * along, wy = yy - ay * along, wz = yz - az * along;
+            double wl = sqrt(wx * wx + wy * wy + wz * wz);
+            double dx, dy, dz;
+            if (wl > 1e-12) {
+                dx = wx / wl;
+                dy = wy / wl;
+                dz = wz / wl;
+            } else {
+                dx = A.e1x[id];
+                dy = A.e1y[id];
+                dz = A.e1z[id];
+            }
+            double rho = A.crho[id];
+            qx = A.mx[id] + dx * rho;
+            qy = A.my[id] + dy * rho;
+            qz = A.mz[id] + dz * rho;
+            row = A.cb_items + A.cb_off[id];
+            end = A.cb_items + A.cb_off[id + 1];
+        }
+        bool blocked = false;
+        for (const uint32_t* p = row + lane; p < end; p += WAVE) {
+            uint32_t k = *p;
+            blocked |= !(ddist(qx, qy, qz, A.cx[k], A.cy[k], A.cz[k]) >= A.R[k] - EPS);
+        }
+        if (!wave_any(blocked)) return bd;
+    }
+}
+
+// lattice::uniform_sign (SES), one wavefront (the workgroup). Returns
+// ST_NONE/OUT/IN in every lane.
 __device__ inline uint8_t uniform_sign_wave(const DArr& A, double x, double y, double z, double rho) {
     const uint32_t lane = lane_id();
     double margin = rho + CLASSIFY_EPS;
@@ -80,19 +216,7 @@ __device__ inline uint8_t uniform_sign_wave(const DArr& A, double x, double y, d
             inside = true;
     });
     double dist = 0.0;
-    if (wave_any(inside)) {
-        double best = cutoff;
-        for_near(A.gs, x, y, z, cutoff + A.max_expanded, lane, WAVE, [&](uint32_t i) {
-            if (A.status[i] == S_EXPOSED) try_sphere_full(A, x, y, z, i, best);
-        });
-        for_near(A.gc, x, y, z, cutoff + A.rho_max, lane, WAVE,
-                 [&](uint32_t c) { try_circle_full(A, x, y, z, c, best); });
-        for_near(A.gv, x, y, z, cutoff, lane, WAVE, [&](uint32_t v) {
-            double d = ddist(x, y, z, A.vx[v], A.vy[v], A.vz[v]);
-            if (d < best) best = d;
-        });
-        dist = wave_min(best);
-    }
+    if (wave_any(inside)) dist = ses_distance_wave(A, x, y, z, cutoff);
     double v = A.probe - dist;
     if (v > margin) return ST_OUT;
     if (v < -margin) return ST_IN;
@@ -156,7 +280,7 @@ __device__ inline void brick_center(const DRun& P, const int32_t key[3], double 
 // Classification of touched bricks (lattice.rs:168-183): one wavefront per
 // brick; keep[t] = 1 where the surface may pass.
 __global__ void k_classify_bricks(DArr A, DRun P, const uint32_t* touched_lin, uint32_t n, uint32_t* keep) {
-    uint32_t w = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    uint32_t w = blockIdx.x;  // one wavefront per workgroup
     if (w >= n) return;
     int32_t key[3];
     key_of_lin(P, touched_lin[w], key);
@@ -170,7 +294,7 @@ __global__ void k_classify_bricks(DArr A, DRun P, const uint32_t* touched_lin, u
 // Sub-block states of surface bricks (kernel.rs:467-482): one wavefront per
 // (brick, sub-block); st[8 * brick + sb].
 __global__ void k_sub_states(DArr A, DRun P, const int32_t* keys, uint32_t n_bricks, uint8_t* st) {
-    uint32_t w = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    uint32_t w = blockIdx.x;  // one wavefront per workgroup
     if (w >= 8 * n_bricks) return;
     uint32_t brick = w >> 3, i = w & 7;
     double sub = (double)P.sub, h = P.h;

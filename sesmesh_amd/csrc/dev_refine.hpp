@@ -3,18 +3,25 @@
 // Restates the edge scan of mesh_brick (reference mesh/extract.rs:146-223),
 // SesField::gather_local / eval_local (field/point.rs:419-489, 496-565,
 // 686-722) and vertex / gradient / attributes (mesh/extract.rs:425-597) in
-// f64 with the CPU's operation order, list order (sorted by f64::total_cmp
-// then index: first candidate wins exact ties) and iteration rules, so every
-// vertex, normal, atom and patch equals the CPU back end's bit for bit.
-// One wavefront per sub-block with crossing edges: it gathers the sub-block's
-// lists into LDS once, then refines the sub-block's edges one per lane.
+// f64 with the CPU's operations, candidate order (sorted by f64::total_cmp
+// then index) and iteration rules, so every vertex, normal, atom and patch
+// equals the CPU back end's bit for bit.
+//
+// One workgroup (4 wavefronts) per sub-block with crossing edges gathers the
+// sub-block's candidate lists into LDS with their f64 data, once; then each
+// wavefront refines one edge at a time, evaluating each point cooperatively:
+// lanes split the candidates, and the winner is found best-first, by
+// (distance, list position) over the candidates, checking the blockers of
+// the nearest unchecked candidate until one is free. This selects exactly
+// the candidate eval_local's sequential first-wins loop selects: the first,
+// in list order, among the free candidates of least distance below the
+// cutoff (a candidate taken earlier in the sequential loop is always
+// replaced by it, and none after it improves on it).
 #pragma once
 #include "dev_common.hpp"
 
 namespace smg {
 
-// Edge ids of a brick: ((z*b + y)*b + x)*3 + axis, ascending; crossing
-// edges as a bitmask of 64-bit words per brick with per-word prefix counts.
 __device__ inline bool edge_crossing(const float* val, uint32_t b, uint32_t side, uint32_t id, uint32_t& x,
                                      uint32_t& y, uint32_t& z, uint32_t& axis) {
     axis = id % 3u;
@@ -27,8 +34,9 @@ __device__ inline bool edge_crossing(const float* val, uint32_t b, uint32_t side
     return (val[i0] < 0.0f) != (val[i0 + st] < 0.0f);
 }
 
-// Per brick (one workgroup of 256): crossing-edge masks, per-word prefix,
-// brick total and per-sub-block counts (mesh/extract.rs:176-216).
+// Per brick (one workgroup of 256): crossing-edge masks (edge ids
+// ((z*b + y)*b + x)*3 + axis, 64 per word), per-word prefix counts, brick
+// total and per-sub-block counts (mesh/extract.rs:176-216).
 __global__ void __launch_bounds__(256) k_edges(DRun P, const float* values, uint32_t n_bricks, uint32_t words,
                                                unsigned long long* mask, uint32_t* wpre, uint32_t* bcount,
                                                uint32_t* scount) {
@@ -77,18 +85,23 @@ __device__ inline bool edge_is_crossing(const unsigned long long* mask, uint32_t
     return (mask[(size_t)brick * words + id / 64] >> (id % 64)) & 1ull;
 }
 
-// Refinement lists of one sub-block (LocalCandidates, field/point.rs:75-95).
+// Capacities of the refinement lists (s, c, v powers of two).
 struct RCaps {
-    uint32_t near, inside, s, c, v, sblk, cblk, hash;
+    uint32_t near, inside, s, c, v, sblk, cblk, hash, edges;
 };
+
+// LDS / scratch layout of the refinement lists: f64 SoA data.
 struct RLayout {
-    uint32_t nid, hk, hv, in, skey, sid, soff, ckey, cid, coff, vkey, vid, sblk, cblk, ebuf, bytes;
+    uint32_t nx, ny, nz, nrm, nid, hk, hv, ix, iy, iz, iR, skey, sid, sx, sy, sz, sR, sb0, sb1, ckey, cid, cd, cb0,
+        cb1, vkey, vid, vx, vy, vz, sblk, cblk, ebuf, dm, bytes;
 };
+
 __host__ __device__ inline uint32_t pow2_at_least(uint32_t v) {
     uint32_t p = 1;
     while (p < v) p <<= 1;
     return p;
 }
+
 __host__ __device__ inline RLayout make_rlayout(const RCaps& c) {
     RLayout L;
     uint32_t o = 0;
@@ -97,78 +110,125 @@ __host__ __device__ inline RLayout make_rlayout(const RCaps& c) {
         o += (bytes + 15u) & ~15u;
         return at;
     };
+    L.nx = take(8 * c.near);
+    L.ny = take(8 * c.near);
+    L.nz = take(8 * c.near);
+    L.nrm = take(8 * c.near);
     L.nid = take(4 * c.near);
     L.hk = take(4 * c.hash);
     L.hv = take(4 * c.hash);
-    L.in = take(4 * c.inside);
+    L.ix = take(8 * c.inside);
+    L.iy = take(8 * c.inside);
+    L.iz = take(8 * c.inside);
+    L.iR = take(8 * c.inside);
     L.skey = take(8 * c.s);
     L.sid = take(4 * c.s);
-    L.soff = take(4 * (c.s + 1));
+    L.sx = take(8 * c.s);
+    L.sy = take(8 * c.s);
+    L.sz = take(8 * c.s);
+    L.sR = take(8 * c.s);
+    L.sb0 = take(4 * c.s);
+    L.sb1 = take(4 * c.s);
     L.ckey = take(8 * c.c);
     L.cid = take(4 * c.c);
-    L.coff = take(4 * (c.c + 1));
+    L.cd = take(80 * c.c);  // 10 f64 per circle, SoA of c.c each
+    L.cb0 = take(4 * c.c);
+    L.cb1 = take(4 * c.c);
     L.vkey = take(8 * c.v);
     L.vid = take(4 * c.v);
+    L.vx = take(8 * c.v);
+    L.vy = take(8 * c.v);
+    L.vz = take(8 * c.v);
     L.sblk = take(2 * c.sblk);
     L.cblk = take(2 * c.cblk);
-    L.ebuf = take(4 * 128);
+    L.ebuf = take(4 * c.edges);
+    // Candidate distances per wavefront, for lists longer than the
+    // register budget (4 wavefronts per workgroup).
+    uint32_t tot = c.s + c.c + c.v;
+    L.dm = take(tot > 256u ? 4u * 8u * tot : 0u);
     L.bytes = o;
     return L;
 }
 
 struct RLists {
-    uint32_t *nid, *hk, *hv, *in;
+    double *nx, *ny, *nz, *nrm;
+    uint32_t *nid, *hk, *hv;
+    double *ix, *iy, *iz, *iR;
     unsigned long long* skey;
-    uint32_t *sid, *soff;
+    uint32_t* sid;
+    double *sx, *sy, *sz, *sR;
+    uint32_t *sb0, *sb1;
     unsigned long long* ckey;
-    uint32_t *cid, *coff;
+    uint32_t* cid;
+    double* cd;
+    uint32_t *cb0, *cb1;
     unsigned long long* vkey;
     uint32_t* vid;
+    double *vx, *vy, *vz;
     uint16_t *sblk, *cblk;
     uint32_t* ebuf;
+    double* dm;
 };
+
 __device__ inline RLists rcarve(char* base, const RLayout& L) {
     RLists s;
+    s.nx = (double*)(base + L.nx);
+    s.ny = (double*)(base + L.ny);
+    s.nz = (double*)(base + L.nz);
+    s.nrm = (double*)(base + L.nrm);
     s.nid = (uint32_t*)(base + L.nid);
     s.hk = (uint32_t*)(base + L.hk);
     s.hv = (uint32_t*)(base + L.hv);
-    s.in = (uint32_t*)(base + L.in);
+    s.ix = (double*)(base + L.ix);
+    s.iy = (double*)(base + L.iy);
+    s.iz = (double*)(base + L.iz);
+    s.iR = (double*)(base + L.iR);
     s.skey = (unsigned long long*)(base + L.skey);
     s.sid = (uint32_t*)(base + L.sid);
-    s.soff = (uint32_t*)(base + L.soff);
+    s.sx = (double*)(base + L.sx);
+    s.sy = (double*)(base + L.sy);
+    s.sz = (double*)(base + L.sz);
+    s.sR = (double*)(base + L.sR);
+    s.sb0 = (uint32_t*)(base + L.sb0);
+    s.sb1 = (uint32_t*)(base + L.sb1);
     s.ckey = (unsigned long long*)(base + L.ckey);
     s.cid = (uint32_t*)(base + L.cid);
-    s.coff = (uint32_t*)(base + L.coff);
+    s.cd = (double*)(base + L.cd);
+    s.cb0 = (uint32_t*)(base + L.cb0);
+    s.cb1 = (uint32_t*)(base + L.cb1);
     s.vkey = (unsigned long long*)(base + L.vkey);
     s.vid = (uint32_t*)(base + L.vid);
+    s.vx = (double*)(base + L.vx);
+    s.vy = (double*)(base + L.vy);
+    s.vz = (double*)(base + L.vz);
     s.sblk = (uint16_t*)(base + L.sblk);
     s.cblk = (uint16_t*)(base + L.cblk);
     s.ebuf = (uint32_t*)(base + L.ebuf);
+    s.dm = (double*)(base + L.dm);
     return s;
 }
 
-enum { R_NEAR = 0, R_IN, R_S, R_C, R_V, R_OVER, R_N };
+enum { R_NEAR = 0, R_IN, R_S, R_C, R_V, R_SB, R_CB, R_E, R_OVER, R_N };
 
-// Sorts (key, id) pairs ascending by key then id, count <= cap, by one
-// wavefront (bitonic network over the next power of two).
-__device__ inline void wave_sort(unsigned long long* key, uint32_t* id, uint32_t count) {
+// Sorts (key, id) ascending by key then id (count <= cap, cap a power of
+// two), by the whole workgroup (bitonic network).
+__device__ inline void block_sort(unsigned long long* key, uint32_t* id, uint32_t count) {
     uint32_t n = pow2_at_least(count);
-    const uint32_t lane = threadIdx.x;
-    for (uint32_t i = count + lane; i < n; i += WAVE) {
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    for (uint32_t i = count + tid; i < n; i += nt) {
         key[i] = ~0ull;
         id[i] = 0xffffffffu;
     }
     __syncthreads();
     for (uint32_t k = 2; k <= n; k <<= 1)
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = lane; i < n; i += WAVE) {
+            for (uint32_t i = tid; i < n; i += nt) {
                 uint32_t l = i ^ j;
                 if (l > i) {
                     unsigned long long ki = key[i], kl = key[l];
                     uint32_t ii = id[i], il = id[l];
                     bool gt = ki > kl || (ki == kl && ii > il);
-                    bool up = (i & k) == 0;
-                    if (gt == up) {
+                    if (gt == ((i & k) == 0)) {
                         key[i] = kl;
                         key[l] = ki;
                         id[i] = il;
@@ -180,17 +240,18 @@ __device__ inline void wave_sort(unsigned long long* key, uint32_t* id, uint32_t
         }
 }
 
-// gather_local (field/point.rs:496-565) by one wavefront. False if a
-// capacity was exceeded.
-__device__ inline bool gather_local_wave(const DArr& A, const RCaps& cap, RLists& s, uint32_t* n, double bx,
-                                         double by, double bz, double rho, double band) {
-    const uint32_t lane = threadIdx.x;
-    for (uint32_t i = lane; i < cap.hash; i += WAVE) s.hk[i] = 0u;
-    if (lane < R_N) n[lane] = 0u;
+// gather_local (field/point.rs:496-565) of the ball (b, rho) by the
+// workgroup, with the f64 data of every list in sorted order. False (in
+// every thread) if a capacity was exceeded.
+__device__ inline bool gather_local_block(const DArr& A, const RCaps& cap, RLists& s, uint32_t* n, double bx,
+                                          double by, double bz, double rho, double band) {
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    for (uint32_t i = tid; i < cap.hash; i += nt) s.hk[i] = 0u;
+    if (tid < R_N) n[tid] = 0u;
     __syncthreads();
     const uint32_t hmask = cap.hash - 1;
     double reach = A.probe + band + rho;
-    for_near(A.gs, bx, by, bz, reach + A.max_expanded, lane, WAVE, [&](uint32_t k) {
+    for_near(A.gs, bx, by, bz, reach + A.max_expanded, tid, nt, [&](uint32_t k) {
         if (A.status[k] == S_CONTAINED) return;
         double R = A.R[k];
         double d = ddist(bx, by, bz, A.cx[k], A.cy[k], A.cz[k]);
@@ -199,16 +260,24 @@ __device__ inline bool gather_local_wave(const DArr& A, const RCaps& cap, RLists
             if (at >= cap.near) {
                 n[R_OVER] = 1u;
             } else {
+                s.nx[at] = A.cx[k];
+                s.ny[at] = A.cy[k];
+                s.nz[at] = A.cz[k];
+                s.nrm[at] = R - EPS;
                 s.nid[at] = k;
                 hash_put(s.hk, s.hv, hmask, k, at);
             }
         }
         if (d < R + rho) {
             uint32_t at = atomicAdd(&n[R_IN], 1u);
-            if (at >= cap.inside)
+            if (at >= cap.inside) {
                 n[R_OVER] = 1u;
-            else
-                s.in[at] = k;
+            } else {
+                s.ix[at] = A.cx[k];
+                s.iy[at] = A.cy[k];
+                s.iz[at] = A.cz[k];
+                s.iR[at] = R;
+            }
         }
         double lb = fabs(d - R);
         if (A.status[k] == S_EXPOSED && lb <= reach) {
@@ -221,7 +290,7 @@ __device__ inline bool gather_local_wave(const DArr& A, const RCaps& cap, RLists
             }
         }
     });
-    for_near(A.gc, bx, by, bz, reach + A.rho_max, lane, WAVE, [&](uint32_t c) {
+    for_near(A.gc, bx, by, bz, reach + A.rho_max, tid, nt, [&](uint32_t c) {
         double yx = bx - A.mx[c], yy = by - A.my[c], yz = bz - A.mz[c];
         double ax = A.ax[c], ay = A.ay[c], az = A.az[c];
         double along = yx * ax + yy * ay + yz * az;
@@ -238,7 +307,7 @@ __device__ inline bool gather_local_wave(const DArr& A, const RCaps& cap, RLists
         s.ckey[at] = total_key(dist);
         s.cid[at] = c;
     });
-    for_near(A.gv, bx, by, bz, reach, lane, WAVE, [&](uint32_t v) {
+    for_near(A.gv, bx, by, bz, reach, tid, nt, [&](uint32_t v) {
         double d = ddist(bx, by, bz, A.vx[v], A.vy[v], A.vz[v]);
         if (!(d <= reach)) return;
         uint32_t at = atomicAdd(&n[R_V], 1u);
@@ -251,164 +320,241 @@ __device__ inline bool gather_local_wave(const DArr& A, const RCaps& cap, RLists
     });
     __syncthreads();
     if (n[R_OVER]) return false;
-    wave_sort(s.skey, s.sid, n[R_S]);
-    wave_sort(s.ckey, s.cid, n[R_C]);
-    wave_sort(s.vkey, s.vid, n[R_V]);
-    // Blockers in sorted candidate order: row members that are near
-    // (filter_blockers, point.rs:152-155), as near-list slots.
-    for (int cls = 0; cls < 2; cls++) {
-        uint32_t cnt = n[cls == 0 ? R_S : R_C];
-        uint32_t* ids = cls == 0 ? s.sid : s.cid;
-        uint32_t* off = cls == 0 ? s.soff : s.coff;
-        const uint32_t* ro = cls == 0 ? A.nb_off : A.cb_off;
-        const uint32_t* ri = cls == 0 ? A.nb_items : A.cb_items;
-        uint16_t* out = cls == 0 ? s.sblk : s.cblk;
-        uint32_t limit = cls == 0 ? cap.sblk : cap.cblk;
-        uint32_t base = 0;
-        for (uint32_t c0 = 0; c0 < cnt; c0 += WAVE) {
-            uint32_t i = c0 + lane, m = 0;
-            if (i < cnt) {
-                uint32_t id = ids[i];
-                for (uint32_t t = ro[id]; t < ro[id + 1]; t++) m += hash_get(s.hk, s.hv, hmask, ri[t]) != 0xffffffffu;
-            }
-            // Exclusive prefix of m across the wave.
-            uint32_t inc = m;
-            for (int o = 1; o < WAVE; o <<= 1) {
-                uint32_t y = __shfl_up(inc, o, WAVE);
-                if ((int)lane >= o) inc += y;
-            }
-            uint32_t at = base + inc - m;
-            uint32_t total = __shfl(inc, WAVE - 1, WAVE);
-            if (i < cnt) {
-                off[i] = at;
-                if (at + m <= limit) {
-                    uint32_t id = ids[i], w = at;
-                    for (uint32_t t = ro[id]; t < ro[id + 1]; t++) {
-                        uint32_t l = hash_get(s.hk, s.hv, hmask, ri[t]);
-                        if (l != 0xffffffffu) out[w++] = (uint16_t)l;
-                    }
-                } else {
-                    n[R_OVER] = 1u;
-                }
-            }
-            base += total;
+    block_sort(s.skey, s.sid, n[R_S]);
+    block_sort(s.ckey, s.cid, n[R_C]);
+    block_sort(s.vkey, s.vid, n[R_V]);
+    // Candidate data in sorted order, and blocker lists (row members that
+    // are near, point.rs:152-155) as near-list slots.
+    uint32_t ns = n[R_S], nc = n[R_C], nv = n[R_V];
+    const uint32_t cc = cap.c;
+    for (uint32_t i = tid; i < ns + nc + nv; i += nt) {
+        if (i < ns) {
+            uint32_t k = s.sid[i];
+            s.sx[i] = A.cx[k];
+            s.sy[i] = A.cy[k];
+            s.sz[i] = A.cz[k];
+            s.sR[i] = A.R[k];
+        } else if (i < ns + nc) {
+            uint32_t q = i - ns, c = s.cid[q];
+            double* d = s.cd;
+            d[q] = A.mx[c];
+            d[cc + q] = A.my[c];
+            d[2 * cc + q] = A.mz[c];
+            d[3 * cc + q] = A.ax[c];
+            d[4 * cc + q] = A.ay[c];
+            d[5 * cc + q] = A.az[c];
+            d[6 * cc + q] = A.e1x[c];
+            d[7 * cc + q] = A.e1y[c];
+            d[8 * cc + q] = A.e1z[c];
+            d[9 * cc + q] = A.crho[c];
+        } else {
+            uint32_t q = i - ns - nc, v = s.vid[q];
+            s.vx[q] = A.vx[v];
+            s.vy[q] = A.vy[v];
+            s.vz[q] = A.vz[v];
         }
-        if (lane == 0) off[cnt] = base;
+        if (i < ns + nc) {
+            bool is_s = i < ns;
+            uint32_t id = is_s ? s.sid[i] : s.cid[i - ns];
+            const uint32_t* row = is_s ? A.nb_items + A.nb_off[id] : A.cb_items + A.cb_off[id];
+            uint32_t len = is_s ? A.nb_off[id + 1] - A.nb_off[id] : A.cb_off[id + 1] - A.cb_off[id];
+            uint32_t cnt = 0;
+            for (uint32_t t = 0; t < len; t++) cnt += hash_get(s.hk, s.hv, hmask, row[t]) != 0xffffffffu;
+            uint32_t at = atomicAdd(&n[is_s ? R_SB : R_CB], cnt);
+            if (at + cnt > (is_s ? cap.sblk : cap.cblk)) {
+                n[R_OVER] = 1u;
+                continue;
+            }
+            uint16_t* out = is_s ? s.sblk : s.cblk;
+            uint32_t w = at;
+            for (uint32_t t = 0; t < len; t++) {
+                uint32_t l = hash_get(s.hk, s.hv, hmask, row[t]);
+                if (l != 0xffffffffu) out[w++] = (uint16_t)l;
+            }
+            if (is_s) {
+                s.sb0[i] = at;
+                s.sb1[i] = at + cnt;
+            } else {
+                s.cb0[i - ns] = at;
+                s.cb1[i - ns] = at + cnt;
+            }
+        }
     }
     __syncthreads();
     return n[R_OVER] == 0u;
 }
 
-struct NearEvalD {
+struct EvalD {
     double value;
     uint32_t kind, id;
     double qx, qy, qz;
 };
 
-// SesField::eval_local (field/point.rs:686-722) with try_sphere_with /
-// try_circle_with / try_vertex (:419-489).
-__device__ inline NearEvalD eval_local_d(const DArr& A, const RLists& s, const uint32_t* n, double x, double y,
-                                         double z, double band) {
-    NearEvalD e;
-    bool inside = false;
-    for (uint32_t q = 0; q < n[R_IN]; q++) {
-        uint32_t k = s.in[q];
-        if (ddist(x, y, z, A.cx[k], A.cy[k], A.cz[k]) < A.R[k]) {
-            inside = true;
-            break;
+// Lexicographic minimum of (d, o) over the wave.
+__device__ inline void wave_argmin(double& d, uint32_t& o) {
+    for (int m = 32; m > 0; m >>= 1) {
+        double od = __shfl_xor(d, m, WAVE);
+        uint32_t oo = __shfl_xor(o, m, WAVE);
+        if (od < d || (od == d && oo < o)) {
+            d = od;
+            o = oo;
         }
     }
+}
+
+constexpr int MAXC = 4;  // candidates per lane held in registers
+
+// Distance of candidate o (list position: S, then C, then V) from (x, y, z)
+// as eval_local compares it (field/point.rs:419-489), +inf where the
+// candidate cannot be taken (no projection, or not below the cutoff).
+__device__ inline double cand_dist(const RLists& s, uint32_t ns, uint32_t nc, uint32_t total, uint32_t ccap,
+                                   uint32_t o, double x, double y, double z, double cutoff) {
+    const double* cd = s.cd;
+    double d = INFINITY;
+    if (o < ns) {
+        double vx = x - s.sx[o], vy = y - s.sy[o], vz = z - s.sz[o];
+        double len = sqrt(vx * vx + vy * vy + vz * vz);
+        double lb = fabs(len - s.sR[o]);
+        if (!(len < 1e-12)) d = lb;
+    } else if (o < ns + nc) {
+        uint32_t q = o - ns;
+        double yx = x - cd[q], yy = y - cd[ccap + q], yz = z - cd[2 * ccap + q];
+        double ax = cd[3 * ccap + q], ay = cd[4 * ccap + q], az = cd[5 * ccap + q], rho = cd[9 * ccap + q];
+        double along = yx * ax + yy * ay + yz * az;
+        double wx = yx - ax * along, wy = yy - ay * along, wz = yz - az * along;
+        double wl = sqrt(wx * wx + wy * wy + wz * wz);
+        d = sqrt(along * along + (wl - rho) * (wl - rho));
+    } else if (o < total) {
+        uint32_t q = o - ns - nc;
+        d = ddist(x, y, z, s.vx[q], s.vy[q], s.vz[q]);
+    }
+    return d < cutoff ? d : INFINITY;
+}
+
+// eval_local (field/point.rs:686-722) at (x, y, z) by one wavefront.
+// `dm` (per wavefront) holds candidate distances when the lists exceed
+// WAVE * MAXC candidates; otherwise registers do.
+__device__ inline EvalD eval_wave(const DArr& A, const RLists& s, const uint32_t* n, uint32_t ccap, double x,
+                                  double y, double z, double band, double* dm) {
+    const uint32_t lane = lane_id();
+    EvalD e;
     e.qx = x;
     e.qy = y;
     e.qz = z;
-    if (!inside) {
+    bool ins = false;
+    for (uint32_t q = lane; q < n[R_IN]; q += WAVE) ins |= ddist(x, y, z, s.ix[q], s.iy[q], s.iz[q]) < s.iR[q];
+    if (!wave_any(ins)) {
         e.value = A.probe;
         e.kind = F_SOLVENT;
         e.id = 0;
         return e;
     }
-    double best = A.probe + band;
-    e.kind = F_BEYOND;
-    e.id = 0;
-    for (uint32_t q = 0; q < n[R_S]; q++) {
-        uint32_t i = s.sid[q];
-        double cx = A.cx[i], cy = A.cy[i], cz = A.cz[i], Ri = A.R[i];
-        double vx = x - cx, vy = y - cy, vz = z - cz;
-        double len = sqrt(vx * vx + vy * vy + vz * vz);
-        double lb = fabs(len - Ri);
-        if (lb >= best || len < 1e-12) continue;
-        double sc = Ri / len;
-        double qx = cx + vx * sc, qy = cy + vy * sc, qz = cz + vz * sc;
-        bool free = true;
-        for (uint32_t t = s.soff[q], te = s.soff[q + 1]; t < te; t++) {
-            uint32_t k = s.nid[s.sblk[t]];
-            if (!(ddist(qx, qy, qz, A.cx[k], A.cy[k], A.cz[k]) >= A.R[k] - EPS)) {
-                free = false;
-                break;
-            }
-        }
-        if (free) {
-            best = lb;
-            e.kind = F_SPHERE;
-            e.id = i;
-            e.qx = qx;
-            e.qy = qy;
-            e.qz = qz;
-        }
+    const uint32_t ns = n[R_S], nc = n[R_C], nv = n[R_V], total = ns + nc + nv;
+    const double cutoff = A.probe + band;
+    const double* cd = s.cd;
+    const bool in_regs = total <= WAVE * MAXC;
+    double dist[MAXC];
+    if (in_regs) {
+        for (int j = 0; j < MAXC; j++) dist[j] = cand_dist(s, ns, nc, total, ccap, lane + WAVE * j, x, y, z, cutoff);
+    } else {
+        for (uint32_t o = lane; o < total; o += WAVE) dm[o] = cand_dist(s, ns, nc, total, ccap, o, x, y, z, cutoff);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
-    for (uint32_t q = 0; q < n[R_C]; q++) {
-        uint32_t c = s.cid[q];
-        double mx = A.mx[c], my = A.my[c], mz = A.mz[c];
-        double ax = A.ax[c], ay = A.ay[c], az = A.az[c], rho = A.crho[c];
-        double yx = x - mx, yy = y - my, yz = z - mz;
-        double along = yx * ax + yy * ay + yz * az;
-        double wx = yx - ax * along, wy = yy - ay * along, wz = yz - az * along;
-        double wl = sqrt(wx * wx + wy * wy + wz * wz);
-        double dist = sqrt(along * along + (wl - rho) * (wl - rho));
-        if (dist >= best) continue;
-        double dx, dy, dz;
-        if (wl > 1e-12) {
-            dx = wx / wl;
-            dy = wy / wl;
-            dz = wz / wl;
+    for (;;) {
+        double bd = INFINITY;
+        uint32_t bo = 0xffffffffu;
+        if (in_regs) {
+            for (int j = 0; j < MAXC; j++)
+                if (dist[j] < bd) {
+                    bd = dist[j];
+                    bo = lane + WAVE * j;
+                }
         } else {
-            dx = A.e1x[c];
-            dy = A.e1y[c];
-            dz = A.e1z[c];
+            for (uint32_t o = lane; o < total; o += WAVE)
+                if (dm[o] < bd) {
+                    bd = dm[o];
+                    bo = o;
+                }
         }
-        double qx = mx + dx * rho, qy = my + dy * rho, qz = mz + dz * rho;
-        bool free = true;
-        for (uint32_t t = s.coff[q], te = s.coff[q + 1]; t < te; t++) {
-            uint32_t k = s.nid[s.cblk[t]];
-            if (!(ddist(qx, qy, qz, A.cx[k], A.cy[k], A.cz[k]) >= A.R[k] - EPS)) {
-                free = false;
-                break;
+        wave_argmin(bd, bo);
+        if (bo == 0xffffffffu) {  // nothing free below the cutoff: Beyond
+            e.value = A.probe - cutoff;
+            e.kind = F_BEYOND;
+            e.id = 0;
+            return e;
+        }
+        if (bo % WAVE == lane) {
+            if (in_regs)
+                dist[bo / WAVE] = INFINITY;
+            else
+                dm[bo] = INFINITY;
+        }
+        double qx, qy, qz;
+        uint32_t b0 = 0, b1 = 0;
+        const uint16_t* blk = s.sblk;
+        uint32_t kind, id;
+        if (bo < ns) {
+            double cx = s.sx[bo], cy = s.sy[bo], cz = s.sz[bo], Ri = s.sR[bo];
+            double vx = x - cx, vy = y - cy, vz = z - cz;
+            double len = sqrt(vx * vx + vy * vy + vz * vz);
+            double sc = Ri / len;
+            qx = cx + vx * sc;
+            qy = cy + vy * sc;
+            qz = cz + vz * sc;
+            b0 = s.sb0[bo];
+            b1 = s.sb1[bo];
+            kind = F_SPHERE;
+            id = s.sid[bo];
+        } else if (bo < ns + nc) {
+            uint32_t q = bo - ns;
+            double mx = cd[q], my = cd[ccap + q], mz = cd[2 * ccap + q];
+            double ax = cd[3 * ccap + q], ay = cd[4 * ccap + q], az = cd[5 * ccap + q], rho = cd[9 * ccap + q];
+            double yx = x - mx, yy = y - my, yz = z - mz;
+            double along = yx * ax + yy * ay + yz * az;
+            double wx = yx - ax * along, wy = yy - ay * along, wz = yz - az * along;
+            double wl = sqrt(wx * wx + wy * wy + wz * wz);
+            double dx, dy, dz;
+            if (wl > 1e-12) {
+                dx = wx / wl;
+                dy = wy / wl;
+                dz = wz / wl;
+            } else {
+                dx = cd[6 * ccap + q];
+                dy = cd[7 * ccap + q];
+                dz = cd[8 * ccap + q];
             }
+            qx = mx + dx * rho;
+            qy = my + dy * rho;
+            qz = mz + dz * rho;
+            b0 = s.cb0[q];
+            b1 = s.cb1[q];
+            blk = s.cblk;
+            kind = F_CIRCLE;
+            id = s.cid[q];
+        } else {
+            uint32_t q = bo - ns - nc;
+            qx = s.vx[q];
+            qy = s.vy[q];
+            qz = s.vz[q];
+            kind = F_VERTEX;
+            id = s.vid[q];
         }
-        if (free) {
-            best = dist;
-            e.kind = F_CIRCLE;
-            e.id = c;
+        bool blocked = false;
+        for (uint32_t t = b0 + lane; t < b1; t += WAVE) {
+            uint32_t k = blk[t];
+            blocked |= !(ddist(qx, qy, qz, s.nx[k], s.ny[k], s.nz[k]) >= s.nrm[k]);
+        }
+        if (!wave_any(blocked)) {
+            e.value = A.probe - bd;
+            e.kind = kind;
+            e.id = id;
             e.qx = qx;
             e.qy = qy;
             e.qz = qz;
+            return e;
         }
     }
-    for (uint32_t q = 0; q < n[R_V]; q++) {
-        uint32_t v = s.vid[q];
-        double px = A.vx[v], py = A.vy[v], pz = A.vz[v];
-        double d = ddist(x, y, z, px, py, pz);
-        if (d < best) {
-            best = d;
-            e.kind = F_VERTEX;
-            e.id = v;
-            e.qx = px;
-            e.qy = py;
-            e.qz = pz;
-        }
-    }
-    e.value = A.probe - best;
-    return e;
 }
 
 // Output arrays of the vertices.
@@ -422,10 +568,10 @@ struct VOut {
 };
 
 // vertex() + attributes() (mesh/extract.rs:425-535, 553-597) of the edge
-// from lattice point ia along `axis`, sampled values fa32, fb32.
-__device__ inline uint32_t refine_vertex(const DArr& A, const DRun& P, const RLists& s, const uint32_t* n,
-                                     const int64_t ia[3], uint32_t axis, float fa32, float fb32, const VOut& o,
-                                     size_t slot) {
+// from lattice point ia along `axis`, by one wavefront; lane 0 writes.
+__device__ inline uint32_t refine_vertex_wave(const DArr& A, const DRun& P, const RLists& s, const uint32_t* n,
+                                              uint32_t ccap, double* dm, const int64_t ia[3], uint32_t axis,
+                                              float fa32, float fb32, const VOut& o, size_t slot) {
     const double h = P.h, band = P.band;
     double pa[3] = {(double)ia[0] * h, (double)ia[1] * h, (double)ia[2] * h};
     int64_t ib[3] = {ia[0], ia[1], ia[2]};
@@ -438,11 +584,11 @@ __device__ inline uint32_t refine_vertex(const DArr& A, const DRun& P, const RLi
     int side = 0;
     bool converged = false;
     uint32_t evals = 0;
-    NearEvalD last;
+    EvalD last;
     double dn = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     for (int it = 0; it < 8; it++) {
         double x = pa[0] + d[0] * t, y = pa[1] + d[1] * t, z = pa[2] + d[2] * t;
-        NearEvalD e = eval_local_d(A, s, n, x, y, z, band);
+        EvalD e = eval_wave(A, s, n, ccap, x, y, z, band, dm);
         evals++;
         last = e;
         double g = e.value;
@@ -482,7 +628,7 @@ __device__ inline uint32_t refine_vertex(const DArr& A, const DRun& P, const RLi
         bool neg0 = g0 < 0.0;
         for (int it = 0; it < 64; it++) {
             t = 0.5 * (t0 + t1);
-            NearEvalD e = eval_local_d(A, s, n, pa[0] + d[0] * t, pa[1] + d[1] * t, pa[2] + d[2] * t, band);
+            EvalD e = eval_wave(A, s, n, ccap, pa[0] + d[0] * t, pa[1] + d[1] * t, pa[2] + d[2] * t, band, dm);
             evals++;
             last = e;
             if (fabs(e.value) <= 1e-5 || (t1 - t0) * dn <= 2.0 * 1e-5) break;
@@ -494,7 +640,6 @@ __device__ inline uint32_t refine_vertex(const DArr& A, const DRun& P, const RLi
     }
     double tc = fmin(fmax(t, P.t_min), 1.0 - P.t_min);
     double v[3] = {pa[0] + d[0] * tc, pa[1] + d[1] * tc, pa[2] + d[2] * tc};
-    // Fallback normal along the edge, inside end to outside end.
     double ow[3];
     for (int a = 0; a < 3; a++) ow[a] = fa < 0.0 ? d[a] : -d[a];
     double ol = sqrt(ow[0] * ow[0] + ow[1] * ow[1] + ow[2] * ow[2]);
@@ -542,44 +687,50 @@ __device__ inline uint32_t refine_vertex(const DArr& A, const DRun& P, const RLi
         uint32_t at[3] = {A.vatoms[3 * last.id], A.vatoms[3 * last.id + 1], A.vatoms[3 * last.id + 2]};
         atom = nearest(at, 3);
     } else {
-        // sphere_eval(v, probe + band + 8) over the whole grid (point.rs:742-756).
+        // sphere_eval(v, probe + band + 8) over the whole grid (point.rs:742-756),
+        // first minimum in scan order: lane 0 alone (rare).
         nrm[0] = outward[0];
         nrm[1] = outward[1];
         nrm[2] = outward[2];
         patch = 0;
-        double cutoff = A.probe + band + 8.0;
-        double bv = cutoff;
-        int64_t bi = -1;
-        for_near(A.gs, v[0], v[1], v[2], cutoff + A.max_expanded, 0, 1, [&](uint32_t k) {
-            if (A.status[k] == S_CONTAINED) return;
-            double vv = ddist(v[0], v[1], v[2], A.cx[k], A.cy[k], A.cz[k]) - A.R[k];
-            if (vv < bv) {
-                bv = vv;
-                bi = k;
-            }
-        });
-        atom = bi < 0 ? 0u : A.orig[bi];
+        if (lane_id() == 0) {
+            double cutoff = A.probe + band + 8.0;
+            double bv = cutoff;
+            int64_t bi = -1;
+            for_near(A.gs, v[0], v[1], v[2], cutoff + A.max_expanded, 0, 1, [&](uint32_t k) {
+                if (A.status[k] == S_CONTAINED) return;
+                double vv = ddist(v[0], v[1], v[2], A.cx[k], A.cy[k], A.cz[k]) - A.R[k];
+                if (vv < bv) {
+                    bv = vv;
+                    bi = k;
+                }
+            });
+            atom = bi < 0 ? 0u : A.orig[bi];
+        }
     }
-    for (int a = 0; a < 3; a++) {
-        o.pos[3 * slot + a] = (float)v[a];
-        o.nrm[3 * slot + a] = (float)nrm[a];
+    if (lane_id() == 0) {
+        for (int a = 0; a < 3; a++) {
+            o.pos[3 * slot + a] = (float)v[a];
+            o.nrm[3 * slot + a] = (float)nrm[a];
+        }
+        o.atom[slot] = atom;
+        o.patch[slot] = patch;
+        if (o.t) o.t[slot] = t;
+        if (o.evals) o.evals[slot] = evals | (converged ? 0x80000000u : 0u);
     }
-    o.atom[slot] = atom;
-    o.patch[slot] = patch;
-    if (o.t) o.t[slot] = t;
-    if (o.evals) o.evals[slot] = evals | (converged ? 0x80000000u : 0u);
     return evals;
 }
 
-// One wavefront per sub-block with crossing edges (`work[w]` = 8*brick+sb):
-// gather_local in LDS (or in scratch slice w), then one edge per lane.
-// Sub-blocks whose lists overflow are appended to `overflow`.
-__global__ void __launch_bounds__(64) k_refine(DArr A, DRun P, RCaps cap, RLayout lay, const int32_t* keys,
-                                               const float* values, const uint32_t* work, uint32_t n_work,
-                                               const unsigned long long* mask, const uint32_t* wpre,
-                                               uint32_t words, const uint32_t* vbase, VOut out, char* scratch,
-                                               size_t slice, uint32_t* overflow, uint32_t* n_overflow,
-                                               unsigned long long* evals_total) {
+// One workgroup (4 wavefronts) per sub-block with crossing edges (`work[w]`
+// = 8*brick + sb): gather_local into LDS (or scratch slice w), compact the
+// crossing edges whose lower sample lies in the sub-block, then refine them
+// one per wavefront. Sub-blocks whose lists overflow go to `overflow`.
+__global__ void __launch_bounds__(256) k_refine(DArr A, DRun P, RCaps cap, RLayout lay, const int32_t* keys,
+                                                const float* values, const uint32_t* work, uint32_t n_work,
+                                                const unsigned long long* mask, const uint32_t* wpre,
+                                                uint32_t words, const uint32_t* vbase, VOut out, char* scratch,
+                                                size_t slice, uint32_t* overflow, uint32_t* n_overflow,
+                                                unsigned long long* evals_total) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     __shared__ uint32_t n[R_N];
     uint32_t w = blockIdx.x;
@@ -588,7 +739,7 @@ __global__ void __launch_bounds__(64) k_refine(DArr A, DRun P, RCaps cap, RLayou
     char* base = scratch ? scratch + (size_t)w * slice : smem;
     RLists s = rcarve(base, lay);
     const int32_t* key = keys + 3 * brick;
-    const uint32_t lane = threadIdx.x, b = P.b, sub = P.sub;
+    const uint32_t b = P.b, sub = P.sub;
     double h = P.h, subd = (double)sub;
     // Ball of the sub-block (mesh/extract.rs:164-205).
     double rho = 0.5 * sqrt(3.0) * subd * h * (1.0 + 1e-9) + 1e-9;
@@ -600,12 +751,8 @@ __global__ void __launch_bounds__(64) k_refine(DArr A, DRun P, RCaps cap, RLayou
         double basep = (double)org[a] * h;
         c[a] = basep + ((double)(o3[a] * sub) + 0.5 * subd) * h;
     }
-    if (!gather_local_wave(A, cap, s, n, c[0], c[1], c[2], rho, P.band)) {
-        if (lane == 0) overflow[atomicAdd(n_overflow, 1u)] = j;
-        return;
-    }
-    const float* val = values + (size_t)brick * P.spb;
-    // Edges whose lower sample lies in this sub-block: x in [lo, hi) per axis.
+    bool ok = gather_local_block(A, cap, s, n, c[0], c[1], c[2], rho, P.band);
+    // Crossing edges whose lower sample lies in this sub-block.
     uint32_t lo[3], hi[3];
     for (int a = 0; a < 3; a++) {
         lo[a] = o3[a] * sub;
@@ -613,50 +760,42 @@ __global__ void __launch_bounds__(64) k_refine(DArr A, DRun P, RCaps cap, RLayou
     }
     uint32_t ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
     uint32_t ncand = ex * ey * ez * 3;
+    if (ok) {
+        for (uint32_t q = threadIdx.x; q < ncand; q += blockDim.x) {
+            uint32_t axis = q % 3, cell = q / 3;
+            uint32_t x = lo[0] + cell % ex, y = lo[1] + (cell / ex) % ey, z = lo[2] + cell / (ex * ey);
+            uint32_t id = ((z * b + y) * b + x) * 3 + axis;
+            if (edge_is_crossing(mask, words, brick, id)) {
+                uint32_t at = atomicAdd(&n[R_E], 1u);
+                if (at < cap.edges)
+                    s.ebuf[at] = id;
+                else
+                    n[R_OVER] = 1u;
+            }
+        }
+        __syncthreads();
+        ok = n[R_OVER] == 0u;
+    }
+    if (!ok) {
+        if (threadIdx.x == 0) overflow[atomicAdd(n_overflow, 1u)] = j;
+        return;
+    }
+    const float* val = values + (size_t)brick * P.spb;
+    const uint32_t wv = threadIdx.x / WAVE, nwv = blockDim.x / WAVE;
     unsigned long long my_evals = 0;
     uint32_t vb = vbase[brick];
-    // Crossing edges are compacted into a buffer by ballot and refined 64 at
-    // a time, one per lane.
-    auto refine_id = [&](uint32_t id) {
+    double* dm = s.dm + (size_t)wv * (cap.s + cap.c + cap.v);
+    for (uint32_t q = wv; q < n[R_E]; q += nwv) {
+        uint32_t id = s.ebuf[q];
         uint32_t axis = id % 3, cell = id / 3;
         uint32_t x = cell % b, y = (cell / b) % b, z = cell / (b * b);
         uint32_t i0 = x + P.side * (y + P.side * z);
         uint32_t st = axis == 0 ? 1u : (axis == 1 ? P.side : P.side * P.side);
         int64_t ia[3] = {org[0] + x, org[1] + y, org[2] + z};
         size_t slot = (size_t)vb + edge_rank(mask, wpre, words, brick, id);
-        my_evals += refine_vertex(A, P, s, n, ia, axis, val[i0], val[i0 + st], out, slot);
-    };
-    uint32_t fill = 0;
-    for (uint32_t c0 = 0; c0 < ncand; c0 += WAVE) {
-        uint32_t q = c0 + lane;
-        bool cross = false;
-        uint32_t id = 0;
-        if (q < ncand) {
-            uint32_t axis = q % 3, cell = q / 3;
-            uint32_t x = lo[0] + cell % ex, y = lo[1] + (cell / ex) % ey, z = lo[2] + cell / (ex * ey);
-            id = ((z * b + y) * b + x) * 3 + axis;
-            cross = edge_is_crossing(mask, words, brick, id);
-        }
-        unsigned long long m = __ballot(cross);
-        uint32_t pos = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-        if (cross) s.ebuf[fill + pos] = id;
-        fill += (uint32_t)__popcll(m);
-        __syncthreads();
-        if (fill >= WAVE) {
-            refine_id(s.ebuf[lane]);
-            uint32_t rest = fill - WAVE;
-            uint32_t moved = lane < rest ? s.ebuf[WAVE + lane] : 0u;
-            __syncthreads();
-            if (lane < rest) s.ebuf[lane] = moved;
-            __syncthreads();
-            fill = rest;
-        }
+        my_evals += refine_vertex_wave(A, P, s, n, cap.c, dm, ia, axis, val[i0], val[i0 + st], out, slot);
     }
-    if (lane < fill) refine_id(s.ebuf[lane]);
-    if (evals_total) {
-        for (int o = 32; o > 0; o >>= 1) my_evals += __shfl_xor(my_evals, o, WAVE);
-        if (lane == 0) atomicAdd(evals_total, my_evals);
-    }
+    if (evals_total && lane_id() == 0 && my_evals) atomicAdd(evals_total, my_evals);
 }
 
 }  // namespace smg

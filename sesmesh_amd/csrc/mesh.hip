@@ -204,7 +204,7 @@ struct smg_device {
     int ordinal = 0;
     hipDeviceProp_t prop{};
     hipStream_t stream = nullptr;
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[16] = {};
     uint32_t* host_u32 = nullptr;  // pinned staging for counts
 };
 
@@ -235,7 +235,7 @@ struct smg_mesh {
     smg_device* dev = nullptr;
     DRun P{};
     size_t touched = 0, bricks = 0, vertices = 0, triangles = 0, sampled = 0, overflow = 0;
-    uint64_t evals = 0;
+    uint64_t evals = 0, samples = 0;
     uint32_t words = 0;
     // Device arrays owned by the mesh.
     int32_t* keys = nullptr;
@@ -657,10 +657,10 @@ int read_u32(smg_device* d, const uint32_t* dev_ptr, uint32_t& v) {
 
 // List capacities: an LDS variant sized for common sub-blocks and a
 // large one in global scratch for the rest.
-constexpr Caps SAMPLE_SMALL{160, 96, 128, 256, 128, 3072, 3072, 512};
+constexpr Caps SAMPLE_SMALL{160, 96, 128, 128, 128, 2048, 2048, 512};
 constexpr Caps SAMPLE_LARGE{4096, 2048, 2048, 8192, 2048, 65536, 65536, 8192};
-constexpr RCaps REFINE_SMALL{192, 96, 128, 256, 128, 3072, 3072, 512};
-constexpr RCaps REFINE_LARGE{4096, 2048, 2048, 8192, 2048, 65536, 65536, 8192};
+constexpr RCaps REFINE_SMALL{128, 64, 64, 128, 64, 1024, 1024, 256, 512};
+constexpr RCaps REFINE_LARGE{4096, 2048, 2048, 8192, 2048, 65536, 65536, 8192, 12288};
 constexpr uint32_t LARGE_CHUNK = 256;
 
 }  // namespace
@@ -776,7 +776,9 @@ static int mesh_stage(smg_resident* r, const smg_mesh_params* prm, smg_mesh* M, 
     HIP_TRY(T.get(&koff, ntouched));
     k_bits_to_list<<<blocks_for(nwords, 256), 256, 0, s>>>(bits, woff, nwords, touched);
     // Brick classification (lattice.rs:168-183), one wavefront per brick.
-    if (ntouched) k_classify_bricks<<<blocks_for((size_t)ntouched * WAVE, 256), 256, 0, s>>>(A, P, touched, ntouched, keep);
+    HIP_TRY(hipEventRecord(ev[6], s));
+    if (ntouched) k_classify_bricks<<<ntouched, WAVE, 0, s>>>(A, P, touched, ntouched, keep);
+    HIP_TRY(hipEventRecord(ev[7], s));
     uint32_t* nb_d;
     TRY(scan_u32(s, T, keep, koff, ntouched, &nb_d));
     uint32_t nb;
@@ -787,7 +789,9 @@ static int mesh_stage(smg_resident* r, const smg_mesh_params* prm, smg_mesh* M, 
     if (ntouched) k_compact_bricks<<<blocks_for(ntouched, 256), 256, 0, s>>>(P, touched, keep, koff, ntouched, M->keys, dense);
     uint8_t* st;
     HIP_TRY(T.get(&st, 8 * (size_t)nb));
-    if (nb) k_sub_states<<<blocks_for((size_t)8 * nb * WAVE, 256), 256, 0, s>>>(A, P, M->keys, nb, st);
+    HIP_TRY(hipEventRecord(ev[8], s));
+    if (nb) k_sub_states<<<8 * nb, WAVE, 0, s>>>(A, P, M->keys, nb, st);
+    HIP_TRY(hipEventRecord(ev[9], s));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[1], s));
 
@@ -807,13 +811,18 @@ static int mesh_stage(smg_resident* r, const smg_mesh_params* prm, smg_mesh* M, 
     TRY(read_u32(d, nsw_d, nsw));
     M->sampled = nsw;
     uint32_t *ovf, *novf;
+    unsigned long long* nsamp;
+    HIP_TRY(T.get(&nsamp, 1));
+    HIP_TRY(hipMemsetAsync(nsamp, 0, 8, s));
     HIP_TRY(T.get(&ovf, std::max<uint32_t>(nsw, 1)));
     HIP_TRY(T.get(&novf, 1));
     HIP_TRY(hipMemsetAsync(novf, 0, 4, s));
     Layout lay_s = make_layout(SAMPLE_SMALL);
+    HIP_TRY(hipEventRecord(ev[10], s));
     if (nsw)
         k_sample<<<nsw, 256, lay_s.bytes, s>>>(A, L, P, SAMPLE_SMALL, lay_s, M->keys, st, swork, nsw, M->values,
-                                                nullptr, 0, ovf, novf);
+                                                nullptr, 0, ovf, novf, nsamp);
+    HIP_TRY(hipEventRecord(ev[11], s));
     HIP_TRY(hipGetLastError());
     uint32_t nov;
     TRY(read_u32(d, novf, nov));
@@ -831,7 +840,7 @@ static int mesh_stage(smg_resident* r, const smg_mesh_params* prm, smg_mesh* M, 
         for (uint32_t c0 = 0; c0 < nov; c0 += chunk) {
             uint32_t cnt = std::min(chunk, nov - c0);
             k_sample<<<cnt, 256, 0, s>>>(A, L, P, SAMPLE_LARGE, lay_l, M->keys, st, ovf + c0, cnt, M->values,
-                                          scratch, lay_l.bytes, ovf2, novf2);
+                                          scratch, lay_l.bytes, ovf2, novf2, nsamp);
         }
         HIP_TRY(hipGetLastError());
         uint32_t bad;
@@ -883,9 +892,11 @@ static int mesh_stage(smg_resident* r, const smg_mesh_params* prm, smg_mesh* M, 
     RLayout rl_s = make_rlayout(REFINE_SMALL);
     uint32_t* rovf;
     HIP_TRY(T.get(&rovf, std::max<uint32_t>(nrw, 1)));
+    HIP_TRY(hipEventRecord(ev[12], s));
     if (nrw)
-        k_refine<<<nrw, 64, rl_s.bytes, s>>>(A, P, REFINE_SMALL, rl_s, M->keys, M->values, rwork, nrw, mask, wpre,
+        k_refine<<<nrw, 256, rl_s.bytes, s>>>(A, P, REFINE_SMALL, rl_s, M->keys, M->values, rwork, nrw, mask, wpre,
                                               words, vbase, vo, nullptr, 0, rovf, novf, evd);
+    HIP_TRY(hipEventRecord(ev[13], s));
     HIP_TRY(hipGetLastError());
     uint32_t rnov;
     TRY(read_u32(d, novf, rnov));
@@ -900,7 +911,7 @@ static int mesh_stage(smg_resident* r, const smg_mesh_params* prm, smg_mesh* M, 
         HIP_TRY(hipMemsetAsync(n2, 0, 4, s));
         for (uint32_t c0 = 0; c0 < rnov; c0 += chunk) {
             uint32_t cnt = std::min(chunk, rnov - c0);
-            k_refine<<<cnt, 64, 0, s>>>(A, P, REFINE_LARGE, rl_l, M->keys, M->values, rovf + c0, cnt, mask, wpre,
+            k_refine<<<cnt, 256, 0, s>>>(A, P, REFINE_LARGE, rl_l, M->keys, M->values, rovf + c0, cnt, mask, wpre,
                                          words, vbase, vo, scratch, rl_l.bytes, o2, n2, evd);
         }
         HIP_TRY(hipGetLastError());
@@ -930,9 +941,11 @@ static int mesh_stage(smg_resident* r, const smg_mesh_params* prm, smg_mesh* M, 
     uint32_t bad;
     TRY(read_u32(d, err, bad));
     if (bad) return fail(SMG_E_INTERNAL, "a triangle references an edge without a vertex");
-    unsigned long long evh = 0;
+    unsigned long long evh = 0, sh = 0;
     HIP_TRY(hipMemcpy(&evh, evd, 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&sh, nsamp, 8, hipMemcpyDeviceToHost));
     M->evals = evh;
+    M->samples = sh;
     if (times) {
         float ms[5];
         for (int q = 0; q < 5; q++) HIP_TRY(hipEventElapsedTime(&ms[q], ev[q], ev[q + 1]));
@@ -943,6 +956,12 @@ static int mesh_stage(smg_resident* r, const smg_mesh_params* prm, smg_mesh* M, 
         times->cells_ms = ms[4];
         HIP_TRY(hipEventElapsedTime(&ms[0], ev[0], ev[5]));
         times->total_ms = ms[0];
+        float k[4];
+        for (int q = 0; q < 4; q++) HIP_TRY(hipEventElapsedTime(&k[q], ev[6 + 2 * q], ev[7 + 2 * q]));
+        times->k_classify_ms = k[0];
+        times->k_substates_ms = k[1];
+        times->k_sample_ms = k[2];
+        times->k_refine_ms = k[3];
     }
     return SMG_OK;
 }
@@ -983,6 +1002,7 @@ extern "C" int smg_mesh_info_get(const smg_mesh* m, smg_mesh_info* info) {
     info->refine_evaluations = m->evals;
     info->sampled_subblocks = m->sampled;
     info->overflow_subblocks = m->overflow;
+    info->evaluated_samples = m->samples;
     for (int q = 0; q < 3; q++) info->frame[q] = m->P.frame[q];
     info->band = m->P.band;
     info->edge_clamp = m->P.t_min;
