@@ -21,26 +21,6 @@ __device__ inline uint32_t cell_case(const float* val, uint32_t side, uint32_t x
     return cs;
 }
 
-// Block-wide exclusive scan of one value per thread (256 threads).
-__device__ inline uint32_t block_scan_256(uint32_t v, uint32_t* tmp, uint32_t& total) {
-    uint32_t lane = lane_id(), wv = threadIdx.x / WAVE;
-    uint32_t inc = v;
-    for (int o = 1; o < WAVE; o <<= 1) {
-        uint32_t y = __shfl_up(inc, o, WAVE);
-        if ((int)lane >= o) inc += y;
-    }
-    if (lane == WAVE - 1) tmp[wv] = inc;
-    __syncthreads();
-    uint32_t before = 0, all = 0;
-    for (uint32_t q = 0; q < blockDim.x / WAVE; q++) {
-        if (q < wv) before += tmp[q];
-        all += tmp[q];
-    }
-    __syncthreads();
-    total = all;
-    return before + inc - v;
-}
-
 // Triangles per brick: thread t takes the contiguous cells [t*per, (t+1)*per)
 // in (z, y, x) order.
 __global__ void __launch_bounds__(256) k_cell_count(DRun P, const float* values, uint32_t n_bricks,

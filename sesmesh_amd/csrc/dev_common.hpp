@@ -61,6 +61,18 @@ struct DRun {
     float h32, fband, fprobe, fcutoff2;
 };
 
+// One molecule of a run: its resident arrangement, its f32 copies and its
+// lattice, plus where its per-molecule arrays start in the run's
+// concatenated buffers. A single mesh is a run of one molecule.
+struct Mol {
+    DArr A;
+    DLoc L;
+    DRun P;
+    uint64_t word_off;   // first word of its touched-brick bitmap
+    uint64_t dense_off;  // first entry of its dense brick-index grid
+    uint32_t first_brick, n_bricks;
+};
+
 __device__ inline double ddist(double ax, double ay, double az, double bx, double by, double bz) {
     double x = ax - bx, y = ay - by, z = az - bz;
     return sqrt(x * x + y * y + z * z);
@@ -119,6 +131,174 @@ __device__ inline uint64_t total_key(double d) {
     int64_t i = __double_as_longlong(d);
     i ^= (int64_t)(((uint64_t)(i >> 63)) >> 1);
     return (uint64_t)i ^ 0x8000000000000000ull;
+}
+// Inverse of total_key for keys of non-negative values.
+__device__ inline double key_value(uint64_t k) { return __longlong_as_double((int64_t)(k ^ 0x8000000000000000ull)); }
+
+// Smallest s >= 0 with sqrt(s) >= m. A correctly rounded sqrt is
+// monotonic, so for every s >= 0: sqrt(s) >= m  <=>  s >= threshold. The
+// distance tests `ddist(p, c) < R` and `!(ddist(p, c) >= R - EPS)` thus
+// become `s < threshold` on the same sum of squares, without the sqrt.
+__device__ inline double sqrt_threshold(double m) {
+    if (!(m > 0.0)) return 0.0;
+    double s = m * m;
+    if (sqrt(s) >= m) {
+        for (;;) {
+            double p = __longlong_as_double(__double_as_longlong(s) - 1);
+            if (p >= 0.0 && sqrt(p) >= m)
+                s = p;
+            else
+                break;
+        }
+    } else {
+        do {
+            s = __longlong_as_double(__double_as_longlong(s) + 1);
+        } while (!(sqrt(s) >= m));
+    }
+    return s;
+}
+
+// Block-wide exclusive scan of one value per thread (blockDim.x a multiple
+// of 64, at most 256); `tmp` holds 4 words of LDS.
+__device__ inline uint32_t block_scan_256(uint32_t v, uint32_t* tmp, uint32_t& total) {
+    uint32_t lane = lane_id(), wv = threadIdx.x / WAVE;
+    uint32_t inc = v;
+    for (int o = 1; o < WAVE; o <<= 1) {
+        uint32_t y = __shfl_up(inc, o, WAVE);
+        if ((int)lane >= o) inc += y;
+    }
+    if (lane == WAVE - 1) tmp[wv] = inc;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (uint32_t q = 0; q < blockDim.x / WAVE; q++) {
+        if (q < wv) before += tmp[q];
+        all += tmp[q];
+    }
+    __syncthreads();
+    total = all;
+    return before + inc - v;
+}
+
+// Open-addressing map global sphere id -> near-list slot in LDS.
+__device__ inline void hash_put(uint32_t* hk, uint32_t* hv, uint32_t mask, uint32_t key, uint32_t val) {
+    uint32_t h = (key * 2654435761u) & mask;
+    for (uint32_t probe = 0; probe <= mask; probe++) {
+        uint32_t prev = atomicCAS(&hk[h], 0u, key + 1u);
+        if (prev == 0u) {
+            hv[h] = val;
+            return;
+        }
+        h = (h + 1u) & mask;
+    }
+}
+__device__ inline uint32_t hash_get(const uint32_t* hk, const uint32_t* hv, uint32_t mask, uint32_t key) {
+    uint32_t h = (key * 2654435761u) & mask;
+    for (uint32_t probe = 0; probe <= mask; probe++) {
+        uint32_t k = hk[h];
+        if (k == key + 1u) return hv[h];
+        if (k == 0u) return 0xffffffffu;
+        h = (h + 1u) & mask;
+    }
+    return 0xffffffffu;
+}
+
+constexpr uint32_t KIND_SHIFT = 30, ID_MASK = (1u << KIND_SHIFT) - 1u;
+
+__host__ __device__ inline uint32_t pow2_at_least(uint32_t v) {
+    uint32_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+// Sorts candidates by (class, key, index) -- class in the top bits of id --
+// ascending, by the whole workgroup (bitonic network over a power of two).
+__device__ inline void sort_candidates(unsigned long long* key, uint32_t* id, uint32_t count) {
+    uint32_t n = pow2_at_least(count);
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    for (uint32_t i = count + tid; i < n; i += nt) {
+        key[i] = ~0ull;
+        id[i] = 0xffffffffu;
+    }
+    __syncthreads();
+    for (uint32_t k = 2; k <= n; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = tid; i < n; i += nt) {
+                uint32_t l = i ^ j;
+                if (l > i) {
+                    unsigned long long ki = key[i], kl = key[l];
+                    uint32_t ii = id[i], il = id[l];
+                    uint32_t ci = ii >> KIND_SHIFT, cl = il >> KIND_SHIFT;
+                    bool gt = ci > cl || (ci == cl && (ki > kl || (ki == kl && ii > il)));
+                    if (gt == ((i & k) == 0)) {
+                        key[i] = kl;
+                        key[l] = ki;
+                        id[i] = il;
+                        id[l] = ii;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+}
+
+constexpr uint32_t ROWS_MAX = 256;
+
+// The items of the cells of the sphere, circle and vertex grids that
+// overlap cubes of half-sizes r[0], r[1], r[2] about (x, y, z) (three
+// CellGrid::for_each_near), visited by the whole workgroup as one flat
+// range: the rows' item ranges are read in parallel, one row per thread,
+// then each thread takes every blockDim.x-th item, so a gather costs three
+// dependent memory round trips instead of three per row. f(grid, item);
+// the visiting order is unspecified. Every thread of the block must call it.
+template <class F>
+__device__ inline void gather3(const DArr& A, double x, double y, double z, const double r[3], uint32_t* tmp4,
+                               F&& f) {
+    __shared__ uint32_t rs[ROWS_MAX], rp[ROWS_MAX];
+    Box b0 = grid_box(A.gs, x, y, z, r[0]);
+    Box b1 = grid_box(A.gc, x, y, z, r[1]);
+    Box b2 = grid_box(A.gv, x, y, z, r[2]);
+    uint32_t n0 = (b0.z1 - b0.z0 + 1) * (b0.y1 - b0.y0 + 1);
+    uint32_t n1 = (b1.z1 - b1.z0 + 1) * (b1.y1 - b1.y0 + 1);
+    uint32_t n2 = (b2.z1 - b2.z0 + 1) * (b2.y1 - b2.y0 + 1);
+    const uint32_t tid = threadIdx.x, nt = blockDim.x, nrows = n0 + n1 + n2;
+    if (nrows > ROWS_MAX || nrows > nt) {
+        for_near(A.gs, x, y, z, r[0], tid, nt, [&](uint32_t k) { f(0u, k); });
+        for_near(A.gc, x, y, z, r[1], tid, nt, [&](uint32_t k) { f(1u, k); });
+        for_near(A.gv, x, y, z, r[2], tid, nt, [&](uint32_t k) { f(2u, k); });
+        return;
+    }
+    uint32_t len = 0, start = 0;
+    if (tid < nrows) {
+        uint32_t k = tid < n0 ? 0u : (tid < n0 + n1 ? 1u : 2u);
+        uint32_t q = tid - (k == 0 ? 0u : (k == 1 ? n0 : n0 + n1));
+        const DGrid& G = k == 0 ? A.gs : (k == 1 ? A.gc : A.gv);
+        const Box& B = k == 0 ? b0 : (k == 1 ? b1 : b2);
+        uint32_t ny = B.y1 - B.y0 + 1;
+        uint32_t row = ((B.z0 + q / ny) * G.dy + (B.y0 + q % ny)) * G.dx;
+        start = G.starts[row + B.x0];
+        len = G.starts[row + B.x1 + 1] - start;
+    }
+    uint32_t total;
+    uint32_t pre = block_scan_256(len, tmp4, total);
+    if (tid < nrows) {
+        rs[tid] = start;
+        rp[tid] = pre;
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < total; i += nt) {
+        uint32_t lo = 0, hi = nrows - 1;  // last row whose range starts at or before i
+        while (lo < hi) {
+            uint32_t mid = (lo + hi + 1) >> 1;
+            if (rp[mid] <= i)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        uint32_t k = lo < n0 ? 0u : (lo < n0 + n1 ? 1u : 2u);
+        const uint32_t* items = k == 0 ? A.gs.items : (k == 1 ? A.gc.items : A.gv.items);
+        f(k, items[rs[lo] + (i - rp[lo])]);
+    }
+    __syncthreads();
 }
 
 }  // namespace smg

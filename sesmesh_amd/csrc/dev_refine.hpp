@@ -3,20 +3,25 @@
 // Restates the edge scan of mesh_brick (reference mesh/extract.rs:146-223),
 // SesField::gather_local / eval_local (field/point.rs:419-489, 496-565,
 // 686-722) and vertex / gradient / attributes (mesh/extract.rs:425-597) in
-// f64 with the CPU's operations, candidate order (sorted by f64::total_cmp
-// then index) and iteration rules, so every vertex, normal, atom and patch
-// equals the CPU back end's bit for bit.
+// f64 with the CPU's operations and iteration rules, so every vertex,
+// normal, atom and patch equals the CPU back end's bit for bit.
 //
-// One workgroup (4 wavefronts) per sub-block with crossing edges gathers the
-// sub-block's candidate lists into LDS with their f64 data, once; then each
-// wavefront refines one edge at a time, evaluating each point cooperatively:
-// lanes split the candidates, and the winner is found best-first, by
-// (distance, list position) over the candidates, checking the blockers of
-// the nearest unchecked candidate until one is free. This selects exactly
-// the candidate eval_local's sequential first-wins loop selects: the first,
-// in list order, among the free candidates of least distance below the
-// cutoff (a candidate taken earlier in the sequential loop is always
-// replaced by it, and none after it improves on it).
+// One workgroup (256 threads) per sub-block with crossing edges. It gathers
+// the sub-block's lists into LDS once: the near spheres, the inside group,
+// and the S, C and V candidates sorted as the CPU orders them (class, then
+// f64::total_cmp of the distance at the ball centre, then index), each
+// with its f64 data and its blockers as a bitset over near-list slots.
+// The crossing edges are then refined by lane groups: g lanes per edge
+// (g = 64 / 2^k, as many groups as edges allow). An evaluation point is
+// split over the group's lanes: each lane scans its share of the
+// candidates in list order, keeping the least (distance, list position)
+// among the free ones below the cutoff, and the group takes the least of
+// its lanes. That is the candidate eval_local's sequential loop keeps: the
+// first, in list order, of the free candidates at the least distance. A
+// lane stops a class's scan once the candidate's centre distance minus the
+// point's offset from the centre (a lower bound of its distance at the
+// point, the distances being 1-Lipschitz) exceeds its best by a margin far
+// above f64 rounding: no later candidate can be taken.
 #pragma once
 #include "dev_common.hpp"
 
@@ -85,22 +90,17 @@ __device__ inline bool edge_is_crossing(const unsigned long long* mask, uint32_t
     return (mask[(size_t)brick * words + id / 64] >> (id % 64)) & 1ull;
 }
 
-// Capacities of the refinement lists (s, c, v powers of two).
+// Capacities of the refinement lists: near spheres (a multiple of 64: the
+// blocker bitsets hold near/64 words), inside group, candidates (a power
+// of two), hash slots (a power of two) and crossing edges.
 struct RCaps {
-    uint32_t near, inside, s, c, v, sblk, cblk, hash, edges;
+    uint32_t near, inside, cand, hash, edges;
 };
 
-// LDS / scratch layout of the refinement lists: f64 SoA data.
+// LDS / scratch layout of the refinement lists (byte offsets).
 struct RLayout {
-    uint32_t nx, ny, nz, nrm, nid, hk, hv, ix, iy, iz, iR, skey, sid, sx, sy, sz, sR, sb0, sb1, ckey, cid, cd, cb0,
-        cb1, vkey, vid, vx, vy, vz, sblk, cblk, ebuf, dm, bytes;
+    uint32_t nx, ny, nz, nin, nblk, hk, hv, ins, ckey, cid, cdat, cbits, coff, cpre, ebuf, bytes;
 };
-
-__host__ __device__ inline uint32_t pow2_at_least(uint32_t v) {
-    uint32_t p = 1;
-    while (p < v) p <<= 1;
-    return p;
-}
 
 __host__ __device__ inline RLayout make_rlayout(const RCaps& c) {
     RLayout L;
@@ -113,61 +113,32 @@ __host__ __device__ inline RLayout make_rlayout(const RCaps& c) {
     L.nx = take(8 * c.near);
     L.ny = take(8 * c.near);
     L.nz = take(8 * c.near);
-    L.nrm = take(8 * c.near);
-    L.nid = take(4 * c.near);
+    L.nin = take(8 * c.near);   // inside threshold of R
+    L.nblk = take(8 * c.near);  // blocking threshold of R - EPS
     L.hk = take(4 * c.hash);
     L.hv = take(4 * c.hash);
-    L.ix = take(8 * c.inside);
-    L.iy = take(8 * c.inside);
-    L.iz = take(8 * c.inside);
-    L.iR = take(8 * c.inside);
-    L.skey = take(8 * c.s);
-    L.sid = take(4 * c.s);
-    L.sx = take(8 * c.s);
-    L.sy = take(8 * c.s);
-    L.sz = take(8 * c.s);
-    L.sR = take(8 * c.s);
-    L.sb0 = take(4 * c.s);
-    L.sb1 = take(4 * c.s);
-    L.ckey = take(8 * c.c);
-    L.cid = take(4 * c.c);
-    L.cd = take(80 * c.c);  // 10 f64 per circle, SoA of c.c each
-    L.cb0 = take(4 * c.c);
-    L.cb1 = take(4 * c.c);
-    L.vkey = take(8 * c.v);
-    L.vid = take(4 * c.v);
-    L.vx = take(8 * c.v);
-    L.vy = take(8 * c.v);
-    L.vz = take(8 * c.v);
-    L.sblk = take(2 * c.sblk);
-    L.cblk = take(2 * c.cblk);
+    L.ins = take(2 * c.inside);
+    L.ckey = take(8 * c.cand);
+    L.cid = take(4 * c.cand);
+    L.cdat = take(7 * 8 * c.cand);
+    L.cbits = take(c.near / 64 * 8 * c.cand);
+    L.coff = take(4 * c.cand);
+    L.cpre = take(4 * c.cand);
     L.ebuf = take(4 * c.edges);
-    // Candidate distances per wavefront, for lists longer than the
-    // register budget (4 wavefronts per workgroup).
-    uint32_t tot = c.s + c.c + c.v;
-    L.dm = take(tot > 256u ? 4u * 8u * tot : 0u);
     L.bytes = o;
     return L;
 }
 
 struct RLists {
-    double *nx, *ny, *nz, *nrm;
-    uint32_t *nid, *hk, *hv;
-    double *ix, *iy, *iz, *iR;
-    unsigned long long* skey;
-    uint32_t* sid;
-    double *sx, *sy, *sz, *sR;
-    uint32_t *sb0, *sb1;
+    double *nx, *ny, *nz, *nin, *nblk;
+    uint32_t *hk, *hv;
+    uint16_t* ins;
     unsigned long long* ckey;
     uint32_t* cid;
-    double* cd;
-    uint32_t *cb0, *cb1;
-    unsigned long long* vkey;
-    uint32_t* vid;
-    double *vx, *vy, *vz;
-    uint16_t *sblk, *cblk;
+    double* cdat;  // 7 SoA rows of `cand` values
+    unsigned long long* cbits;  // near/64 rows of `cand` words
+    uint32_t *coff, *cpre;
     uint32_t* ebuf;
-    double* dm;
 };
 
 __device__ inline RLists rcarve(char* base, const RLayout& L) {
@@ -175,213 +146,160 @@ __device__ inline RLists rcarve(char* base, const RLayout& L) {
     s.nx = (double*)(base + L.nx);
     s.ny = (double*)(base + L.ny);
     s.nz = (double*)(base + L.nz);
-    s.nrm = (double*)(base + L.nrm);
-    s.nid = (uint32_t*)(base + L.nid);
+    s.nin = (double*)(base + L.nin);
+    s.nblk = (double*)(base + L.nblk);
     s.hk = (uint32_t*)(base + L.hk);
     s.hv = (uint32_t*)(base + L.hv);
-    s.ix = (double*)(base + L.ix);
-    s.iy = (double*)(base + L.iy);
-    s.iz = (double*)(base + L.iz);
-    s.iR = (double*)(base + L.iR);
-    s.skey = (unsigned long long*)(base + L.skey);
-    s.sid = (uint32_t*)(base + L.sid);
-    s.sx = (double*)(base + L.sx);
-    s.sy = (double*)(base + L.sy);
-    s.sz = (double*)(base + L.sz);
-    s.sR = (double*)(base + L.sR);
-    s.sb0 = (uint32_t*)(base + L.sb0);
-    s.sb1 = (uint32_t*)(base + L.sb1);
+    s.ins = (uint16_t*)(base + L.ins);
     s.ckey = (unsigned long long*)(base + L.ckey);
     s.cid = (uint32_t*)(base + L.cid);
-    s.cd = (double*)(base + L.cd);
-    s.cb0 = (uint32_t*)(base + L.cb0);
-    s.cb1 = (uint32_t*)(base + L.cb1);
-    s.vkey = (unsigned long long*)(base + L.vkey);
-    s.vid = (uint32_t*)(base + L.vid);
-    s.vx = (double*)(base + L.vx);
-    s.vy = (double*)(base + L.vy);
-    s.vz = (double*)(base + L.vz);
-    s.sblk = (uint16_t*)(base + L.sblk);
-    s.cblk = (uint16_t*)(base + L.cblk);
+    s.cdat = (double*)(base + L.cdat);
+    s.cbits = (unsigned long long*)(base + L.cbits);
+    s.coff = (uint32_t*)(base + L.coff);
+    s.cpre = (uint32_t*)(base + L.cpre);
     s.ebuf = (uint32_t*)(base + L.ebuf);
-    s.dm = (double*)(base + L.dm);
     return s;
 }
 
-enum { R_NEAR = 0, R_IN, R_S, R_C, R_V, R_SB, R_CB, R_E, R_OVER, R_N };
-
-// Sorts (key, id) ascending by key then id (count <= cap, cap a power of
-// two), by the whole workgroup (bitonic network).
-__device__ inline void block_sort(unsigned long long* key, uint32_t* id, uint32_t count) {
-    uint32_t n = pow2_at_least(count);
-    const uint32_t tid = threadIdx.x, nt = blockDim.x;
-    for (uint32_t i = count + tid; i < n; i += nt) {
-        key[i] = ~0ull;
-        id[i] = 0xffffffffu;
-    }
-    __syncthreads();
-    for (uint32_t k = 2; k <= n; k <<= 1)
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = tid; i < n; i += nt) {
-                uint32_t l = i ^ j;
-                if (l > i) {
-                    unsigned long long ki = key[i], kl = key[l];
-                    uint32_t ii = id[i], il = id[l];
-                    bool gt = ki > kl || (ki == kl && ii > il);
-                    if (gt == ((i & k) == 0)) {
-                        key[i] = kl;
-                        key[l] = ki;
-                        id[i] = il;
-                        id[l] = ii;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-}
+enum { R_NEAR = 0, R_IN, R_S, R_C, R_V, R_CAND, R_E, R_OVER, R_N };
 
 // gather_local (field/point.rs:496-565) of the ball (b, rho) by the
-// workgroup, with the f64 data of every list in sorted order. False (in
-// every thread) if a capacity was exceeded.
-__device__ inline bool gather_local_block(const DArr& A, const RCaps& cap, RLists& s, uint32_t* n, double bx,
-                                          double by, double bz, double rho, double band) {
+// workgroup: near spheres with their thresholds, the inside group as
+// near-list slots, the S / C / V candidates in the CPU's list order with
+// their f64 data and blocker bitsets (row members that are near,
+// point.rs:152-155). False (in every thread) if a capacity was exceeded.
+__device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s, uint32_t* n, uint32_t* tmp4,
+                                     double bx, double by, double bz, double rho, double band) {
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     for (uint32_t i = tid; i < cap.hash; i += nt) s.hk[i] = 0u;
     if (tid < R_N) n[tid] = 0u;
     __syncthreads();
     const uint32_t hmask = cap.hash - 1;
-    double reach = A.probe + band + rho;
-    for_near(A.gs, bx, by, bz, reach + A.max_expanded, tid, nt, [&](uint32_t k) {
-        if (A.status[k] == S_CONTAINED) return;
-        double R = A.R[k];
-        double d = ddist(bx, by, bz, A.cx[k], A.cy[k], A.cz[k]);
-        if (d < R + reach) {
-            uint32_t at = atomicAdd(&n[R_NEAR], 1u);
-            if (at >= cap.near) {
-                n[R_OVER] = 1u;
-            } else {
-                s.nx[at] = A.cx[k];
-                s.ny[at] = A.cy[k];
-                s.nz[at] = A.cz[k];
-                s.nrm[at] = R - EPS;
-                s.nid[at] = k;
+    const double reach = A.probe + band + rho;
+    const double radii[3] = {reach + A.max_expanded, reach + A.rho_max, reach};
+    gather3(A, bx, by, bz, radii, tmp4, [&](uint32_t grid, uint32_t k) {
+        if (grid == 0) {
+            if (A.status[k] == S_CONTAINED) return;
+            double R = A.R[k], cx = A.cx[k], cy = A.cy[k], cz = A.cz[k];
+            double d = ddist(bx, by, bz, cx, cy, cz);
+            if (d < R + reach) {
+                uint32_t at = atomicAdd(&n[R_NEAR], 1u);
+                if (at >= cap.near) {
+                    n[R_OVER] = 1u;
+                    return;
+                }
+                s.nx[at] = cx;
+                s.ny[at] = cy;
+                s.nz[at] = cz;
+                s.nin[at] = sqrt_threshold(R);
+                s.nblk[at] = sqrt_threshold(R - EPS);
                 hash_put(s.hk, s.hv, hmask, k, at);
+                if (d < R + rho) {
+                    uint32_t q = atomicAdd(&n[R_IN], 1u);
+                    if (q >= cap.inside)
+                        n[R_OVER] = 1u;
+                    else
+                        s.ins[q] = (uint16_t)at;
+                }
             }
-        }
-        if (d < R + rho) {
-            uint32_t at = atomicAdd(&n[R_IN], 1u);
-            if (at >= cap.inside) {
+            double lb = fabs(d - R);
+            if (A.status[k] == S_EXPOSED && lb <= reach) {
+                atomicAdd(&n[R_S], 1u);
+                uint32_t q = atomicAdd(&n[R_CAND], 1u);
+                if (q >= cap.cand) {
+                    n[R_OVER] = 1u;
+                } else {
+                    s.ckey[q] = total_key(lb);
+                    s.cid[q] = k;
+                }
+            }
+        } else if (grid == 1) {
+            double yx = bx - A.mx[k], yy = by - A.my[k], yz = bz - A.mz[k];
+            double ax = A.ax[k], ay = A.ay[k], az = A.az[k];
+            double along = yx * ax + yy * ay + yz * az;
+            double wx = yx - ax * along, wy = yy - ay * along, wz = yz - az * along;
+            double wl = sqrt(wx * wx + wy * wy + wz * wz);
+            double rr = A.crho[k];
+            double dist = sqrt(along * along + (wl - rr) * (wl - rr));
+            if (!(dist <= reach)) return;
+            atomicAdd(&n[R_C], 1u);
+            uint32_t q = atomicAdd(&n[R_CAND], 1u);
+            if (q >= cap.cand) {
                 n[R_OVER] = 1u;
-            } else {
-                s.ix[at] = A.cx[k];
-                s.iy[at] = A.cy[k];
-                s.iz[at] = A.cz[k];
-                s.iR[at] = R;
+                return;
             }
-        }
-        double lb = fabs(d - R);
-        if (A.status[k] == S_EXPOSED && lb <= reach) {
-            uint32_t at = atomicAdd(&n[R_S], 1u);
-            if (at >= cap.s) {
-                n[R_OVER] = 1u;
-            } else {
-                s.skey[at] = total_key(lb);
-                s.sid[at] = k;
-            }
-        }
-    });
-    for_near(A.gc, bx, by, bz, reach + A.rho_max, tid, nt, [&](uint32_t c) {
-        double yx = bx - A.mx[c], yy = by - A.my[c], yz = bz - A.mz[c];
-        double ax = A.ax[c], ay = A.ay[c], az = A.az[c];
-        double along = yx * ax + yy * ay + yz * az;
-        double wx = yx - ax * along, wy = yy - ay * along, wz = yz - az * along;
-        double wl = sqrt(wx * wx + wy * wy + wz * wz);
-        double rr = A.crho[c];
-        double dist = sqrt(along * along + (wl - rr) * (wl - rr));
-        if (!(dist <= reach)) return;
-        uint32_t at = atomicAdd(&n[R_C], 1u);
-        if (at >= cap.c) {
-            n[R_OVER] = 1u;
-            return;
-        }
-        s.ckey[at] = total_key(dist);
-        s.cid[at] = c;
-    });
-    for_near(A.gv, bx, by, bz, reach, tid, nt, [&](uint32_t v) {
-        double d = ddist(bx, by, bz, A.vx[v], A.vy[v], A.vz[v]);
-        if (!(d <= reach)) return;
-        uint32_t at = atomicAdd(&n[R_V], 1u);
-        if (at >= cap.v) {
-            n[R_OVER] = 1u;
-            return;
-        }
-        s.vkey[at] = total_key(d);
-        s.vid[at] = v;
-    });
-    __syncthreads();
-    if (n[R_OVER]) return false;
-    block_sort(s.skey, s.sid, n[R_S]);
-    block_sort(s.ckey, s.cid, n[R_C]);
-    block_sort(s.vkey, s.vid, n[R_V]);
-    // Candidate data in sorted order, and blocker lists (row members that
-    // are near, point.rs:152-155) as near-list slots.
-    uint32_t ns = n[R_S], nc = n[R_C], nv = n[R_V];
-    const uint32_t cc = cap.c;
-    for (uint32_t i = tid; i < ns + nc + nv; i += nt) {
-        if (i < ns) {
-            uint32_t k = s.sid[i];
-            s.sx[i] = A.cx[k];
-            s.sy[i] = A.cy[k];
-            s.sz[i] = A.cz[k];
-            s.sR[i] = A.R[k];
-        } else if (i < ns + nc) {
-            uint32_t q = i - ns, c = s.cid[q];
-            double* d = s.cd;
-            d[q] = A.mx[c];
-            d[cc + q] = A.my[c];
-            d[2 * cc + q] = A.mz[c];
-            d[3 * cc + q] = A.ax[c];
-            d[4 * cc + q] = A.ay[c];
-            d[5 * cc + q] = A.az[c];
-            d[6 * cc + q] = A.e1x[c];
-            d[7 * cc + q] = A.e1y[c];
-            d[8 * cc + q] = A.e1z[c];
-            d[9 * cc + q] = A.crho[c];
+            s.ckey[q] = total_key(dist);
+            s.cid[q] = k | (1u << KIND_SHIFT);
         } else {
-            uint32_t q = i - ns - nc, v = s.vid[q];
-            s.vx[q] = A.vx[v];
-            s.vy[q] = A.vy[v];
-            s.vz[q] = A.vz[v];
-        }
-        if (i < ns + nc) {
-            bool is_s = i < ns;
-            uint32_t id = is_s ? s.sid[i] : s.cid[i - ns];
-            const uint32_t* row = is_s ? A.nb_items + A.nb_off[id] : A.cb_items + A.cb_off[id];
-            uint32_t len = is_s ? A.nb_off[id + 1] - A.nb_off[id] : A.cb_off[id + 1] - A.cb_off[id];
-            uint32_t cnt = 0;
-            for (uint32_t t = 0; t < len; t++) cnt += hash_get(s.hk, s.hv, hmask, row[t]) != 0xffffffffu;
-            uint32_t at = atomicAdd(&n[is_s ? R_SB : R_CB], cnt);
-            if (at + cnt > (is_s ? cap.sblk : cap.cblk)) {
+            double d = ddist(bx, by, bz, A.vx[k], A.vy[k], A.vz[k]);
+            if (!(d <= reach)) return;
+            atomicAdd(&n[R_V], 1u);
+            uint32_t q = atomicAdd(&n[R_CAND], 1u);
+            if (q >= cap.cand) {
                 n[R_OVER] = 1u;
-                continue;
+                return;
             }
-            uint16_t* out = is_s ? s.sblk : s.cblk;
-            uint32_t w = at;
-            for (uint32_t t = 0; t < len; t++) {
-                uint32_t l = hash_get(s.hk, s.hv, hmask, row[t]);
-                if (l != 0xffffffffu) out[w++] = (uint16_t)l;
-            }
-            if (is_s) {
-                s.sb0[i] = at;
-                s.sb1[i] = at + cnt;
-            } else {
-                s.cb0[i - ns] = at;
-                s.cb1[i - ns] = at + cnt;
-            }
+            s.ckey[q] = total_key(d);
+            s.cid[q] = k | (2u << KIND_SHIFT);
         }
+    });
+    if (n[R_OVER]) return false;
+    const uint32_t total = n[R_CAND], ns = n[R_S], nc = n[R_C];
+    sort_candidates(s.ckey, s.cid, total);
+    // Candidate data, cleared bitsets and blocker rows.
+    const uint32_t cc = cap.cand, nw = cap.near / 64;
+    uint32_t carry = 0;
+    for (uint32_t p0 = 0; p0 < total; p0 += nt) {
+        uint32_t p = p0 + tid, len = 0;
+        if (p < total) {
+            uint32_t k = s.cid[p] & ID_MASK;
+            double* d = s.cdat + p;
+            if (p < ns) {
+                d[0] = A.cx[k];
+                d[cc] = A.cy[k];
+                d[2 * cc] = A.cz[k];
+                d[3 * cc] = A.R[k];
+                s.coff[p] = A.nb_off[k];
+                len = A.nb_off[k + 1] - A.nb_off[k];
+            } else if (p < ns + nc) {
+                d[0] = A.mx[k];
+                d[cc] = A.my[k];
+                d[2 * cc] = A.mz[k];
+                d[3 * cc] = A.ax[k];
+                d[4 * cc] = A.ay[k];
+                d[5 * cc] = A.az[k];
+                d[6 * cc] = A.crho[k];
+                s.coff[p] = A.cb_off[k];
+                len = A.cb_off[k + 1] - A.cb_off[k];
+            } else {
+                d[0] = A.vx[k];
+                d[cc] = A.vy[k];
+                d[2 * cc] = A.vz[k];
+            }
+            for (uint32_t w = 0; w < nw; w++) s.cbits[w * cc + p] = 0ull;
+        }
+        uint32_t chunk;
+        uint32_t pre = block_scan_256(len, tmp4, chunk);
+        if (p < total) s.cpre[p] = carry + pre;
+        carry += chunk;
     }
     __syncthreads();
-    return n[R_OVER] == 0u;
+    // Blocker bitsets: one (candidate, row member) pair per thread.
+    for (uint32_t i = tid; i < carry; i += nt) {
+        uint32_t lo = 0, hi = ns + nc - 1;  // last candidate whose row starts at or before i
+        while (lo < hi) {
+            uint32_t mid = (lo + hi + 1) >> 1;
+            if (s.cpre[mid] <= i)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const uint32_t* items = lo < ns ? A.nb_items : A.cb_items;
+        uint32_t slot = hash_get(s.hk, s.hv, hmask, items[s.coff[lo] + (i - s.cpre[lo])]);
+        if (slot != 0xffffffffu) atomicOr(&s.cbits[(slot >> 6) * cc + lo], 1ull << (slot & 63));
+    }
+    __syncthreads();
+    return true;
 }
 
 struct EvalD {
@@ -390,171 +308,168 @@ struct EvalD {
     double qx, qy, qz;
 };
 
-// Lexicographic minimum of (d, o) over the wave.
-__device__ inline void wave_argmin(double& d, uint32_t& o) {
-    for (int m = 32; m > 0; m >>= 1) {
-        double od = __shfl_xor(d, m, WAVE);
-        uint32_t oo = __shfl_xor(o, m, WAVE);
-        if (od < d || (od == d && oo < o)) {
-            d = od;
-            o = oo;
+// True if (qx, qy, qz) lies inside a blocker of candidate p:
+// !(ddist(q, c) >= R - EPS) for a near row member.
+__device__ inline bool blocked(const RLists& s, uint32_t cc, uint32_t nw, uint32_t p, double qx, double qy,
+                               double qz) {
+    for (uint32_t w = 0; w < nw; w++) {
+        unsigned long long m = s.cbits[w * cc + p];
+        while (m) {
+            uint32_t k = w * 64 + (uint32_t)__builtin_ctzll(m);
+            m &= m - 1ull;
+            double ex = qx - s.nx[k], ey = qy - s.ny[k], ez = qz - s.nz[k];
+            if (ex * ex + ey * ey + ez * ez < s.nblk[k]) return true;
         }
     }
+    return false;
 }
 
-constexpr int MAXC = 4;  // candidates per lane held in registers
+// Takes (d, p) if it precedes (best, bo): less distance, or the same and
+// earlier in list order.
+__device__ inline bool precedes(double d, uint32_t p, double best, uint32_t bo) {
+    return d < best || (d == best && p < bo);
+}
 
-// Distance of candidate o (list position: S, then C, then V) from (x, y, z)
-// as eval_local compares it (field/point.rs:419-489), +inf where the
-// candidate cannot be taken (no projection, or not below the cutoff).
-__device__ inline double cand_dist(const RLists& s, uint32_t ns, uint32_t nc, uint32_t total, uint32_t ccap,
-                                   uint32_t o, double x, double y, double z, double cutoff) {
-    const double* cd = s.cd;
-    double d = INFINITY;
-    if (o < ns) {
-        double vx = x - s.sx[o], vy = y - s.sy[o], vz = z - s.sz[o];
+// Projection of x onto candidate p (its probe position, point.rs:419-489).
+__device__ inline void project(const DArr& A, const RLists& s, uint32_t cc, uint32_t ns, uint32_t nc, uint32_t p,
+                               double x, double y, double z, double& qx, double& qy, double& qz) {
+    const double* d = s.cdat + p;
+    if (p < ns) {
+        double cx = d[0], cy = d[cc], cz = d[2 * cc], R = d[3 * cc];
+        double vx = x - cx, vy = y - cy, vz = z - cz;
         double len = sqrt(vx * vx + vy * vy + vz * vz);
-        double lb = fabs(len - s.sR[o]);
-        if (!(len < 1e-12)) d = lb;
-    } else if (o < ns + nc) {
-        uint32_t q = o - ns;
-        double yx = x - cd[q], yy = y - cd[ccap + q], yz = z - cd[2 * ccap + q];
-        double ax = cd[3 * ccap + q], ay = cd[4 * ccap + q], az = cd[5 * ccap + q], rho = cd[9 * ccap + q];
+        double sc = R / len;
+        qx = cx + vx * sc;
+        qy = cy + vy * sc;
+        qz = cz + vz * sc;
+    } else if (p < ns + nc) {
+        double mx = d[0], my = d[cc], mz = d[2 * cc], ax = d[3 * cc], ay = d[4 * cc], az = d[5 * cc],
+               rho = d[6 * cc];
+        double yx = x - mx, yy = y - my, yz = z - mz;
         double along = yx * ax + yy * ay + yz * az;
         double wx = yx - ax * along, wy = yy - ay * along, wz = yz - az * along;
         double wl = sqrt(wx * wx + wy * wy + wz * wz);
-        d = sqrt(along * along + (wl - rho) * (wl - rho));
-    } else if (o < total) {
-        uint32_t q = o - ns - nc;
-        d = ddist(x, y, z, s.vx[q], s.vy[q], s.vz[q]);
+        double dx, dy, dz;
+        if (wl > 1e-12) {
+            dx = wx / wl;
+            dy = wy / wl;
+            dz = wz / wl;
+        } else {
+            uint32_t c = s.cid[p] & ID_MASK;
+            dx = A.e1x[c];
+            dy = A.e1y[c];
+            dz = A.e1z[c];
+        }
+        qx = mx + dx * rho;
+        qy = my + dy * rho;
+        qz = mz + dz * rho;
+    } else {
+        qx = d[0];
+        qy = d[cc];
+        qz = d[2 * cc];
     }
-    return d < cutoff ? d : INFINITY;
 }
 
-// eval_local (field/point.rs:686-722) at (x, y, z) by one wavefront.
-// `dm` (per wavefront) holds candidate distances when the lists exceed
-// WAVE * MAXC candidates; otherwise registers do.
-__device__ inline EvalD eval_wave(const DArr& A, const RLists& s, const uint32_t* n, uint32_t ccap, double x,
-                                  double y, double z, double band, double* dm) {
-    const uint32_t lane = lane_id();
+// Lane group of a workgroup: g lanes (a power of two, at most 64) starting
+// at lane `shift` of the wavefront; r is the lane's rank in the group.
+struct Group {
+    uint32_t g, r, shift;
+    unsigned long long mask;
+};
+
+__device__ inline bool group_any(const Group& G, bool p) { return ((__ballot(p) >> G.shift) & G.mask) != 0ull; }
+
+// eval_local (field/point.rs:686-722) at (x, y, z), split over a lane group.
+// (cx, cy, cz) is the centre of the gathered ball.
+__device__ inline EvalD eval_group(const DArr& A, const RLists& s, const uint32_t* n, uint32_t cc, uint32_t nw,
+                                   const Group& G, double cx, double cy, double cz, double x, double y, double z,
+                                   double cutoff) {
     EvalD e;
     e.qx = x;
     e.qy = y;
     e.qz = z;
     bool ins = false;
-    for (uint32_t q = lane; q < n[R_IN]; q += WAVE) ins |= ddist(x, y, z, s.ix[q], s.iy[q], s.iz[q]) < s.iR[q];
-    if (!wave_any(ins)) {
+    const uint32_t n_in = n[R_IN];
+    for (uint32_t q = G.r; q < n_in; q += G.g) {
+        uint32_t k = s.ins[q];
+        double dx = x - s.nx[k], dy = y - s.ny[k], dz = z - s.nz[k];
+        ins |= dx * dx + dy * dy + dz * dz < s.nin[k];
+    }
+    if (!group_any(G, ins)) {
         e.value = A.probe;
         e.kind = F_SOLVENT;
         e.id = 0;
         return e;
     }
-    const uint32_t ns = n[R_S], nc = n[R_C], nv = n[R_V], total = ns + nc + nv;
-    const double cutoff = A.probe + band;
-    const double* cd = s.cd;
-    const bool in_regs = total <= WAVE * MAXC;
-    double dist[MAXC];
-    if (in_regs) {
-        for (int j = 0; j < MAXC; j++) dist[j] = cand_dist(s, ns, nc, total, ccap, lane + WAVE * j, x, y, z, cutoff);
-    } else {
-        for (uint32_t o = lane; o < total; o += WAVE) dm[o] = cand_dist(s, ns, nc, total, ccap, o, x, y, z, cutoff);
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const uint32_t ns = n[R_S], nc = n[R_C], total = n[R_CAND];
+    double ox = x - cx, oy = y - cy, oz = z - cz;
+    const double delta = sqrt(ox * ox + oy * oy + oz * oz) + 1e-7;
+    double best = cutoff;
+    uint32_t bo = 0xffffffffu;
+    // Class S.
+    for (uint32_t p = G.r; p < ns; p += G.g) {
+        if (key_value(s.ckey[p]) - delta > best) break;
+        const double* d = s.cdat + p;
+        double px = d[0], py = d[cc], pz = d[2 * cc], R = d[3 * cc];
+        double vx = x - px, vy = y - py, vz = z - pz;
+        double len = sqrt(vx * vx + vy * vy + vz * vz);
+        if (len < 1e-12) continue;
+        double dist = fabs(len - R);
+        if (!(dist < cutoff) || !precedes(dist, p, best, bo)) continue;
+        double sc = R / len;
+        if (!blocked(s, cc, nw, p, px + vx * sc, py + vy * sc, pz + vz * sc)) {
+            best = dist;
+            bo = p;
+        }
     }
-    for (;;) {
-        double bd = INFINITY;
-        uint32_t bo = 0xffffffffu;
-        if (in_regs) {
-            for (int j = 0; j < MAXC; j++)
-                if (dist[j] < bd) {
-                    bd = dist[j];
-                    bo = lane + WAVE * j;
-                }
-        } else {
-            for (uint32_t o = lane; o < total; o += WAVE)
-                if (dm[o] < bd) {
-                    bd = dm[o];
-                    bo = o;
-                }
-        }
-        wave_argmin(bd, bo);
-        if (bo == 0xffffffffu) {  // nothing free below the cutoff: Beyond
-            e.value = A.probe - cutoff;
-            e.kind = F_BEYOND;
-            e.id = 0;
-            return e;
-        }
-        if (bo % WAVE == lane) {
-            if (in_regs)
-                dist[bo / WAVE] = INFINITY;
-            else
-                dm[bo] = INFINITY;
-        }
+    // Class C.
+    for (uint32_t p = ns + G.r; p < ns + nc; p += G.g) {
+        if (key_value(s.ckey[p]) - delta > best) break;
+        const double* d = s.cdat + p;
+        double mx = d[0], my = d[cc], mz = d[2 * cc], ax = d[3 * cc], ay = d[4 * cc], az = d[5 * cc],
+               rho = d[6 * cc];
+        double yx = x - mx, yy = y - my, yz = z - mz;
+        double along = yx * ax + yy * ay + yz * az;
+        double wx = yx - ax * along, wy = yy - ay * along, wz = yz - az * along;
+        double wl = sqrt(wx * wx + wy * wy + wz * wz);
+        double dist = sqrt(along * along + (wl - rho) * (wl - rho));
+        if (!(dist < cutoff) || !precedes(dist, p, best, bo)) continue;
         double qx, qy, qz;
-        uint32_t b0 = 0, b1 = 0;
-        const uint16_t* blk = s.sblk;
-        uint32_t kind, id;
-        if (bo < ns) {
-            double cx = s.sx[bo], cy = s.sy[bo], cz = s.sz[bo], Ri = s.sR[bo];
-            double vx = x - cx, vy = y - cy, vz = z - cz;
-            double len = sqrt(vx * vx + vy * vy + vz * vz);
-            double sc = Ri / len;
-            qx = cx + vx * sc;
-            qy = cy + vy * sc;
-            qz = cz + vz * sc;
-            b0 = s.sb0[bo];
-            b1 = s.sb1[bo];
-            kind = F_SPHERE;
-            id = s.sid[bo];
-        } else if (bo < ns + nc) {
-            uint32_t q = bo - ns;
-            double mx = cd[q], my = cd[ccap + q], mz = cd[2 * ccap + q];
-            double ax = cd[3 * ccap + q], ay = cd[4 * ccap + q], az = cd[5 * ccap + q], rho = cd[9 * ccap + q];
-            double yx = x - mx, yy = y - my, yz = z - mz;
-            double along = yx * ax + yy * ay + yz * az;
-            double wx = yx - ax * along, wy = yy - ay * along, wz = yz - az * along;
-            double wl = sqrt(wx * wx + wy * wy + wz * wz);
-            double dx, dy, dz;
-            if (wl > 1e-12) {
-                dx = wx / wl;
-                dy = wy / wl;
-                dz = wz / wl;
-            } else {
-                dx = cd[6 * ccap + q];
-                dy = cd[7 * ccap + q];
-                dz = cd[8 * ccap + q];
-            }
-            qx = mx + dx * rho;
-            qy = my + dy * rho;
-            qz = mz + dz * rho;
-            b0 = s.cb0[q];
-            b1 = s.cb1[q];
-            blk = s.cblk;
-            kind = F_CIRCLE;
-            id = s.cid[q];
-        } else {
-            uint32_t q = bo - ns - nc;
-            qx = s.vx[q];
-            qy = s.vy[q];
-            qz = s.vz[q];
-            kind = F_VERTEX;
-            id = s.vid[q];
-        }
-        bool blocked = false;
-        for (uint32_t t = b0 + lane; t < b1; t += WAVE) {
-            uint32_t k = blk[t];
-            blocked |= !(ddist(qx, qy, qz, s.nx[k], s.ny[k], s.nz[k]) >= s.nrm[k]);
-        }
-        if (!wave_any(blocked)) {
-            e.value = A.probe - bd;
-            e.kind = kind;
-            e.id = id;
-            e.qx = qx;
-            e.qy = qy;
-            e.qz = qz;
-            return e;
+        project(A, s, cc, ns, nc, p, x, y, z, qx, qy, qz);
+        if (!blocked(s, cc, nw, p, qx, qy, qz)) {
+            best = dist;
+            bo = p;
         }
     }
+    // Class V (no blockers).
+    for (uint32_t p = ns + nc + G.r; p < total; p += G.g) {
+        if (key_value(s.ckey[p]) - delta > best) break;
+        const double* d = s.cdat + p;
+        double dist = ddist(x, y, z, d[0], d[cc], d[2 * cc]);
+        if (dist < cutoff && precedes(dist, p, best, bo)) {
+            best = dist;
+            bo = p;
+        }
+    }
+    for (uint32_t o = G.g >> 1; o > 0; o >>= 1) {
+        double od = __shfl_xor(best, (int)o, WAVE);
+        uint32_t oo = __shfl_xor(bo, (int)o, WAVE);
+        if (precedes(od, oo, best, bo)) {
+            best = od;
+            bo = oo;
+        }
+    }
+    if (bo == 0xffffffffu) {  // nothing free below the cutoff: Beyond
+        e.value = A.probe - cutoff;
+        e.kind = F_BEYOND;
+        e.id = 0;
+        return e;
+    }
+    project(A, s, cc, ns, nc, bo, x, y, z, e.qx, e.qy, e.qz);
+    e.value = A.probe - best;
+    e.kind = bo < ns ? F_SPHERE : (bo < ns + nc ? F_CIRCLE : F_VERTEX);
+    e.id = s.cid[bo] & ID_MASK;
+    return e;
 }
 
 // Output arrays of the vertices.
@@ -567,81 +482,115 @@ struct VOut {
     uint32_t* evals;
 };
 
-// vertex() + attributes() (mesh/extract.rs:425-535, 553-597) of the edge
-// from lattice point ia along `axis`, by one wavefront; lane 0 writes.
-__device__ inline uint32_t refine_vertex_wave(const DArr& A, const DRun& P, const RLists& s, const uint32_t* n,
-                                              uint32_t ccap, double* dm, const int64_t ia[3], uint32_t axis,
-                                              float fa32, float fb32, const VOut& o, size_t slot) {
-    const double h = P.h, band = P.band;
-    double pa[3] = {(double)ia[0] * h, (double)ia[1] * h, (double)ia[2] * h};
+// State of one edge's root search (mesh/extract.rs:425-535): up to 8
+// Newton / Illinois steps, then up to 64 bisections.
+struct EdgeState {
+    double pa[3], d[3], fa, dn;
+    double t, t0, g0, t1, g1;
+    EvalD last;
+    size_t slot;
+    int side;
+    uint32_t it, evals;
+    bool bisect, neg0, converged;
+};
+
+__device__ inline void edge_start(const DRun& P, const float* val, const int64_t org[3], uint32_t id, size_t slot,
+                                  EdgeState& E) {
+    const uint32_t b = P.b;
+    uint32_t axis = id % 3, cell = id / 3;
+    uint32_t x = cell % b, y = (cell / b) % b, z = cell / (b * b);
+    uint32_t i0 = x + P.side * (y + P.side * z);
+    uint32_t st = axis == 0 ? 1u : (axis == 1 ? P.side : P.side * P.side);
+    int64_t ia[3] = {org[0] + x, org[1] + y, org[2] + z};
     int64_t ib[3] = {ia[0], ia[1], ia[2]};
     ib[axis] += 1;
-    double pb[3] = {(double)ib[0] * h, (double)ib[1] * h, (double)ib[2] * h};
-    double d[3] = {pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2]};
-    double fa = (double)fa32, fb = (double)fb32;
-    double t = fmin(fmax(fa / (fa - fb), 0.0), 1.0);
-    double t0 = 0.0, g0 = fa, t1 = 1.0, g1 = fb;
-    int side = 0;
-    bool converged = false;
-    uint32_t evals = 0;
-    EvalD last;
-    double dn = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-    for (int it = 0; it < 8; it++) {
-        double x = pa[0] + d[0] * t, y = pa[1] + d[1] * t, z = pa[2] + d[2] * t;
-        EvalD e = eval_wave(A, s, n, ccap, x, y, z, band, dm);
-        evals++;
-        last = e;
-        double g = e.value;
+    const double h = P.h;
+    for (int a = 0; a < 3; a++) {
+        E.pa[a] = (double)ia[a] * h;
+        E.d[a] = (double)ib[a] * h - E.pa[a];
+    }
+    double fa = (double)val[i0], fb = (double)val[i0 + st];
+    E.fa = fa;
+    E.t = fmin(fmax(fa / (fa - fb), 0.0), 1.0);
+    E.t0 = 0.0;
+    E.g0 = fa;
+    E.t1 = 1.0;
+    E.g1 = fb;
+    E.side = 0;
+    E.it = 0;
+    E.evals = 0;
+    E.bisect = false;
+    E.neg0 = false;
+    E.converged = false;
+    E.dn = sqrt(E.d[0] * E.d[0] + E.d[1] * E.d[1] + E.d[2] * E.d[2]);
+    E.slot = slot;
+}
+
+// Applies evaluation e at E.t; true when the search is over.
+__device__ inline bool edge_step(EdgeState& E, const EvalD& e) {
+    E.evals++;
+    E.last = e;
+    double g = e.value;
+    if (!E.bisect) {
         if (fabs(g) <= 1e-5) {
-            converged = true;
-            break;
+            E.converged = true;
+            return true;
         }
-        if ((g < 0.0) == (g0 < 0.0)) {
-            t0 = t;
-            g0 = g;
-            if (side == -1) g1 *= 0.5;
-            side = -1;
+        if ((g < 0.0) == (E.g0 < 0.0)) {
+            E.t0 = E.t;
+            E.g0 = g;
+            if (E.side == -1) E.g1 *= 0.5;
+            E.side = -1;
         } else {
-            t1 = t;
-            g1 = g;
-            if (side == 1) g0 *= 0.5;
-            side = 1;
+            E.t1 = E.t;
+            E.g1 = g;
+            if (E.side == 1) E.g0 *= 0.5;
+            E.side = 1;
         }
         double slope = 0.0;
         if (e.kind != F_SOLVENT && e.kind != F_BEYOND) {
+            double x = E.pa[0] + E.d[0] * E.t, y = E.pa[1] + E.d[1] * E.t, z = E.pa[2] + E.d[2] * E.t;
             double wx = e.qx - x, wy = e.qy - y, wz = e.qz - z;
             double len = sqrt(wx * wx + wy * wy + wz * wz);
             if (len > 1e-12) {
                 double nx = wx / len, ny = wy / len, nz = wz / len;
-                slope = nx * d[0] + ny * d[1] + nz * d[2];
+                slope = nx * E.d[0] + ny * E.d[1] + nz * E.d[2];
             }
         }
-        double newton = t - g / slope;
-        if (fabs(slope) > 1e-3 * dn && newton > t0 && newton < t1) {
-            t = newton;
+        double newton = E.t - g / slope;
+        if (fabs(slope) > 1e-3 * E.dn && newton > E.t0 && newton < E.t1) {
+            E.t = newton;
         } else {
-            double sv = (t0 * g1 - t1 * g0) / (g1 - g0);
-            t = (isfinite(sv) && sv > t0 && sv < t1) ? sv : 0.5 * (t0 + t1);
+            double sv = (E.t0 * E.g1 - E.t1 * E.g0) / (E.g1 - E.g0);
+            E.t = (isfinite(sv) && sv > E.t0 && sv < E.t1) ? sv : 0.5 * (E.t0 + E.t1);
         }
-    }
-    if (!converged) {
-        bool neg0 = g0 < 0.0;
-        for (int it = 0; it < 64; it++) {
-            t = 0.5 * (t0 + t1);
-            EvalD e = eval_wave(A, s, n, ccap, pa[0] + d[0] * t, pa[1] + d[1] * t, pa[2] + d[2] * t, band, dm);
-            evals++;
-            last = e;
-            if (fabs(e.value) <= 1e-5 || (t1 - t0) * dn <= 2.0 * 1e-5) break;
-            if ((e.value < 0.0) == neg0)
-                t0 = t;
-            else
-                t1 = t;
+        if (++E.it == 8) {
+            E.bisect = true;
+            E.neg0 = E.g0 < 0.0;
+            E.it = 0;
+            E.t = 0.5 * (E.t0 + E.t1);
         }
+        return false;
     }
+    if (fabs(g) <= 1e-5 || (E.t1 - E.t0) * E.dn <= 2.0 * 1e-5) return true;
+    if ((g < 0.0) == E.neg0)
+        E.t0 = E.t;
+    else
+        E.t1 = E.t;
+    if (++E.it == 64) return true;
+    E.t = 0.5 * (E.t0 + E.t1);
+    return false;
+}
+
+// The vertex and its attributes (mesh/extract.rs:535, 553-597); the
+// group's rank-0 lane writes them.
+__device__ inline void edge_finish(const DArr& A, const DRun& P, const EdgeState& E, const Group& G, const VOut& o) {
+    const double t = E.t;
+    const EvalD& last = E.last;
     double tc = fmin(fmax(t, P.t_min), 1.0 - P.t_min);
-    double v[3] = {pa[0] + d[0] * tc, pa[1] + d[1] * tc, pa[2] + d[2] * tc};
+    double v[3] = {E.pa[0] + E.d[0] * tc, E.pa[1] + E.d[1] * tc, E.pa[2] + E.d[2] * tc};
     double ow[3];
-    for (int a = 0; a < 3; a++) ow[a] = fa < 0.0 ? d[a] : -d[a];
+    for (int a = 0; a < 3; a++) ow[a] = E.fa < 0.0 ? E.d[a] : -E.d[a];
     double ol = sqrt(ow[0] * ow[0] + ow[1] * ow[1] + ow[2] * ow[2]);
     double outward[3] = {ow[0] / ol, ow[1] / ol, ow[2] / ol};
     double nrm[3];
@@ -672,6 +621,7 @@ __device__ inline uint32_t refine_vertex_wave(const DArr& A, const DRun& P, cons
         }
         return A.orig[bk];
     };
+    if (G.r != 0) return;
     if (last.kind == F_SPHERE) {
         unit(last.qx - v[0], last.qy - v[1], last.qz - v[2]);
         patch = 0;
@@ -688,43 +638,39 @@ __device__ inline uint32_t refine_vertex_wave(const DArr& A, const DRun& P, cons
         atom = nearest(at, 3);
     } else {
         // sphere_eval(v, probe + band + 8) over the whole grid (point.rs:742-756),
-        // first minimum in scan order: lane 0 alone (rare).
+        // first minimum in scan order (rare).
         nrm[0] = outward[0];
         nrm[1] = outward[1];
         nrm[2] = outward[2];
         patch = 0;
-        if (lane_id() == 0) {
-            double cutoff = A.probe + band + 8.0;
-            double bv = cutoff;
-            int64_t bi = -1;
-            for_near(A.gs, v[0], v[1], v[2], cutoff + A.max_expanded, 0, 1, [&](uint32_t k) {
-                if (A.status[k] == S_CONTAINED) return;
-                double vv = ddist(v[0], v[1], v[2], A.cx[k], A.cy[k], A.cz[k]) - A.R[k];
-                if (vv < bv) {
-                    bv = vv;
-                    bi = k;
-                }
-            });
-            atom = bi < 0 ? 0u : A.orig[bi];
-        }
+        double cutoff = A.probe + P.band + 8.0;
+        double bv = cutoff;
+        int64_t bi = -1;
+        for_near(A.gs, v[0], v[1], v[2], cutoff + A.max_expanded, 0, 1, [&](uint32_t k) {
+            if (A.status[k] == S_CONTAINED) return;
+            double vv = ddist(v[0], v[1], v[2], A.cx[k], A.cy[k], A.cz[k]) - A.R[k];
+            if (vv < bv) {
+                bv = vv;
+                bi = k;
+            }
+        });
+        atom = bi < 0 ? 0u : A.orig[bi];
     }
-    if (lane_id() == 0) {
-        for (int a = 0; a < 3; a++) {
-            o.pos[3 * slot + a] = (float)v[a];
-            o.nrm[3 * slot + a] = (float)nrm[a];
-        }
-        o.atom[slot] = atom;
-        o.patch[slot] = patch;
-        if (o.t) o.t[slot] = t;
-        if (o.evals) o.evals[slot] = evals | (converged ? 0x80000000u : 0u);
+    const size_t slot = E.slot;
+    for (int a = 0; a < 3; a++) {
+        o.pos[3 * slot + a] = (float)v[a];
+        o.nrm[3 * slot + a] = (float)nrm[a];
     }
-    return evals;
+    o.atom[slot] = atom;
+    o.patch[slot] = patch;
+    if (o.t) o.t[slot] = t;
+    if (o.evals) o.evals[slot] = E.evals | (E.converged ? 0x80000000u : 0u);
 }
 
-// One workgroup (4 wavefronts) per sub-block with crossing edges (`work[w]`
+// One workgroup (256 threads) per sub-block with crossing edges (`work[w]`
 // = 8*brick + sb): gather_local into LDS (or scratch slice w), compact the
 // crossing edges whose lower sample lies in the sub-block, then refine them
-// one per wavefront. Sub-blocks whose lists overflow go to `overflow`.
+// by lane groups. Sub-blocks whose lists overflow go to `overflow`.
 __global__ void __launch_bounds__(256) k_refine(DArr A, DRun P, RCaps cap, RLayout lay, const int32_t* keys,
                                                 const float* values, const uint32_t* work, uint32_t n_work,
                                                 const unsigned long long* mask, const uint32_t* wpre,
@@ -733,6 +679,8 @@ __global__ void __launch_bounds__(256) k_refine(DArr A, DRun P, RCaps cap, RLayo
                                                 unsigned long long* evals_total) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     __shared__ uint32_t n[R_N];
+    __shared__ uint32_t tmp4[4];
+    __shared__ unsigned long long evals_sum;
     uint32_t w = blockIdx.x;
     if (w >= n_work) return;
     uint32_t j = work[w], brick = j >> 3, sb = j & 7;
@@ -741,6 +689,7 @@ __global__ void __launch_bounds__(256) k_refine(DArr A, DRun P, RCaps cap, RLayo
     const int32_t* key = keys + 3 * brick;
     const uint32_t b = P.b, sub = P.sub;
     double h = P.h, subd = (double)sub;
+    if (threadIdx.x == 0) evals_sum = 0;
     // Ball of the sub-block (mesh/extract.rs:164-205).
     double rho = 0.5 * sqrt(3.0) * subd * h * (1.0 + 1e-9) + 1e-9;
     uint32_t o3[3] = {sb & 1, (sb >> 1) & 1, sb >> 2};
@@ -751,8 +700,8 @@ __global__ void __launch_bounds__(256) k_refine(DArr A, DRun P, RCaps cap, RLayo
         double basep = (double)org[a] * h;
         c[a] = basep + ((double)(o3[a] * sub) + 0.5 * subd) * h;
     }
-    bool ok = gather_local_block(A, cap, s, n, c[0], c[1], c[2], rho, P.band);
-    // Crossing edges whose lower sample lies in this sub-block.
+    bool ok = gather_refine(A, cap, s, n, tmp4, c[0], c[1], c[2], rho, P.band);
+    // Crossing edges whose lower sample lies in this sub-block, ascending.
     uint32_t lo[3], hi[3];
     for (int a = 0; a < 3; a++) {
         lo[a] = o3[a] * sub;
@@ -781,21 +730,47 @@ __global__ void __launch_bounds__(256) k_refine(DArr A, DRun P, RCaps cap, RLayo
         return;
     }
     const float* val = values + (size_t)brick * P.spb;
-    const uint32_t wv = threadIdx.x / WAVE, nwv = blockDim.x / WAVE;
-    unsigned long long my_evals = 0;
-    uint32_t vb = vbase[brick];
-    double* dm = s.dm + (size_t)wv * (cap.s + cap.c + cap.v);
-    for (uint32_t q = wv; q < n[R_E]; q += nwv) {
+    const uint32_t ne = n[R_E];
+    // Lane groups: the largest g <= 64 with at least `ne` groups.
+    uint32_t g = WAVE;
+    while (g > 1 && blockDim.x / g < ne) g >>= 1;
+    Group G;
+    G.g = g;
+    G.r = threadIdx.x % g;
+    G.shift = (threadIdx.x % WAVE) - G.r;
+    G.mask = g == WAVE ? ~0ull : ((1ull << g) - 1ull);
+    const uint32_t ngroups = blockDim.x / g, nw = cap.near / 64;
+    const double cutoff = A.probe + P.band;
+    const uint32_t vb = vbase[brick];
+    uint32_t q = threadIdx.x / g;
+    bool active = q < ne;
+    EdgeState E;
+    if (active) {
         uint32_t id = s.ebuf[q];
-        uint32_t axis = id % 3, cell = id / 3;
-        uint32_t x = cell % b, y = (cell / b) % b, z = cell / (b * b);
-        uint32_t i0 = x + P.side * (y + P.side * z);
-        uint32_t st = axis == 0 ? 1u : (axis == 1 ? P.side : P.side * P.side);
-        int64_t ia[3] = {org[0] + x, org[1] + y, org[2] + z};
-        size_t slot = (size_t)vb + edge_rank(mask, wpre, words, brick, id);
-        my_evals += refine_vertex_wave(A, P, s, n, cap.c, dm, ia, axis, val[i0], val[i0 + st], out, slot);
+        edge_start(P, val, org, id, (size_t)vb + edge_rank(mask, wpre, words, brick, id), E);
     }
-    if (evals_total && lane_id() == 0 && my_evals) atomicAdd(evals_total, my_evals);
+    unsigned long long my_evals = 0;
+    while (__ballot(active) != 0ull) {
+        if (active) {
+            double x = E.pa[0] + E.d[0] * E.t, y = E.pa[1] + E.d[1] * E.t, z = E.pa[2] + E.d[2] * E.t;
+            EvalD e = eval_group(A, s, n, cap.cand, nw, G, c[0], c[1], c[2], x, y, z, cutoff);
+            if (edge_step(E, e)) {
+                edge_finish(A, P, E, G, out);
+                my_evals += E.evals;
+                q += ngroups;
+                active = q < ne;
+                if (active) {
+                    uint32_t id = s.ebuf[q];
+                    edge_start(P, val, org, id, (size_t)vb + edge_rank(mask, wpre, words, brick, id), E);
+                }
+            }
+        }
+    }
+    if (evals_total) {
+        if (G.r == 0 && my_evals) atomicAdd(&evals_sum, my_evals);
+        __syncthreads();
+        if (threadIdx.x == 0 && evals_sum) atomicAdd(evals_total, evals_sum);
+    }
 }
 
 }  // namespace smg

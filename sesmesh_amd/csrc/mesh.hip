@@ -515,8 +515,8 @@ static int upload_grid(smg_resident* r, const smg_grid_ref& g, DGrid& d) {
 extern "C" int smg_resident_new(smg_device* dev, const smg_arrangement_ref* a, smg_resident** out) {
     *out = nullptr;
     if (!dev || !a) return fail(SMG_E_INVALID, "null device or arrangement");
-    if (a->n_spheres >= (1u << 31) || a->n_circles >= (1u << 31) || a->n_vertices >= (1u << 31))
-        return fail(SMG_E_LENGTH, "arrangement too large for 32-bit indices");
+    if (a->n_spheres >= (1u << 30) || a->n_circles >= (1u << 30) || a->n_vertices >= (1u << 30))
+        return fail(SMG_E_LENGTH, "arrangement too large for 30-bit candidate indices");
     if (!(a->probe > 0.0)) return fail(SMG_E_UNSUPPORTED, "the device path computes the SES (probe > 0)");
     HIP_TRY(hipSetDevice(dev->ordinal));
     smg_resident* r = new smg_resident();
@@ -655,13 +655,14 @@ int read_u32(smg_device* d, const uint32_t* dev_ptr, uint32_t& v) {
         }                       \
     } while (0)
 
-// List capacities: an LDS variant sized for common sub-blocks and a
-// large one in global scratch for the rest.
-constexpr Caps SAMPLE_SMALL{160, 96, 128, 128, 128, 2048, 2048, 512};
-constexpr Caps SAMPLE_LARGE{4096, 2048, 2048, 8192, 2048, 65536, 65536, 8192};
-constexpr RCaps REFINE_SMALL{128, 64, 64, 128, 64, 1024, 1024, 256, 512};
-constexpr RCaps REFINE_LARGE{4096, 2048, 2048, 8192, 2048, 65536, 65536, 8192, 12288};
-constexpr uint32_t LARGE_CHUNK = 256;
+// List capacities, tried in turn: LDS variants sized for common and for
+// large sub-blocks, then global scratch for the rest. Each pass takes the
+// previous pass's overflow list.
+constexpr Caps SAMPLE_CAPS[3] = {{128, 64, 256, 64, 256}, {256, 128, 512, 128, 512}, {1024, 512, 4096, 2048, 4096}};
+constexpr RCaps REFINE_CAPS[3] = {{128, 64, 256, 256, 0}, {256, 128, 512, 512, 0}, {1024, 512, 4096, 4096, 0}};
+constexpr uint32_t LDS_EDGES = 512;      // crossing-edge buffer of the LDS passes
+constexpr uint32_t SCRATCH_CHUNK = 256;   // workgroups per scratch launch
+constexpr uint32_t STATIC_LDS = 6144;     // static LDS of k_sample / k_refine, bound
 
 }  // namespace
 
@@ -810,42 +811,43 @@ static int mesh_stage(smg_resident* r, const smg_mesh_params* prm, smg_mesh* M, 
     uint32_t nsw;
     TRY(read_u32(d, nsw_d, nsw));
     M->sampled = nsw;
-    uint32_t *ovf, *novf;
     unsigned long long* nsamp;
     HIP_TRY(T.get(&nsamp, 1));
     HIP_TRY(hipMemsetAsync(nsamp, 0, 8, s));
-    HIP_TRY(T.get(&ovf, std::max<uint32_t>(nsw, 1)));
-    HIP_TRY(T.get(&novf, 1));
-    HIP_TRY(hipMemsetAsync(novf, 0, 4, s));
-    Layout lay_s = make_layout(SAMPLE_SMALL);
-    HIP_TRY(hipEventRecord(ev[10], s));
-    if (nsw)
-        k_sample<<<nsw, 256, lay_s.bytes, s>>>(A, L, P, SAMPLE_SMALL, lay_s, M->keys, st, swork, nsw, M->values,
-                                                nullptr, 0, ovf, novf, nsamp);
-    HIP_TRY(hipEventRecord(ev[11], s));
-    HIP_TRY(hipGetLastError());
-    uint32_t nov;
-    TRY(read_u32(d, novf, nov));
-    M->overflow = nov;
-    if (nov) {
-        Layout lay_l = make_layout(SAMPLE_LARGE);
-        uint32_t chunk = std::min(nov, LARGE_CHUNK);
-        char* scratch;
-        HIP_TRY(T.get(&scratch, (size_t)chunk * lay_l.bytes));
-        uint32_t* novf2;
-        HIP_TRY(T.get(&novf2, 1));
-        HIP_TRY(hipMemsetAsync(novf2, 0, 4, s));
-        uint32_t* ovf2;
-        HIP_TRY(T.get(&ovf2, nov));
-        for (uint32_t c0 = 0; c0 < nov; c0 += chunk) {
-            uint32_t cnt = std::min(chunk, nov - c0);
-            k_sample<<<cnt, 256, 0, s>>>(A, L, P, SAMPLE_LARGE, lay_l, M->keys, st, ovf + c0, cnt, M->values,
-                                          scratch, lay_l.bytes, ovf2, novf2, nsamp);
+    const size_t lds_max = d->prop.sharedMemPerBlock;
+    M->overflow = 0;
+    {
+        const uint32_t* list = swork;
+        uint32_t count = nsw;
+        HIP_TRY(hipEventRecord(ev[10], s));
+        for (int pass = 0; pass < 3 && count; pass++) {
+            const Caps& cap = SAMPLE_CAPS[pass];
+            Layout lay = make_layout(cap);
+            bool lds = pass < 2;
+            if (lds && lay.bytes + STATIC_LDS > lds_max) continue;
+            uint32_t *next, *nnext;
+            HIP_TRY(T.get(&next, count));
+            HIP_TRY(T.get(&nnext, 1));
+            HIP_TRY(hipMemsetAsync(nnext, 0, 4, s));
+            if (lds) {
+                k_sample<<<count, 256, lay.bytes, s>>>(A, L, P, cap, lay, M->keys, st, list, count, M->values, nullptr,
+                                                       0, next, nnext, nsamp);
+            } else {
+                uint32_t chunk = std::min(count, SCRATCH_CHUNK);
+                char* scratch;
+                HIP_TRY(T.get(&scratch, (size_t)chunk * lay.bytes));
+                for (uint32_t c0 = 0; c0 < count; c0 += chunk)
+                    k_sample<<<std::min(chunk, count - c0), 256, 0, s>>>(A, L, P, cap, lay, M->keys, st, list + c0,
+                                                                         std::min(chunk, count - c0), M->values,
+                                                                         scratch, lay.bytes, next, nnext, nsamp);
+            }
+            if (pass == 0) HIP_TRY(hipEventRecord(ev[11], s));
+            HIP_TRY(hipGetLastError());
+            TRY(read_u32(d, nnext, count));
+            if (pass == 0) M->overflow = count;
+            list = next;
         }
-        HIP_TRY(hipGetLastError());
-        uint32_t bad;
-        TRY(read_u32(d, novf2, bad));
-        if (bad) return fail(SMG_E_INTERNAL, std::to_string(bad) + " sub-blocks exceed the largest candidate lists");
+        if (count) return fail(SMG_E_INTERNAL, std::to_string(count) + " sub-blocks exceed the largest candidate lists");
     }
     HIP_TRY(hipEventRecord(ev[2], s));
 
@@ -888,36 +890,39 @@ static int mesh_stage(smg_resident* r, const smg_mesh_params* prm, smg_mesh* M, 
     unsigned long long* evd;
     HIP_TRY(T.get(&evd, 1));
     HIP_TRY(hipMemsetAsync(evd, 0, 8, s));
-    HIP_TRY(hipMemsetAsync(novf, 0, 4, s));
-    RLayout rl_s = make_rlayout(REFINE_SMALL);
-    uint32_t* rovf;
-    HIP_TRY(T.get(&rovf, std::max<uint32_t>(nrw, 1)));
-    HIP_TRY(hipEventRecord(ev[12], s));
-    if (nrw)
-        k_refine<<<nrw, 256, rl_s.bytes, s>>>(A, P, REFINE_SMALL, rl_s, M->keys, M->values, rwork, nrw, mask, wpre,
-                                              words, vbase, vo, nullptr, 0, rovf, novf, evd);
-    HIP_TRY(hipEventRecord(ev[13], s));
-    HIP_TRY(hipGetLastError());
-    uint32_t rnov;
-    TRY(read_u32(d, novf, rnov));
-    if (rnov) {
-        RLayout rl_l = make_rlayout(REFINE_LARGE);
-        uint32_t chunk = std::min(rnov, LARGE_CHUNK);
-        char* scratch;
-        HIP_TRY(T.get(&scratch, (size_t)chunk * rl_l.bytes));
-        uint32_t *n2, *o2;
-        HIP_TRY(T.get(&n2, 1));
-        HIP_TRY(T.get(&o2, rnov));
-        HIP_TRY(hipMemsetAsync(n2, 0, 4, s));
-        for (uint32_t c0 = 0; c0 < rnov; c0 += chunk) {
-            uint32_t cnt = std::min(chunk, rnov - c0);
-            k_refine<<<cnt, 256, 0, s>>>(A, P, REFINE_LARGE, rl_l, M->keys, M->values, rovf + c0, cnt, mask, wpre,
-                                         words, vbase, vo, scratch, rl_l.bytes, o2, n2, evd);
+    {
+        const uint32_t* list = rwork;
+        uint32_t count = nrw, sub_edges = 3 * P.sub * P.sub * P.sub;
+        HIP_TRY(hipEventRecord(ev[12], s));
+        for (int pass = 0; pass < 3 && count; pass++) {
+            RCaps cap = REFINE_CAPS[pass];
+            bool lds = pass < 2;
+            cap.edges = lds ? std::min(sub_edges, LDS_EDGES) : sub_edges;
+            RLayout lay = make_rlayout(cap);
+            if (lds && lay.bytes + STATIC_LDS > lds_max) continue;
+            uint32_t *next, *nnext;
+            HIP_TRY(T.get(&next, count));
+            HIP_TRY(T.get(&nnext, 1));
+            HIP_TRY(hipMemsetAsync(nnext, 0, 4, s));
+            if (lds) {
+                k_refine<<<count, 256, lay.bytes, s>>>(A, P, cap, lay, M->keys, M->values, list, count, mask, wpre,
+                                                       words, vbase, vo, nullptr, 0, next, nnext, evd);
+            } else {
+                uint32_t chunk = std::min(count, SCRATCH_CHUNK);
+                char* scratch;
+                HIP_TRY(T.get(&scratch, (size_t)chunk * lay.bytes));
+                for (uint32_t c0 = 0; c0 < count; c0 += chunk)
+                    k_refine<<<std::min(chunk, count - c0), 256, 0, s>>>(A, P, cap, lay, M->keys, M->values, list + c0,
+                                                                         std::min(chunk, count - c0), mask, wpre,
+                                                                         words, vbase, vo, scratch, lay.bytes, next,
+                                                                         nnext, evd);
+            }
+            if (pass == 0) HIP_TRY(hipEventRecord(ev[13], s));
+            HIP_TRY(hipGetLastError());
+            TRY(read_u32(d, nnext, count));
+            list = next;
         }
-        HIP_TRY(hipGetLastError());
-        uint32_t bad;
-        TRY(read_u32(d, n2, bad));
-        if (bad) return fail(SMG_E_INTERNAL, std::to_string(bad) + " sub-blocks exceed the largest refinement lists");
+        if (count) return fail(SMG_E_INTERNAL, std::to_string(count) + " sub-blocks exceed the largest refinement lists");
     }
     HIP_TRY(hipEventRecord(ev[4], s));
 
