@@ -210,9 +210,14 @@ __host__ __device__ inline uint32_t pow2_at_least(uint32_t v) {
     return p;
 }
 
-// Sorts candidates by (class, key, index) -- class in the top bits of id --
-// ascending, by the whole workgroup (bitonic network over a power of two).
-__device__ inline void sort_candidates(unsigned long long* key, uint32_t* id, uint32_t count) {
+// Candidate order: (class, key, index), the class in the top bits of id.
+__device__ inline bool cand_less(unsigned long long ka, uint32_t ia, unsigned long long kb, uint32_t ib) {
+    uint32_t ca = ia >> KIND_SHIFT, cb = ib >> KIND_SHIFT;
+    return ca < cb || (ca == cb && (ka < kb || (ka == kb && ia < ib)));
+}
+
+// Bitonic network over a power of two, by the whole workgroup.
+__device__ inline void sort_bitonic(unsigned long long* key, uint32_t* id, uint32_t count) {
     uint32_t n = pow2_at_least(count);
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     for (uint32_t i = count + tid; i < n; i += nt) {
@@ -239,6 +244,37 @@ __device__ inline void sort_candidates(unsigned long long* key, uint32_t* id, ui
             }
             __syncthreads();
         }
+}
+
+// Sorts `count` distinct candidates by cand_less, in place, by the whole
+// workgroup: up to 4 per thread are ranked against all others (one pass of
+// broadcast LDS reads and two barriers); longer lists take the bitonic
+// network (capacity a power of two).
+__device__ inline void sort_candidates(unsigned long long* key, uint32_t* id, uint32_t count) {
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    if (count > 4 * nt) {
+        sort_bitonic(key, id, count);
+        return;
+    }
+    unsigned long long mk[4];
+    uint32_t mi[4], rank[4] = {0, 0, 0, 0};
+    for (int j = 0; j < 4; j++) {
+        uint32_t e = tid + j * nt;
+        mk[j] = e < count ? key[e] : ~0ull;
+        mi[j] = e < count ? id[e] : 0xffffffffu;
+    }
+    for (uint32_t o = 0; o < count; o++) {
+        unsigned long long k = key[o];
+        uint32_t i = id[o];
+        for (int j = 0; j < 4; j++) rank[j] += cand_less(k, i, mk[j], mi[j]) ? 1u : 0u;
+    }
+    __syncthreads();
+    for (int j = 0; j < 4; j++)
+        if (tid + j * nt < count) {
+            key[rank[j]] = mk[j];
+            id[rank[j]] = mi[j];
+        }
+    __syncthreads();
 }
 
 constexpr uint32_t ROWS_MAX = 256;

@@ -99,8 +99,15 @@ struct RCaps {
 
 // LDS / scratch layout of the refinement lists (byte offsets).
 struct RLayout {
-    uint32_t nx, ny, nz, nin, nblk, hk, hv, ins, ckey, cid, cdat, cbits, coff, cpre, ebuf, bytes;
+    uint32_t nq, nin, hk, hv, ins, ckey, cid, cdat, cbits, coff, cpre, ebuf, bytes;
 };
+
+// Near sphere record: centre and the blocking threshold of R - EPS.
+struct alignas(16) NearD {
+    double x, y, z, blk;
+};
+
+constexpr uint32_t CDAT = 8;  // doubles per candidate record
 
 __host__ __device__ inline RLayout make_rlayout(const RCaps& c) {
     RLayout L;
@@ -110,17 +117,14 @@ __host__ __device__ inline RLayout make_rlayout(const RCaps& c) {
         o += (bytes + 15u) & ~15u;
         return at;
     };
-    L.nx = take(8 * c.near);
-    L.ny = take(8 * c.near);
-    L.nz = take(8 * c.near);
-    L.nin = take(8 * c.near);   // inside threshold of R
-    L.nblk = take(8 * c.near);  // blocking threshold of R - EPS
+    L.nq = take(32 * c.near);
+    L.nin = take(8 * c.near);  // inside threshold of R
     L.hk = take(4 * c.hash);
     L.hv = take(4 * c.hash);
     L.ins = take(2 * c.inside);
     L.ckey = take(8 * c.cand);
     L.cid = take(4 * c.cand);
-    L.cdat = take(7 * 8 * c.cand);
+    L.cdat = take(8 * CDAT * c.cand);
     L.cbits = take(c.near / 64 * 8 * c.cand);
     L.coff = take(4 * c.cand);
     L.cpre = take(4 * c.cand);
@@ -130,12 +134,15 @@ __host__ __device__ inline RLayout make_rlayout(const RCaps& c) {
 }
 
 struct RLists {
-    double *nx, *ny, *nz, *nin, *nblk;
+    NearD* nq;
+    double* nin;
     uint32_t *hk, *hv;
     uint16_t* ins;
     unsigned long long* ckey;
     uint32_t* cid;
-    double* cdat;  // 7 SoA rows of `cand` values
+    // Candidate records: S (x, y, z, R), C (mx, my, mz, ax, ay, az, rho),
+    // V (x, y, z).
+    double* cdat;
     unsigned long long* cbits;  // near/64 rows of `cand` words
     uint32_t *coff, *cpre;
     uint32_t* ebuf;
@@ -143,11 +150,8 @@ struct RLists {
 
 __device__ inline RLists rcarve(char* base, const RLayout& L) {
     RLists s;
-    s.nx = (double*)(base + L.nx);
-    s.ny = (double*)(base + L.ny);
-    s.nz = (double*)(base + L.nz);
+    s.nq = (NearD*)(base + L.nq);
     s.nin = (double*)(base + L.nin);
-    s.nblk = (double*)(base + L.nblk);
     s.hk = (uint32_t*)(base + L.hk);
     s.hv = (uint32_t*)(base + L.hv);
     s.ins = (uint16_t*)(base + L.ins);
@@ -188,11 +192,13 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
                     n[R_OVER] = 1u;
                     return;
                 }
-                s.nx[at] = cx;
-                s.ny[at] = cy;
-                s.nz[at] = cz;
+                NearD q;
+                q.x = cx;
+                q.y = cy;
+                q.z = cz;
+                q.blk = sqrt_threshold(R - EPS);
+                s.nq[at] = q;
                 s.nin[at] = sqrt_threshold(R);
-                s.nblk[at] = sqrt_threshold(R - EPS);
                 hash_put(s.hk, s.hv, hmask, k, at);
                 if (d < R + rho) {
                     uint32_t q = atomicAdd(&n[R_IN], 1u);
@@ -253,28 +259,28 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
         uint32_t p = p0 + tid, len = 0;
         if (p < total) {
             uint32_t k = s.cid[p] & ID_MASK;
-            double* d = s.cdat + p;
+            double* d = s.cdat + (size_t)CDAT * p;
             if (p < ns) {
                 d[0] = A.cx[k];
-                d[cc] = A.cy[k];
-                d[2 * cc] = A.cz[k];
-                d[3 * cc] = A.R[k];
+                d[1] = A.cy[k];
+                d[2] = A.cz[k];
+                d[3] = A.R[k];
                 s.coff[p] = A.nb_off[k];
                 len = A.nb_off[k + 1] - A.nb_off[k];
             } else if (p < ns + nc) {
                 d[0] = A.mx[k];
-                d[cc] = A.my[k];
-                d[2 * cc] = A.mz[k];
-                d[3 * cc] = A.ax[k];
-                d[4 * cc] = A.ay[k];
-                d[5 * cc] = A.az[k];
-                d[6 * cc] = A.crho[k];
+                d[1] = A.my[k];
+                d[2] = A.mz[k];
+                d[3] = A.ax[k];
+                d[4] = A.ay[k];
+                d[5] = A.az[k];
+                d[6] = A.crho[k];
                 s.coff[p] = A.cb_off[k];
                 len = A.cb_off[k + 1] - A.cb_off[k];
             } else {
                 d[0] = A.vx[k];
-                d[cc] = A.vy[k];
-                d[2 * cc] = A.vz[k];
+                d[1] = A.vy[k];
+                d[2] = A.vz[k];
             }
             for (uint32_t w = 0; w < nw; w++) s.cbits[w * cc + p] = 0ull;
         }
@@ -315,10 +321,10 @@ __device__ inline bool blocked(const RLists& s, uint32_t cc, uint32_t nw, uint32
     for (uint32_t w = 0; w < nw; w++) {
         unsigned long long m = s.cbits[w * cc + p];
         while (m) {
-            uint32_t k = w * 64 + (uint32_t)__builtin_ctzll(m);
+            NearD b = s.nq[w * 64 + (uint32_t)__builtin_ctzll(m)];
             m &= m - 1ull;
-            double ex = qx - s.nx[k], ey = qy - s.ny[k], ez = qz - s.nz[k];
-            if (ex * ex + ey * ey + ez * ez < s.nblk[k]) return true;
+            double ex = qx - b.x, ey = qy - b.y, ez = qz - b.z;
+            if (ex * ex + ey * ey + ez * ez < b.blk) return true;
         }
     }
     return false;
@@ -330,12 +336,32 @@ __device__ inline bool precedes(double d, uint32_t p, double best, uint32_t bo) 
     return d < best || (d == best && p < bo);
 }
 
+// Distance from (x, y, z) of a class-K candidate (record d) as eval_local
+// compares it (field/point.rs:419-489); +inf for a sphere through the point.
+template <int K>
+__device__ inline double cand_dist(const double* d, double x, double y, double z) {
+    if (K == 0) {
+        double vx = x - d[0], vy = y - d[1], vz = z - d[2];
+        double len = sqrt(vx * vx + vy * vy + vz * vz);
+        return len < 1e-12 ? INFINITY : fabs(len - d[3]);
+    }
+    if (K == 1) {
+        double yx = x - d[0], yy = y - d[1], yz = z - d[2];
+        double ax = d[3], ay = d[4], az = d[5], rho = d[6];
+        double along = yx * ax + yy * ay + yz * az;
+        double wx = yx - ax * along, wy = yy - ay * along, wz = yz - az * along;
+        double wl = sqrt(wx * wx + wy * wy + wz * wz);
+        return sqrt(along * along + (wl - rho) * (wl - rho));
+    }
+    return ddist(x, y, z, d[0], d[1], d[2]);
+}
+
 // Projection of x onto candidate p (its probe position, point.rs:419-489).
-__device__ inline void project(const DArr& A, const RLists& s, uint32_t cc, uint32_t ns, uint32_t nc, uint32_t p,
-                               double x, double y, double z, double& qx, double& qy, double& qz) {
-    const double* d = s.cdat + p;
+__device__ inline void project(const DArr& A, const RLists& s, uint32_t ns, uint32_t nc, uint32_t p, double x,
+                               double y, double z, double& qx, double& qy, double& qz) {
+    const double* d = s.cdat + (size_t)CDAT * p;
     if (p < ns) {
-        double cx = d[0], cy = d[cc], cz = d[2 * cc], R = d[3 * cc];
+        double cx = d[0], cy = d[1], cz = d[2], R = d[3];
         double vx = x - cx, vy = y - cy, vz = z - cz;
         double len = sqrt(vx * vx + vy * vy + vz * vz);
         double sc = R / len;
@@ -343,8 +369,7 @@ __device__ inline void project(const DArr& A, const RLists& s, uint32_t cc, uint
         qy = cy + vy * sc;
         qz = cz + vz * sc;
     } else if (p < ns + nc) {
-        double mx = d[0], my = d[cc], mz = d[2 * cc], ax = d[3 * cc], ay = d[4 * cc], az = d[5 * cc],
-               rho = d[6 * cc];
+        double mx = d[0], my = d[1], mz = d[2], ax = d[3], ay = d[4], az = d[5], rho = d[6];
         double yx = x - mx, yy = y - my, yz = z - mz;
         double along = yx * ax + yy * ay + yz * az;
         double wx = yx - ax * along, wy = yy - ay * along, wz = yz - az * along;
@@ -365,8 +390,8 @@ __device__ inline void project(const DArr& A, const RLists& s, uint32_t cc, uint
         qz = mz + dz * rho;
     } else {
         qx = d[0];
-        qy = d[cc];
-        qz = d[2 * cc];
+        qy = d[1];
+        qz = d[2];
     }
 }
 
@@ -379,8 +404,56 @@ struct Group {
 
 __device__ inline bool group_any(const Group& G, bool p) { return ((__ballot(p) >> G.shift) & G.mask) != 0ull; }
 
+// The least (best, bo) over the group's lanes.
+__device__ inline void group_least(const Group& G, double& best, uint32_t& bo) {
+    for (uint32_t o = G.g >> 1; o > 0; o >>= 1) {
+        double od = __shfl_xor(best, (int)o, WAVE);
+        uint32_t oo = __shfl_xor(bo, (int)o, WAVE);
+        if (precedes(od, oo, best, bo)) {
+            best = od;
+            bo = oo;
+        }
+    }
+}
+
+// One class's candidates [a, e) of this lane (a + r, a + r + g, ...),
+// W distances at a time, then taken in list order: a candidate is checked
+// against its blockers only if it precedes the best so far. The scan stops
+// at the first step whose lower bound key - delta exceeds best. The calls
+// use W = 1: on gfx950 four wavefronts per SIMD (at most 128 VGPRs) hide
+// the LDS latency better than wider batches do.
+template <int K, int W>
+__device__ inline void scan_class(const DArr& A, const RLists& s, uint32_t cc, uint32_t nw, uint32_t ns, uint32_t nc,
+                                  uint32_t a, uint32_t e, const Group& G, double x, double y, double z,
+                                  double delta, double cutoff, double& best, uint32_t& bo) {
+    for (uint32_t p0 = a + G.r; p0 < e; p0 += W * G.g) {
+        double lim = best + delta;
+        double dist[W];
+        bool stop = false;
+        for (int j = 0; j < W; j++) {
+            uint32_t p = p0 + j * G.g;
+            bool use = p < e && !stop && !(key_value(s.ckey[p]) > lim);
+            stop |= !use;
+            dist[j] = use ? cand_dist<K>(s.cdat + (size_t)CDAT * p, x, y, z) : INFINITY;
+        }
+        for (int j = 0; j < W; j++) {
+            uint32_t p = p0 + j * G.g;
+            if (!(dist[j] < cutoff) || !precedes(dist[j], p, best, bo)) continue;
+            if (K != 2) {
+                double qx, qy, qz;
+                project(A, s, ns, nc, p, x, y, z, qx, qy, qz);
+                if (blocked(s, cc, nw, p, qx, qy, qz)) continue;
+            }
+            best = dist[j];
+            bo = p;
+        }
+        if (stop) break;
+    }
+}
+
 // eval_local (field/point.rs:686-722) at (x, y, z), split over a lane group.
-// (cx, cy, cz) is the centre of the gathered ball.
+// (cx, cy, cz) is the centre of the gathered ball. The group shares its
+// best after each class, which only tightens later tests.
 __device__ inline EvalD eval_group(const DArr& A, const RLists& s, const uint32_t* n, uint32_t cc, uint32_t nw,
                                    const Group& G, double cx, double cy, double cz, double x, double y, double z,
                                    double cutoff) {
@@ -392,7 +465,8 @@ __device__ inline EvalD eval_group(const DArr& A, const RLists& s, const uint32_
     const uint32_t n_in = n[R_IN];
     for (uint32_t q = G.r; q < n_in; q += G.g) {
         uint32_t k = s.ins[q];
-        double dx = x - s.nx[k], dy = y - s.ny[k], dz = z - s.nz[k];
+        NearD b = s.nq[k];
+        double dx = x - b.x, dy = y - b.y, dz = z - b.z;
         ins |= dx * dx + dy * dy + dz * dz < s.nin[k];
     }
     if (!group_any(G, ins)) {
@@ -406,66 +480,19 @@ __device__ inline EvalD eval_group(const DArr& A, const RLists& s, const uint32_
     const double delta = sqrt(ox * ox + oy * oy + oz * oz) + 1e-7;
     double best = cutoff;
     uint32_t bo = 0xffffffffu;
-    // Class S.
-    for (uint32_t p = G.r; p < ns; p += G.g) {
-        if (key_value(s.ckey[p]) - delta > best) break;
-        const double* d = s.cdat + p;
-        double px = d[0], py = d[cc], pz = d[2 * cc], R = d[3 * cc];
-        double vx = x - px, vy = y - py, vz = z - pz;
-        double len = sqrt(vx * vx + vy * vy + vz * vz);
-        if (len < 1e-12) continue;
-        double dist = fabs(len - R);
-        if (!(dist < cutoff) || !precedes(dist, p, best, bo)) continue;
-        double sc = R / len;
-        if (!blocked(s, cc, nw, p, px + vx * sc, py + vy * sc, pz + vz * sc)) {
-            best = dist;
-            bo = p;
-        }
-    }
-    // Class C.
-    for (uint32_t p = ns + G.r; p < ns + nc; p += G.g) {
-        if (key_value(s.ckey[p]) - delta > best) break;
-        const double* d = s.cdat + p;
-        double mx = d[0], my = d[cc], mz = d[2 * cc], ax = d[3 * cc], ay = d[4 * cc], az = d[5 * cc],
-               rho = d[6 * cc];
-        double yx = x - mx, yy = y - my, yz = z - mz;
-        double along = yx * ax + yy * ay + yz * az;
-        double wx = yx - ax * along, wy = yy - ay * along, wz = yz - az * along;
-        double wl = sqrt(wx * wx + wy * wy + wz * wz);
-        double dist = sqrt(along * along + (wl - rho) * (wl - rho));
-        if (!(dist < cutoff) || !precedes(dist, p, best, bo)) continue;
-        double qx, qy, qz;
-        project(A, s, cc, ns, nc, p, x, y, z, qx, qy, qz);
-        if (!blocked(s, cc, nw, p, qx, qy, qz)) {
-            best = dist;
-            bo = p;
-        }
-    }
-    // Class V (no blockers).
-    for (uint32_t p = ns + nc + G.r; p < total; p += G.g) {
-        if (key_value(s.ckey[p]) - delta > best) break;
-        const double* d = s.cdat + p;
-        double dist = ddist(x, y, z, d[0], d[cc], d[2 * cc]);
-        if (dist < cutoff && precedes(dist, p, best, bo)) {
-            best = dist;
-            bo = p;
-        }
-    }
-    for (uint32_t o = G.g >> 1; o > 0; o >>= 1) {
-        double od = __shfl_xor(best, (int)o, WAVE);
-        uint32_t oo = __shfl_xor(bo, (int)o, WAVE);
-        if (precedes(od, oo, best, bo)) {
-            best = od;
-            bo = oo;
-        }
-    }
+    scan_class<0, 1>(A, s, cc, nw, ns, nc, 0, ns, G, x, y, z, delta, cutoff, best, bo);
+    group_least(G, best, bo);
+    scan_class<1, 1>(A, s, cc, nw, ns, nc, ns, ns + nc, G, x, y, z, delta, cutoff, best, bo);
+    group_least(G, best, bo);
+    scan_class<2, 1>(A, s, cc, nw, ns, nc, ns + nc, total, G, x, y, z, delta, cutoff, best, bo);
+    group_least(G, best, bo);
     if (bo == 0xffffffffu) {  // nothing free below the cutoff: Beyond
         e.value = A.probe - cutoff;
         e.kind = F_BEYOND;
         e.id = 0;
         return e;
     }
-    project(A, s, cc, ns, nc, bo, x, y, z, e.qx, e.qy, e.qz);
+    project(A, s, ns, nc, bo, x, y, z, e.qx, e.qy, e.qz);
     e.value = A.probe - best;
     e.kind = bo < ns ? F_SPHERE : (bo < ns + nc ? F_CIRCLE : F_VERTEX);
     e.id = s.cid[bo] & ID_MASK;
@@ -483,114 +510,132 @@ struct VOut {
 };
 
 // State of one edge's root search (mesh/extract.rs:425-535): up to 8
-// Newton / Illinois steps, then up to 64 bisections.
+// Newton / Illinois steps, then up to 64 bisections. The end points and
+// direction are recomputed from the lattice point when needed, with the
+// same operations, so they keep their bits.
 struct EdgeState {
-    double pa[3], d[3], fa, dn;
+    int32_t ia[3];  // lower lattice point
+    uint32_t id;    // edge id in its brick; axis = id % 3
+    float fa;       // sample value at the lower point
+    uint32_t st;    // it (bits 0-6), side + 1 (7-8), bisect (9), neg0 (10), converged (11), evals (16-31)
     double t, t0, g0, t1, g1;
     EvalD last;
-    size_t slot;
-    int side;
-    uint32_t it, evals;
-    bool bisect, neg0, converged;
 };
 
-__device__ inline void edge_start(const DRun& P, const float* val, const int64_t org[3], uint32_t id, size_t slot,
-                                  EdgeState& E) {
+struct EdgeGeom {
+    double pa[3], d[3], dn;
+};
+
+__device__ inline EdgeGeom edge_geom(const EdgeState& E, double h) {
+    EdgeGeom G;
+    uint32_t axis = E.id % 3;
+    for (int a = 0; a < 3; a++) {
+        int64_t ib = (int64_t)E.ia[a] + (a == (int)axis ? 1 : 0);
+        G.pa[a] = (double)E.ia[a] * h;
+        G.d[a] = (double)ib * h - G.pa[a];
+    }
+    G.dn = sqrt(G.d[0] * G.d[0] + G.d[1] * G.d[1] + G.d[2] * G.d[2]);
+    return G;
+}
+
+__device__ inline void edge_start(const DRun& P, const float* val, const int64_t org[3], uint32_t id, EdgeState& E) {
     const uint32_t b = P.b;
     uint32_t axis = id % 3, cell = id / 3;
     uint32_t x = cell % b, y = (cell / b) % b, z = cell / (b * b);
     uint32_t i0 = x + P.side * (y + P.side * z);
-    uint32_t st = axis == 0 ? 1u : (axis == 1 ? P.side : P.side * P.side);
-    int64_t ia[3] = {org[0] + x, org[1] + y, org[2] + z};
-    int64_t ib[3] = {ia[0], ia[1], ia[2]};
-    ib[axis] += 1;
-    const double h = P.h;
-    for (int a = 0; a < 3; a++) {
-        E.pa[a] = (double)ia[a] * h;
-        E.d[a] = (double)ib[a] * h - E.pa[a];
-    }
-    double fa = (double)val[i0], fb = (double)val[i0 + st];
-    E.fa = fa;
+    uint32_t stp = axis == 0 ? 1u : (axis == 1 ? P.side : P.side * P.side);
+    E.ia[0] = (int32_t)(org[0] + x);
+    E.ia[1] = (int32_t)(org[1] + y);
+    E.ia[2] = (int32_t)(org[2] + z);
+    E.id = id;
+    float fa32 = val[i0], fb32 = val[i0 + stp];
+    E.fa = fa32;
+    double fa = (double)fa32, fb = (double)fb32;
     E.t = fmin(fmax(fa / (fa - fb), 0.0), 1.0);
     E.t0 = 0.0;
     E.g0 = fa;
     E.t1 = 1.0;
     E.g1 = fb;
-    E.side = 0;
-    E.it = 0;
-    E.evals = 0;
-    E.bisect = false;
-    E.neg0 = false;
-    E.converged = false;
-    E.dn = sqrt(E.d[0] * E.d[0] + E.d[1] * E.d[1] + E.d[2] * E.d[2]);
-    E.slot = slot;
+    E.st = 1u << 7;  // side 0
 }
 
 // Applies evaluation e at E.t; true when the search is over.
-__device__ inline bool edge_step(EdgeState& E, const EvalD& e) {
-    E.evals++;
+__device__ inline bool edge_step(EdgeState& E, const EvalD& e, double h) {
+    E.st += 1u << 16;  // evals
     E.last = e;
     double g = e.value;
-    if (!E.bisect) {
+    uint32_t it = E.st & 127u;
+    if (!(E.st & (1u << 9))) {
         if (fabs(g) <= 1e-5) {
-            E.converged = true;
+            E.st |= 1u << 11;  // converged
             return true;
         }
+        int side = (int)((E.st >> 7) & 3u) - 1;
         if ((g < 0.0) == (E.g0 < 0.0)) {
             E.t0 = E.t;
             E.g0 = g;
-            if (E.side == -1) E.g1 *= 0.5;
-            E.side = -1;
+            if (side == -1) E.g1 *= 0.5;
+            side = -1;
         } else {
             E.t1 = E.t;
             E.g1 = g;
-            if (E.side == 1) E.g0 *= 0.5;
-            E.side = 1;
+            if (side == 1) E.g0 *= 0.5;
+            side = 1;
         }
+        EdgeGeom Q = edge_geom(E, h);
         double slope = 0.0;
         if (e.kind != F_SOLVENT && e.kind != F_BEYOND) {
-            double x = E.pa[0] + E.d[0] * E.t, y = E.pa[1] + E.d[1] * E.t, z = E.pa[2] + E.d[2] * E.t;
+            double x = Q.pa[0] + Q.d[0] * E.t, y = Q.pa[1] + Q.d[1] * E.t, z = Q.pa[2] + Q.d[2] * E.t;
             double wx = e.qx - x, wy = e.qy - y, wz = e.qz - z;
             double len = sqrt(wx * wx + wy * wy + wz * wz);
             if (len > 1e-12) {
                 double nx = wx / len, ny = wy / len, nz = wz / len;
-                slope = nx * E.d[0] + ny * E.d[1] + nz * E.d[2];
+                slope = nx * Q.d[0] + ny * Q.d[1] + nz * Q.d[2];
             }
         }
         double newton = E.t - g / slope;
-        if (fabs(slope) > 1e-3 * E.dn && newton > E.t0 && newton < E.t1) {
+        if (fabs(slope) > 1e-3 * Q.dn && newton > E.t0 && newton < E.t1) {
             E.t = newton;
         } else {
             double sv = (E.t0 * E.g1 - E.t1 * E.g0) / (E.g1 - E.g0);
             E.t = (isfinite(sv) && sv > E.t0 && sv < E.t1) ? sv : 0.5 * (E.t0 + E.t1);
         }
-        if (++E.it == 8) {
-            E.bisect = true;
-            E.neg0 = E.g0 < 0.0;
-            E.it = 0;
+        it++;
+        uint32_t st = (E.st & ~(127u | (3u << 7))) | ((uint32_t)(side + 1) << 7);
+        if (it == 8) {
+            st |= 1u << 9;                      // bisect
+            if (E.g0 < 0.0) st |= 1u << 10;     // neg0
             E.t = 0.5 * (E.t0 + E.t1);
+        } else {
+            st |= it;
         }
+        E.st = st;
         return false;
     }
-    if (fabs(g) <= 1e-5 || (E.t1 - E.t0) * E.dn <= 2.0 * 1e-5) return true;
-    if ((g < 0.0) == E.neg0)
+    double dn = edge_geom(E, h).dn;
+    if (fabs(g) <= 1e-5 || (E.t1 - E.t0) * dn <= 2.0 * 1e-5) return true;
+    if ((g < 0.0) == ((E.st & (1u << 10)) != 0))
         E.t0 = E.t;
     else
         E.t1 = E.t;
-    if (++E.it == 64) return true;
+    if (++it == 64) return true;
+    E.st = (E.st & ~127u) | it;
     E.t = 0.5 * (E.t0 + E.t1);
     return false;
 }
 
-// The vertex and its attributes (mesh/extract.rs:535, 553-597); the
-// group's rank-0 lane writes them.
-__device__ inline void edge_finish(const DArr& A, const DRun& P, const EdgeState& E, const Group& G, const VOut& o) {
+// The vertex and its attributes (mesh/extract.rs:535, 553-597) into `slot`;
+// the group's rank-0 lane writes them.
+__device__ inline void edge_finish(const DArr& A, const DRun& P, const EdgeState& E, const Group& G, size_t slot,
+                                   const VOut& o) {
+    if (G.r != 0) return;
     const double t = E.t;
     const EvalD& last = E.last;
+    EdgeGeom Q = edge_geom(E, P.h);
     double tc = fmin(fmax(t, P.t_min), 1.0 - P.t_min);
-    double v[3] = {E.pa[0] + E.d[0] * tc, E.pa[1] + E.d[1] * tc, E.pa[2] + E.d[2] * tc};
+    double v[3] = {Q.pa[0] + Q.d[0] * tc, Q.pa[1] + Q.d[1] * tc, Q.pa[2] + Q.d[2] * tc};
     double ow[3];
-    for (int a = 0; a < 3; a++) ow[a] = E.fa < 0.0 ? E.d[a] : -E.d[a];
+    for (int a = 0; a < 3; a++) ow[a] = (double)E.fa < 0.0 ? Q.d[a] : -Q.d[a];
     double ol = sqrt(ow[0] * ow[0] + ow[1] * ow[1] + ow[2] * ow[2]);
     double outward[3] = {ow[0] / ol, ow[1] / ol, ow[2] / ol};
     double nrm[3];
@@ -621,7 +666,6 @@ __device__ inline void edge_finish(const DArr& A, const DRun& P, const EdgeState
         }
         return A.orig[bk];
     };
-    if (G.r != 0) return;
     if (last.kind == F_SPHERE) {
         unit(last.qx - v[0], last.qy - v[1], last.qz - v[2]);
         patch = 0;
@@ -656,7 +700,6 @@ __device__ inline void edge_finish(const DArr& A, const DRun& P, const EdgeState
         });
         atom = bi < 0 ? 0u : A.orig[bi];
     }
-    const size_t slot = E.slot;
     for (int a = 0; a < 3; a++) {
         o.pos[3 * slot + a] = (float)v[a];
         o.nrm[3 * slot + a] = (float)nrm[a];
@@ -664,13 +707,15 @@ __device__ inline void edge_finish(const DArr& A, const DRun& P, const EdgeState
     o.atom[slot] = atom;
     o.patch[slot] = patch;
     if (o.t) o.t[slot] = t;
-    if (o.evals) o.evals[slot] = E.evals | (E.converged ? 0x80000000u : 0u);
+    if (o.evals) o.evals[slot] = (E.st >> 16) | ((E.st & (1u << 11)) ? 0x80000000u : 0u);
 }
 
 // One workgroup (256 threads) per sub-block with crossing edges (`work[w]`
-// = 8*brick + sb): gather_local into LDS (or scratch slice w), compact the
-// crossing edges whose lower sample lies in the sub-block, then refine them
-// by lane groups. Sub-blocks whose lists overflow go to `overflow`.
+// = 8*brick + sb): gather_local into LDS (GLOBAL false) or into scratch
+// slice w (GLOBAL true), compact the crossing edges whose lower sample lies
+// in the sub-block, then refine them by lane groups. Sub-blocks whose lists
+// overflow go to `overflow`.
+template <bool GLOBAL>
 __global__ void __launch_bounds__(256) k_refine(DArr A, DRun P, RCaps cap, RLayout lay, const int32_t* keys,
                                                 const float* values, const uint32_t* work, uint32_t n_work,
                                                 const unsigned long long* mask, const uint32_t* wpre,
@@ -684,7 +729,7 @@ __global__ void __launch_bounds__(256) k_refine(DArr A, DRun P, RCaps cap, RLayo
     uint32_t w = blockIdx.x;
     if (w >= n_work) return;
     uint32_t j = work[w], brick = j >> 3, sb = j & 7;
-    char* base = scratch ? scratch + (size_t)w * slice : smem;
+    char* base = GLOBAL ? scratch + (size_t)w * slice : smem;
     RLists s = rcarve(base, lay);
     const int32_t* key = keys + 3 * brick;
     const uint32_t b = P.b, sub = P.sub;
@@ -741,28 +786,28 @@ __global__ void __launch_bounds__(256) k_refine(DArr A, DRun P, RCaps cap, RLayo
     G.mask = g == WAVE ? ~0ull : ((1ull << g) - 1ull);
     const uint32_t ngroups = blockDim.x / g, nw = cap.near / 64;
     const double cutoff = A.probe + P.band;
-    const uint32_t vb = vbase[brick];
     uint32_t q = threadIdx.x / g;
     bool active = q < ne;
     EdgeState E;
-    if (active) {
-        uint32_t id = s.ebuf[q];
-        edge_start(P, val, org, id, (size_t)vb + edge_rank(mask, wpre, words, brick, id), E);
-    }
+    if (active) edge_start(P, val, org, s.ebuf[q], E);
     unsigned long long my_evals = 0;
     while (__ballot(active) != 0ull) {
         if (active) {
-            double x = E.pa[0] + E.d[0] * E.t, y = E.pa[1] + E.d[1] * E.t, z = E.pa[2] + E.d[2] * E.t;
+            double x, y, z;
+            {
+                EdgeGeom Q = edge_geom(E, h);
+                x = Q.pa[0] + Q.d[0] * E.t;
+                y = Q.pa[1] + Q.d[1] * E.t;
+                z = Q.pa[2] + Q.d[2] * E.t;
+            }
             EvalD e = eval_group(A, s, n, cap.cand, nw, G, c[0], c[1], c[2], x, y, z, cutoff);
-            if (edge_step(E, e)) {
-                edge_finish(A, P, E, G, out);
-                my_evals += E.evals;
+            if (edge_step(E, e, h)) {
+                size_t slot = (size_t)vbase[brick] + edge_rank(mask, wpre, words, brick, E.id);
+                edge_finish(A, P, E, G, slot, out);
+                my_evals += E.st >> 16;
                 q += ngroups;
                 active = q < ne;
-                if (active) {
-                    uint32_t id = s.ebuf[q];
-                    edge_start(P, val, org, id, (size_t)vb + edge_rank(mask, wpre, words, brick, id), E);
-                }
+                if (active) edge_start(P, val, org, s.ebuf[q], E);
             }
         }
     }

@@ -127,11 +127,20 @@ __device__ inline bool blocked_f(const Lists& s, uint32_t cc, uint32_t nw, uint3
     for (uint32_t w = 0; w < nw; w++) {
         unsigned long long m = s.cbits[w * cc + p];
         while (m) {
-            uint32_t k = w * 64 + (uint32_t)__builtin_ctzll(m);
+            uint32_t k[4];
+            k[0] = w * 64 + (uint32_t)__builtin_ctzll(m);
             m &= m - 1ull;
-            float4 b = s.nd[k];
-            float ex = qx - b.x, ey = qy - b.y, ez = qz - b.z;
-            if (!(ex * ex + ey * ey + ez * ez >= b.w)) return true;
+            for (int j = 1; j < 4; j++) {
+                k[j] = m ? w * 64 + (uint32_t)__builtin_ctzll(m) : k[0];
+                m &= m - 1ull;
+            }
+            bool hit = false;
+            for (int j = 0; j < 4; j++) {
+                float4 b = s.nd[k[j]];
+                float ex = qx - b.x, ey = qy - b.y, ez = qz - b.z;
+                hit |= !(ex * ex + ey * ey + ez * ez >= b.w);
+            }
+            if (hit) return true;
         }
     }
     return false;
@@ -157,40 +166,62 @@ __device__ inline float ses_value(const Lists& s, const uint32_t* n, uint32_t cc
         float d2 = dx * dx + dy * dy + dz * dz;
         best2 = d2 < best2 ? d2 : best2;
     }
+    // Class S and C, four candidates' distances at a time; a candidate's
+    // projection and blockers are tested only if it improves on best2.
     const uint32_t ns = n[C_S], nsc = ns + n[C_C];
-    for (uint32_t i = 0; i < ns; i++) {
-        float4 c = s.cdat[3 * i];
-        float dx = x - c.x, dy = y - c.y, dz = z - c.z;
-        float dist = sqrtf(dx * dx + dy * dy + dz * dz);
-        float lb = fabsf(dist - c.w);
-        float lb2 = lb * lb;
-        if (!(lb2 < best2 && 1e-6f < dist)) continue;
-        float sc = c.w / dist;
-        if (!blocked_f(s, cc, nw, i, c.x + dx * sc, c.y + dy * sc, c.z + dz * sc)) best2 = lb2;
-    }
-    for (uint32_t c = ns; c < nsc; c++) {
-        float4 m = s.cdat[3 * c], u = s.cdat[3 * c + 1];
-        float rho = m.w;
-        float yx = x - m.x, yy = y - m.y, yz = z - m.z;
-        float along = yx * u.x + yy * u.y + yz * u.z;
-        float wx = yx - u.x * along, wy = yy - u.y * along, wz = yz - u.z * along;
-        float wl = sqrtf(wx * wx + wy * wy + wz * wz);
-        float t = wl - rho;
-        float d2 = along * along + t * t;
-        if (!(d2 < best2)) continue;
-        float qx, qy, qz;
-        if (wl < 1e-6f) {
-            float4 e = s.cdat[3 * c + 2];
-            qx = m.x + e.x * rho;
-            qy = m.y + e.y * rho;
-            qz = m.z + e.z * rho;
-        } else {
-            float sc = rho / wl;
-            qx = m.x + wx * sc;
-            qy = m.y + wy * sc;
-            qz = m.z + wz * sc;
+    for (uint32_t i0 = 0; i0 < ns; i0 += 4) {
+        float lb2[4], dist[4];
+        for (int j = 0; j < 4; j++) {
+            uint32_t i = min(i0 + j, ns - 1);
+            float4 c = s.cdat[3 * i];
+            float dx = x - c.x, dy = y - c.y, dz = z - c.z;
+            dist[j] = sqrtf(dx * dx + dy * dy + dz * dz);
+            float lb = fabsf(dist[j] - c.w);
+            lb2[j] = i0 + j < ns ? lb * lb : INFINITY;
         }
-        if (!blocked_f(s, cc, nw, c, qx, qy, qz)) best2 = d2;
+        for (int j = 0; j < 4; j++) {
+            if (!(lb2[j] < best2 && 1e-6f < dist[j])) continue;
+            uint32_t i = i0 + j;
+            float4 c = s.cdat[3 * i];
+            float dx = x - c.x, dy = y - c.y, dz = z - c.z;
+            float sc = c.w / dist[j];
+            if (!blocked_f(s, cc, nw, i, c.x + dx * sc, c.y + dy * sc, c.z + dz * sc)) best2 = lb2[j];
+        }
+    }
+    for (uint32_t c0 = ns; c0 < nsc; c0 += 4) {
+        float d2[4], wl[4];
+        for (int j = 0; j < 4; j++) {
+            uint32_t c = min(c0 + j, nsc - 1);
+            float4 m = s.cdat[3 * c], u = s.cdat[3 * c + 1];
+            float yx = x - m.x, yy = y - m.y, yz = z - m.z;
+            float along = yx * u.x + yy * u.y + yz * u.z;
+            float wx = yx - u.x * along, wy = yy - u.y * along, wz = yz - u.z * along;
+            wl[j] = sqrtf(wx * wx + wy * wy + wz * wz);
+            float t = wl[j] - m.w;
+            d2[j] = c0 + j < nsc ? along * along + t * t : INFINITY;
+        }
+        for (int j = 0; j < 4; j++) {
+            if (!(d2[j] < best2)) continue;
+            uint32_t c = c0 + j;
+            float4 m = s.cdat[3 * c], u = s.cdat[3 * c + 1];
+            float rho = m.w;
+            float qx, qy, qz;
+            if (wl[j] < 1e-6f) {
+                float4 e = s.cdat[3 * c + 2];
+                qx = m.x + e.x * rho;
+                qy = m.y + e.y * rho;
+                qz = m.z + e.z * rho;
+            } else {
+                float yx = x - m.x, yy = y - m.y, yz = z - m.z;
+                float along = yx * u.x + yy * u.y + yz * u.z;
+                float wx = yx - u.x * along, wy = yy - u.y * along, wz = yz - u.z * along;
+                float sc = rho / wl[j];
+                qx = m.x + wx * sc;
+                qy = m.y + wy * sc;
+                qz = m.z + wz * sc;
+            }
+            if (!blocked_f(s, cc, nw, c, qx, qy, qz)) best2 = d2[j];
+        }
     }
     return probe - sqrtf(best2);
 }
@@ -321,9 +352,10 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
 }
 
 // Samples one sub-block per workgroup. `work[w]` = 8 * brick + sb of a
-// non-uniform sub-block. With `scratch` == nullptr the lists live in
-// dynamic LDS; otherwise in scratch slice w of `slice` bytes. Sub-blocks
-// whose lists do not fit are appended to `overflow`.
+// non-uniform sub-block. The lists live in dynamic LDS (GLOBAL false) or in
+// scratch slice w of `slice` bytes (GLOBAL true). Sub-blocks whose lists do
+// not fit are appended to `overflow`.
+template <bool GLOBAL>
 __global__ void __launch_bounds__(256) k_sample(DArr A, DLoc Lc, DRun P, Caps cap, Layout lay,
                                                 const int32_t* keys, const uint8_t* st, const uint32_t* work,
                                                 uint32_t n_work, float* values, char* scratch, size_t slice,
@@ -337,7 +369,7 @@ __global__ void __launch_bounds__(256) k_sample(DArr A, DLoc Lc, DRun P, Caps ca
     uint32_t w = blockIdx.x;
     if (w >= n_work) return;
     uint32_t j = work[w], brick = j >> 3, sb = j & 7;
-    char* base = scratch ? scratch + (size_t)w * slice : smem;
+    char* base = GLOBAL ? scratch + (size_t)w * slice : smem;
     Lists s = carve(base, lay);
     const int32_t* key = keys + 3 * brick;
     if (threadIdx.x == 0) n_owned = 0;

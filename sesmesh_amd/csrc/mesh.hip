@@ -830,14 +830,14 @@ static int mesh_stage(smg_resident* r, const smg_mesh_params* prm, smg_mesh* M, 
             HIP_TRY(T.get(&nnext, 1));
             HIP_TRY(hipMemsetAsync(nnext, 0, 4, s));
             if (lds) {
-                k_sample<<<count, 256, lay.bytes, s>>>(A, L, P, cap, lay, M->keys, st, list, count, M->values, nullptr,
+                k_sample<false><<<count, 256, lay.bytes, s>>>(A, L, P, cap, lay, M->keys, st, list, count, M->values, nullptr,
                                                        0, next, nnext, nsamp);
             } else {
                 uint32_t chunk = std::min(count, SCRATCH_CHUNK);
                 char* scratch;
                 HIP_TRY(T.get(&scratch, (size_t)chunk * lay.bytes));
                 for (uint32_t c0 = 0; c0 < count; c0 += chunk)
-                    k_sample<<<std::min(chunk, count - c0), 256, 0, s>>>(A, L, P, cap, lay, M->keys, st, list + c0,
+                    k_sample<true><<<std::min(chunk, count - c0), 256, 0, s>>>(A, L, P, cap, lay, M->keys, st, list + c0,
                                                                          std::min(chunk, count - c0), M->values,
                                                                          scratch, lay.bytes, next, nnext, nsamp);
             }
@@ -905,14 +905,14 @@ static int mesh_stage(smg_resident* r, const smg_mesh_params* prm, smg_mesh* M, 
             HIP_TRY(T.get(&nnext, 1));
             HIP_TRY(hipMemsetAsync(nnext, 0, 4, s));
             if (lds) {
-                k_refine<<<count, 256, lay.bytes, s>>>(A, P, cap, lay, M->keys, M->values, list, count, mask, wpre,
+                k_refine<false><<<count, 256, lay.bytes, s>>>(A, P, cap, lay, M->keys, M->values, list, count, mask, wpre,
                                                        words, vbase, vo, nullptr, 0, next, nnext, evd);
             } else {
                 uint32_t chunk = std::min(count, SCRATCH_CHUNK);
                 char* scratch;
                 HIP_TRY(T.get(&scratch, (size_t)chunk * lay.bytes));
                 for (uint32_t c0 = 0; c0 < count; c0 += chunk)
-                    k_refine<<<std::min(chunk, count - c0), 256, 0, s>>>(A, P, cap, lay, M->keys, M->values, list + c0,
+                    k_refine<true><<<std::min(chunk, count - c0), 256, 0, s>>>(A, P, cap, lay, M->keys, M->values, list + c0,
                                                                          std::min(chunk, count - c0), mask, wpre,
                                                                          words, vbase, vo, scratch, lay.bytes, next,
                                                                          nnext, evd);
