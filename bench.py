@@ -89,10 +89,12 @@ def cpu_baseline(xyzr, probe, spacing, brick, threads, sample_stride):
     }
 
 
-def batched_plain(dev, count, seed):
-    """Per-molecule launches of the mesh stage on `count` seeded molecules
-    (atoms log-uniform in 20-5000, probe 1.2-1.4 A, spacing 0.2/0.25/0.5 A):
-    arrangements built and resident first; the timed loop is the mesh stages."""
+def batched(dev, count, seed, repeats=3):
+    """Batched serving on `count` seeded molecules (atoms log-uniform in
+    20-5000, probe 1.2-1.4 A, spacing 0.2/0.25/0.5 A): arrangements built and
+    resident first. Timed: per-molecule launches of the mesh stage ("plain")
+    against one batched pass over all of them (smg_mesh_many); best of
+    `repeats` each. Every part's counts must equal its plain run's."""
     import sesmesh_amd
 
     atoms, probe, spacing, seeds = sesmesh_amd.synth_batch_spec(seed, count)
@@ -100,15 +102,31 @@ def batched_plain(dev, count, seed):
     for n, p, sd in zip(atoms, probe, seeds):
         xyzr = sesmesh_amd.synth_globule(int(sd), int(n))
         residents.append(sesmesh_amd.Resident(dev, sesmesh_amd.Arrangement(xyzr, float(p))))
-    t = time.perf_counter()
-    verts = 0
-    for r, h in zip(residents, spacing):
-        m = r.mesh(float(h), 10)
-        verts += m.n_vertices
-        del m
-    dt = time.perf_counter() - t
-    return {"molecules": count, "atoms": int(atoms.sum()), "vertices": int(verts), "seconds": dt,
-            "molecules_per_s": count / dt}
+    plain_s, batch_s, plain_counts = [], [], None
+    for _ in range(repeats):
+        t = time.perf_counter()
+        counts = []
+        for r, h in zip(residents, spacing):
+            m = r.mesh(float(h), 10)
+            counts.append((m.n_vertices, m.n_triangles, m.info.refine_evaluations))
+            del m
+        plain_s.append(time.perf_counter() - t)
+        plain_counts = counts
+    progress(f"batched: plain {min(plain_s):.3f} s")
+    for _ in range(repeats):
+        t = time.perf_counter()
+        mb = sesmesh_amd.mesh_many(residents, spacing, 10)
+        batch_s.append(time.perf_counter() - t)
+        counts = [(p.vertices, p.triangles, p.refine_evaluations) for p in mb.parts]
+        if counts != plain_counts:
+            raise RuntimeError("batched parts differ from their single-molecule runs")
+        del mb
+    progress(f"batched: one pass {min(batch_s):.3f} s")
+    verts = int(sum(c[0] for c in plain_counts))
+    return {"molecules": count, "atoms": int(atoms.sum()), "vertices": verts,
+            "plain": {"seconds": min(plain_s), "molecules_per_s": count / min(plain_s)},
+            "batched": {"seconds": min(batch_s), "molecules_per_s": count / min(batch_s)},
+            "speedup": min(plain_s) / min(batch_s)}
 
 
 def main():
@@ -160,7 +178,7 @@ def main():
     f32_per_sample = ops["f32_ops"] / ops["samples"] if ops else OPS_PER_SAMPLE_F32
 
     kernels = {
-        "k_refine": ("mfma" if False else "valu-fp64", ph["k_refine_ms"], f64_per_eval * info.refine_evaluations,
+        "k_refine": ("valu-fp64", ph["k_refine_ms"], f64_per_eval * info.refine_evaluations,
                      "fp64"),
         "k_sample": ("valu-fp32", ph["k_sample_ms"], f32_per_sample * info.evaluated_samples, "fp32"),
     }
@@ -181,8 +199,9 @@ def main():
     batch = None
     if args.batch:
         progress(f"batched: {args.batch} molecules")
-        batch = {"plain": batched_plain(dev, args.batch, seed=4)}
-        progress(f"batched: {batch['plain']['molecules_per_s']:.1f} molecules/s")
+        batch = batched(dev, args.batch, seed=4)
+        progress(f"batched: {batch['batched']['molecules_per_s']:.1f} molecules/s, "
+                 f"{batch['speedup']:.2f}x the plain per-molecule launches")
 
     out = {
         "metric": METRIC, "value": s_per_mesh, "unit": "s/mesh", "n_gpus": 1, "steps": args.steps,

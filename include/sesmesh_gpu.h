@@ -180,6 +180,24 @@ int smg_mesh_info_get(const smg_mesh* m, smg_mesh_info* info);
 int smg_mesh_copy(const smg_mesh* m, float* positions, float* normals, uint32_t* atoms, uint8_t* patches,
                   uint32_t* triangles, int32_t* bricks, float* values, double* t, uint32_t* evals);
 
+/* Batched mesh stage: meshes `count` resident molecules of one device in one
+ * pass (every kernel runs once over the concatenated work of all of them;
+ * params[i] for residents[i], one brick size for the run). Part i of the
+ * result is byte for byte the mesh smg_mesh_new makes of residents[i]
+ * alone: the same bricks, samples, vertices and triangles (vertex indices
+ * local to the part). The reference has no multi-molecule call; this
+ * replaces a loop of `fold_resident` runs (crates/sesmesh/src/field/
+ * gpu.rs:532-712) over molecules, the caller of batched serving.
+ * smg_mesh_new(r, p, ...) is smg_mesh_many(&r, p, 1, ...), and
+ * smg_mesh_info_get / smg_mesh_copy read part 0. */
+int smg_mesh_many(smg_resident* const* residents, const smg_mesh_params* params, size_t count, smg_mesh** out,
+                  smg_mesh_times* times);
+size_t smg_mesh_count(const smg_mesh* m);
+int smg_mesh_part_info(const smg_mesh* m, size_t part, smg_mesh_info* info);
+int smg_mesh_part_copy(const smg_mesh* m, size_t part, float* positions, float* normals, uint32_t* atoms,
+                       uint8_t* patches, uint32_t* triangles, int32_t* bricks, float* values, double* t,
+                       uint32_t* evals);
+
 #ifdef __cplusplus
 }
 #endif

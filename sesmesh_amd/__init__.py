@@ -90,7 +90,8 @@ class MeshInfo(ctypes.Structure):
 ABI_FUNCTIONS = [
     "smg_last_error", "smg_device_open", "smg_device_close", "smg_selftest", "smg_arrangement_build",
     "smg_arrangement_free", "smg_arrangement_view", "smg_resident_new", "smg_resident_free", "smg_resident_bytes",
-    "smg_mesh_new", "smg_mesh_free", "smg_mesh_info_get", "smg_mesh_copy",
+    "smg_mesh_new", "smg_mesh_free", "smg_mesh_info_get", "smg_mesh_copy", "smg_mesh_many", "smg_mesh_count",
+    "smg_mesh_part_info", "smg_mesh_part_copy",
 ]
 
 _lib = None
@@ -114,6 +115,11 @@ def lib():
         L = ctypes.CDLL(str(LIB_PATH))
         L.smg_last_error.restype = ctypes.c_char_p
         L.smg_resident_bytes.restype = ctypes.c_size_t
+        L.smg_mesh_count.restype = ctypes.c_size_t
+        L.smg_mesh_part_info.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+        L.smg_mesh_part_copy.argtypes = [ctypes.c_void_p, ctypes.c_size_t] + [ctypes.c_void_p] * 9
+        L.smg_mesh_many.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
+                                    ctypes.c_void_p]
         for name in ("smg_device_close", "smg_arrangement_free", "smg_resident_free", "smg_mesh_free"):
             getattr(L, name).restype = None
         _lib = L
@@ -232,31 +238,54 @@ class Resident:
             self._h = _vp()
 
 
-class Mesh:
-    """One mesh stage on the device; the mesh stays in HBM until copied."""
+def _mesh_params(spacing, brick, refine, sizing):
+    return MeshParams(spacing, brick, 0 if refine == "exact" else 1, sizing, 0)
 
-    def __init__(self, resident, spacing, brick, refine, sizing):
-        p = MeshParams(spacing, brick, 0 if refine == "exact" else 1, sizing, 0)
+
+class Mesh:
+    """The mesh stage of one or more molecules on the device (one part per
+    molecule); the meshes stay in HBM until copied."""
+
+    def __init__(self, resident, spacing, brick, refine, sizing, _batch=None):
         self._h = _vp()
         self.times = MeshTimes()
-        _check(lib().smg_mesh_new(resident._h, ctypes.byref(p), ctypes.byref(self._h), ctypes.byref(self.times)))
-        self.resident = resident
-        info = MeshInfo()
-        _check(lib().smg_mesh_info_get(self._h, ctypes.byref(info)))
-        self.info = info
+        if _batch is None:
+            p = _mesh_params(spacing, brick, refine, sizing)
+            _check(lib().smg_mesh_new(resident._h, ctypes.byref(p), ctypes.byref(self._h),
+                                      ctypes.byref(self.times)))
+            self.residents = [resident]
+        else:
+            residents, params = _batch
+            handles = (ctypes.c_void_p * len(residents))(*[r._h.value for r in residents])
+            arr = (MeshParams * len(params))(*params)
+            _check(lib().smg_mesh_many(handles, arr, len(residents), ctypes.byref(self._h),
+                                       ctypes.byref(self.times)))
+            self.residents = list(residents)
         self.brick = brick
+        self.parts = [self._part_info(i) for i in range(lib().smg_mesh_count(self._h))]
+        info = self.parts[0]
+        self.info = info
         self.n_vertices, self.n_triangles, self.n_bricks = info.vertices, info.triangles, info.surface_bricks
 
-    def arrays(self, values=False):
-        nv, nt, nb = self.n_vertices, self.n_triangles, self.n_bricks
+    def _part_info(self, i):
+        info = MeshInfo()
+        _check(lib().smg_mesh_part_info(self._h, i, ctypes.byref(info)))
+        return info
+
+    def __len__(self):
+        return len(self.parts)
+
+    def arrays(self, values=False, part=0):
+        info = self.parts[part]
+        nv, nt, nb = info.vertices, info.triangles, info.surface_bricks
         out = {"positions": np.zeros((nv, 3), np.float32), "normals": np.zeros((nv, 3), np.float32),
                "atoms": np.zeros(nv, np.uint32), "patches": np.zeros(nv, np.uint8),
                "triangles": np.zeros((nt, 3), np.uint32), "bricks": np.zeros((nb, 3), np.int32),
                "t": np.zeros(nv, np.float64), "evals": np.zeros(nv, np.uint32)}
         vals = np.zeros((nb, (self.brick + 1) ** 3), np.float32) if values else None
-        _check(lib().smg_mesh_copy(self._h, _ptr(out["positions"]), _ptr(out["normals"]), _ptr(out["atoms"]),
-                                   _ptr(out["patches"]), _ptr(out["triangles"]), _ptr(out["bricks"]), _ptr(vals),
-                                   _ptr(out["t"]), _ptr(out["evals"])))
+        _check(lib().smg_mesh_part_copy(self._h, part, _ptr(out["positions"]), _ptr(out["normals"]),
+                                        _ptr(out["atoms"]), _ptr(out["patches"]), _ptr(out["triangles"]),
+                                        _ptr(out["bricks"]), _ptr(vals), _ptr(out["t"]), _ptr(out["evals"])))
         if values:
             out["values"] = vals
         return out
@@ -266,6 +295,15 @@ class Mesh:
         if h:
             lib().smg_mesh_free(h)
             self._h = _vp()
+
+
+def mesh_many(residents, spacings, brick=10, refine="exact", sizing=0):
+    """The mesh stage of many resident molecules (of one device) in one
+    batched pass (smg_mesh_many); part i equals residents[i].mesh(spacings[i])."""
+    params = [_mesh_params(float(h), brick, refine, sizing) for h in spacings]
+    if len(params) != len(residents):
+        raise ValueError("one spacing per molecule")
+    return Mesh(None, None, brick, refine, sizing, _batch=(residents, params))
 
 
 @dataclass

@@ -23,17 +23,18 @@ __device__ inline uint32_t cell_case(const float* val, uint32_t side, uint32_t x
 
 // Triangles per brick: thread t takes the contiguous cells [t*per, (t+1)*per)
 // in (z, y, x) order.
-__global__ void __launch_bounds__(256) k_cell_count(DRun P, const float* values, uint32_t n_bricks,
+__global__ void __launch_bounds__(256) k_cell_count(uint32_t b, const float* values, uint32_t n_bricks,
                                                     uint32_t* tcount) {
     __shared__ uint32_t tmp[4];
     uint32_t brick = blockIdx.x;
     if (brick >= n_bricks) return;
-    const float* val = values + (size_t)brick * P.spb;
-    uint32_t b = P.b, ncell = b * b * b, per = (ncell + blockDim.x - 1) / blockDim.x;
+    const uint32_t side = b + 1;
+    const float* val = values + (size_t)brick * side * side * side;
+    uint32_t ncell = b * b * b, per = (ncell + blockDim.x - 1) / blockDim.x;
     uint32_t mine = 0;
     for (uint32_t c = threadIdx.x * per; c < min(ncell, (threadIdx.x + 1) * per); c++) {
         uint32_t x = c % b, y = (c / b) % b, z = c / (b * b);
-        uint32_t cs = cell_case(val, P.side, x, y, z);
+        uint32_t cs = cell_case(val, side, x, y, z);
         mine += c_tri_offset[cs + 1] - c_tri_offset[cs];
     }
     uint32_t total;
@@ -48,8 +49,12 @@ __device__ inline int64_t dense_index(const DRun& P, int64_t kx, int64_t ky, int
 }
 
 // Emits the triangles of every brick at tbase[brick] (mesh/extract.rs:245-260
-// and :629-667). A reference that does not resolve sets *err.
-__global__ void __launch_bounds__(256) k_cell_emit(DRun P, const float* values, const int32_t* keys,
+// and :629-667), with vertex indices local to the brick's molecule (its
+// first vertex is that of its first brick, fbrick[m]). Neighbour bricks
+// are found in the molecule's dense index grid. A reference that does not
+// resolve sets *err.
+__global__ void __launch_bounds__(256) k_cell_emit(const Mol* __restrict__ mols, const uint32_t* bmol,
+                                                   const uint32_t* fbrick, const float* values, const int32_t* keys,
                                                    uint32_t n_bricks, const uint32_t* tbase,
                                                    const uint32_t* dense, const uint32_t* vbase,
                                                    const unsigned long long* mask, const uint32_t* wpre,
@@ -57,6 +62,10 @@ __global__ void __launch_bounds__(256) k_cell_emit(DRun P, const float* values, 
     __shared__ uint32_t tmp[4];
     uint32_t brick = blockIdx.x;
     if (brick >= n_bricks) return;
+    const uint32_t m = bmol[brick];
+    const DRun& P = mols[m].P;
+    const uint32_t* mdense = dense + mols[m].dense_off;
+    const uint32_t vfirst = vbase[fbrick[m]];
     const float* val = values + (size_t)brick * P.spb;
     uint32_t b = P.b, ncell = b * b * b, per = (ncell + blockDim.x - 1) / blockDim.x;
     uint32_t c_lo = threadIdx.x * per, c_hi = min(ncell, (threadIdx.x + 1) * per);
@@ -90,14 +99,14 @@ __global__ void __launch_bounds__(256) k_cell_emit(DRun P, const float* values, 
                 if (target[code] == -2) {
                     int64_t di = dense_index(P, key[0] + (code & 1), key[1] + ((code >> 1) & 1),
                                              key[2] + ((code >> 2) & 1));
-                    target[code] = (di < 0 || dense[di] == 0xffffffffu) ? -1 : (int64_t)dense[di];
+                    target[code] = (di < 0 || mdense[di] == 0xffffffffu) ? -1 : (int64_t)mdense[di];
                 }
                 int64_t tb = target[code];
                 uint32_t vi = 0;
                 if (tb < 0 || !edge_is_crossing(mask, words, (uint32_t)tb, id)) {
                     atomicOr(err, 1u);
                 } else {
-                    vi = vbase[tb] + edge_rank(mask, wpre, words, (uint32_t)tb, id);
+                    vi = vbase[tb] + edge_rank(mask, wpre, words, (uint32_t)tb, id) - vfirst;
                 }
                 tris[3 * (size_t)at + q] = vi;
             }

@@ -70,7 +70,7 @@ struct CandList {
 // free (the minimum over free candidates below the cutoff, as the
 // sequential loop finds it). Falls back to each lane scanning its own
 // candidates when the list outgrows the LDS.
-__device__ inline double ses_distance_wave(const DArr& A, double x, double y, double z, double cutoff) {
+__device__ __forceinline__ double ses_distance_wave(const DArr& A, double x, double y, double z, double cutoff) {
     __shared__ CandList L;
     const uint32_t lane = lane_id();
     if (lane == 0) L.n = 0;
@@ -149,7 +149,7 @@ __device__ inline double ses_distance_wave(const DArr& A, double x, double y, do
         __syncthreads();
         if (kind == 2) return bd;
         double qx, qy, qz;
-        const uint32_t *row, *end;
+        gptr<const uint32_t> row, end;
         if (kind == 0) {
             double vx = x - A.cx[id], vy = y - A.cy[id], vz = z - A.cz[id];
             double len = sqrt(vx * vx + vy * vy + vz * vz);
@@ -183,7 +183,7 @@ __device__ inline double ses_distance_wave(const DArr& A, double x, double y, do
             end = A.cb_items + A.cb_off[id + 1];
         }
         bool blocked = false;
-        for (const uint32_t* p = row + lane; p < end; p += WAVE) {
+        for (gptr<const uint32_t> p = row + lane; p < end; p += WAVE) {
             uint32_t k = *p;
             blocked |= !(ddist(qx, qy, qz, A.cx[k], A.cy[k], A.cz[k]) >= A.R[k] - EPS);
         }
@@ -193,7 +193,7 @@ __device__ inline double ses_distance_wave(const DArr& A, double x, double y, do
 
 // lattice::uniform_sign (SES), one wavefront (the workgroup). Returns
 // ST_NONE/OUT/IN in every lane.
-__device__ inline uint8_t uniform_sign_wave(const DArr& A, double x, double y, double z, double rho) {
+__device__ __forceinline__ uint8_t uniform_sign_wave(const DArr& A, double x, double y, double z, double rho) {
     const uint32_t lane = lane_id();
     double margin = rho + CLASSIFY_EPS;
     double s_sas = INFINITY, s_vdw = INFINITY;
@@ -226,41 +226,34 @@ __device__ inline uint8_t uniform_sign_wave(const DArr& A, double x, double y, d
 // Lattice::brick_of (lattice.rs:88-90).
 __device__ inline int64_t brick_of(double v, double bh) { return (int64_t)floor(v / bh); }
 
-// Brick key range of every non-contained sphere's ball (lattice.rs:119-132):
-// global min/max per axis, as biased unsigned values for atomics.
-__global__ void k_brick_range(DArr A, double bh, unsigned long long* lohi) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= A.n_spheres || A.status[i] == S_CONTAINED) return;
-    double r = A.R[i] + CLASSIFY_EPS;
-    double c[3] = {A.cx[i], A.cy[i], A.cz[i]};
-    for (int a = 0; a < 3; a++) {
-        int64_t lo = brick_of(c[a] - r, bh), hi = brick_of(c[a] + r, bh);
-        // Keys beyond +-2^40 saturate: the host rejects anything past the key range.
-        lo = lo < -(int64_t(1) << 40) ? -(int64_t(1) << 40) : (lo > (int64_t(1) << 40) ? (int64_t(1) << 40) : lo);
-        hi = hi < -(int64_t(1) << 40) ? -(int64_t(1) << 40) : (hi > (int64_t(1) << 40) ? (int64_t(1) << 40) : hi);
-        atomicMin(&lohi[a], (unsigned long long)(lo + (int64_t(1) << 41)));
-        atomicMax(&lohi[3 + a], (unsigned long long)(hi + (int64_t(1) << 41)));
-    }
-}
-
-// Marks the touched bricks of every non-contained sphere in a dense bitmap
-// over the brick box (lattice.rs:119-139); set bits in ascending linear
-// order are the sorted packed keys.
-__global__ void k_mark_touched(DArr A, DRun P, double bh, uint32_t* bits) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= A.n_spheres || A.status[i] == S_CONTAINED) return;
-    double r = A.R[i] + CLASSIFY_EPS;
-    double c[3] = {A.cx[i], A.cy[i], A.cz[i]};
+// Marks the touched bricks of every non-contained sphere (lattice.rs:
+// 119-139) in its molecule's dense bitmap over a box holding all of them:
+// set bits in ascending linear order are the sorted packed keys. One
+// thread per sphere of the run (sph_off: first sphere of each molecule).
+__global__ void k_mark_touched(const Mol* __restrict__ mols, const uint64_t* sph_off, uint32_t nm,
+                               uint32_t* bits) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= sph_off[nm]) return;
+    uint32_t m = find_mol(sph_off, nm, i);
+    const Mol& M = mols[m];
+    const DArr& A = M.A;
+    const DRun& P = M.P;
+    uint32_t k = (uint32_t)(i - sph_off[m]);
+    if (A.status[k] == S_CONTAINED) return;
+    double bh = (double)P.b * P.h;
+    double r = A.R[k] + CLASSIFY_EPS;
+    double c[3] = {A.cx[k], A.cy[k], A.cz[k]};
     int64_t lo[3], hi[3];
     for (int a = 0; a < 3; a++) {
         lo[a] = brick_of(c[a] - r, bh) - P.blo[a];
         hi[a] = brick_of(c[a] + r, bh) - P.blo[a];
     }
+    uint32_t* mb = bits + M.word_off;
     for (int64_t z = lo[2]; z <= hi[2]; z++)
         for (int64_t y = lo[1]; y <= hi[1]; y++)
             for (int64_t x = lo[0]; x <= hi[0]; x++) {
                 uint64_t lin = ((uint64_t)z * P.bdim[1] + (uint64_t)y) * P.bdim[0] + (uint64_t)x;
-                atomicOr(&bits[lin >> 5], 1u << (lin & 31));
+                atomicOr(&mb[lin >> 5], 1u << (lin & 31));
             }
 }
 
@@ -278,25 +271,33 @@ __device__ inline void brick_center(const DRun& P, const int32_t key[3], double 
 }
 
 // Classification of touched bricks (lattice.rs:168-183): one wavefront per
-// brick; keep[t] = 1 where the surface may pass.
-__global__ void k_classify_bricks(DArr A, DRun P, const uint32_t* touched_lin, uint32_t n, uint32_t* keep) {
+// brick; keep[t] = 1 where the surface may pass. touched[t] is a bit index
+// of the run's concatenated bitmaps (wofs: first word of each molecule's).
+__global__ void k_classify_bricks(const Mol* __restrict__ mols, const uint64_t* wofs, uint32_t nm,
+                                  const uint32_t* touched, uint32_t n, uint32_t* keep) {
     uint32_t w = blockIdx.x;  // one wavefront per workgroup
     if (w >= n) return;
+    uint32_t bit = touched[w];
+    uint32_t m = find_mol(wofs, nm, (uint64_t)(bit >> 5));
+    const Mol& M = mols[m];
     int32_t key[3];
-    key_of_lin(P, touched_lin[w], key);
+    key_of_lin(M.P, bit - 32 * wofs[m], key);
     double c[3];
-    brick_center(P, key, c);
-    double rho = 0.5 * sqrt(3.0) * (double)P.b * P.h;
-    uint8_t s = uniform_sign_wave(A, c[0], c[1], c[2], rho);
+    brick_center(M.P, key, c);
+    double rho = 0.5 * sqrt(3.0) * (double)M.P.b * M.P.h;
+    uint8_t s = uniform_sign_wave(M.A, c[0], c[1], c[2], rho);
     if (lane_id() == 0) keep[w] = s == ST_NONE ? 1u : 0u;
 }
 
 // Sub-block states of surface bricks (kernel.rs:467-482): one wavefront per
 // (brick, sub-block); st[8 * brick + sb].
-__global__ void k_sub_states(DArr A, DRun P, const int32_t* keys, uint32_t n_bricks, uint8_t* st) {
+__global__ void k_sub_states(const Mol* __restrict__ mols, const uint32_t* bmol, const int32_t* keys,
+                             uint32_t n_bricks, uint8_t* st) {
     uint32_t w = blockIdx.x;  // one wavefront per workgroup
     if (w >= 8 * n_bricks) return;
     uint32_t brick = w >> 3, i = w & 7;
+    const Mol& M = mols[bmol[brick]];
+    const DRun& P = M.P;
     double sub = (double)P.sub, h = P.h;
     double rho = 0.5 * sqrt(3.0) * sub * h;
     double half = sub * 0.5;
@@ -306,7 +307,7 @@ __global__ void k_sub_states(DArr A, DRun P, const int32_t* keys, uint32_t n_bri
         double off = (double)((i >> a) & 1) * sub + half;
         c[a] = base + off * h;
     }
-    uint8_t s = uniform_sign_wave(A, c[0], c[1], c[2], rho);
+    uint8_t s = uniform_sign_wave(M.A, c[0], c[1], c[2], rho);
     if (lane_id() == 0) st[w] = s;
 }
 

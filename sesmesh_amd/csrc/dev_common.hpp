@@ -20,25 +20,35 @@ enum : uint8_t { ST_NONE = 0, ST_OUT = 1, ST_IN = 2 };  // sub-block / brick sta
 enum : uint32_t { S_EXPOSED = 0, S_BURIED = 1, S_CONTAINED = 2 };
 enum : uint32_t { F_SOLVENT = 0, F_BEYOND = 1, F_SPHERE = 2, F_CIRCLE = 3, F_VERTEX = 4 };
 
+// Pointers into device global memory (address space 1). The arrangement's
+// pointers are read from the molecule table in memory; typed like this,
+// loads through them compile to global loads, not flat ones.
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+template <class T>
+inline gptr<T> to_global(T* p) {
+    return (gptr<T>)p;
+}
+
 // A cell grid (reference geom/cells.rs:9-130) in HBM.
 struct DGrid {
-    const uint32_t* starts;
-    const uint32_t* items;
+    gptr<const uint32_t> starts;
+    gptr<const uint32_t> items;
     double ox, oy, oz, cell;
     uint32_t dx, dy, dz, pad;
 };
 
 // The resident f64 arrangement, SoA.
 struct DArr {
-    const double *cx, *cy, *cz, *R, *r;
-    const uint8_t* status;
-    const uint32_t* orig;
-    const uint32_t *nb_off, *nb_items;
-    const double *mx, *my, *mz, *ax, *ay, *az, *e1x, *e1y, *e1z, *crho;
-    const uint32_t *ci, *cj;
-    const uint32_t *cb_off, *cb_items;
-    const double *vx, *vy, *vz;
-    const uint32_t* vatoms;
+    gptr<const double> cx, cy, cz, R, r;
+    gptr<const uint8_t> status;
+    gptr<const uint32_t> orig;
+    gptr<const uint32_t> nb_off, nb_items;
+    gptr<const double> mx, my, mz, ax, ay, az, e1x, e1y, e1z, crho;
+    gptr<const uint32_t> ci, cj;
+    gptr<const uint32_t> cb_off, cb_items;
+    gptr<const double> vx, vy, vz;
+    gptr<const uint32_t> vatoms;
     DGrid gs, gc, gv;
     uint32_t n_spheres, n_circles, n_vertices, pad;
     double probe, max_expanded, rho_max;
@@ -46,9 +56,9 @@ struct DArr {
 
 // Frame-local f32 copies (reference field/gpu.rs:253-318, kernel.rs:604-747).
 struct DLoc {
-    float *px, *py, *pz, *pr, *pr2, *pr2tol;
-    float *mx, *my, *mz, *ux, *uy, *uz, *ex, *ey, *ez, *rho;
-    float *vx, *vy, *vz;
+    gptr<float> px, py, pz, pr, pr2, pr2tol;
+    gptr<float> mx, my, mz, ux, uy, uz, ex, ey, ez, rho;
+    gptr<float> vx, vy, vz;
 };
 
 // Lattice and stage parameters of one mesh run.
@@ -70,8 +80,22 @@ struct Mol {
     DRun P;
     uint64_t word_off;   // first word of its touched-brick bitmap
     uint64_t dense_off;  // first entry of its dense brick-index grid
-    uint32_t first_brick, n_bricks;
 };
+
+// The molecule owning item i of a concatenation: the last m < nm with
+// off[m] <= i (off ascending, off[0] = 0).
+template <class T>
+__device__ inline uint32_t find_mol(const T* off, uint32_t nm, T i) {
+    uint32_t lo = 0, hi = nm - 1;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= i)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
 
 __device__ inline double ddist(double ax, double ay, double az, double bx, double by, double bz) {
     double x = ax - bx, y = ay - by, z = az - bz;
@@ -331,7 +355,7 @@ __device__ inline void gather3(const DArr& A, double x, double y, double z, cons
                 hi = mid - 1;
         }
         uint32_t k = lo < n0 ? 0u : (lo < n0 + n1 ? 1u : 2u);
-        const uint32_t* items = k == 0 ? A.gs.items : (k == 1 ? A.gc.items : A.gv.items);
+        gptr<const uint32_t> items = k == 0 ? A.gs.items : (k == 1 ? A.gc.items : A.gv.items);
         f(k, items[rs[lo] + (i - rp[lo])]);
     }
     __syncthreads();

@@ -42,23 +42,24 @@ __device__ inline bool edge_crossing(const float* val, uint32_t b, uint32_t side
 // Per brick (one workgroup of 256): crossing-edge masks (edge ids
 // ((z*b + y)*b + x)*3 + axis, 64 per word), per-word prefix counts, brick
 // total and per-sub-block counts (mesh/extract.rs:176-216).
-__global__ void __launch_bounds__(256) k_edges(DRun P, const float* values, uint32_t n_bricks, uint32_t words,
+__global__ void __launch_bounds__(256) k_edges(uint32_t b, const float* values, uint32_t n_bricks, uint32_t words,
                                                unsigned long long* mask, uint32_t* wpre, uint32_t* bcount,
                                                uint32_t* scount) {
     __shared__ uint32_t wc[2048];
     __shared__ uint32_t sc[8];
     uint32_t brick = blockIdx.x;
     if (brick >= n_bricks) return;
-    const float* val = values + (size_t)brick * P.spb;
-    uint32_t b = P.b, nid = 3 * b * b * b, lane = lane_id(), wv = threadIdx.x / WAVE;
+    const uint32_t side = b + 1, sub = b / 2;
+    const float* val = values + (size_t)brick * side * side * side;
+    uint32_t nid = 3 * b * b * b, lane = lane_id(), wv = threadIdx.x / WAVE;
     if (threadIdx.x < 8) sc[threadIdx.x] = 0;
     __syncthreads();
     for (uint32_t w = wv; w < words; w += blockDim.x / WAVE) {
         uint32_t id = w * 64 + lane, x = 0, y = 0, z = 0, axis = 0;
-        bool c = id < nid && edge_crossing(val, b, P.side, id, x, y, z, axis);
+        bool c = id < nid && edge_crossing(val, b, side, id, x, y, z, axis);
         unsigned long long m = __ballot(c);
         if (c) {
-            uint32_t s = min(x / P.sub, 1u) + 2 * min(y / P.sub, 1u) + 4 * min(z / P.sub, 1u);
+            uint32_t s = min(x / sub, 1u) + 2 * min(y / sub, 1u) + 4 * min(z / sub, 1u);
             atomicAdd(&sc[s], 1u);
         }
         if (lane == 0) {
@@ -300,7 +301,7 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
             else
                 hi = mid - 1;
         }
-        const uint32_t* items = lo < ns ? A.nb_items : A.cb_items;
+        gptr<const uint32_t> items = lo < ns ? A.nb_items : A.cb_items;
         uint32_t slot = hash_get(s.hk, s.hv, hmask, items[s.coff[lo] + (i - s.cpre[lo])]);
         if (slot != 0xffffffffu) atomicOr(&s.cbits[(slot >> 6) * cc + lo], 1ull << (slot & 63));
     }
@@ -716,8 +717,9 @@ __device__ inline void edge_finish(const DArr& A, const DRun& P, const EdgeState
 // in the sub-block, then refine them by lane groups. Sub-blocks whose lists
 // overflow go to `overflow`.
 template <bool GLOBAL>
-__global__ void __launch_bounds__(256) k_refine(DArr A, DRun P, RCaps cap, RLayout lay, const int32_t* keys,
-                                                const float* values, const uint32_t* work, uint32_t n_work,
+__global__ void __launch_bounds__(256) k_refine(const Mol* __restrict__ mols, const uint32_t* bmol, RCaps cap,
+                                                RLayout lay, const int32_t* keys, const float* values,
+                                                const uint32_t* work, uint32_t n_work,
                                                 const unsigned long long* mask, const uint32_t* wpre,
                                                 uint32_t words, const uint32_t* vbase, VOut out, char* scratch,
                                                 size_t slice, uint32_t* overflow, uint32_t* n_overflow,
@@ -729,6 +731,9 @@ __global__ void __launch_bounds__(256) k_refine(DArr A, DRun P, RCaps cap, RLayo
     uint32_t w = blockIdx.x;
     if (w >= n_work) return;
     uint32_t j = work[w], brick = j >> 3, sb = j & 7;
+    const uint32_t mol = bmol[brick];
+    const DArr& A = mols[mol].A;
+    const DRun& P = mols[mol].P;
     char* base = GLOBAL ? scratch + (size_t)w * slice : smem;
     RLists s = rcarve(base, lay);
     const int32_t* key = keys + 3 * brick;
@@ -814,7 +819,7 @@ __global__ void __launch_bounds__(256) k_refine(DArr A, DRun P, RCaps cap, RLayo
     if (evals_total) {
         if (G.r == 0 && my_evals) atomicAdd(&evals_sum, my_evals);
         __syncthreads();
-        if (threadIdx.x == 0 && evals_sum) atomicAdd(evals_total, evals_sum);
+        if (threadIdx.x == 0 && evals_sum) atomicAdd(&evals_total[mol], evals_sum);
     }
 }
 

@@ -78,9 +78,11 @@ __device__ inline uint32_t pack_states(const uint8_t* st) {
 }
 
 // Samples of brick `brick` owned by no sub-block get +-band (kernel.rs:533-534).
-__global__ void k_assign_uniform(DRun P, const uint8_t* st, uint32_t n_bricks, float* values) {
+__global__ void k_assign_uniform(const Mol* __restrict__ mols, const uint32_t* bmol, const uint8_t* st,
+                                 uint32_t n_bricks, float* values) {
     uint32_t brick = blockIdx.x;
     if (brick >= n_bricks) return;
+    const DRun& P = mols[bmol[brick]].P;
     uint32_t s = pack_states(st + 8 * brick);
     float* out = values + (size_t)brick * P.spb;
     for (uint32_t i = threadIdx.x; i < P.spb; i += blockDim.x) {
@@ -343,7 +345,7 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
             else
                 hi = mid - 1;
         }
-        const uint32_t* items = lo < ns ? A.nb_items : A.cb_items;
+        gptr<const uint32_t> items = lo < ns ? A.nb_items : A.cb_items;
         uint32_t slot = hash_get(s.hk, s.hv, hmask, items[s.coff[lo] + (i - s.cpre[lo])]);
         if (slot != 0xffffffffu) atomicOr(&s.cbits[(slot >> 6) * cc + lo], 1ull << (slot & 63));
     }
@@ -356,10 +358,10 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
 // scratch slice w of `slice` bytes (GLOBAL true). Sub-blocks whose lists do
 // not fit are appended to `overflow`.
 template <bool GLOBAL>
-__global__ void __launch_bounds__(256) k_sample(DArr A, DLoc Lc, DRun P, Caps cap, Layout lay,
-                                                const int32_t* keys, const uint8_t* st, const uint32_t* work,
-                                                uint32_t n_work, float* values, char* scratch, size_t slice,
-                                                uint32_t* overflow, uint32_t* n_overflow,
+__global__ void __launch_bounds__(256) k_sample(const Mol* __restrict__ mols, const uint32_t* bmol, Caps cap,
+                                                Layout lay, const int32_t* keys, const uint8_t* st,
+                                                const uint32_t* work, uint32_t n_work, float* values, char* scratch,
+                                                size_t slice, uint32_t* overflow, uint32_t* n_overflow,
                                                 unsigned long long* n_samples) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     __shared__ uint32_t n[C_N];
@@ -369,6 +371,10 @@ __global__ void __launch_bounds__(256) k_sample(DArr A, DLoc Lc, DRun P, Caps ca
     uint32_t w = blockIdx.x;
     if (w >= n_work) return;
     uint32_t j = work[w], brick = j >> 3, sb = j & 7;
+    const uint32_t m = bmol[brick];
+    const DArr& A = mols[m].A;
+    const DLoc& Lc = mols[m].L;
+    const DRun& P = mols[m].P;
     char* base = GLOBAL ? scratch + (size_t)w * slice : smem;
     Lists s = carve(base, lay);
     const int32_t* key = keys + 3 * brick;
@@ -395,8 +401,8 @@ __global__ void __launch_bounds__(256) k_sample(DArr A, DLoc Lc, DRun P, Caps ca
             if (sample_owner(stb, sub, lx, ly, lz, positive) == (int)sb) owned[atomicAdd(&n_owned, 1u)] = t;
         }
         __syncthreads();
-        uint32_t m = n_owned;
-        for (uint32_t q = threadIdx.x; q < m; q += blockDim.x) {
+        uint32_t cnt = n_owned;
+        for (uint32_t q = threadIdx.x; q < cnt; q += blockDim.x) {
             uint32_t t = owned[q];
             uint32_t lx = bx + t % box, ly = by + (t / box) % box, lz = bz + t / (box * box);
             // Lattice::local (lattice.rs:69-71): integer offset, then one f32 multiply.
@@ -405,7 +411,7 @@ __global__ void __launch_bounds__(256) k_sample(DArr A, DLoc Lc, DRun P, Caps ca
             float z = (float)(int32_t)(org[2] + lz) * P.h32;
             out[lx + side * (ly + side * lz)] = ses_value(s, n, cc, nw, x, y, z, P.fprobe, P.fcutoff2);
         }
-        if (threadIdx.x == 0 && n_samples) atomicAdd(n_samples, (unsigned long long)m);
+        if (threadIdx.x == 0 && n_samples) atomicAdd(&n_samples[m], (unsigned long long)cnt);
         __syncthreads();
         if (threadIdx.x == 0) n_owned = 0;
         __syncthreads();

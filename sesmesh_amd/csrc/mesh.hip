@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -107,18 +108,47 @@ __global__ void k_bits_to_list(const uint32_t* bits, const uint32_t* off, size_t
     }
 }
 
-// Surface bricks: keys in order, and the dense index grid.
-__global__ void k_compact_bricks(DRun P, const uint32_t* touched, const uint32_t* keep, const uint32_t* off,
-                                 uint32_t n, int32_t* keys, uint32_t* dense) {
+// Surface bricks: keys (molecule lattice) in order, their molecules, and
+// each molecule's dense index grid.
+__global__ void k_compact_bricks(const Mol* __restrict__ mols, const uint64_t* wofs, uint32_t nm,
+                                 const uint32_t* touched, const uint32_t* keep, const uint32_t* off, uint32_t n,
+                                 int32_t* keys, uint32_t* bmol, uint32_t* dense) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || !keep[i]) return;
-    uint32_t at = off[i];
+    uint32_t at = off[i], bit = touched[i];
+    uint32_t m = find_mol(wofs, nm, (uint64_t)(bit >> 5));
+    uint64_t lin = bit - 32 * wofs[m];
     int32_t key[3];
-    key_of_lin(P, touched[i], key);
+    key_of_lin(mols[m].P, lin, key);
     keys[3 * at] = key[0];
     keys[3 * at + 1] = key[1];
     keys[3 * at + 2] = key[2];
-    dense[touched[i]] = at;
+    bmol[at] = m;
+    dense[mols[m].dense_off + lin] = at;
+}
+
+// First touched brick and first surface brick of every molecule
+// (rows[m] for m <= nm; row nm holds the totals).
+__global__ void k_mol_ranges(const uint64_t* wofs, uint32_t nm, const uint32_t* woff, uint64_t nwords,
+                             uint32_t ntouched, const uint32_t* koff, uint32_t nb, uint32_t* tfirst,
+                             uint32_t* fbrick) {
+    uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m > nm) return;
+    uint32_t tf = (m < nm && wofs[m] < nwords) ? woff[wofs[m]] : ntouched;
+    tfirst[m] = tf;
+    fbrick[m] = tf < ntouched ? koff[tf] : nb;
+}
+
+// Where each molecule's sub-blocks to sample, vertices and triangles start.
+__global__ void k_mol_summary(uint32_t nm, const uint32_t* fbrick, uint32_t nb, const uint32_t* soff, uint32_t nsw,
+                              const uint32_t* vbase, uint32_t nverts, const uint32_t* tbase, uint32_t ntris,
+                              uint32_t* sfirst, uint32_t* vfirst, uint32_t* trfirst) {
+    uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m > nm) return;
+    uint32_t fb = fbrick[m];
+    sfirst[m] = fb < nb ? soff[8 * fb] : nsw;
+    vfirst[m] = fb < nb ? vbase[fb] : nverts;
+    trfirst[m] = fb < nb ? tbase[fb] : ntris;
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
@@ -140,9 +170,18 @@ __global__ void k_compact_index(const uint32_t* flag, const uint32_t* off, uint3
     if (i < n && flag[i]) out[off[i]] = i;
 }
 
-// Frame-local f32 copies (field/gpu.rs:253-318, kernel.rs:627-637, 717-746).
-__global__ void k_localize(DArr A, DLoc L, double fx, double fy, double fz) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+// Frame-local f32 copies (field/gpu.rs:253-318, kernel.rs:627-637, 717-746);
+// one thread per index below max(spheres, circles, vertices) of each
+// molecule (loc_off: first index of each molecule).
+__global__ void k_localize(const Mol* __restrict__ mols, const uint64_t* loc_off, uint32_t nm) {
+    uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= loc_off[nm]) return;
+    uint32_t m = find_mol(loc_off, nm, g);
+    const DArr& A = mols[m].A;
+    const DLoc& L = mols[m].L;
+    const DRun& P = mols[m].P;
+    const double fx = (double)P.frame[0] * P.h, fy = (double)P.frame[1] * P.h, fz = (double)P.frame[2] * P.h;
+    uint32_t i = (uint32_t)(g - loc_off[m]);
     if (i < A.n_spheres) {
         L.px[i] = (float)(A.cx[i] - fx);
         L.py[i] = (float)(A.cy[i] - fy);
@@ -231,12 +270,14 @@ struct smg_resident {
     size_t n_spheres = 0;
 };
 
+// The meshes of one run: one part per molecule, concatenated in the
+// device arrays; part i owns bricks [bfirst[i], bfirst[i+1]), vertices
+// [vfirst[i], vfirst[i+1]) and triangles [tfirst[i], tfirst[i+1]).
 struct smg_mesh {
     smg_device* dev = nullptr;
-    DRun P{};
-    size_t touched = 0, bricks = 0, vertices = 0, triangles = 0, sampled = 0, overflow = 0;
-    uint64_t evals = 0, samples = 0;
-    uint32_t words = 0;
+    uint32_t spb = 0;  // samples per brick
+    std::vector<smg_mesh_info> info;
+    std::vector<uint32_t> bfirst, vfirst, tfirst;
     // Device arrays owned by the mesh.
     int32_t* keys = nullptr;
     float* values = nullptr;
@@ -477,51 +518,45 @@ extern "C" int smg_arrangement_view(const smg_arrangement* p, smg_arrangement_re
 }
 
 // ---------------------------------------------------------------- resident
-template <class T>
-static int upload(DevBuf& b, const T* src, size_t count, const T** dst) {
-    void* p = nullptr;
-    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-    HIP_TRY(hipMalloc(&p, bytes));
-    b.ptrs.push_back(p);
-    b.bytes += bytes;
-    if (count) HIP_TRY(hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice));
-    *dst = (const T*)p;
-    return SMG_OK;
-}
-
-#define UP(src, count, dst)                                   \
-    do {                                                      \
-        int rc_ = upload(r->buf, (src), (count), &(dst));     \
-        if (rc_ != SMG_OK) {                                  \
-            smg_resident_free(r);                             \
-            return rc_;                                       \
-        }                                                     \
-    } while (0)
-
-static int upload_grid(smg_resident* r, const smg_grid_ref& g, DGrid& d) {
-    size_t cells = (size_t)g.dims[0] * g.dims[1] * g.dims[2];
-    int rc = upload(r->buf, g.starts, cells + 1, &d.starts);
-    if (rc == SMG_OK) rc = upload(r->buf, g.items, g.n_items, &d.items);
-    d.ox = g.origin[0];
-    d.oy = g.origin[1];
-    d.oz = g.origin[2];
-    d.cell = g.cell;
-    d.dx = g.dims[0];
-    d.dy = g.dims[1];
-    d.dz = g.dims[2];
-    return rc;
-}
+namespace {
+// Packs the arrangement's arrays into one host buffer (256-byte aligned
+// pieces) so a resident costs one allocation and one copy.
+struct Packer {
+    std::vector<char> bytes;
+    std::vector<std::pair<size_t, void*>> fix;  // offset, address of the pointer field
+    template <class T>
+    void add(const T* src, size_t count, gptr<const T>* field) {
+        size_t at = (bytes.size() + 255) & ~size_t(255);
+        bytes.resize(at + std::max<size_t>(count, 1) * sizeof(T));
+        if (count) std::memcpy(bytes.data() + at, src, count * sizeof(T));
+        fix.push_back({at, (void*)field});
+    }
+    void grid(const smg_grid_ref& g, DGrid& d) {
+        size_t cells = (size_t)g.dims[0] * g.dims[1] * g.dims[2];
+        add(g.starts, cells + 1, &d.starts);
+        add(g.items, g.n_items, &d.items);
+        d.ox = g.origin[0];
+        d.oy = g.origin[1];
+        d.oz = g.origin[2];
+        d.cell = g.cell;
+        d.dx = g.dims[0];
+        d.dy = g.dims[1];
+        d.dz = g.dims[2];
+    }
+};
+}  // namespace
 
 extern "C" int smg_resident_new(smg_device* dev, const smg_arrangement_ref* a, smg_resident** out) {
+    if (!out) return fail(SMG_E_INVALID, "null output");
     *out = nullptr;
     if (!dev || !a) return fail(SMG_E_INVALID, "null device or arrangement");
     if (a->n_spheres >= (1u << 30) || a->n_circles >= (1u << 30) || a->n_vertices >= (1u << 30))
         return fail(SMG_E_LENGTH, "arrangement too large for 30-bit candidate indices");
     if (!(a->probe > 0.0)) return fail(SMG_E_UNSUPPORTED, "the device path computes the SES (probe > 0)");
     HIP_TRY(hipSetDevice(dev->ordinal));
-    smg_resident* r = new smg_resident();
-    r->dev = dev;
     size_t n = a->n_spheres, nc = a->n_circles, nv = a->n_vertices;
+    std::unique_ptr<smg_resident> r(new smg_resident());
+    r->dev = dev;
     r->n_spheres = n;
     std::vector<double> x(n), y(n), z(n);
     for (size_t i = 0; i < n; i++) {
@@ -537,16 +572,6 @@ extern "C" int smg_resident_new(smg_device* dev, const smg_arrangement_ref* a, s
                 r->hi[q] = std::fmax(r->hi[q], a->center[3 * i + q]);
             }
     }
-    DArr& A = r->A;
-    UP(x.data(), n, A.cx);
-    UP(y.data(), n, A.cy);
-    UP(z.data(), n, A.cz);
-    UP(a->expanded, n, A.R);
-    UP(a->radius, n, A.r);
-    UP(a->status, n, A.status);
-    UP(a->orig, n, A.orig);
-    UP(a->nb_off, n + 1, A.nb_off);
-    UP(a->nb_items, (size_t)a->nb_off[n], A.nb_items);
     std::vector<double> cf[10];
     std::vector<uint32_t> ci(nc), cj(nc);
     for (int q = 0; q < 10; q++) cf[q].resize(nc);
@@ -555,34 +580,42 @@ extern "C" int smg_resident_new(smg_device* dev, const smg_arrangement_ref* a, s
         ci[c] = a->circle_ij[2 * c];
         cj[c] = a->circle_ij[2 * c + 1];
     }
-    UP(cf[0].data(), nc, A.mx);
-    UP(cf[1].data(), nc, A.my);
-    UP(cf[2].data(), nc, A.mz);
-    UP(cf[3].data(), nc, A.ax);
-    UP(cf[4].data(), nc, A.ay);
-    UP(cf[5].data(), nc, A.az);
-    UP(cf[6].data(), nc, A.e1x);
-    UP(cf[7].data(), nc, A.e1y);
-    UP(cf[8].data(), nc, A.e1z);
-    UP(cf[9].data(), nc, A.crho);
-    UP(ci.data(), nc, A.ci);
-    UP(cj.data(), nc, A.cj);
-    UP(a->cb_off, nc + 1, A.cb_off);
-    UP(a->cb_items, (size_t)a->cb_off[nc], A.cb_items);
     std::vector<double> vxyz[3];
     for (int q = 0; q < 3; q++) vxyz[q].resize(nv);
     for (size_t v = 0; v < nv; v++)
         for (int q = 0; q < 3; q++) vxyz[q][v] = a->vertex[3 * v + q];
-    UP(vxyz[0].data(), nv, A.vx);
-    UP(vxyz[1].data(), nv, A.vy);
-    UP(vxyz[2].data(), nv, A.vz);
-    UP(a->vertex_atoms, 3 * nv, A.vatoms);
-    int rc = upload_grid(r, a->sphere_grid, A.gs);
-    if (rc == SMG_OK) rc = upload_grid(r, a->circle_grid, A.gc);
-    if (rc == SMG_OK) rc = upload_grid(r, a->vertex_grid, A.gv);
-    if (rc != SMG_OK) {
-        smg_resident_free(r);
-        return rc;
+    DArr& A = r->A;
+    Packer pk;
+    pk.add(x.data(), n, &A.cx);
+    pk.add(y.data(), n, &A.cy);
+    pk.add(z.data(), n, &A.cz);
+    pk.add(a->expanded, n, &A.R);
+    pk.add(a->radius, n, &A.r);
+    pk.add(a->status, n, &A.status);
+    pk.add(a->orig, n, &A.orig);
+    pk.add(a->nb_off, n + 1, &A.nb_off);
+    pk.add(a->nb_items, (size_t)a->nb_off[n], &A.nb_items);
+    gptr<const double>* cfield[10] = {&A.mx, &A.my, &A.mz, &A.ax, &A.ay, &A.az, &A.e1x, &A.e1y, &A.e1z, &A.crho};
+    for (int q = 0; q < 10; q++) pk.add(cf[q].data(), nc, cfield[q]);
+    pk.add(ci.data(), nc, &A.ci);
+    pk.add(cj.data(), nc, &A.cj);
+    pk.add(a->cb_off, nc + 1, &A.cb_off);
+    pk.add(a->cb_items, (size_t)a->cb_off[nc], &A.cb_items);
+    pk.add(vxyz[0].data(), nv, &A.vx);
+    pk.add(vxyz[1].data(), nv, &A.vy);
+    pk.add(vxyz[2].data(), nv, &A.vz);
+    pk.add(a->vertex_atoms, 3 * nv, &A.vatoms);
+    pk.grid(a->sphere_grid, A.gs);
+    pk.grid(a->circle_grid, A.gc);
+    pk.grid(a->vertex_grid, A.gv);
+    void* p = nullptr;
+    HIP_TRY(hipMalloc(&p, pk.bytes.size()));
+    r->buf.ptrs.push_back(p);
+    r->buf.bytes = pk.bytes.size();
+    HIP_TRY(hipMemcpy(p, pk.bytes.data(), pk.bytes.size(), hipMemcpyHostToDevice));
+    for (auto& f : pk.fix) {
+        uintptr_t v = (uintptr_t)p + f.first;
+        std::memcpy(f.second, &v, sizeof(v));
     }
     A.n_spheres = (uint32_t)n;
     A.n_circles = (uint32_t)nc;
@@ -590,7 +623,7 @@ extern "C" int smg_resident_new(smg_device* dev, const smg_arrangement_ref* a, s
     A.probe = a->probe;
     A.max_expanded = a->max_expanded;
     A.rho_max = a->rho_max;
-    *out = r;
+    *out = r.release();
     return SMG_OK;
 }
 
@@ -622,8 +655,8 @@ struct Temps {
     }
 };
 
-// Exclusive scan of n u32 values into `out`; the total is written to
-// host[slot] after a synchronisation by the caller.
+// Exclusive scan of n u32 values into `out`; *total_dev receives the device
+// address of the total.
 int scan_u32(hipStream_t s, Temps& T, const uint32_t* in, uint32_t* out, size_t n, uint32_t** total_dev) {
     size_t tiles = std::max<size_t>((n + SCAN_TILE - 1) / SCAN_TILE, 1);
     uint32_t* sums;
@@ -664,18 +697,16 @@ constexpr uint32_t LDS_EDGES = 512;      // crossing-edge buffer of the LDS pass
 constexpr uint32_t SCRATCH_CHUNK = 256;   // workgroups per scratch launch
 constexpr uint32_t STATIC_LDS = 6144;     // static LDS of k_sample / k_refine, bound
 
-}  // namespace
-
-static int mesh_stage(smg_resident* r, const smg_mesh_params* prm, smg_mesh* M, smg_mesh_times* times) {
-    smg_device* d = r->dev;
-    hipStream_t s = d->stream;
+// Engine::new's lattice for one molecule (field/kernel.rs:204-261) and a
+// brick box holding every touched brick: the centres' bounding box grown by
+// the largest expanded radius, with the operations k_mark_touched uses, so
+// the box bounds hold for each sphere (rounding is monotonic). Lattice
+// order inside any such box is the packed-key order.
+int molecule_run(const smg_resident* r, const smg_mesh_params* prm, uint32_t b, DRun& P, bool& empty) {
     const DArr& A = r->A;
-    HIP_TRY(hipSetDevice(d->ordinal));
-    // Engine::new (field/kernel.rs:204-261).
     double spacing = prm->spacing;
-    uint32_t b = prm->brick;
     if (!(std::isfinite(spacing) && spacing >= 0.01)) return fail(SMG_E_INVALID, "grid spacing must be at least 0.01 A");
-    if (b < 4 || b > 32 || b % 2) return fail(SMG_E_UNSUPPORTED, "the device path needs an even brick size in 4..=32");
+    if (prm->brick != b) return fail(SMG_E_UNSUPPORTED, "one brick size per run");
     if (prm->refine != SMG_REFINE_EXACT) return fail(SMG_E_UNSUPPORTED, "only exact refinement is implemented");
     double center[3] = {0, 0, 0}, half = 0.0;
     if (r->n_spheres) {
@@ -686,7 +717,7 @@ static int mesh_stage(smg_resident* r, const smg_mesh_params* prm, smg_mesh* M, 
         }
         half = 0.5 * std::fmax(std::fmax(dd[0], dd[1]), dd[2]);
     }
-    DRun& P = M->P;
+    P = DRun{};
     P.h = spacing;
     P.b = b;
     P.sub = b / 2;
@@ -707,98 +738,142 @@ static int mesh_stage(smg_resident* r, const smg_mesh_params* prm, smg_mesh* M, 
                        A.max_expanded + A.probe + 2.0 * P.band;
     {  // mesh/extract.rs:47-51
         double cm = std::fmin(std::fmax(coord_max, 1.0), (double)3.4028234663852886e38 / 4.0);
-        float m = (float)cm;
+        float mf = (float)cm;
         uint32_t bits;
-        std::memcpy(&bits, &m, 4);
+        std::memcpy(&bits, &mf, 4);
         bits += 1;
         float nx;
         std::memcpy(&nx, &bits, 4);
-        P.t_min = std::fmin(std::fmax(4.0 * (double)(nx - m) / spacing, 1e-4), 0.25);
+        P.t_min = std::fmin(std::fmax(4.0 * (double)(nx - mf) / spacing, 1e-4), 0.25);
     }
+    empty = r->n_spheres == 0;
+    if (empty) return SMG_OK;
+    double bh = (double)b * spacing, rr = A.max_expanded + CLASSIFY_EPS;
+    uint64_t cells = 1;
+    for (int q = 0; q < 3; q++) {
+        double flo = std::floor((r->lo[q] - rr) / bh), fhi = std::floor((r->hi[q] + rr) / bh);
+        if (!(std::fabs(flo) < (double)(KEY_BIAS - 1) && std::fabs(fhi) < (double)(KEY_BIAS - 1)))
+            return fail(SMG_E_INVALID, "coordinates too far from the origin for grid spacing " + std::to_string(spacing) + " A");
+        P.blo[q] = (int64_t)flo;
+        P.bdim[q] = (uint32_t)((int64_t)fhi - (int64_t)flo + 1);
+        cells *= P.bdim[q];
+    }
+    if (cells >= (1ull << 31)) return fail(SMG_E_UNSUPPORTED, "brick box too large for the dense touched-brick map");
+    return SMG_OK;
+}
+
+}  // namespace
+
+// The mesh stage of `nm` molecules in one pass: every kernel runs once over
+// the concatenated work of all molecules, each work item finding its
+// molecule's arrangement, f32 copies and lattice in the molecule table.
+static int mesh_many(smg_resident* const* R, const smg_mesh_params* prm, size_t nm, smg_mesh* M,
+                     smg_mesh_times* times) {
+    if (nm == 0) return fail(SMG_E_INVALID, "no molecules");
+    if (nm >= (1u << 24)) return fail(SMG_E_LENGTH, "too many molecules in one run");
+    smg_device* d = R[0]->dev;
+    for (size_t m = 0; m < nm; m++)
+        if (!R[m] || R[m]->dev != d) return fail(SMG_E_INVALID, "all molecules of a run must be resident on one device");
+    hipStream_t s = d->stream;
+    HIP_TRY(hipSetDevice(d->ordinal));
+    const uint32_t b = prm[0].brick;
+    if (b < 4 || b > 32 || b % 2) return fail(SMG_E_UNSUPPORTED, "the device path needs an even brick size in 4..=32");
+    const uint32_t spb = (b + 1) * (b + 1) * (b + 1);
+    M->spb = spb;
+    // Molecule table, offsets of the concatenated per-molecule arrays.
+    std::vector<Mol> H(nm);
+    std::vector<uint64_t> sph_off(nm + 1, 0), loc_off(nm + 1, 0), wofs(nm + 1, 0);
+    uint64_t dense_total = 0;
+    for (size_t m = 0; m < nm; m++) {
+        Mol& X = H[m];
+        X.A = R[m]->A;
+        bool empty = false;
+        TRY(molecule_run(R[m], &prm[m], b, X.P, empty));
+        uint64_t cells = empty ? 0 : (uint64_t)X.P.bdim[0] * X.P.bdim[1] * X.P.bdim[2];
+        uint64_t nmax = std::max<uint64_t>({X.A.n_spheres, X.A.n_circles, X.A.n_vertices});
+        sph_off[m + 1] = sph_off[m] + X.A.n_spheres;
+        loc_off[m + 1] = loc_off[m] + nmax;
+        wofs[m + 1] = wofs[m] + (cells + 31) / 32;
+        X.word_off = wofs[m];
+        X.dense_off = dense_total;
+        dense_total += cells;
+    }
+    const uint64_t nwords = wofs[nm];
+    if (nwords >= (1ull << 27)) return fail(SMG_E_UNSUPPORTED, "brick boxes of the run too large for 32-bit bit indices");
     Temps T(s);
     hipEvent_t* ev = d->ev;
     HIP_TRY(hipEventRecord(ev[0], s));
-
-    // Frame-local f32 copies.
-    DLoc L{};
-    size_t nmax = std::max<size_t>({A.n_spheres, A.n_circles, A.n_vertices, 1});
     float* lbuf;
-    HIP_TRY(T.get(&lbuf, 19 * nmax));
-    float** fields[19] = {&L.px, &L.py, &L.pz, &L.pr, &L.pr2, &L.pr2tol, &L.mx, &L.my, &L.mz, &L.ux,
-                          &L.uy, &L.uz, &L.ex, &L.ey, &L.ez, &L.rho,   &L.vx, &L.vy, &L.vz};
-    for (int q = 0; q < 19; q++) *fields[q] = lbuf + q * nmax;
-    double fo[3] = {(double)P.frame[0] * spacing, (double)P.frame[1] * spacing, (double)P.frame[2] * spacing};
-    k_localize<<<blocks_for(nmax, 256), 256, 0, s>>>(A, L, fo[0], fo[1], fo[2]);
+    HIP_TRY(T.get(&lbuf, 19 * loc_off[nm]));
+    for (size_t m = 0; m < nm; m++) {
+        DLoc& L = H[m].L;
+        size_t nmax = loc_off[m + 1] - loc_off[m];
+        float* base = lbuf + 19 * loc_off[m];
+        gptr<float>* fields[19] = {&L.px, &L.py, &L.pz, &L.pr, &L.pr2, &L.pr2tol, &L.mx, &L.my, &L.mz, &L.ux,
+                                   &L.uy, &L.uz, &L.ex, &L.ey, &L.ez, &L.rho,   &L.vx, &L.vy, &L.vz};
+        for (int q = 0; q < 19; q++) *fields[q] = to_global(base + q * nmax);
+    }
+    Mol* mols;
+    uint64_t *sph_d, *loc_d, *wofs_d;
+    HIP_TRY(T.get(&mols, nm));
+    HIP_TRY(T.get(&sph_d, nm + 1));
+    HIP_TRY(T.get(&loc_d, nm + 1));
+    HIP_TRY(T.get(&wofs_d, nm + 1));
+    HIP_TRY(hipMemcpyAsync(mols, H.data(), nm * sizeof(Mol), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(sph_d, sph_off.data(), (nm + 1) * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(loc_d, loc_off.data(), (nm + 1) * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(wofs_d, wofs.data(), (nm + 1) * 8, hipMemcpyHostToDevice, s));
+    const uint32_t nmu = (uint32_t)nm;
+    if (loc_off[nm]) k_localize<<<blocks_for(loc_off[nm], 256), 256, 0, s>>>(mols, loc_d, nmu);
     HIP_TRY(hipGetLastError());
 
-    // Touched bricks (lattice.rs:105-163) in a dense bitmap over their box.
-    double bh = (double)b * spacing;
-    unsigned long long* lohi;
-    HIP_TRY(T.get(&lohi, 6));
-    unsigned long long init[6] = {~0ull, ~0ull, ~0ull, 0ull, 0ull, 0ull};
-    HIP_TRY(hipMemcpyAsync(lohi, init, sizeof(init), hipMemcpyHostToDevice, s));
-    if (A.n_spheres) k_brick_range<<<blocks_for(A.n_spheres, 256), 256, 0, s>>>(A, bh, lohi);
-    unsigned long long hl[6];
-    HIP_TRY(hipMemcpyAsync(hl, lohi, sizeof(hl), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    M->touched = 0;
-    if (hl[0] == ~0ull) {  // no non-contained sphere: no surface
-        M->bricks = M->vertices = M->triangles = 0;
-        return SMG_OK;
-    }
-    uint64_t total_cells = 1;
-    for (int q = 0; q < 3; q++) {
-        int64_t lo = (int64_t)hl[q] - (int64_t(1) << 41), hi = (int64_t)hl[3 + q] - (int64_t(1) << 41);
-        if (std::llabs(lo) >= KEY_BIAS - 1 || std::llabs(hi) >= KEY_BIAS - 1)
-            return fail(SMG_E_INVALID, "coordinates too far from the origin for grid spacing " + std::to_string(spacing) + " A");
-        P.blo[q] = lo;
-        P.bdim[q] = (uint32_t)(hi - lo + 1);
-        total_cells *= P.bdim[q];
-    }
-    if (total_cells >= (1ull << 32))
-        return fail(SMG_E_UNSUPPORTED, "brick box too large for the dense touched-brick map");
-    size_t nwords = (size_t)((total_cells + 31) / 32);
+    // Touched bricks (lattice.rs:105-163) in per-molecule dense bitmaps.
     uint32_t *bits, *wcount, *woff, *dense;
     HIP_TRY(T.get(&bits, nwords));
     HIP_TRY(T.get(&wcount, nwords));
     HIP_TRY(T.get(&woff, nwords));
-    HIP_TRY(T.get(&dense, total_cells));
-    HIP_TRY(hipMemsetAsync(bits, 0, nwords * 4, s));
-    k_mark_touched<<<blocks_for(A.n_spheres, 128), 128, 0, s>>>(A, P, bh, bits);
-    k_popcount<<<blocks_for(nwords, 256), 256, 0, s>>>(bits, wcount, nwords);
+    HIP_TRY(T.get(&dense, dense_total));
+    if (nwords) HIP_TRY(hipMemsetAsync(bits, 0, nwords * 4, s));
+    if (sph_off[nm] && nwords) k_mark_touched<<<blocks_for(sph_off[nm], 128), 128, 0, s>>>(mols, sph_d, nmu, bits);
+    if (nwords) k_popcount<<<blocks_for(nwords, 256), 256, 0, s>>>(bits, wcount, nwords);
     uint32_t* ntouched_d;
     TRY(scan_u32(s, T, wcount, woff, nwords, &ntouched_d));
     uint32_t ntouched;
     TRY(read_u32(d, ntouched_d, ntouched));
-    M->touched = ntouched;
     uint32_t *touched, *keep, *koff;
     HIP_TRY(T.get(&touched, ntouched));
     HIP_TRY(T.get(&keep, ntouched));
     HIP_TRY(T.get(&koff, ntouched));
-    k_bits_to_list<<<blocks_for(nwords, 256), 256, 0, s>>>(bits, woff, nwords, touched);
+    if (nwords) k_bits_to_list<<<blocks_for(nwords, 256), 256, 0, s>>>(bits, woff, nwords, touched);
     // Brick classification (lattice.rs:168-183), one wavefront per brick.
     HIP_TRY(hipEventRecord(ev[6], s));
-    if (ntouched) k_classify_bricks<<<ntouched, WAVE, 0, s>>>(A, P, touched, ntouched, keep);
+    if (ntouched) k_classify_bricks<<<ntouched, WAVE, 0, s>>>(mols, wofs_d, nmu, touched, ntouched, keep);
     HIP_TRY(hipEventRecord(ev[7], s));
     uint32_t* nb_d;
     TRY(scan_u32(s, T, keep, koff, ntouched, &nb_d));
     uint32_t nb;
     TRY(read_u32(d, nb_d, nb));
-    M->bricks = nb;
     HIP_TRY(hipMalloc(&M->keys, std::max<size_t>(nb, 1) * 3 * sizeof(int32_t)));
-    k_fill_u32<<<blocks_for(total_cells, 256), 256, 0, s>>>(dense, 0xffffffffu, total_cells);
-    if (ntouched) k_compact_bricks<<<blocks_for(ntouched, 256), 256, 0, s>>>(P, touched, keep, koff, ntouched, M->keys, dense);
+    uint32_t *bmol, *tfirst, *fbrick;
+    HIP_TRY(T.get(&bmol, nb));
+    HIP_TRY(T.get(&tfirst, nm + 1));
+    HIP_TRY(T.get(&fbrick, nm + 1));
+    if (dense_total) k_fill_u32<<<blocks_for(dense_total, 256), 256, 0, s>>>(dense, 0xffffffffu, dense_total);
+    if (ntouched)
+        k_compact_bricks<<<blocks_for(ntouched, 256), 256, 0, s>>>(mols, wofs_d, nmu, touched, keep, koff, ntouched,
+                                                                    M->keys, bmol, dense);
+    k_mol_ranges<<<blocks_for(nm + 1, 256), 256, 0, s>>>(wofs_d, nmu, woff, nwords, ntouched, koff, nb, tfirst, fbrick);
     uint8_t* st;
     HIP_TRY(T.get(&st, 8 * (size_t)nb));
     HIP_TRY(hipEventRecord(ev[8], s));
-    if (nb) k_sub_states<<<8 * nb, WAVE, 0, s>>>(A, P, M->keys, nb, st);
+    if (nb) k_sub_states<<<8 * nb, WAVE, 0, s>>>(mols, bmol, M->keys, nb, st);
     HIP_TRY(hipEventRecord(ev[9], s));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[1], s));
 
     // Samples (field/kernel.rs:415-449).
-    HIP_TRY(hipMalloc(&M->values, std::max<size_t>((size_t)nb * P.spb, 1) * sizeof(float)));
-    if (nb) k_assign_uniform<<<nb, 256, 0, s>>>(P, st, nb, M->values);
+    HIP_TRY(hipMalloc(&M->values, std::max<size_t>((size_t)nb * spb, 1) * sizeof(float)));
+    if (nb) k_assign_uniform<<<nb, 256, 0, s>>>(mols, bmol, st, nb, M->values);
     uint32_t nsub = 8 * nb;
     uint32_t *sflag, *soff, *swork;
     HIP_TRY(T.get(&sflag, nsub));
@@ -810,12 +885,15 @@ static int mesh_stage(smg_resident* r, const smg_mesh_params* prm, smg_mesh* M, 
     if (nsub) k_compact_index<<<blocks_for(nsub, 256), 256, 0, s>>>(sflag, soff, nsub, swork);
     uint32_t nsw;
     TRY(read_u32(d, nsw_d, nsw));
-    M->sampled = nsw;
-    unsigned long long* nsamp;
-    HIP_TRY(T.get(&nsamp, 1));
-    HIP_TRY(hipMemsetAsync(nsamp, 0, 8, s));
+    unsigned long long *nsamp, *evd;
+    uint32_t* ovcount;
+    HIP_TRY(T.get(&nsamp, nm));
+    HIP_TRY(T.get(&evd, nm));
+    HIP_TRY(T.get(&ovcount, 1));
+    HIP_TRY(hipMemsetAsync(nsamp, 0, nm * 8, s));
+    HIP_TRY(hipMemsetAsync(evd, 0, nm * 8, s));
     const size_t lds_max = d->prop.sharedMemPerBlock;
-    M->overflow = 0;
+    uint32_t overflow0 = 0;
     {
         const uint32_t* list = swork;
         uint32_t count = nsw;
@@ -830,21 +908,21 @@ static int mesh_stage(smg_resident* r, const smg_mesh_params* prm, smg_mesh* M, 
             HIP_TRY(T.get(&nnext, 1));
             HIP_TRY(hipMemsetAsync(nnext, 0, 4, s));
             if (lds) {
-                k_sample<false><<<count, 256, lay.bytes, s>>>(A, L, P, cap, lay, M->keys, st, list, count, M->values, nullptr,
-                                                       0, next, nnext, nsamp);
+                k_sample<false><<<count, 256, lay.bytes, s>>>(mols, bmol, cap, lay, M->keys, st, list, count,
+                                                              M->values, nullptr, 0, next, nnext, nsamp);
             } else {
                 uint32_t chunk = std::min(count, SCRATCH_CHUNK);
                 char* scratch;
                 HIP_TRY(T.get(&scratch, (size_t)chunk * lay.bytes));
                 for (uint32_t c0 = 0; c0 < count; c0 += chunk)
-                    k_sample<true><<<std::min(chunk, count - c0), 256, 0, s>>>(A, L, P, cap, lay, M->keys, st, list + c0,
-                                                                         std::min(chunk, count - c0), M->values,
-                                                                         scratch, lay.bytes, next, nnext, nsamp);
+                    k_sample<true><<<std::min(chunk, count - c0), 256, 0, s>>>(
+                        mols, bmol, cap, lay, M->keys, st, list + c0, std::min(chunk, count - c0), M->values, scratch,
+                        lay.bytes, next, nnext, nsamp);
             }
             if (pass == 0) HIP_TRY(hipEventRecord(ev[11], s));
             HIP_TRY(hipGetLastError());
             TRY(read_u32(d, nnext, count));
-            if (pass == 0) M->overflow = count;
+            if (pass == 0) overflow0 = count;
             list = next;
         }
         if (count) return fail(SMG_E_INTERNAL, std::to_string(count) + " sub-blocks exceed the largest candidate lists");
@@ -853,15 +931,14 @@ static int mesh_stage(smg_resident* r, const smg_mesh_params* prm, smg_mesh* M, 
 
     // Crossing edges and vertex numbering (mesh/extract.rs:176-216, 606-627).
     uint32_t words = (3 * b * b * b + 63) / 64;
-    M->words = words;
     unsigned long long* mask;
     uint32_t *wpre, *bcount, *scount, *vbase;
     HIP_TRY(T.get(&mask, (size_t)nb * words));
     HIP_TRY(T.get(&wpre, (size_t)nb * words));
     HIP_TRY(T.get(&bcount, nb));
     HIP_TRY(T.get(&scount, nsub));
-    HIP_TRY(T.get(&vbase, nb));
-    if (nb) k_edges<<<nb, 256, 0, s>>>(P, M->values, nb, words, mask, wpre, bcount, scount);
+    HIP_TRY(T.get(&vbase, nb + 1));
+    if (nb) k_edges<<<nb, 256, 0, s>>>(b, M->values, nb, words, mask, wpre, bcount, scount);
     uint32_t* nv_d;
     TRY(scan_u32(s, T, bcount, vbase, nb, &nv_d));
     uint32_t *rflag, *roff, *rwork;
@@ -875,7 +952,6 @@ static int mesh_stage(smg_resident* r, const smg_mesh_params* prm, smg_mesh* M, 
     uint32_t nverts, nrw;
     TRY(read_u32(d, nv_d, nverts));
     TRY(read_u32(d, nrw_d, nrw));
-    M->vertices = nverts;
     HIP_TRY(hipEventRecord(ev[3], s));
 
     // Exact refinement and attributes (mesh/extract.rs:425-597).
@@ -887,12 +963,9 @@ static int mesh_stage(smg_resident* r, const smg_mesh_params* prm, smg_mesh* M, 
     HIP_TRY(hipMalloc(&M->t, nvv * sizeof(double)));
     HIP_TRY(hipMalloc(&M->vevals, nvv * sizeof(uint32_t)));
     VOut vo{M->pos, M->nrm, M->atom, M->patch, M->t, M->vevals};
-    unsigned long long* evd;
-    HIP_TRY(T.get(&evd, 1));
-    HIP_TRY(hipMemsetAsync(evd, 0, 8, s));
     {
         const uint32_t* list = rwork;
-        uint32_t count = nrw, sub_edges = 3 * P.sub * P.sub * P.sub;
+        uint32_t count = nrw, sub_edges = 3 * (b / 2) * (b / 2) * (b / 2);
         HIP_TRY(hipEventRecord(ev[12], s));
         for (int pass = 0; pass < 3 && count; pass++) {
             RCaps cap = REFINE_CAPS[pass];
@@ -905,17 +978,16 @@ static int mesh_stage(smg_resident* r, const smg_mesh_params* prm, smg_mesh* M, 
             HIP_TRY(T.get(&nnext, 1));
             HIP_TRY(hipMemsetAsync(nnext, 0, 4, s));
             if (lds) {
-                k_refine<false><<<count, 256, lay.bytes, s>>>(A, P, cap, lay, M->keys, M->values, list, count, mask, wpre,
-                                                       words, vbase, vo, nullptr, 0, next, nnext, evd);
+                k_refine<false><<<count, 256, lay.bytes, s>>>(mols, bmol, cap, lay, M->keys, M->values, list, count, mask,
+                                                              wpre, words, vbase, vo, nullptr, 0, next, nnext, evd);
             } else {
                 uint32_t chunk = std::min(count, SCRATCH_CHUNK);
                 char* scratch;
                 HIP_TRY(T.get(&scratch, (size_t)chunk * lay.bytes));
                 for (uint32_t c0 = 0; c0 < count; c0 += chunk)
-                    k_refine<true><<<std::min(chunk, count - c0), 256, 0, s>>>(A, P, cap, lay, M->keys, M->values, list + c0,
-                                                                         std::min(chunk, count - c0), mask, wpre,
-                                                                         words, vbase, vo, scratch, lay.bytes, next,
-                                                                         nnext, evd);
+                    k_refine<true><<<std::min(chunk, count - c0), 256, 0, s>>>(
+                        mols, bmol, cap, lay, M->keys, M->values, list + c0, std::min(chunk, count - c0), mask, wpre,
+                        words, vbase, vo, scratch, lay.bytes, next, nnext, evd);
             }
             if (pass == 0) HIP_TRY(hipEventRecord(ev[13], s));
             HIP_TRY(hipGetLastError());
@@ -929,28 +1001,59 @@ static int mesh_stage(smg_resident* r, const smg_mesh_params* prm, smg_mesh* M, 
     // Cell pass and assembly (mesh/extract.rs:226-265, 606-691).
     uint32_t *tcount, *tbase, *err;
     HIP_TRY(T.get(&tcount, nb));
-    HIP_TRY(T.get(&tbase, nb));
+    HIP_TRY(T.get(&tbase, nb + 1));
     HIP_TRY(T.get(&err, 1));
     HIP_TRY(hipMemsetAsync(err, 0, 4, s));
-    if (nb) k_cell_count<<<nb, 256, 0, s>>>(P, M->values, nb, tcount);
+    if (nb) k_cell_count<<<nb, 256, 0, s>>>(b, M->values, nb, tcount);
     uint32_t* nt_d;
     TRY(scan_u32(s, T, tcount, tbase, nb, &nt_d));
     uint32_t ntris;
     TRY(read_u32(d, nt_d, ntris));
-    M->triangles = ntris;
     HIP_TRY(hipMalloc(&M->tris, std::max<size_t>(ntris, 1) * 3 * sizeof(uint32_t)));
     if (nb)
-        k_cell_emit<<<nb, 256, 0, s>>>(P, M->values, M->keys, nb, tbase, dense, vbase, mask, wpre, words, M->tris, err);
+        k_cell_emit<<<nb, 256, 0, s>>>(mols, bmol, fbrick, M->values, M->keys, nb, tbase, dense, vbase, mask, wpre,
+                                       words, M->tris, err);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[5], s));
-    uint32_t bad;
-    TRY(read_u32(d, err, bad));
+    // Per-molecule ranges.
+    uint32_t *sfirst, *vfirst, *trfirst;
+    HIP_TRY(T.get(&sfirst, nm + 1));
+    HIP_TRY(T.get(&vfirst, nm + 1));
+    HIP_TRY(T.get(&trfirst, nm + 1));
+    k_mol_summary<<<blocks_for(nm + 1, 256), 256, 0, s>>>(nmu, fbrick, nb, soff, nsw, vbase, nverts, tbase, ntris,
+                                                          sfirst, vfirst, trfirst);
+    HIP_TRY(hipGetLastError());
+    std::vector<uint32_t> h_tf(nm + 1), h_fb(nm + 1), h_sf(nm + 1), h_vf(nm + 1), h_trf(nm + 1);
+    std::vector<unsigned long long> h_ns(nm), h_ev(nm);
+    uint32_t bad = 0;
+    HIP_TRY(hipMemcpyAsync(h_tf.data(), tfirst, (nm + 1) * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(h_fb.data(), fbrick, (nm + 1) * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(h_sf.data(), sfirst, (nm + 1) * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(h_vf.data(), vfirst, (nm + 1) * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(h_trf.data(), trfirst, (nm + 1) * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(h_ns.data(), nsamp, nm * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(h_ev.data(), evd, nm * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&bad, err, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
     if (bad) return fail(SMG_E_INTERNAL, "a triangle references an edge without a vertex");
-    unsigned long long evh = 0, sh = 0;
-    HIP_TRY(hipMemcpy(&evh, evd, 8, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(&sh, nsamp, 8, hipMemcpyDeviceToHost));
-    M->evals = evh;
-    M->samples = sh;
+    M->info.assign(nm, smg_mesh_info{});
+    M->bfirst = h_fb;
+    M->vfirst = h_vf;
+    M->tfirst = h_trf;
+    for (size_t m = 0; m < nm; m++) {
+        smg_mesh_info& I = M->info[m];
+        I.touched_bricks = h_tf[m + 1] - h_tf[m];
+        I.surface_bricks = h_fb[m + 1] - h_fb[m];
+        I.vertices = h_vf[m + 1] - h_vf[m];
+        I.triangles = h_trf[m + 1] - h_trf[m];
+        I.refine_evaluations = h_ev[m];
+        I.sampled_subblocks = h_sf[m + 1] - h_sf[m];
+        I.overflow_subblocks = nm == 1 ? overflow0 : 0;
+        I.evaluated_samples = h_ns[m];
+        for (int q = 0; q < 3; q++) I.frame[q] = H[m].P.frame[q];
+        I.band = H[m].P.band;
+        I.edge_clamp = H[m].P.t_min;
+    }
     if (times) {
         float ms[5];
         for (int q = 0; q < 5; q++) HIP_TRY(hipEventElapsedTime(&ms[q], ev[q], ev[q + 1]));
@@ -971,24 +1074,35 @@ static int mesh_stage(smg_resident* r, const smg_mesh_params* prm, smg_mesh* M, 
     return SMG_OK;
 }
 
-extern "C" int smg_mesh_new(smg_resident* r, const smg_mesh_params* p, smg_mesh** out, smg_mesh_times* times) {
+extern "C" int smg_mesh_many(smg_resident* const* residents, const smg_mesh_params* params, size_t count,
+                             smg_mesh** out, smg_mesh_times* times) {
+    if (!out) return fail(SMG_E_INVALID, "null output");
     *out = nullptr;
-    if (!r || !p) return fail(SMG_E_INVALID, "null resident or parameters");
+    if (!residents || !params || count == 0) return fail(SMG_E_INVALID, "no molecules");
+    for (size_t m = 0; m < count; m++)
+        if (!residents[m]) return fail(SMG_E_INVALID, "null resident");
     smg_mesh* M = new smg_mesh();
-    M->dev = r->dev;
+    M->dev = residents[0]->dev;
     if (times) std::memset(times, 0, sizeof(*times));
-    int rc = mesh_stage(r, p, M, times);
+    int rc = mesh_many(residents, params, count, M, times);
     if (rc == SMG_OK) {
-        hipError_t e = hipStreamSynchronize(r->dev->stream);
+        hipError_t e = hipStreamSynchronize(M->dev->stream);
         if (e != hipSuccess) rc = fail(SMG_E_CALL, std::string("mesh stage: ") + hipGetErrorString(e));
     }
     if (rc != SMG_OK) {
-        (void)hipStreamSynchronize(r->dev->stream);
+        (void)hipStreamSynchronize(M->dev->stream);
         delete M;
         return rc;
     }
     *out = M;
     return SMG_OK;
+}
+
+extern "C" int smg_mesh_new(smg_resident* r, const smg_mesh_params* p, smg_mesh** out, smg_mesh_times* times) {
+    if (!out) return fail(SMG_E_INVALID, "null output");
+    *out = nullptr;
+    if (!r || !p) return fail(SMG_E_INVALID, "null resident or parameters");
+    return smg_mesh_many(&r, p, 1, out, times);
 }
 
 extern "C" void smg_mesh_free(smg_mesh* m) {
@@ -997,38 +1111,42 @@ extern "C" void smg_mesh_free(smg_mesh* m) {
     delete m;
 }
 
-extern "C" int smg_mesh_info_get(const smg_mesh* m, smg_mesh_info* info) {
+extern "C" size_t smg_mesh_count(const smg_mesh* m) { return m ? m->info.size() : 0; }
+
+extern "C" int smg_mesh_part_info(const smg_mesh* m, size_t part, smg_mesh_info* info) {
     if (!m || !info) return fail(SMG_E_INVALID, "null mesh");
-    std::memset(info, 0, sizeof(*info));
-    info->touched_bricks = m->touched;
-    info->surface_bricks = m->bricks;
-    info->vertices = m->vertices;
-    info->triangles = m->triangles;
-    info->refine_evaluations = m->evals;
-    info->sampled_subblocks = m->sampled;
-    info->overflow_subblocks = m->overflow;
-    info->evaluated_samples = m->samples;
-    for (int q = 0; q < 3; q++) info->frame[q] = m->P.frame[q];
-    info->band = m->P.band;
-    info->edge_clamp = m->P.t_min;
+    if (part >= m->info.size()) return fail(SMG_E_INVALID, "no part " + std::to_string(part));
+    *info = m->info[part];
+    return SMG_OK;
+}
+
+extern "C" int smg_mesh_info_get(const smg_mesh* m, smg_mesh_info* info) { return smg_mesh_part_info(m, 0, info); }
+
+extern "C" int smg_mesh_part_copy(const smg_mesh* m, size_t part, float* positions, float* normals, uint32_t* atoms,
+                                  uint8_t* patches, uint32_t* triangles, int32_t* bricks, float* values, double* t,
+                                  uint32_t* evals) {
+    if (!m) return fail(SMG_E_INVALID, "null mesh");
+    if (part >= m->info.size()) return fail(SMG_E_INVALID, "no part " + std::to_string(part));
+    HIP_TRY(hipSetDevice(m->dev->ordinal));
+    size_t v0 = m->vfirst[part], nv = m->vfirst[part + 1] - v0;
+    size_t t0 = m->tfirst[part], nt = m->tfirst[part + 1] - t0;
+    size_t b0 = m->bfirst[part], nb = m->bfirst[part + 1] - b0;
+    if (positions && nv) HIP_TRY(hipMemcpy(positions, m->pos + 3 * v0, 3 * nv * 4, hipMemcpyDeviceToHost));
+    if (normals && nv) HIP_TRY(hipMemcpy(normals, m->nrm + 3 * v0, 3 * nv * 4, hipMemcpyDeviceToHost));
+    if (atoms && nv) HIP_TRY(hipMemcpy(atoms, m->atom + v0, nv * 4, hipMemcpyDeviceToHost));
+    if (patches && nv) HIP_TRY(hipMemcpy(patches, m->patch + v0, nv, hipMemcpyDeviceToHost));
+    if (t && nv) HIP_TRY(hipMemcpy(t, m->t + v0, nv * 8, hipMemcpyDeviceToHost));
+    if (evals && nv) HIP_TRY(hipMemcpy(evals, m->vevals + v0, nv * 4, hipMemcpyDeviceToHost));
+    if (triangles && nt) HIP_TRY(hipMemcpy(triangles, m->tris + 3 * t0, 3 * nt * 4, hipMemcpyDeviceToHost));
+    if (bricks && nb) HIP_TRY(hipMemcpy(bricks, m->keys + 3 * b0, 3 * nb * 4, hipMemcpyDeviceToHost));
+    if (values && nb)
+        HIP_TRY(hipMemcpy(values, m->values + b0 * m->spb, nb * m->spb * 4, hipMemcpyDeviceToHost));
     return SMG_OK;
 }
 
 extern "C" int smg_mesh_copy(const smg_mesh* m, float* positions, float* normals, uint32_t* atoms, uint8_t* patches,
                              uint32_t* triangles, int32_t* bricks, float* values, double* t, uint32_t* evals) {
-    if (!m) return fail(SMG_E_INVALID, "null mesh");
-    HIP_TRY(hipSetDevice(m->dev->ordinal));
-    size_t nv = m->vertices, nt = m->triangles, nb = m->bricks;
-    if (positions && nv) HIP_TRY(hipMemcpy(positions, m->pos, 3 * nv * 4, hipMemcpyDeviceToHost));
-    if (normals && nv) HIP_TRY(hipMemcpy(normals, m->nrm, 3 * nv * 4, hipMemcpyDeviceToHost));
-    if (atoms && nv) HIP_TRY(hipMemcpy(atoms, m->atom, nv * 4, hipMemcpyDeviceToHost));
-    if (patches && nv) HIP_TRY(hipMemcpy(patches, m->patch, nv, hipMemcpyDeviceToHost));
-    if (t && nv) HIP_TRY(hipMemcpy(t, m->t, nv * 8, hipMemcpyDeviceToHost));
-    if (evals && nv) HIP_TRY(hipMemcpy(evals, m->vevals, nv * 4, hipMemcpyDeviceToHost));
-    if (triangles && nt) HIP_TRY(hipMemcpy(triangles, m->tris, 3 * nt * 4, hipMemcpyDeviceToHost));
-    if (bricks && nb) HIP_TRY(hipMemcpy(bricks, m->keys, 3 * nb * 4, hipMemcpyDeviceToHost));
-    if (values && nb) HIP_TRY(hipMemcpy(values, m->values, (size_t)nb * m->P.spb * 4, hipMemcpyDeviceToHost));
-    return SMG_OK;
+    return smg_mesh_part_copy(m, 0, positions, normals, atoms, patches, triangles, bricks, values, t, evals);
 }
 
 // The generated Marching Cubes table (for the tests).
