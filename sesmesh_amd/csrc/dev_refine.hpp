@@ -253,57 +253,44 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
     if (n[R_OVER]) return false;
     const uint32_t total = n[R_CAND], ns = n[R_S], nc = n[R_C];
     sort_candidates(s.ckey, s.cid, total);
-    // Candidate data, cleared bitsets and blocker rows.
+    // Candidate data, cleared bitsets and blocker row lengths (in cpre).
     const uint32_t cc = cap.cand, nw = cap.near / 64;
-    uint32_t carry = 0;
-    for (uint32_t p0 = 0; p0 < total; p0 += nt) {
-        uint32_t p = p0 + tid, len = 0;
-        if (p < total) {
-            uint32_t k = s.cid[p] & ID_MASK;
-            double* d = s.cdat + (size_t)CDAT * p;
-            if (p < ns) {
-                d[0] = A.cx[k];
-                d[1] = A.cy[k];
-                d[2] = A.cz[k];
-                d[3] = A.R[k];
-                s.coff[p] = A.nb_off[k];
-                len = A.nb_off[k + 1] - A.nb_off[k];
-            } else if (p < ns + nc) {
-                d[0] = A.mx[k];
-                d[1] = A.my[k];
-                d[2] = A.mz[k];
-                d[3] = A.ax[k];
-                d[4] = A.ay[k];
-                d[5] = A.az[k];
-                d[6] = A.crho[k];
-                s.coff[p] = A.cb_off[k];
-                len = A.cb_off[k + 1] - A.cb_off[k];
-            } else {
-                d[0] = A.vx[k];
-                d[1] = A.vy[k];
-                d[2] = A.vz[k];
-            }
-            for (uint32_t w = 0; w < nw; w++) s.cbits[w * cc + p] = 0ull;
+    for (uint32_t p = tid; p < total; p += nt) {
+        uint32_t k = s.cid[p] & ID_MASK;
+        double* d = s.cdat + (size_t)CDAT * p;
+        if (p < ns) {
+            d[0] = A.cx[k];
+            d[1] = A.cy[k];
+            d[2] = A.cz[k];
+            d[3] = A.R[k];
+            s.coff[p] = A.nb_off[k];
+            s.cpre[p] = A.nb_off[k + 1] - A.nb_off[k];
+        } else if (p < ns + nc) {
+            d[0] = A.mx[k];
+            d[1] = A.my[k];
+            d[2] = A.mz[k];
+            d[3] = A.ax[k];
+            d[4] = A.ay[k];
+            d[5] = A.az[k];
+            d[6] = A.crho[k];
+            s.coff[p] = A.cb_off[k];
+            s.cpre[p] = A.cb_off[k + 1] - A.cb_off[k];
+        } else {
+            d[0] = A.vx[k];
+            d[1] = A.vy[k];
+            d[2] = A.vz[k];
         }
-        uint32_t chunk;
-        uint32_t pre = block_scan_256(len, tmp4, chunk);
-        if (p < total) s.cpre[p] = carry + pre;
-        carry += chunk;
+        for (uint32_t w = 0; w < nw; w++) s.cbits[w * cc + p] = 0ull;
     }
     __syncthreads();
-    // Blocker bitsets: one (candidate, row member) pair per thread.
-    for (uint32_t i = tid; i < carry; i += nt) {
-        uint32_t lo = 0, hi = ns + nc - 1;  // last candidate whose row starts at or before i
-        while (lo < hi) {
-            uint32_t mid = (lo + hi + 1) >> 1;
-            if (s.cpre[mid] <= i)
-                lo = mid;
-            else
-                hi = mid - 1;
+    // Blocker bitsets: 8 lanes per S or C candidate row.
+    for (uint32_t p = tid / 8; p < ns + nc; p += nt / 8) {
+        gptr<const uint32_t> row = (p < ns ? A.nb_items : A.cb_items) + s.coff[p];
+        const uint32_t len = s.cpre[p];
+        for (uint32_t q = tid % 8; q < len; q += 8) {
+            uint32_t slot = hash_get(s.hk, s.hv, hmask, row[q]);
+            if (slot != 0xffffffffu) atomicOr(&s.cbits[(slot >> 6) * cc + p], 1ull << (slot & 63));
         }
-        gptr<const uint32_t> items = lo < ns ? A.nb_items : A.cb_items;
-        uint32_t slot = hash_get(s.hk, s.hv, hmask, items[s.coff[lo] + (i - s.cpre[lo])]);
-        if (slot != 0xffffffffu) atomicOr(&s.cbits[(slot >> 6) * cc + lo], 1ull << (slot & 63));
     }
     __syncthreads();
     return true;

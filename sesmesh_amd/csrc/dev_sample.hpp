@@ -312,42 +312,30 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
     // neighbour / circle-blocker rows restricted to near spheres
     // (kernel.rs:685-735).
     const uint32_t cc = cap.cand, nw = cap.near / 64;
-    uint32_t carry = 0;
-    for (uint32_t p0 = 0; p0 < total; p0 += nt) {
-        uint32_t p = p0 + tid, len = 0;
-        if (p < total) {
-            uint32_t k = s.cid[p] & ID_MASK;
-            if (p < ns) {
-                s.cdat[3 * p] = make_float4(Lc.px[k], Lc.py[k], Lc.pz[k], Lc.pr[k]);
-                s.coff[p] = A.nb_off[k];
-                len = A.nb_off[k + 1] - A.nb_off[k];
-            } else {
-                s.cdat[3 * p] = make_float4(Lc.mx[k], Lc.my[k], Lc.mz[k], Lc.rho[k]);
-                s.cdat[3 * p + 1] = make_float4(Lc.ux[k], Lc.uy[k], Lc.uz[k], 0.0f);
-                s.cdat[3 * p + 2] = make_float4(Lc.ex[k], Lc.ey[k], Lc.ez[k], 0.0f);
-                s.coff[p] = A.cb_off[k];
-                len = A.cb_off[k + 1] - A.cb_off[k];
-            }
-            for (uint32_t w = 0; w < nw; w++) s.cbits[w * cc + p] = 0ull;
+    for (uint32_t p = tid; p < total; p += nt) {
+        uint32_t k = s.cid[p] & ID_MASK;
+        if (p < ns) {
+            s.cdat[3 * p] = make_float4(Lc.px[k], Lc.py[k], Lc.pz[k], Lc.pr[k]);
+            s.coff[p] = A.nb_off[k];
+            s.cpre[p] = A.nb_off[k + 1] - A.nb_off[k];
+        } else {
+            s.cdat[3 * p] = make_float4(Lc.mx[k], Lc.my[k], Lc.mz[k], Lc.rho[k]);
+            s.cdat[3 * p + 1] = make_float4(Lc.ux[k], Lc.uy[k], Lc.uz[k], 0.0f);
+            s.cdat[3 * p + 2] = make_float4(Lc.ex[k], Lc.ey[k], Lc.ez[k], 0.0f);
+            s.coff[p] = A.cb_off[k];
+            s.cpre[p] = A.cb_off[k + 1] - A.cb_off[k];
         }
-        uint32_t chunk;
-        uint32_t pre = block_scan_256(len, tmp4, chunk);
-        if (p < total) s.cpre[p] = carry + pre;
-        carry += chunk;
+        for (uint32_t w = 0; w < nw; w++) s.cbits[w * cc + p] = 0ull;
     }
     __syncthreads();
-    for (uint32_t i = tid; i < carry; i += nt) {
-        uint32_t lo = 0, hi = total - 1;  // last candidate whose row starts at or before i
-        while (lo < hi) {
-            uint32_t mid = (lo + hi + 1) >> 1;
-            if (s.cpre[mid] <= i)
-                lo = mid;
-            else
-                hi = mid - 1;
+    // Blocker bitsets: 8 lanes per candidate row (cpre holds the row length).
+    for (uint32_t p = tid / 8; p < total; p += nt / 8) {
+        gptr<const uint32_t> row = (p < ns ? A.nb_items : A.cb_items) + s.coff[p];
+        const uint32_t len = s.cpre[p];
+        for (uint32_t q = tid % 8; q < len; q += 8) {
+            uint32_t slot = hash_get(s.hk, s.hv, hmask, row[q]);
+            if (slot != 0xffffffffu) atomicOr(&s.cbits[(slot >> 6) * cc + p], 1ull << (slot & 63));
         }
-        gptr<const uint32_t> items = lo < ns ? A.nb_items : A.cb_items;
-        uint32_t slot = hash_get(s.hk, s.hv, hmask, items[s.coff[lo] + (i - s.cpre[lo])]);
-        if (slot != 0xffffffffu) atomicOr(&s.cbits[(slot >> 6) * cc + lo], 1ull << (slot & 63));
     }
     __syncthreads();
     return true;
