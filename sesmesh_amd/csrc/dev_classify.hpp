@@ -271,12 +271,15 @@ __device__ inline void brick_center(const DRun& P, const int32_t key[3], double 
 }
 
 // Classification of touched bricks (lattice.rs:168-183): one wavefront per
-// brick; keep[t] = 1 where the surface may pass. touched[t] is a bit index
-// of the run's concatenated bitmaps (wofs: first word of each molecule's).
+// brick (work[w], a touched-list index, or w without a list); keep[t] = 1
+// where the surface may pass. touched[t] is a bit index of the run's
+// concatenated bitmaps (wofs: first word of each molecule's). Most touched
+// bricks are settled by the sphere pass alone, so a wavefront per brick
+// beats a shared gather here.
 __global__ void k_classify_bricks(const Mol* __restrict__ mols, const uint64_t* wofs, uint32_t nm,
-                                  const uint32_t* touched, uint32_t n, uint32_t* keep) {
-    uint32_t w = blockIdx.x;  // one wavefront per workgroup
-    if (w >= n) return;
+                                  const uint32_t* touched, const uint32_t* work, uint32_t n, uint32_t* keep) {
+    if (blockIdx.x >= n) return;
+    uint32_t w = work ? work[blockIdx.x] : blockIdx.x;  // one wavefront per workgroup
     uint32_t bit = touched[w];
     uint32_t m = find_mol(wofs, nm, (uint64_t)(bit >> 5));
     const Mol& M = mols[m];
@@ -290,12 +293,13 @@ __global__ void k_classify_bricks(const Mol* __restrict__ mols, const uint64_t* 
 }
 
 // Sub-block states of surface bricks (kernel.rs:467-482): one wavefront per
-// (brick, sub-block); st[8 * brick + sb].
+// (brick, sub-block), the bricks listed in work; st[8 * brick + sb]. Used for
+// bricks whose shared-gather lists overflow.
 __global__ void k_sub_states(const Mol* __restrict__ mols, const uint32_t* bmol, const int32_t* keys,
-                             uint32_t n_bricks, uint8_t* st) {
+                             const uint32_t* work, uint32_t n_bricks, uint8_t* st) {
     uint32_t w = blockIdx.x;  // one wavefront per workgroup
     if (w >= 8 * n_bricks) return;
-    uint32_t brick = w >> 3, i = w & 7;
+    uint32_t brick = work[w >> 3], i = w & 7;
     const Mol& M = mols[bmol[brick]];
     const DRun& P = M.P;
     double sub = (double)P.sub, h = P.h;
@@ -308,7 +312,7 @@ __global__ void k_sub_states(const Mol* __restrict__ mols, const uint32_t* bmol,
         c[a] = base + off * h;
     }
     uint8_t s = uniform_sign_wave(M.A, c[0], c[1], c[2], rho);
-    if (lane_id() == 0) st[w] = s;
+    if (lane_id() == 0) st[8 * brick + i] = s;
 }
 
 }  // namespace smg

@@ -93,14 +93,15 @@ __device__ inline bool edge_is_crossing(const unsigned long long* mask, uint32_t
 
 // Capacities of the refinement lists: near spheres (a multiple of 64: the
 // blocker bitsets hold near/64 words), inside group, candidates (a power
-// of two), hash slots (a power of two) and crossing edges.
+// of two), hash slots (a power of two) and crossing edges; `radii` 1 keeps
+// each near sphere's expanded and van der Waals radii too.
 struct RCaps {
-    uint32_t near, inside, cand, hash, edges;
+    uint32_t near, inside, cand, hash, edges, radii;
 };
 
 // LDS / scratch layout of the refinement lists (byte offsets).
 struct RLayout {
-    uint32_t nq, nin, hk, hv, ins, ckey, cid, cdat, cbits, coff, cpre, ebuf, bytes;
+    uint32_t nq, nin, nR, nr, hk, hv, ins, ckey, cid, cdat, cbits, coff, cpre, ebuf, bytes;
 };
 
 // Near sphere record: centre and the blocking threshold of R - EPS.
@@ -120,6 +121,8 @@ __host__ __device__ inline RLayout make_rlayout(const RCaps& c) {
     };
     L.nq = take(32 * c.near);
     L.nin = take(8 * c.near);  // inside threshold of R
+    L.nR = take(c.radii ? 8 * c.near : 0);
+    L.nr = take(c.radii ? 8 * c.near : 0);
     L.hk = take(4 * c.hash);
     L.hv = take(4 * c.hash);
     L.ins = take(2 * c.inside);
@@ -137,6 +140,7 @@ __host__ __device__ inline RLayout make_rlayout(const RCaps& c) {
 struct RLists {
     NearD* nq;
     double* nin;
+    double *nR, *nr;  // radii of near spheres (when gathered)
     uint32_t *hk, *hv;
     uint16_t* ins;
     unsigned long long* ckey;
@@ -153,6 +157,8 @@ __device__ inline RLists rcarve(char* base, const RLayout& L) {
     RLists s;
     s.nq = (NearD*)(base + L.nq);
     s.nin = (double*)(base + L.nin);
+    s.nR = (double*)(base + L.nR);
+    s.nr = (double*)(base + L.nr);
     s.hk = (uint32_t*)(base + L.hk);
     s.hv = (uint32_t*)(base + L.hv);
     s.ins = (uint16_t*)(base + L.ins);
@@ -200,6 +206,10 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
                 q.blk = sqrt_threshold(R - EPS);
                 s.nq[at] = q;
                 s.nin[at] = sqrt_threshold(R);
+                if (cap.radii) {
+                    s.nR[at] = R;
+                    s.nr[at] = A.r[k];
+                }
                 hash_put(s.hk, s.hv, hmask, k, at);
                 if (d < R + rho) {
                     uint32_t q = atomicAdd(&n[R_IN], 1u);

@@ -18,6 +18,7 @@
 #include "dev_sample.hpp"
 #include "dev_refine.hpp"
 #include "dev_cells.hpp"
+#include "dev_uniform.hpp"
 #include "host.hpp"
 
 namespace smg {
@@ -149,6 +150,11 @@ __global__ void k_mol_summary(uint32_t nm, const uint32_t* fbrick, uint32_t nb, 
     sfirst[m] = fb < nb ? soff[8 * fb] : nsw;
     vfirst[m] = fb < nb ? vbase[fb] : nverts;
     trfirst[m] = fb < nb ? tbase[fb] : ntris;
+}
+
+__global__ void k_iota(uint32_t* p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i;
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
@@ -693,9 +699,53 @@ int read_u32(smg_device* d, const uint32_t* dev_ptr, uint32_t& v) {
 // previous pass's overflow list.
 constexpr Caps SAMPLE_CAPS[3] = {{128, 64, 256, 64, 256}, {256, 128, 512, 128, 512}, {1024, 512, 4096, 2048, 4096}};
 constexpr RCaps REFINE_CAPS[3] = {{128, 64, 256, 256, 0}, {256, 128, 512, 512, 0}, {1024, 512, 4096, 4096, 0}};
+constexpr RCaps UNIFORM_CAPS[2] = {{192, 128, 256, 512, 0, 1}, {256, 128, 512, 512, 0, 1}};
 constexpr uint32_t LDS_EDGES = 512;      // crossing-edge buffer of the LDS passes
 constexpr uint32_t SCRATCH_CHUNK = 256;   // workgroups per scratch launch
 constexpr uint32_t STATIC_LDS = 6144;     // static LDS of k_sample / k_refine, bound
+
+// Uniform-sign classification of `n` units (touched bricks or surface
+// bricks): shared-gather workgroups with the LDS capacities in turn, then
+// one wavefront per ball for units whose lists still overflow. Records
+// `done` after the first pass (the kernel's time).
+template <uint32_t MODE>
+int run_uniform(smg_device* d, Temps& T, const Mol* mols, const uint64_t* wofs, uint32_t nm, const uint32_t* touched,
+                const uint32_t* bmol, const int32_t* keys, uint32_t n, uint32_t* keep, uint8_t* st, hipEvent_t done) {
+    hipStream_t s = d->stream;
+    const uint32_t* list = nullptr;
+    uint32_t count = n;
+    for (int pass = 0; pass < 2 && count; pass++) {
+        const RCaps& cap = UNIFORM_CAPS[pass];
+        RLayout lay = make_rlayout(cap);
+        if (lay.bytes + STATIC_LDS > d->prop.sharedMemPerBlock) continue;
+        uint32_t *next, *nnext;
+        HIP_TRY(T.get(&next, count));
+        HIP_TRY(T.get(&nnext, 1));
+        HIP_TRY(hipMemsetAsync(nnext, 0, 4, s));
+        k_uniform<MODE, false><<<count, 256, lay.bytes, s>>>(mols, wofs, nm, touched, bmol, keys, cap, lay, list, count,
+                                                             keep, st, nullptr, 0, next, nnext);
+        HIP_TRY(hipGetLastError());
+        if (pass == 0) HIP_TRY(hipEventRecord(done, s));
+        TRY(read_u32(d, nnext, count));
+        list = next;
+    }
+    if (count) {
+        if (!list) {  // no LDS pass fits: every unit
+            uint32_t* all;
+            HIP_TRY(T.get(&all, count));
+            k_iota<<<blocks_for(count, 256), 256, 0, s>>>(all, count);
+            list = all;
+            HIP_TRY(hipEventRecord(done, s));
+        }
+        if (MODE == UNIFORM_TOUCHED)
+            k_classify_bricks<<<count, WAVE, 0, s>>>(mols, wofs, nm, touched, list, count, keep);
+        else
+            k_sub_states<<<8 * count, WAVE, 0, s>>>(mols, bmol, keys, list, count, st);
+        HIP_TRY(hipGetLastError());
+    }
+    if (n == 0) HIP_TRY(hipEventRecord(done, s));
+    return SMG_OK;
+}
 
 // Engine::new's lattice for one molecule (field/kernel.rs:204-261) and a
 // brick box holding every touched brick: the centres' bounding box grown by
@@ -847,7 +897,7 @@ static int mesh_many(smg_resident* const* R, const smg_mesh_params* prm, size_t 
     if (nwords) k_bits_to_list<<<blocks_for(nwords, 256), 256, 0, s>>>(bits, woff, nwords, touched);
     // Brick classification (lattice.rs:168-183), one wavefront per brick.
     HIP_TRY(hipEventRecord(ev[6], s));
-    if (ntouched) k_classify_bricks<<<ntouched, WAVE, 0, s>>>(mols, wofs_d, nmu, touched, ntouched, keep);
+    if (ntouched) k_classify_bricks<<<ntouched, WAVE, 0, s>>>(mols, wofs_d, nmu, touched, nullptr, ntouched, keep);
     HIP_TRY(hipEventRecord(ev[7], s));
     uint32_t* nb_d;
     TRY(scan_u32(s, T, keep, koff, ntouched, &nb_d));
@@ -866,8 +916,7 @@ static int mesh_many(smg_resident* const* R, const smg_mesh_params* prm, size_t 
     uint8_t* st;
     HIP_TRY(T.get(&st, 8 * (size_t)nb));
     HIP_TRY(hipEventRecord(ev[8], s));
-    if (nb) k_sub_states<<<8 * nb, WAVE, 0, s>>>(mols, bmol, M->keys, nb, st);
-    HIP_TRY(hipEventRecord(ev[9], s));
+    TRY(run_uniform<UNIFORM_SUBBLOCKS>(d, T, mols, wofs_d, nmu, nullptr, bmol, M->keys, nb, nullptr, st, ev[9]));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[1], s));
 
