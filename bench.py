@@ -89,6 +89,39 @@ def cpu_baseline(xyzr, probe, spacing, brick, threads, sample_stride):
     }
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload: rocprofv3 FETCH_SIZE
+    and WRITE_SIZE passes calibrated by a copy of known size, as committed
+    under profiles/ (None if absent)."""
+    path = ROOT / "profiles" / "mesh_stage_58k_rings_0.25A_hbm_traffic.json"
+    try:
+        data = json.loads(path.read_text())
+    except (OSError, ValueError):
+        return None, None
+    for name, k in data["kernels"].items():
+        if name.split("<")[0] == kernel:
+            return k["hbm_bytes"], str(path.relative_to(ROOT))
+    return None, None
+
+
+def device_copy(nbytes):
+    """One device-to-device hipMemcpy of `nbytes` (the runtime's copy kernel):
+    the known-size kernel that calibrates the HBM byte counters."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(src, 1, ctypes.c_size_t(nbytes)) != 0 or hip.hipDeviceSynchronize() != 0:
+        raise RuntimeError("hipMemset failed")
+    if hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3) != 0 or hip.hipDeviceSynchronize() != 0:
+        raise RuntimeError("hipMemcpy failed")
+    hip.hipFree(src)
+    hip.hipFree(dst)
+
+
 def batched(dev, count, seed, repeats=3):
     """Batched serving on `count` seeded molecules (atoms log-uniform in
     20-5000, probe 1.2-1.4 A, spacing 0.2/0.25/0.5 A): arrangements built and
@@ -137,6 +170,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch", type=int, default=128, help="molecules in the batched measurement (0: skip)")
     ap.add_argument("--cpu-stride", type=int, default=4)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="after the meshes, one device-to-device copy of this many MiB (the known-size kernel "
+                         "that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported: the mesh stage would shard by "
@@ -167,6 +203,9 @@ def main():
     s_per_mesh = sum(times) / args.steps
     progress(f"GPU: {s_per_mesh:.4f} s/mesh over {args.steps} steps")
     ph = {k: float(np.median([p[k] for p in phase])) for k in phase[0]}
+    if args.copy_mib:
+        device_copy(args.copy_mib << 20)
+        progress(f"copied {args.copy_mib} MiB device to device")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -185,8 +224,10 @@ def main():
     dom = max(kernels, key=lambda k: kernels[k][1])
     _, ms, flops, prec = kernels[dom]
     achieved = flops / (ms * 1e-3) / 1e12
+    traffic, traffic_src = measured_traffic(dom)
     roofline = {"kernel": dom, "bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS[prec],
-                "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[prec], "traffic": None,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[prec], "traffic": traffic,
+                "traffic_source": traffic_src,
                 "precision": prec, "ms": ms,
                 "units": (f"{f64_per_eval:.1f} algorithmic fp64 ops per evaluation x {info.refine_evaluations} "
                           f"evaluations" if dom == "k_refine" else
