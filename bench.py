@@ -170,6 +170,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch", type=int, default=128, help="molecules in the batched measurement (0: skip)")
     ap.add_argument("--cpu-stride", type=int, default=4)
+    ap.add_argument("--precision", type=int, default=0, metavar="COPIES",
+                    help="precision study: exact vs fast refinement on COPIES (60: the 2.4M-atom shell) "
+                         "icosahedral copies of a 40k-atom globule at 1.0 A (0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the meshes, one device-to-device copy of this many MiB (the known-size kernel "
                          "that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -237,6 +240,19 @@ def main():
         a = kfl / (kms * 1e-3) / 1e12
         others[k] = {"ms": kms, "achieved_tflops": a, "frac": a / PEAK_TFLOPS[kp], "precision": kp}
 
+    precision = None
+    if args.precision:
+        shell = sesmesh_amd.synth_shell(seed=3, copies=args.precision)
+        progress(f"precision study: {len(shell)} atoms at 1.0 A")
+        t0 = time.perf_counter()
+        shell_res = sesmesh_amd.Resident(dev, sesmesh_amd.Arrangement(shell, probe))
+        t_build = time.perf_counter() - t0
+        precision = sesmesh_amd.precision_report(shell_res, 1.0, brick)
+        precision.update({"atoms": int(len(shell)), "spacing": 1.0, "arrangement_and_upload_s": t_build})
+        del shell_res
+        progress(f"precision study: fast max |F| {precision['fast']['max_abs_F']:.2e} A, "
+                 f"area {precision['rel_area_error']:.2e}, volume {precision['rel_volume_error']:.2e}")
+
     batch = None
     if args.batch:
         progress(f"batched: {args.batch} molecules")
@@ -258,7 +274,7 @@ def main():
         "cpu_baseline": ({k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")} if cpu else None),
         "best_s": min(times), "median_s": float(np.median(times)),
         "phases_ms": ph, "kernels": others, "arrangement_s": t_arr,
-        "batched": batch, "device": dev.name,
+        "batched": batch, "precision_study": precision, "device": dev.name,
     }
     if cpu:
         out["speedup_vs_cpu_baseline"] = cpu["value"] / s_per_mesh

@@ -180,6 +180,13 @@ int smg_mesh_info_get(const smg_mesh* m, smg_mesh_info* info);
 int smg_mesh_copy(const smg_mesh* m, float* positions, float* normals, uint32_t* atoms, uint8_t* patches,
                   uint32_t* triangles, int32_t* bricks, float* values, double* t, uint32_t* evals);
 
+/* The exact clamped SES field at n points (xyz f64 each): values[i] =
+ * probe - ses_eval(x, probe + band).distance, the distance being 0 outside
+ * the SAS; f64, bit for bit the CPU's SesField::clamped_value
+ * (crates/sesmesh/src/field/point.rs:759-765, ses_eval :384-411). Used to
+ * measure how far a fast-mode vertex lies from the exact surface. */
+int smg_field_clamped(smg_resident* r, const double* points, size_t n, double band, double* values);
+
 /* Batched mesh stage: meshes `count` resident molecules of one device in one
  * pass (every kernel runs once over the concatenated work of all of them;
  * params[i] for residents[i], one brick size for the run). Part i of the

@@ -91,7 +91,7 @@ ABI_FUNCTIONS = [
     "smg_last_error", "smg_device_open", "smg_device_close", "smg_selftest", "smg_arrangement_build",
     "smg_arrangement_free", "smg_arrangement_view", "smg_resident_new", "smg_resident_free", "smg_resident_bytes",
     "smg_mesh_new", "smg_mesh_free", "smg_mesh_info_get", "smg_mesh_copy", "smg_mesh_many", "smg_mesh_count",
-    "smg_mesh_part_info", "smg_mesh_part_copy",
+    "smg_mesh_part_info", "smg_mesh_part_copy", "smg_field_clamped",
 ]
 
 _lib = None
@@ -118,6 +118,8 @@ def lib():
         L.smg_mesh_count.restype = ctypes.c_size_t
         L.smg_mesh_part_info.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
         L.smg_mesh_part_copy.argtypes = [ctypes.c_void_p, ctypes.c_size_t] + [ctypes.c_void_p] * 9
+        L.smg_field_clamped.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_double,
+                                        ctypes.c_void_p]
         L.smg_mesh_many.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
                                     ctypes.c_void_p]
         for name in ("smg_device_close", "smg_arrangement_free", "smg_resident_free", "smg_mesh_free"):
@@ -230,6 +232,15 @@ class Resident:
 
     def mesh(self, spacing=0.5, brick=10, refine="exact", sizing=0):
         return Mesh(self, spacing, brick, refine, sizing)
+
+    def field_clamped(self, points, band):
+        """Exact f64 clamped SES field at points (n x 3): probe minus the
+        distance to the probe-position set clamped at probe + band
+        (reference SesField::clamped_value, field/point.rs:759-765)."""
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(len(pts), np.float64)
+        _check(lib().smg_field_clamped(self._h, _ptr(pts), len(pts), float(band), _ptr(out)))
+        return out
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -351,6 +362,61 @@ def synth_rings(seed=2, n_globule=4100, per_ring=7, ring_radius=55.0, separation
     out = np.zeros((2 * per_ring * n_globule, 4), np.float64)
     lib().smg_synth_rings(ctypes.c_uint64(seed), ctypes.c_size_t(n_globule), ctypes.c_double(radius),
                           ctypes.c_int(per_ring), ctypes.c_double(ring_radius), ctypes.c_double(separation), _ptr(out))
+    return out
+
+
+def synth_shell(seed=3, n_globule=40000, copies=60, shell_radius=450.0):
+    """60 icosahedrally placed copies of one seeded globule on a shell of
+    radius 450 A (the 2.4M-atom precision-study assembly)."""
+    radius = 14.0 * (n_globule / 330.0) ** (1.0 / 3.0)
+    out = np.zeros((copies * n_globule, 4), np.float64)
+    lib().smg_synth_shell(ctypes.c_uint64(seed), ctypes.c_size_t(n_globule), ctypes.c_double(radius),
+                          ctypes.c_int(copies), ctypes.c_double(shell_radius), _ptr(out))
+    return out
+
+
+def mesh_area_volume(positions, triangles):
+    """Area and enclosed volume of a triangle mesh, summed in f64 in
+    triangle order (reference mesh/mod.rs area / volume)."""
+    p = positions.astype(np.float64)
+    a, b, c = p[triangles[:, 0]], p[triangles[:, 1]], p[triangles[:, 2]]
+    area = 0.5 * np.linalg.norm(np.cross(b - a, c - a), axis=1).sum()
+    volume = np.einsum("ij,ij->i", a, np.cross(b, c)).sum() / 6.0
+    return float(area), float(volume)
+
+
+def precision_report(resident, spacing, brick=10):
+    """Fast (f32 compensated) against exact (f64) refinement of one
+    molecule: |F| at every fast and exact vertex (F: the exact f64 clamped
+    field), and the relative area and volume error of the fast mesh."""
+    import time
+
+    out = {}
+    meshes = {}
+    for mode in ("exact", "fast"):
+        t = time.perf_counter()
+        m = resident.mesh(spacing, brick, refine=mode)
+        out[f"{mode}_s"] = time.perf_counter() - t
+        out[f"{mode}_refine_ms"] = m.times.k_refine_ms
+        g = m.arrays()
+        f = np.abs(resident.field_clamped(g["positions"].astype(np.float64), 4.0))
+        av = mesh_area_volume(g["positions"], g["triangles"])
+        meshes[mode] = g
+        out[mode] = {"vertices": int(m.n_vertices), "triangles": int(m.n_triangles),
+                     "max_abs_F": float(f.max()) if len(f) else 0.0,
+                     "mean_abs_F": float(f.mean()) if len(f) else 0.0,
+                     "area": av[0], "volume": av[1]}
+        del m
+    e, f = out["exact"], out["fast"]
+    out["rel_area_error"] = abs(f["area"] - e["area"]) / e["area"] if e["area"] else 0.0
+    out["rel_volume_error"] = abs(f["volume"] - e["volume"]) / abs(e["volume"]) if e["volume"] else 0.0
+    same = (e["vertices"] == f["vertices"]) and (e["triangles"] == f["triangles"])
+    out["same_topology"] = bool(same and np.array_equal(meshes["exact"]["triangles"], meshes["fast"]["triangles"]))
+    if e["vertices"] == f["vertices"]:
+        d = np.linalg.norm(meshes["exact"]["positions"].astype(np.float64) -
+                           meshes["fast"]["positions"].astype(np.float64), axis=1)
+        out["max_vertex_shift"] = float(d.max()) if len(d) else 0.0
+        out["mean_vertex_shift"] = float(d.mean()) if len(d) else 0.0
     return out
 
 

@@ -19,6 +19,7 @@
 #include "dev_refine.hpp"
 #include "dev_cells.hpp"
 #include "dev_uniform.hpp"
+#include "dev_fast.hpp"
 #include "host.hpp"
 
 namespace smg {
@@ -642,6 +643,45 @@ extern "C" void smg_resident_free(smg_resident* r) {
 
 extern "C" size_t smg_resident_bytes(const smg_resident* r) { return r ? r->buf.bytes : 0; }
 
+// ------------------------------------------------------------ field values
+namespace {
+// SesField::clamped_value (field/point.rs:759-765) at points[w], one
+// wavefront per point: probe - ses_eval(x, probe + band).distance, where
+// the distance is 0 outside the SAS (inside_sas, point.rs:372-381).
+__global__ void k_field_clamped(DArr A, const double* points, uint32_t n, double band, double* out) {
+    uint32_t w = blockIdx.x;
+    if (w >= n) return;
+    double x = points[3 * w], y = points[3 * w + 1], z = points[3 * w + 2];
+    bool inside = false;
+    for_near(A.gs, x, y, z, A.max_expanded, lane_id(), WAVE, [&](uint32_t k) {
+        if (!inside && A.status[k] != S_CONTAINED && ddist(x, y, z, A.cx[k], A.cy[k], A.cz[k]) < A.R[k]) inside = true;
+    });
+    double dist = wave_any(inside) ? ses_distance_wave(A, x, y, z, A.probe + band) : 0.0;
+    if (lane_id() == 0) out[w] = A.probe - dist;
+}
+}  // namespace
+
+extern "C" int smg_field_clamped(smg_resident* r, const double* points, size_t n, double band, double* values) {
+    if (!r || (n && (!points || !values))) return fail(SMG_E_INVALID, "null resident, points or values");
+    if (n >= (1u << 31)) return fail(SMG_E_LENGTH, "too many points");
+    if (!(band >= 0.0)) return fail(SMG_E_INVALID, "band must be non-negative");
+    if (n == 0) return SMG_OK;
+    smg_device* d = r->dev;
+    HIP_TRY(hipSetDevice(d->ordinal));
+    hipStream_t s = d->stream;
+    double *pd, *vd;
+    HIP_TRY(hipMallocAsync((void**)&pd, n * 24, s));
+    HIP_TRY(hipMallocAsync((void**)&vd, n * 8, s));
+    HIP_TRY(hipMemcpyAsync(pd, points, n * 24, hipMemcpyHostToDevice, s));
+    k_field_clamped<<<(uint32_t)n, WAVE, 0, s>>>(r->A, pd, (uint32_t)n, band, vd);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(values, vd, n * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipFreeAsync(pd, s));
+    HIP_TRY(hipFreeAsync(vd, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return SMG_OK;
+}
+
 // ------------------------------------------------------------- mesh stage
 namespace {
 
@@ -757,7 +797,8 @@ int molecule_run(const smg_resident* r, const smg_mesh_params* prm, uint32_t b, 
     double spacing = prm->spacing;
     if (!(std::isfinite(spacing) && spacing >= 0.01)) return fail(SMG_E_INVALID, "grid spacing must be at least 0.01 A");
     if (prm->brick != b) return fail(SMG_E_UNSUPPORTED, "one brick size per run");
-    if (prm->refine != SMG_REFINE_EXACT) return fail(SMG_E_UNSUPPORTED, "only exact refinement is implemented");
+    if (prm->refine != SMG_REFINE_EXACT && prm->refine != SMG_REFINE_FAST)
+        return fail(SMG_E_INVALID, "unknown refinement mode " + std::to_string(prm->refine));
     double center[3] = {0, 0, 0}, half = 0.0;
     if (r->n_spheres) {
         double dd[3];
@@ -830,6 +871,9 @@ static int mesh_many(smg_resident* const* R, const smg_mesh_params* prm, size_t 
     if (b < 4 || b > 32 || b % 2) return fail(SMG_E_UNSUPPORTED, "the device path needs an even brick size in 4..=32");
     const uint32_t spb = (b + 1) * (b + 1) * (b + 1);
     M->spb = spb;
+    const bool fast = prm[0].refine == SMG_REFINE_FAST;
+    for (size_t m = 0; m < nm; m++)
+        if ((prm[m].refine == SMG_REFINE_FAST) != fast) return fail(SMG_E_UNSUPPORTED, "one refinement mode per run");
     // Molecule table, offsets of the concatenated per-molecule arrays.
     std::vector<Mol> H(nm);
     std::vector<uint64_t> sph_off(nm + 1, 0), loc_off(nm + 1, 0), wofs(nm + 1, 0);
@@ -1021,22 +1065,35 @@ static int mesh_many(smg_resident* const* R, const smg_mesh_params* prm, size_t 
             bool lds = pass < 2;
             cap.edges = lds ? std::min(sub_edges, LDS_EDGES) : sub_edges;
             RLayout lay = make_rlayout(cap);
-            if (lds && lay.bytes + STATIC_LDS > lds_max) continue;
+            FLayout flay = make_flayout(cap, lay.bytes);
+            const uint32_t bytes = fast ? flay.bytes : lay.bytes;
+            if (lds && bytes + STATIC_LDS > lds_max) continue;
             uint32_t *next, *nnext;
             HIP_TRY(T.get(&next, count));
             HIP_TRY(T.get(&nnext, 1));
             HIP_TRY(hipMemsetAsync(nnext, 0, 4, s));
-            if (lds) {
+            if (lds && fast) {
+                k_refine_fast<false><<<count, 256, bytes, s>>>(mols, bmol, cap, lay, flay, M->keys, M->values, list,
+                                                               count, mask, wpre, words, vbase, vo, nullptr, 0, next,
+                                                               nnext, evd);
+            } else if (lds) {
                 k_refine<false><<<count, 256, lay.bytes, s>>>(mols, bmol, cap, lay, M->keys, M->values, list, count, mask,
                                                               wpre, words, vbase, vo, nullptr, 0, next, nnext, evd);
             } else {
                 uint32_t chunk = std::min(count, SCRATCH_CHUNK);
                 char* scratch;
-                HIP_TRY(T.get(&scratch, (size_t)chunk * lay.bytes));
-                for (uint32_t c0 = 0; c0 < count; c0 += chunk)
-                    k_refine<true><<<std::min(chunk, count - c0), 256, 0, s>>>(
-                        mols, bmol, cap, lay, M->keys, M->values, list + c0, std::min(chunk, count - c0), mask, wpre,
-                        words, vbase, vo, scratch, lay.bytes, next, nnext, evd);
+                HIP_TRY(T.get(&scratch, (size_t)chunk * bytes));
+                for (uint32_t c0 = 0; c0 < count; c0 += chunk) {
+                    uint32_t cnt = std::min(chunk, count - c0);
+                    if (fast)
+                        k_refine_fast<true><<<cnt, 256, 0, s>>>(mols, bmol, cap, lay, flay, M->keys, M->values,
+                                                                 list + c0, cnt, mask, wpre, words, vbase, vo, scratch,
+                                                                 bytes, next, nnext, evd);
+                    else
+                        k_refine<true><<<cnt, 256, 0, s>>>(mols, bmol, cap, lay, M->keys, M->values, list + c0, cnt,
+                                                            mask, wpre, words, vbase, vo, scratch, lay.bytes, next,
+                                                            nnext, evd);
+                }
             }
             if (pass == 0) HIP_TRY(hipEventRecord(ev[13], s));
             HIP_TRY(hipGetLastError());

@@ -8,6 +8,7 @@
 // tip with no such step hands over to a random earlier atom (branching).
 // Elements C/N/O/S with probabilities 0.62/0.17/0.19/0.02 and Bondi radii
 // 1.70/1.55/1.52/1.80 A (reference radii/tables.rs:25-41).
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <unordered_map>
@@ -128,6 +129,63 @@ void smg_synth_rings(uint64_t seed, size_t n_globule, double radius, int per_rin
                 xyzr[4 * at + 2] = one[4 * i + 2] + z0;
                 xyzr[4 * at + 3] = one[4 * i + 3];
             }
+        }
+}
+
+// The 60 rotations of the icosahedral group, as the closure (breadth
+// first, from the identity) of a 5-fold rotation about the vertex axis
+// (0, 1, phi) and the 2-fold rotation about z of the icosahedron with
+// vertices (0, +-1, +-phi) and their cyclic permutations.
+static std::vector<std::array<double, 9>> icosahedral_rotations() {
+    auto mul = [](const std::array<double, 9>& a, const std::array<double, 9>& b) {
+        std::array<double, 9> c{};
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                for (int k = 0; k < 3; k++) c[3 * i + j] += a[3 * i + k] * b[3 * k + j];
+        return c;
+    };
+    auto axis_angle = [](double x, double y, double z, double th) {
+        double n = std::sqrt(x * x + y * y + z * z);
+        x /= n, y /= n, z /= n;
+        double c = std::cos(th), s = std::sin(th), t = 1.0 - c;
+        return std::array<double, 9>{t * x * x + c,     t * x * y - s * z, t * x * z + s * y,
+                                     t * x * y + s * z, t * y * y + c,     t * y * z - s * x,
+                                     t * x * z - s * y, t * y * z + s * x, t * z * z + c};
+    };
+    const double phi = 0.5 * (1.0 + std::sqrt(5.0)), tau = 6.283185307179586;
+    std::array<double, 9> gens[2] = {axis_angle(0.0, 1.0, phi, tau / 5.0), axis_angle(0.0, 0.0, 1.0, tau / 2.0)};
+    std::vector<std::array<double, 9>> group{{1, 0, 0, 0, 1, 0, 0, 0, 1}};
+    for (size_t q = 0; q < group.size() && group.size() < 60; q++)
+        for (auto& g : gens) {
+            std::array<double, 9> r = mul(g, group[q]);
+            bool seen = false;
+            for (auto& e : group) {
+                double d = 0;
+                for (int i = 0; i < 9; i++) d = std::fmax(d, std::fabs(e[i] - r[i]));
+                if (d < 1e-9) seen = true;
+            }
+            if (!seen) group.push_back(r);
+        }
+    return group;
+}
+
+// `copies` (at most 60) copies of one globule on a shell: copy k is the
+// globule shifted by `shell_radius` along a fixed direction off every
+// symmetry axis, then turned by icosahedral rotation k.
+void smg_synth_shell(uint64_t seed, size_t n_globule, double radius, int copies, double shell_radius,
+                     double* xyzr) {
+    std::vector<double> one(4 * n_globule);
+    smg_synth_globule(seed, n_globule, radius, one.data());
+    std::vector<std::array<double, 9>> rot = icosahedral_rotations();
+    double o[3] = {0.31, 0.17, 0.93}, on = std::sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
+    for (int a = 0; a < 3; a++) o[a] *= shell_radius / on;
+    size_t at = 0;
+    for (int k = 0; k < copies && k < (int)rot.size(); k++)
+        for (size_t i = 0; i < n_globule; i++, at++) {
+            double p[3] = {one[4 * i] + o[0], one[4 * i + 1] + o[1], one[4 * i + 2] + o[2]};
+            for (int a = 0; a < 3; a++)
+                xyzr[4 * at + a] = rot[k][3 * a] * p[0] + rot[k][3 * a + 1] * p[1] + rot[k][3 * a + 2] * p[2];
+            xyzr[4 * at + 3] = one[4 * i + 3];
         }
 }
 
