@@ -273,13 +273,35 @@ __device__ inline void sort_bitonic(unsigned long long* key, uint32_t* id, uint3
 // Sorts `count` distinct candidates by cand_less, in place, by the whole
 // workgroup. Each element's rank (the number of elements before it) is
 // counted by a group of g lanes, each over a 1/g share of the list, and
-// summed across the group; then every element moves to its rank. Lists
-// longer than the workgroup take the bitonic network (capacity a power of
-// two).
+// summed across the group; then every element moves to its rank. Up to 4
+// elements per thread are ranked by their thread alone; longer lists take
+// the bitonic network (capacity a power of two).
 __device__ inline void sort_candidates(unsigned long long* key, uint32_t* id, uint32_t count) {
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
-    if (count > nt) {
+    if (count > 4 * nt) {
         sort_bitonic(key, id, count);
+        return;
+    }
+    if (count > nt) {
+        unsigned long long mk[4];
+        uint32_t mi[4], rank[4] = {0, 0, 0, 0};
+        for (int j = 0; j < 4; j++) {
+            uint32_t e = tid + j * nt;
+            mk[j] = e < count ? key[e] : ~0ull;
+            mi[j] = e < count ? id[e] : 0xffffffffu;
+        }
+        for (uint32_t o = 0; o < count; o++) {
+            unsigned long long k = key[o];
+            uint32_t i = id[o];
+            for (int j = 0; j < 4; j++) rank[j] += cand_less(k, i, mk[j], mi[j]) ? 1u : 0u;
+        }
+        __syncthreads();
+        for (int j = 0; j < 4; j++)
+            if (tid + j * nt < count) {
+                key[rank[j]] = mk[j];
+                id[rank[j]] = mi[j];
+            }
+        __syncthreads();
         return;
     }
     uint32_t g = 1;  // lanes per element: the most with g * count <= nt, at most 16

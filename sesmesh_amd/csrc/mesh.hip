@@ -734,12 +734,20 @@ int read_u32(smg_device* d, const uint32_t* dev_ptr, uint32_t& v) {
         }                       \
     } while (0)
 
-// List capacities, tried in turn: LDS variants sized for common and for
-// large sub-blocks, then global scratch for the rest. Each pass takes the
-// previous pass's overflow list.
-constexpr Caps SAMPLE_CAPS[3] = {{128, 64, 256, 64, 256}, {256, 128, 512, 128, 512}, {1024, 512, 4096, 2048, 4096}};
-constexpr RCaps REFINE_CAPS[3] = {{128, 64, 256, 256, 0}, {256, 128, 512, 512, 0}, {1024, 512, 4096, 4096, 0}};
-constexpr RCaps UNIFORM_CAPS[2] = {{192, 128, 256, 512, 0, 1}, {256, 128, 512, 512, 0, 1}};
+// List capacities, tried in turn: LDS variants sized for common sub-blocks
+// (several workgroups per CU) and for those of coarse lattices (up to the
+// 160 KB of LDS one gfx950 workgroup may hold), then global scratch for the
+// rest. Each pass takes the previous pass's overflow list.
+constexpr int TIERS = 4;  // the last one in global scratch
+constexpr Caps SAMPLE_CAPS[TIERS] = {{128, 64, 256, 64, 256},
+                                     {256, 128, 384, 128, 512},
+                                     {640, 256, 768, 256, 2048},
+                                     {1024, 512, 4096, 2048, 4096}};
+constexpr RCaps REFINE_CAPS[TIERS] = {
+    {128, 64, 256, 256, 0}, {256, 128, 384, 512, 0}, {640, 256, 704, 1024, 0}, {1024, 512, 4096, 4096, 0}};
+// Sub-block classification: one LDS tier; bricks beyond it take one
+// wavefront per ball, which beats larger tiers at one or two workgroups per CU.
+constexpr RCaps UNIFORM_CAPS[1] = {{192, 128, 256, 512, 0, 1}};
 constexpr uint32_t LDS_EDGES = 512;      // crossing-edge buffer of the LDS passes
 constexpr uint32_t SCRATCH_CHUNK = 256;   // workgroups per scratch launch
 constexpr uint32_t STATIC_LDS = 6144;     // static LDS of k_sample / k_refine, bound
@@ -754,7 +762,7 @@ int run_uniform(smg_device* d, Temps& T, const Mol* mols, const uint64_t* wofs, 
     hipStream_t s = d->stream;
     const uint32_t* list = nullptr;
     uint32_t count = n;
-    for (int pass = 0; pass < 2 && count; pass++) {
+    for (int pass = 0; pass < 1 && count; pass++) {
         const RCaps& cap = UNIFORM_CAPS[pass];
         RLayout lay = make_rlayout(cap);
         if (lay.bytes + STATIC_LDS > d->prop.sharedMemPerBlock) continue;
@@ -991,10 +999,10 @@ static int mesh_many(smg_resident* const* R, const smg_mesh_params* prm, size_t 
         const uint32_t* list = swork;
         uint32_t count = nsw;
         HIP_TRY(hipEventRecord(ev[10], s));
-        for (int pass = 0; pass < 3 && count; pass++) {
+        for (int pass = 0; pass < TIERS && count; pass++) {
             const Caps& cap = SAMPLE_CAPS[pass];
             Layout lay = make_layout(cap);
-            bool lds = pass < 2;
+            bool lds = pass < TIERS - 1;
             if (lds && lay.bytes + STATIC_LDS > lds_max) continue;
             uint32_t *next, *nnext;
             HIP_TRY(T.get(&next, count));
@@ -1060,9 +1068,9 @@ static int mesh_many(smg_resident* const* R, const smg_mesh_params* prm, size_t 
         const uint32_t* list = rwork;
         uint32_t count = nrw, sub_edges = 3 * (b / 2) * (b / 2) * (b / 2);
         HIP_TRY(hipEventRecord(ev[12], s));
-        for (int pass = 0; pass < 3 && count; pass++) {
+        for (int pass = 0; pass < TIERS && count; pass++) {
             RCaps cap = REFINE_CAPS[pass];
-            bool lds = pass < 2;
+            bool lds = pass < TIERS - 1;
             cap.edges = lds ? std::min(sub_edges, LDS_EDGES) : sub_edges;
             RLayout lay = make_rlayout(cap);
             FLayout flay = make_flayout(cap, lay.bytes);
