@@ -11,7 +11,21 @@
 // the reference algorithm executed in IEEE f64/f32.
 //
 // Scope: the SES kind only (the workloads this project measures are SES).
-// Pinning status of each part is listed in oracle/README.md.
+//
+// How each part is pinned (tests/test_oracle_golden.py, test_oracle_math.py):
+// * the whole path (arrangement, classification, sampling, refinement, MC
+//   table, assembly, area/volume) reproduces the reference's recorded
+//   validation runs (docs/validation/compare/runs.csv, campaign "acc"):
+//   vertex/triangle/component/cavity counts and every printed digit of area
+//   and volume for 1CRN, 1EJG, 1UBQ, 3FX2, 2PTN at 0.5 and 0.25 A, 1AON at
+//   0.5 A;
+// * the MC table hashes equal to the reference's table data;
+// * ses_eval and the arrangement agree with a brute-force evaluator over all
+//   spheres, pairs and triples within 1e-9;
+// * lattice samples of a single sphere equal the closed form;
+// * refined vertices lie within 1e-4 h + 2e-6 of the surface, meshes are
+//   closed and oriented, and results do not depend on threads or brick size.
+// No part is left unpinned.
 #pragma once
 
 #include <array>

@@ -1,8 +1,9 @@
 // Cell pass and assembly on the device: Marching Cubes cases from sample
-// signs, triangles as edge references resolved to global vertex indices
-// (reference mesh/extract.rs:226-265 and :606-691). Vertex indices are the
-// brick's vertex base plus the edge's rank among the target brick's crossing
-// edges, so triangles come out in the CPU's brick / cell / table order.
+// signs, triangles as edge references resolved to vertex indices local to
+// the brick's molecule (reference mesh/extract.rs:226-265 and :606-691).
+// Vertex indices are the brick's vertex base, less the molecule's first
+// vertex, plus the edge's rank among the target brick's crossing edges, so
+// triangles come out in the CPU's brick / cell / table order.
 #pragma once
 #include "dev_common.hpp"
 
