@@ -40,6 +40,7 @@ typedef struct smg_device smg_device;
 typedef struct smg_arrangement smg_arrangement;
 typedef struct smg_resident smg_resident;
 typedef struct smg_mesh smg_mesh;
+typedef struct smg_plan smg_plan;
 typedef struct smg_batch smg_batch;
 
 /* Thread-local message of the last failure (never NULL). */
@@ -204,6 +205,60 @@ int smg_mesh_part_info(const smg_mesh* m, size_t part, smg_mesh_info* info);
 int smg_mesh_part_copy(const smg_mesh* m, size_t part, float* positions, float* normals, uint32_t* atoms,
                        uint8_t* patches, uint32_t* triangles, int32_t* bricks, float* values, double* t,
                        uint32_t* evals);
+
+/* ------------------------------------------------------------------ plans
+ * A capture-ready mesh stage for serving loops. No counterpart in the
+ * reference: it is the loop a service runs around fold_resident
+ * (crates/sesmesh/src/field/gpu.rs:532-712), one call per batch of
+ * molecules.
+ *
+ * A plan fixes capacities for its runs. Its buffers are allocated once, on
+ * the first run, and the whole pipeline is then captured into a hipGraph:
+ * every count stays in device memory, and persistent workgroups take work
+ * from device-side queues. Each later run uploads the run's molecule table
+ * and replays the graph, with one synchronisation at the end.
+ *
+ * A run whose counts exceed the capacities is redone by smg_mesh_many, the
+ * exact count-then-allocate path. Such counts are found on the host for
+ * molecules, spheres and brick boxes, and on the device for surface bricks,
+ * vertices and triangles. The plan then grows to hold that run, and its
+ * buffers and graph are rebuilt on the next run. The mesh is the same
+ * either way: byte for byte what smg_mesh_many makes.
+ *
+ * The mesh of a run that fits views the plan's buffers. It must be freed
+ * (smg_mesh_free) before the plan runs again; if the plan is freed first,
+ * the view is detached and its copies fail. A run that falls back returns
+ * a mesh of its own. */
+typedef struct {
+    uint32_t molecules; /* per run (table rows) */
+    uint32_t brick;     /* brick size of every run */
+    int32_t refine;     /* SMG_REFINE_* of every run */
+    uint32_t pad;
+    uint64_t locals;         /* sum over molecules of max(spheres, circles, SAS vertices) */
+    uint64_t spheres;        /* sum of spheres */
+    uint64_t box_bricks;     /* sum of the molecules' brick-box cells */
+    uint64_t surface_bricks; /* device counts: */
+    uint64_t vertices;
+    uint64_t triangles;
+} smg_plan_caps;
+
+typedef struct {
+    uint64_t runs, replays, fallbacks, captures;
+    uint32_t last_overflow; /* 1 surface bricks, 2 vertices, 4 triangles, 8 table */
+    uint32_t pad;
+} smg_plan_stats;
+
+int smg_plan_new(smg_device* dev, const smg_plan_caps* caps, smg_plan** out);
+/* Capacities holding the run of these molecules. The table-side capacities
+ * are exact. The device-side ones come from `m`, the result of
+ * smg_mesh_many for the same molecules, grown by the fraction `margin`; they
+ * are zero when m is NULL. */
+int smg_plan_caps_for(smg_resident* const* residents, const smg_mesh_params* params, size_t count,
+                      const smg_mesh* m, double margin, smg_plan_caps* caps);
+int smg_plan_run(smg_plan* p, smg_resident* const* residents, const smg_mesh_params* params, size_t count,
+                 smg_mesh** out, smg_mesh_times* times);
+int smg_plan_stats_get(const smg_plan* p, smg_plan_stats* stats);
+void smg_plan_free(smg_plan* p);
 
 #ifdef __cplusplus
 }

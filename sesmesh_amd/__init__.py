@@ -85,13 +85,33 @@ class MeshInfo(ctypes.Structure):
                 ("edge_clamp", ctypes.c_double)]
 
 
+class PlanCaps(ctypes.Structure):
+    """smg_plan_caps of include/sesmesh_gpu.h."""
+    _fields_ = [("molecules", ctypes.c_uint32), ("brick", ctypes.c_uint32), ("refine", ctypes.c_int32),
+                ("pad", ctypes.c_uint32), ("locals", ctypes.c_uint64), ("spheres", ctypes.c_uint64),
+                ("box_bricks", ctypes.c_uint64), ("surface_bricks", ctypes.c_uint64),
+                ("vertices", ctypes.c_uint64), ("triangles", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+class PlanStats(ctypes.Structure):
+    _fields_ = [("runs", ctypes.c_uint64), ("replays", ctypes.c_uint64), ("fallbacks", ctypes.c_uint64),
+                ("captures", ctypes.c_uint64), ("last_overflow", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
 # Every function include/sesmesh_gpu.h declares, plus the helpers the
 # library exports for tests and synthetic inputs.
 ABI_FUNCTIONS = [
     "smg_last_error", "smg_device_open", "smg_device_close", "smg_selftest", "smg_arrangement_build",
     "smg_arrangement_free", "smg_arrangement_view", "smg_resident_new", "smg_resident_free", "smg_resident_bytes",
     "smg_mesh_new", "smg_mesh_free", "smg_mesh_info_get", "smg_mesh_copy", "smg_mesh_many", "smg_mesh_count",
-    "smg_mesh_part_info", "smg_mesh_part_copy", "smg_field_clamped",
+    "smg_mesh_part_info", "smg_mesh_part_copy", "smg_field_clamped", "smg_plan_new", "smg_plan_caps_for",
+    "smg_plan_run", "smg_plan_stats_get", "smg_plan_free",
 ]
 
 _lib = None
@@ -122,7 +142,12 @@ def lib():
                                         ctypes.c_void_p]
         L.smg_mesh_many.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
                                     ctypes.c_void_p]
-        for name in ("smg_device_close", "smg_arrangement_free", "smg_resident_free", "smg_mesh_free"):
+        L.smg_plan_caps_for.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
+                                        ctypes.c_double, ctypes.c_void_p]
+        L.smg_plan_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                   ctypes.c_void_p, ctypes.c_void_p]
+        for name in ("smg_device_close", "smg_arrangement_free", "smg_resident_free", "smg_mesh_free",
+                     "smg_plan_free"):
             getattr(L, name).restype = None
         _lib = L
     return _lib
@@ -257,10 +282,19 @@ class Mesh:
     """The mesh stage of one or more molecules on the device (one part per
     molecule); the meshes stay in HBM until copied."""
 
-    def __init__(self, resident, spacing, brick, refine, sizing, _batch=None):
+    def __init__(self, resident, spacing, brick, refine, sizing, _batch=None, _plan=None):
         self._h = _vp()
         self.times = MeshTimes()
-        if _batch is None:
+        self._plan = None
+        if _plan is not None:
+            plan, residents, params = _plan
+            handles = (ctypes.c_void_p * len(residents))(*[r._h.value for r in residents])
+            arr = (MeshParams * len(params))(*params)
+            _check(lib().smg_plan_run(plan._h, handles, arr, len(residents), ctypes.byref(self._h),
+                                      ctypes.byref(self.times)))
+            self.residents = list(residents)
+            self._plan = plan  # a view of the plan's buffers: keep the plan alive
+        elif _batch is None:
             p = _mesh_params(spacing, brick, refine, sizing)
             _check(lib().smg_mesh_new(resident._h, ctypes.byref(p), ctypes.byref(self._h),
                                       ctypes.byref(self.times)))
@@ -315,6 +349,48 @@ def mesh_many(residents, spacings, brick=10, refine="exact", sizing=0):
     if len(params) != len(residents):
         raise ValueError("one spacing per molecule")
     return Mesh(None, None, brick, refine, sizing, _batch=(residents, params))
+
+
+class Plan:
+    """A capture-ready batched mesh stage (smg_plan_*): buffers sized once by
+    capacities, the whole pipeline replayed as a hipGraph; runs beyond the
+    capacities fall back to smg_mesh_many and grow the plan."""
+
+    def __init__(self, device, caps):
+        self._h = _vp()
+        self.device = device
+        self.brick, self.refine = caps.brick, "exact" if caps.refine == 0 else "fast"
+        _check(lib().smg_plan_new(device._h, ctypes.byref(caps), ctypes.byref(self._h)))
+
+    @staticmethod
+    def caps_for(residents, spacings, brick=10, refine="exact", mesh=None, margin=0.25):
+        """Capacities for these molecules: exact on the table side; from
+        `mesh` (their smg_mesh_many result) grown by `margin` on the device side."""
+        params = (MeshParams * len(residents))(*[_mesh_params(float(h), brick, refine, 0) for h in spacings])
+        handles = (ctypes.c_void_p * len(residents))(*[r._h.value for r in residents])
+        caps = PlanCaps()
+        _check(lib().smg_plan_caps_for(handles, params, len(residents), mesh._h if mesh is not None else None,
+                                       ctypes.c_double(margin), ctypes.byref(caps)))
+        return caps
+
+    def run(self, residents, spacings):
+        """The meshes of `residents` (part i equals smg_mesh_many's part i). A
+        mesh that views the plan must be dropped before the next run."""
+        params = [_mesh_params(float(h), self.brick, self.refine, 0) for h in spacings]
+        if len(params) != len(residents):
+            raise ValueError("one spacing per molecule")
+        return Mesh(None, None, self.brick, self.refine, 0, _plan=(self, residents, params))
+
+    def stats(self):
+        st = PlanStats()
+        _check(lib().smg_plan_stats_get(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().smg_plan_free(h)
+            self._h = _vp()
 
 
 @dataclass

@@ -23,24 +23,26 @@ __device__ inline uint32_t cell_case(const float* val, uint32_t side, uint32_t x
 }
 
 // Triangles per brick: thread t takes the contiguous cells [t*per, (t+1)*per)
-// in (z, y, x) order.
-__global__ void __launch_bounds__(256) k_cell_count(uint32_t b, const float* values, uint32_t n_bricks,
+// in (z, y, x) order. One workgroup per brick, striding over *n_bricks.
+__global__ void __launch_bounds__(256) k_cell_count(uint32_t b, const float* values, const uint32_t* n_bricks,
                                                     uint32_t* tcount) {
     __shared__ uint32_t tmp[4];
-    uint32_t brick = blockIdx.x;
-    if (brick >= n_bricks) return;
-    const uint32_t side = b + 1;
-    const float* val = values + (size_t)brick * side * side * side;
-    uint32_t ncell = b * b * b, per = (ncell + blockDim.x - 1) / blockDim.x;
-    uint32_t mine = 0;
-    for (uint32_t c = threadIdx.x * per; c < min(ncell, (threadIdx.x + 1) * per); c++) {
-        uint32_t x = c % b, y = (c / b) % b, z = c / (b * b);
-        uint32_t cs = cell_case(val, side, x, y, z);
-        mine += c_tri_offset[cs + 1] - c_tri_offset[cs];
+    const uint32_t nb = *n_bricks;
+    for (uint32_t brick = blockIdx.x; brick < nb; brick += gridDim.x) {
+        const uint32_t side = b + 1;
+        const float* val = values + (size_t)brick * side * side * side;
+        uint32_t ncell = b * b * b, per = (ncell + blockDim.x - 1) / blockDim.x;
+        uint32_t mine = 0;
+        for (uint32_t c = threadIdx.x * per; c < min(ncell, (threadIdx.x + 1) * per); c++) {
+            uint32_t x = c % b, y = (c / b) % b, z = c / (b * b);
+            uint32_t cs = cell_case(val, side, x, y, z);
+            mine += c_tri_offset[cs + 1] - c_tri_offset[cs];
+        }
+        uint32_t total;
+        block_scan_256(mine, tmp, total);
+        if (threadIdx.x == 0) tcount[brick] = total;
+        __syncthreads();
     }
-    uint32_t total;
-    block_scan_256(mine, tmp, total);
-    if (threadIdx.x == 0) tcount[brick] = total;
 }
 
 __device__ inline int64_t dense_index(const DRun& P, int64_t kx, int64_t ky, int64_t kz) {
@@ -53,66 +55,69 @@ __device__ inline int64_t dense_index(const DRun& P, int64_t kx, int64_t ky, int
 // and :629-667), with vertex indices local to the brick's molecule (its
 // first vertex is that of its first brick, fbrick[m]). Neighbour bricks
 // are found in the molecule's dense index grid. A reference that does not
-// resolve sets *err.
+// resolve sets *err; triangles at or beyond `tcap` are not written (the
+// run's triangle count exceeds the buffer and is reported by its caller).
 __global__ void __launch_bounds__(256) k_cell_emit(const Mol* __restrict__ mols, const uint32_t* bmol,
                                                    const uint32_t* fbrick, const float* values, const int32_t* keys,
-                                                   uint32_t n_bricks, const uint32_t* tbase,
+                                                   const uint32_t* n_bricks, const uint32_t* tbase,
                                                    const uint32_t* dense, const uint32_t* vbase,
                                                    const unsigned long long* mask, const uint32_t* wpre,
-                                                   uint32_t words, uint32_t* tris, uint32_t* err) {
+                                                   uint32_t words, uint32_t* tris, size_t tcap, uint32_t* err) {
     __shared__ uint32_t tmp[4];
-    uint32_t brick = blockIdx.x;
-    if (brick >= n_bricks) return;
-    const uint32_t m = bmol[brick];
-    const DRun& P = mols[m].P;
-    const uint32_t* mdense = dense + mols[m].dense_off;
-    const uint32_t vfirst = vbase[fbrick[m]];
-    const float* val = values + (size_t)brick * P.spb;
-    uint32_t b = P.b, ncell = b * b * b, per = (ncell + blockDim.x - 1) / blockDim.x;
-    uint32_t c_lo = threadIdx.x * per, c_hi = min(ncell, (threadIdx.x + 1) * per);
-    uint32_t mine = 0;
-    for (uint32_t c = c_lo; c < c_hi; c++) {
-        uint32_t cs = cell_case(val, P.side, c % b, (c / b) % b, c / (b * b));
-        mine += c_tri_offset[cs + 1] - c_tri_offset[cs];
-    }
-    uint32_t total;
-    uint32_t at = tbase[brick] + block_scan_256(mine, tmp, total);
-    int64_t key[3] = {keys[3 * brick], keys[3 * brick + 1], keys[3 * brick + 2]};
-    int64_t target[8];
-    for (int q = 0; q < 8; q++) target[q] = -2;
-    target[0] = brick;
-    for (uint32_t c = c_lo; c < c_hi; c++) {
-        uint32_t x = c % b, y = (c / b) % b, z = c / (b * b);
-        uint32_t cs = cell_case(val, P.side, x, y, z);
-        for (uint32_t t = c_tri_offset[cs]; t < c_tri_offset[cs + 1]; t++) {
-            for (int q = 0; q < 3; q++) {
-                uint32_t e = c_tri_edges[3 * t + q];
-                uint32_t axis = e / 4, lo = c_edge_lower[e];
-                uint32_t l[3] = {x + (lo & 1), y + ((lo >> 1) & 1), z + ((lo >> 2) & 1)};
-                uint32_t code = 0, lc[3] = {0, 0, 0};
-                for (int a = 0; a < 3; a++) {
-                    if (l[a] == b)
-                        code |= 1u << a;
-                    else
-                        lc[a] = l[a];
-                }
-                uint32_t id = ((lc[2] * b + lc[1]) * b + lc[0]) * 3 + axis;
-                if (target[code] == -2) {
-                    int64_t di = dense_index(P, key[0] + (code & 1), key[1] + ((code >> 1) & 1),
-                                             key[2] + ((code >> 2) & 1));
-                    target[code] = (di < 0 || mdense[di] == 0xffffffffu) ? -1 : (int64_t)mdense[di];
-                }
-                int64_t tb = target[code];
-                uint32_t vi = 0;
-                if (tb < 0 || !edge_is_crossing(mask, words, (uint32_t)tb, id)) {
-                    atomicOr(err, 1u);
-                } else {
-                    vi = vbase[tb] + edge_rank(mask, wpre, words, (uint32_t)tb, id) - vfirst;
-                }
-                tris[3 * (size_t)at + q] = vi;
-            }
-            at++;
+    const uint32_t nb = *n_bricks;
+    for (uint32_t brick = blockIdx.x; brick < nb; brick += gridDim.x) {
+        const uint32_t m = bmol[brick];
+        const DRun& P = mols[m].P;
+        const uint32_t* mdense = dense + mols[m].dense_off;
+        const uint32_t vfirst = vbase[fbrick[m]];
+        const float* val = values + (size_t)brick * P.spb;
+        uint32_t b = P.b, ncell = b * b * b, per = (ncell + blockDim.x - 1) / blockDim.x;
+        uint32_t c_lo = threadIdx.x * per, c_hi = min(ncell, (threadIdx.x + 1) * per);
+        uint32_t mine = 0;
+        for (uint32_t c = c_lo; c < c_hi; c++) {
+            uint32_t cs = cell_case(val, P.side, c % b, (c / b) % b, c / (b * b));
+            mine += c_tri_offset[cs + 1] - c_tri_offset[cs];
         }
+        uint32_t total;
+        size_t at = (size_t)tbase[brick] + block_scan_256(mine, tmp, total);
+        int64_t key[3] = {keys[3 * brick], keys[3 * brick + 1], keys[3 * brick + 2]};
+        int64_t target[8];
+        for (int q = 0; q < 8; q++) target[q] = -2;
+        target[0] = brick;
+        for (uint32_t c = c_lo; c < c_hi; c++) {
+            uint32_t x = c % b, y = (c / b) % b, z = c / (b * b);
+            uint32_t cs = cell_case(val, P.side, x, y, z);
+            for (uint32_t t = c_tri_offset[cs]; t < c_tri_offset[cs + 1]; t++) {
+                for (int q = 0; q < 3; q++) {
+                    uint32_t e = c_tri_edges[3 * t + q];
+                    uint32_t axis = e / 4, lo = c_edge_lower[e];
+                    uint32_t l[3] = {x + (lo & 1), y + ((lo >> 1) & 1), z + ((lo >> 2) & 1)};
+                    uint32_t code = 0, lc[3] = {0, 0, 0};
+                    for (int a = 0; a < 3; a++) {
+                        if (l[a] == b)
+                            code |= 1u << a;
+                        else
+                            lc[a] = l[a];
+                    }
+                    uint32_t id = ((lc[2] * b + lc[1]) * b + lc[0]) * 3 + axis;
+                    if (target[code] == -2) {
+                        int64_t di = dense_index(P, key[0] + (code & 1), key[1] + ((code >> 1) & 1),
+                                                 key[2] + ((code >> 2) & 1));
+                        target[code] = (di < 0 || mdense[di] == 0xffffffffu) ? -1 : (int64_t)mdense[di];
+                    }
+                    int64_t tb = target[code];
+                    uint32_t vi = 0;
+                    if (tb < 0 || !edge_is_crossing(mask, words, (uint32_t)tb, id)) {
+                        atomicOr(err, 1u);
+                    } else {
+                        vi = vbase[tb] + edge_rank(mask, wpre, words, (uint32_t)tb, id) - vfirst;
+                    }
+                    if (at < tcap) tris[3 * at + q] = vi;
+                }
+                at++;
+            }
+        }
+        __syncthreads();
     }
 }
 

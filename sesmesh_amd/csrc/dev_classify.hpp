@@ -232,29 +232,30 @@ __device__ inline int64_t brick_of(double v, double bh) { return (int64_t)floor(
 // thread per sphere of the run (sph_off: first sphere of each molecule).
 __global__ void k_mark_touched(const Mol* __restrict__ mols, const uint64_t* sph_off, uint32_t nm,
                                uint32_t* bits) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= sph_off[nm]) return;
-    uint32_t m = find_mol(sph_off, nm, i);
-    const Mol& M = mols[m];
-    const DArr& A = M.A;
-    const DRun& P = M.P;
-    uint32_t k = (uint32_t)(i - sph_off[m]);
-    if (A.status[k] == S_CONTAINED) return;
-    double bh = (double)P.b * P.h;
-    double r = A.R[k] + CLASSIFY_EPS;
-    double c[3] = {A.cx[k], A.cy[k], A.cz[k]};
-    int64_t lo[3], hi[3];
-    for (int a = 0; a < 3; a++) {
-        lo[a] = brick_of(c[a] - r, bh) - P.blo[a];
-        hi[a] = brick_of(c[a] + r, bh) - P.blo[a];
+    const uint64_t n = sph_off[nm];
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t m = find_mol(sph_off, nm, i);
+        const Mol& M = mols[m];
+        const DArr& A = M.A;
+        const DRun& P = M.P;
+        uint32_t k = (uint32_t)(i - sph_off[m]);
+        if (A.status[k] == S_CONTAINED) continue;
+        double bh = (double)P.b * P.h;
+        double r = A.R[k] + CLASSIFY_EPS;
+        double c[3] = {A.cx[k], A.cy[k], A.cz[k]};
+        int64_t lo[3], hi[3];
+        for (int a = 0; a < 3; a++) {
+            lo[a] = brick_of(c[a] - r, bh) - P.blo[a];
+            hi[a] = brick_of(c[a] + r, bh) - P.blo[a];
+        }
+        uint32_t* mb = bits + M.word_off;
+        for (int64_t z = lo[2]; z <= hi[2]; z++)
+            for (int64_t y = lo[1]; y <= hi[1]; y++)
+                for (int64_t x = lo[0]; x <= hi[0]; x++) {
+                    uint64_t lin = ((uint64_t)z * P.bdim[1] + (uint64_t)y) * P.bdim[0] + (uint64_t)x;
+                    atomicOr(&mb[lin >> 5], 1u << (lin & 31));
+                }
     }
-    uint32_t* mb = bits + M.word_off;
-    for (int64_t z = lo[2]; z <= hi[2]; z++)
-        for (int64_t y = lo[1]; y <= hi[1]; y++)
-            for (int64_t x = lo[0]; x <= hi[0]; x++) {
-                uint64_t lin = ((uint64_t)z * P.bdim[1] + (uint64_t)y) * P.bdim[0] + (uint64_t)x;
-                atomicOr(&mb[lin >> 5], 1u << (lin & 31));
-            }
 }
 
 __device__ inline void key_of_lin(const DRun& P, uint64_t lin, int32_t key[3]) {
@@ -271,48 +272,51 @@ __device__ inline void brick_center(const DRun& P, const int32_t key[3], double 
 }
 
 // Classification of touched bricks (lattice.rs:168-183): one wavefront per
-// brick (work[w], a touched-list index, or w without a list); keep[t] = 1
-// where the surface may pass. touched[t] is a bit index of the run's
+// brick, item blockIdx.x of the queue q of touched-list indices; keep[t] =
+// 1 where the surface may pass. touched[t] is a bit index of the run's
 // concatenated bitmaps (wofs: first word of each molecule's). Most touched
 // bricks are settled by the sphere pass alone, so a wavefront per brick
 // beats a shared gather here.
-__global__ void k_classify_bricks(const Mol* __restrict__ mols, const uint64_t* wofs, uint32_t nm,
-                                  const uint32_t* touched, const uint32_t* work, uint32_t n, uint32_t* keep) {
-    if (blockIdx.x >= n) return;
-    uint32_t w = work ? work[blockIdx.x] : blockIdx.x;  // one wavefront per workgroup
-    uint32_t bit = touched[w];
-    uint32_t m = find_mol(wofs, nm, (uint64_t)(bit >> 5));
-    const Mol& M = mols[m];
-    int32_t key[3];
-    key_of_lin(M.P, bit - 32 * wofs[m], key);
-    double c[3];
-    brick_center(M.P, key, c);
-    double rho = 0.5 * sqrt(3.0) * (double)M.P.b * M.P.h;
-    uint8_t s = uniform_sign_wave(M.A, c[0], c[1], c[2], rho);
-    if (lane_id() == 0) keep[w] = s == ST_NONE ? 1u : 0u;
+__global__ void __launch_bounds__(64) k_classify_bricks(const Mol* __restrict__ mols, const uint64_t* wofs,
+                                                        uint32_t nm, const uint32_t* touched, Queue q,
+                                                        uint32_t* keep) {
+    for_each_item<false>(q, nullptr, [&](uint32_t w) {
+        uint32_t bit = touched[w];
+        uint32_t m = find_mol(wofs, nm, (uint64_t)(bit >> 5));
+        const Mol& M = mols[m];
+        int32_t key[3];
+        key_of_lin(M.P, bit - 32 * wofs[m], key);
+        double c[3];
+        brick_center(M.P, key, c);
+        double rho = 0.5 * sqrt(3.0) * (double)M.P.b * M.P.h;
+        uint8_t s = uniform_sign_wave(M.A, c[0], c[1], c[2], rho);
+        if (lane_id() == 0) keep[w] = s == ST_NONE ? 1u : 0u;
+    });
 }
 
 // Sub-block states of surface bricks (kernel.rs:467-482): one wavefront per
-// (brick, sub-block), the bricks listed in work; st[8 * brick + sb]. Used for
-// bricks whose shared-gather lists overflow.
-__global__ void k_sub_states(const Mol* __restrict__ mols, const uint32_t* bmol, const int32_t* keys,
-                             const uint32_t* work, uint32_t n_bricks, uint8_t* st) {
-    uint32_t w = blockIdx.x;  // one wavefront per workgroup
-    if (w >= 8 * n_bricks) return;
-    uint32_t brick = work[w >> 3], i = w & 7;
-    const Mol& M = mols[bmol[brick]];
-    const DRun& P = M.P;
-    double sub = (double)P.sub, h = P.h;
-    double rho = 0.5 * sqrt(3.0) * sub * h;
-    double half = sub * 0.5;
-    double c[3];
-    for (int a = 0; a < 3; a++) {
-        double base = (double)((int64_t)keys[3 * brick + a] * (int64_t)P.b) * h;
-        double off = (double)((i >> a) & 1) * sub + half;
-        c[a] = base + off * h;
-    }
-    uint8_t s = uniform_sign_wave(M.A, c[0], c[1], c[2], rho);
-    if (lane_id() == 0) st[8 * brick + i] = s;
+// brick of the queue q (persistent), its eight sub-blocks in turn; st[8 *
+// brick + sb]. Used for bricks whose shared-gather lists overflow.
+__global__ void __launch_bounds__(64) k_sub_states(const Mol* __restrict__ mols, const uint32_t* bmol,
+                                                   const int32_t* keys, Queue q, uint8_t* st) {
+    __shared__ uint32_t ticket;
+    for_each_item<true>(q, &ticket, [&](uint32_t brick) {
+        const Mol& M = mols[bmol[brick]];
+        const DRun& P = M.P;
+        double sub = (double)P.sub, h = P.h;
+        double rho = 0.5 * sqrt(3.0) * sub * h;
+        double half = sub * 0.5;
+        for (uint32_t i = 0; i < 8; i++) {
+            double c[3];
+            for (int a = 0; a < 3; a++) {
+                double base = (double)((int64_t)keys[3 * brick + a] * (int64_t)P.b) * h;
+                double off = (double)((i >> a) & 1) * sub + half;
+                c[a] = base + off * h;
+            }
+            uint8_t s = uniform_sign_wave(M.A, c[0], c[1], c[2], rho);
+            if (lane_id() == 0) st[8 * brick + i] = s;
+        }
+    });
 }
 
 }  // namespace smg

@@ -82,6 +82,51 @@ struct Mol {
     uint64_t dense_off;  // first entry of its dense brick-index grid
 };
 
+// Work queue of a persistent launch: items 0 .. *count - 1 (entries of
+// `list`, or the indices themselves without one), handed out one per
+// workgroup from the ticket counter `next`, which is zero before the launch.
+// Every workgroup loops until the queue is empty, so the grid is sized by
+// residency, not by the count, and the count may be known only on the
+// device (a stream-captured pipeline never reads it on the host).
+struct Queue {
+    const uint32_t* list;
+    const uint32_t* count;
+    uint32_t* next;
+};
+
+// Takes the workgroup's next item; false once the queue is empty. Every
+// thread of the workgroup calls it (it holds two barriers).
+__device__ __forceinline__ bool queue_take(const Queue& q, uint32_t* ticket, uint32_t& item) {
+    if (threadIdx.x == 0) *ticket = atomicAdd(q.next, 1u);
+    __syncthreads();
+    // The ticket and item are the workgroup's: reading them as wave-uniform
+    // keeps everything derived from them (molecule, lattice, arrangement
+    // pointers) in scalar registers.
+    uint32_t t = __builtin_amdgcn_readfirstlane(*ticket);
+    __syncthreads();
+    if (t >= *q.count) return false;
+    item = __builtin_amdgcn_readfirstlane(q.list ? q.list[t] : t);
+    return true;
+}
+
+// Runs body(item) for the workgroup's items. PERSIST: loops over the queue
+// (a residency-sized grid). Otherwise one item per workgroup, blockIdx.x
+// of the queue's items (a grid of at least the count; workgroups past it do
+// nothing): cheaper for short items and for the first, largest pass.
+template <bool PERSIST, class F>
+__device__ __forceinline__ void for_each_item(const Queue& q, uint32_t* ticket, F&& body) {
+    if (PERSIST) {
+        uint32_t j;
+        while (queue_take(q, ticket, j)) {
+            body(j);
+            __syncthreads();
+        }
+    } else {
+        uint32_t t = blockIdx.x;
+        if (t < *q.count) body(q.list ? q.list[t] : t);
+    }
+}
+
 // The molecule owning item i of a concatenation: the last m < nm with
 // off[m] <= i (off ascending, off[0] = 0).
 template <class T>

@@ -379,26 +379,23 @@ __device__ inline void edgef_finish(const DArr& A, const DRun& P, const EdgeF& E
     edge_finish(A, P, X, G, slot, o);
 }
 
-// k_refine in fast mode: the same work list, lists and outputs.
+// refine_sub in fast mode: the same work, lists and outputs.
 template <bool GLOBAL>
-__global__ void __launch_bounds__(256) k_refine_fast(const Mol* __restrict__ mols, const uint32_t* bmol, RCaps cap,
-                                                     RLayout lay, FLayout flay, const int32_t* keys,
-                                                     const float* values, const uint32_t* work, uint32_t n_work,
-                                                     const unsigned long long* mask, const uint32_t* wpre,
-                                                     uint32_t words, const uint32_t* vbase, VOut out, char* scratch,
-                                                     size_t slice, uint32_t* overflow, uint32_t* n_overflow,
-                                                     unsigned long long* evals_total) {
+__device__ __forceinline__ void refine_sub_fast(const Mol* __restrict__ mols, const uint32_t* bmol, const RCaps& cap,
+                                                const RLayout& lay, const FLayout& flay, const int32_t* keys,
+                                                const float* values, uint32_t j, const unsigned long long* mask,
+                                                const uint32_t* wpre, uint32_t words, const uint32_t* vbase,
+                                                const VOut& out, char* scratch, size_t slice, uint32_t* overflow,
+                                                uint32_t* n_overflow, unsigned long long* evals_total) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     __shared__ uint32_t n[R_N];
     __shared__ uint32_t tmp4[4];
     __shared__ unsigned long long evals_sum;
-    uint32_t w = blockIdx.x;
-    if (w >= n_work) return;
-    uint32_t j = work[w], brick = j >> 3, sb = j & 7;
+    uint32_t brick = j >> 3, sb = j & 7;
     const uint32_t mol = bmol[brick];
     const DArr& A = mols[mol].A;
     const DRun& P = mols[mol].P;
-    char* base = GLOBAL ? scratch + (size_t)w * slice : smem;
+    char* base = GLOBAL ? scratch + (size_t)blockIdx.x * slice : smem;
     RLists s = rcarve(base, lay);
     FLists f = fcarve(base, flay);
     const int32_t* key = keys + 3 * brick;
@@ -479,6 +476,20 @@ __global__ void __launch_bounds__(256) k_refine_fast(const Mol* __restrict__ mol
         __syncthreads();
         if (threadIdx.x == 0 && evals_sum) atomicAdd(&evals_total[mol], evals_sum);
     }
+}
+
+template <bool GLOBAL, bool PERSIST>
+__global__ void __launch_bounds__(256) k_refine_fast(const Mol* __restrict__ mols, const uint32_t* bmol, RCaps cap,
+                                                     RLayout lay, FLayout flay, const int32_t* keys,
+                                                     const float* values, Queue q, const unsigned long long* mask,
+                                                     const uint32_t* wpre, uint32_t words, const uint32_t* vbase,
+                                                     VOut out, char* scratch, size_t slice, uint32_t* overflow,
+                                                     uint32_t* n_overflow, unsigned long long* evals_total) {
+    __shared__ uint32_t ticket;
+    for_each_item<PERSIST>(q, &ticket, [&](uint32_t j) {
+        refine_sub_fast<GLOBAL>(mols, bmol, cap, lay, flay, keys, values, j, mask, wpre, words, vbase, out, scratch,
+                                slice, overflow, n_overflow, evals_total);
+    });
 }
 
 }  // namespace smg

@@ -39,44 +39,47 @@ __device__ inline bool edge_crossing(const float* val, uint32_t b, uint32_t side
     return (val[i0] < 0.0f) != (val[i0 + st] < 0.0f);
 }
 
-// Per brick (one workgroup of 256): crossing-edge masks (edge ids
-// ((z*b + y)*b + x)*3 + axis, 64 per word), per-word prefix counts, brick
-// total and per-sub-block counts (mesh/extract.rs:176-216).
-__global__ void __launch_bounds__(256) k_edges(uint32_t b, const float* values, uint32_t n_bricks, uint32_t words,
-                                               unsigned long long* mask, uint32_t* wpre, uint32_t* bcount,
-                                               uint32_t* scount) {
+// Per brick (one workgroup of 256, striding over the *n_bricks bricks):
+// crossing-edge masks (edge ids ((z*b + y)*b + x)*3 + axis, 64 per word),
+// per-word prefix counts, brick total and per-sub-block counts
+// (mesh/extract.rs:176-216).
+__global__ void __launch_bounds__(256) k_edges(uint32_t b, const float* values, const uint32_t* n_bricks,
+                                               uint32_t words, unsigned long long* mask, uint32_t* wpre,
+                                               uint32_t* bcount, uint32_t* scount) {
     __shared__ uint32_t wc[2048];
     __shared__ uint32_t sc[8];
-    uint32_t brick = blockIdx.x;
-    if (brick >= n_bricks) return;
-    const uint32_t side = b + 1, sub = b / 2;
-    const float* val = values + (size_t)brick * side * side * side;
-    uint32_t nid = 3 * b * b * b, lane = lane_id(), wv = threadIdx.x / WAVE;
-    if (threadIdx.x < 8) sc[threadIdx.x] = 0;
-    __syncthreads();
-    for (uint32_t w = wv; w < words; w += blockDim.x / WAVE) {
-        uint32_t id = w * 64 + lane, x = 0, y = 0, z = 0, axis = 0;
-        bool c = id < nid && edge_crossing(val, b, side, id, x, y, z, axis);
-        unsigned long long m = __ballot(c);
-        if (c) {
-            uint32_t s = min(x / sub, 1u) + 2 * min(y / sub, 1u) + 4 * min(z / sub, 1u);
-            atomicAdd(&sc[s], 1u);
+    const uint32_t nb = *n_bricks;
+    for (uint32_t brick = blockIdx.x; brick < nb; brick += gridDim.x) {
+        const uint32_t side = b + 1, sub = b / 2;
+        const float* val = values + (size_t)brick * side * side * side;
+        uint32_t nid = 3 * b * b * b, lane = lane_id(), wv = threadIdx.x / WAVE;
+        if (threadIdx.x < 8) sc[threadIdx.x] = 0;
+        __syncthreads();
+        for (uint32_t w = wv; w < words; w += blockDim.x / WAVE) {
+            uint32_t id = w * 64 + lane, x = 0, y = 0, z = 0, axis = 0;
+            bool c = id < nid && edge_crossing(val, b, side, id, x, y, z, axis);
+            unsigned long long m = __ballot(c);
+            if (c) {
+                uint32_t s = min(x / sub, 1u) + 2 * min(y / sub, 1u) + 4 * min(z / sub, 1u);
+                atomicAdd(&sc[s], 1u);
+            }
+            if (lane == 0) {
+                mask[(size_t)brick * words + w] = m;
+                wc[w] = (uint32_t)__popcll(m);
+            }
         }
-        if (lane == 0) {
-            mask[(size_t)brick * words + w] = m;
-            wc[w] = (uint32_t)__popcll(m);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t acc = 0;
+            for (uint32_t w = 0; w < words; w++) {
+                wpre[(size_t)brick * words + w] = acc;
+                acc += wc[w];
+            }
+            bcount[brick] = acc;
         }
+        if (threadIdx.x < 8) scount[8 * brick + threadIdx.x] = sc[threadIdx.x];
+        __syncthreads();
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        for (uint32_t w = 0; w < words; w++) {
-            wpre[(size_t)brick * words + w] = acc;
-            acc += wc[w];
-        }
-        bcount[brick] = acc;
-    }
-    if (threadIdx.x < 8) scount[8 * brick + threadIdx.x] = sc[threadIdx.x];
 }
 
 __device__ inline uint32_t edge_rank(const unsigned long long* mask, const uint32_t* wpre, uint32_t words,
@@ -505,6 +508,7 @@ struct VOut {
     uint8_t* patch;
     double* t;
     uint32_t* evals;
+    size_t cap;  // vertices the arrays hold; slots beyond are not written
 };
 
 // State of one edge's root search (mesh/extract.rs:425-535): up to 8
@@ -626,7 +630,7 @@ __device__ inline bool edge_step(EdgeState& E, const EvalD& e, double h) {
 // the group's rank-0 lane writes them.
 __device__ inline void edge_finish(const DArr& A, const DRun& P, const EdgeState& E, const Group& G, size_t slot,
                                    const VOut& o) {
-    if (G.r != 0) return;
+    if (G.r != 0 || slot >= o.cap) return;
     const double t = E.t;
     const EvalD& last = E.last;
     EdgeGeom Q = edge_geom(E, P.h);
@@ -708,30 +712,27 @@ __device__ inline void edge_finish(const DArr& A, const DRun& P, const EdgeState
     if (o.evals) o.evals[slot] = (E.st >> 16) | ((E.st & (1u << 11)) ? 0x80000000u : 0u);
 }
 
-// One workgroup (256 threads) per sub-block with crossing edges (`work[w]`
-// = 8*brick + sb): gather_local into LDS (GLOBAL false) or into scratch
-// slice w (GLOBAL true), compact the crossing edges whose lower sample lies
-// in the sub-block, then refine them by lane groups. Sub-blocks whose lists
-// overflow go to `overflow`.
+// One workgroup (256 threads) per sub-block with crossing edges (j =
+// 8*brick + sb): gather_local into LDS (GLOBAL false) or into scratch slice
+// blockIdx.x (GLOBAL true), compact the crossing edges whose lower sample
+// lies in the sub-block, then refine them by lane groups. Sub-blocks whose
+// lists overflow go to `overflow`.
 template <bool GLOBAL>
-__global__ void __launch_bounds__(256) k_refine(const Mol* __restrict__ mols, const uint32_t* bmol, RCaps cap,
-                                                RLayout lay, const int32_t* keys, const float* values,
-                                                const uint32_t* work, uint32_t n_work,
-                                                const unsigned long long* mask, const uint32_t* wpre,
-                                                uint32_t words, const uint32_t* vbase, VOut out, char* scratch,
-                                                size_t slice, uint32_t* overflow, uint32_t* n_overflow,
-                                                unsigned long long* evals_total) {
+__device__ __forceinline__ void refine_sub(const Mol* __restrict__ mols, const uint32_t* bmol, const RCaps& cap,
+                                           const RLayout& lay, const int32_t* keys, const float* values, uint32_t j,
+                                           const unsigned long long* mask, const uint32_t* wpre, uint32_t words,
+                                           const uint32_t* vbase, const VOut& out, char* scratch, size_t slice,
+                                           uint32_t* overflow, uint32_t* n_overflow,
+                                           unsigned long long* evals_total) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     __shared__ uint32_t n[R_N];
     __shared__ uint32_t tmp4[4];
     __shared__ unsigned long long evals_sum;
-    uint32_t w = blockIdx.x;
-    if (w >= n_work) return;
-    uint32_t j = work[w], brick = j >> 3, sb = j & 7;
+    uint32_t brick = j >> 3, sb = j & 7;
     const uint32_t mol = bmol[brick];
     const DArr& A = mols[mol].A;
     const DRun& P = mols[mol].P;
-    char* base = GLOBAL ? scratch + (size_t)w * slice : smem;
+    char* base = GLOBAL ? scratch + (size_t)blockIdx.x * slice : smem;
     RLists s = rcarve(base, lay);
     const int32_t* key = keys + 3 * brick;
     const uint32_t b = P.b, sub = P.sub;
@@ -818,6 +819,22 @@ __global__ void __launch_bounds__(256) k_refine(const Mol* __restrict__ mols, co
         __syncthreads();
         if (threadIdx.x == 0 && evals_sum) atomicAdd(&evals_total[mol], evals_sum);
     }
+}
+
+// PERSIST: residency-sized grid looping over the queue; otherwise one
+// sub-block per workgroup.
+template <bool GLOBAL, bool PERSIST>
+__global__ void __launch_bounds__(256) k_refine(const Mol* __restrict__ mols, const uint32_t* bmol, RCaps cap,
+                                                RLayout lay, const int32_t* keys, const float* values, Queue q,
+                                                const unsigned long long* mask, const uint32_t* wpre,
+                                                uint32_t words, const uint32_t* vbase, VOut out, char* scratch,
+                                                size_t slice, uint32_t* overflow, uint32_t* n_overflow,
+                                                unsigned long long* evals_total) {
+    __shared__ uint32_t ticket;
+    for_each_item<PERSIST>(q, &ticket, [&](uint32_t j) {
+        refine_sub<GLOBAL>(mols, bmol, cap, lay, keys, values, j, mask, wpre, words, vbase, out, scratch, slice,
+                           overflow, n_overflow, evals_total);
+    });
 }
 
 }  // namespace smg

@@ -77,18 +77,20 @@ __device__ inline uint32_t pack_states(const uint8_t* st) {
     return p;
 }
 
-// Samples of brick `brick` owned by no sub-block get +-band (kernel.rs:533-534).
+// Samples owned by no sub-block get +-band (kernel.rs:533-534); one
+// workgroup per brick, striding over the *n_bricks bricks.
 __global__ void k_assign_uniform(const Mol* __restrict__ mols, const uint32_t* bmol, const uint8_t* st,
-                                 uint32_t n_bricks, float* values) {
-    uint32_t brick = blockIdx.x;
-    if (brick >= n_bricks) return;
-    const DRun& P = mols[bmol[brick]].P;
-    uint32_t s = pack_states(st + 8 * brick);
-    float* out = values + (size_t)brick * P.spb;
-    for (uint32_t i = threadIdx.x; i < P.spb; i += blockDim.x) {
-        uint32_t x = i % P.side, y = (i / P.side) % P.side, z = i / (P.side * P.side);
-        bool positive;
-        if (sample_owner(s, P.sub, x, y, z, positive) < 0) out[i] = positive ? P.fband : -P.fband;
+                                 const uint32_t* n_bricks, float* values) {
+    const uint32_t nb = *n_bricks;
+    for (uint32_t brick = blockIdx.x; brick < nb; brick += gridDim.x) {
+        const DRun& P = mols[bmol[brick]].P;
+        uint32_t s = pack_states(st + 8 * brick);
+        float* out = values + (size_t)brick * P.spb;
+        for (uint32_t i = threadIdx.x; i < P.spb; i += blockDim.x) {
+            uint32_t x = i % P.side, y = (i / P.side) % P.side, z = i / (P.side * P.side);
+            bool positive;
+            if (sample_owner(s, P.sub, x, y, z, positive) < 0) out[i] = positive ? P.fband : -P.fband;
+        }
     }
 }
 
@@ -341,29 +343,26 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
     return true;
 }
 
-// Samples one sub-block per workgroup. `work[w]` = 8 * brick + sb of a
-// non-uniform sub-block. The lists live in dynamic LDS (GLOBAL false) or in
-// scratch slice w of `slice` bytes (GLOBAL true). Sub-blocks whose lists do
+// Samples one sub-block per workgroup: j = 8 * brick + sb of a non-uniform
+// sub-block. The lists live in dynamic LDS (GLOBAL false) or in scratch
+// slice blockIdx.x of `slice` bytes (GLOBAL true). Sub-blocks whose lists do
 // not fit are appended to `overflow`.
 template <bool GLOBAL>
-__global__ void __launch_bounds__(256) k_sample(const Mol* __restrict__ mols, const uint32_t* bmol, Caps cap,
-                                                Layout lay, const int32_t* keys, const uint8_t* st,
-                                                const uint32_t* work, uint32_t n_work, float* values, char* scratch,
-                                                size_t slice, uint32_t* overflow, uint32_t* n_overflow,
-                                                unsigned long long* n_samples) {
+__device__ __forceinline__ void sample_sub(const Mol* __restrict__ mols, const uint32_t* bmol, const Caps& cap,
+                                           const Layout& lay, const int32_t* keys, const uint8_t* st, uint32_t j,
+                                           float* values, char* scratch, size_t slice, uint32_t* overflow,
+                                           uint32_t* n_overflow, unsigned long long* n_samples) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     __shared__ uint32_t n[C_N];
     __shared__ uint32_t tmp4[4];
     __shared__ uint32_t owned[512];
     __shared__ uint32_t n_owned;
-    uint32_t w = blockIdx.x;
-    if (w >= n_work) return;
-    uint32_t j = work[w], brick = j >> 3, sb = j & 7;
+    uint32_t brick = j >> 3, sb = j & 7;
     const uint32_t m = bmol[brick];
     const DArr& A = mols[m].A;
     const DLoc& Lc = mols[m].L;
     const DRun& P = mols[m].P;
-    char* base = GLOBAL ? scratch + (size_t)w * slice : smem;
+    char* base = GLOBAL ? scratch + (size_t)blockIdx.x * slice : smem;
     Lists s = carve(base, lay);
     const int32_t* key = keys + 3 * brick;
     if (threadIdx.x == 0) n_owned = 0;
@@ -404,6 +403,20 @@ __global__ void __launch_bounds__(256) k_sample(const Mol* __restrict__ mols, co
         if (threadIdx.x == 0) n_owned = 0;
         __syncthreads();
     }
+}
+
+// PERSIST: residency-sized grid looping over the queue; otherwise one
+// sub-block per workgroup.
+template <bool GLOBAL, bool PERSIST>
+__global__ void __launch_bounds__(256) k_sample(const Mol* __restrict__ mols, const uint32_t* bmol, Caps cap,
+                                                Layout lay, const int32_t* keys, const uint8_t* st, Queue q,
+                                                float* values, char* scratch, size_t slice, uint32_t* overflow,
+                                                uint32_t* n_overflow, unsigned long long* n_samples) {
+    __shared__ uint32_t ticket;
+    for_each_item<PERSIST>(q, &ticket, [&](uint32_t j) {
+        sample_sub<GLOBAL>(mols, bmol, cap, lay, keys, st, j, values, scratch, slice, overflow, n_overflow,
+                           n_samples);
+    });
 }
 
 }  // namespace smg

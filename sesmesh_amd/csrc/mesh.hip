@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -48,9 +49,19 @@ namespace {
 
 constexpr uint32_t SCAN_BLOCK = 256, SCAN_ITEMS = 8, SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
 
-// Exclusive scan of tiles: out[i] = sum(in[tile start..i)), sums[tile].
-__global__ void __launch_bounds__(256) k_scan_tiles(const uint32_t* in, uint32_t* out, uint32_t* sums, size_t n) {
+// Counts live in device memory (a captured run never reads them on the
+// host); kernels over a counted range stride over it with whatever grid
+// they were given.
+#define GRID_STRIDE(i, n) \
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < (size_t)(n); i += (size_t)gridDim.x * blockDim.x)
+
+// Exclusive scan of in[0 .. *n), tile pass: out[i] = sum(in[tile start..i)),
+// sums[tile]. Tiles at or past the count do nothing.
+__global__ void __launch_bounds__(256) k_scan_tiles(const uint32_t* in, uint32_t* out, uint32_t* sums,
+                                                    const uint32_t* n_dev) {
     __shared__ uint32_t tmp[4];
+    const size_t n = *n_dev;
+    if ((size_t)blockIdx.x * SCAN_TILE >= n) return;
     size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
     uint32_t v[SCAN_ITEMS], acc = 0;
     for (uint32_t k = 0; k < SCAN_ITEMS; k++) {
@@ -66,9 +77,11 @@ __global__ void __launch_bounds__(256) k_scan_tiles(const uint32_t* in, uint32_t
     if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
-// In-place exclusive scan of the tile sums by one block; total at sums[n].
-__global__ void __launch_bounds__(256) k_scan_sums(uint32_t* sums, size_t n) {
+// In-place exclusive scan of the tile sums of the count *n by one block;
+// the grand total goes to *total.
+__global__ void __launch_bounds__(256) k_scan_sums(uint32_t* sums, const uint32_t* n_dev, uint32_t* total_out) {
     __shared__ uint32_t tmp[4];
+    const size_t n = ((size_t)*n_dev + SCAN_TILE - 1) / SCAN_TILE;
     uint32_t carry = 0;
     for (size_t c0 = 0; c0 < n; c0 += SCAN_TILE) {
         size_t base = c0 + (size_t)threadIdx.x * SCAN_ITEMS;
@@ -85,104 +98,110 @@ __global__ void __launch_bounds__(256) k_scan_sums(uint32_t* sums, size_t n) {
         }
         carry += total;
     }
-    if (threadIdx.x == 0) sums[n] = carry;
+    if (threadIdx.x == 0) *total_out = carry;
 }
 
-__global__ void k_scan_add(uint32_t* out, const uint32_t* sums, size_t n) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] += sums[i / SCAN_TILE];
+__global__ void k_scan_add(uint32_t* out, const uint32_t* sums, const uint32_t* n_dev) {
+    GRID_STRIDE(i, *n_dev) out[i] += sums[i / SCAN_TILE];
 }
 
-__global__ void k_popcount(const uint32_t* bits, uint32_t* cnt, size_t n) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) cnt[i] = (uint32_t)__popc(bits[i]);
+__global__ void k_popcount(const uint32_t* bits, uint32_t* cnt, const uint32_t* n) {
+    GRID_STRIDE(i, *n) cnt[i] = (uint32_t)__popc(bits[i]);
 }
 
 // Set bits of the touched bitmap, in ascending order, as linear indices.
-__global__ void k_bits_to_list(const uint32_t* bits, const uint32_t* off, size_t n, uint32_t* out) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t w = bits[i], at = off[i];
-    while (w) {
-        uint32_t bit = __ffs(w) - 1;
-        out[at++] = (uint32_t)(i * 32 + bit);
-        w &= w - 1;
+__global__ void k_bits_to_list(const uint32_t* bits, const uint32_t* off, const uint32_t* n, uint32_t* out) {
+    GRID_STRIDE(i, *n) {
+        uint32_t w = bits[i], at = off[i];
+        while (w) {
+            uint32_t bit = __ffs(w) - 1;
+            out[at++] = (uint32_t)(i * 32 + bit);
+            w &= w - 1;
+        }
     }
 }
 
 // Surface bricks: keys (molecule lattice) in order, their molecules, and
-// each molecule's dense index grid.
+// each molecule's dense index grid. Bricks at or past `cap` are dropped
+// (the caller has clamped the count and flagged the run).
 __global__ void k_compact_bricks(const Mol* __restrict__ mols, const uint64_t* wofs, uint32_t nm,
-                                 const uint32_t* touched, const uint32_t* keep, const uint32_t* off, uint32_t n,
-                                 int32_t* keys, uint32_t* bmol, uint32_t* dense) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !keep[i]) return;
-    uint32_t at = off[i], bit = touched[i];
-    uint32_t m = find_mol(wofs, nm, (uint64_t)(bit >> 5));
-    uint64_t lin = bit - 32 * wofs[m];
-    int32_t key[3];
-    key_of_lin(mols[m].P, lin, key);
-    keys[3 * at] = key[0];
-    keys[3 * at + 1] = key[1];
-    keys[3 * at + 2] = key[2];
-    bmol[at] = m;
-    dense[mols[m].dense_off + lin] = at;
+                                 const uint32_t* touched, const uint32_t* keep, const uint32_t* off,
+                                 const uint32_t* n, uint32_t cap, int32_t* keys, uint32_t* bmol, uint32_t* dense) {
+    GRID_STRIDE(i, *n) {
+        if (!keep[i] || off[i] >= cap) continue;
+        uint32_t at = off[i], bit = touched[i];
+        uint32_t m = find_mol(wofs, nm, (uint64_t)(bit >> 5));
+        uint64_t lin = bit - 32 * wofs[m];
+        int32_t key[3];
+        key_of_lin(mols[m].P, lin, key);
+        keys[3 * at] = key[0];
+        keys[3 * at + 1] = key[1];
+        keys[3 * at + 2] = key[2];
+        bmol[at] = m;
+        dense[mols[m].dense_off + lin] = at;
+    }
 }
 
 // First touched brick and first surface brick of every molecule
 // (rows[m] for m <= nm; row nm holds the totals).
-__global__ void k_mol_ranges(const uint64_t* wofs, uint32_t nm, const uint32_t* woff, uint64_t nwords,
-                             uint32_t ntouched, const uint32_t* koff, uint32_t nb, uint32_t* tfirst,
+__global__ void k_mol_ranges(const uint64_t* wofs, uint32_t nm, const uint32_t* woff, const uint32_t* nwords,
+                             const uint32_t* ntouched, const uint32_t* koff, const uint32_t* nb, uint32_t* tfirst,
                              uint32_t* fbrick) {
-    uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m > nm) return;
-    uint32_t tf = (m < nm && wofs[m] < nwords) ? woff[wofs[m]] : ntouched;
-    tfirst[m] = tf;
-    fbrick[m] = tf < ntouched ? koff[tf] : nb;
+    GRID_STRIDE(m, nm + 1) {
+        uint32_t nt = *ntouched;
+        uint32_t tf = (m < nm && wofs[m] < *nwords) ? woff[wofs[m]] : nt;
+        tfirst[m] = tf;
+        fbrick[m] = tf < nt ? min(koff[tf], *nb) : *nb;
+    }
 }
 
 // Where each molecule's sub-blocks to sample, vertices and triangles start.
-__global__ void k_mol_summary(uint32_t nm, const uint32_t* fbrick, uint32_t nb, const uint32_t* soff, uint32_t nsw,
-                              const uint32_t* vbase, uint32_t nverts, const uint32_t* tbase, uint32_t ntris,
-                              uint32_t* sfirst, uint32_t* vfirst, uint32_t* trfirst) {
-    uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m > nm) return;
-    uint32_t fb = fbrick[m];
-    sfirst[m] = fb < nb ? soff[8 * fb] : nsw;
-    vfirst[m] = fb < nb ? vbase[fb] : nverts;
-    trfirst[m] = fb < nb ? tbase[fb] : ntris;
+__global__ void k_mol_summary(uint32_t nm, const uint32_t* fbrick, const uint32_t* nb, const uint32_t* soff,
+                              const uint32_t* nsw, const uint32_t* vbase, const uint32_t* nverts,
+                              const uint32_t* tbase, const uint32_t* ntris, uint32_t* sfirst, uint32_t* vfirst,
+                              uint32_t* trfirst) {
+    GRID_STRIDE(m, nm + 1) {
+        uint32_t fb = fbrick[m];
+        bool in = fb < *nb;
+        sfirst[m] = in ? soff[8 * fb] : *nsw;
+        vfirst[m] = in ? min(vbase[fb], *nverts) : *nverts;
+        trfirst[m] = in ? min(tbase[fb], *ntris) : *ntris;
+    }
 }
 
-__global__ void k_iota(uint32_t* p, uint32_t n) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = i;
+// p[i] = v for i below n, or below *n_dev when given.
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n, const uint32_t* n_dev) {
+    GRID_STRIDE(i, n_dev ? (size_t)*n_dev : n) p[i] = v;
 }
 
-__global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
+// *n = min(*n, cap), setting `bit` in *flags when it was larger; out2, when
+// given, receives k * the clamped count.
+__global__ void k_clamp(uint32_t* n, uint32_t cap, uint32_t* flags, uint32_t bit, uint32_t* out2, uint32_t k) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t v = *n;
+    if (v > cap) {
+        v = cap;
+        *n = cap;
+        atomicOr(flags, bit);
+    }
+    if (out2) *out2 = k * v;
 }
 
 // Flags of sub-blocks to sample (state NONE) or to refine (edges > 0).
-__global__ void k_flag_states(const uint8_t* st, uint32_t n, uint32_t* flag) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) flag[i] = st[i] == ST_NONE ? 1u : 0u;
+__global__ void k_flag_states(const uint8_t* st, const uint32_t* n, uint32_t* flag) {
+    GRID_STRIDE(i, *n) flag[i] = st[i] == ST_NONE ? 1u : 0u;
 }
-__global__ void k_flag_nonzero(const uint32_t* c, uint32_t n, uint32_t* flag) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) flag[i] = c[i] ? 1u : 0u;
+__global__ void k_flag_nonzero(const uint32_t* c, const uint32_t* n, uint32_t* flag) {
+    GRID_STRIDE(i, *n) flag[i] = c[i] ? 1u : 0u;
 }
-__global__ void k_compact_index(const uint32_t* flag, const uint32_t* off, uint32_t n, uint32_t* out) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && flag[i]) out[off[i]] = i;
+__global__ void k_compact_index(const uint32_t* flag, const uint32_t* off, const uint32_t* n, uint32_t* out) {
+    GRID_STRIDE(i, *n) if (flag[i]) out[off[i]] = (uint32_t)i;
 }
 
 // Frame-local f32 copies (field/gpu.rs:253-318, kernel.rs:627-637, 717-746);
 // one thread per index below max(spheres, circles, vertices) of each
 // molecule (loc_off: first index of each molecule).
-__global__ void k_localize(const Mol* __restrict__ mols, const uint64_t* loc_off, uint32_t nm) {
-    uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= loc_off[nm]) return;
+__device__ inline void localize_one(const Mol* __restrict__ mols, const uint64_t* loc_off, uint32_t nm, uint64_t g) {
     uint32_t m = find_mol(loc_off, nm, g);
     const DArr& A = mols[m].A;
     const DLoc& L = mols[m].L;
@@ -216,6 +235,10 @@ __global__ void k_localize(const Mol* __restrict__ mols, const uint64_t* loc_off
         L.vy[i] = (float)(A.vy[i] - fy);
         L.vz[i] = (float)(A.vz[i] - fz);
     }
+}
+
+__global__ void k_localize(const Mol* __restrict__ mols, const uint64_t* loc_off, uint32_t nm) {
+    GRID_STRIDE(g, loc_off[nm]) localize_one(mols, loc_off, nm, g);
 }
 
 // Self-test operations (reference crates/sesmesh-gpu/src/selftest.rs:25-38
@@ -252,6 +275,10 @@ struct smg_device {
     hipStream_t stream = nullptr;
     hipEvent_t ev[16] = {};
     uint32_t* host_u32 = nullptr;  // pinned staging for counts
+    // Workgroups of 256 (or 64) threads resident on the whole device, per
+    // kernel and dynamic LDS size: the grid of a persistent launch.
+    std::map<std::pair<const void*, size_t>, uint32_t> resident;
+    std::mutex mu;
 };
 
 struct smg_arrangement {
@@ -285,7 +312,11 @@ struct smg_mesh {
     uint32_t spb = 0;  // samples per brick
     std::vector<smg_mesh_info> info;
     std::vector<uint32_t> bfirst, vfirst, tfirst;
-    // Device arrays owned by the mesh.
+    // Device arrays, owned by the mesh unless it views a plan's buffers
+    // (then `plan` holds the view; a view whose plan was freed is detached
+    // and has no arrays).
+    smg_plan* plan = nullptr;
+    bool detached = false;
     int32_t* keys = nullptr;
     float* values = nullptr;
     float* pos = nullptr;
@@ -295,11 +326,7 @@ struct smg_mesh {
     double* t = nullptr;
     uint32_t* vevals = nullptr;
     uint32_t* tris = nullptr;
-    ~smg_mesh() {
-        void* ps[] = {keys, values, pos, nrm, atom, patch, t, vevals, tris};
-        for (void* p : ps)
-            if (p) (void)hipFree(p);
-    }
+    ~smg_mesh();
 };
 
 // ------------------------------------------------------------------ device
@@ -685,7 +712,7 @@ extern "C" int smg_field_clamped(smg_resident* r, const double* points, size_t n
 // ------------------------------------------------------------- mesh stage
 namespace {
 
-// Temporary device buffers of one run, freed at the end.
+// Temporary device buffers of one run, freed at the end (stream-ordered).
 struct Temps {
     std::vector<void*> ptrs;
     hipStream_t s;
@@ -693,31 +720,12 @@ struct Temps {
     ~Temps() {
         for (void* p : ptrs) (void)hipFreeAsync(p, s);
     }
-    template <class T>
-    hipError_t get(T** p, size_t count) {
-        hipError_t e = hipMallocAsync((void**)p, std::max<size_t>(count, 1) * sizeof(T), s);
+    hipError_t get(void** p, size_t bytes) {
+        hipError_t e = hipMallocAsync(p, std::max<size_t>(bytes, 1), s);
         if (e == hipSuccess) ptrs.push_back(*p);
         return e;
     }
 };
-
-// Exclusive scan of n u32 values into `out`; *total_dev receives the device
-// address of the total.
-int scan_u32(hipStream_t s, Temps& T, const uint32_t* in, uint32_t* out, size_t n, uint32_t** total_dev) {
-    size_t tiles = std::max<size_t>((n + SCAN_TILE - 1) / SCAN_TILE, 1);
-    uint32_t* sums;
-    HIP_TRY(T.get(&sums, tiles + 1));
-    if (n) {
-        k_scan_tiles<<<(uint32_t)tiles, SCAN_BLOCK, 0, s>>>(in, out, sums, n);
-        k_scan_sums<<<1, SCAN_BLOCK, 0, s>>>(sums, tiles);
-        k_scan_add<<<blocks_for(n, 256), 256, 0, s>>>(out, sums, n);
-    } else {
-        k_fill_u32<<<1, 64, 0, s>>>(sums + tiles, 0u, 1);
-    }
-    HIP_TRY(hipGetLastError());
-    *total_dev = sums + tiles;
-    return SMG_OK;
-}
 
 int read_u32(smg_device* d, const uint32_t* dev_ptr, uint32_t& v) {
     HIP_TRY(hipMemcpyAsync(d->host_u32, dev_ptr, 4, hipMemcpyDeviceToHost, d->stream));
@@ -746,52 +754,110 @@ constexpr Caps SAMPLE_CAPS[TIERS] = {{128, 64, 256, 64, 256},
 constexpr RCaps REFINE_CAPS[TIERS] = {
     {128, 64, 256, 256, 0}, {256, 128, 384, 512, 0}, {640, 256, 704, 1024, 0}, {1024, 512, 4096, 4096, 0}};
 // Sub-block classification: one LDS tier; bricks beyond it take one
-// wavefront per ball, which beats larger tiers at one or two workgroups per CU.
-constexpr RCaps UNIFORM_CAPS[1] = {{192, 128, 256, 512, 0, 1}};
+// wavefront per brick, which beats larger tiers at one or two workgroups per CU.
+constexpr RCaps UNIFORM_CAPS = {192, 128, 256, 512, 0, 1};
 constexpr uint32_t LDS_EDGES = 512;      // crossing-edge buffer of the LDS passes
-constexpr uint32_t SCRATCH_CHUNK = 256;   // workgroups per scratch launch
+constexpr uint32_t SCRATCH_CHUNK = 256;   // workgroups of a scratch pass (one slice each)
 constexpr uint32_t STATIC_LDS = 6144;     // static LDS of k_sample / k_refine, bound
+constexpr uint32_t NO_CAP = 0xffffffffu;
 
-// Uniform-sign classification of `n` units (touched bricks or surface
-// bricks): shared-gather workgroups with the LDS capacities in turn, then
-// one wavefront per ball for units whose lists still overflow. Records
-// `done` after the first pass (the kernel's time).
-template <uint32_t MODE>
-int run_uniform(smg_device* d, Temps& T, const Mol* mols, const uint64_t* wofs, uint32_t nm, const uint32_t* touched,
-                const uint32_t* bmol, const int32_t* keys, uint32_t n, uint32_t* keep, uint8_t* st, hipEvent_t done) {
-    hipStream_t s = d->stream;
-    const uint32_t* list = nullptr;
-    uint32_t count = n;
-    for (int pass = 0; pass < 1 && count; pass++) {
-        const RCaps& cap = UNIFORM_CAPS[pass];
-        RLayout lay = make_rlayout(cap);
-        if (lay.bytes + STATIC_LDS > d->prop.sharedMemPerBlock) continue;
-        uint32_t *next, *nnext;
-        HIP_TRY(T.get(&next, count));
-        HIP_TRY(T.get(&nnext, 1));
-        HIP_TRY(hipMemsetAsync(nnext, 0, 4, s));
-        k_uniform<MODE, false><<<count, 256, lay.bytes, s>>>(mols, wofs, nm, touched, bmol, keys, cap, lay, list, count,
-                                                             keep, st, nullptr, 0, next, nnext);
-        HIP_TRY(hipGetLastError());
-        if (pass == 0) HIP_TRY(hipEventRecord(done, s));
-        TRY(read_u32(d, nnext, count));
-        list = next;
-    }
-    if (count) {
-        if (!list) {  // no LDS pass fits: every unit
-            uint32_t* all;
-            HIP_TRY(T.get(&all, count));
-            k_iota<<<blocks_for(count, 256), 256, 0, s>>>(all, count);
-            list = all;
-            HIP_TRY(hipEventRecord(done, s));
+// Device counters of a run (u32 slots, zero at the start of the run).
+enum : uint32_t {
+    K_TOUCHED, K_NB, K_NSUB, K_NSW, K_NV, K_NRW, K_NT, K_FLAGS, K_ERR, K_UOVER,
+    K_SOVER0, K_ROVER0 = K_SOVER0 + TIERS,
+    K_Q_SUBST = K_ROVER0 + TIERS, K_Q_SAMPLE0, K_Q_REFINE0 = K_Q_SAMPLE0 + TIERS,
+    K_N = K_Q_REFINE0 + TIERS
+};
+// K_FLAGS bits: a device count exceeded its capacity.
+enum : uint32_t { OVER_BRICKS = 1, OVER_VERTICES = 2, OVER_TRIANGLES = 4, OVER_TABLE = 8 };
+
+// Run-wide extents the kernels read from the uploaded table.
+struct RunHdr {
+    uint32_t nwords, dense, locals, spheres;
+};
+
+inline uint32_t g1(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(n, 1u << 30)); }
+inline uint32_t stride_grid(uint64_t n, uint32_t bs = 256) { return g1(std::min<uint64_t>(blocks_for(n, bs), 8192)); }
+
+// Workgroups of `block` threads of `kernel` with `lds` dynamic LDS bytes that
+// the whole device holds at once.
+uint32_t resident(smg_device* d, const void* kernel, uint32_t block, size_t lds) {
+    std::lock_guard<std::mutex> lk(d->mu);
+    auto key = std::make_pair(kernel, lds * 1024 + block);
+    auto it = d->resident.find(key);
+    if (it != d->resident.end()) return it->second;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, (int)block, lds) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    uint32_t n = (uint32_t)per_cu * (uint32_t)std::max(d->prop.multiProcessorCount, 1);
+    d->resident[key] = n;
+    return n;
+}
+
+// Where a run's buffers come from and how it learns its sizes.
+// Exact: a size is read back from the device when a buffer depends on it
+// (three synchronisations), temporaries come from the stream-ordered pool
+// and the mesh owns its outputs. Planned: sizes are a plan's capacities and
+// nothing is read back before the end, so the whole run can be captured
+// into a graph; every buffer is a plan slot, allocated on the plan's first
+// run and reused, in the same order, by every later run and by the capture.
+struct Run {
+    smg_device* d = nullptr;
+    hipStream_t s = nullptr;
+    Temps* T = nullptr;                       // exact
+    std::vector<void*>* slots = nullptr;      // planned
+    std::vector<size_t>* slot_bytes = nullptr;
+    size_t next_slot = 0;
+    float* lbuf = nullptr;  // the run's f32-copy buffer (its table points into it)
+    bool planned() const { return slots != nullptr; }
+    int slot(void** p, size_t bytes) {
+        if (next_slot < slots->size()) {
+            if ((*slot_bytes)[next_slot] != bytes) return fail(SMG_E_INTERNAL, "plan buffers changed between runs");
+            *p = (*slots)[next_slot++];
+            return SMG_OK;
         }
-        if (MODE == UNIFORM_TOUCHED)
-            k_classify_bricks<<<count, WAVE, 0, s>>>(mols, wofs, nm, touched, list, count, keep);
-        else
-            k_sub_states<<<8 * count, WAVE, 0, s>>>(mols, bmol, keys, list, count, st);
-        HIP_TRY(hipGetLastError());
+        void* q = nullptr;
+        HIP_TRY(hipMalloc(&q, bytes));
+        slots->push_back(q);
+        slot_bytes->push_back(bytes);
+        next_slot++;
+        *p = q;
+        return SMG_OK;
     }
-    if (n == 0) HIP_TRY(hipEventRecord(done, s));
+    template <class X>
+    int temp(X** p, size_t count) {
+        size_t bytes = std::max<size_t>(count, 1) * sizeof(X);
+        if (planned()) return slot((void**)p, bytes);
+        HIP_TRY(T->get((void**)p, bytes));
+        return SMG_OK;
+    }
+    template <class X>
+    int output(X** p, size_t count) {
+        size_t bytes = std::max<size_t>(count, 1) * sizeof(X);
+        if (planned()) return slot((void**)p, bytes);
+        HIP_TRY(hipMalloc((void**)p, bytes));
+        return SMG_OK;
+    }
+    // A device count: read back (exact) or the capacity (planned).
+    int size(const uint32_t* dev, uint32_t cap, uint32_t& v) {
+        if (planned()) {
+            v = cap;
+            return SMG_OK;
+        }
+        return read_u32(d, dev, v);
+    }
+};
+
+// Exclusive scan of in[0 .. *n) into out, at most `cap` items; the total
+// goes to *total (device).
+int scan_u32(Run& X, const uint32_t* in, uint32_t* out, const uint32_t* n, uint64_t cap, uint32_t* total) {
+    uint64_t tiles = std::max<uint64_t>((cap + SCAN_TILE - 1) / SCAN_TILE, 1);
+    uint32_t* sums;
+    TRY(X.temp(&sums, tiles));
+    k_scan_tiles<<<g1(tiles), SCAN_BLOCK, 0, X.s>>>(in, out, sums, n);
+    k_scan_sums<<<1, SCAN_BLOCK, 0, X.s>>>(sums, n, total);
+    k_scan_add<<<stride_grid(cap), 256, 0, X.s>>>(out, sums, n);
+    HIP_TRY(hipGetLastError());
     return SMG_OK;
 }
 
@@ -861,214 +927,327 @@ int molecule_run(const smg_resident* r, const smg_mesh_params* prm, uint32_t b, 
     return SMG_OK;
 }
 
-}  // namespace
+// A run's molecule table: one row per molecule (rows past `nm` are empty
+// molecules, so a plan's table keeps its size), with the offsets of each
+// molecule's spheres, f32 copies and bitmap words in the concatenations.
+struct Table {
+    size_t nm = 0, rows = 0;
+    uint32_t b = 0;
+    bool fast = false;
+    std::vector<Mol> H;
+    std::vector<uint64_t> sph, loc, wofs;  // rows + 1 each
+    uint64_t dense = 0;
+    uint64_t words() const { return wofs[rows]; }
+    uint64_t locals() const { return loc[rows]; }
+    uint64_t spheres() const { return sph[rows]; }
+};
 
-// The mesh stage of `nm` molecules in one pass: every kernel runs once over
-// the concatenated work of all molecules, each work item finding its
-// molecule's arrangement, f32 copies and lattice in the molecule table.
-static int mesh_many(smg_resident* const* R, const smg_mesh_params* prm, size_t nm, smg_mesh* M,
-                     smg_mesh_times* times) {
+int build_table(smg_resident* const* R, const smg_mesh_params* prm, size_t nm, size_t rows, Table& t) {
     if (nm == 0) return fail(SMG_E_INVALID, "no molecules");
     if (nm >= (1u << 24)) return fail(SMG_E_LENGTH, "too many molecules in one run");
     smg_device* d = R[0]->dev;
     for (size_t m = 0; m < nm; m++)
         if (!R[m] || R[m]->dev != d) return fail(SMG_E_INVALID, "all molecules of a run must be resident on one device");
-    hipStream_t s = d->stream;
-    HIP_TRY(hipSetDevice(d->ordinal));
     const uint32_t b = prm[0].brick;
     if (b < 4 || b > 32 || b % 2) return fail(SMG_E_UNSUPPORTED, "the device path needs an even brick size in 4..=32");
-    const uint32_t spb = (b + 1) * (b + 1) * (b + 1);
-    M->spb = spb;
-    const bool fast = prm[0].refine == SMG_REFINE_FAST;
+    t.fast = prm[0].refine == SMG_REFINE_FAST;
     for (size_t m = 0; m < nm; m++)
-        if ((prm[m].refine == SMG_REFINE_FAST) != fast) return fail(SMG_E_UNSUPPORTED, "one refinement mode per run");
-    // Molecule table, offsets of the concatenated per-molecule arrays.
-    std::vector<Mol> H(nm);
-    std::vector<uint64_t> sph_off(nm + 1, 0), loc_off(nm + 1, 0), wofs(nm + 1, 0);
-    uint64_t dense_total = 0;
-    for (size_t m = 0; m < nm; m++) {
-        Mol& X = H[m];
-        X.A = R[m]->A;
-        bool empty = false;
-        TRY(molecule_run(R[m], &prm[m], b, X.P, empty));
-        uint64_t cells = empty ? 0 : (uint64_t)X.P.bdim[0] * X.P.bdim[1] * X.P.bdim[2];
-        uint64_t nmax = std::max<uint64_t>({X.A.n_spheres, X.A.n_circles, X.A.n_vertices});
-        sph_off[m + 1] = sph_off[m] + X.A.n_spheres;
-        loc_off[m + 1] = loc_off[m] + nmax;
-        wofs[m + 1] = wofs[m] + (cells + 31) / 32;
-        X.word_off = wofs[m];
-        X.dense_off = dense_total;
-        dense_total += cells;
+        if ((prm[m].refine == SMG_REFINE_FAST) != t.fast) return fail(SMG_E_UNSUPPORTED, "one refinement mode per run");
+    t.nm = nm;
+    t.rows = std::max(rows, nm);
+    t.b = b;
+    t.H.assign(t.rows, Mol{});
+    t.sph.assign(t.rows + 1, 0);
+    t.loc.assign(t.rows + 1, 0);
+    t.wofs.assign(t.rows + 1, 0);
+    t.dense = 0;
+    for (size_t m = 0; m < t.rows; m++) {
+        Mol& X = t.H[m];
+        uint64_t cells = 0, nmax = 0;
+        if (m < nm) {
+            X.A = R[m]->A;
+            bool empty = false;
+            TRY(molecule_run(R[m], &prm[m], b, X.P, empty));
+            cells = empty ? 0 : (uint64_t)X.P.bdim[0] * X.P.bdim[1] * X.P.bdim[2];
+            nmax = std::max<uint64_t>({X.A.n_spheres, X.A.n_circles, X.A.n_vertices});
+        }
+        t.sph[m + 1] = t.sph[m] + X.A.n_spheres;
+        t.loc[m + 1] = t.loc[m] + nmax;
+        t.wofs[m + 1] = t.wofs[m] + (cells + 31) / 32;
+        X.word_off = t.wofs[m];
+        X.dense_off = t.dense;
+        t.dense += cells;
     }
-    const uint64_t nwords = wofs[nm];
-    if (nwords >= (1ull << 27)) return fail(SMG_E_UNSUPPORTED, "brick boxes of the run too large for 32-bit bit indices");
-    Temps T(s);
-    hipEvent_t* ev = d->ev;
-    HIP_TRY(hipEventRecord(ev[0], s));
-    float* lbuf;
-    HIP_TRY(T.get(&lbuf, 19 * loc_off[nm]));
-    for (size_t m = 0; m < nm; m++) {
-        DLoc& L = H[m].L;
-        size_t nmax = loc_off[m + 1] - loc_off[m];
-        float* base = lbuf + 19 * loc_off[m];
-        gptr<float>* fields[19] = {&L.px, &L.py, &L.pz, &L.pr, &L.pr2, &L.pr2tol, &L.mx, &L.my, &L.mz, &L.ux,
-                                   &L.uy, &L.uz, &L.ex, &L.ey, &L.ez, &L.rho,   &L.vx, &L.vy, &L.vz};
+    if (t.words() >= (1ull << 27)) return fail(SMG_E_UNSUPPORTED, "brick boxes of the run too large for 32-bit bit indices");
+    if (t.locals() >= (1ull << 32) || t.spheres() >= (1ull << 32))
+        return fail(SMG_E_UNSUPPORTED, "too many spheres in one run");
+    return SMG_OK;
+}
+
+// The uploaded image of a table: header, rows, three offset arrays.
+struct StageLayout {
+    size_t mols, sph, loc, wofs, bytes;
+};
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+StageLayout stage_layout(size_t rows) {
+    StageLayout L;
+    L.mols = 256;
+    L.sph = align256(L.mols + rows * sizeof(Mol));
+    L.loc = align256(L.sph + (rows + 1) * 8);
+    L.wofs = align256(L.loc + (rows + 1) * 8);
+    L.bytes = align256(L.wofs + (rows + 1) * 8);
+    return L;
+}
+
+// Writes the table into `host` (laid out as stage_layout), pointing each
+// molecule's f32 copies into `lbuf` (19 arrays per molecule).
+void write_stage(Table& t, float* lbuf, char* host) {
+    StageLayout L = stage_layout(t.rows);
+    for (size_t m = 0; m < t.rows; m++) {
+        DLoc& Lc = t.H[m].L;
+        size_t nmax = t.loc[m + 1] - t.loc[m];
+        float* base = lbuf + 19 * t.loc[m];
+        gptr<float>* fields[19] = {&Lc.px, &Lc.py, &Lc.pz, &Lc.pr, &Lc.pr2, &Lc.pr2tol, &Lc.mx, &Lc.my, &Lc.mz, &Lc.ux,
+                                   &Lc.uy, &Lc.uz, &Lc.ex, &Lc.ey, &Lc.ez, &Lc.rho,    &Lc.vx, &Lc.vy, &Lc.vz};
         for (int q = 0; q < 19; q++) *fields[q] = to_global(base + q * nmax);
     }
-    Mol* mols;
-    uint64_t *sph_d, *loc_d, *wofs_d;
-    HIP_TRY(T.get(&mols, nm));
-    HIP_TRY(T.get(&sph_d, nm + 1));
-    HIP_TRY(T.get(&loc_d, nm + 1));
-    HIP_TRY(T.get(&wofs_d, nm + 1));
-    HIP_TRY(hipMemcpyAsync(mols, H.data(), nm * sizeof(Mol), hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(sph_d, sph_off.data(), (nm + 1) * 8, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(loc_d, loc_off.data(), (nm + 1) * 8, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(wofs_d, wofs.data(), (nm + 1) * 8, hipMemcpyHostToDevice, s));
-    const uint32_t nmu = (uint32_t)nm;
-    if (loc_off[nm]) k_localize<<<blocks_for(loc_off[nm], 256), 256, 0, s>>>(mols, loc_d, nmu);
+    RunHdr h{(uint32_t)t.words(), (uint32_t)t.dense, (uint32_t)t.locals(), (uint32_t)t.spheres()};
+    std::memset(host, 0, L.bytes);
+    std::memcpy(host, &h, sizeof(h));
+    std::memcpy(host + L.mols, t.H.data(), t.rows * sizeof(Mol));
+    std::memcpy(host + L.sph, t.sph.data(), (t.rows + 1) * 8);
+    std::memcpy(host + L.loc, t.loc.data(), (t.rows + 1) * 8);
+    std::memcpy(host + L.wofs, t.wofs.data(), (t.rows + 1) * 8);
+}
+
+// Host-side extents a run's buffers are sized for, and the capacities of
+// the device counts (NO_CAP: none, the exact path).
+struct Extents {
+    uint64_t rows, locals, spheres, words, dense;
+    uint32_t nb, nv, nt;
+};
+
+// The per-run summary read back at the end, one image on the device and
+// on the host: samples and evaluations per molecule (u64), the counters,
+// then per-molecule first touched brick, surface brick, sampled sub-block,
+// vertex and triangle (rows + 1 each, the last holding the totals).
+struct SumLayout {
+    size_t nsamp, evals, cnt, tfirst, fbrick, sfirst, vfirst, trfirst, zero_words, bytes;
+};
+SumLayout sum_layout(size_t rows) {
+    SumLayout L;
+    L.nsamp = 0;
+    L.evals = 8 * rows;
+    L.cnt = 16 * rows;
+    L.tfirst = L.cnt + 4 * K_N;
+    L.fbrick = L.tfirst + 4 * (rows + 1);
+    L.sfirst = L.fbrick + 4 * (rows + 1);
+    L.vfirst = L.sfirst + 4 * (rows + 1);
+    L.trfirst = L.vfirst + 4 * (rows + 1);
+    L.bytes = L.trfirst + 4 * (rows + 1);
+    L.zero_words = L.tfirst / 4;
+    return L;
+}
+
+// The mesh stage of a table's molecules in one pass: every kernel runs once
+// over the concatenated work of all of them, each work item finding its
+// molecule's arrangement, f32 copies and lattice in the table. Sets M's
+// device arrays and copies the summary into `sum_host` (asynchronously).
+int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_host, smg_mesh* M, bool timed) {
+    smg_device* d = X.d;
+    hipStream_t s = X.s;
+    const uint32_t b = t.b, spb = (b + 1) * (b + 1) * (b + 1);
+    const uint32_t rows = (uint32_t)E.rows;
+    const bool fast = t.fast;
+    M->spb = spb;
+    auto rec = [&](int i) -> int {
+        if (timed) HIP_TRY(hipEventRecord(d->ev[i], s));
+        return SMG_OK;
+    };
+    // Table upload and run counters.
+    StageLayout SL = stage_layout(rows);
+    SumLayout UL = sum_layout(rows);
+    char *stage, *sum;
+    float* lbuf;
+    TRY(X.temp(&stage, SL.bytes));
+    TRY(X.temp(&sum, UL.bytes));
+    TRY(X.temp(&lbuf, 19 * E.locals));
+    X.lbuf = lbuf;
+    write_stage(t, lbuf, stage_host);
+    TRY(rec(0));
+    HIP_TRY(hipMemcpyAsync(stage, stage_host, SL.bytes, hipMemcpyHostToDevice, s));
+    const RunHdr* hdr = (const RunHdr*)stage;
+    const Mol* mols = (const Mol*)(stage + SL.mols);
+    const uint64_t* sph_d = (const uint64_t*)(stage + SL.sph);
+    const uint64_t* loc_d = (const uint64_t*)(stage + SL.loc);
+    const uint64_t* wofs_d = (const uint64_t*)(stage + SL.wofs);
+    unsigned long long* nsamp = (unsigned long long*)(sum + UL.nsamp);
+    unsigned long long* evd = (unsigned long long*)(sum + UL.evals);
+    uint32_t* cnt = (uint32_t*)(sum + UL.cnt);
+    uint32_t* tfirst = (uint32_t*)(sum + UL.tfirst);
+    uint32_t* fbrick = (uint32_t*)(sum + UL.fbrick);
+    uint32_t* sfirst = (uint32_t*)(sum + UL.sfirst);
+    uint32_t* vfirst = (uint32_t*)(sum + UL.vfirst);
+    uint32_t* trfirst = (uint32_t*)(sum + UL.trfirst);
+    k_fill_u32<<<stride_grid(UL.zero_words), 256, 0, s>>>((uint32_t*)sum, 0u, UL.zero_words, nullptr);
+    k_localize<<<stride_grid(E.locals), 256, 0, s>>>(mols, loc_d, rows);
     HIP_TRY(hipGetLastError());
 
     // Touched bricks (lattice.rs:105-163) in per-molecule dense bitmaps.
-    uint32_t *bits, *wcount, *woff, *dense;
-    HIP_TRY(T.get(&bits, nwords));
-    HIP_TRY(T.get(&wcount, nwords));
-    HIP_TRY(T.get(&woff, nwords));
-    HIP_TRY(T.get(&dense, dense_total));
-    if (nwords) HIP_TRY(hipMemsetAsync(bits, 0, nwords * 4, s));
-    if (sph_off[nm] && nwords) k_mark_touched<<<blocks_for(sph_off[nm], 128), 128, 0, s>>>(mols, sph_d, nmu, bits);
-    if (nwords) k_popcount<<<blocks_for(nwords, 256), 256, 0, s>>>(bits, wcount, nwords);
-    uint32_t* ntouched_d;
-    TRY(scan_u32(s, T, wcount, woff, nwords, &ntouched_d));
-    uint32_t ntouched;
-    TRY(read_u32(d, ntouched_d, ntouched));
-    uint32_t *touched, *keep, *koff;
-    HIP_TRY(T.get(&touched, ntouched));
-    HIP_TRY(T.get(&keep, ntouched));
-    HIP_TRY(T.get(&koff, ntouched));
-    if (nwords) k_bits_to_list<<<blocks_for(nwords, 256), 256, 0, s>>>(bits, woff, nwords, touched);
+    uint32_t *bits, *wcount, *woff, *dense, *touched, *keep, *koff;
+    TRY(X.temp(&bits, E.words));
+    TRY(X.temp(&wcount, E.words));
+    TRY(X.temp(&woff, E.words));
+    TRY(X.temp(&dense, E.dense));
+    TRY(X.temp(&touched, E.dense));
+    TRY(X.temp(&keep, E.dense));
+    TRY(X.temp(&koff, E.dense));
+    k_fill_u32<<<stride_grid(E.words), 256, 0, s>>>(bits, 0u, 0, &hdr->nwords);
+    k_mark_touched<<<stride_grid(E.spheres, 128), 128, 0, s>>>(mols, sph_d, rows, bits);
+    k_popcount<<<stride_grid(E.words), 256, 0, s>>>(bits, wcount, &hdr->nwords);
+    TRY(scan_u32(X, wcount, woff, &hdr->nwords, E.words, &cnt[K_TOUCHED]));
+    k_bits_to_list<<<stride_grid(E.words), 256, 0, s>>>(bits, woff, &hdr->nwords, touched);
     // Brick classification (lattice.rs:168-183), one wavefront per brick.
-    HIP_TRY(hipEventRecord(ev[6], s));
-    if (ntouched) k_classify_bricks<<<ntouched, WAVE, 0, s>>>(mols, wofs_d, nmu, touched, nullptr, ntouched, keep);
-    HIP_TRY(hipEventRecord(ev[7], s));
-    uint32_t* nb_d;
-    TRY(scan_u32(s, T, keep, koff, ntouched, &nb_d));
-    uint32_t nb;
-    TRY(read_u32(d, nb_d, nb));
-    HIP_TRY(hipMalloc(&M->keys, std::max<size_t>(nb, 1) * 3 * sizeof(int32_t)));
-    uint32_t *bmol, *tfirst, *fbrick;
-    HIP_TRY(T.get(&bmol, nb));
-    HIP_TRY(T.get(&tfirst, nm + 1));
-    HIP_TRY(T.get(&fbrick, nm + 1));
-    if (dense_total) k_fill_u32<<<blocks_for(dense_total, 256), 256, 0, s>>>(dense, 0xffffffffu, dense_total);
-    if (ntouched)
-        k_compact_bricks<<<blocks_for(ntouched, 256), 256, 0, s>>>(mols, wofs_d, nmu, touched, keep, koff, ntouched,
-                                                                    M->keys, bmol, dense);
-    k_mol_ranges<<<blocks_for(nm + 1, 256), 256, 0, s>>>(wofs_d, nmu, woff, nwords, ntouched, koff, nb, tfirst, fbrick);
-    uint8_t* st;
-    HIP_TRY(T.get(&st, 8 * (size_t)nb));
-    HIP_TRY(hipEventRecord(ev[8], s));
-    TRY(run_uniform<UNIFORM_SUBBLOCKS>(d, T, mols, wofs_d, nmu, nullptr, bmol, M->keys, nb, nullptr, st, ev[9]));
+    TRY(rec(6));
+    {
+        Queue q{nullptr, &cnt[K_TOUCHED], nullptr};
+        k_classify_bricks<<<g1(E.dense), WAVE, 0, s>>>(mols, wofs_d, rows, touched, q, keep);
+    }
+    TRY(rec(7));
+    TRY(scan_u32(X, keep, koff, &cnt[K_TOUCHED], E.dense, &cnt[K_NB]));
+    k_clamp<<<1, 64, 0, s>>>(&cnt[K_NB], E.nb, &cnt[K_FLAGS], OVER_BRICKS, &cnt[K_NSUB], 8u);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(ev[1], s));
+    uint32_t nb;
+    TRY(X.size(&cnt[K_NB], E.nb, nb));
+    uint32_t* bmol;
+    uint8_t* st;
+    TRY(X.output(&M->keys, (size_t)nb * 3));
+    TRY(X.temp(&bmol, nb));
+    TRY(X.temp(&st, 8 * (size_t)nb));
+    k_fill_u32<<<stride_grid(E.dense), 256, 0, s>>>(dense, 0xffffffffu, 0, &hdr->dense);
+    k_compact_bricks<<<stride_grid(E.dense), 256, 0, s>>>(mols, wofs_d, rows, touched, keep, koff, &cnt[K_TOUCHED],
+                                                            nb, M->keys, bmol, dense);
+    k_mol_ranges<<<stride_grid(rows + 1), 256, 0, s>>>(wofs_d, rows, woff, &hdr->nwords, &cnt[K_TOUCHED], koff,
+                                                         &cnt[K_NB], tfirst, fbrick);
+    HIP_TRY(hipGetLastError());
+    // Sub-block states (kernel.rs:467-482): one shared gather per brick, one
+    // wavefront per brick for bricks whose lists overflow it.
+    TRY(rec(8));
+    {
+        const size_t lds_max = d->prop.sharedMemPerBlock;
+        RLayout lay = make_rlayout(UNIFORM_CAPS);
+        Queue rest{nullptr, &cnt[K_NB], &cnt[K_Q_SUBST]};
+        if (lay.bytes + STATIC_LDS <= lds_max) {
+            uint32_t* over;
+            TRY(X.temp(&over, nb));
+            Queue q{nullptr, &cnt[K_NB], nullptr};
+            k_uniform<<<g1(nb), 256, lay.bytes, s>>>(mols, bmol, M->keys, UNIFORM_CAPS, lay, q, st, over,
+                                                     &cnt[K_UOVER]);
+            rest = Queue{over, &cnt[K_UOVER], &cnt[K_Q_SUBST]};
+        }
+        TRY(rec(9));
+        uint32_t grid = g1(std::min<uint64_t>(nb, resident(d, (const void*)k_sub_states, WAVE, 0)));
+        k_sub_states<<<grid, WAVE, 0, s>>>(mols, bmol, M->keys, rest, st);
+        HIP_TRY(hipGetLastError());
+    }
+    TRY(rec(1));
 
     // Samples (field/kernel.rs:415-449).
-    HIP_TRY(hipMalloc(&M->values, std::max<size_t>((size_t)nb * spb, 1) * sizeof(float)));
-    if (nb) k_assign_uniform<<<nb, 256, 0, s>>>(mols, bmol, st, nb, M->values);
-    uint32_t nsub = 8 * nb;
+    TRY(X.output(&M->values, (size_t)nb * spb));
+    k_assign_uniform<<<stride_grid(nb, 1), 256, 0, s>>>(mols, bmol, st, &cnt[K_NB], M->values);
+    const uint64_t nsub = 8 * (uint64_t)nb;
     uint32_t *sflag, *soff, *swork;
-    HIP_TRY(T.get(&sflag, nsub));
-    HIP_TRY(T.get(&soff, nsub));
-    HIP_TRY(T.get(&swork, nsub));
-    if (nsub) k_flag_states<<<blocks_for(nsub, 256), 256, 0, s>>>(st, nsub, sflag);
-    uint32_t* nsw_d;
-    TRY(scan_u32(s, T, sflag, soff, nsub, &nsw_d));
-    if (nsub) k_compact_index<<<blocks_for(nsub, 256), 256, 0, s>>>(sflag, soff, nsub, swork);
-    uint32_t nsw;
-    TRY(read_u32(d, nsw_d, nsw));
-    unsigned long long *nsamp, *evd;
-    uint32_t* ovcount;
-    HIP_TRY(T.get(&nsamp, nm));
-    HIP_TRY(T.get(&evd, nm));
-    HIP_TRY(T.get(&ovcount, 1));
-    HIP_TRY(hipMemsetAsync(nsamp, 0, nm * 8, s));
-    HIP_TRY(hipMemsetAsync(evd, 0, nm * 8, s));
+    TRY(X.temp(&sflag, nsub));
+    TRY(X.temp(&soff, nsub));
+    TRY(X.temp(&swork, nsub));
+    k_flag_states<<<stride_grid(nsub), 256, 0, s>>>(st, &cnt[K_NSUB], sflag);
+    TRY(scan_u32(X, sflag, soff, &cnt[K_NSUB], nsub, &cnt[K_NSW]));
+    k_compact_index<<<stride_grid(nsub), 256, 0, s>>>(sflag, soff, &cnt[K_NSUB], swork);
+    HIP_TRY(hipGetLastError());
     const size_t lds_max = d->prop.sharedMemPerBlock;
-    uint32_t overflow0 = 0;
+    // The first pass takes one sub-block per workgroup (a grid of the count,
+    // or of its capacity when the run is planned); the overflow passes are
+    // persistent over their queues.
+    uint32_t nsw;
+    TRY(X.size(&cnt[K_NSW], (uint32_t)nsub, nsw));
     {
         const uint32_t* list = swork;
-        uint32_t count = nsw;
-        HIP_TRY(hipEventRecord(ev[10], s));
-        for (int pass = 0; pass < TIERS && count; pass++) {
+        const uint32_t* count = &cnt[K_NSW];
+        bool first = true;
+        TRY(rec(10));
+        for (int pass = 0; pass < TIERS; pass++) {
             const Caps& cap = SAMPLE_CAPS[pass];
             Layout lay = make_layout(cap);
             bool lds = pass < TIERS - 1;
             if (lds && lay.bytes + STATIC_LDS > lds_max) continue;
-            uint32_t *next, *nnext;
-            HIP_TRY(T.get(&next, count));
-            HIP_TRY(T.get(&nnext, 1));
-            HIP_TRY(hipMemsetAsync(nnext, 0, 4, s));
-            if (lds) {
-                k_sample<false><<<count, 256, lay.bytes, s>>>(mols, bmol, cap, lay, M->keys, st, list, count,
-                                                              M->values, nullptr, 0, next, nnext, nsamp);
+            uint32_t* next;
+            TRY(X.temp(&next, nsub));
+            Queue q{list, count, &cnt[K_Q_SAMPLE0 + pass]};
+            uint32_t* nnext = &cnt[K_SOVER0 + pass];
+            if (lds && first) {
+                k_sample<false, false><<<g1(nsw), 256, lay.bytes, s>>>(mols, bmol, cap, lay, M->keys, st, q,
+                                                                       M->values, nullptr, 0, next, nnext, nsamp);
+            } else if (lds) {
+                uint32_t grid =
+                    g1(std::min<uint64_t>(nsub, resident(d, (const void*)k_sample<false, true>, 256, lay.bytes)));
+                k_sample<false, true><<<grid, 256, lay.bytes, s>>>(mols, bmol, cap, lay, M->keys, st, q, M->values,
+                                                                   nullptr, 0, next, nnext, nsamp);
             } else {
-                uint32_t chunk = std::min(count, SCRATCH_CHUNK);
+                uint32_t grid = g1(std::min<uint64_t>(nsub, SCRATCH_CHUNK));
                 char* scratch;
-                HIP_TRY(T.get(&scratch, (size_t)chunk * lay.bytes));
-                for (uint32_t c0 = 0; c0 < count; c0 += chunk)
-                    k_sample<true><<<std::min(chunk, count - c0), 256, 0, s>>>(
-                        mols, bmol, cap, lay, M->keys, st, list + c0, std::min(chunk, count - c0), M->values, scratch,
-                        lay.bytes, next, nnext, nsamp);
+                TRY(X.temp(&scratch, (size_t)grid * lay.bytes));
+                k_sample<true, true><<<grid, 256, 0, s>>>(mols, bmol, cap, lay, M->keys, st, q, M->values, scratch,
+                                                          lay.bytes, next, nnext, nsamp);
             }
-            if (pass == 0) HIP_TRY(hipEventRecord(ev[11], s));
             HIP_TRY(hipGetLastError());
-            TRY(read_u32(d, nnext, count));
-            if (pass == 0) overflow0 = count;
+            if (first) TRY(rec(11));
+            first = false;
             list = next;
+            count = nnext;
         }
-        if (count) return fail(SMG_E_INTERNAL, std::to_string(count) + " sub-blocks exceed the largest candidate lists");
     }
-    HIP_TRY(hipEventRecord(ev[2], s));
+    TRY(rec(2));
 
     // Crossing edges and vertex numbering (mesh/extract.rs:176-216, 606-627).
-    uint32_t words = (3 * b * b * b + 63) / 64;
+    const uint32_t words = (3 * b * b * b + 63) / 64;
     unsigned long long* mask;
     uint32_t *wpre, *bcount, *scount, *vbase;
-    HIP_TRY(T.get(&mask, (size_t)nb * words));
-    HIP_TRY(T.get(&wpre, (size_t)nb * words));
-    HIP_TRY(T.get(&bcount, nb));
-    HIP_TRY(T.get(&scount, nsub));
-    HIP_TRY(T.get(&vbase, nb + 1));
-    if (nb) k_edges<<<nb, 256, 0, s>>>(b, M->values, nb, words, mask, wpre, bcount, scount);
-    uint32_t* nv_d;
-    TRY(scan_u32(s, T, bcount, vbase, nb, &nv_d));
+    TRY(X.temp(&mask, (size_t)nb * words));
+    TRY(X.temp(&wpre, (size_t)nb * words));
+    TRY(X.temp(&bcount, nb));
+    TRY(X.temp(&scount, nsub));
+    TRY(X.temp(&vbase, (size_t)nb + 1));
+    k_edges<<<stride_grid(nb, 1), 256, 0, s>>>(b, M->values, &cnt[K_NB], words, mask, wpre, bcount, scount);
+    TRY(scan_u32(X, bcount, vbase, &cnt[K_NB], nb, &cnt[K_NV]));
+    k_clamp<<<1, 64, 0, s>>>(&cnt[K_NV], E.nv, &cnt[K_FLAGS], OVER_VERTICES, nullptr, 0u);
     uint32_t *rflag, *roff, *rwork;
-    HIP_TRY(T.get(&rflag, nsub));
-    HIP_TRY(T.get(&roff, nsub));
-    HIP_TRY(T.get(&rwork, nsub));
-    if (nsub) k_flag_nonzero<<<blocks_for(nsub, 256), 256, 0, s>>>(scount, nsub, rflag);
-    uint32_t* nrw_d;
-    TRY(scan_u32(s, T, rflag, roff, nsub, &nrw_d));
-    if (nsub) k_compact_index<<<blocks_for(nsub, 256), 256, 0, s>>>(rflag, roff, nsub, rwork);
+    TRY(X.temp(&rflag, nsub));
+    TRY(X.temp(&roff, nsub));
+    TRY(X.temp(&rwork, nsub));
+    k_flag_nonzero<<<stride_grid(nsub), 256, 0, s>>>(scount, &cnt[K_NSUB], rflag);
+    TRY(scan_u32(X, rflag, roff, &cnt[K_NSUB], nsub, &cnt[K_NRW]));
+    k_compact_index<<<stride_grid(nsub), 256, 0, s>>>(rflag, roff, &cnt[K_NSUB], rwork);
+    HIP_TRY(hipGetLastError());
     uint32_t nverts, nrw;
-    TRY(read_u32(d, nv_d, nverts));
-    TRY(read_u32(d, nrw_d, nrw));
-    HIP_TRY(hipEventRecord(ev[3], s));
+    TRY(X.size(&cnt[K_NV], E.nv, nverts));
+    TRY(X.size(&cnt[K_NRW], (uint32_t)nsub, nrw));
+    TRY(rec(3));
 
-    // Exact refinement and attributes (mesh/extract.rs:425-597).
-    size_t nvv = std::max<size_t>(nverts, 1);
-    HIP_TRY(hipMalloc(&M->pos, 3 * nvv * sizeof(float)));
-    HIP_TRY(hipMalloc(&M->nrm, 3 * nvv * sizeof(float)));
-    HIP_TRY(hipMalloc(&M->atom, nvv * sizeof(uint32_t)));
-    HIP_TRY(hipMalloc(&M->patch, nvv));
-    HIP_TRY(hipMalloc(&M->t, nvv * sizeof(double)));
-    HIP_TRY(hipMalloc(&M->vevals, nvv * sizeof(uint32_t)));
-    VOut vo{M->pos, M->nrm, M->atom, M->patch, M->t, M->vevals};
+    // Refinement and attributes (mesh/extract.rs:425-597).
+    TRY(X.output(&M->pos, 3 * (size_t)nverts));
+    TRY(X.output(&M->nrm, 3 * (size_t)nverts));
+    TRY(X.output(&M->atom, nverts));
+    TRY(X.output(&M->patch, nverts));
+    TRY(X.output(&M->t, nverts));
+    TRY(X.output(&M->vevals, nverts));
+    VOut vo{M->pos, M->nrm, M->atom, M->patch, M->t, M->vevals, nverts};
     {
         const uint32_t* list = rwork;
-        uint32_t count = nrw, sub_edges = 3 * (b / 2) * (b / 2) * (b / 2);
-        HIP_TRY(hipEventRecord(ev[12], s));
-        for (int pass = 0; pass < TIERS && count; pass++) {
+        const uint32_t* count = &cnt[K_NRW];
+        const uint32_t sub_edges = 3 * (b / 2) * (b / 2) * (b / 2);
+        bool first = true;
+        TRY(rec(12));
+        for (int pass = 0; pass < TIERS; pass++) {
             RCaps cap = REFINE_CAPS[pass];
             bool lds = pass < TIERS - 1;
             cap.edges = lds ? std::min(sub_edges, LDS_EDGES) : sub_edges;
@@ -1076,99 +1255,118 @@ static int mesh_many(smg_resident* const* R, const smg_mesh_params* prm, size_t 
             FLayout flay = make_flayout(cap, lay.bytes);
             const uint32_t bytes = fast ? flay.bytes : lay.bytes;
             if (lds && bytes + STATIC_LDS > lds_max) continue;
-            uint32_t *next, *nnext;
-            HIP_TRY(T.get(&next, count));
-            HIP_TRY(T.get(&nnext, 1));
-            HIP_TRY(hipMemsetAsync(nnext, 0, 4, s));
-            if (lds && fast) {
-                k_refine_fast<false><<<count, 256, bytes, s>>>(mols, bmol, cap, lay, flay, M->keys, M->values, list,
-                                                               count, mask, wpre, words, vbase, vo, nullptr, 0, next,
-                                                               nnext, evd);
+            uint32_t* next;
+            TRY(X.temp(&next, nsub));
+            Queue q{list, count, &cnt[K_Q_REFINE0 + pass]};
+            uint32_t* nnext = &cnt[K_ROVER0 + pass];
+            if (lds && first) {
+                if (fast)
+                    k_refine_fast<false, false><<<g1(nrw), 256, bytes, s>>>(mols, bmol, cap, lay, flay, M->keys,
+                                                                            M->values, q, mask, wpre, words, vbase,
+                                                                            vo, nullptr, 0, next, nnext, evd);
+                else
+                    k_refine<false, false><<<g1(nrw), 256, bytes, s>>>(mols, bmol, cap, lay, M->keys, M->values, q,
+                                                                       mask, wpre, words, vbase, vo, nullptr, 0, next,
+                                                                       nnext, evd);
             } else if (lds) {
-                k_refine<false><<<count, 256, lay.bytes, s>>>(mols, bmol, cap, lay, M->keys, M->values, list, count, mask,
-                                                              wpre, words, vbase, vo, nullptr, 0, next, nnext, evd);
+                const void* kern =
+                    fast ? (const void*)k_refine_fast<false, true> : (const void*)k_refine<false, true>;
+                uint32_t grid = g1(std::min<uint64_t>(nsub, resident(d, kern, 256, bytes)));
+                if (fast)
+                    k_refine_fast<false, true><<<grid, 256, bytes, s>>>(mols, bmol, cap, lay, flay, M->keys,
+                                                                        M->values, q, mask, wpre, words, vbase, vo,
+                                                                        nullptr, 0, next, nnext, evd);
+                else
+                    k_refine<false, true><<<grid, 256, bytes, s>>>(mols, bmol, cap, lay, M->keys, M->values, q, mask,
+                                                                   wpre, words, vbase, vo, nullptr, 0, next, nnext,
+                                                                   evd);
             } else {
-                uint32_t chunk = std::min(count, SCRATCH_CHUNK);
+                uint32_t grid = g1(std::min<uint64_t>(nsub, SCRATCH_CHUNK));
                 char* scratch;
-                HIP_TRY(T.get(&scratch, (size_t)chunk * bytes));
-                for (uint32_t c0 = 0; c0 < count; c0 += chunk) {
-                    uint32_t cnt = std::min(chunk, count - c0);
-                    if (fast)
-                        k_refine_fast<true><<<cnt, 256, 0, s>>>(mols, bmol, cap, lay, flay, M->keys, M->values,
-                                                                 list + c0, cnt, mask, wpre, words, vbase, vo, scratch,
-                                                                 bytes, next, nnext, evd);
-                    else
-                        k_refine<true><<<cnt, 256, 0, s>>>(mols, bmol, cap, lay, M->keys, M->values, list + c0, cnt,
-                                                            mask, wpre, words, vbase, vo, scratch, lay.bytes, next,
-                                                            nnext, evd);
-                }
+                TRY(X.temp(&scratch, (size_t)grid * bytes));
+                if (fast)
+                    k_refine_fast<true, true><<<grid, 256, 0, s>>>(mols, bmol, cap, lay, flay, M->keys, M->values, q,
+                                                                   mask, wpre, words, vbase, vo, scratch, bytes, next,
+                                                                   nnext, evd);
+                else
+                    k_refine<true, true><<<grid, 256, 0, s>>>(mols, bmol, cap, lay, M->keys, M->values, q, mask,
+                                                              wpre, words, vbase, vo, scratch, bytes, next, nnext,
+                                                              evd);
             }
-            if (pass == 0) HIP_TRY(hipEventRecord(ev[13], s));
             HIP_TRY(hipGetLastError());
-            TRY(read_u32(d, nnext, count));
+            if (first) TRY(rec(13));
+            first = false;
             list = next;
+            count = nnext;
         }
-        if (count) return fail(SMG_E_INTERNAL, std::to_string(count) + " sub-blocks exceed the largest refinement lists");
     }
-    HIP_TRY(hipEventRecord(ev[4], s));
+    TRY(rec(4));
 
     // Cell pass and assembly (mesh/extract.rs:226-265, 606-691).
-    uint32_t *tcount, *tbase, *err;
-    HIP_TRY(T.get(&tcount, nb));
-    HIP_TRY(T.get(&tbase, nb + 1));
-    HIP_TRY(T.get(&err, 1));
-    HIP_TRY(hipMemsetAsync(err, 0, 4, s));
-    if (nb) k_cell_count<<<nb, 256, 0, s>>>(b, M->values, nb, tcount);
-    uint32_t* nt_d;
-    TRY(scan_u32(s, T, tcount, tbase, nb, &nt_d));
+    uint32_t *tcount, *tbase;
+    TRY(X.temp(&tcount, nb));
+    TRY(X.temp(&tbase, (size_t)nb + 1));
+    k_cell_count<<<stride_grid(nb, 1), 256, 0, s>>>(b, M->values, &cnt[K_NB], tcount);
+    TRY(scan_u32(X, tcount, tbase, &cnt[K_NB], nb, &cnt[K_NT]));
+    k_clamp<<<1, 64, 0, s>>>(&cnt[K_NT], E.nt, &cnt[K_FLAGS], OVER_TRIANGLES, nullptr, 0u);
+    HIP_TRY(hipGetLastError());
     uint32_t ntris;
-    TRY(read_u32(d, nt_d, ntris));
-    HIP_TRY(hipMalloc(&M->tris, std::max<size_t>(ntris, 1) * 3 * sizeof(uint32_t)));
-    if (nb)
-        k_cell_emit<<<nb, 256, 0, s>>>(mols, bmol, fbrick, M->values, M->keys, nb, tbase, dense, vbase, mask, wpre,
-                                       words, M->tris, err);
+    TRY(X.size(&cnt[K_NT], E.nt, ntris));
+    TRY(X.output(&M->tris, 3 * (size_t)ntris));
+    k_cell_emit<<<stride_grid(nb, 1), 256, 0, s>>>(mols, bmol, fbrick, M->values, M->keys, &cnt[K_NB], tbase, dense,
+                                                   vbase, mask, wpre, words, M->tris, ntris, &cnt[K_ERR]);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(ev[5], s));
-    // Per-molecule ranges.
-    uint32_t *sfirst, *vfirst, *trfirst;
-    HIP_TRY(T.get(&sfirst, nm + 1));
-    HIP_TRY(T.get(&vfirst, nm + 1));
-    HIP_TRY(T.get(&trfirst, nm + 1));
-    k_mol_summary<<<blocks_for(nm + 1, 256), 256, 0, s>>>(nmu, fbrick, nb, soff, nsw, vbase, nverts, tbase, ntris,
-                                                          sfirst, vfirst, trfirst);
+    TRY(rec(5));
+    k_mol_summary<<<stride_grid(rows + 1), 256, 0, s>>>(rows, fbrick, &cnt[K_NB], soff, &cnt[K_NSW], vbase,
+                                                          &cnt[K_NV], tbase, &cnt[K_NT], sfirst, vfirst, trfirst);
     HIP_TRY(hipGetLastError());
-    std::vector<uint32_t> h_tf(nm + 1), h_fb(nm + 1), h_sf(nm + 1), h_vf(nm + 1), h_trf(nm + 1);
-    std::vector<unsigned long long> h_ns(nm), h_ev(nm);
-    uint32_t bad = 0;
-    HIP_TRY(hipMemcpyAsync(h_tf.data(), tfirst, (nm + 1) * 4, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(h_fb.data(), fbrick, (nm + 1) * 4, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(h_sf.data(), sfirst, (nm + 1) * 4, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(h_vf.data(), vfirst, (nm + 1) * 4, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(h_trf.data(), trfirst, (nm + 1) * 4, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(h_ns.data(), nsamp, nm * 8, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(h_ev.data(), evd, nm * 8, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(&bad, err, 4, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    if (bad) return fail(SMG_E_INTERNAL, "a triangle references an edge without a vertex");
+    HIP_TRY(hipMemcpyAsync(sum_host, sum, UL.bytes, hipMemcpyDeviceToHost, s));
+    return SMG_OK;
+}
+
+// Reads a finished run's summary into the mesh: per-molecule parts and
+// counts. `flags` receives the capacity-overflow bits (the parts are then
+// not set).
+int finish(smg_mesh* M, const Table& t, const char* sum_host, bool timed, smg_mesh_times* times, uint32_t& flags) {
+    SumLayout UL = sum_layout(t.rows);
+    const unsigned long long* nsamp = (const unsigned long long*)(sum_host + UL.nsamp);
+    const unsigned long long* evd = (const unsigned long long*)(sum_host + UL.evals);
+    const uint32_t* cnt = (const uint32_t*)(sum_host + UL.cnt);
+    const uint32_t* tf = (const uint32_t*)(sum_host + UL.tfirst);
+    const uint32_t* fb = (const uint32_t*)(sum_host + UL.fbrick);
+    const uint32_t* sf = (const uint32_t*)(sum_host + UL.sfirst);
+    const uint32_t* vf = (const uint32_t*)(sum_host + UL.vfirst);
+    const uint32_t* trf = (const uint32_t*)(sum_host + UL.trfirst);
+    flags = cnt[K_FLAGS];
+    if (flags) return SMG_OK;
+    if (cnt[K_SOVER0 + TIERS - 1])
+        return fail(SMG_E_INTERNAL, std::to_string(cnt[K_SOVER0 + TIERS - 1]) +
+                                        " sub-blocks exceed the largest candidate lists");
+    if (cnt[K_ROVER0 + TIERS - 1])
+        return fail(SMG_E_INTERNAL, std::to_string(cnt[K_ROVER0 + TIERS - 1]) +
+                                        " sub-blocks exceed the largest refinement lists");
+    if (cnt[K_ERR]) return fail(SMG_E_INTERNAL, "a triangle references an edge without a vertex");
+    const size_t nm = t.nm;
     M->info.assign(nm, smg_mesh_info{});
-    M->bfirst = h_fb;
-    M->vfirst = h_vf;
-    M->tfirst = h_trf;
+    M->bfirst.assign(fb, fb + nm + 1);
+    M->vfirst.assign(vf, vf + nm + 1);
+    M->tfirst.assign(trf, trf + nm + 1);
     for (size_t m = 0; m < nm; m++) {
         smg_mesh_info& I = M->info[m];
-        I.touched_bricks = h_tf[m + 1] - h_tf[m];
-        I.surface_bricks = h_fb[m + 1] - h_fb[m];
-        I.vertices = h_vf[m + 1] - h_vf[m];
-        I.triangles = h_trf[m + 1] - h_trf[m];
-        I.refine_evaluations = h_ev[m];
-        I.sampled_subblocks = h_sf[m + 1] - h_sf[m];
-        I.overflow_subblocks = nm == 1 ? overflow0 : 0;
-        I.evaluated_samples = h_ns[m];
-        for (int q = 0; q < 3; q++) I.frame[q] = H[m].P.frame[q];
-        I.band = H[m].P.band;
-        I.edge_clamp = H[m].P.t_min;
+        I.touched_bricks = tf[m + 1] - tf[m];
+        I.surface_bricks = M->bfirst[m + 1] - M->bfirst[m];
+        I.vertices = M->vfirst[m + 1] - M->vfirst[m];
+        I.triangles = M->tfirst[m + 1] - M->tfirst[m];
+        I.refine_evaluations = evd[m];
+        I.sampled_subblocks = sf[m + 1] - sf[m];
+        I.overflow_subblocks = nm == 1 ? cnt[K_SOVER0] : 0;
+        I.evaluated_samples = nsamp[m];
+        for (int q = 0; q < 3; q++) I.frame[q] = t.H[m].P.frame[q];
+        I.band = t.H[m].P.band;
+        I.edge_clamp = t.H[m].P.t_min;
     }
-    if (times) {
+    if (timed && times) {
+        hipEvent_t* ev = M->dev->ev;
         float ms[5];
         for (int q = 0; q < 5; q++) HIP_TRY(hipEventElapsedTime(&ms[q], ev[q], ev[q + 1]));
         times->classify_ms = ms[0];
@@ -1185,6 +1383,30 @@ static int mesh_many(smg_resident* const* R, const smg_mesh_params* prm, size_t 
         times->k_sample_ms = k[2];
         times->k_refine_ms = k[3];
     }
+    return SMG_OK;
+}
+
+}  // namespace
+
+// The exact path: sizes read back from the device as the run goes.
+static int mesh_many(smg_resident* const* R, const smg_mesh_params* prm, size_t nm, smg_mesh* M,
+                     smg_mesh_times* times) {
+    Table t;
+    TRY(build_table(R, prm, nm, nm, t));
+    smg_device* d = R[0]->dev;
+    HIP_TRY(hipSetDevice(d->ordinal));
+    Temps T(d->stream);
+    Run X;
+    X.d = d;
+    X.s = d->stream;
+    X.T = &T;
+    Extents E{t.rows, t.locals(), t.spheres(), t.words(), t.dense, NO_CAP, NO_CAP, NO_CAP};
+    std::vector<char> stage(stage_layout(t.rows).bytes), sum(sum_layout(t.rows).bytes);
+    TRY(pipeline(X, t, E, stage.data(), sum.data(), M, times != nullptr));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    uint32_t flags = 0;
+    TRY(finish(M, t, sum.data(), times != nullptr, times, flags));
+    if (flags) return fail(SMG_E_INTERNAL, "device counts exceeded the sizes read back");
     return SMG_OK;
 }
 
@@ -1219,10 +1441,263 @@ extern "C" int smg_mesh_new(smg_resident* r, const smg_mesh_params* p, smg_mesh*
     return smg_mesh_many(&r, p, 1, out, times);
 }
 
+// ------------------------------------------------------------------ plans
+struct smg_plan {
+    smg_device* dev = nullptr;
+    smg_plan_caps caps{};
+    std::vector<void*> slots;
+    std::vector<size_t> slot_bytes;
+    char* stage_h = nullptr;  // pinned: the table image the graph uploads
+    char* sum_h = nullptr;    // pinned: the summary the graph reads back
+    size_t stage_bytes = 0, sum_bytes = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    smg_plan_stats stats{};
+    std::vector<smg_mesh*> views;  // meshes viewing the plan's buffers
+    float* lbuf = nullptr;  // slot of the f32 copies (the table image points into it)
+    void* out[9] = {};      // slots of the mesh arrays: keys, values, pos, nrm, atom, patch, t, evals, tris
+    void release() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        lbuf = nullptr;
+        for (void*& o : out) o = nullptr;
+        for (void* p : slots) (void)hipFree(p);
+        slots.clear();
+        slot_bytes.clear();
+        if (stage_h) (void)hipHostFree(stage_h);
+        if (sum_h) (void)hipHostFree(sum_h);
+        stage_h = sum_h = nullptr;
+        stage_bytes = sum_bytes = 0;
+    }
+    ~smg_plan() {
+        for (smg_mesh* v : views) {
+            v->plan = nullptr;
+            v->detached = true;
+        }
+        release();
+    }
+};
+
+smg_mesh::~smg_mesh() {
+    if (plan) {
+        auto& v = plan->views;
+        v.erase(std::remove(v.begin(), v.end(), this), v.end());
+        return;
+    }
+    if (detached) return;
+    void* ps[] = {keys, values, pos, nrm, atom, patch, t, vevals, tris};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+}
+
 extern "C" void smg_mesh_free(smg_mesh* m) {
     if (!m) return;
     (void)hipSetDevice(m->dev->ordinal);
     delete m;
+}
+
+namespace {
+
+Extents plan_extents(const smg_plan_caps& c) {
+    auto u32cap = [](uint64_t v) { return (uint32_t)std::min<uint64_t>(v, NO_CAP - 1); };
+    return Extents{c.molecules, c.locals, c.spheres, c.box_bricks / 32 + c.molecules, c.box_bricks,
+                   u32cap(c.surface_bricks), u32cap(c.vertices), u32cap(c.triangles)};
+}
+
+int check_caps(const smg_plan_caps* c) {
+    if (!c) return fail(SMG_E_INVALID, "null capacities");
+    if (c->molecules == 0 || c->molecules >= (1u << 24)) return fail(SMG_E_INVALID, "plan molecules out of range");
+    if (c->brick < 4 || c->brick > 32 || c->brick % 2) return fail(SMG_E_UNSUPPORTED, "even brick size in 4..=32");
+    if (c->refine != SMG_REFINE_EXACT && c->refine != SMG_REFINE_FAST) return fail(SMG_E_INVALID, "refinement mode");
+    if (c->locals >= (1ull << 32) || c->spheres >= (1ull << 32) || c->box_bricks >= (1ull << 32) ||
+        c->surface_bricks >= (1ull << 29) || c->vertices >= (1ull << 32) || c->triangles >= (1ull << 32))
+        return fail(SMG_E_LENGTH, "plan capacities beyond 32-bit counts");
+    return SMG_OK;
+}
+
+// Runs the plan's pipeline once on the stream (planned sizes); with
+// `capture`, records it into the plan's graph instead.
+int plan_pipeline(smg_plan* p, Table& t, smg_mesh* M, bool capture) {
+    smg_device* d = p->dev;
+    Run X;
+    X.d = d;
+    X.s = d->stream;
+    X.slots = &p->slots;
+    X.slot_bytes = &p->slot_bytes;
+    Extents E = plan_extents(p->caps);
+    if (!capture) {
+        TRY(pipeline(X, t, E, p->stage_h, p->sum_h, M, false));
+        p->lbuf = X.lbuf;
+        void* o[9] = {M->keys, M->values, M->pos, M->nrm, M->atom, M->patch, M->t, M->vevals, M->tris};
+        for (int q = 0; q < 9; q++) p->out[q] = o[q];
+        return SMG_OK;
+    }
+    HIP_TRY(hipStreamBeginCapture(d->stream, hipStreamCaptureModeRelaxed));
+    int rc = pipeline(X, t, E, p->stage_h, p->sum_h, M, false);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(d->stream, &g);
+    if (rc != SMG_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    HIP_TRY(e);
+    p->graph = g;
+    HIP_TRY(hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0));
+    p->stats.captures++;
+    return SMG_OK;
+}
+
+}  // namespace
+
+extern "C" int smg_plan_new(smg_device* dev, const smg_plan_caps* caps, smg_plan** out) {
+    if (!out) return fail(SMG_E_INVALID, "null output");
+    *out = nullptr;
+    if (!dev) return fail(SMG_E_INVALID, "null device");
+    TRY(check_caps(caps));
+    smg_plan* p = new smg_plan();
+    p->dev = dev;
+    p->caps = *caps;
+    *out = p;
+    return SMG_OK;
+}
+
+extern "C" void smg_plan_free(smg_plan* p) {
+    if (!p) return;
+    (void)hipSetDevice(p->dev->ordinal);
+    (void)hipStreamSynchronize(p->dev->stream);
+    delete p;
+}
+
+extern "C" int smg_plan_stats_get(const smg_plan* p, smg_plan_stats* s) {
+    if (!p || !s) return fail(SMG_E_INVALID, "null plan");
+    *s = p->stats;
+    return SMG_OK;
+}
+
+extern "C" int smg_plan_caps_for(smg_resident* const* residents, const smg_mesh_params* params, size_t count,
+                                 const smg_mesh* m, double margin, smg_plan_caps* caps) {
+    if (!residents || !params || !caps || count == 0) return fail(SMG_E_INVALID, "no molecules");
+    if (!(margin >= 0.0 && margin <= 100.0)) return fail(SMG_E_INVALID, "margin out of range");
+    Table t;
+    TRY(build_table(residents, params, count, count, t));
+    smg_plan_caps c{};
+    c.molecules = (uint32_t)count;
+    c.brick = t.b;
+    c.refine = t.fast ? SMG_REFINE_FAST : SMG_REFINE_EXACT;
+    c.locals = t.locals();
+    c.spheres = t.spheres();
+    c.box_bricks = t.dense;
+    if (m) {
+        if (m->info.size() != count) return fail(SMG_E_INVALID, "mesh is not of these molecules");
+        auto grow = [&](uint64_t v) { return (uint64_t)std::ceil((double)v * (1.0 + margin)) + 1; };
+        c.surface_bricks = grow(m->bfirst[count]);
+        c.vertices = grow(m->vfirst[count]);
+        c.triangles = grow(m->tfirst[count]);
+    }
+    *caps = c;
+    return SMG_OK;
+}
+
+extern "C" int smg_plan_run(smg_plan* p, smg_resident* const* residents, const smg_mesh_params* params, size_t count,
+                            smg_mesh** out, smg_mesh_times* times) {
+    if (!out) return fail(SMG_E_INVALID, "null output");
+    *out = nullptr;
+    if (!p || !residents || !params || count == 0) return fail(SMG_E_INVALID, "no molecules");
+    for (size_t m = 0; m < count; m++)
+        if (!residents[m] || residents[m]->dev != p->dev) return fail(SMG_E_INVALID, "molecule not on the plan's device");
+    if (!p->views.empty()) return fail(SMG_E_INVALID, "free the mesh of the plan's previous run first");
+    if (params[0].brick != p->caps.brick || params[0].refine != p->caps.refine)
+        return fail(SMG_E_INVALID, "brick size and refinement mode must be the plan's");
+    if (times) std::memset(times, 0, sizeof(*times));
+    smg_device* d = p->dev;
+    HIP_TRY(hipSetDevice(d->ordinal));
+    p->stats.runs++;
+    Table t;
+    uint32_t flags = 0;
+    if (count > p->caps.molecules) {
+        flags = OVER_TABLE;
+    } else {
+        TRY(build_table(residents, params, count, p->caps.molecules, t));
+        if (t.locals() > p->caps.locals || t.spheres() > p->caps.spheres || t.dense > p->caps.box_bricks)
+            flags = OVER_TABLE;
+    }
+    if (!flags) {
+        if (!p->stage_h) {
+            p->stage_bytes = stage_layout(t.rows).bytes;
+            p->sum_bytes = sum_layout(t.rows).bytes;
+            HIP_TRY(hipHostMalloc((void**)&p->stage_h, p->stage_bytes, hipHostMallocDefault));
+            HIP_TRY(hipHostMalloc((void**)&p->sum_h, p->sum_bytes, hipHostMallocDefault));
+        }
+        std::unique_ptr<smg_mesh> V(new smg_mesh());
+        V->dev = d;
+        V->plan = p;  // a view: the arrays are the plan's
+        p->views.push_back(V.get());
+        V->spb = (t.b + 1) * (t.b + 1) * (t.b + 1);
+        HIP_TRY(hipEventRecord(d->ev[14], d->stream));
+        if (p->exec) {
+            // The graph uploads the table image and uses the buffers of the
+            // capture: refill the image, then replay.
+            write_stage(t, p->lbuf, p->stage_h);
+            HIP_TRY(hipGraphLaunch(p->exec, d->stream));
+            p->stats.replays++;
+            V->keys = (int32_t*)p->out[0];
+            V->values = (float*)p->out[1];
+            V->pos = (float*)p->out[2];
+            V->nrm = (float*)p->out[3];
+            V->atom = (uint32_t*)p->out[4];
+            V->patch = (uint8_t*)p->out[5];
+            V->t = (double*)p->out[6];
+            V->vevals = (uint32_t*)p->out[7];
+            V->tris = (uint32_t*)p->out[8];
+        } else {
+            TRY(plan_pipeline(p, t, V.get(), false));
+        }
+        HIP_TRY(hipEventRecord(d->ev[15], d->stream));
+        HIP_TRY(hipStreamSynchronize(d->stream));
+        TRY(finish(V.get(), t, p->sum_h, false, nullptr, flags));
+        if (!flags) {
+            if (!p->exec) {
+                // Record the graph the next runs replay (nothing executes).
+                smg_mesh C;
+                C.dev = d;
+                C.plan = p;
+                p->views.push_back(&C);
+                TRY(plan_pipeline(p, t, &C, true));
+            }
+            if (times) {
+                float ms = 0.0f;
+                HIP_TRY(hipEventElapsedTime(&ms, d->ev[14], d->ev[15]));
+                times->total_ms = ms;
+            }
+            *out = V.release();
+            return SMG_OK;
+        }
+    }
+    // A count beyond the plan: the exact path makes the mesh, and the plan
+    // grows to hold this run (buffers and graph rebuilt on the next run).
+    p->stats.fallbacks++;
+    p->stats.last_overflow = flags;
+    smg_mesh* M = nullptr;
+    TRY(smg_mesh_many(residents, params, count, &M, times));
+    smg_plan_caps c{};
+    if (smg_plan_caps_for(residents, params, count, M, 0.25, &c) == SMG_OK) {
+        smg_plan_caps& P = p->caps;
+        P.molecules = std::max(P.molecules, c.molecules);
+        P.locals = std::max(P.locals, c.locals);
+        P.spheres = std::max(P.spheres, c.spheres);
+        P.box_bricks = std::max(P.box_bricks, c.box_bricks);
+        P.surface_bricks = std::max(P.surface_bricks, c.surface_bricks);
+        P.vertices = std::max(P.vertices, c.vertices);
+        P.triangles = std::max(P.triangles, c.triangles);
+        if (check_caps(&P) == SMG_OK) {
+            (void)hipStreamSynchronize(d->stream);
+            p->release();
+        }
+    }
+    *out = M;
+    return SMG_OK;
 }
 
 extern "C" size_t smg_mesh_count(const smg_mesh* m) { return m ? m->info.size() : 0; }
@@ -1241,6 +1716,7 @@ extern "C" int smg_mesh_part_copy(const smg_mesh* m, size_t part, float* positio
                                   uint32_t* evals) {
     if (!m) return fail(SMG_E_INVALID, "null mesh");
     if (part >= m->info.size()) return fail(SMG_E_INVALID, "no part " + std::to_string(part));
+    if (m->detached) return fail(SMG_E_INVALID, "the mesh viewed a plan that has been freed");
     HIP_TRY(hipSetDevice(m->dev->ordinal));
     size_t v0 = m->vfirst[part], nv = m->vfirst[part + 1] - v0;
     size_t t0 = m->tfirst[part], nt = m->tfirst[part + 1] - t0;
