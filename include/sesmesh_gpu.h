@@ -206,6 +206,28 @@ int smg_mesh_part_copy(const smg_mesh* m, size_t part, float* positions, float* 
                        uint8_t* patches, uint32_t* triangles, int32_t* bricks, float* values, double* t,
                        uint32_t* evals);
 
+/* ----------------------------------------------------------------- tuning
+ * Launch shapes of the mesh stage on a device. Every setting gives the same
+ * meshes, byte for byte; the settings trade LDS capacity, occupancy and
+ * lanes per edge. No counterpart in the reference, whose launch shapes are
+ * fixed (crates/sesmesh-gpu/src/hip/resident.rs:675-683). */
+typedef struct {
+    int32_t sample_first_tier; /* 0..2: LDS capacity tier the sampling passes start at */
+    int32_t refine_first_tier; /* 0..2: the same for refinement */
+    int32_t refine_group_max;  /* power of two <= 64: most lanes refining one edge */
+    int32_t uniform_shared;    /* 1: sub-block states from one shared gather per brick; 0: a wavefront per brick */
+    int32_t reserved;
+} smg_tuning;
+/* Sets the tuning of the device's next runs (NULL: the defaults). */
+int smg_device_set_tuning(smg_device* dev, const smg_tuning* tuning);
+int smg_device_get_tuning(const smg_device* dev, smg_tuning* tuning);
+/* Autotuner: times the mesh stage of these molecules (device time, best of
+ * `repeats` runs) under candidate tunings, one knob at a time from the
+ * device's current tuning, keeps the fastest on the device and returns it
+ * with its time in ms. Plans capture the tuning current when they record. */
+int smg_tune(smg_resident* const* residents, const smg_mesh_params* params, size_t count, int repeats,
+             smg_tuning* best, double* best_ms);
+
 /* ------------------------------------------------------------------ plans
  * A capture-ready mesh stage for serving loops. No counterpart in the
  * reference: it is the loop a service runs around fold_resident

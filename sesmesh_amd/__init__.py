@@ -96,6 +96,20 @@ class PlanCaps(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
 
 
+class Tuning(ctypes.Structure):
+    """smg_tuning of include/sesmesh_gpu.h: launch shapes that leave the
+    meshes unchanged."""
+    _fields_ = [("sample_first_tier", ctypes.c_int32), ("refine_first_tier", ctypes.c_int32),
+                ("refine_group_max", ctypes.c_int32), ("uniform_shared", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+DEFAULT_TUNING = dict(sample_first_tier=0, refine_first_tier=0, refine_group_max=64, uniform_shared=1)
+
+
 class PlanStats(ctypes.Structure):
     _fields_ = [("runs", ctypes.c_uint64), ("replays", ctypes.c_uint64), ("fallbacks", ctypes.c_uint64),
                 ("captures", ctypes.c_uint64), ("last_overflow", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
@@ -111,7 +125,8 @@ ABI_FUNCTIONS = [
     "smg_arrangement_free", "smg_arrangement_view", "smg_resident_new", "smg_resident_free", "smg_resident_bytes",
     "smg_mesh_new", "smg_mesh_free", "smg_mesh_info_get", "smg_mesh_copy", "smg_mesh_many", "smg_mesh_count",
     "smg_mesh_part_info", "smg_mesh_part_copy", "smg_field_clamped", "smg_plan_new", "smg_plan_caps_for",
-    "smg_plan_run", "smg_plan_stats_get", "smg_plan_free",
+    "smg_plan_run", "smg_plan_stats_get", "smg_plan_free", "smg_device_set_tuning", "smg_device_get_tuning",
+    "smg_tune",
 ]
 
 _lib = None
@@ -146,6 +161,8 @@ def lib():
                                         ctypes.c_double, ctypes.c_void_p]
         L.smg_plan_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
                                    ctypes.c_void_p, ctypes.c_void_p]
+        L.smg_tune.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p,
+                               ctypes.c_void_p]
         for name in ("smg_device_close", "smg_arrangement_free", "smg_resident_free", "smg_mesh_free",
                      "smg_plan_free"):
             getattr(L, name).restype = None
@@ -178,6 +195,21 @@ class Device:
         self.total_memory = info.total_memory
         self.lds_per_block = info.lds_per_block
         self.wavefront = info.wavefront
+
+    @property
+    def tuning(self):
+        t = Tuning()
+        _check(lib().smg_device_get_tuning(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
+    @tuning.setter
+    def tuning(self, values):
+        """Sets the launch shapes of the next runs (None: the defaults)."""
+        if values is None:
+            _check(lib().smg_device_set_tuning(self._h, None))
+            return
+        t = Tuning(**{**DEFAULT_TUNING, **dict(values)})
+        _check(lib().smg_device_set_tuning(self._h, ctypes.byref(t)))
 
     def selftest(self, n=1 << 16, seed=1):
         m32, m64 = ctypes.c_uint64(), ctypes.c_uint64()
@@ -391,6 +423,26 @@ class Plan:
         if h:
             lib().smg_plan_free(h)
             self._h = _vp()
+
+
+def tune(residents, spacings, brick=10, refine="exact", repeats=2, bricks=None):
+    """Autotuner over the launch shapes (smg_tune: coordinate descent on the
+    device time of the mesh stage of `residents`), and over the brick size
+    when `bricks` lists candidates (the meshes then equal the CPU back end's
+    at the chosen brick size). Leaves the fastest tuning on the device and
+    returns {"brick", "tuning", "ms", "trials"}."""
+    dev = residents[0].device
+    trials = []
+    for b in (bricks or [brick]):
+        params = (MeshParams * len(residents))(*[_mesh_params(float(h), int(b), refine, 0) for h in spacings])
+        handles = (ctypes.c_void_p * len(residents))(*[r._h.value for r in residents])
+        dev.tuning = None
+        best, ms = Tuning(), ctypes.c_double()
+        _check(lib().smg_tune(handles, params, len(residents), int(repeats), ctypes.byref(best), ctypes.byref(ms)))
+        trials.append({"brick": int(b), "tuning": best.as_dict(), "ms": ms.value})
+    win = min(trials, key=lambda t: t["ms"])
+    dev.tuning = win["tuning"]
+    return {**win, "trials": trials}
 
 
 @dataclass

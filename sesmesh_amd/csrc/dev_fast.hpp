@@ -442,7 +442,7 @@ __device__ __forceinline__ void refine_sub_fast(const Mol* __restrict__ mols, co
     to_local(s, f, n, c);
     const float* val = values + (size_t)brick * P.spb;
     const uint32_t ne = n[R_E];
-    uint32_t g = WAVE;
+    uint32_t g = cap.gmax ? cap.gmax : (uint32_t)WAVE;
     while (g > 1 && blockDim.x / g < ne) g >>= 1;
     Group G;
     G.g = g;

@@ -97,9 +97,11 @@ __device__ inline bool edge_is_crossing(const unsigned long long* mask, uint32_t
 // Capacities of the refinement lists: near spheres (a multiple of 64: the
 // blocker bitsets hold near/64 words), inside group, candidates (a power
 // of two), hash slots (a power of two) and crossing edges; `radii` 1 keeps
-// each near sphere's expanded and van der Waals radii too.
+// each near sphere's expanded and van der Waals radii too. `gmax` bounds
+// the lanes per edge (0: a wavefront); any power of two gives the same
+// vertices.
 struct RCaps {
-    uint32_t near, inside, cand, hash, edges, radii;
+    uint32_t near, inside, cand, hash, edges, radii, gmax;
 };
 
 // LDS / scratch layout of the refinement lists (byte offsets).
@@ -779,8 +781,8 @@ __device__ __forceinline__ void refine_sub(const Mol* __restrict__ mols, const u
     }
     const float* val = values + (size_t)brick * P.spb;
     const uint32_t ne = n[R_E];
-    // Lane groups: the largest g <= 64 with at least `ne` groups.
-    uint32_t g = WAVE;
+    // Lane groups: the largest g <= gmax with at least `ne` groups.
+    uint32_t g = cap.gmax ? cap.gmax : (uint32_t)WAVE;
     while (g > 1 && blockDim.x / g < ne) g >>= 1;
     Group G;
     G.g = g;

@@ -279,6 +279,7 @@ struct smg_device {
     // kernel and dynamic LDS size: the grid of a persistent launch.
     std::map<std::pair<const void*, size_t>, uint32_t> resident;
     std::mutex mu;
+    smg_tuning tune{0, 0, 64, 1, 0};  // launch shapes of the next runs (smg_tune)
 };
 
 struct smg_arrangement {
@@ -751,11 +752,13 @@ constexpr Caps SAMPLE_CAPS[TIERS] = {{128, 64, 256, 64, 256},
                                      {256, 128, 384, 128, 512},
                                      {640, 256, 768, 256, 2048},
                                      {1024, 512, 4096, 2048, 4096}};
-constexpr RCaps REFINE_CAPS[TIERS] = {
-    {128, 64, 256, 256, 0}, {256, 128, 384, 512, 0}, {640, 256, 704, 1024, 0}, {1024, 512, 4096, 4096, 0}};
+constexpr RCaps REFINE_CAPS[TIERS] = {{128, 64, 256, 256, 0, 0, 0},
+                                      {256, 128, 384, 512, 0, 0, 0},
+                                      {640, 256, 704, 1024, 0, 0, 0},
+                                      {1024, 512, 4096, 4096, 0, 0, 0}};
 // Sub-block classification: one LDS tier; bricks beyond it take one
 // wavefront per brick, which beats larger tiers at one or two workgroups per CU.
-constexpr RCaps UNIFORM_CAPS = {192, 128, 256, 512, 0, 1};
+constexpr RCaps UNIFORM_CAPS = {192, 128, 256, 512, 0, 1, 0};
 constexpr uint32_t LDS_EDGES = 512;      // crossing-edge buffer of the LDS passes
 constexpr uint32_t SCRATCH_CHUNK = 256;   // workgroups of a scratch pass (one slice each)
 constexpr uint32_t STATIC_LDS = 6144;     // static LDS of k_sample / k_refine, bound
@@ -1059,6 +1062,7 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     const uint32_t b = t.b, spb = (b + 1) * (b + 1) * (b + 1);
     const uint32_t rows = (uint32_t)E.rows;
     const bool fast = t.fast;
+    const smg_tuning tune = d->tune;
     M->spb = spb;
     auto rec = [&](int i) -> int {
         if (timed) HIP_TRY(hipEventRecord(d->ev[i], s));
@@ -1137,7 +1141,7 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
         const size_t lds_max = d->prop.sharedMemPerBlock;
         RLayout lay = make_rlayout(UNIFORM_CAPS);
         Queue rest{nullptr, &cnt[K_NB], &cnt[K_Q_SUBST]};
-        if (lay.bytes + STATIC_LDS <= lds_max) {
+        if (tune.uniform_shared && lay.bytes + STATIC_LDS <= lds_max) {
             uint32_t* over;
             TRY(X.temp(&over, nb));
             Queue q{nullptr, &cnt[K_NB], nullptr};
@@ -1179,7 +1183,7 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
             const Caps& cap = SAMPLE_CAPS[pass];
             Layout lay = make_layout(cap);
             bool lds = pass < TIERS - 1;
-            if (lds && lay.bytes + STATIC_LDS > lds_max) continue;
+            if (lds && (pass < tune.sample_first_tier || lay.bytes + STATIC_LDS > lds_max)) continue;
             uint32_t* next;
             TRY(X.temp(&next, nsub));
             Queue q{list, count, &cnt[K_Q_SAMPLE0 + pass]};
@@ -1251,10 +1255,11 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
             RCaps cap = REFINE_CAPS[pass];
             bool lds = pass < TIERS - 1;
             cap.edges = lds ? std::min(sub_edges, LDS_EDGES) : sub_edges;
+            cap.gmax = (uint32_t)tune.refine_group_max;
             RLayout lay = make_rlayout(cap);
             FLayout flay = make_flayout(cap, lay.bytes);
             const uint32_t bytes = fast ? flay.bytes : lay.bytes;
-            if (lds && bytes + STATIC_LDS > lds_max) continue;
+            if (lds && (pass < tune.refine_first_tier || bytes + STATIC_LDS > lds_max)) continue;
             uint32_t* next;
             TRY(X.temp(&next, nsub));
             Queue q{list, count, &cnt[K_Q_REFINE0 + pass]};
@@ -1439,6 +1444,83 @@ extern "C" int smg_mesh_new(smg_resident* r, const smg_mesh_params* p, smg_mesh*
     *out = nullptr;
     if (!r || !p) return fail(SMG_E_INVALID, "null resident or parameters");
     return smg_mesh_many(&r, p, 1, out, times);
+}
+
+// ------------------------------------------------------------------ tuning
+static int check_tuning(const smg_tuning& t) {
+    if (t.sample_first_tier < 0 || t.sample_first_tier > TIERS - 2 || t.refine_first_tier < 0 ||
+        t.refine_first_tier > TIERS - 2)
+        return fail(SMG_E_INVALID, "first tiers must be LDS tiers 0.." + std::to_string(TIERS - 2));
+    int g = t.refine_group_max;
+    if (g < 1 || g > WAVE || (g & (g - 1))) return fail(SMG_E_INVALID, "refine_group_max must be a power of two <= 64");
+    if (t.uniform_shared != 0 && t.uniform_shared != 1) return fail(SMG_E_INVALID, "uniform_shared is 0 or 1");
+    return SMG_OK;
+}
+
+extern "C" int smg_device_set_tuning(smg_device* d, const smg_tuning* t) {
+    if (!d) return fail(SMG_E_INVALID, "null device");
+    smg_tuning v = t ? *t : smg_tuning{0, 0, 64, 1, 0};
+    TRY(check_tuning(v));
+    d->tune = v;
+    return SMG_OK;
+}
+
+extern "C" int smg_device_get_tuning(const smg_device* d, smg_tuning* t) {
+    if (!d || !t) return fail(SMG_E_INVALID, "null device");
+    *t = d->tune;
+    return SMG_OK;
+}
+
+// Device time (ms, best of `repeats`) of the mesh stage under tuning t.
+static int time_tuning(smg_resident* const* rs, const smg_mesh_params* ps, size_t count, int repeats,
+                       const smg_tuning& t, double& best) {
+    smg_device* d = rs[0]->dev;
+    d->tune = t;
+    best = INFINITY;
+    for (int r = 0; r < repeats; r++) {
+        smg_mesh* m = nullptr;
+        smg_mesh_times tm{};
+        TRY(smg_mesh_many(rs, ps, count, &m, &tm));
+        smg_mesh_free(m);
+        best = std::min(best, tm.total_ms);
+    }
+    return SMG_OK;
+}
+
+extern "C" int smg_tune(smg_resident* const* rs, const smg_mesh_params* ps, size_t count, int repeats,
+                        smg_tuning* best, double* best_ms) {
+    if (!rs || !ps || count == 0 || !rs[0]) return fail(SMG_E_INVALID, "no molecules");
+    if (repeats < 1 || repeats > 100) return fail(SMG_E_INVALID, "repeats out of range");
+    smg_device* d = rs[0]->dev;
+    smg_tuning cur = d->tune, start = cur;
+    double cur_ms = 0.0;
+    int rc = time_tuning(rs, ps, count, repeats, cur, cur_ms);
+    // Coordinate descent over the knobs, from the device's current tuning.
+    const int tiers[] = {0, 1, 2};
+    const int groups[] = {64, 32, 16, 8};
+    const int shared[] = {1, 0};
+    for (int knob = 0; knob < 4 && rc == SMG_OK; knob++) {
+        const int* vals = knob < 2 ? tiers : (knob == 2 ? groups : shared);
+        int nv = knob < 2 ? 3 : (knob == 2 ? 4 : 2);
+        for (int i = 0; i < nv && rc == SMG_OK; i++) {
+            smg_tuning t = cur;
+            int* field = knob == 0 ? &t.sample_first_tier
+                                   : (knob == 1 ? &t.refine_first_tier : (knob == 2 ? &t.refine_group_max : &t.uniform_shared));
+            if (*field == vals[i]) continue;
+            *field = vals[i];
+            double ms = 0.0;
+            rc = time_tuning(rs, ps, count, repeats, t, ms);
+            if (rc == SMG_OK && ms < cur_ms) {
+                cur = t;
+                cur_ms = ms;
+            }
+        }
+    }
+    d->tune = rc == SMG_OK ? cur : start;
+    if (rc != SMG_OK) return rc;
+    if (best) *best = cur;
+    if (best_ms) *best_ms = cur_ms;
+    return SMG_OK;
 }
 
 // ------------------------------------------------------------------ plans
