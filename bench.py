@@ -8,10 +8,15 @@ stage on the device: classification, candidate gather and sampling,
 crossing edges, f64 vertex refinement, cell pass and assembly, with the
 arrangement resident in HBM (its host build is reported, not timed).
 
-Also measured: batched throughput on seeded molecules of 20-5000 atoms
-(molecules/s), and, unless --no-cpu-baseline, the CPU oracle (a scalar C++
-restatement of the reference CPU back end) on a bounded sample of the same
-workload with all host threads.
+The timed steps replay a capture-ready plan: the whole mesh stage recorded
+once as a hipGraph over buffers sized once (the serving path); every step
+runs every kernel. Also measured: the plain path (one smg_mesh_new per
+mesh), the autotuner's launch shapes, batched throughput on seeded
+molecules of 20-5000 atoms (molecules/s: per-molecule launches, one batched
+pass, a replayed plan), the learned-sizing study, and, unless
+--no-cpu-baseline, the CPU oracle (a scalar C++ restatement of the
+reference CPU back end) on a bounded sample of the same workload with all
+host threads.
 
 Prints one JSON line.
 """
@@ -126,8 +131,9 @@ def batched(dev, count, seed, repeats=3):
     """Batched serving on `count` seeded molecules (atoms log-uniform in
     20-5000, probe 1.2-1.4 A, spacing 0.2/0.25/0.5 A): arrangements built and
     resident first. Timed: per-molecule launches of the mesh stage ("plain")
-    against one batched pass over all of them (smg_mesh_many); best of
-    `repeats` each. Every part's counts must equal its plain run's."""
+    against one batched pass over all of them (smg_mesh_many) and against a
+    capture-ready plan replaying that pass as a hipGraph; best of `repeats`
+    each. Every part's counts must equal its plain run's."""
     import sesmesh_amd
 
     atoms, probe, spacing, seeds = sesmesh_amd.synth_batch_spec(seed, count)
@@ -155,11 +161,74 @@ def batched(dev, count, seed, repeats=3):
             raise RuntimeError("batched parts differ from their single-molecule runs")
         del mb
     progress(f"batched: one pass {min(batch_s):.3f} s")
+    # The same batch through a capture-ready plan: one graph replay per pass.
+    mb = sesmesh_amd.mesh_many(residents, spacing, 10)
+    plan = sesmesh_amd.Plan(dev, sesmesh_amd.Plan.caps_for(residents, spacing, mesh=mb, margin=0.05))
+    del mb
+    plan_s = []
+    for i in range(repeats + 1):
+        t = time.perf_counter()
+        v = plan.run(residents, spacing)
+        dt = time.perf_counter() - t
+        counts = [(p.vertices, p.triangles, p.refine_evaluations) for p in v.parts]
+        if counts != plain_counts:
+            raise RuntimeError("plan parts differ from their single-molecule runs")
+        del v
+        if i:
+            plan_s.append(dt)
+    progress(f"batched: plan replay {min(plan_s):.3f} s")
     verts = int(sum(c[0] for c in plain_counts))
     return {"molecules": count, "atoms": int(atoms.sum()), "vertices": verts,
             "plain": {"seconds": min(plain_s), "molecules_per_s": count / min(plain_s)},
             "batched": {"seconds": min(batch_s), "molecules_per_s": count / min(batch_s)},
-            "speedup": min(plain_s) / min(batch_s)}
+            "plan": {"seconds": min(plan_s), "molecules_per_s": count / min(plan_s), "stats": plan.stats()},
+            "speedup": min(plain_s) / min(batch_s), "plan_speedup": min(plain_s) / min(plan_s)}
+
+
+def sizing_study(dev, rings, brick):
+    """Learned sizing: the model trained on the device on touched bricks of
+    seeded globules (100 to 50k atoms, 0.35-1.0 A) labelled by the exact
+    pipeline, then the 58k-atom assembly at 0.75 and 1.0 A meshed without
+    sizing, with the default placement and with placement on every tier:
+    device times and wasted gathers (retries). Meshes must be identical."""
+    import sesmesh_amd
+
+    model = sesmesh_amd.Model(dev, seed=7)
+    t = time.perf_counter()
+    spec = [(2000, 0.35), (6000, 0.4), (15000, 0.5), (30000, 0.6), (4000, 0.75), (20000, 0.8), (12000, 1.0),
+            (40000, 1.0), (8000, 0.9), (25000, 0.7), (700, 0.5), (50000, 0.45)]
+    for i, (n, h) in enumerate(spec):
+        model.collect([sesmesh_amd.Resident(dev, sesmesh_amd.Arrangement(sesmesh_amd.synth_globule(100 + i, n), 1.4))],
+                      [h])
+    collect_s = time.perf_counter() - t
+    before = model.train(epochs=0, batch=4096, lr=1e-3)
+    after = model.train(epochs=40, batch=4096, lr=2e-3)
+    model.use()
+    out = {"training": after, "untrained": before, "collect_s": collect_s, "molecules": len(spec), "runs": {}}
+    try:
+        for h in (0.75, 1.0):
+            rows = {}
+            ref = None
+            for name, sizing, policy in (("none", 0, None), ("default", 1, (0.3, 1)), ("every_tier", 1, (1.0, 2))):
+                if policy:
+                    model.policy(*policy)
+                best = None
+                for _ in range(3):
+                    m = rings.mesh(h, brick, sizing=sizing)
+                    if best is None or m.times.total_ms < best[0]:
+                        best = (m.times.total_ms, m.times.sample_ms, m.times.refine_ms)
+                    a = m.arrays()
+                    if ref is None:
+                        ref = a
+                    elif not all(np.array_equal(ref[k], a[k]) for k in ("positions", "triangles")):
+                        raise RuntimeError("sizing changed the mesh")
+                rows[name] = {"total_ms": best[0], "sample_ms": best[1], "refine_ms": best[2],
+                              "retries": int(m.info.sample_retries + m.info.refine_retries)}
+            out["runs"][str(h)] = rows
+    finally:
+        model.policy()
+        sesmesh_amd.lib().smg_device_set_model(dev._h, None)
+    return out
 
 
 def main():
@@ -173,6 +242,8 @@ def main():
     ap.add_argument("--precision", type=int, default=0, metavar="COPIES",
                     help="precision study: exact vs fast refinement on COPIES (60: the 2.4M-atom shell) "
                          "icosahedral copies of a 40k-atom globule at 1.0 A (0: skip)")
+    ap.add_argument("--no-sizing", dest="sizing", action="store_false",
+                    help="skip the learned-sizing study (training on seeded globules, 0.75 / 1.0 A meshes)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the meshes, one device-to-device copy of this many MiB (the known-size kernel "
                          "that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -192,20 +263,45 @@ def main():
     t_arr = time.perf_counter() - t0
     dev = sesmesh_amd.Device(0)
     res = sesmesh_amd.Resident(dev, arr)
-    for _ in range(args.warmup):
+
+    # The plain GPU path: one smg_mesh_new per mesh (exact counts read back,
+    # outputs allocated and freed per mesh), default launch shapes.
+    plain = []
+    for i in range(args.warmup + max(2, args.steps // 2)):
+        t = time.perf_counter()
         m = res.mesh(spacing, brick)
         del m
+        if i >= args.warmup:
+            plain.append(time.perf_counter() - t)
+    plain_s = float(np.median(plain))
+    progress(f"plain path: {plain_s:.4f} s/mesh")
+
+    # Autotuner (untimed): launch shapes for this workload.
+    tuned = sesmesh_amd.tune([res], [spacing], brick=brick, repeats=2)
+    progress(f"autotuner: {tuned['tuning']} ({tuned['ms']:.2f} ms device time)")
+
+    # Timed steps: the capture-ready plan (buffers sized once, the whole
+    # pipeline replayed as a hipGraph each step, phase events recorded in
+    # the graph). Warm-up builds the plan and records the graph.
+    m = res.mesh(spacing, brick)
+    plan = sesmesh_amd.Plan(dev, sesmesh_amd.Plan.caps_for([res], [spacing], mesh=m, margin=0.05))
+    del m
+    for _ in range(max(2, args.warmup)):
+        v = plan.run([res], [spacing])
+        del v
     times, phase = [], []
     for _ in range(args.steps):
         t = time.perf_counter()
-        m = res.mesh(spacing, brick)
+        v = plan.run([res], [spacing])
         times.append(time.perf_counter() - t)
-        phase.append(m.times.as_dict())
-        info = m.info
-        del m
+        phase.append(v.times.as_dict())
+        info = v.info
+        del v
     s_per_mesh = sum(times) / args.steps
-    progress(f"GPU: {s_per_mesh:.4f} s/mesh over {args.steps} steps")
+    stats = plan.stats()
+    progress(f"GPU: {s_per_mesh:.4f} s/mesh over {args.steps} steps (plan replays {stats['replays']})")
     ph = {k: float(np.median([p[k] for p in phase])) for k in phase[0]}
+    del plan
     if args.copy_mib:
         device_copy(args.copy_mib << 20)
         progress(f"copied {args.copy_mib} MiB device to device")
@@ -257,8 +353,16 @@ def main():
     if args.batch:
         progress(f"batched: {args.batch} molecules")
         batch = batched(dev, args.batch, seed=4)
-        progress(f"batched: {batch['batched']['molecules_per_s']:.1f} molecules/s, "
-                 f"{batch['speedup']:.2f}x the plain per-molecule launches")
+        progress(f"batched: {batch['batched']['molecules_per_s']:.1f} molecules/s one pass, "
+                 f"{batch['plan']['molecules_per_s']:.1f} replayed plan; "
+                 f"{batch['speedup']:.2f}x / {batch['plan_speedup']:.2f}x the plain per-molecule launches")
+
+    sizing = None
+    if args.sizing:
+        progress("sizing study: collecting and training")
+        sizing = sizing_study(dev, res, brick)
+        progress(f"sizing study: accuracy {sizing['training']['occupancy_accuracy']:.3f}, "
+                 f"load rmse {sizing['training']['load_rmse']:.3f}")
 
     out = {
         "metric": METRIC, "value": s_per_mesh, "unit": "s/mesh", "n_gpus": 1, "steps": args.steps,
@@ -273,8 +377,12 @@ def main():
         "roofline": roofline,
         "cpu_baseline": ({k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")} if cpu else None),
         "best_s": min(times), "median_s": float(np.median(times)),
+        "path": ("capture-ready plan: whole mesh stage replayed as a hipGraph per step, buffers sized once, "
+                 "autotuned launch shapes"),
+        "plain_path_s": plain_s, "speedup_vs_plain_path": plain_s / s_per_mesh,
+        "tuning": tuned["tuning"], "plan_stats": stats,
         "phases_ms": ph, "kernels": others, "arrangement_s": t_arr,
-        "batched": batch, "precision_study": precision, "device": dev.name,
+        "batched": batch, "sizing_study": sizing, "precision_study": precision, "device": dev.name,
     }
     if cpu:
         out["speedup_vs_cpu_baseline"] = cpu["value"] / s_per_mesh

@@ -41,6 +41,7 @@ typedef struct smg_arrangement smg_arrangement;
 typedef struct smg_resident smg_resident;
 typedef struct smg_mesh smg_mesh;
 typedef struct smg_plan smg_plan;
+typedef struct smg_model smg_model;
 typedef struct smg_batch smg_batch;
 
 /* Thread-local message of the last failure (never NULL). */
@@ -135,7 +136,7 @@ typedef struct {
     double spacing;
     uint32_t brick;       /* even, 4..=32 (2x2x2 sub-blocks) */
     int32_t refine;       /* SMG_REFINE_* */
-    int32_t sizing;       /* 0 count-scan-fill lists; 1 learned/LDS sizing */
+    int32_t sizing;       /* 0: lists tried from the first LDS tier; 1: from the tier the device's sizing model predicts */
     int32_t reserved;
 } smg_mesh_params;
 
@@ -169,6 +170,9 @@ typedef struct {
     uint64_t evaluated_samples; /* samples the SES lane kernel evaluated */
     int64_t frame[3];
     double band, edge_clamp;
+    /* Of the whole run (every part holds the same): sub-blocks a sampling or
+       refinement pass took after an earlier pass's lists could not hold them. */
+    uint64_t sample_retries, refine_retries;
 } smg_mesh_info;
 int smg_mesh_info_get(const smg_mesh* m, smg_mesh_info* info);
 
@@ -216,7 +220,7 @@ typedef struct {
     int32_t refine_first_tier; /* 0..2: the same for refinement */
     int32_t refine_group_max;  /* power of two <= 64: most lanes refining one edge */
     int32_t uniform_shared;    /* 1: sub-block states from one shared gather per brick; 0: a wavefront per brick */
-    int32_t reserved;
+    int32_t scratch_workgroups; /* workgroups of the global-scratch passes, 16..8192 (0: 1024); ~0.9 MB each */
 } smg_tuning;
 /* Sets the tuning of the device's next runs (NULL: the defaults). */
 int smg_device_set_tuning(smg_device* dev, const smg_tuning* tuning);
@@ -227,6 +231,65 @@ int smg_device_get_tuning(const smg_device* dev, smg_tuning* tuning);
  * with its time in ms. Plans capture the tuning current when they record. */
 int smg_tune(smg_resident* const* residents, const smg_mesh_params* params, size_t count, int repeats,
              smg_tuning* best, double* best_ms);
+
+/* ---------------------------------------------------------- sizing model
+ * Learned per-brick sizing. No counterpart in the reference, which sizes
+ * its lists by counting.
+ *
+ * The model is an MLP with layers 64 -> 128 -> 128 -> 2 and ReLU between
+ * them: 25 090 parameters. Layers 1 and 2 run on the BF16 matrix cores with
+ * FP32 accumulation; the rest is FP32. It reads 64 features of a brick: the
+ * square roots of the sphere counts in 4 radial shells x 16 directions
+ * around the brick centre, within reach (brick half-diagonal + 2 probe +
+ * largest expanded radius). It predicts two things: an occupancy logit (is
+ * the brick a surface brick) and log(1 + candidate load). The candidate load
+ * is max(S + C, 2 near, 4 SAS vertices, 4 inside) of the brick's largest
+ * sampling lists.
+ *
+ * Runs with params.sizing = 1 send each sub-block first to the LDS tier
+ * its predicted load calls for (smg_model_set_policy). The exact gathers check every list,
+ * so a wrong prediction costs one retry at the next tier and the mesh is
+ * unchanged. smg_mesh_info counts the retries.
+ *
+ * Training is on the device, with this library's own forward and backward
+ * kernels and Adam. The samples are touched bricks, labelled by the exact
+ * classifier (occupancy) and the exact gathers (load). */
+typedef struct {
+    uint64_t samples;
+    uint32_t epochs, batch;
+    double occupancy_loss;     /* mean binary cross-entropy */
+    double occupancy_accuracy; /* fraction of bricks whose logit sign matches */
+    double load_rmse;          /* RMS error of log(1 + load) over surface bricks */
+    double train_ms;
+} smg_train_stats;
+
+int smg_model_new(smg_device* dev, uint64_t seed, smg_model** out); /* He-uniform weights from `seed` */
+void smg_model_free(smg_model* m);
+size_t smg_model_param_count(void); /* 25090: W1 [64][128], b1, W2 [128][128], b2, W3 [128][2], b3 */
+int smg_model_get_params(const smg_model* m, float* params);
+int smg_model_set_params(smg_model* m, const float* params); /* resets the optimiser state */
+/* Runs the exact mesh stage of these molecules and appends their touched
+ * bricks (features, occupancy, log(1 + load)) to the training set. */
+int smg_model_collect(smg_model* m, smg_resident* const* residents, const smg_mesh_params* params, size_t count);
+size_t smg_model_samples(const smg_model* m);
+/* Copies up to `cap` samples of the training set (any array may be NULL). */
+int smg_model_dataset(const smg_model* m, float* features, uint8_t* occupied, float* log_load, size_t cap, size_t* n);
+/* Outputs (occupancy logit, log(1 + load)) of n feature rows. */
+int smg_model_forward(smg_model* m, const float* features, size_t n, float* out);
+/* Parameter gradients of the mean loss over n rows: per row, BCE(occupancy)
+ * + 0.5 occupied (predicted - log_load)^2. loss[0], loss[1] receive the two
+ * mean terms. No update. */
+int smg_model_gradients(smg_model* m, const float* features, const uint8_t* occupied, const float* log_load,
+                        size_t n, float* grads, double* loss);
+/* Adam (beta 0.9, 0.999) over shuffled batches of the training set. */
+int smg_model_train(smg_model* m, uint32_t epochs, uint32_t batch, float lr, uint64_t seed, smg_train_stats* stats);
+/* Placement: a sub-block goes first to the smallest tier holding scale x
+ * its predicted load, up to tier max_tier (0..3). Default scale 0.3, tier 1:
+ * larger tiers run at lower occupancy, so only sub-blocks predicted well
+ * past the first tier skip it. */
+int smg_model_set_policy(smg_model* m, float scale, int32_t max_tier);
+/* The model runs with sizing 1 on this device use (NULL: none). */
+int smg_device_set_model(smg_device* dev, smg_model* m);
 
 /* ------------------------------------------------------------------ plans
  * A capture-ready mesh stage for serving loops. No counterpart in the

@@ -82,7 +82,8 @@ class MeshInfo(ctypes.Structure):
                 ("refine_evaluations", ctypes.c_uint64), ("sampled_subblocks", ctypes.c_size_t),
                 ("overflow_subblocks", ctypes.c_size_t), ("evaluated_samples", ctypes.c_uint64),
                 ("frame", ctypes.c_int64 * 3), ("band", ctypes.c_double),
-                ("edge_clamp", ctypes.c_double)]
+                ("edge_clamp", ctypes.c_double), ("sample_retries", ctypes.c_uint64),
+                ("refine_retries", ctypes.c_uint64)]
 
 
 class PlanCaps(ctypes.Structure):
@@ -101,13 +102,23 @@ class Tuning(ctypes.Structure):
     meshes unchanged."""
     _fields_ = [("sample_first_tier", ctypes.c_int32), ("refine_first_tier", ctypes.c_int32),
                 ("refine_group_max", ctypes.c_int32), ("uniform_shared", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("scratch_workgroups", ctypes.c_int32)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
-DEFAULT_TUNING = dict(sample_first_tier=0, refine_first_tier=0, refine_group_max=64, uniform_shared=1)
+DEFAULT_TUNING = dict(sample_first_tier=0, refine_first_tier=0, refine_group_max=64, uniform_shared=1,
+                      scratch_workgroups=0)
+
+
+class TrainStats(ctypes.Structure):
+    _fields_ = [("samples", ctypes.c_uint64), ("epochs", ctypes.c_uint32), ("batch", ctypes.c_uint32),
+                ("occupancy_loss", ctypes.c_double), ("occupancy_accuracy", ctypes.c_double),
+                ("load_rmse", ctypes.c_double), ("train_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class PlanStats(ctypes.Structure):
@@ -126,7 +137,9 @@ ABI_FUNCTIONS = [
     "smg_mesh_new", "smg_mesh_free", "smg_mesh_info_get", "smg_mesh_copy", "smg_mesh_many", "smg_mesh_count",
     "smg_mesh_part_info", "smg_mesh_part_copy", "smg_field_clamped", "smg_plan_new", "smg_plan_caps_for",
     "smg_plan_run", "smg_plan_stats_get", "smg_plan_free", "smg_device_set_tuning", "smg_device_get_tuning",
-    "smg_tune",
+    "smg_tune", "smg_model_new", "smg_model_free", "smg_model_param_count", "smg_model_get_params",
+    "smg_model_set_params", "smg_model_collect", "smg_model_samples", "smg_model_dataset", "smg_model_forward",
+    "smg_model_gradients", "smg_model_train", "smg_device_set_model", "smg_model_set_policy",
 ]
 
 _lib = None
@@ -163,8 +176,19 @@ def lib():
                                    ctypes.c_void_p, ctypes.c_void_p]
         L.smg_tune.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p,
                                ctypes.c_void_p]
+        L.smg_model_new.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
+        L.smg_model_param_count.restype = ctypes.c_size_t
+        L.smg_model_samples.restype = ctypes.c_size_t
+        L.smg_model_dataset.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 3 + [ctypes.c_size_t, ctypes.c_void_p]
+        L.smg_model_forward.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+        L.smg_model_gradients.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 3 + [ctypes.c_size_t,
+                                                                                      ctypes.c_void_p, ctypes.c_void_p]
+        L.smg_model_train.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
+                                      ctypes.c_uint64, ctypes.c_void_p]
+        L.smg_model_collect.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        L.smg_model_set_policy.argtypes = [ctypes.c_void_p, ctypes.c_float, ctypes.c_int32]
         for name in ("smg_device_close", "smg_arrangement_free", "smg_resident_free", "smg_mesh_free",
-                     "smg_plan_free"):
+                     "smg_plan_free", "smg_model_free"):
             getattr(L, name).restype = None
         _lib = L
     return _lib
@@ -422,6 +446,86 @@ class Plan:
         h = getattr(self, "_h", None)
         if h:
             lib().smg_plan_free(h)
+            self._h = _vp()
+
+
+class Model:
+    """The learned per-brick sizing model (smg_model_*): a 64-128-128-2 MLP on
+    the BF16 matrix cores, trained on the device on bricks labelled by the
+    exact classifier and gathers. It only chooses which LDS capacity a
+    sub-block is tried at first; meshes do not depend on it."""
+
+    N_PARAMS = 25090
+
+    def __init__(self, device, seed=1):
+        self._h = _vp()
+        self.device = device
+        _check(lib().smg_model_new(device._h, ctypes.c_uint64(seed), ctypes.byref(self._h)))
+
+    @property
+    def params(self):
+        out = np.zeros(self.N_PARAMS, np.float32)
+        _check(lib().smg_model_get_params(self._h, _ptr(out)))
+        return out
+
+    @params.setter
+    def params(self, values):
+        v = np.ascontiguousarray(values, np.float32)
+        if v.shape != (self.N_PARAMS,):
+            raise ValueError(f"{self.N_PARAMS} parameters")
+        _check(lib().smg_model_set_params(self._h, _ptr(v)))
+
+    def collect(self, residents, spacings, brick=10):
+        """Appends the touched bricks of these molecules' exact mesh stage to
+        the training set."""
+        params = (MeshParams * len(residents))(*[_mesh_params(float(h), brick, "exact", 0) for h in spacings])
+        handles = (ctypes.c_void_p * len(residents))(*[r._h.value for r in residents])
+        _check(lib().smg_model_collect(self._h, handles, params, len(residents)))
+
+    @property
+    def samples(self):
+        return lib().smg_model_samples(self._h)
+
+    def dataset(self):
+        n = self.samples
+        x, occ, t = np.zeros((n, 64), np.float32), np.zeros(n, np.uint8), np.zeros(n, np.float32)
+        got = ctypes.c_size_t()
+        _check(lib().smg_model_dataset(self._h, _ptr(x), _ptr(occ), _ptr(t), n, ctypes.byref(got)))
+        return x, occ, t
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 64)
+        y = np.zeros((len(x), 2), np.float32)
+        _check(lib().smg_model_forward(self._h, _ptr(x), len(x), _ptr(y)))
+        return y
+
+    def gradients(self, x, occ, target):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 64)
+        occ = np.ascontiguousarray(occ, np.uint8)
+        target = np.ascontiguousarray(target, np.float32)
+        g = np.zeros(self.N_PARAMS, np.float32)
+        loss = np.zeros(2, np.float64)
+        _check(lib().smg_model_gradients(self._h, _ptr(x), _ptr(occ), _ptr(target), len(x), _ptr(g), _ptr(loss)))
+        return g, loss
+
+    def train(self, epochs=10, batch=4096, lr=1e-3, seed=1):
+        st = TrainStats()
+        _check(lib().smg_model_train(self._h, epochs, batch, ctypes.c_float(lr), ctypes.c_uint64(seed),
+                                     ctypes.byref(st)))
+        return st.as_dict()
+
+    def policy(self, scale=0.3, max_tier=1):
+        """Placement: first tier holding scale x the predicted load, at most max_tier."""
+        _check(lib().smg_model_set_policy(self._h, ctypes.c_float(scale), int(max_tier)))
+
+    def use(self):
+        """Runs with sizing 1 on the model's device use this model."""
+        _check(lib().smg_device_set_model(self.device._h, self._h))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().smg_model_free(h)
             self._h = _vp()
 
 

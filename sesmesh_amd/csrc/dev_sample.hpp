@@ -343,6 +343,12 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
     return true;
 }
 
+// Candidate load of a sub-block's sampling lists in units of the first
+// tier's candidate capacity: max(S + C, 2 near, 4 SAS vertices, 4 inside).
+__device__ inline uint32_t candidate_load(uint32_t cand, uint32_t near, uint32_t verts, uint32_t inside) {
+    return max(max(cand, 2u * near), max(4u * verts, 4u * inside));
+}
+
 // Samples one sub-block per workgroup: j = 8 * brick + sb of a non-uniform
 // sub-block. The lists live in dynamic LDS (GLOBAL false) or in scratch
 // slice blockIdx.x of `slice` bytes (GLOBAL true). Sub-blocks whose lists do
@@ -351,7 +357,7 @@ template <bool GLOBAL>
 __device__ __forceinline__ void sample_sub(const Mol* __restrict__ mols, const uint32_t* bmol, const Caps& cap,
                                            const Layout& lay, const int32_t* keys, const uint8_t* st, uint32_t j,
                                            float* values, char* scratch, size_t slice, uint32_t* overflow,
-                                           uint32_t* n_overflow, unsigned long long* n_samples) {
+                                           uint32_t* n_overflow, unsigned long long* n_samples, uint32_t* loads) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     __shared__ uint32_t n[C_N];
     __shared__ uint32_t tmp4[4];
@@ -371,6 +377,7 @@ __device__ __forceinline__ void sample_sub(const Mol* __restrict__ mols, const u
         if (threadIdx.x == 0) overflow[atomicAdd(n_overflow, 1u)] = j;
         return;
     }
+    if (loads && threadIdx.x == 0) atomicMax(&loads[brick], candidate_load(n[C_CAND], n[C_NEAR], n[C_V], n[C_IN]));
     const uint32_t cc = cap.cand, nw = cap.near / 64;
     uint32_t stb = pack_states(st + 8 * brick);
     uint32_t sub = P.sub, box = sub + 1, side = P.side;
@@ -411,11 +418,11 @@ template <bool GLOBAL, bool PERSIST>
 __global__ void __launch_bounds__(256) k_sample(const Mol* __restrict__ mols, const uint32_t* bmol, Caps cap,
                                                 Layout lay, const int32_t* keys, const uint8_t* st, Queue q,
                                                 float* values, char* scratch, size_t slice, uint32_t* overflow,
-                                                uint32_t* n_overflow, unsigned long long* n_samples) {
+                                                uint32_t* n_overflow, unsigned long long* n_samples, uint32_t* loads) {
     __shared__ uint32_t ticket;
     for_each_item<PERSIST>(q, &ticket, [&](uint32_t j) {
         sample_sub<GLOBAL>(mols, bmol, cap, lay, keys, st, j, values, scratch, slice, overflow, n_overflow,
-                           n_samples);
+                           n_samples, loads);
     });
 }
 

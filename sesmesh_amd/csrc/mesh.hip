@@ -21,6 +21,7 @@
 #include "dev_cells.hpp"
 #include "dev_uniform.hpp"
 #include "dev_fast.hpp"
+#include "dev_mlp.hpp"
 #include "host.hpp"
 
 namespace smg {
@@ -280,6 +281,31 @@ struct smg_device {
     std::map<std::pair<const void*, size_t>, uint32_t> resident;
     std::mutex mu;
     smg_tuning tune{0, 0, 64, 1, 0};  // launch shapes of the next runs (smg_tune)
+    smg_model* model = nullptr;       // sizing model of runs with sizing 1
+};
+
+// The sizing model: FP32 master parameters, Adam moments, BF16 copies of
+// W1 and W2 for the matrix cores, and its labelled training set (features
+// of touched bricks, occupancy, log candidate load).
+struct smg_model {
+    smg_device* dev = nullptr;
+    float *P = nullptr, *Mo = nullptr, *Ve = nullptr;
+    uint16_t *W1b = nullptr, *W2b = nullptr;
+    uint64_t step = 0;
+    float* X = nullptr;
+    uint8_t* occ = nullptr;
+    float* tgt = nullptr;
+    size_t n = 0, cap = 0;
+    // Placement: a larger tier runs a sub-block at lower occupancy, so a
+    // prediction may only skip the first tier, and only for loads well
+    // past it (measured: never slower than no sizing on 0.5-1.0 A).
+    float scale = 0.3f;
+    uint32_t max_tier = 1;
+    ~smg_model() {
+        void* ps[] = {P, Mo, Ve, W1b, W2b, X, occ, tgt};
+        for (void* p : ps)
+            if (p) (void)hipFree(p);
+    }
 };
 
 struct smg_arrangement {
@@ -760,17 +786,24 @@ constexpr RCaps REFINE_CAPS[TIERS] = {{128, 64, 256, 256, 0, 0, 0},
 // wavefront per brick, which beats larger tiers at one or two workgroups per CU.
 constexpr RCaps UNIFORM_CAPS = {192, 128, 256, 512, 0, 1, 0};
 constexpr uint32_t LDS_EDGES = 512;      // crossing-edge buffer of the LDS passes
-constexpr uint32_t SCRATCH_CHUNK = 256;   // workgroups of a scratch pass (one slice each)
+constexpr uint32_t SCRATCH_CHUNK = 1024;  // default workgroups of a scratch pass (one list slice each)
 constexpr uint32_t STATIC_LDS = 6144;     // static LDS of k_sample / k_refine, bound
 constexpr uint32_t NO_CAP = 0xffffffffu;
 
 // Device counters of a run (u32 slots, zero at the start of the run).
 enum : uint32_t {
     K_TOUCHED, K_NB, K_NSUB, K_NSW, K_NV, K_NRW, K_NT, K_FLAGS, K_ERR, K_UOVER,
-    K_SOVER0, K_ROVER0 = K_SOVER0 + TIERS,
-    K_Q_SUBST = K_ROVER0 + TIERS, K_Q_SAMPLE0, K_Q_REFINE0 = K_Q_SAMPLE0 + TIERS,
+    // Per pass: items in its list (predicted there, then appended by the
+    // passes before it), items predicted there, and the last pass's overflow.
+    K_SLIST0, K_SPRED0 = K_SLIST0 + TIERS, K_SLAST = K_SPRED0 + TIERS,
+    K_RLIST0, K_RPRED0 = K_RLIST0 + TIERS, K_RLAST = K_RPRED0 + TIERS,
+    K_Q_SUBST, K_Q_SAMPLE0, K_Q_REFINE0 = K_Q_SAMPLE0 + TIERS,
     K_N = K_Q_REFINE0 + TIERS
 };
+// Capacity of each tier in candidate-load units (max(S + C, 2 near, 4 SAS
+// vertices, 4 inside) of the sampling lists; the refinement lists alike).
+constexpr float SAMPLE_LOAD_CAP[TIERS] = {256.0f, 384.0f, 768.0f, 2048.0f};
+constexpr float REFINE_LOAD_CAP[TIERS] = {256.0f, 384.0f, 704.0f, 2048.0f};
 // K_FLAGS bits: a device count exceeded its capacity.
 enum : uint32_t { OVER_BRICKS = 1, OVER_VERTICES = 2, OVER_TRIANGLES = 4, OVER_TABLE = 8 };
 
@@ -1052,17 +1085,30 @@ SumLayout sum_layout(size_t rows) {
     return L;
 }
 
+// Learned sizing of a run: `predict` sends each sub-block first to the LDS
+// tier its brick's predicted load fits; `collect` appends the run's touched
+// bricks, labelled by the exact classifier and the exact gathers, to the
+// model's training set.
+struct Sizing {
+    smg_model* predict = nullptr;
+    smg_model* collect = nullptr;
+};
+
+int grow_dataset(smg_model* m, size_t need);
+
 // The mesh stage of a table's molecules in one pass: every kernel runs once
 // over the concatenated work of all of them, each work item finding its
 // molecule's arrangement, f32 copies and lattice in the table. Sets M's
 // device arrays and copies the summary into `sum_host` (asynchronously).
-int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_host, smg_mesh* M, bool timed) {
+int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_host, smg_mesh* M, bool timed,
+             const Sizing& sz) {
     smg_device* d = X.d;
     hipStream_t s = X.s;
     const uint32_t b = t.b, spb = (b + 1) * (b + 1) * (b + 1);
     const uint32_t rows = (uint32_t)E.rows;
     const bool fast = t.fast;
     const smg_tuning tune = d->tune;
+    const uint32_t scratch_wgs = tune.scratch_workgroups > 0 ? (uint32_t)tune.scratch_workgroups : SCRATCH_CHUNK;
     M->spb = spb;
     auto rec = [&](int i) -> int {
         if (timed) HIP_TRY(hipEventRecord(d->ev[i], s));
@@ -1134,6 +1180,35 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     k_mol_ranges<<<stride_grid(rows + 1), 256, 0, s>>>(wofs_d, rows, woff, &hdr->nwords, &cnt[K_TOUCHED], koff,
                                                          &cnt[K_NB], tfirst, fbrick);
     HIP_TRY(hipGetLastError());
+    // Predicted candidate loads of the surface bricks (learned sizing).
+    float* pred = nullptr;
+    if (sz.predict) {
+        float* feat;
+        TRY(X.temp(&feat, (size_t)nb * MLP_IN));
+        TRY(X.temp(&pred, (size_t)nb * MLP_OUT));
+        k_features<<<g1(nb), WAVE, 0, s>>>(mols, wofs_d, rows, nullptr, M->keys, bmol, Queue{nullptr, &cnt[K_NB], nullptr},
+                                           feat);
+        k_mlp_forward<<<g1(std::min<uint64_t>((nb + MLP_ROWS - 1) / MLP_ROWS, 4096)), 256, 0, s>>>(
+            feat, &cnt[K_NB], sz.predict->P, sz.predict->W1b, sz.predict->W2b, pred);
+        HIP_TRY(hipGetLastError());
+    }
+    // Training labels: features of every touched brick, and per surface
+    // brick the largest candidate load of its sampled sub-blocks.
+    float *cfeat = nullptr, *ctgt = nullptr;
+    uint8_t* cocc = nullptr;
+    uint32_t* cload = nullptr;
+    uint32_t ntouched = 0;
+    if (sz.collect) {
+        TRY(X.size(&cnt[K_TOUCHED], (uint32_t)E.dense, ntouched));
+        TRY(X.temp(&cfeat, (size_t)ntouched * MLP_IN));
+        TRY(X.temp(&cocc, ntouched));
+        TRY(X.temp(&ctgt, ntouched));
+        TRY(X.temp(&cload, nb));
+        k_features<<<g1(ntouched), WAVE, 0, s>>>(mols, wofs_d, rows, touched, nullptr, nullptr,
+                                                 Queue{nullptr, &cnt[K_TOUCHED], nullptr}, cfeat);
+        k_fill_u32<<<stride_grid(nb), 256, 0, s>>>(cload, 0u, 0, &cnt[K_NB]);
+        HIP_TRY(hipGetLastError());
+    }
     // Sub-block states (kernel.rs:467-482): one shared gather per brick, one
     // wavefront per brick for bricks whose lists overflow it.
     TRY(rec(8));
@@ -1169,46 +1244,87 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     k_compact_index<<<stride_grid(nsub), 256, 0, s>>>(sflag, soff, &cnt[K_NSUB], swork);
     HIP_TRY(hipGetLastError());
     const size_t lds_max = d->prop.sharedMemPerBlock;
-    // The first pass takes one sub-block per workgroup (a grid of the count,
-    // or of its capacity when the run is planned); the overflow passes are
-    // persistent over their queues.
+    // Passes over the LDS tiers from the tuning's first one that fit, then
+    // global scratch; each takes a list: sub-blocks predicted for it (all
+    // of them in the first pass without learned sizing) followed by those
+    // the passes before it could not hold. The first pass takes one
+    // sub-block per workgroup (a grid of the count, or of its capacity when
+    // the run is planned); later passes are persistent over their queues.
     uint32_t nsw;
     TRY(X.size(&cnt[K_NSW], (uint32_t)nsub, nsw));
     {
-        const uint32_t* list = swork;
-        const uint32_t* count = &cnt[K_NSW];
-        bool first = true;
+        bool runs[TIERS];
+        for (int p = 0; p < TIERS; p++) {
+            Layout lay = make_layout(SAMPLE_CAPS[p]);
+            runs[p] = p == TIERS - 1 || (p >= tune.sample_first_tier && lay.bytes + STATIC_LDS <= lds_max);
+        }
+        int first = 0;
+        while (!runs[first]) first++;
+        uint32_t* lists[TIERS + 1] = {};
+        for (int p = 0; p < TIERS; p++)
+            if (runs[p] && (p != first || pred)) TRY(X.temp(&lists[p], nsub));
+        TRY(X.temp(&lists[TIERS], nsub));  // overflow of the last pass (an error)
+        const uint32_t* count0 = &cnt[K_NSW];
+        if (pred) {
+            TierSplit T{};
+            for (int t = 0; t < TIERS; t++) {
+                int p = std::max(t, first);
+                while (!runs[p]) p++;
+                T.list[t] = lists[t];
+                T.pass[t] = (uint32_t)p;
+                T.cap[t] = SAMPLE_LOAD_CAP[t];
+            }
+            T.scale = sz.predict->scale;
+            T.max_tier = sz.predict->max_tier;
+            k_split_tiers<<<stride_grid(nsub), 256, 0, s>>>(swork, &cnt[K_NSW], pred, T, &cnt[K_SLIST0], &cnt[K_SPRED0]);
+            HIP_TRY(hipGetLastError());
+            count0 = &cnt[K_SLIST0 + first];
+        } else {
+            lists[first] = swork;
+        }
         TRY(rec(10));
-        for (int pass = 0; pass < TIERS; pass++) {
+        for (int pass = first; pass < TIERS; pass++) {
+            if (!runs[pass]) continue;
             const Caps& cap = SAMPLE_CAPS[pass];
             Layout lay = make_layout(cap);
-            bool lds = pass < TIERS - 1;
-            if (lds && (pass < tune.sample_first_tier || lay.bytes + STATIC_LDS > lds_max)) continue;
-            uint32_t* next;
-            TRY(X.temp(&next, nsub));
-            Queue q{list, count, &cnt[K_Q_SAMPLE0 + pass]};
-            uint32_t* nnext = &cnt[K_SOVER0 + pass];
-            if (lds && first) {
-                k_sample<false, false><<<g1(nsw), 256, lay.bytes, s>>>(mols, bmol, cap, lay, M->keys, st, q,
-                                                                       M->values, nullptr, 0, next, nnext, nsamp);
-            } else if (lds) {
+            int nx = pass + 1;
+            while (nx < TIERS && !runs[nx]) nx++;
+            uint32_t* next = lists[nx];
+            uint32_t* nnext = nx < TIERS ? &cnt[K_SLIST0 + nx] : &cnt[K_SLAST];
+            Queue q{lists[pass], pass == first ? count0 : &cnt[K_SLIST0 + pass], &cnt[K_Q_SAMPLE0 + pass]};
+            // One sub-block per workgroup where the grid can be the count:
+            // every LDS pass of an exact run (its count read back), the
+            // first pass of a planned one (its capacity).
+            uint32_t items = (uint32_t)nsub;
+            if (!X.planned()) {
+                TRY(X.size(q.count, 0, items));
+                if (items == 0 && pass != first) continue;
+            } else if (pass == first) {
+                items = nsw;
+            }
+            if (pass < TIERS - 1 && (!X.planned() || pass == first)) {
+                k_sample<false, false><<<g1(items), 256, lay.bytes, s>>>(mols, bmol, cap, lay, M->keys, st, q,
+                                                                         M->values, nullptr, 0, next, nnext, nsamp,
+                                                                         cload);
+            } else if (pass < TIERS - 1) {
                 uint32_t grid =
                     g1(std::min<uint64_t>(nsub, resident(d, (const void*)k_sample<false, true>, 256, lay.bytes)));
                 k_sample<false, true><<<grid, 256, lay.bytes, s>>>(mols, bmol, cap, lay, M->keys, st, q, M->values,
-                                                                   nullptr, 0, next, nnext, nsamp);
+                                                                   nullptr, 0, next, nnext, nsamp, cload);
             } else {
-                uint32_t grid = g1(std::min<uint64_t>(nsub, SCRATCH_CHUNK));
+                uint32_t grid = g1(std::min<uint64_t>(items, scratch_wgs));
                 char* scratch;
                 TRY(X.temp(&scratch, (size_t)grid * lay.bytes));
                 k_sample<true, true><<<grid, 256, 0, s>>>(mols, bmol, cap, lay, M->keys, st, q, M->values, scratch,
-                                                          lay.bytes, next, nnext, nsamp);
+                                                          lay.bytes, next, nnext, nsamp, cload);
             }
             HIP_TRY(hipGetLastError());
-            if (first) TRY(rec(11));
-            first = false;
-            list = next;
-            count = nnext;
+            if (pass == first) TRY(rec(11));
         }
+    }
+    if (sz.collect) {
+        k_labels<<<stride_grid(ntouched), 256, 0, s>>>(keep, koff, &cnt[K_TOUCHED], cload, &cnt[K_NB], cocc, ctgt);
+        HIP_TRY(hipGetLastError());
     }
     TRY(rec(2));
 
@@ -1246,34 +1362,76 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     TRY(X.output(&M->vevals, nverts));
     VOut vo{M->pos, M->nrm, M->atom, M->patch, M->t, M->vevals, nverts};
     {
-        const uint32_t* list = rwork;
-        const uint32_t* count = &cnt[K_NRW];
         const uint32_t sub_edges = 3 * (b / 2) * (b / 2) * (b / 2);
-        bool first = true;
-        TRY(rec(12));
-        for (int pass = 0; pass < TIERS; pass++) {
-            RCaps cap = REFINE_CAPS[pass];
-            bool lds = pass < TIERS - 1;
+        bool runs[TIERS];
+        uint32_t bytes_of[TIERS];
+        RCaps caps[TIERS];
+        RLayout lays[TIERS];
+        FLayout flays[TIERS];
+        for (int p = 0; p < TIERS; p++) {
+            RCaps cap = REFINE_CAPS[p];
+            bool lds = p < TIERS - 1;
             cap.edges = lds ? std::min(sub_edges, LDS_EDGES) : sub_edges;
             cap.gmax = (uint32_t)tune.refine_group_max;
-            RLayout lay = make_rlayout(cap);
-            FLayout flay = make_flayout(cap, lay.bytes);
-            const uint32_t bytes = fast ? flay.bytes : lay.bytes;
-            if (lds && (pass < tune.refine_first_tier || bytes + STATIC_LDS > lds_max)) continue;
-            uint32_t* next;
-            TRY(X.temp(&next, nsub));
-            Queue q{list, count, &cnt[K_Q_REFINE0 + pass]};
-            uint32_t* nnext = &cnt[K_ROVER0 + pass];
-            if (lds && first) {
+            caps[p] = cap;
+            lays[p] = make_rlayout(cap);
+            flays[p] = make_flayout(cap, lays[p].bytes);
+            bytes_of[p] = fast ? flays[p].bytes : lays[p].bytes;
+            runs[p] = !lds || (p >= tune.refine_first_tier && bytes_of[p] + STATIC_LDS <= lds_max);
+        }
+        int first = 0;
+        while (!runs[first]) first++;
+        uint32_t* lists[TIERS + 1] = {};
+        for (int p = 0; p < TIERS; p++)
+            if (runs[p] && (p != first || pred)) TRY(X.temp(&lists[p], nsub));
+        TRY(X.temp(&lists[TIERS], nsub));
+        const uint32_t* count0 = &cnt[K_NRW];
+        if (pred) {
+            TierSplit T{};
+            for (int t = 0; t < TIERS; t++) {
+                int p = std::max(t, first);
+                while (!runs[p]) p++;
+                T.list[t] = lists[t];
+                T.pass[t] = (uint32_t)p;
+                T.cap[t] = REFINE_LOAD_CAP[t];
+            }
+            T.scale = sz.predict->scale;
+            T.max_tier = sz.predict->max_tier;
+            k_split_tiers<<<stride_grid(nsub), 256, 0, s>>>(rwork, &cnt[K_NRW], pred, T, &cnt[K_RLIST0], &cnt[K_RPRED0]);
+            HIP_TRY(hipGetLastError());
+            count0 = &cnt[K_RLIST0 + first];
+        } else {
+            lists[first] = rwork;
+        }
+        TRY(rec(12));
+        for (int pass = first; pass < TIERS; pass++) {
+            if (!runs[pass]) continue;
+            const RCaps& cap = caps[pass];
+            const RLayout& lay = lays[pass];
+            const FLayout& flay = flays[pass];
+            const uint32_t bytes = bytes_of[pass];
+            int nx = pass + 1;
+            while (nx < TIERS && !runs[nx]) nx++;
+            uint32_t* next = lists[nx];
+            uint32_t* nnext = nx < TIERS ? &cnt[K_RLIST0 + nx] : &cnt[K_RLAST];
+            Queue q{lists[pass], pass == first ? count0 : &cnt[K_RLIST0 + pass], &cnt[K_Q_REFINE0 + pass]};
+            uint32_t items = (uint32_t)nsub;
+            if (!X.planned()) {
+                TRY(X.size(q.count, 0, items));
+                if (items == 0 && pass != first) continue;
+            } else if (pass == first) {
+                items = nrw;
+            }
+            if (pass < TIERS - 1 && (!X.planned() || pass == first)) {
                 if (fast)
-                    k_refine_fast<false, false><<<g1(nrw), 256, bytes, s>>>(mols, bmol, cap, lay, flay, M->keys,
-                                                                            M->values, q, mask, wpre, words, vbase,
-                                                                            vo, nullptr, 0, next, nnext, evd);
+                    k_refine_fast<false, false><<<g1(items), 256, bytes, s>>>(mols, bmol, cap, lay, flay, M->keys,
+                                                                              M->values, q, mask, wpre, words, vbase,
+                                                                              vo, nullptr, 0, next, nnext, evd);
                 else
-                    k_refine<false, false><<<g1(nrw), 256, bytes, s>>>(mols, bmol, cap, lay, M->keys, M->values, q,
-                                                                       mask, wpre, words, vbase, vo, nullptr, 0, next,
-                                                                       nnext, evd);
-            } else if (lds) {
+                    k_refine<false, false><<<g1(items), 256, bytes, s>>>(mols, bmol, cap, lay, M->keys, M->values, q,
+                                                                         mask, wpre, words, vbase, vo, nullptr, 0,
+                                                                         next, nnext, evd);
+            } else if (pass < TIERS - 1) {
                 const void* kern =
                     fast ? (const void*)k_refine_fast<false, true> : (const void*)k_refine<false, true>;
                 uint32_t grid = g1(std::min<uint64_t>(nsub, resident(d, kern, 256, bytes)));
@@ -1286,7 +1444,7 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
                                                                    wpre, words, vbase, vo, nullptr, 0, next, nnext,
                                                                    evd);
             } else {
-                uint32_t grid = g1(std::min<uint64_t>(nsub, SCRATCH_CHUNK));
+                uint32_t grid = g1(std::min<uint64_t>(items, scratch_wgs));
                 char* scratch;
                 TRY(X.temp(&scratch, (size_t)grid * bytes));
                 if (fast)
@@ -1299,10 +1457,7 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
                                                               evd);
             }
             HIP_TRY(hipGetLastError());
-            if (first) TRY(rec(13));
-            first = false;
-            list = next;
-            count = nnext;
+            if (pass == first) TRY(rec(13));
         }
     }
     TRY(rec(4));
@@ -1326,6 +1481,14 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
                                                           &cnt[K_NV], tbase, &cnt[K_NT], sfirst, vfirst, trfirst);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(sum_host, sum, UL.bytes, hipMemcpyDeviceToHost, s));
+    if (sz.collect && ntouched) {
+        smg_model* mo = sz.collect;
+        TRY(grow_dataset(mo, mo->n + ntouched));
+        HIP_TRY(hipMemcpyAsync(mo->X + mo->n * MLP_IN, cfeat, (size_t)ntouched * MLP_IN * 4, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(mo->occ + mo->n, cocc, ntouched, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(mo->tgt + mo->n, ctgt, (size_t)ntouched * 4, hipMemcpyDeviceToDevice, s));
+        mo->n += ntouched;
+    }
     return SMG_OK;
 }
 
@@ -1344,12 +1507,16 @@ int finish(smg_mesh* M, const Table& t, const char* sum_host, bool timed, smg_me
     const uint32_t* trf = (const uint32_t*)(sum_host + UL.trfirst);
     flags = cnt[K_FLAGS];
     if (flags) return SMG_OK;
-    if (cnt[K_SOVER0 + TIERS - 1])
-        return fail(SMG_E_INTERNAL, std::to_string(cnt[K_SOVER0 + TIERS - 1]) +
-                                        " sub-blocks exceed the largest candidate lists");
-    if (cnt[K_ROVER0 + TIERS - 1])
-        return fail(SMG_E_INTERNAL, std::to_string(cnt[K_ROVER0 + TIERS - 1]) +
-                                        " sub-blocks exceed the largest refinement lists");
+    if (cnt[K_SLAST])
+        return fail(SMG_E_INTERNAL, std::to_string(cnt[K_SLAST]) + " sub-blocks exceed the largest candidate lists");
+    if (cnt[K_RLAST])
+        return fail(SMG_E_INTERNAL, std::to_string(cnt[K_RLAST]) + " sub-blocks exceed the largest refinement lists");
+    // Retries: sub-blocks a pass took after an earlier pass could not hold them.
+    uint64_t sretry = 0, rretry = 0;
+    for (int p = 0; p < TIERS; p++) {
+        sretry += cnt[K_SLIST0 + p] - cnt[K_SPRED0 + p];
+        rretry += cnt[K_RLIST0 + p] - cnt[K_RPRED0 + p];
+    }
     if (cnt[K_ERR]) return fail(SMG_E_INTERNAL, "a triangle references an edge without a vertex");
     const size_t nm = t.nm;
     M->info.assign(nm, smg_mesh_info{});
@@ -1364,7 +1531,9 @@ int finish(smg_mesh* M, const Table& t, const char* sum_host, bool timed, smg_me
         I.triangles = M->tfirst[m + 1] - M->tfirst[m];
         I.refine_evaluations = evd[m];
         I.sampled_subblocks = sf[m + 1] - sf[m];
-        I.overflow_subblocks = nm == 1 ? cnt[K_SOVER0] : 0;
+        I.overflow_subblocks = nm == 1 ? sretry : 0;
+        I.sample_retries = sretry;
+        I.refine_retries = rretry;
         I.evaluated_samples = nsamp[m];
         for (int q = 0; q < 3; q++) I.frame[q] = t.H[m].P.frame[q];
         I.band = t.H[m].P.band;
@@ -1393,12 +1562,30 @@ int finish(smg_mesh* M, const Table& t, const char* sum_host, bool timed, smg_me
 
 }  // namespace
 
-// The exact path: sizes read back from the device as the run goes.
+// The learned sizing a run asks for (params[0].sizing: 0 none, 1 the
+// device's sizing model).
+static int run_sizing(smg_device* d, const smg_mesh_params* prm, size_t nm, Sizing& sz) {
+    for (size_t m = 0; m < nm; m++)
+        if (prm[m].sizing != prm[0].sizing) return fail(SMG_E_UNSUPPORTED, "one sizing mode per run");
+    if (prm[0].sizing == 1) {
+        if (!d->model) return fail(SMG_E_INVALID, "sizing 1 needs a sizing model on the device (smg_device_set_model)");
+        sz.predict = d->model;
+    } else if (prm[0].sizing != 0) {
+        return fail(SMG_E_INVALID, "unknown sizing mode " + std::to_string(prm[0].sizing));
+    }
+    return SMG_OK;
+}
+
+// The exact path: sizes read back from the device as the run goes. With
+// `collect`, the run's labelled bricks go to that model's training set.
 static int mesh_many(smg_resident* const* R, const smg_mesh_params* prm, size_t nm, smg_mesh* M,
-                     smg_mesh_times* times) {
+                     smg_mesh_times* times, smg_model* collect = nullptr) {
     Table t;
     TRY(build_table(R, prm, nm, nm, t));
     smg_device* d = R[0]->dev;
+    Sizing sz;
+    TRY(run_sizing(d, prm, nm, sz));
+    sz.collect = collect;
     HIP_TRY(hipSetDevice(d->ordinal));
     Temps T(d->stream);
     Run X;
@@ -1407,7 +1594,7 @@ static int mesh_many(smg_resident* const* R, const smg_mesh_params* prm, size_t 
     X.T = &T;
     Extents E{t.rows, t.locals(), t.spheres(), t.words(), t.dense, NO_CAP, NO_CAP, NO_CAP};
     std::vector<char> stage(stage_layout(t.rows).bytes), sum(sum_layout(t.rows).bytes);
-    TRY(pipeline(X, t, E, stage.data(), sum.data(), M, times != nullptr));
+    TRY(pipeline(X, t, E, stage.data(), sum.data(), M, times != nullptr, sz));
     HIP_TRY(hipStreamSynchronize(d->stream));
     uint32_t flags = 0;
     TRY(finish(M, t, sum.data(), times != nullptr, times, flags));
@@ -1454,6 +1641,8 @@ static int check_tuning(const smg_tuning& t) {
     int g = t.refine_group_max;
     if (g < 1 || g > WAVE || (g & (g - 1))) return fail(SMG_E_INVALID, "refine_group_max must be a power of two <= 64");
     if (t.uniform_shared != 0 && t.uniform_shared != 1) return fail(SMG_E_INVALID, "uniform_shared is 0 or 1");
+    if (t.scratch_workgroups != 0 && (t.scratch_workgroups < 16 || t.scratch_workgroups > 8192))
+        return fail(SMG_E_INVALID, "scratch_workgroups is 0 (default) or 16..8192");
     return SMG_OK;
 }
 
@@ -1499,13 +1688,17 @@ extern "C" int smg_tune(smg_resident* const* rs, const smg_mesh_params* ps, size
     const int tiers[] = {0, 1, 2};
     const int groups[] = {64, 32, 16, 8};
     const int shared[] = {1, 0};
-    for (int knob = 0; knob < 4 && rc == SMG_OK; knob++) {
-        const int* vals = knob < 2 ? tiers : (knob == 2 ? groups : shared);
-        int nv = knob < 2 ? 3 : (knob == 2 ? 4 : 2);
+    const int scratch[] = {0, 256, 2048};
+    for (int knob = 0; knob < 5 && rc == SMG_OK; knob++) {
+        const int* vals = knob < 2 ? tiers : (knob == 2 ? groups : (knob == 3 ? shared : scratch));
+        int nv = knob < 2 ? 3 : (knob == 2 ? 4 : (knob == 3 ? 2 : 3));
         for (int i = 0; i < nv && rc == SMG_OK; i++) {
             smg_tuning t = cur;
-            int* field = knob == 0 ? &t.sample_first_tier
-                                   : (knob == 1 ? &t.refine_first_tier : (knob == 2 ? &t.refine_group_max : &t.uniform_shared));
+            int* field = knob == 0   ? &t.sample_first_tier
+                         : knob == 1 ? &t.refine_first_tier
+                         : knob == 2 ? &t.refine_group_max
+                         : knob == 3 ? &t.uniform_shared
+                                     : &t.scratch_workgroups;
             if (*field == vals[i]) continue;
             *field = vals[i];
             double ms = 0.0;
@@ -1523,6 +1716,315 @@ extern "C" int smg_tune(smg_resident* const* rs, const smg_mesh_params* ps, size
     return SMG_OK;
 }
 
+// ------------------------------------------------------------ sizing model
+namespace {
+
+int grow_dataset(smg_model* m, size_t need) {
+    if (need <= m->cap) return SMG_OK;
+    size_t cap = std::max<size_t>(need, m->cap * 2 + 4096);
+    float *X = nullptr, *tgt = nullptr;
+    uint8_t* occ = nullptr;
+    HIP_TRY(hipMalloc(&X, cap * MLP_IN * sizeof(float)));
+    HIP_TRY(hipMalloc(&occ, cap));
+    HIP_TRY(hipMalloc(&tgt, cap * sizeof(float)));
+    hipStream_t s = m->dev->stream;
+    if (m->n) {
+        HIP_TRY(hipMemcpyAsync(X, m->X, m->n * MLP_IN * sizeof(float), hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(occ, m->occ, m->n, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(tgt, m->tgt, m->n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    if (m->X) (void)hipFree(m->X);
+    if (m->occ) (void)hipFree(m->occ);
+    if (m->tgt) (void)hipFree(m->tgt);
+    m->X = X;
+    m->occ = occ;
+    m->tgt = tgt;
+    m->cap = cap;
+    return SMG_OK;
+}
+
+// Scratch of the training kernels: index list, partial gradients of up to
+// `nw` workgroups, their loss terms and the summed gradient.
+struct TrainBufs {
+    uint32_t* idx = nullptr;
+    float *G = nullptr, *S = nullptr, *grad = nullptr;
+    ~TrainBufs() {
+        void* ps[] = {idx, G, S, grad};
+        for (void* p : ps)
+            if (p) (void)hipFree(p);
+    }
+    int alloc(size_t rows, uint32_t nw) {
+        HIP_TRY(hipMalloc(&idx, std::max<size_t>(rows, 1) * 4));
+        HIP_TRY(hipMalloc(&G, (size_t)nw * MLP_PARAMS * 4));
+        HIP_TRY(hipMalloc(&S, (size_t)nw * MLP_STATS * 4));
+        HIP_TRY(hipMalloc(&grad, MLP_PARAMS * 4));
+        return SMG_OK;
+    }
+};
+
+constexpr float LOAD_WEIGHT = 1.0f;  // weight of the load term in the loss
+
+// One fwd/bwd over rows idx[0..nb) of (X, occ, tgt): summed gradient into
+// B.grad (when wanted) and loss terms summed into S_host; with lr > 0, an
+// Adam step.
+int mlp_step(smg_model* m, const float* X, const uint8_t* occ, const float* tgt, const uint32_t* idx, uint32_t nb,
+             TrainBufs& B, float lr, bool want_grad, double S_host[MLP_STATS]) {
+    hipStream_t s = m->dev->stream;
+    uint32_t nw = (nb + MLP_ROWS - 1) / MLP_ROWS;
+    k_mlp_grad<<<nw, 256, 0, s>>>(X, occ, tgt, idx, nb, LOAD_WEIGHT, m->P, m->W1b, m->W2b, B.G, B.S);
+    HIP_TRY(hipGetLastError());
+    float bc1 = 1.0f, bc2 = 1.0f;
+    if (lr > 0.0f) {
+        m->step++;
+        bc1 = 1.0f - std::pow(0.9f, (float)m->step);
+        bc2 = 1.0f - std::pow(0.999f, (float)m->step);
+    }
+    k_mlp_adam<<<(MLP_PARAMS + 255) / 256, 256, 0, s>>>(B.G, nw, m->P, m->Mo, m->Ve, m->W1b, m->W2b,
+                                                        want_grad ? B.grad : nullptr, lr, 0.9f, 0.999f, 1e-8f, bc1,
+                                                        bc2);
+    HIP_TRY(hipGetLastError());
+    if (S_host) {
+        std::vector<float> st((size_t)nw * MLP_STATS);
+        HIP_TRY(hipMemcpyAsync(st.data(), B.S, st.size() * 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        for (uint32_t w = 0; w < nw; w++)
+            for (int k = 0; k < MLP_STATS; k++) S_host[k] += st[(size_t)w * MLP_STATS + k];
+    }
+    return SMG_OK;
+}
+
+// Deterministic initial weights: uniform in +-sqrt(6 / fan_in) (He), zero biases.
+void init_params(uint64_t seed, std::vector<float>& P) {
+    P.assign(MLP_PARAMS, 0.0f);
+    uint64_t x = seed * 0x9E3779B97F4A7C15ull + 0x2545F4914F6CDD1Dull;
+    auto next = [&]() {
+        x ^= x >> 12;
+        x ^= x << 25;
+        x ^= x >> 27;
+        return (double)((x * 0x2545F4914F6CDD1Dull) >> 11) / (double)(1ull << 53);
+    };
+    auto fill = [&](int off, int fan_in, int count) {
+        double a = std::sqrt(6.0 / fan_in);
+        for (int i = 0; i < count; i++) P[off + i] = (float)((2.0 * next() - 1.0) * a);
+    };
+    fill(P_W1, MLP_IN, MLP_IN * MLP_H);
+    fill(P_W2, MLP_H, MLP_H * MLP_H);
+    fill(P_W3, MLP_H, MLP_H * MLP_OUT);
+}
+
+int upload_params(smg_model* m, const float* host) {
+    hipStream_t s = m->dev->stream;
+    HIP_TRY(hipMemcpyAsync(m->P, host, MLP_PARAMS * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemsetAsync(m->Mo, 0, MLP_PARAMS * 4, s));
+    HIP_TRY(hipMemsetAsync(m->Ve, 0, MLP_PARAMS * 4, s));
+    k_mlp_cast<<<(MLP_H * MLP_H + 255) / 256, 256, 0, s>>>(m->P, m->W1b, m->W2b);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    m->step = 0;
+    return SMG_OK;
+}
+
+}  // namespace
+
+extern "C" int smg_model_new(smg_device* dev, uint64_t seed, smg_model** out) {
+    if (!out) return fail(SMG_E_INVALID, "null output");
+    *out = nullptr;
+    if (!dev) return fail(SMG_E_INVALID, "null device");
+    HIP_TRY(hipSetDevice(dev->ordinal));
+    std::unique_ptr<smg_model> m(new smg_model());
+    m->dev = dev;
+    HIP_TRY(hipMalloc(&m->P, MLP_PARAMS * 4));
+    HIP_TRY(hipMalloc(&m->Mo, MLP_PARAMS * 4));
+    HIP_TRY(hipMalloc(&m->Ve, MLP_PARAMS * 4));
+    HIP_TRY(hipMalloc(&m->W1b, MLP_IN * MLP_H * 2));
+    HIP_TRY(hipMalloc(&m->W2b, MLP_H * MLP_H * 2));
+    std::vector<float> P;
+    init_params(seed, P);
+    TRY(upload_params(m.get(), P.data()));
+    *out = m.release();
+    return SMG_OK;
+}
+
+extern "C" void smg_model_free(smg_model* m) {
+    if (!m) return;
+    (void)hipSetDevice(m->dev->ordinal);
+    (void)hipStreamSynchronize(m->dev->stream);
+    if (m->dev->model == m) m->dev->model = nullptr;
+    delete m;
+}
+
+extern "C" size_t smg_model_param_count(void) { return MLP_PARAMS; }
+
+extern "C" int smg_model_get_params(const smg_model* m, float* params) {
+    if (!m || !params) return fail(SMG_E_INVALID, "null model");
+    HIP_TRY(hipSetDevice(m->dev->ordinal));
+    HIP_TRY(hipMemcpyAsync(params, m->P, MLP_PARAMS * 4, hipMemcpyDeviceToHost, m->dev->stream));
+    HIP_TRY(hipStreamSynchronize(m->dev->stream));
+    return SMG_OK;
+}
+
+extern "C" int smg_model_set_params(smg_model* m, const float* params) {
+    if (!m || !params) return fail(SMG_E_INVALID, "null model");
+    for (int i = 0; i < MLP_PARAMS; i++)
+        if (!std::isfinite(params[i])) return fail(SMG_E_INVALID, "parameters must be finite");
+    HIP_TRY(hipSetDevice(m->dev->ordinal));
+    return upload_params(m, params);
+}
+
+extern "C" int smg_model_collect(smg_model* m, smg_resident* const* rs, const smg_mesh_params* ps, size_t count) {
+    if (!m || !rs || !ps || count == 0) return fail(SMG_E_INVALID, "no molecules");
+    for (size_t i = 0; i < count; i++)
+        if (!rs[i] || rs[i]->dev != m->dev) return fail(SMG_E_INVALID, "molecule not on the model's device");
+    std::vector<smg_mesh_params> q(ps, ps + count);
+    for (auto& x : q) x.sizing = 0;
+    std::unique_ptr<smg_mesh> M(new smg_mesh());
+    M->dev = m->dev;
+    TRY(mesh_many(rs, q.data(), count, M.get(), nullptr, m));
+    HIP_TRY(hipStreamSynchronize(m->dev->stream));
+    return SMG_OK;
+}
+
+extern "C" size_t smg_model_samples(const smg_model* m) { return m ? m->n : 0; }
+
+extern "C" int smg_model_dataset(const smg_model* m, float* features, uint8_t* occupied, float* log_load, size_t cap,
+                                 size_t* n) {
+    if (!m || !n) return fail(SMG_E_INVALID, "null model");
+    *n = m->n;
+    size_t k = std::min(cap, m->n);
+    HIP_TRY(hipSetDevice(m->dev->ordinal));
+    if (features && k) HIP_TRY(hipMemcpy(features, m->X, k * MLP_IN * 4, hipMemcpyDeviceToHost));
+    if (occupied && k) HIP_TRY(hipMemcpy(occupied, m->occ, k, hipMemcpyDeviceToHost));
+    if (log_load && k) HIP_TRY(hipMemcpy(log_load, m->tgt, k * 4, hipMemcpyDeviceToHost));
+    return SMG_OK;
+}
+
+extern "C" int smg_model_forward(smg_model* m, const float* features, size_t n, float* out) {
+    if (!m || (!features && n) || (!out && n)) return fail(SMG_E_INVALID, "null model or arrays");
+    if (n >= (1u << 28)) return fail(SMG_E_LENGTH, "too many rows");
+    if (n == 0) return SMG_OK;
+    HIP_TRY(hipSetDevice(m->dev->ordinal));
+    hipStream_t s = m->dev->stream;
+    Temps T(s);
+    float *X, *Y;
+    uint32_t* nd;
+    HIP_TRY(T.get((void**)&X, n * MLP_IN * 4));
+    HIP_TRY(T.get((void**)&Y, n * MLP_OUT * 4));
+    HIP_TRY(T.get((void**)&nd, 4));
+    uint32_t n32 = (uint32_t)n;
+    HIP_TRY(hipMemcpyAsync(X, features, n * MLP_IN * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(nd, &n32, 4, hipMemcpyHostToDevice, s));
+    k_mlp_forward<<<g1(std::min<size_t>((n + MLP_ROWS - 1) / MLP_ROWS, 4096)), 256, 0, s>>>(X, nd, m->P, m->W1b,
+                                                                                           m->W2b, Y);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, Y, n * MLP_OUT * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return SMG_OK;
+}
+
+extern "C" int smg_model_gradients(smg_model* m, const float* features, const uint8_t* occupied, const float* log_load,
+                                   size_t n, float* grads, double* loss) {
+    if (!m || !features || !occupied || !log_load || !grads) return fail(SMG_E_INVALID, "null model or arrays");
+    if (n == 0 || n > 65536) return fail(SMG_E_LENGTH, "gradient batches hold 1..65536 rows");
+    HIP_TRY(hipSetDevice(m->dev->ordinal));
+    hipStream_t s = m->dev->stream;
+    Temps T(s);
+    float *X, *tg;
+    uint8_t* oc;
+    HIP_TRY(T.get((void**)&X, n * MLP_IN * 4));
+    HIP_TRY(T.get((void**)&tg, n * 4));
+    HIP_TRY(T.get((void**)&oc, n));
+    HIP_TRY(hipMemcpyAsync(X, features, n * MLP_IN * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(tg, log_load, n * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(oc, occupied, n, hipMemcpyHostToDevice, s));
+    TrainBufs B;
+    TRY(B.alloc(n, (uint32_t)((n + MLP_ROWS - 1) / MLP_ROWS)));
+    std::vector<uint32_t> idx(n);
+    for (size_t i = 0; i < n; i++) idx[i] = (uint32_t)i;
+    HIP_TRY(hipMemcpyAsync(B.idx, idx.data(), n * 4, hipMemcpyHostToDevice, s));
+    double S[MLP_STATS] = {0, 0, 0, 0};
+    TRY(mlp_step(m, X, oc, tg, B.idx, (uint32_t)n, B, 0.0f, true, S));
+    HIP_TRY(hipMemcpyAsync(grads, B.grad, MLP_PARAMS * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (loss) {
+        loss[0] = S[0] / (double)n;
+        loss[1] = 0.5 * LOAD_WEIGHT * S[1] / (double)n;
+    }
+    return SMG_OK;
+}
+
+extern "C" int smg_model_train(smg_model* m, uint32_t epochs, uint32_t batch, float lr, uint64_t seed,
+                               smg_train_stats* st) {
+    if (!m) return fail(SMG_E_INVALID, "null model");
+    if (m->n == 0) return fail(SMG_E_INVALID, "the model has no training samples (smg_model_collect)");
+    if (batch < MLP_ROWS || batch > (1u << 20) || !(lr > 0.0f && lr < 1.0f) || epochs > 100000)
+        return fail(SMG_E_INVALID, "batch 64..2^20 rows, 0 < lr < 1");
+    HIP_TRY(hipSetDevice(m->dev->ordinal));
+    hipStream_t s = m->dev->stream;
+    const size_t n = m->n;
+    TrainBufs B;
+    TRY(B.alloc(n, (batch + MLP_ROWS - 1) / MLP_ROWS));
+    std::vector<uint32_t> idx(n);
+    for (size_t i = 0; i < n; i++) idx[i] = (uint32_t)i;
+    uint64_t x = seed * 0x9E3779B97F4A7C15ull + 1;
+    auto rnd = [&]() {
+        x ^= x >> 12;
+        x ^= x << 25;
+        x ^= x >> 27;
+        return x * 0x2545F4914F6CDD1Dull;
+    };
+    HIP_TRY(hipEventRecord(m->dev->ev[14], s));
+    for (uint32_t e = 0; e < epochs; e++) {
+        for (size_t i = n - 1; i > 0; i--) std::swap(idx[i], idx[rnd() % (i + 1)]);
+        HIP_TRY(hipMemcpyAsync(B.idx, idx.data(), n * 4, hipMemcpyHostToDevice, s));
+        for (size_t b0 = 0; b0 < n; b0 += batch) {
+            uint32_t nb = (uint32_t)std::min<size_t>(batch, n - b0);
+            TRY(mlp_step(m, m->X, m->occ, m->tgt, B.idx + b0, nb, B, lr, false, nullptr));
+        }
+        HIP_TRY(hipStreamSynchronize(s));  // the index list is rewritten next epoch
+    }
+    HIP_TRY(hipEventRecord(m->dev->ev[15], s));
+    if (st) {
+        // Loss over the whole set with the trained weights (no update).
+        for (size_t i = 0; i < n; i++) idx[i] = (uint32_t)i;
+        HIP_TRY(hipMemcpyAsync(B.idx, idx.data(), n * 4, hipMemcpyHostToDevice, s));
+        double S[MLP_STATS] = {0, 0, 0, 0};
+        for (size_t b0 = 0; b0 < n; b0 += batch) {
+            uint32_t nb = (uint32_t)std::min<size_t>(batch, n - b0);
+            double part[MLP_STATS] = {0, 0, 0, 0};
+            TRY(mlp_step(m, m->X, m->occ, m->tgt, B.idx + b0, nb, B, 0.0f, false, part));
+            // k_mlp_grad scales by the batch, the loss terms are sums.
+            for (int k = 0; k < MLP_STATS; k++) S[k] += part[k];
+        }
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, m->dev->ev[14], m->dev->ev[15]));
+        st->samples = n;
+        st->epochs = epochs;
+        st->batch = batch;
+        st->occupancy_loss = S[0] / (double)n;
+        st->occupancy_accuracy = S[2] / (double)n;
+        st->load_rmse = S[3] > 0.0 ? std::sqrt(S[1] / S[3]) : 0.0;
+        st->train_ms = ms;
+    }
+    return SMG_OK;
+}
+
+extern "C" int smg_model_set_policy(smg_model* m, float scale, int32_t max_tier) {
+    if (!m) return fail(SMG_E_INVALID, "null model");
+    if (!(scale > 0.0f && scale < 100.0f) || max_tier < 0 || max_tier > TIERS - 1)
+        return fail(SMG_E_INVALID, "scale in (0, 100), max_tier 0.." + std::to_string(TIERS - 1));
+    m->scale = scale;
+    m->max_tier = (uint32_t)max_tier;
+    return SMG_OK;
+}
+
+extern "C" int smg_device_set_model(smg_device* d, smg_model* m) {
+    if (!d) return fail(SMG_E_INVALID, "null device");
+    if (m && m->dev != d) return fail(SMG_E_INVALID, "the model lives on another device");
+    d->model = m;
+    return SMG_OK;
+}
+
 // ------------------------------------------------------------------ plans
 struct smg_plan {
     smg_device* dev = nullptr;
@@ -1537,6 +2039,8 @@ struct smg_plan {
     smg_plan_stats stats{};
     std::vector<smg_mesh*> views;  // meshes viewing the plan's buffers
     float* lbuf = nullptr;  // slot of the f32 copies (the table image points into it)
+    smg_model* model = nullptr;  // sizing model the graph was recorded with
+    bool timed = false;          // the graph records the phase events
     void* out[9] = {};      // slots of the mesh arrays: keys, values, pos, nrm, atom, patch, t, evals, tris
     void release() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -1544,6 +2048,8 @@ struct smg_plan {
         exec = nullptr;
         graph = nullptr;
         lbuf = nullptr;
+        model = nullptr;
+        timed = false;
         for (void*& o : out) o = nullptr;
         for (void* p : slots) (void)hipFree(p);
         slots.clear();
@@ -1600,8 +2106,9 @@ int check_caps(const smg_plan_caps* c) {
 }
 
 // Runs the plan's pipeline once on the stream (planned sizes); with
-// `capture`, records it into the plan's graph instead.
-int plan_pipeline(smg_plan* p, Table& t, smg_mesh* M, bool capture) {
+// `capture`, records it into the plan's graph instead, with the phase
+// events when `timed` (event records become graph nodes).
+int plan_pipeline(smg_plan* p, Table& t, smg_mesh* M, bool capture, const Sizing& sz, bool timed) {
     smg_device* d = p->dev;
     Run X;
     X.d = d;
@@ -1610,25 +2117,32 @@ int plan_pipeline(smg_plan* p, Table& t, smg_mesh* M, bool capture) {
     X.slot_bytes = &p->slot_bytes;
     Extents E = plan_extents(p->caps);
     if (!capture) {
-        TRY(pipeline(X, t, E, p->stage_h, p->sum_h, M, false));
+        TRY(pipeline(X, t, E, p->stage_h, p->sum_h, M, timed, sz));
         p->lbuf = X.lbuf;
         void* o[9] = {M->keys, M->values, M->pos, M->nrm, M->atom, M->patch, M->t, M->vevals, M->tris};
         for (int q = 0; q < 9; q++) p->out[q] = o[q];
         return SMG_OK;
     }
     HIP_TRY(hipStreamBeginCapture(d->stream, hipStreamCaptureModeRelaxed));
-    int rc = pipeline(X, t, E, p->stage_h, p->sum_h, M, false);
+    int rc = pipeline(X, t, E, p->stage_h, p->sum_h, M, timed, sz);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(d->stream, &g);
-    if (rc != SMG_OK) {
-        if (g) (void)hipGraphDestroy(g);
-        return rc;
+    if (rc == SMG_OK && e != hipSuccess) rc = fail(SMG_E_CALL, std::string("graph capture: ") + hipGetErrorString(e));
+    if (rc == SMG_OK) {
+        hipGraphExec_t x = nullptr;
+        hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        if (ei == hipSuccess) {
+            p->graph = g;
+            p->exec = x;
+            p->timed = timed;
+            p->stats.captures++;
+            return SMG_OK;
+        }
+        rc = fail(SMG_E_CALL, std::string("graph instantiation: ") + hipGetErrorString(ei));
     }
-    HIP_TRY(e);
-    p->graph = g;
-    HIP_TRY(hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0));
-    p->stats.captures++;
-    return SMG_OK;
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    return rc;
 }
 
 }  // namespace
@@ -1695,6 +2209,10 @@ extern "C" int smg_plan_run(smg_plan* p, smg_resident* const* residents, const s
     if (times) std::memset(times, 0, sizeof(*times));
     smg_device* d = p->dev;
     HIP_TRY(hipSetDevice(d->ordinal));
+    Sizing sz;
+    TRY(run_sizing(d, params, count, sz));
+    if (p->exec && sz.predict != p->model) return fail(SMG_E_INVALID, "the plan was recorded with another sizing");
+    p->model = sz.predict;
     p->stats.runs++;
     Table t;
     uint32_t flags = 0;
@@ -1734,19 +2252,22 @@ extern "C" int smg_plan_run(smg_plan* p, smg_resident* const* residents, const s
             V->vevals = (uint32_t*)p->out[7];
             V->tris = (uint32_t*)p->out[8];
         } else {
-            TRY(plan_pipeline(p, t, V.get(), false));
+            TRY(plan_pipeline(p, t, V.get(), false, sz, times != nullptr));
         }
+        const bool timed = times != nullptr && (p->exec ? p->timed : true);
         HIP_TRY(hipEventRecord(d->ev[15], d->stream));
         HIP_TRY(hipStreamSynchronize(d->stream));
-        TRY(finish(V.get(), t, p->sum_h, false, nullptr, flags));
+        TRY(finish(V.get(), t, p->sum_h, timed, times, flags));
         if (!flags) {
             if (!p->exec) {
-                // Record the graph the next runs replay (nothing executes).
+                // Record the graph the next runs replay (nothing executes),
+                // with the phase events if the runtime records them in a
+                // graph, else without.
                 smg_mesh C;
                 C.dev = d;
                 C.plan = p;
                 p->views.push_back(&C);
-                TRY(plan_pipeline(p, t, &C, true));
+                if (plan_pipeline(p, t, &C, true, sz, true) != SMG_OK) TRY(plan_pipeline(p, t, &C, true, sz, false));
             }
             if (times) {
                 float ms = 0.0f;

@@ -19,6 +19,8 @@ TUNINGS = [
     dict(refine_group_max=1),
     dict(refine_group_max=8, uniform_shared=0),
     dict(sample_first_tier=1, refine_first_tier=1, refine_group_max=32, uniform_shared=0),
+    dict(scratch_workgroups=16),
+    dict(sample_first_tier=2, refine_first_tier=2, scratch_workgroups=4096),
 ]
 
 
@@ -80,7 +82,8 @@ def test_autotuner_over_brick_sizes(tuned_device):
 def test_invalid_tunings_are_rejected(tuned_device):
     tuned_device.tuning = None
     for bad in (dict(sample_first_tier=3), dict(refine_group_max=12), dict(refine_group_max=128),
-                dict(uniform_shared=2), dict(refine_first_tier=-1)):
+                dict(uniform_shared=2), dict(refine_first_tier=-1), dict(scratch_workgroups=8),
+                dict(scratch_workgroups=9000)):
         with pytest.raises(sesmesh_amd.BackendError):
             tuned_device.tuning = bad
     assert tuned_device.tuning == sesmesh_amd.DEFAULT_TUNING
