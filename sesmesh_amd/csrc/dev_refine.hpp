@@ -824,9 +824,11 @@ __device__ __forceinline__ void refine_sub(const Mol* __restrict__ mols, const u
 }
 
 // PERSIST: residency-sized grid looping over the queue; otherwise one
-// sub-block per workgroup.
+// sub-block per workgroup. Five wavefronts per SIMD (at most 96 VGPRs, a
+// few spilled) beat four without spills: the candidate and blocker loops
+// wait on LDS and want wavefronts to hide it.
 template <bool GLOBAL, bool PERSIST>
-__global__ void __launch_bounds__(256) k_refine(const Mol* __restrict__ mols, const uint32_t* bmol, RCaps cap,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_refine(const Mol* __restrict__ mols, const uint32_t* bmol, RCaps cap,
                                                 RLayout lay, const int32_t* keys, const float* values, Queue q,
                                                 const unsigned long long* mask, const uint32_t* wpre,
                                                 uint32_t words, const uint32_t* vbase, VOut out, char* scratch,

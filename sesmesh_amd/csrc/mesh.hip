@@ -774,11 +774,14 @@ int read_u32(smg_device* d, const uint32_t* dev_ptr, uint32_t& v) {
 // 160 KB of LDS one gfx950 workgroup may hold), then global scratch for the
 // rest. Each pass takes the previous pass's overflow list.
 constexpr int TIERS = 4;  // the last one in global scratch
-constexpr Caps SAMPLE_CAPS[TIERS] = {{128, 64, 256, 64, 256},
+// The first tiers hold 192 candidates: 6 sampling and 5 refinement
+// workgroups fit a CU's 160 KB of LDS (256 held 5 and 4), and 0.25 A
+// sub-blocks need at most ~170.
+constexpr Caps SAMPLE_CAPS[TIERS] = {{128, 64, 192, 64, 256},
                                      {256, 128, 384, 128, 512},
                                      {640, 256, 768, 256, 2048},
                                      {1024, 512, 4096, 2048, 4096}};
-constexpr RCaps REFINE_CAPS[TIERS] = {{128, 64, 256, 256, 0, 0, 0},
+constexpr RCaps REFINE_CAPS[TIERS] = {{128, 64, 192, 256, 0, 0, 0},
                                       {256, 128, 384, 512, 0, 0, 0},
                                       {640, 256, 704, 1024, 0, 0, 0},
                                       {1024, 512, 4096, 4096, 0, 0, 0}};
@@ -802,8 +805,8 @@ enum : uint32_t {
 };
 // Capacity of each tier in candidate-load units (max(S + C, 2 near, 4 SAS
 // vertices, 4 inside) of the sampling lists; the refinement lists alike).
-constexpr float SAMPLE_LOAD_CAP[TIERS] = {256.0f, 384.0f, 768.0f, 2048.0f};
-constexpr float REFINE_LOAD_CAP[TIERS] = {256.0f, 384.0f, 704.0f, 2048.0f};
+constexpr float SAMPLE_LOAD_CAP[TIERS] = {192.0f, 384.0f, 768.0f, 2048.0f};
+constexpr float REFINE_LOAD_CAP[TIERS] = {192.0f, 384.0f, 704.0f, 2048.0f};
 // K_FLAGS bits: a device count exceeded its capacity.
 enum : uint32_t { OVER_BRICKS = 1, OVER_VERTICES = 2, OVER_TRIANGLES = 4, OVER_TABLE = 8 };
 
