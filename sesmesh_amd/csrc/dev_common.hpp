@@ -38,7 +38,20 @@ struct DGrid {
     uint32_t dx, dy, dz, pad;
 };
 
-// The resident f64 arrangement, SoA.
+// Packed records read by the hot gathers: one 32- or 64-byte load per
+// candidate instead of one load per field.
+struct alignas(32) SphereRec {
+    double x, y, z, R;  // centre, expanded radius
+};
+struct alignas(64) CircleRec {
+    double mx, my, mz, ax, ay, az, rho, pad;  // centre, axis, radius
+};
+struct alignas(32) VertexRec {
+    double x, y, z, pad;
+};
+
+// The resident f64 arrangement, SoA, plus packed records of the fields the
+// gathers read together.
 struct DArr {
     gptr<const double> cx, cy, cz, R, r;
     gptr<const uint8_t> status;
@@ -49,17 +62,53 @@ struct DArr {
     gptr<const uint32_t> cb_off, cb_items;
     gptr<const double> vx, vy, vz;
     gptr<const uint32_t> vatoms;
+    gptr<const SphereRec> sph;
+    gptr<const CircleRec> cir;
+    gptr<const VertexRec> ver;
     DGrid gs, gc, gv;
     uint32_t n_spheres, n_circles, n_vertices, pad;
     double probe, max_expanded, rho_max;
 };
 
-// Frame-local f32 copies (reference field/gpu.rs:253-318, kernel.rs:604-747).
+// Frame-local f32 copies (reference field/gpu.rs:253-318, kernel.rs:604-747),
+// one float4 per record: spheres (x, y, z, expanded radius), circles
+// (centre, radius), (axis, 0), (e1, 0), SAS vertices (x, y, z, 0).
 struct DLoc {
-    gptr<float> px, py, pz, pr, pr2, pr2tol;
-    gptr<float> mx, my, mz, ux, uy, uz, ex, ey, ez, rho;
-    gptr<float> vx, vy, vz;
+    gptr<float4> p4, m4, u4, e4, v4;
 };
+
+// Loads of the packed records through global pointers (as vector loads).
+typedef double dvec4 __attribute__((ext_vector_type(4)));
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+__device__ inline SphereRec load_rec(gptr<const SphereRec> p, uint32_t k) {
+    dvec4 v = ((gptr<const dvec4>)p)[k];
+    return SphereRec{v.x, v.y, v.z, v.w};
+}
+__device__ inline CircleRec load_rec(gptr<const CircleRec> p, uint32_t k) {
+    gptr<const dvec4> q = (gptr<const dvec4>)p + 2 * (size_t)k;
+    dvec4 a = q[0], b = q[1];
+    return CircleRec{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+}
+__device__ inline VertexRec load_rec(gptr<const VertexRec> p, uint32_t k) {
+    dvec4 v = ((gptr<const dvec4>)p)[k];
+    return VertexRec{v.x, v.y, v.z, v.w};
+}
+__device__ inline float4 load_f4(gptr<float4> p, uint32_t k) {
+    fvec4 v = ((gptr<const fvec4>)p)[k];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ inline void store_f4(gptr<float4> p, uint32_t k, float x, float y, float z, float w) {
+    fvec4 v = {x, y, z, w};
+    ((gptr<fvec4>)p)[k] = v;
+}
+
+// Squared f32 radius and squared radius less the inside tolerance of a
+// sphere record (kernel.rs:627-637).
+__device__ inline float rec_r2(float r) { return r * r; }
+__device__ inline float rec_r2tol(float r) {
+    float rt = fmaxf(r - INSIDE_TOL, 0.0f);
+    return rt * rt;
+}
 
 // Lattice and stage parameters of one mesh run.
 struct DRun {

@@ -196,7 +196,8 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
     gather3(A, bx, by, bz, radii, tmp4, [&](uint32_t grid, uint32_t k) {
         if (grid == 0) {
             if (A.status[k] == S_CONTAINED) return;
-            double R = A.R[k], cx = A.cx[k], cy = A.cy[k], cz = A.cz[k];
+            const SphereRec S = load_rec(A.sph, k);
+            double R = S.R, cx = S.x, cy = S.y, cz = S.z;
             double d = ddist(bx, by, bz, cx, cy, cz);
             if (d < R + reach) {
                 uint32_t at = atomicAdd(&n[R_NEAR], 1u);
@@ -236,12 +237,13 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
                 }
             }
         } else if (grid == 1) {
-            double yx = bx - A.mx[k], yy = by - A.my[k], yz = bz - A.mz[k];
-            double ax = A.ax[k], ay = A.ay[k], az = A.az[k];
+            const CircleRec C = load_rec(A.cir, k);
+            double yx = bx - C.mx, yy = by - C.my, yz = bz - C.mz;
+            double ax = C.ax, ay = C.ay, az = C.az;
             double along = yx * ax + yy * ay + yz * az;
             double wx = yx - ax * along, wy = yy - ay * along, wz = yz - az * along;
             double wl = sqrt(wx * wx + wy * wy + wz * wz);
-            double rr = A.crho[k];
+            double rr = C.rho;
             double dist = sqrt(along * along + (wl - rr) * (wl - rr));
             if (!(dist <= reach)) return;
             atomicAdd(&n[R_C], 1u);
@@ -253,7 +255,8 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
             s.ckey[q] = total_key(dist);
             s.cid[q] = k | (1u << KIND_SHIFT);
         } else {
-            double d = ddist(bx, by, bz, A.vx[k], A.vy[k], A.vz[k]);
+            const VertexRec V = load_rec(A.ver, k);
+            double d = ddist(bx, by, bz, V.x, V.y, V.z);
             if (!(d <= reach)) return;
             atomicAdd(&n[R_V], 1u);
             uint32_t q = atomicAdd(&n[R_CAND], 1u);
@@ -274,26 +277,29 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
         uint32_t k = s.cid[p] & ID_MASK;
         double* d = s.cdat + (size_t)CDAT * p;
         if (p < ns) {
-            d[0] = A.cx[k];
-            d[1] = A.cy[k];
-            d[2] = A.cz[k];
-            d[3] = A.R[k];
+            const SphereRec S = load_rec(A.sph, k);
+            d[0] = S.x;
+            d[1] = S.y;
+            d[2] = S.z;
+            d[3] = S.R;
             s.coff[p] = A.nb_off[k];
             s.cpre[p] = A.nb_off[k + 1] - A.nb_off[k];
         } else if (p < ns + nc) {
-            d[0] = A.mx[k];
-            d[1] = A.my[k];
-            d[2] = A.mz[k];
-            d[3] = A.ax[k];
-            d[4] = A.ay[k];
-            d[5] = A.az[k];
-            d[6] = A.crho[k];
+            const CircleRec C = load_rec(A.cir, k);
+            d[0] = C.mx;
+            d[1] = C.my;
+            d[2] = C.mz;
+            d[3] = C.ax;
+            d[4] = C.ay;
+            d[5] = C.az;
+            d[6] = C.rho;
             s.coff[p] = A.cb_off[k];
             s.cpre[p] = A.cb_off[k + 1] - A.cb_off[k];
         } else {
-            d[0] = A.vx[k];
-            d[1] = A.vy[k];
-            d[2] = A.vz[k];
+            const VertexRec V = load_rec(A.ver, k);
+            d[0] = V.x;
+            d[1] = V.y;
+            d[2] = V.z;
         }
         for (uint32_t w = 0; w < nw; w++) s.cbits[w * cc + p] = 0ull;
     }

@@ -252,22 +252,24 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
     gather3(A, b[0], b[1], b[2], radii, tmp4, [&](uint32_t grid, uint32_t k) {
         if (grid == 0) {  // near spheres, the inside group and class S (gather_near, gather_ses)
             if (A.status[k] == S_CONTAINED) return;
-            double R = A.R[k];
-            double d = ddist(b[0], b[1], b[2], A.cx[k], A.cy[k], A.cz[k]);
+            const SphereRec S = load_rec(A.sph, k);
+            double R = S.R;
+            double d = ddist(b[0], b[1], b[2], S.x, S.y, S.z);
             if (d >= R + reach) return;
             uint32_t at = atomicAdd(&n[C_NEAR], 1u);
             if (at >= cap.near) {
                 n[C_OVER] = 1u;
                 return;
             }
-            s.nd[at] = make_float4(Lc.px[k], Lc.py[k], Lc.pz[k], Lc.pr2tol[k]);
+            const float4 p = load_f4(Lc.p4, k);
+            s.nd[at] = make_float4(p.x, p.y, p.z, rec_r2tol(p.w));
             hash_put(s.hk, s.hv, hmask, k, at);
             if (d < R + rho) {
                 uint32_t q = atomicAdd(&n[C_IN], 1u);
                 if (q >= cap.inside)
                     n[C_OVER] = 1u;
                 else
-                    s.id[q] = make_float4(Lc.px[k], Lc.py[k], Lc.pz[k], Lc.pr2[k]);
+                    s.id[q] = make_float4(p.x, p.y, p.z, rec_r2(p.w));
             }
             double lb = fabs(d - R);
             if (A.status[k] == S_EXPOSED && lb <= reach) {
@@ -281,12 +283,13 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
                 }
             }
         } else if (grid == 1) {  // class C (gather_ses, kernel.rs:701-736)
-            double yx = b[0] - A.mx[k], yy = b[1] - A.my[k], yz = b[2] - A.mz[k];
-            double ax = A.ax[k], ay = A.ay[k], az = A.az[k];
+            const CircleRec C = load_rec(A.cir, k);
+            double yx = b[0] - C.mx, yy = b[1] - C.my, yz = b[2] - C.mz;
+            double ax = C.ax, ay = C.ay, az = C.az;
             double along = yx * ax + yy * ay + yz * az;
             double wx = yx - ax * along, wy = yy - ay * along, wz = yz - az * along;
             double wl = sqrt(wx * wx + wy * wy + wz * wz);
-            double rr = A.crho[k];
+            double rr = C.rho;
             double dist = sqrt(along * along + (wl - rr) * (wl - rr));
             if (!(dist <= reach)) return;
             atomicAdd(&n[C_C], 1u);
@@ -298,13 +301,14 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
             s.ckey[q] = total_key(dist);
             s.cid[q] = k | (1u << KIND_SHIFT);
         } else {  // class V (kernel.rs:738-747)
-            if (!(ddist(b[0], b[1], b[2], A.vx[k], A.vy[k], A.vz[k]) <= reach)) return;
+            const VertexRec V = load_rec(A.ver, k);
+            if (!(ddist(b[0], b[1], b[2], V.x, V.y, V.z) <= reach)) return;
             uint32_t q = atomicAdd(&n[C_V], 1u);
             if (q >= cap.v) {
                 n[C_OVER] = 1u;
                 return;
             }
-            s.vd[q] = make_float4(Lc.vx[k], Lc.vy[k], Lc.vz[k], 0.0f);
+            s.vd[q] = load_f4(Lc.v4, k);
         }
     });
     if (n[C_OVER]) return false;
@@ -317,13 +321,13 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
     for (uint32_t p = tid; p < total; p += nt) {
         uint32_t k = s.cid[p] & ID_MASK;
         if (p < ns) {
-            s.cdat[3 * p] = make_float4(Lc.px[k], Lc.py[k], Lc.pz[k], Lc.pr[k]);
+            s.cdat[3 * p] = load_f4(Lc.p4, k);
             s.coff[p] = A.nb_off[k];
             s.cpre[p] = A.nb_off[k + 1] - A.nb_off[k];
         } else {
-            s.cdat[3 * p] = make_float4(Lc.mx[k], Lc.my[k], Lc.mz[k], Lc.rho[k]);
-            s.cdat[3 * p + 1] = make_float4(Lc.ux[k], Lc.uy[k], Lc.uz[k], 0.0f);
-            s.cdat[3 * p + 2] = make_float4(Lc.ex[k], Lc.ey[k], Lc.ez[k], 0.0f);
+            s.cdat[3 * p] = load_f4(Lc.m4, k);
+            s.cdat[3 * p + 1] = load_f4(Lc.u4, k);
+            s.cdat[3 * p + 2] = load_f4(Lc.e4, k);
             s.coff[p] = A.cb_off[k];
             s.cpre[p] = A.cb_off[k + 1] - A.cb_off[k];
         }

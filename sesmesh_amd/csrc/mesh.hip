@@ -210,31 +210,18 @@ __device__ inline void localize_one(const Mol* __restrict__ mols, const uint64_t
     const double fx = (double)P.frame[0] * P.h, fy = (double)P.frame[1] * P.h, fz = (double)P.frame[2] * P.h;
     uint32_t i = (uint32_t)(g - loc_off[m]);
     if (i < A.n_spheres) {
-        L.px[i] = (float)(A.cx[i] - fx);
-        L.py[i] = (float)(A.cy[i] - fy);
-        L.pz[i] = (float)(A.cz[i] - fz);
-        float r32 = (float)A.R[i];
-        L.pr[i] = r32;
-        L.pr2[i] = r32 * r32;
-        float rt = fmaxf(r32 - INSIDE_TOL, 0.0f);
-        L.pr2tol[i] = rt * rt;
+        SphereRec S = load_rec(A.sph, i);
+        store_f4(L.p4, i, (float)(S.x - fx), (float)(S.y - fy), (float)(S.z - fz), (float)S.R);
     }
     if (i < A.n_circles) {
-        L.mx[i] = (float)(A.mx[i] - fx);
-        L.my[i] = (float)(A.my[i] - fy);
-        L.mz[i] = (float)(A.mz[i] - fz);
-        L.ux[i] = (float)A.ax[i];
-        L.uy[i] = (float)A.ay[i];
-        L.uz[i] = (float)A.az[i];
-        L.ex[i] = (float)A.e1x[i];
-        L.ey[i] = (float)A.e1y[i];
-        L.ez[i] = (float)A.e1z[i];
-        L.rho[i] = (float)A.crho[i];
+        CircleRec C = load_rec(A.cir, i);
+        store_f4(L.m4, i, (float)(C.mx - fx), (float)(C.my - fy), (float)(C.mz - fz), (float)C.rho);
+        store_f4(L.u4, i, (float)C.ax, (float)C.ay, (float)C.az, 0.0f);
+        store_f4(L.e4, i, (float)A.e1x[i], (float)A.e1y[i], (float)A.e1z[i], 0.0f);
     }
     if (i < A.n_vertices) {
-        L.vx[i] = (float)(A.vx[i] - fx);
-        L.vy[i] = (float)(A.vy[i] - fy);
-        L.vz[i] = (float)(A.vz[i] - fz);
+        VertexRec V = load_rec(A.ver, i);
+        store_f4(L.v4, i, (float)(V.x - fx), (float)(V.y - fy), (float)(V.z - fz), 0.0f);
     }
 }
 
@@ -666,6 +653,16 @@ extern "C" int smg_resident_new(smg_device* dev, const smg_arrangement_ref* a, s
     pk.add(vxyz[1].data(), nv, &A.vy);
     pk.add(vxyz[2].data(), nv, &A.vz);
     pk.add(a->vertex_atoms, 3 * nv, &A.vatoms);
+    std::vector<SphereRec> srec(n);
+    for (size_t i = 0; i < n; i++) srec[i] = SphereRec{x[i], y[i], z[i], a->expanded[i]};
+    std::vector<CircleRec> crec(nc);
+    for (size_t c = 0; c < nc; c++)
+        crec[c] = CircleRec{cf[0][c], cf[1][c], cf[2][c], cf[3][c], cf[4][c], cf[5][c], cf[9][c], 0.0};
+    std::vector<VertexRec> vrec(nv);
+    for (size_t v = 0; v < nv; v++) vrec[v] = VertexRec{vxyz[0][v], vxyz[1][v], vxyz[2][v], 0.0};
+    pk.add(srec.data(), n, &A.sph);
+    pk.add(crec.data(), nc, &A.cir);
+    pk.add(vrec.data(), nv, &A.ver);
     pk.grid(a->sphere_grid, A.gs);
     pk.grid(a->circle_grid, A.gc);
     pk.grid(a->vertex_grid, A.gv);
@@ -1038,17 +1035,20 @@ StageLayout stage_layout(size_t rows) {
     return L;
 }
 
+// f32 copies per molecule: LOC_ARRAYS float4 arrays of max(spheres,
+// circles, SAS vertices) entries.
+constexpr int LOC_ARRAYS = 5;
+
 // Writes the table into `host` (laid out as stage_layout), pointing each
-// molecule's f32 copies into `lbuf` (19 arrays per molecule).
+// molecule's f32 copies into `lbuf`.
 void write_stage(Table& t, float* lbuf, char* host) {
     StageLayout L = stage_layout(t.rows);
     for (size_t m = 0; m < t.rows; m++) {
         DLoc& Lc = t.H[m].L;
         size_t nmax = t.loc[m + 1] - t.loc[m];
-        float* base = lbuf + 19 * t.loc[m];
-        gptr<float>* fields[19] = {&Lc.px, &Lc.py, &Lc.pz, &Lc.pr, &Lc.pr2, &Lc.pr2tol, &Lc.mx, &Lc.my, &Lc.mz, &Lc.ux,
-                                   &Lc.uy, &Lc.uz, &Lc.ex, &Lc.ey, &Lc.ez, &Lc.rho,    &Lc.vx, &Lc.vy, &Lc.vz};
-        for (int q = 0; q < 19; q++) *fields[q] = to_global(base + q * nmax);
+        float4* base = (float4*)lbuf + LOC_ARRAYS * t.loc[m];
+        gptr<float4>* fields[LOC_ARRAYS] = {&Lc.p4, &Lc.m4, &Lc.u4, &Lc.e4, &Lc.v4};
+        for (int q = 0; q < LOC_ARRAYS; q++) *fields[q] = to_global(base + q * nmax);
     }
     RunHdr h{(uint32_t)t.words(), (uint32_t)t.dense, (uint32_t)t.locals(), (uint32_t)t.spheres()};
     std::memset(host, 0, L.bytes);
@@ -1124,7 +1124,7 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     float* lbuf;
     TRY(X.temp(&stage, SL.bytes));
     TRY(X.temp(&sum, UL.bytes));
-    TRY(X.temp(&lbuf, 19 * E.locals));
+    TRY(X.temp(&lbuf, 4 * LOC_ARRAYS * E.locals));
     X.lbuf = lbuf;
     write_stage(t, lbuf, stage_host);
     TRY(rec(0));
