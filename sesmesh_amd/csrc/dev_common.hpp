@@ -11,6 +11,10 @@
 namespace smg {
 
 constexpr int WAVE = 64;
+// Workgroup size of the passes over the large LDS tiers: with one or two
+// workgroups per CU, 1024 threads give each SIMD four wavefronts to hide
+// LDS and memory latency with, where 256 would give it one.
+constexpr int WIDE = 1024;
 constexpr double EPS = 1e-9;            // geom/arrangement.rs:26
 constexpr double CLASSIFY_EPS = 2e-3;   // field/lattice.rs:20
 constexpr float INSIDE_TOL = 2e-5f;     // field/kernel.rs:42

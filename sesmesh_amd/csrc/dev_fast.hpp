@@ -389,7 +389,7 @@ __device__ __forceinline__ void refine_sub_fast(const Mol* __restrict__ mols, co
                                                 uint32_t* n_overflow, unsigned long long* evals_total) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     __shared__ uint32_t n[R_N];
-    __shared__ uint32_t tmp4[4];
+    __shared__ uint32_t tmp4[WIDE / WAVE];  // block scans: one slot per wavefront
     __shared__ unsigned long long evals_sum;
     uint32_t brick = j >> 3, sb = j & 7;
     const uint32_t mol = bmol[brick];
@@ -480,6 +480,21 @@ __device__ __forceinline__ void refine_sub_fast(const Mol* __restrict__ mols, co
 
 template <bool GLOBAL, bool PERSIST>
 __global__ void __launch_bounds__(256) k_refine_fast(const Mol* __restrict__ mols, const uint32_t* bmol, RCaps cap,
+                                                     RLayout lay, FLayout flay, const int32_t* keys,
+                                                     const float* values, Queue q, const unsigned long long* mask,
+                                                     const uint32_t* wpre, uint32_t words, const uint32_t* vbase,
+                                                     VOut out, char* scratch, size_t slice, uint32_t* overflow,
+                                                     uint32_t* n_overflow, unsigned long long* evals_total) {
+    __shared__ uint32_t ticket;
+    for_each_item<PERSIST>(q, &ticket, [&](uint32_t j) {
+        refine_sub_fast<GLOBAL>(mols, bmol, cap, lay, flay, keys, values, j, mask, wpre, words, vbase, out, scratch,
+                                slice, overflow, n_overflow, evals_total);
+    });
+}
+
+// k_refine_fast with WIDE threads, for the large tiers.
+template <bool GLOBAL, bool PERSIST>
+__global__ void __launch_bounds__(WIDE) k_refine_fast_wide(const Mol* __restrict__ mols, const uint32_t* bmol, RCaps cap,
                                                      RLayout lay, FLayout flay, const int32_t* keys,
                                                      const float* values, Queue q, const unsigned long long* mask,
                                                      const uint32_t* wpre, uint32_t words, const uint32_t* vbase,

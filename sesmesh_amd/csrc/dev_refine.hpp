@@ -734,7 +734,7 @@ __device__ __forceinline__ void refine_sub(const Mol* __restrict__ mols, const u
                                            unsigned long long* evals_total) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     __shared__ uint32_t n[R_N];
-    __shared__ uint32_t tmp4[4];
+    __shared__ uint32_t tmp4[WIDE / WAVE];  // block scans: one slot per wavefront
     __shared__ unsigned long long evals_sum;
     uint32_t brick = j >> 3, sb = j & 7;
     const uint32_t mol = bmol[brick];
@@ -835,6 +835,21 @@ __device__ __forceinline__ void refine_sub(const Mol* __restrict__ mols, const u
 // wait on LDS and want wavefronts to hide it.
 template <bool GLOBAL, bool PERSIST>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_refine(const Mol* __restrict__ mols, const uint32_t* bmol, RCaps cap,
+                                                RLayout lay, const int32_t* keys, const float* values, Queue q,
+                                                const unsigned long long* mask, const uint32_t* wpre,
+                                                uint32_t words, const uint32_t* vbase, VOut out, char* scratch,
+                                                size_t slice, uint32_t* overflow, uint32_t* n_overflow,
+                                                unsigned long long* evals_total) {
+    __shared__ uint32_t ticket;
+    for_each_item<PERSIST>(q, &ticket, [&](uint32_t j) {
+        refine_sub<GLOBAL>(mols, bmol, cap, lay, keys, values, j, mask, wpre, words, vbase, out, scratch, slice,
+                           overflow, n_overflow, evals_total);
+    });
+}
+
+// k_refine with WIDE threads, for the large tiers.
+template <bool GLOBAL, bool PERSIST>
+__global__ void __launch_bounds__(WIDE) k_refine_wide(const Mol* __restrict__ mols, const uint32_t* bmol, RCaps cap,
                                                 RLayout lay, const int32_t* keys, const float* values, Queue q,
                                                 const unsigned long long* mask, const uint32_t* wpre,
                                                 uint32_t words, const uint32_t* vbase, VOut out, char* scratch,

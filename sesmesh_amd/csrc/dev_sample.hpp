@@ -364,7 +364,7 @@ __device__ __forceinline__ void sample_sub(const Mol* __restrict__ mols, const u
                                            uint32_t* n_overflow, unsigned long long* n_samples, uint32_t* loads) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     __shared__ uint32_t n[C_N];
-    __shared__ uint32_t tmp4[4];
+    __shared__ uint32_t tmp4[4];  // block scans: one slot per wavefront
     __shared__ uint32_t owned[512];
     __shared__ uint32_t n_owned;
     uint32_t brick = j >> 3, sb = j & 7;

@@ -1305,6 +1305,9 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
             } else if (pass == first) {
                 items = nsw;
             }
+            // 256 threads in every pass: the lane kernel takes one sample per
+            // thread (at most (sub + 1)^3 per sub-block), so wider
+            // workgroups would idle (measured: no gain on the large tiers).
             if (pass < TIERS - 1 && (!X.planned() || pass == first)) {
                 k_sample<false, false><<<g1(items), 256, lay.bytes, s>>>(mols, bmol, cap, lay, M->keys, st, q,
                                                                          M->values, nullptr, 0, next, nnext, nsamp,
@@ -1425,23 +1428,45 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
             } else if (pass == first) {
                 items = nrw;
             }
+            // The large tiers (one or two workgroups per CU) run WIDE
+            // threads: more wavefronts per SIMD, more edges in flight.
+            const bool wide = pass >= 1;
             if (pass < TIERS - 1 && (!X.planned() || pass == first)) {
-                if (fast)
+                if (fast && wide)
+                    k_refine_fast_wide<false, false><<<g1(items), WIDE, bytes, s>>>(
+                        mols, bmol, cap, lay, flay, M->keys, M->values, q, mask, wpre, words, vbase, vo, nullptr, 0,
+                        next, nnext, evd);
+                else if (fast)
                     k_refine_fast<false, false><<<g1(items), 256, bytes, s>>>(mols, bmol, cap, lay, flay, M->keys,
                                                                               M->values, q, mask, wpre, words, vbase,
                                                                               vo, nullptr, 0, next, nnext, evd);
+                else if (wide)
+                    k_refine_wide<false, false><<<g1(items), WIDE, bytes, s>>>(mols, bmol, cap, lay, M->keys,
+                                                                               M->values, q, mask, wpre, words, vbase,
+                                                                               vo, nullptr, 0, next, nnext, evd);
                 else
                     k_refine<false, false><<<g1(items), 256, bytes, s>>>(mols, bmol, cap, lay, M->keys, M->values, q,
                                                                          mask, wpre, words, vbase, vo, nullptr, 0,
                                                                          next, nnext, evd);
             } else if (pass < TIERS - 1) {
-                const void* kern =
-                    fast ? (const void*)k_refine_fast<false, true> : (const void*)k_refine<false, true>;
-                uint32_t grid = g1(std::min<uint64_t>(nsub, resident(d, kern, 256, bytes)));
-                if (fast)
+                const void* kern = fast ? (wide ? (const void*)k_refine_fast_wide<false, true>
+                                                : (const void*)k_refine_fast<false, true>)
+                                        : (wide ? (const void*)k_refine_wide<false, true>
+                                                : (const void*)k_refine<false, true>);
+                const uint32_t block = wide ? WIDE : 256;
+                uint32_t grid = g1(std::min<uint64_t>(nsub, resident(d, kern, block, bytes)));
+                if (fast && wide)
+                    k_refine_fast_wide<false, true><<<grid, WIDE, bytes, s>>>(mols, bmol, cap, lay, flay, M->keys,
+                                                                              M->values, q, mask, wpre, words, vbase,
+                                                                              vo, nullptr, 0, next, nnext, evd);
+                else if (fast)
                     k_refine_fast<false, true><<<grid, 256, bytes, s>>>(mols, bmol, cap, lay, flay, M->keys,
                                                                         M->values, q, mask, wpre, words, vbase, vo,
                                                                         nullptr, 0, next, nnext, evd);
+                else if (wide)
+                    k_refine_wide<false, true><<<grid, WIDE, bytes, s>>>(mols, bmol, cap, lay, M->keys, M->values, q,
+                                                                         mask, wpre, words, vbase, vo, nullptr, 0,
+                                                                         next, nnext, evd);
                 else
                     k_refine<false, true><<<grid, 256, bytes, s>>>(mols, bmol, cap, lay, M->keys, M->values, q, mask,
                                                                    wpre, words, vbase, vo, nullptr, 0, next, nnext,
@@ -1451,13 +1476,13 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
                 char* scratch;
                 TRY(X.temp(&scratch, (size_t)grid * bytes));
                 if (fast)
-                    k_refine_fast<true, true><<<grid, 256, 0, s>>>(mols, bmol, cap, lay, flay, M->keys, M->values, q,
-                                                                   mask, wpre, words, vbase, vo, scratch, bytes, next,
-                                                                   nnext, evd);
+                    k_refine_fast_wide<true, true><<<grid, WIDE, 0, s>>>(mols, bmol, cap, lay, flay, M->keys,
+                                                                         M->values, q, mask, wpre, words, vbase, vo,
+                                                                         scratch, bytes, next, nnext, evd);
                 else
-                    k_refine<true, true><<<grid, 256, 0, s>>>(mols, bmol, cap, lay, M->keys, M->values, q, mask,
-                                                              wpre, words, vbase, vo, scratch, bytes, next, nnext,
-                                                              evd);
+                    k_refine_wide<true, true><<<grid, WIDE, 0, s>>>(mols, bmol, cap, lay, M->keys, M->values, q, mask,
+                                                                    wpre, words, vbase, vo, scratch, bytes, next,
+                                                                    nnext, evd);
             }
             HIP_TRY(hipGetLastError());
             if (pass == first) TRY(rec(13));
