@@ -242,6 +242,10 @@ def main():
     ap.add_argument("--precision", type=int, default=0, metavar="COPIES",
                     help="precision study: exact vs fast refinement on COPIES (60: the 2.4M-atom shell) "
                          "icosahedral copies of a 40k-atom globule at 1.0 A (0: skip)")
+    ap.add_argument("--no-plain", dest="plain", action="store_false",
+                    help="skip timing the plain path (one smg_mesh_new per mesh)")
+    ap.add_argument("--no-tune", dest="tune", action="store_false",
+                    help="skip the autotuner (default launch shapes)")
     ap.add_argument("--no-sizing", dest="sizing", action="store_false",
                     help="skip the learned-sizing study (training on seeded globules, 0.75 / 1.0 A meshes)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -266,19 +270,23 @@ def main():
 
     # The plain GPU path: one smg_mesh_new per mesh (exact counts read back,
     # outputs allocated and freed per mesh), default launch shapes.
-    plain = []
-    for i in range(args.warmup + max(2, args.steps // 2)):
-        t = time.perf_counter()
-        m = res.mesh(spacing, brick)
-        del m
-        if i >= args.warmup:
-            plain.append(time.perf_counter() - t)
-    plain_s = float(np.median(plain))
-    progress(f"plain path: {plain_s:.4f} s/mesh")
+    plain_s = None
+    if args.plain:
+        plain = []
+        for i in range(args.warmup + max(2, args.steps // 2)):
+            t = time.perf_counter()
+            m = res.mesh(spacing, brick)
+            del m
+            if i >= args.warmup:
+                plain.append(time.perf_counter() - t)
+        plain_s = float(np.median(plain))
+        progress(f"plain path: {plain_s:.4f} s/mesh")
 
     # Autotuner (untimed): launch shapes for this workload.
-    tuned = sesmesh_amd.tune([res], [spacing], brick=brick, repeats=2)
-    progress(f"autotuner: {tuned['tuning']} ({tuned['ms']:.2f} ms device time)")
+    tuned = {"tuning": dev.tuning, "ms": None}
+    if args.tune:
+        tuned = sesmesh_amd.tune([res], [spacing], brick=brick, repeats=2)
+        progress(f"autotuner: {tuned['tuning']} ({tuned['ms']:.2f} ms device time)")
 
     # Timed steps: the capture-ready plan (buffers sized once, the whole
     # pipeline replayed as a hipGraph each step, phase events recorded in
@@ -379,7 +387,7 @@ def main():
         "best_s": min(times), "median_s": float(np.median(times)),
         "path": ("capture-ready plan: whole mesh stage replayed as a hipGraph per step, buffers sized once, "
                  "autotuned launch shapes"),
-        "plain_path_s": plain_s, "speedup_vs_plain_path": plain_s / s_per_mesh,
+        "plain_path_s": plain_s, "speedup_vs_plain_path": plain_s / s_per_mesh if plain_s else None,
         "tuning": tuned["tuning"], "plan_stats": stats,
         "phases_ms": ph, "kernels": others, "arrangement_s": t_arr,
         "batched": batch, "sizing_study": sizing, "precision_study": precision, "device": dev.name,
