@@ -94,11 +94,14 @@ __device__ __forceinline__ void uniform_brick(const Mol* __restrict__ mols, cons
     if (G.r == 0) st[8 * unit + ball] = state;
 }
 
-// One workgroup per surface brick, item blockIdx.x of the queue q.
+// One workgroup per surface brick of the queue q (PERSIST: a residency-
+// sized grid looping over it; otherwise item blockIdx.x).
+template <bool PERSIST>
 __global__ void __launch_bounds__(256) k_uniform(const Mol* __restrict__ mols, const uint32_t* bmol,
                                                  const int32_t* keys, RCaps cap, RLayout lay, Queue q, uint8_t* st,
                                                  uint32_t* overflow, uint32_t* n_overflow) {
-    for_each_item<false>(q, nullptr, [&](uint32_t unit) {
+    __shared__ uint32_t ticket;
+    for_each_item<PERSIST>(q, &ticket, [&](uint32_t unit) {
         uniform_brick<false>(mols, bmol, keys, cap, lay, unit, st, nullptr, 0, overflow, n_overflow);
     });
 }

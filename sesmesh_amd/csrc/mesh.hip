@@ -782,9 +782,10 @@ constexpr RCaps REFINE_CAPS[TIERS] = {{128, 64, 192, 256, 0, 0, 0},
                                       {256, 128, 384, 512, 0, 0, 0},
                                       {640, 256, 704, 1024, 0, 0, 0},
                                       {1024, 512, 4096, 4096, 0, 0, 0}};
-// Sub-block classification: one LDS tier; bricks beyond it take one
-// wavefront per brick, which beats larger tiers at one or two workgroups per CU.
-constexpr RCaps UNIFORM_CAPS = {192, 128, 256, 512, 0, 1, 0};
+// Sub-block classification: two LDS tiers (5 and 3 workgroups per CU);
+// bricks beyond them take one wavefront per brick.
+constexpr int UNIFORM_TIERS = 2;
+constexpr RCaps UNIFORM_CAPS[UNIFORM_TIERS] = {{128, 64, 192, 256, 0, 1, 0}, {192, 128, 256, 512, 0, 1, 0}};
 constexpr uint32_t LDS_EDGES = 512;      // crossing-edge buffer of the LDS passes
 constexpr uint32_t SCRATCH_CHUNK = 1024;  // default workgroups of a scratch pass (one list slice each)
 constexpr uint32_t STATIC_LDS = 6144;     // static LDS of k_sample / k_refine, bound
@@ -792,7 +793,8 @@ constexpr uint32_t NO_CAP = 0xffffffffu;
 
 // Device counters of a run (u32 slots, zero at the start of the run).
 enum : uint32_t {
-    K_TOUCHED, K_NB, K_NSUB, K_NSW, K_NV, K_NRW, K_NT, K_FLAGS, K_ERR, K_UOVER,
+    K_TOUCHED, K_NB, K_NSUB, K_NSW, K_NV, K_NRW, K_NT, K_FLAGS, K_ERR,
+    K_UOVER, K_Q_UNIFORM = K_UOVER + UNIFORM_TIERS,  // per sub-block classification tier: overflow count
     // Per pass: items in its list (predicted there, then appended by the
     // passes before it), items predicted there, and the last pass's overflow.
     K_SLIST0, K_SPRED0 = K_SLIST0 + TIERS, K_SLAST = K_SPRED0 + TIERS,
@@ -1212,22 +1214,38 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
         k_fill_u32<<<stride_grid(nb), 256, 0, s>>>(cload, 0u, 0, &cnt[K_NB]);
         HIP_TRY(hipGetLastError());
     }
-    // Sub-block states (kernel.rs:467-482): one shared gather per brick, one
-    // wavefront per brick for bricks whose lists overflow it.
+    // Sub-block states (kernel.rs:467-482): one shared gather per brick in
+    // the LDS tiers in turn, one wavefront per brick for bricks whose lists
+    // overflow them all.
     TRY(rec(8));
     {
         const size_t lds_max = d->prop.sharedMemPerBlock;
-        RLayout lay = make_rlayout(UNIFORM_CAPS);
-        Queue rest{nullptr, &cnt[K_NB], &cnt[K_Q_SUBST]};
-        if (tune.uniform_shared && lay.bytes + STATIC_LDS <= lds_max) {
+        const uint32_t* list = nullptr;  // the first pass takes every surface brick
+        const uint32_t* count = &cnt[K_NB];
+        bool first = true;
+        for (int u = 0; u < UNIFORM_TIERS && tune.uniform_shared; u++) {
+            RLayout lay = make_rlayout(UNIFORM_CAPS[u]);
+            if (lay.bytes + STATIC_LDS > lds_max) continue;
             uint32_t* over;
             TRY(X.temp(&over, nb));
-            Queue q{nullptr, &cnt[K_NB], nullptr};
-            k_uniform<<<g1(nb), 256, lay.bytes, s>>>(mols, bmol, M->keys, UNIFORM_CAPS, lay, q, st, over,
-                                                     &cnt[K_UOVER]);
-            rest = Queue{over, &cnt[K_UOVER], &cnt[K_Q_SUBST]};
+            Queue q{list, count, &cnt[K_Q_UNIFORM]};
+            uint32_t items = nb;
+            if (!first && !X.planned()) TRY(X.size(count, 0, items));
+            if (first || !X.planned())
+                k_uniform<false><<<g1(items), 256, lay.bytes, s>>>(mols, bmol, M->keys, UNIFORM_CAPS[u], lay, q, st,
+                                                                   over, &cnt[K_UOVER + u]);
+            else
+                k_uniform<true><<<g1(std::min<uint64_t>(nb, resident(d, (const void*)k_uniform<true>, 256, lay.bytes))),
+                                  256, lay.bytes, s>>>(mols, bmol, M->keys, UNIFORM_CAPS[u], lay, q, st, over,
+                                                       &cnt[K_UOVER + u]);
+            HIP_TRY(hipGetLastError());
+            if (first) TRY(rec(9));
+            first = false;
+            list = over;
+            count = &cnt[K_UOVER + u];
         }
-        TRY(rec(9));
+        if (first) TRY(rec(9));
+        Queue rest{list, count, &cnt[K_Q_SUBST]};
         uint32_t grid = g1(std::min<uint64_t>(nb, resident(d, (const void*)k_sub_states, WAVE, 0)));
         k_sub_states<<<grid, WAVE, 0, s>>>(mols, bmol, M->keys, rest, st);
         HIP_TRY(hipGetLastError());
