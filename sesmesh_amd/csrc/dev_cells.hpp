@@ -23,14 +23,24 @@ __device__ inline uint32_t cell_case(const float* val, uint32_t side, uint32_t x
 }
 
 // Triangles per brick: thread t takes the contiguous cells [t*per, (t+1)*per)
-// in (z, y, x) order. One workgroup per brick, striding over *n_bricks.
+// in (z, y, x) order. One workgroup per brick, striding over *n_bricks;
+// with `staged`, the brick's samples are first copied to dynamic LDS
+// ((b+1)^3 floats), each being read by up to eight cells.
 __global__ void __launch_bounds__(256) k_cell_count(uint32_t b, const float* values, const uint32_t* n_bricks,
-                                                    uint32_t* tcount) {
+                                                    uint32_t* tcount, uint32_t staged) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
     __shared__ uint32_t tmp[4];
     const uint32_t nb = *n_bricks;
     for (uint32_t brick = blockIdx.x; brick < nb; brick += gridDim.x) {
-        const uint32_t side = b + 1;
-        const float* val = values + (size_t)brick * side * side * side;
+        const uint32_t side = b + 1, spb = side * side * side;
+        const float* gval = values + (size_t)brick * spb;
+        const float* val = gval;
+        if (staged) {
+            float* lv = (float*)smem;
+            for (uint32_t i = threadIdx.x; i < spb; i += blockDim.x) lv[i] = gval[i];
+            __syncthreads();
+            val = lv;
+        }
         uint32_t ncell = b * b * b, per = (ncell + blockDim.x - 1) / blockDim.x;
         uint32_t mine = 0;
         for (uint32_t c = threadIdx.x * per; c < min(ncell, (threadIdx.x + 1) * per); c++) {
@@ -57,12 +67,17 @@ __device__ inline int64_t dense_index(const DRun& P, int64_t kx, int64_t ky, int
 // are found in the molecule's dense index grid. A reference that does not
 // resolve sets *err; triangles at or beyond `tcap` are not written (the
 // run's triangle count exceeds the buffer and is reported by its caller).
+// With `staged`, the brick's samples, edge masks and per-word prefix counts
+// are first copied to dynamic LDS ((b+1)^3 floats, then `words` u64 and
+// u32): most triangle corners reference the brick's own edges.
 __global__ void __launch_bounds__(256) k_cell_emit(const Mol* __restrict__ mols, const uint32_t* bmol,
                                                    const uint32_t* fbrick, const float* values, const int32_t* keys,
                                                    const uint32_t* n_bricks, const uint32_t* tbase,
                                                    const uint32_t* dense, const uint32_t* vbase,
                                                    const unsigned long long* mask, const uint32_t* wpre,
-                                                   uint32_t words, uint32_t* tris, size_t tcap, uint32_t* err) {
+                                                   uint32_t words, uint32_t* tris, size_t tcap, uint32_t* err,
+                                                   uint32_t staged) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
     __shared__ uint32_t tmp[4];
     const uint32_t nb = *n_bricks;
     for (uint32_t brick = blockIdx.x; brick < nb; brick += gridDim.x) {
@@ -71,6 +86,22 @@ __global__ void __launch_bounds__(256) k_cell_emit(const Mol* __restrict__ mols,
         const uint32_t* mdense = dense + mols[m].dense_off;
         const uint32_t vfirst = vbase[fbrick[m]];
         const float* val = values + (size_t)brick * P.spb;
+        const unsigned long long* lmask = mask + (size_t)brick * words;
+        const uint32_t* lpre = wpre + (size_t)brick * words;
+        if (staged) {
+            float* sv = (float*)smem;
+            unsigned long long* sm = (unsigned long long*)(smem + ((4 * (size_t)P.spb + 7) & ~(size_t)7));
+            uint32_t* sp = (uint32_t*)(sm + words);
+            for (uint32_t i = threadIdx.x; i < P.spb; i += blockDim.x) sv[i] = val[i];
+            for (uint32_t w = threadIdx.x; w < words; w += blockDim.x) {
+                sm[w] = lmask[w];
+                sp[w] = lpre[w];
+            }
+            __syncthreads();
+            val = sv;
+            lmask = sm;
+            lpre = sp;
+        }
         uint32_t b = P.b, ncell = b * b * b, per = (ncell + blockDim.x - 1) / blockDim.x;
         uint32_t c_lo = threadIdx.x * per, c_hi = min(ncell, (threadIdx.x + 1) * per);
         uint32_t mine = 0;
@@ -107,7 +138,12 @@ __global__ void __launch_bounds__(256) k_cell_emit(const Mol* __restrict__ mols,
                     }
                     int64_t tb = target[code];
                     uint32_t vi = 0;
-                    if (tb < 0 || !edge_is_crossing(mask, words, (uint32_t)tb, id)) {
+                    if (code == 0) {  // the brick's own edge (masks staged in LDS)
+                        if (!edge_is_crossing(lmask, words, 0u, id))
+                            atomicOr(err, 1u);
+                        else
+                            vi = vbase[brick] + edge_rank(lmask, lpre, words, 0u, id) - vfirst;
+                    } else if (tb < 0 || !edge_is_crossing(mask, words, (uint32_t)tb, id)) {
                         atomicOr(err, 1u);
                     } else {
                         vi = vbase[tb] + edge_rank(mask, wpre, words, (uint32_t)tb, id) - vfirst;

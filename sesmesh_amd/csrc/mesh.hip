@@ -1512,15 +1512,20 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     uint32_t *tcount, *tbase;
     TRY(X.temp(&tcount, nb));
     TRY(X.temp(&tbase, (size_t)nb + 1));
-    k_cell_count<<<stride_grid(nb, 1), 256, 0, s>>>(b, M->values, &cnt[K_NB], tcount);
+    // The cell kernels stage a brick's samples (and its edge masks) in LDS
+    // when they fit.
+    const uint32_t cell_lds = ((4 * spb + 7) & ~7u) + 12 * words;
+    const uint32_t staged = cell_lds + 1024 <= d->prop.sharedMemPerBlock ? 1u : 0u;
+    k_cell_count<<<stride_grid(nb, 1), 256, staged ? 4 * spb : 0, s>>>(b, M->values, &cnt[K_NB], tcount, staged);
     TRY(scan_u32(X, tcount, tbase, &cnt[K_NB], nb, &cnt[K_NT]));
     k_clamp<<<1, 64, 0, s>>>(&cnt[K_NT], E.nt, &cnt[K_FLAGS], OVER_TRIANGLES, nullptr, 0u);
     HIP_TRY(hipGetLastError());
     uint32_t ntris;
     TRY(X.size(&cnt[K_NT], E.nt, ntris));
     TRY(X.output(&M->tris, 3 * (size_t)ntris));
-    k_cell_emit<<<stride_grid(nb, 1), 256, 0, s>>>(mols, bmol, fbrick, M->values, M->keys, &cnt[K_NB], tbase, dense,
-                                                   vbase, mask, wpre, words, M->tris, ntris, &cnt[K_ERR]);
+    k_cell_emit<<<stride_grid(nb, 1), 256, staged ? cell_lds : 0, s>>>(mols, bmol, fbrick, M->values, M->keys,
+                                                                        &cnt[K_NB], tbase, dense, vbase, mask, wpre,
+                                                                        words, M->tris, ntris, &cnt[K_ERR], staged);
     HIP_TRY(hipGetLastError());
     TRY(rec(5));
     k_mol_summary<<<stride_grid(rows + 1), 256, 0, s>>>(rows, fbrick, &cnt[K_NB], soff, &cnt[K_NSW], vbase,
