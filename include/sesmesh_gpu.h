@@ -216,8 +216,8 @@ int smg_mesh_part_copy(const smg_mesh* m, size_t part, float* positions, float* 
  * lanes per edge. No counterpart in the reference, whose launch shapes are
  * fixed (crates/sesmesh-gpu/src/hip/resident.rs:675-683). */
 typedef struct {
-    int32_t sample_first_tier; /* 0..2: LDS capacity tier the sampling passes start at */
-    int32_t refine_first_tier; /* 0..2: the same for refinement */
+    int32_t sample_first_tier; /* 0..3: LDS capacity tier the sampling passes start at */
+    int32_t refine_first_tier; /* 0..3: the same for refinement */
     int32_t refine_group_max;  /* power of two <= 64: most lanes refining one edge */
     int32_t uniform_shared;    /* 1: sub-block states from one shared gather per brick; 0: a wavefront per brick */
     int32_t scratch_workgroups; /* workgroups of the global-scratch passes, 16..8192 (0: 1024); ~0.9 MB each */
@@ -284,7 +284,7 @@ int smg_model_gradients(smg_model* m, const float* features, const uint8_t* occu
 /* Adam (beta 0.9, 0.999) over shuffled batches of the training set. */
 int smg_model_train(smg_model* m, uint32_t epochs, uint32_t batch, float lr, uint64_t seed, smg_train_stats* stats);
 /* Placement: a sub-block goes first to the smallest tier holding scale x
- * its predicted load, up to tier max_tier (0..3). Default scale 0.3, tier 1:
+ * its predicted load, up to tier max_tier (0..4). Default scale 0.3, tier 1:
  * larger tiers run at lower occupancy, so only sub-blocks predicted well
  * past the first tier skip it. */
 int smg_model_set_policy(smg_model* m, float scale, int32_t max_tier);

@@ -93,10 +93,11 @@ __global__ void __launch_bounds__(64) k_features(const Mol* __restrict__ mols, c
 
 // Where a pass's sub-blocks go: the lists of the passes that run, the pass
 // that runs each capacity tier, and each tier's capacity in load units.
+constexpr int SPLIT_TIERS = 5;
 struct TierSplit {
-    uint32_t* list[4];
-    uint32_t pass[4];
-    float cap[4];
+    uint32_t* list[SPLIT_TIERS];
+    uint32_t pass[SPLIT_TIERS];
+    float cap[SPLIT_TIERS];
     float scale;        // predicted load is scaled by this before the tier choice
     uint32_t max_tier;  // highest tier a prediction may choose
 };
@@ -114,7 +115,7 @@ __global__ void k_split_tiers(const uint32_t* work, const uint32_t* n, const flo
     for (uint32_t i0 = blockIdx.x * blockDim.x + (threadIdx.x & ~(uint32_t)(WAVE - 1)); i0 < cnt;
          i0 += gridDim.x * blockDim.x) {
         const uint32_t i = i0 + lane;
-        uint32_t j = 0, p = 4;
+        uint32_t j = 0, p = SPLIT_TIERS;
         if (i < cnt) {
             j = work[i];
             float load = expm1f(fmaxf(pred[2 * (size_t)(j >> 3) + 1], 0.0f)) * T.scale;
@@ -122,7 +123,7 @@ __global__ void k_split_tiers(const uint32_t* work, const uint32_t* n, const flo
             while (t < T.max_tier && load > T.cap[t]) t++;
             p = T.pass[t];
         }
-        for (uint32_t q = 0; q < 4; q++) {
+        for (uint32_t q = 0; q < (uint32_t)SPLIT_TIERS; q++) {
             unsigned long long mask = __ballot(p == q);
             if (!mask) continue;
             uint32_t leader = (uint32_t)__ffsll((long long)mask) - 1u, base = 0;

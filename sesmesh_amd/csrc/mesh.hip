@@ -770,18 +770,24 @@ int read_u32(smg_device* d, const uint32_t* dev_ptr, uint32_t& v) {
 // (several workgroups per CU) and for those of coarse lattices (up to the
 // 160 KB of LDS one gfx950 workgroup may hold), then global scratch for the
 // rest. Each pass takes the previous pass's overflow list.
-constexpr int TIERS = 4;  // the last one in global scratch
+constexpr int TIERS = 5;  // the last one in global scratch
 // The first tiers hold 192 candidates: 6 sampling and 5 refinement
 // workgroups fit a CU's 160 KB of LDS (256 held 5 and 4), and 0.25 A
-// sub-blocks need at most ~170.
+// sub-blocks need at most ~170. The second holds what most 0.5 A
+// sub-blocks need (near spheres to ~190, candidates to ~250) at 4 and 3
+// workgroups per CU.
 constexpr Caps SAMPLE_CAPS[TIERS] = {{128, 64, 192, 64, 256},
+                                     {192, 96, 256, 64, 512},
                                      {256, 128, 384, 128, 512},
                                      {640, 256, 768, 256, 2048},
                                      {1024, 512, 4096, 2048, 4096}};
 constexpr RCaps REFINE_CAPS[TIERS] = {{128, 64, 192, 256, 0, 0, 0},
+                                      {192, 96, 256, 512, 0, 0, 0},
                                       {256, 128, 384, 512, 0, 0, 0},
                                       {640, 256, 704, 1024, 0, 0, 0},
                                       {1024, 512, 4096, 4096, 0, 0, 0}};
+constexpr int WIDE_FROM = 2;  // refinement passes over tiers from this one run WIDE threads
+static_assert(TIERS == SPLIT_TIERS, "the learned-sizing split covers every tier");
 // Sub-block classification: two LDS tiers (5 and 3 workgroups per CU);
 // bricks beyond them take one wavefront per brick.
 constexpr int UNIFORM_TIERS = 2;
@@ -804,8 +810,8 @@ enum : uint32_t {
 };
 // Capacity of each tier in candidate-load units (max(S + C, 2 near, 4 SAS
 // vertices, 4 inside) of the sampling lists; the refinement lists alike).
-constexpr float SAMPLE_LOAD_CAP[TIERS] = {192.0f, 384.0f, 768.0f, 2048.0f};
-constexpr float REFINE_LOAD_CAP[TIERS] = {192.0f, 384.0f, 704.0f, 2048.0f};
+constexpr float SAMPLE_LOAD_CAP[TIERS] = {192.0f, 256.0f, 384.0f, 768.0f, 2048.0f};
+constexpr float REFINE_LOAD_CAP[TIERS] = {192.0f, 256.0f, 384.0f, 704.0f, 2048.0f};
 // K_FLAGS bits: a device count exceeded its capacity.
 enum : uint32_t { OVER_BRICKS = 1, OVER_VERTICES = 2, OVER_TRIANGLES = 4, OVER_TABLE = 8 };
 
@@ -1448,7 +1454,7 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
             }
             // The large tiers (one or two workgroups per CU) run WIDE
             // threads: more wavefronts per SIMD, more edges in flight.
-            const bool wide = pass >= 1;
+            const bool wide = pass >= WIDE_FROM;
             if (pass < TIERS - 1 && (!X.planned() || pass == first)) {
                 if (fast && wide)
                     k_refine_fast_wide<false, false><<<g1(items), WIDE, bytes, s>>>(

@@ -81,7 +81,7 @@ def test_autotuner_over_brick_sizes(tuned_device):
 
 def test_invalid_tunings_are_rejected(tuned_device):
     tuned_device.tuning = None
-    for bad in (dict(sample_first_tier=3), dict(refine_group_max=12), dict(refine_group_max=128),
+    for bad in (dict(sample_first_tier=4), dict(refine_group_max=12), dict(refine_group_max=128),
                 dict(uniform_shared=2), dict(refine_first_tier=-1), dict(scratch_workgroups=8),
                 dict(scratch_workgroups=9000)):
         with pytest.raises(sesmesh_amd.BackendError):
