@@ -325,6 +325,7 @@ __device__ inline uint32_t hash_get(const uint32_t* hk, const uint32_t* hv, uint
 }
 
 constexpr uint32_t KIND_SHIFT = 30, ID_MASK = (1u << KIND_SHIFT) - 1u;
+constexpr uint32_t KEY_BUCKETS = 16;  // distance buckets per class of the sampling candidates
 
 __host__ __device__ inline uint32_t pow2_at_least(uint32_t v) {
     uint32_t p = 1;

@@ -3,11 +3,13 @@
 // Restates Engine::assign / prepare_sub / gather_near / gather_ses and the
 // SES lane kernel (reference field/kernel.rs:489-751, 837-952). Membership
 // decisions are made in f64 exactly as the CPU makes them; the candidate
-// data stored for sampling are the CPU's frame-local f32 copies. The class S
-// and C candidates are sorted nearest-first, as the CPU sorts them, so most
-// candidates fail the distance test before their projection and blocker
-// tests; the sampled value does not depend on the order: it is the minimum
-// over the valid candidates, and "inside" is an any(). Blockers are bitsets
+// data stored for sampling are the CPU's frame-local f32 copies. The CPU
+// sorts the class S and C candidates nearest-first; the sampled value does
+// not depend on the order (it is the minimum over the valid candidates, and
+// "inside" is an any()), so here they are only bucketed: class S before C,
+// each in 16 distance buckets, nearest bucket first, in O(n). Near
+// candidates still come early, so most later ones fail the distance test
+// before their projection and blocker tests. Blockers are bitsets
 // over near-list slots. Lists live in LDS (or, for sub-blocks whose lists
 // overflow the LDS capacities, in a global scratch slice).
 #pragma once
@@ -24,7 +26,7 @@ struct Caps {
 
 // Byte layout of the lists for given capacities (host and device agree).
 struct Layout {
-    uint32_t nd, id, hk, hv, ckey, cid, cdat, cbits, coff, cpre, vd, bytes;
+    uint32_t nd, id, hk, hv, cbk, cid, cdat, cbits, coff, cpre, vd, bytes;
 };
 
 __host__ __device__ inline Layout make_layout(const Caps& c) {
@@ -39,7 +41,7 @@ __host__ __device__ inline Layout make_layout(const Caps& c) {
     L.id = take(16 * c.inside);  // x, y, z, r2
     L.hk = take(4 * c.hash);
     L.hv = take(4 * c.hash);
-    L.ckey = take(8 * c.cand);
+    L.cbk = take(4 * c.cand);
     L.cid = take(4 * c.cand);
     L.cdat = take(48 * c.cand);  // S: (x, y, z, r); C: (mx, my, mz, rho), (ux, uy, uz, -), (ex, ey, ez, -)
     L.cbits = take(c.near / 64 * 8 * c.cand);
@@ -98,7 +100,7 @@ struct Lists {
     float4* nd;
     float4* id;
     uint32_t *hk, *hv;
-    unsigned long long* ckey;
+    uint32_t* cbk;  // candidate's key bucket (class S: 0-15, class C: 16-31)
     uint32_t* cid;
     float4* cdat;
     unsigned long long* cbits;  // near/64 rows of `cand` words
@@ -112,7 +114,7 @@ __device__ inline Lists carve(char* base, const Layout& L) {
     s.id = (float4*)(base + L.id);
     s.hk = (uint32_t*)(base + L.hk);
     s.hv = (uint32_t*)(base + L.hv);
-    s.ckey = (unsigned long long*)(base + L.ckey);
+    s.cbk = (uint32_t*)(base + L.cbk);
     s.cid = (uint32_t*)(base + L.cid);
     s.cdat = (float4*)(base + L.cdat);
     s.cbits = (unsigned long long*)(base + L.cbits);
@@ -235,8 +237,10 @@ __device__ inline float ses_value(const Lists& s, const uint32_t* n, uint32_t cc
 __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, const Caps& cap, Lists& s,
                                   uint32_t* n, uint32_t* tmp4, const int32_t* key, uint32_t sb) {
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    __shared__ uint32_t bfill[2 * KEY_BUCKETS];
     for (uint32_t i = tid; i < cap.hash; i += nt) s.hk[i] = 0u;
     if (tid < C_N) n[tid] = 0u;
+    if (tid < 2 * KEY_BUCKETS) bfill[tid] = 0u;
     __syncthreads();
     double h = P.h, sub = (double)P.sub, half = sub * 0.5;
     double rho = 0.5 * sqrt(3.0) * sub * h;
@@ -249,6 +253,8 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
     }
     const uint32_t hmask = cap.hash - 1;
     const double radii[3] = {reach + A.max_expanded, reach + A.rho_max, reach};
+    const double per_bucket = KEY_BUCKETS / reach;
+    auto bucket = [&](double key) { return min(KEY_BUCKETS - 1u, (uint32_t)(key * per_bucket)); };
     gather3(A, b[0], b[1], b[2], radii, tmp4, [&](uint32_t grid, uint32_t k) {
         if (grid == 0) {  // near spheres, the inside group and class S (gather_near, gather_ses)
             if (A.status[k] == S_CONTAINED) return;
@@ -278,8 +284,10 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
                 if (q >= cap.cand) {
                     n[C_OVER] = 1u;
                 } else {
-                    s.ckey[q] = total_key(lb);
+                    const uint32_t bk = bucket(lb);
+                    s.cbk[q] = bk;
                     s.cid[q] = k;
+                    atomicAdd(&bfill[bk], 1u);
                 }
             }
         } else if (grid == 1) {  // class C (gather_ses, kernel.rs:701-736)
@@ -298,8 +306,10 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
                 n[C_OVER] = 1u;
                 return;
             }
-            s.ckey[q] = total_key(dist);
-            s.cid[q] = k | (1u << KIND_SHIFT);
+            const uint32_t bk = KEY_BUCKETS + bucket(dist);
+            s.cbk[q] = bk;
+            s.cid[q] = k;
+            atomicAdd(&bfill[bk], 1u);
         } else {  // class V (kernel.rs:738-747)
             const VertexRec V = load_rec(A.ver, k);
             if (!(ddist(b[0], b[1], b[2], V.x, V.y, V.z) <= reach)) return;
@@ -313,13 +323,25 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
     });
     if (n[C_OVER]) return false;
     const uint32_t total = n[C_CAND], ns = n[C_S];
-    sort_candidates(s.ckey, s.cid, total);
-    // Candidate records in sorted order, cleared bitsets and blocker rows:
-    // neighbour / circle-blocker rows restricted to near spheres
-    // (kernel.rs:685-735).
+    // Bucket starts: an exclusive scan of the counts by the first wavefront.
+    if (tid < WAVE) {
+        const uint32_t c = tid < 2 * KEY_BUCKETS ? bfill[tid] : 0u;
+        uint32_t inc = c;
+        for (uint32_t o = 1; o < 2 * KEY_BUCKETS; o <<= 1) {
+            uint32_t y = __shfl_up(inc, o, WAVE);
+            if (tid >= o) inc += y;
+        }
+        if (tid < 2 * KEY_BUCKETS) bfill[tid] = inc - c;
+    }
+    __syncthreads();
+    // Each candidate takes the next slot p of its bucket; there its record,
+    // cleared bitset and blocker row (neighbour / circle-blocker rows
+    // restricted to near spheres, kernel.rs:685-735) are stored. Class S
+    // fills [0, ns).
     const uint32_t cc = cap.cand, nw = cap.near / 64;
-    for (uint32_t p = tid; p < total; p += nt) {
-        uint32_t k = s.cid[p] & ID_MASK;
+    for (uint32_t q = tid; q < total; q += nt) {
+        const uint32_t p = atomicAdd(&bfill[s.cbk[q]], 1u);
+        const uint32_t k = s.cid[q];
         if (p < ns) {
             s.cdat[3 * p] = load_f4(Lc.p4, k);
             s.coff[p] = A.nb_off[k];
