@@ -325,98 +325,12 @@ __device__ inline uint32_t hash_get(const uint32_t* hk, const uint32_t* hv, uint
 }
 
 constexpr uint32_t KIND_SHIFT = 30, ID_MASK = (1u << KIND_SHIFT) - 1u;
-constexpr uint32_t KEY_BUCKETS = 16;  // distance buckets per class of the sampling candidates
-
-__host__ __device__ inline uint32_t pow2_at_least(uint32_t v) {
-    uint32_t p = 1;
-    while (p < v) p <<= 1;
-    return p;
-}
+constexpr uint32_t KEY_BUCKETS = 16;  // key buckets per candidate class in the gathers
 
 // Candidate order: (class, key, index), the class in the top bits of id.
 __device__ inline bool cand_less(unsigned long long ka, uint32_t ia, unsigned long long kb, uint32_t ib) {
     uint32_t ca = ia >> KIND_SHIFT, cb = ib >> KIND_SHIFT;
     return ca < cb || (ca == cb && (ka < kb || (ka == kb && ia < ib)));
-}
-
-// Bitonic network over a power of two, by the whole workgroup.
-__device__ inline void sort_bitonic(unsigned long long* key, uint32_t* id, uint32_t count) {
-    uint32_t n = pow2_at_least(count);
-    const uint32_t tid = threadIdx.x, nt = blockDim.x;
-    for (uint32_t i = count + tid; i < n; i += nt) {
-        key[i] = ~0ull;
-        id[i] = 0xffffffffu;
-    }
-    __syncthreads();
-    for (uint32_t k = 2; k <= n; k <<= 1)
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = tid; i < n; i += nt) {
-                uint32_t l = i ^ j;
-                if (l > i) {
-                    unsigned long long ki = key[i], kl = key[l];
-                    uint32_t ii = id[i], il = id[l];
-                    uint32_t ci = ii >> KIND_SHIFT, cl = il >> KIND_SHIFT;
-                    bool gt = ci > cl || (ci == cl && (ki > kl || (ki == kl && ii > il)));
-                    if (gt == ((i & k) == 0)) {
-                        key[i] = kl;
-                        key[l] = ki;
-                        id[i] = il;
-                        id[l] = ii;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-}
-
-// Sorts `count` distinct candidates by cand_less, in place, by the whole
-// workgroup. Each element's rank (the number of elements before it) is
-// counted by a group of g lanes, each over a 1/g share of the list, and
-// summed across the group; then every element moves to its rank. Up to 4
-// elements per thread are ranked by their thread alone; longer lists take
-// the bitonic network (capacity a power of two).
-__device__ inline void sort_candidates(unsigned long long* key, uint32_t* id, uint32_t count) {
-    const uint32_t tid = threadIdx.x, nt = blockDim.x;
-    if (count > 4 * nt) {
-        sort_bitonic(key, id, count);
-        return;
-    }
-    if (count > nt) {
-        unsigned long long mk[4];
-        uint32_t mi[4], rank[4] = {0, 0, 0, 0};
-        for (int j = 0; j < 4; j++) {
-            uint32_t e = tid + j * nt;
-            mk[j] = e < count ? key[e] : ~0ull;
-            mi[j] = e < count ? id[e] : 0xffffffffu;
-        }
-        for (uint32_t o = 0; o < count; o++) {
-            unsigned long long k = key[o];
-            uint32_t i = id[o];
-            for (int j = 0; j < 4; j++) rank[j] += cand_less(k, i, mk[j], mi[j]) ? 1u : 0u;
-        }
-        __syncthreads();
-        for (int j = 0; j < 4; j++)
-            if (tid + j * nt < count) {
-                key[rank[j]] = mk[j];
-                id[rank[j]] = mi[j];
-            }
-        __syncthreads();
-        return;
-    }
-    uint32_t g = 1;  // lanes per element: the most with g * count <= nt, at most 16
-    while (g < 16 && 2 * g * count <= nt) g <<= 1;
-    const uint32_t e = tid / g, r = tid % g;
-    unsigned long long mk = e < count ? key[e] : ~0ull;
-    uint32_t mi = e < count ? id[e] : 0xffffffffu, rank = 0;
-    if (e < count)
-        for (uint32_t o = r; o < count; o += g) rank += cand_less(key[o], id[o], mk, mi) ? 1u : 0u;
-    for (uint32_t o = 1; o < g; o <<= 1) rank += __shfl_xor(rank, (int)o, WAVE);
-    __syncthreads();
-    if (e < count && r == 0) {
-        key[rank] = mk;
-        id[rank] = mi;
-    }
-    __syncthreads();
 }
 
 constexpr uint32_t ROWS_MAX = 256;

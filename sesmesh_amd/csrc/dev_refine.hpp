@@ -187,12 +187,18 @@ enum { R_NEAR = 0, R_IN, R_S, R_C, R_V, R_CAND, R_E, R_OVER, R_N };
 __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s, uint32_t* n, uint32_t* tmp4,
                                      double bx, double by, double bz, double rho, double band) {
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    __shared__ uint32_t bstart[WAVE], bfill[WAVE];  // per key bucket: 16 for each of S, C, V
     for (uint32_t i = tid; i < cap.hash; i += nt) s.hk[i] = 0u;
     if (tid < R_N) n[tid] = 0u;
+    if (tid < WAVE) bstart[tid] = 0u;
     __syncthreads();
     const uint32_t hmask = cap.hash - 1;
     const double reach = A.probe + band + rho;
     const double radii[3] = {reach + A.max_expanded, reach + A.rho_max, reach};
+    const double per_bucket = KEY_BUCKETS / reach;
+    auto bucket = [&](uint32_t cls, double key) {
+        return cls * KEY_BUCKETS + min(KEY_BUCKETS - 1u, (uint32_t)(key * per_bucket));
+    };
     gather3(A, bx, by, bz, radii, tmp4, [&](uint32_t grid, uint32_t k) {
         if (grid == 0) {
             if (A.status[k] == S_CONTAINED) return;
@@ -234,6 +240,7 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
                 } else {
                     s.ckey[q] = total_key(lb);
                     s.cid[q] = k;
+                    atomicAdd(&bstart[bucket(0u, lb)], 1u);
                 }
             }
         } else if (grid == 1) {
@@ -254,6 +261,7 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
             }
             s.ckey[q] = total_key(dist);
             s.cid[q] = k | (1u << KIND_SHIFT);
+            atomicAdd(&bstart[bucket(1u, dist)], 1u);
         } else {
             const VertexRec V = load_rec(A.ver, k);
             double d = ddist(bx, by, bz, V.x, V.y, V.z);
@@ -266,11 +274,50 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
             }
             s.ckey[q] = total_key(d);
             s.cid[q] = k | (2u << KIND_SHIFT);
+            atomicAdd(&bstart[bucket(2u, d)], 1u);
         }
     });
     if (n[R_OVER]) return false;
     const uint32_t total = n[R_CAND], ns = n[R_S], nc = n[R_C];
-    sort_candidates(s.ckey, s.cid, total);
+    // The CPU's list order, (class, key, index) by cand_less, sorted exactly
+    // through the buckets: a candidate's bucket is monotone in (class, key),
+    // so the candidates are first grouped by bucket (in any order within
+    // one, into the free record area), then each is ranked against its own
+    // bucket only.
+    if (tid < WAVE) {
+        const uint32_t c = bstart[tid];
+        uint32_t inc = c;
+        for (uint32_t o = 1; o < WAVE; o <<= 1) {
+            uint32_t y = __shfl_up(inc, o, WAVE);
+            if (tid >= o) inc += y;
+        }
+        bstart[tid] = inc - c;
+        bfill[tid] = inc - c;
+    }
+    __syncthreads();
+    unsigned long long* tkey = (unsigned long long*)s.cdat;
+    uint32_t* tcid = (uint32_t*)(tkey + cap.cand);
+    auto bucket_of = [&](unsigned long long key, uint32_t id) {
+        return bucket(id >> KIND_SHIFT, key_value(key));
+    };
+    for (uint32_t q = tid; q < total; q += nt) {
+        const unsigned long long key = s.ckey[q];
+        const uint32_t id = s.cid[q];
+        const uint32_t t = atomicAdd(&bfill[bucket_of(key, id)], 1u);
+        tkey[t] = key;
+        tcid[t] = id;
+    }
+    __syncthreads();
+    for (uint32_t t = tid; t < total; t += nt) {
+        const unsigned long long key = tkey[t];
+        const uint32_t id = tcid[t], b = bucket_of(key, id);
+        uint32_t rank = bstart[b];
+        const uint32_t end = bstart[b + 1];
+        for (uint32_t u = bstart[b]; u < end; u++) rank += cand_less(tkey[u], tcid[u], key, id) ? 1u : 0u;
+        s.ckey[rank] = key;
+        s.cid[rank] = id;
+    }
+    __syncthreads();
     // Candidate data, cleared bitsets and blocker row lengths (in cpre).
     const uint32_t cc = cap.cand, nw = cap.near / 64;
     for (uint32_t p = tid; p < total; p += nt) {
