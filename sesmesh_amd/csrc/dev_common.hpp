@@ -327,6 +327,31 @@ __device__ inline uint32_t hash_get(const uint32_t* hk, const uint32_t* hv, uint
 constexpr uint32_t KIND_SHIFT = 30, ID_MASK = (1u << KIND_SHIFT) - 1u;
 constexpr uint32_t KEY_BUCKETS = 16;  // key buckets per candidate class in the gathers
 
+// Blocker bitsets of candidate rows p < rows: row p lists sphere indices
+// (nb_items for p < ns, else cb_items, from off[p], len[p] of them); bit
+// `slot` of row p is set for each listed sphere the near hash holds at
+// `slot`. Row p's bits are words bits[w * cc + p]. 8 lanes per row, each
+// loading 4 of its items before it probes any, so 4 loads are in flight per
+// lane. By the whole workgroup; the caller synchronises afterwards.
+__device__ inline void blocker_bits(gptr<const uint32_t> nb_items, gptr<const uint32_t> cb_items, uint32_t ns,
+                                    uint32_t rows, const uint32_t* off, const uint32_t* len, const uint32_t* hk,
+                                    const uint32_t* hv, uint32_t hmask, unsigned long long* bits, uint32_t cc) {
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    for (uint32_t p = tid / 8; p < rows; p += nt / 8) {
+        gptr<const uint32_t> row = (p < ns ? nb_items : cb_items) + off[p];
+        const uint32_t n = len[p];
+        for (uint32_t q0 = tid % 8; q0 < n; q0 += 32) {
+            uint32_t it[4];
+            for (int u = 0; u < 4; u++) it[u] = q0 + 8 * u < n ? row[q0 + 8 * u] : 0xffffffffu;
+            for (int u = 0; u < 4; u++) {
+                if (it[u] == 0xffffffffu) continue;
+                const uint32_t slot = hash_get(hk, hv, hmask, it[u]);
+                if (slot != 0xffffffffu) atomicOr(&bits[(slot >> 6) * cc + p], 1ull << (slot & 63));
+            }
+        }
+    }
+}
+
 // Candidate order: (class, key, index), the class in the top bits of id.
 __device__ inline bool cand_less(unsigned long long ka, uint32_t ia, unsigned long long kb, uint32_t ib) {
     uint32_t ca = ia >> KIND_SHIFT, cb = ib >> KIND_SHIFT;

@@ -351,15 +351,8 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
         for (uint32_t w = 0; w < nw; w++) s.cbits[w * cc + p] = 0ull;
     }
     __syncthreads();
-    // Blocker bitsets: 8 lanes per S or C candidate row.
-    for (uint32_t p = tid / 8; p < ns + nc; p += nt / 8) {
-        gptr<const uint32_t> row = (p < ns ? A.nb_items : A.cb_items) + s.coff[p];
-        const uint32_t len = s.cpre[p];
-        for (uint32_t q = tid % 8; q < len; q += 8) {
-            uint32_t slot = hash_get(s.hk, s.hv, hmask, row[q]);
-            if (slot != 0xffffffffu) atomicOr(&s.cbits[(slot >> 6) * cc + p], 1ull << (slot & 63));
-        }
-    }
+    // Blocker bitsets of the S and C rows (cpre holds the row length).
+    blocker_bits(A.nb_items, A.cb_items, ns, ns + nc, s.coff, s.cpre, s.hk, s.hv, hmask, s.cbits, cc);
     __syncthreads();
     return true;
 }
