@@ -352,6 +352,119 @@ __device__ inline void blocker_bits(gptr<const uint32_t> nb_items, gptr<const ui
     }
 }
 
+// Candidates hidden everywhere. A class S or C candidate is tested, at a
+// point x of a sub-block, by projecting x onto it and checking whether the
+// projection lies inside one of its blockers; if it does the candidate
+// cannot give the value. When one blocker contains, with a margin, every
+// projection of every point of the sub-block's ball, the candidate can
+// never give the value there, and dropping it from the lists changes no
+// result: the value is a minimum over the unblocked candidates (sampling),
+// and the refinement's least unblocked candidate, ties broken by list order,
+// is the same once blocked candidates leave the sorted list.
+//
+// The projections of the ball (b, rb) onto sphere (c, R) fill the cap of
+// directions within alpha of b - c, sin(alpha) = rb / |b - c| (when the ball
+// holds c, every direction). The farthest cap point from blocker centre ck
+// is at angle min(theta + alpha, pi) from ck - c (theta the angle between
+// b - c and ck - c), at squared distance R^2 + d^2 - 2 R d cos(that angle).
+// For a circle (centre m, unit axis u, radius r) the projections fill the
+// arc within beta of the in-plane direction of b - m, sin(beta) = rb / |w|
+// (the whole circle when |w| <= rb, which also holds the degenerate axis
+// case); with L the in-plane distance of ck from the axis the farthest arc
+// point is at r^2 + |ck - m|^2 - 2 r L cos(min(delta + beta, pi)).
+//
+// COVER_PAD widens the ball for the f32 rounding of the sample points, and
+// COVER_MARGIN (squared distance below (Rk - COVER_MARGIN)^2) exceeds the
+// rounding of the f32 sampling arithmetic at coordinates up to 2000 (about
+// 1e-3) by a factor of four, and the f64 refinement's and the blocker
+// tolerances (2e-5, 1e-9) by far more, so every such projection tests
+// blocked in either precision.
+constexpr double COVER_PAD = 1e-3, COVER_MARGIN = 4e-3;
+
+__device__ inline double clamp_unit(double v) { return fmin(1.0, fmax(-1.0, v)); }
+
+// cos(min(t + a, pi)) from cos t (t in [0, pi]) and sin a, cos a (a in [0, pi/2]).
+__device__ inline double cos_widened(double ct, double sa, double ca) {
+    if (ct < -ca) return -1.0;  // t + a >= pi
+    return ct * ca - sqrt(fmax(0.0, 1.0 - ct * ct)) * sa;
+}
+
+__device__ inline bool cap_covered(const double b[3], double rb, const dvec4& c, const dvec4& k) {
+    const double lim = k.w - COVER_MARGIN;
+    if (!(lim > 0.0)) return false;
+    const double vx = b[0] - c.x, vy = b[1] - c.y, vz = b[2] - c.z;
+    const double D2 = vx * vx + vy * vy + vz * vz;
+    if (!(D2 > rb * rb)) return false;
+    const double ex = k.x - c.x, ey = k.y - c.y, ez = k.z - c.z;
+    const double d2 = ex * ex + ey * ey + ez * ez, d = sqrt(d2), D = sqrt(D2);
+    const double sa = rb / D, ca = sqrt(fmax(0.0, 1.0 - sa * sa));
+    const double ct = d > 0.0 ? clamp_unit((vx * ex + vy * ey + vz * ez) / (D * d)) : 1.0;
+    const double far2 = c.w * c.w + d2 - 2.0 * c.w * d * cos_widened(ct, sa, ca);
+    return far2 < lim * lim;
+}
+
+// Circle record: a = (mx, my, mz, ax), e = (ay, az, rho, -).
+__device__ inline bool arc_covered(const double b[3], double rb, const dvec4& a, const dvec4& e, const dvec4& k) {
+    const double lim = k.w - COVER_MARGIN;
+    if (!(lim > 0.0)) return false;
+    const double ux = a.w, uy = e.x, uz = e.y, r = e.z;
+    const double yx = b[0] - a.x, yy = b[1] - a.y, yz = b[2] - a.z;
+    const double al = yx * ux + yy * uy + yz * uz;
+    const double wx = yx - ux * al, wy = yy - uy * al, wz = yz - uz * al;
+    const double W2 = wx * wx + wy * wy + wz * wz;
+    const double fx = k.x - a.x, fy = k.y - a.y, fz = k.z - a.z;
+    const double fa = fx * ux + fy * uy + fz * uz;
+    const double px = fx - ux * fa, py = fy - uy * fa, pz = fz - uz * fa;
+    const double L = sqrt(px * px + py * py + pz * pz);
+    double cphi = -1.0;
+    if (W2 > rb * rb) {
+        const double W = sqrt(W2), sb = rb / W, cb = sqrt(fmax(0.0, 1.0 - sb * sb));
+        const double cd = L > 0.0 ? clamp_unit((wx * px + wy * py + wz * pz) / (W * L)) : 1.0;
+        cphi = cos_widened(cd, sb, cb);
+    }
+    const double far2 = r * r + (fx * fx + fy * fy + fz * fz) - 2.0 * r * L * cphi;
+    return far2 < lim * lim;
+}
+
+// Marks the candidates q < total hidden everywhere in the ball (b, rb):
+// kind(q) is 0 for class S (sphere id(q)), 1 for class C (circle id(q)),
+// anything else for candidates without blockers; mark(q) is called once per
+// hidden candidate, by one lane. A blocker counts only if the near hash
+// holds it, as in the bitsets. 8 lanes per candidate row. By the whole
+// workgroup; the caller synchronises afterwards.
+template <class Kind, class Id, class Mark>
+__device__ inline void mark_hidden(const DArr& A, const double b[3], double rb, uint32_t total, const uint32_t* hk,
+                                   const uint32_t* hv, uint32_t hmask, Kind&& kind, Id&& id, Mark&& mark) {
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    for (uint32_t q = tid / 8; q < total; q += nt / 8) {
+        const uint32_t kd = kind(q);
+        bool covered = false;
+        if (kd <= 1u) {
+            const uint32_t k = id(q);
+            gptr<const uint32_t> off = kd == 0u ? A.nb_off : A.cb_off;
+            gptr<const uint32_t> items = kd == 0u ? A.nb_items : A.cb_items;
+            const uint32_t e = off[k + 1];
+            dvec4 g0, g1 = dvec4{0.0, 0.0, 0.0, 0.0};
+            if (kd == 0u) {
+                g0 = ((gptr<const dvec4>)A.sph)[k];
+            } else {
+                gptr<const dvec4> cr = (gptr<const dvec4>)A.cir + 2 * (size_t)k;
+                g0 = cr[0];
+                g1 = cr[1];
+            }
+            for (uint32_t t = off[k] + tid % 8; t < e && !covered; t += 8) {
+                const uint32_t it = items[t];
+                if (hash_get(hk, hv, hmask, it) == 0xffffffffu) continue;
+                const dvec4 ks = ((gptr<const dvec4>)A.sph)[it];
+                covered = kd == 0u ? cap_covered(b, rb, g0, ks) : arc_covered(b, rb, g0, g1, ks);
+            }
+        }
+        const unsigned long long bal = __ballot(covered);
+        const uint32_t lane = lane_id();
+        if ((lane & 7u) == 0u && ((bal >> lane) & 0xffull) != 0ull) mark(q);
+    }
+}
+
 // Candidate order: (class, key, index), the class in the top bits of id.
 __device__ inline bool cand_less(unsigned long long ka, uint32_t ia, unsigned long long kb, uint32_t ib) {
     uint32_t ca = ia >> KIND_SHIFT, cb = ib >> KIND_SHIFT;

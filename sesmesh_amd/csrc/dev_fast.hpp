@@ -411,7 +411,7 @@ __device__ __forceinline__ void refine_sub_fast(const Mol* __restrict__ mols, co
         double basep = (double)org[a] * h;
         c[a] = basep + ((double)(o3[a] * sub) + 0.5 * subd) * h;
     }
-    bool ok = gather_refine(A, cap, s, n, tmp4, c[0], c[1], c[2], rho, P.band);
+    bool ok = gather_refine(A, cap, s, n, tmp4, c[0], c[1], c[2], rho, P.band, true);
     uint32_t lo[3], hi[3];
     for (int a = 0; a < 3; a++) {
         lo[a] = o3[a] * sub;

@@ -184,8 +184,10 @@ enum { R_NEAR = 0, R_IN, R_S, R_C, R_V, R_CAND, R_E, R_OVER, R_N };
 // near-list slots, the S / C / V candidates in the CPU's list order with
 // their f64 data and blocker bitsets (row members that are near,
 // point.rs:152-155). False (in every thread) if a capacity was exceeded.
+// With `hide`, candidates that one blocker hides from the whole ball are
+// left out (mark_hidden): for callers that evaluate many points of the ball.
 __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s, uint32_t* n, uint32_t* tmp4,
-                                     double bx, double by, double bz, double rho, double band) {
+                                     double bx, double by, double bz, double rho, double band, bool hide) {
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     __shared__ uint32_t bstart[WAVE], bfill[WAVE];  // per key bucket: 16 for each of S, C, V
     for (uint32_t i = tid; i < cap.hash; i += nt) s.hk[i] = 0u;
@@ -240,7 +242,6 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
                 } else {
                     s.ckey[q] = total_key(lb);
                     s.cid[q] = k;
-                    atomicAdd(&bstart[bucket(0u, lb)], 1u);
                 }
             }
         } else if (grid == 1) {
@@ -261,7 +262,6 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
             }
             s.ckey[q] = total_key(dist);
             s.cid[q] = k | (1u << KIND_SHIFT);
-            atomicAdd(&bstart[bucket(1u, dist)], 1u);
         } else {
             const VertexRec V = load_rec(A.ver, k);
             double d = ddist(bx, by, bz, V.x, V.y, V.z);
@@ -274,11 +274,26 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
             }
             s.ckey[q] = total_key(d);
             s.cid[q] = k | (2u << KIND_SHIFT);
-            atomicAdd(&bstart[bucket(2u, d)], 1u);
         }
     });
     if (n[R_OVER]) return false;
-    const uint32_t total = n[R_CAND], ns = n[R_S], nc = n[R_C];
+    const uint32_t gathered = n[R_CAND];
+    // Candidates one blocker hides from the whole ball leave the lists (their
+    // key becomes all ones); the list order of the others is unchanged.
+    constexpr unsigned long long HIDDEN = ~0ull;
+    if (hide) {
+        const double bc[3] = {bx, by, bz};
+        mark_hidden(
+            A, bc, rho + COVER_PAD, gathered, s.hk, s.hv, hmask, [&](uint32_t q) { return s.cid[q] >> KIND_SHIFT; },
+            [&](uint32_t q) { return s.cid[q] & ID_MASK; }, [&](uint32_t q) { s.ckey[q] = HIDDEN; });
+    }
+    __syncthreads();
+    auto bucket_of = [&](unsigned long long key, uint32_t id) {
+        return bucket(id >> KIND_SHIFT, key_value(key));
+    };
+    for (uint32_t q = tid; q < gathered; q += nt)
+        if (s.ckey[q] != HIDDEN) atomicAdd(&bstart[bucket_of(s.ckey[q], s.cid[q])], 1u);
+    __syncthreads();
     // The CPU's list order, (class, key, index) by cand_less, sorted exactly
     // through the buckets: a candidate's bucket is monotone in (class, key),
     // so the candidates are first grouped by bucket (in any order within
@@ -295,13 +310,17 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
         bfill[tid] = inc - c;
     }
     __syncthreads();
+    const uint32_t ns = bstart[KEY_BUCKETS], nc = bstart[2 * KEY_BUCKETS] - ns, total = bstart[3 * KEY_BUCKETS];
+    if (tid == 0) {
+        n[R_S] = ns;
+        n[R_C] = nc;
+        n[R_CAND] = total;
+    }
     unsigned long long* tkey = (unsigned long long*)s.cdat;
     uint32_t* tcid = (uint32_t*)(tkey + cap.cand);
-    auto bucket_of = [&](unsigned long long key, uint32_t id) {
-        return bucket(id >> KIND_SHIFT, key_value(key));
-    };
-    for (uint32_t q = tid; q < total; q += nt) {
+    for (uint32_t q = tid; q < gathered; q += nt) {
         const unsigned long long key = s.ckey[q];
+        if (key == HIDDEN) continue;
         const uint32_t id = s.cid[q];
         const uint32_t t = atomicAdd(&bfill[bucket_of(key, id)], 1u);
         tkey[t] = key;
@@ -796,7 +815,7 @@ __device__ __forceinline__ void refine_sub(const Mol* __restrict__ mols, const u
         double basep = (double)org[a] * h;
         c[a] = basep + ((double)(o3[a] * sub) + 0.5 * subd) * h;
     }
-    bool ok = gather_refine(A, cap, s, n, tmp4, c[0], c[1], c[2], rho, P.band);
+    bool ok = gather_refine(A, cap, s, n, tmp4, c[0], c[1], c[2], rho, P.band, true);
     // Crossing edges whose lower sample lies in this sub-block, ascending.
     uint32_t lo[3], hi[3];
     for (int a = 0; a < 3; a++) {

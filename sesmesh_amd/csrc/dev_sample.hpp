@@ -237,10 +237,10 @@ __device__ inline float ses_value(const Lists& s, const uint32_t* n, uint32_t cc
 __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, const Caps& cap, Lists& s,
                                   uint32_t* n, uint32_t* tmp4, const int32_t* key, uint32_t sb) {
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
-    __shared__ uint32_t bfill[2 * KEY_BUCKETS];
+    __shared__ uint32_t bfill[WAVE];
     for (uint32_t i = tid; i < cap.hash; i += nt) s.hk[i] = 0u;
     if (tid < C_N) n[tid] = 0u;
-    if (tid < 2 * KEY_BUCKETS) bfill[tid] = 0u;
+    if (tid < WAVE) bfill[tid] = 0u;
     __syncthreads();
     double h = P.h, sub = (double)P.sub, half = sub * 0.5;
     double rho = 0.5 * sqrt(3.0) * sub * h;
@@ -284,10 +284,8 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
                 if (q >= cap.cand) {
                     n[C_OVER] = 1u;
                 } else {
-                    const uint32_t bk = bucket(lb);
-                    s.cbk[q] = bk;
+                    s.cbk[q] = bucket(lb);
                     s.cid[q] = k;
-                    atomicAdd(&bfill[bk], 1u);
                 }
             }
         } else if (grid == 1) {  // class C (gather_ses, kernel.rs:701-736)
@@ -306,10 +304,8 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
                 n[C_OVER] = 1u;
                 return;
             }
-            const uint32_t bk = KEY_BUCKETS + bucket(dist);
-            s.cbk[q] = bk;
+            s.cbk[q] = KEY_BUCKETS + bucket(dist);
             s.cid[q] = k;
-            atomicAdd(&bfill[bk], 1u);
         } else {  // class V (kernel.rs:738-747)
             const VertexRec V = load_rec(A.ver, k);
             if (!(ddist(b[0], b[1], b[2], V.x, V.y, V.z) <= reach)) return;
@@ -322,24 +318,42 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
         }
     });
     if (n[C_OVER]) return false;
-    const uint32_t total = n[C_CAND], ns = n[C_S];
-    // Bucket starts: an exclusive scan of the counts by the first wavefront.
+    const uint32_t gathered = n[C_CAND];
+    // Candidates one blocker hides from the whole sub-block leave the lists.
+    constexpr uint32_t HIDDEN = 0x80000000u;
+    mark_hidden(
+        A, b, rho + COVER_PAD, gathered, s.hk, s.hv, hmask, [&](uint32_t q) { return s.cbk[q] < KEY_BUCKETS ? 0u : 1u; },
+        [&](uint32_t q) { return s.cid[q]; }, [&](uint32_t q) { s.cbk[q] |= HIDDEN; });
+    __syncthreads();
+    for (uint32_t q = tid; q < gathered; q += nt)
+        if (s.cbk[q] < HIDDEN) atomicAdd(&bfill[s.cbk[q]], 1u);
+    __syncthreads();
+    // Bucket starts: an exclusive scan of the counts by the first wavefront;
+    // the kept class S candidates end at the start of the first class C
+    // bucket, all kept ones at the end of the last.
     if (tid < WAVE) {
-        const uint32_t c = tid < 2 * KEY_BUCKETS ? bfill[tid] : 0u;
+        const uint32_t c = bfill[tid];
         uint32_t inc = c;
-        for (uint32_t o = 1; o < 2 * KEY_BUCKETS; o <<= 1) {
+        for (uint32_t o = 1; o < WAVE; o <<= 1) {
             uint32_t y = __shfl_up(inc, o, WAVE);
             if (tid >= o) inc += y;
         }
-        if (tid < 2 * KEY_BUCKETS) bfill[tid] = inc - c;
+        bfill[tid] = inc - c;
     }
     __syncthreads();
-    // Each candidate takes the next slot p of its bucket; there its record,
-    // cleared bitset and blocker row (neighbour / circle-blocker rows
+    const uint32_t ns = bfill[KEY_BUCKETS], total = bfill[2 * KEY_BUCKETS];
+    __syncthreads();
+    if (tid == 0) {
+        n[C_S] = ns;
+        n[C_C] = total - ns;
+    }
+    // Each kept candidate takes the next slot p of its bucket; there its
+    // record, cleared bitset and blocker row (neighbour / circle-blocker rows
     // restricted to near spheres, kernel.rs:685-735) are stored. Class S
     // fills [0, ns).
     const uint32_t cc = cap.cand, nw = cap.near / 64;
-    for (uint32_t q = tid; q < total; q += nt) {
+    for (uint32_t q = tid; q < gathered; q += nt) {
+        if (s.cbk[q] >= HIDDEN) continue;
         const uint32_t p = atomicAdd(&bfill[s.cbk[q]], 1u);
         const uint32_t k = s.cid[q];
         if (p < ns) {
@@ -432,12 +446,14 @@ __device__ __forceinline__ void sample_sub(const Mol* __restrict__ mols, const u
 }
 
 // PERSIST: residency-sized grid looping over the queue; otherwise one
-// sub-block per workgroup.
+// sub-block per workgroup. At most 80 VGPRs (6 wavefronts per SIMD), so six
+// workgroups with the first tier's lists fit a CU: faster than five
+// unbounded ones despite a few spilled registers.
 template <bool GLOBAL, bool PERSIST>
-__global__ void __launch_bounds__(256) k_sample(const Mol* __restrict__ mols, const uint32_t* bmol, Caps cap,
-                                                Layout lay, const int32_t* keys, const uint8_t* st, Queue q,
-                                                float* values, char* scratch, size_t slice, uint32_t* overflow,
-                                                uint32_t* n_overflow, unsigned long long* n_samples, uint32_t* loads) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
+k_sample(const Mol* __restrict__ mols, const uint32_t* bmol, Caps cap, Layout lay, const int32_t* keys,
+         const uint8_t* st, Queue q, float* values, char* scratch, size_t slice, uint32_t* overflow,
+         uint32_t* n_overflow, unsigned long long* n_samples, uint32_t* loads) {
     __shared__ uint32_t ticket;
     for_each_item<PERSIST>(q, &ticket, [&](uint32_t j) {
         sample_sub<GLOBAL>(mols, bmol, cap, lay, keys, st, j, values, scratch, slice, overflow, n_overflow,

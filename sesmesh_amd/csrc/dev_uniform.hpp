@@ -46,7 +46,7 @@ __device__ __forceinline__ void uniform_brick(const Mol* __restrict__ mols, cons
     const double cutoff = A.probe + margin + CLASSIFY_EPS;
     char* base = GLOBAL ? scratch + (size_t)blockIdx.x * slice : smem;
     RLists s = rcarve(base, lay);
-    bool ok = gather_refine(A, cap, s, n, tmp4, C[0], C[1], C[2], off, margin + CLASSIFY_EPS);
+    bool ok = gather_refine(A, cap, s, n, tmp4, C[0], C[1], C[2], off, margin + CLASSIFY_EPS, false);
     if (!ok) {
         if (threadIdx.x == 0) overflow[atomicAdd(n_overflow, 1u)] = unit;
         return;
