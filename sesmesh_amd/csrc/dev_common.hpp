@@ -381,26 +381,30 @@ __device__ inline void blocker_bits(gptr<const uint32_t> nb_items, gptr<const ui
 // blocked in either precision.
 constexpr double COVER_PAD = 1e-3, COVER_MARGIN = 4e-3;
 
-__device__ inline double clamp_unit(double v) { return fmin(1.0, fmax(-1.0, v)); }
-
-// cos(min(t + a, pi)) from cos t (t in [0, pi]) and sin a, cos a (a in [0, pi/2]).
-__device__ inline double cos_widened(double ct, double sa, double ca) {
-    if (ct < -ca) return -1.0;  // t + a >= pi
-    return ct * ca - sqrt(fmax(0.0, 1.0 - ct * ct)) * sa;
-}
-
+// Both tests are multiplied through so that they need no division and at
+// most two square roots. Cap: with v = b - c, e = ck - c, D2 = |v|^2,
+// h2 = D2 - rb^2, d = |e|, cos(theta + alpha) d = (v.e sqrt(h2) - |v x e| rb)
+// / D2; theta + alpha >= pi exactly when v.e <= -d sqrt(h2), and then the
+// farthest point is at R + d. Arc: the same with w (b - m in the plane), p
+// (ck - m in the plane), L = |p|, W2 = |w|^2, h2 = W2 - rb^2; the full
+// circle (h2 <= 0) or an arc reaching the far side gives r^2 + |ck - m|^2 +
+// 2 r L.
 __device__ inline bool cap_covered(const double b[3], double rb, const dvec4& c, const dvec4& k) {
     const double lim = k.w - COVER_MARGIN;
     if (!(lim > 0.0)) return false;
     const double vx = b[0] - c.x, vy = b[1] - c.y, vz = b[2] - c.z;
-    const double D2 = vx * vx + vy * vy + vz * vz;
-    if (!(D2 > rb * rb)) return false;
+    const double h2 = (vx * vx + vy * vy + vz * vz) - rb * rb;
+    if (!(h2 > 0.0)) return false;  // the ball reaches the centre: every direction
     const double ex = k.x - c.x, ey = k.y - c.y, ez = k.z - c.z;
-    const double d2 = ex * ex + ey * ey + ez * ez, d = sqrt(d2), D = sqrt(D2);
-    const double sa = rb / D, ca = sqrt(fmax(0.0, 1.0 - sa * sa));
-    const double ct = d > 0.0 ? clamp_unit((vx * ex + vy * ey + vz * ez) / (D * d)) : 1.0;
-    const double far2 = c.w * c.w + d2 - 2.0 * c.w * d * cos_widened(ct, sa, ca);
-    return far2 < lim * lim;
+    const double d2 = ex * ex + ey * ey + ez * ez, dot = vx * ex + vy * ey + vz * ez;
+    const double R = c.w;
+    if (dot <= 0.0 && dot * dot >= d2 * h2) {
+        const double m = lim - R;  // R + d < lim
+        return m > 0.0 && d2 < m * m;
+    }
+    const double cx = vy * ez - vz * ey, cy = vz * ex - vx * ez, cz = vx * ey - vy * ex;
+    const double D2 = h2 + rb * rb;
+    return (R * R + d2 - lim * lim) * D2 < 2.0 * R * (dot * sqrt(h2) - sqrt(cx * cx + cy * cy + cz * cz) * rb);
 }
 
 // Circle record: a = (mx, my, mz, ax), e = (ay, az, rho, -).
@@ -411,19 +415,15 @@ __device__ inline bool arc_covered(const double b[3], double rb, const dvec4& a,
     const double yx = b[0] - a.x, yy = b[1] - a.y, yz = b[2] - a.z;
     const double al = yx * ux + yy * uy + yz * uz;
     const double wx = yx - ux * al, wy = yy - uy * al, wz = yz - uz * al;
-    const double W2 = wx * wx + wy * wy + wz * wz;
+    const double W2 = wx * wx + wy * wy + wz * wz, h2 = W2 - rb * rb;
     const double fx = k.x - a.x, fy = k.y - a.y, fz = k.z - a.z;
     const double fa = fx * ux + fy * uy + fz * uz;
     const double px = fx - ux * fa, py = fy - uy * fa, pz = fz - uz * fa;
-    const double L = sqrt(px * px + py * py + pz * pz);
-    double cphi = -1.0;
-    if (W2 > rb * rb) {
-        const double W = sqrt(W2), sb = rb / W, cb = sqrt(fmax(0.0, 1.0 - sb * sb));
-        const double cd = L > 0.0 ? clamp_unit((wx * px + wy * py + wz * pz) / (W * L)) : 1.0;
-        cphi = cos_widened(cd, sb, cb);
-    }
-    const double far2 = r * r + (fx * fx + fy * fy + fz * fz) - 2.0 * r * L * cphi;
-    return far2 < lim * lim;
+    const double L2 = px * px + py * py + pz * pz, wp = wx * px + wy * py + wz * pz;
+    const double A0 = r * r + (fx * fx + fy * fy + fz * fz) - lim * lim;
+    if (!(h2 > 0.0) || (wp <= 0.0 && wp * wp >= L2 * h2)) return A0 < 0.0 && A0 * A0 > 4.0 * r * r * L2;
+    const double cx = wy * pz - wz * py, cy = wz * px - wx * pz, cz = wx * py - wy * px;
+    return A0 * W2 < 2.0 * r * (wp * sqrt(h2) - sqrt(cx * cx + cy * cy + cz * cz) * rb);
 }
 
 // Marks the candidates q < total hidden everywhere in the ball (b, rb):
@@ -452,11 +452,18 @@ __device__ inline void mark_hidden(const DArr& A, const double b[3], double rb, 
                 g0 = cr[0];
                 g1 = cr[1];
             }
-            for (uint32_t t = off[k] + tid % 8; t < e && !covered; t += 8) {
-                const uint32_t it = items[t];
-                if (hash_get(hk, hv, hmask, it) == 0xffffffffu) continue;
-                const dvec4 ks = ((gptr<const dvec4>)A.sph)[it];
-                covered = kd == 0u ? cap_covered(b, rb, g0, ks) : arc_covered(b, rb, g0, g1, ks);
+            // Rounds of 8 row items; the group stops at the first round in
+            // which one of its lanes finds a covering blocker.
+            for (uint32_t t0 = off[k]; t0 < e; t0 += 8) {
+                const uint32_t t = t0 + tid % 8;
+                if (t < e) {
+                    const uint32_t it = items[t];
+                    if (hash_get(hk, hv, hmask, it) != 0xffffffffu) {
+                        const dvec4 ks = ((gptr<const dvec4>)A.sph)[it];
+                        covered = kd == 0u ? cap_covered(b, rb, g0, ks) : arc_covered(b, rb, g0, g1, ks);
+                    }
+                }
+                if (((__ballot(covered) >> (lane_id() & ~7u)) & 0xffull) != 0ull) break;
             }
         }
         const unsigned long long bal = __ballot(covered);
