@@ -13,6 +13,20 @@ __constant__ uint16_t c_tri_offset[257];
 __constant__ uint8_t c_tri_edges[820 * 3];
 __constant__ uint8_t c_edge_lower[12];
 
+// The Marching Cubes tables in LDS: indexed by each lane's own case, the
+// constant-memory copies are vector loads; one copy per workgroup.
+struct McTables {
+    uint16_t off[257];
+    uint8_t edges[820 * 3];
+    uint8_t lower[12];
+};
+__device__ inline void load_tables(McTables& t) {
+    for (uint32_t i = threadIdx.x; i < 257; i += blockDim.x) t.off[i] = c_tri_offset[i];
+    for (uint32_t i = threadIdx.x; i < 820 * 3; i += blockDim.x) t.edges[i] = c_tri_edges[i];
+    if (threadIdx.x < 12) t.lower[threadIdx.x] = c_edge_lower[threadIdx.x];
+    __syncthreads();
+}
+
 __device__ inline uint32_t cell_case(const float* val, uint32_t side, uint32_t x, uint32_t y, uint32_t z) {
     uint32_t i0 = x + side * (y + side * z), cs = 0;
     for (uint32_t c = 0; c < 8; c++) {
@@ -30,6 +44,8 @@ __global__ void __launch_bounds__(256) k_cell_count(uint32_t b, const float* val
                                                     uint32_t* tcount, uint32_t staged) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     __shared__ uint32_t tmp[4];
+    __shared__ McTables tab;
+    load_tables(tab);
     const uint32_t nb = *n_bricks;
     for (uint32_t brick = blockIdx.x; brick < nb; brick += gridDim.x) {
         const uint32_t side = b + 1, spb = side * side * side;
@@ -46,7 +62,7 @@ __global__ void __launch_bounds__(256) k_cell_count(uint32_t b, const float* val
         for (uint32_t c = threadIdx.x * per; c < min(ncell, (threadIdx.x + 1) * per); c++) {
             uint32_t x = c % b, y = (c / b) % b, z = c / (b * b);
             uint32_t cs = cell_case(val, side, x, y, z);
-            mine += c_tri_offset[cs + 1] - c_tri_offset[cs];
+            mine += tab.off[cs + 1] - tab.off[cs];
         }
         uint32_t total;
         block_scan_256(mine, tmp, total);
@@ -79,15 +95,30 @@ __global__ void __launch_bounds__(256) k_cell_emit(const Mol* __restrict__ mols,
                                                    uint32_t staged) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     __shared__ uint32_t tmp[4];
+    __shared__ int32_t target[8];  // brick at key + (code bits), -1 if none
+    __shared__ uint32_t tvbase[8];  // its vertex base less the molecule's first vertex
+    __shared__ McTables tab;
+    load_tables(tab);
     const uint32_t nb = *n_bricks;
     for (uint32_t brick = blockIdx.x; brick < nb; brick += gridDim.x) {
         const uint32_t m = bmol[brick];
         const DRun& P = mols[m].P;
-        const uint32_t* mdense = dense + mols[m].dense_off;
-        const uint32_t vfirst = vbase[fbrick[m]];
         const float* val = values + (size_t)brick * P.spb;
         const unsigned long long* lmask = mask + (size_t)brick * words;
         const uint32_t* lpre = wpre + (size_t)brick * words;
+        int64_t key[3] = {keys[3 * brick], keys[3 * brick + 1], keys[3 * brick + 2]};
+        if (threadIdx.x < 8) {  // the eight bricks whose edges the cells reference, once per brick
+            const uint32_t code = threadIdx.x;
+            int64_t tb = brick;
+            if (code) {
+                const int64_t di =
+                    dense_index(P, key[0] + (code & 1), key[1] + ((code >> 1) & 1), key[2] + ((code >> 2) & 1));
+                const uint32_t* mdense = dense + mols[m].dense_off;
+                tb = (di < 0 || mdense[di] == 0xffffffffu) ? -1 : (int64_t)mdense[di];
+            }
+            target[code] = (int32_t)tb;
+            tvbase[code] = tb < 0 ? 0u : vbase[tb] - vbase[fbrick[m]];
+        }
         if (staged) {
             float* sv = (float*)smem;
             unsigned long long* sm = (unsigned long long*)(smem + ((4 * (size_t)P.spb + 7) & ~(size_t)7));
@@ -97,31 +128,27 @@ __global__ void __launch_bounds__(256) k_cell_emit(const Mol* __restrict__ mols,
                 sm[w] = lmask[w];
                 sp[w] = lpre[w];
             }
-            __syncthreads();
             val = sv;
             lmask = sm;
             lpre = sp;
         }
+        __syncthreads();
         uint32_t b = P.b, ncell = b * b * b, per = (ncell + blockDim.x - 1) / blockDim.x;
         uint32_t c_lo = threadIdx.x * per, c_hi = min(ncell, (threadIdx.x + 1) * per);
         uint32_t mine = 0;
         for (uint32_t c = c_lo; c < c_hi; c++) {
             uint32_t cs = cell_case(val, P.side, c % b, (c / b) % b, c / (b * b));
-            mine += c_tri_offset[cs + 1] - c_tri_offset[cs];
+            mine += tab.off[cs + 1] - tab.off[cs];
         }
         uint32_t total;
         size_t at = (size_t)tbase[brick] + block_scan_256(mine, tmp, total);
-        int64_t key[3] = {keys[3 * brick], keys[3 * brick + 1], keys[3 * brick + 2]};
-        int64_t target[8];
-        for (int q = 0; q < 8; q++) target[q] = -2;
-        target[0] = brick;
         for (uint32_t c = c_lo; c < c_hi; c++) {
             uint32_t x = c % b, y = (c / b) % b, z = c / (b * b);
             uint32_t cs = cell_case(val, P.side, x, y, z);
-            for (uint32_t t = c_tri_offset[cs]; t < c_tri_offset[cs + 1]; t++) {
+            for (uint32_t t = tab.off[cs]; t < tab.off[cs + 1]; t++) {
                 for (int q = 0; q < 3; q++) {
-                    uint32_t e = c_tri_edges[3 * t + q];
-                    uint32_t axis = e / 4, lo = c_edge_lower[e];
+                    uint32_t e = tab.edges[3 * t + q];
+                    uint32_t axis = e / 4, lo = tab.lower[e];
                     uint32_t l[3] = {x + (lo & 1), y + ((lo >> 1) & 1), z + ((lo >> 2) & 1)};
                     uint32_t code = 0, lc[3] = {0, 0, 0};
                     for (int a = 0; a < 3; a++) {
@@ -131,22 +158,17 @@ __global__ void __launch_bounds__(256) k_cell_emit(const Mol* __restrict__ mols,
                             lc[a] = l[a];
                     }
                     uint32_t id = ((lc[2] * b + lc[1]) * b + lc[0]) * 3 + axis;
-                    if (target[code] == -2) {
-                        int64_t di = dense_index(P, key[0] + (code & 1), key[1] + ((code >> 1) & 1),
-                                                 key[2] + ((code >> 2) & 1));
-                        target[code] = (di < 0 || mdense[di] == 0xffffffffu) ? -1 : (int64_t)mdense[di];
-                    }
-                    int64_t tb = target[code];
+                    const int32_t tb = target[code];
                     uint32_t vi = 0;
                     if (code == 0) {  // the brick's own edge (masks staged in LDS)
                         if (!edge_is_crossing(lmask, words, 0u, id))
                             atomicOr(err, 1u);
                         else
-                            vi = vbase[brick] + edge_rank(lmask, lpre, words, 0u, id) - vfirst;
+                            vi = tvbase[0] + edge_rank(lmask, lpre, words, 0u, id);
                     } else if (tb < 0 || !edge_is_crossing(mask, words, (uint32_t)tb, id)) {
                         atomicOr(err, 1u);
                     } else {
-                        vi = vbase[tb] + edge_rank(mask, wpre, words, (uint32_t)tb, id) - vfirst;
+                        vi = tvbase[code] + edge_rank(mask, wpre, words, (uint32_t)tb, id);
                     }
                     if (at < tcap) tris[3 * at + q] = vi;
                 }
