@@ -325,6 +325,20 @@ __device__ inline uint32_t hash_get(const uint32_t* hk, const uint32_t* hv, uint
 }
 
 constexpr uint32_t KIND_SHIFT = 30, ID_MASK = (1u << KIND_SHIFT) - 1u;
+
+// The candidates the sampling gather dropped as hidden, per sampled
+// sub-block j: cnt[j] entries (candidate id | class << KIND_SHIFT, class 0
+// for S, 1 for C) at ids + off[j]; cnt[j] is NO_LIST where there is no list
+// (a sub-block not sampled, or the pool of `cap` entries full). `used`
+// counts the entries taken. The refinement of the sub-block reuses them:
+// the sampling ball plus COVER_PAD holds the refinement's ball, its near
+// spheres and candidates are among the refinement's, and the cover limit
+// is the same, so each of them is hidden there too.
+struct HidList {
+    uint32_t *off, *cnt, *ids, *used;
+    uint32_t cap;
+};
+constexpr uint32_t NO_LIST = 0xffffffffu;
 constexpr uint32_t KEY_BUCKETS = 16;  // key buckets per candidate class in the gathers
 
 // Blocker bitsets of candidate rows p < rows: row p lists sphere indices

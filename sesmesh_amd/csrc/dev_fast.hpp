@@ -386,7 +386,8 @@ __device__ __forceinline__ void refine_sub_fast(const Mol* __restrict__ mols, co
                                                 const float* values, uint32_t j, const unsigned long long* mask,
                                                 const uint32_t* wpre, uint32_t words, const uint32_t* vbase,
                                                 const VOut& out, char* scratch, size_t slice, uint32_t* overflow,
-                                                uint32_t* n_overflow, unsigned long long* evals_total) {
+                                                uint32_t* n_overflow, unsigned long long* evals_total,
+                                                const HidList& hid) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     __shared__ uint32_t n[R_N];
     __shared__ uint32_t tmp4[WIDE / WAVE];  // block scans: one slot per wavefront
@@ -411,7 +412,10 @@ __device__ __forceinline__ void refine_sub_fast(const Mol* __restrict__ mols, co
         double basep = (double)org[a] * h;
         c[a] = basep + ((double)(o3[a] * sub) + 0.5 * subd) * h;
     }
-    bool ok = gather_refine(A, cap, s, n, tmp4, c[0], c[1], c[2], rho, P.band, true);
+    const uint32_t* hl = nullptr;  // the sampling's hidden candidates of this sub-block
+    uint32_t hn = hid.cnt ? hid.cnt[j] : NO_LIST;
+    if (hn != NO_LIST) hl = hid.ids + hid.off[j];
+    bool ok = gather_refine(A, cap, s, n, tmp4, c[0], c[1], c[2], rho, P.band, true, hl, hn);
     uint32_t lo[3], hi[3];
     for (int a = 0; a < 3; a++) {
         lo[a] = o3[a] * sub;
@@ -481,14 +485,15 @@ __device__ __forceinline__ void refine_sub_fast(const Mol* __restrict__ mols, co
 template <bool GLOBAL, bool PERSIST>
 __global__ void __launch_bounds__(256) k_refine_fast(const Mol* __restrict__ mols, const uint32_t* bmol, RCaps cap,
                                                      RLayout lay, FLayout flay, const int32_t* keys,
-                                                     const float* values, Queue q, const unsigned long long* mask,
+                                                     const float* values, Queue q, HidList hid,
+                                                     const unsigned long long* mask,
                                                      const uint32_t* wpre, uint32_t words, const uint32_t* vbase,
                                                      VOut out, char* scratch, size_t slice, uint32_t* overflow,
                                                      uint32_t* n_overflow, unsigned long long* evals_total) {
     __shared__ uint32_t ticket;
     for_each_item<PERSIST>(q, &ticket, [&](uint32_t j) {
         refine_sub_fast<GLOBAL>(mols, bmol, cap, lay, flay, keys, values, j, mask, wpre, words, vbase, out, scratch,
-                                slice, overflow, n_overflow, evals_total);
+                                slice, overflow, n_overflow, evals_total, hid);
     });
 }
 
@@ -496,14 +501,15 @@ __global__ void __launch_bounds__(256) k_refine_fast(const Mol* __restrict__ mol
 template <bool GLOBAL, bool PERSIST>
 __global__ void __launch_bounds__(WIDE) k_refine_fast_wide(const Mol* __restrict__ mols, const uint32_t* bmol, RCaps cap,
                                                      RLayout lay, FLayout flay, const int32_t* keys,
-                                                     const float* values, Queue q, const unsigned long long* mask,
+                                                     const float* values, Queue q, HidList hid,
+                                                     const unsigned long long* mask,
                                                      const uint32_t* wpre, uint32_t words, const uint32_t* vbase,
                                                      VOut out, char* scratch, size_t slice, uint32_t* overflow,
                                                      uint32_t* n_overflow, unsigned long long* evals_total) {
     __shared__ uint32_t ticket;
     for_each_item<PERSIST>(q, &ticket, [&](uint32_t j) {
         refine_sub_fast<GLOBAL>(mols, bmol, cap, lay, flay, keys, values, j, mask, wpre, words, vbase, out, scratch,
-                                slice, overflow, n_overflow, evals_total);
+                                slice, overflow, n_overflow, evals_total, hid);
     });
 }
 

@@ -185,9 +185,12 @@ enum { R_NEAR = 0, R_IN, R_S, R_C, R_V, R_CAND, R_E, R_OVER, R_N };
 // their f64 data and blocker bitsets (row members that are near,
 // point.rs:152-155). False (in every thread) if a capacity was exceeded.
 // With `hide`, candidates that one blocker hides from the whole ball are
-// left out (mark_hidden): for callers that evaluate many points of the ball.
+// left out: for callers that evaluate many points of the ball. The hidden
+// ones are taken from `hl` (hn entries, see HidList) when the sampling of
+// the same sub-block listed them, else found with mark_hidden.
 __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s, uint32_t* n, uint32_t* tmp4,
-                                     double bx, double by, double bz, double rho, double band, bool hide) {
+                                     double bx, double by, double bz, double rho, double band, bool hide,
+                                     const uint32_t* hl = nullptr, uint32_t hn = NO_LIST) {
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     __shared__ uint32_t bstart[WAVE], bfill[WAVE];  // per key bucket: 16 for each of S, C, V
     for (uint32_t i = tid; i < cap.hash; i += nt) s.hk[i] = 0u;
@@ -280,19 +283,34 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
     const uint32_t gathered = n[R_CAND];
     // Candidates one blocker hides from the whole ball leave the lists (their
     // key becomes all ones); the list order of the others is unchanged.
-    constexpr unsigned long long HIDDEN = ~0ull;
-    if (hide) {
+    constexpr unsigned long long HIDDEN_KEY = ~0ull;
+    if (hide && hn != NO_LIST) {
+        // Through the near hash, under keys with the top bit set (no sphere
+        // id has it), while the table stays at most half full; a listed
+        // candidate that is not entered is kept, which is always safe.
+        const uint32_t used = n[R_NEAR], room = cap.hash / 2 > used ? cap.hash / 2 - used : 0u;
+        for (uint32_t i = tid; i < min(hn, room); i += nt) {
+            const uint32_t key = hl[i] | 0x80000000u;
+            if (key != 0xffffffffu) hash_put(s.hk, s.hv, hmask, key, 0u);
+        }
+        __syncthreads();
+        for (uint32_t q = tid; q < gathered; q += nt) {
+            const uint32_t id = s.cid[q], key = id | 0x80000000u;
+            if ((id >> KIND_SHIFT) <= 1u && key != 0xffffffffu && hash_get(s.hk, s.hv, hmask, key) != NO_LIST)
+                s.ckey[q] = HIDDEN_KEY;
+        }
+    } else if (hide) {
         const double bc[3] = {bx, by, bz};
         mark_hidden(
             A, bc, rho + COVER_PAD, gathered, s.hk, s.hv, hmask, [&](uint32_t q) { return s.cid[q] >> KIND_SHIFT; },
-            [&](uint32_t q) { return s.cid[q] & ID_MASK; }, [&](uint32_t q) { s.ckey[q] = HIDDEN; });
+            [&](uint32_t q) { return s.cid[q] & ID_MASK; }, [&](uint32_t q) { s.ckey[q] = HIDDEN_KEY; });
     }
     __syncthreads();
     auto bucket_of = [&](unsigned long long key, uint32_t id) {
         return bucket(id >> KIND_SHIFT, key_value(key));
     };
     for (uint32_t q = tid; q < gathered; q += nt)
-        if (s.ckey[q] != HIDDEN) atomicAdd(&bstart[bucket_of(s.ckey[q], s.cid[q])], 1u);
+        if (s.ckey[q] != HIDDEN_KEY) atomicAdd(&bstart[bucket_of(s.ckey[q], s.cid[q])], 1u);
     __syncthreads();
     // The CPU's list order, (class, key, index) by cand_less, sorted exactly
     // through the buckets: a candidate's bucket is monotone in (class, key),
@@ -320,7 +338,7 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
     uint32_t* tcid = (uint32_t*)(tkey + cap.cand);
     for (uint32_t q = tid; q < gathered; q += nt) {
         const unsigned long long key = s.ckey[q];
-        if (key == HIDDEN) continue;
+        if (key == HIDDEN_KEY) continue;
         const uint32_t id = s.cid[q];
         const uint32_t t = atomicAdd(&bfill[bucket_of(key, id)], 1u);
         tkey[t] = key;
@@ -790,7 +808,7 @@ __device__ __forceinline__ void refine_sub(const Mol* __restrict__ mols, const u
                                            const unsigned long long* mask, const uint32_t* wpre, uint32_t words,
                                            const uint32_t* vbase, const VOut& out, char* scratch, size_t slice,
                                            uint32_t* overflow, uint32_t* n_overflow,
-                                           unsigned long long* evals_total) {
+                                           unsigned long long* evals_total, const HidList& hid) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     __shared__ uint32_t n[R_N];
     __shared__ uint32_t tmp4[WIDE / WAVE];  // block scans: one slot per wavefront
@@ -815,7 +833,10 @@ __device__ __forceinline__ void refine_sub(const Mol* __restrict__ mols, const u
         double basep = (double)org[a] * h;
         c[a] = basep + ((double)(o3[a] * sub) + 0.5 * subd) * h;
     }
-    bool ok = gather_refine(A, cap, s, n, tmp4, c[0], c[1], c[2], rho, P.band, true);
+    const uint32_t* hl = nullptr;  // the sampling's hidden candidates of this sub-block
+    uint32_t hn = hid.cnt ? hid.cnt[j] : NO_LIST;
+    if (hn != NO_LIST) hl = hid.ids + hid.off[j];
+    bool ok = gather_refine(A, cap, s, n, tmp4, c[0], c[1], c[2], rho, P.band, true, hl, hn);
     // Crossing edges whose lower sample lies in this sub-block, ascending.
     uint32_t lo[3], hi[3];
     for (int a = 0; a < 3; a++) {
@@ -895,14 +916,14 @@ __device__ __forceinline__ void refine_sub(const Mol* __restrict__ mols, const u
 template <bool GLOBAL, bool PERSIST>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_refine(const Mol* __restrict__ mols, const uint32_t* bmol, RCaps cap,
                                                 RLayout lay, const int32_t* keys, const float* values, Queue q,
-                                                const unsigned long long* mask, const uint32_t* wpre,
+                                                HidList hid, const unsigned long long* mask, const uint32_t* wpre,
                                                 uint32_t words, const uint32_t* vbase, VOut out, char* scratch,
                                                 size_t slice, uint32_t* overflow, uint32_t* n_overflow,
                                                 unsigned long long* evals_total) {
     __shared__ uint32_t ticket;
     for_each_item<PERSIST>(q, &ticket, [&](uint32_t j) {
         refine_sub<GLOBAL>(mols, bmol, cap, lay, keys, values, j, mask, wpre, words, vbase, out, scratch, slice,
-                           overflow, n_overflow, evals_total);
+                           overflow, n_overflow, evals_total, hid);
     });
 }
 
@@ -910,14 +931,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
 template <bool GLOBAL, bool PERSIST>
 __global__ void __launch_bounds__(WIDE) k_refine_wide(const Mol* __restrict__ mols, const uint32_t* bmol, RCaps cap,
                                                 RLayout lay, const int32_t* keys, const float* values, Queue q,
-                                                const unsigned long long* mask, const uint32_t* wpre,
+                                                HidList hid, const unsigned long long* mask, const uint32_t* wpre,
                                                 uint32_t words, const uint32_t* vbase, VOut out, char* scratch,
                                                 size_t slice, uint32_t* overflow, uint32_t* n_overflow,
                                                 unsigned long long* evals_total) {
     __shared__ uint32_t ticket;
     for_each_item<PERSIST>(q, &ticket, [&](uint32_t j) {
         refine_sub<GLOBAL>(mols, bmol, cap, lay, keys, values, j, mask, wpre, words, vbase, out, scratch, slice,
-                           overflow, n_overflow, evals_total);
+                           overflow, n_overflow, evals_total, hid);
     });
 }
 

@@ -126,6 +126,7 @@ __device__ inline Lists carve(char* base, const Layout& L) {
 
 // Counters of the list builder.
 enum { C_NEAR = 0, C_IN, C_S, C_C, C_V, C_CAND, C_OVER, C_N };
+constexpr uint32_t HIDDEN = 0x80000000u;  // a candidate's bucket word when it is hidden
 
 // True if (qx, qy, qz) lies inside a blocker of candidate p (kernel.rs:
 // 898-906, 935-943): !(|q - c|^2 >= (r - tol)^2) for a near row member.
@@ -320,7 +321,6 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
     if (n[C_OVER]) return false;
     const uint32_t gathered = n[C_CAND];
     // Candidates one blocker hides from the whole sub-block leave the lists.
-    constexpr uint32_t HIDDEN = 0x80000000u;
     mark_hidden(
         A, b, rho + COVER_PAD, gathered, s.hk, s.hv, hmask, [&](uint32_t q) { return s.cbk[q] < KEY_BUCKETS ? 0u : 1u; },
         [&](uint32_t q) { return s.cid[q]; }, [&](uint32_t q) { s.cbk[q] |= HIDDEN; });
@@ -390,7 +390,8 @@ template <bool GLOBAL>
 __device__ __forceinline__ void sample_sub(const Mol* __restrict__ mols, const uint32_t* bmol, const Caps& cap,
                                            const Layout& lay, const int32_t* keys, const uint8_t* st, uint32_t j,
                                            float* values, char* scratch, size_t slice, uint32_t* overflow,
-                                           uint32_t* n_overflow, unsigned long long* n_samples, uint32_t* loads) {
+                                           uint32_t* n_overflow, unsigned long long* n_samples, uint32_t* loads,
+                                           const HidList& hid) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     __shared__ uint32_t n[C_N];
     __shared__ uint32_t tmp4[4];  // block scans: one slot per wavefront
@@ -411,6 +412,27 @@ __device__ __forceinline__ void sample_sub(const Mol* __restrict__ mols, const u
         return;
     }
     if (loads && threadIdx.x == 0) atomicMax(&loads[brick], candidate_load(n[C_CAND], n[C_NEAR], n[C_V], n[C_IN]));
+    if (hid.ids) {  // the hidden candidates, for this sub-block's refinement
+        __shared__ uint32_t h_at, h_fill;
+        const uint32_t gathered = n[C_CAND], nh = gathered - n[C_S] - n[C_C];
+        if (threadIdx.x == 0) {
+            const uint32_t at = atomicAdd(hid.used, nh);
+            h_at = at <= hid.cap && nh <= hid.cap - at ? at : NO_LIST;
+            h_fill = 0;
+            if (h_at != NO_LIST) {
+                hid.off[j] = at;
+                hid.cnt[j] = nh;
+            }
+        }
+        __syncthreads();
+        if (h_at != NO_LIST)
+            for (uint32_t q = threadIdx.x; q < gathered; q += blockDim.x) {
+                const uint32_t bk = s.cbk[q];
+                if (bk >= HIDDEN)
+                    hid.ids[h_at + atomicAdd(&h_fill, 1u)] =
+                        s.cid[q] | (bk - HIDDEN >= KEY_BUCKETS ? 1u << KIND_SHIFT : 0u);
+            }
+    }
     const uint32_t cc = cap.cand, nw = cap.near / 64;
     uint32_t stb = pack_states(st + 8 * brick);
     uint32_t sub = P.sub, box = sub + 1, side = P.side;
@@ -453,11 +475,11 @@ template <bool GLOBAL, bool PERSIST>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
 k_sample(const Mol* __restrict__ mols, const uint32_t* bmol, Caps cap, Layout lay, const int32_t* keys,
          const uint8_t* st, Queue q, float* values, char* scratch, size_t slice, uint32_t* overflow,
-         uint32_t* n_overflow, unsigned long long* n_samples, uint32_t* loads) {
+         uint32_t* n_overflow, unsigned long long* n_samples, uint32_t* loads, HidList hid) {
     __shared__ uint32_t ticket;
     for_each_item<PERSIST>(q, &ticket, [&](uint32_t j) {
         sample_sub<GLOBAL>(mols, bmol, cap, lay, keys, st, j, values, scratch, slice, overflow, n_overflow,
-                           n_samples, loads);
+                           n_samples, loads, hid);
     });
 }
 

@@ -795,6 +795,7 @@ constexpr RCaps UNIFORM_CAPS[UNIFORM_TIERS] = {{128, 64, 192, 256, 0, 1, 0}, {19
 constexpr uint32_t LDS_EDGES = 512;      // crossing-edge buffer of the LDS passes
 constexpr uint32_t SCRATCH_CHUNK = 1024;  // default workgroups of a scratch pass (one list slice each)
 constexpr uint32_t STATIC_LDS = 6144;     // static LDS of k_sample / k_refine, bound
+constexpr uint32_t HID_PER_SUB = 64;      // hidden-candidate pool entries per sampled sub-block
 constexpr uint32_t NO_CAP = 0xffffffffu;
 
 // Device counters of a run (u32 slots, zero at the start of the run).
@@ -806,7 +807,8 @@ enum : uint32_t {
     K_SLIST0, K_SPRED0 = K_SLIST0 + TIERS, K_SLAST = K_SPRED0 + TIERS,
     K_RLIST0, K_RPRED0 = K_RLIST0 + TIERS, K_RLAST = K_RPRED0 + TIERS,
     K_Q_SUBST, K_Q_SAMPLE0, K_Q_REFINE0 = K_Q_SAMPLE0 + TIERS,
-    K_N = K_Q_REFINE0 + TIERS
+    K_HID = K_Q_REFINE0 + TIERS,  // entries taken from the hidden-candidate pool
+    K_N
 };
 // Capacity of each tier in candidate-load units (max(S + C, 2 near, 4 SAS
 // vertices, 4 inside) of the sampling lists; the refinement lists alike).
@@ -1279,6 +1281,18 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     // the run is planned); later passes are persistent over their queues.
     uint32_t nsw;
     TRY(X.size(&cnt[K_NSW], (uint32_t)nsub, nsw));
+    // The hidden candidates the sampling finds, per sub-block, for the
+    // refinement (HidList): one pool of HID_PER_SUB entries per sampled
+    // sub-block; a sub-block that finds it full gets no list.
+    HidList hid{};
+    TRY(X.temp(&hid.off, nsub));
+    TRY(X.temp(&hid.cnt, nsub));
+    const uint64_t hcap = std::min<uint64_t>((uint64_t)HID_PER_SUB * std::max<uint32_t>(nsw, 1u), 0xfffffff0ull);
+    TRY(X.temp(&hid.ids, hcap));
+    hid.used = &cnt[K_HID];
+    hid.cap = (uint32_t)hcap;
+    k_fill_u32<<<stride_grid(nsub), 256, 0, s>>>(hid.cnt, NO_LIST, nsub, &cnt[K_NSUB]);
+    HIP_TRY(hipGetLastError());
     {
         bool runs[TIERS];
         for (int p = 0; p < TIERS; p++) {
@@ -1335,18 +1349,18 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
             if (pass < TIERS - 1 && (!X.planned() || pass == first)) {
                 k_sample<false, false><<<g1(items), 256, lay.bytes, s>>>(mols, bmol, cap, lay, M->keys, st, q,
                                                                          M->values, nullptr, 0, next, nnext, nsamp,
-                                                                         cload);
+                                                                         cload, hid);
             } else if (pass < TIERS - 1) {
                 uint32_t grid =
                     g1(std::min<uint64_t>(nsub, resident(d, (const void*)k_sample<false, true>, 256, lay.bytes)));
                 k_sample<false, true><<<grid, 256, lay.bytes, s>>>(mols, bmol, cap, lay, M->keys, st, q, M->values,
-                                                                   nullptr, 0, next, nnext, nsamp, cload);
+                                                                   nullptr, 0, next, nnext, nsamp, cload, hid);
             } else {
                 uint32_t grid = g1(std::min<uint64_t>(items, scratch_wgs));
                 char* scratch;
                 TRY(X.temp(&scratch, (size_t)grid * lay.bytes));
                 k_sample<true, true><<<grid, 256, 0, s>>>(mols, bmol, cap, lay, M->keys, st, q, M->values, scratch,
-                                                          lay.bytes, next, nnext, nsamp, cload);
+                                                          lay.bytes, next, nnext, nsamp, cload, hid);
             }
             HIP_TRY(hipGetLastError());
             if (pass == first) TRY(rec(11));
@@ -1458,18 +1472,18 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
             if (pass < TIERS - 1 && (!X.planned() || pass == first)) {
                 if (fast && wide)
                     k_refine_fast_wide<false, false><<<g1(items), WIDE, bytes, s>>>(
-                        mols, bmol, cap, lay, flay, M->keys, M->values, q, mask, wpre, words, vbase, vo, nullptr, 0,
+                        mols, bmol, cap, lay, flay, M->keys, M->values, q, hid, mask, wpre, words, vbase, vo, nullptr, 0,
                         next, nnext, evd);
                 else if (fast)
                     k_refine_fast<false, false><<<g1(items), 256, bytes, s>>>(mols, bmol, cap, lay, flay, M->keys,
-                                                                              M->values, q, mask, wpre, words, vbase,
+                                                                              M->values, q, hid, mask, wpre, words, vbase,
                                                                               vo, nullptr, 0, next, nnext, evd);
                 else if (wide)
                     k_refine_wide<false, false><<<g1(items), WIDE, bytes, s>>>(mols, bmol, cap, lay, M->keys,
-                                                                               M->values, q, mask, wpre, words, vbase,
+                                                                               M->values, q, hid, mask, wpre, words, vbase,
                                                                                vo, nullptr, 0, next, nnext, evd);
                 else
-                    k_refine<false, false><<<g1(items), 256, bytes, s>>>(mols, bmol, cap, lay, M->keys, M->values, q,
+                    k_refine<false, false><<<g1(items), 256, bytes, s>>>(mols, bmol, cap, lay, M->keys, M->values, q, hid,
                                                                          mask, wpre, words, vbase, vo, nullptr, 0,
                                                                          next, nnext, evd);
             } else if (pass < TIERS - 1) {
@@ -1481,18 +1495,18 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
                 uint32_t grid = g1(std::min<uint64_t>(nsub, resident(d, kern, block, bytes)));
                 if (fast && wide)
                     k_refine_fast_wide<false, true><<<grid, WIDE, bytes, s>>>(mols, bmol, cap, lay, flay, M->keys,
-                                                                              M->values, q, mask, wpre, words, vbase,
+                                                                              M->values, q, hid, mask, wpre, words, vbase,
                                                                               vo, nullptr, 0, next, nnext, evd);
                 else if (fast)
                     k_refine_fast<false, true><<<grid, 256, bytes, s>>>(mols, bmol, cap, lay, flay, M->keys,
-                                                                        M->values, q, mask, wpre, words, vbase, vo,
+                                                                        M->values, q, hid, mask, wpre, words, vbase, vo,
                                                                         nullptr, 0, next, nnext, evd);
                 else if (wide)
-                    k_refine_wide<false, true><<<grid, WIDE, bytes, s>>>(mols, bmol, cap, lay, M->keys, M->values, q,
+                    k_refine_wide<false, true><<<grid, WIDE, bytes, s>>>(mols, bmol, cap, lay, M->keys, M->values, q, hid,
                                                                          mask, wpre, words, vbase, vo, nullptr, 0,
                                                                          next, nnext, evd);
                 else
-                    k_refine<false, true><<<grid, 256, bytes, s>>>(mols, bmol, cap, lay, M->keys, M->values, q, mask,
+                    k_refine<false, true><<<grid, 256, bytes, s>>>(mols, bmol, cap, lay, M->keys, M->values, q, hid, mask,
                                                                    wpre, words, vbase, vo, nullptr, 0, next, nnext,
                                                                    evd);
             } else {
@@ -1501,10 +1515,10 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
                 TRY(X.temp(&scratch, (size_t)grid * bytes));
                 if (fast)
                     k_refine_fast_wide<true, true><<<grid, WIDE, 0, s>>>(mols, bmol, cap, lay, flay, M->keys,
-                                                                         M->values, q, mask, wpre, words, vbase, vo,
+                                                                         M->values, q, hid, mask, wpre, words, vbase, vo,
                                                                          scratch, bytes, next, nnext, evd);
                 else
-                    k_refine_wide<true, true><<<grid, WIDE, 0, s>>>(mols, bmol, cap, lay, M->keys, M->values, q, mask,
+                    k_refine_wide<true, true><<<grid, WIDE, 0, s>>>(mols, bmol, cap, lay, M->keys, M->values, q, hid, mask,
                                                                     wpre, words, vbase, vo, scratch, bytes, next,
                                                                     nnext, evd);
             }
