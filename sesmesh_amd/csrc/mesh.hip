@@ -788,10 +788,11 @@ constexpr RCaps REFINE_CAPS[TIERS] = {{128, 64, 192, 256, 0, 0, 0},
                                       {1024, 512, 4096, 4096, 0, 0, 0}};
 constexpr int WIDE_FROM = 2;  // refinement passes over tiers from this one run WIDE threads
 static_assert(TIERS == SPLIT_TIERS, "the learned-sizing split covers every tier");
-// Sub-block classification: two LDS tiers (5 and 3 workgroups per CU);
-// bricks beyond them take one wavefront per brick.
-constexpr int UNIFORM_TIERS = 2;
-constexpr RCaps UNIFORM_CAPS[UNIFORM_TIERS] = {{128, 64, 192, 256, 0, 1, 0}, {192, 128, 256, 512, 0, 1, 0}};
+// Sub-block classification: three LDS tiers (5, 3 and 1 workgroups per
+// CU); bricks beyond them take one wavefront per brick.
+constexpr int UNIFORM_TIERS = 3;
+constexpr RCaps UNIFORM_CAPS[UNIFORM_TIERS] = {
+    {128, 64, 192, 256, 0, 1, 0}, {192, 128, 256, 512, 0, 1, 0}, {384, 256, 640, 1024, 0, 1, 0}};
 constexpr uint32_t LDS_EDGES = 512;      // crossing-edge buffer of the LDS passes
 constexpr uint32_t SCRATCH_CHUNK = 1024;  // default workgroups of a scratch pass (one list slice each)
 constexpr uint32_t STATIC_LDS = 6144;     // static LDS of k_sample / k_refine, bound
@@ -801,10 +802,11 @@ constexpr uint32_t NO_CAP = 0xffffffffu;
 // Device counters of a run (u32 slots, zero at the start of the run).
 enum : uint32_t {
     K_TOUCHED, K_NB, K_NSUB, K_NSW, K_NV, K_NRW, K_NT, K_FLAGS, K_ERR,
-    K_UOVER, K_Q_UNIFORM = K_UOVER + UNIFORM_TIERS,  // per sub-block classification tier: overflow count
+    // Per sub-block classification tier: overflow count, queue ticket.
+    K_UOVER, K_Q_UNIFORM0 = K_UOVER + UNIFORM_TIERS,
     // Per pass: items in its list (predicted there, then appended by the
     // passes before it), items predicted there, and the last pass's overflow.
-    K_SLIST0, K_SPRED0 = K_SLIST0 + TIERS, K_SLAST = K_SPRED0 + TIERS,
+    K_SLIST0 = K_Q_UNIFORM0 + UNIFORM_TIERS, K_SPRED0 = K_SLIST0 + TIERS, K_SLAST = K_SPRED0 + TIERS,
     K_RLIST0, K_RPRED0 = K_RLIST0 + TIERS, K_RLAST = K_RPRED0 + TIERS,
     K_Q_SUBST, K_Q_SAMPLE0, K_Q_REFINE0 = K_Q_SAMPLE0 + TIERS,
     K_HID = K_Q_REFINE0 + TIERS,  // entries taken from the hidden-candidate pool
@@ -1236,7 +1238,7 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
             if (lay.bytes + STATIC_LDS > lds_max) continue;
             uint32_t* over;
             TRY(X.temp(&over, nb));
-            Queue q{list, count, &cnt[K_Q_UNIFORM]};
+            Queue q{list, count, &cnt[K_Q_UNIFORM0 + u]};
             uint32_t items = nb;
             if (!first && !X.planned()) TRY(X.size(count, 0, items));
             if (first || !X.planned())
