@@ -440,49 +440,43 @@ __device__ inline bool arc_covered(const double b[3], double rb, const dvec4& a,
     return A0 * W2 < 2.0 * r * (wp * sqrt(h2) - sqrt(cx * cx + cy * cy + cz * cz) * rb);
 }
 
-// Marks the candidates q < total hidden everywhere in the ball (b, rb):
-// kind(q) is 0 for class S (sphere id(q)), 1 for class C (circle id(q)),
-// anything else for candidates without blockers; mark(q) is called once per
-// hidden candidate, by one lane. A blocker counts only if the near hash
-// holds it, as in the bitsets. 8 lanes per candidate row. By the whole
-// workgroup; the caller synchronises afterwards.
-template <class Kind, class Id, class Mark>
-__device__ inline void mark_hidden(const DArr& A, const double b[3], double rb, uint32_t total, const uint32_t* hk,
-                                   const uint32_t* hv, uint32_t hmask, Kind&& kind, Id&& id, Mark&& mark) {
+// Marks the candidates q < total hidden everywhere in the ball (b, rb). A
+// blocker that covers a candidate's cap or arc holds points of its sphere
+// (or circle) well inside, so it overlaps that sphere (both circle spheres)
+// by more than the rows' EPS: neighbour rows hold every such sphere
+// (geom/arrangement.rs:519-550) and circle-blocker rows every common
+// neighbour cutting the circle (:672-751). So the near list itself is
+// searched, without the rows. kind(q) is 0 for class S (sphere id(q)), 1
+// for class C (circle id(q)), anything else for candidates without
+// blockers; geom(kind, id, g0, g1) gives a candidate's record in the frame
+// of b (S: centre, radius; C: centre, axis x | axis y, axis z, radius);
+// blocker(slot) the near sphere at a near-list slot as (centre, its
+// blocking radius less COVER_MARGIN); mark(q) is called once per hidden
+// candidate, by one lane. 4 lanes per candidate. By the whole workgroup;
+// the caller synchronises afterwards.
+template <class Kind, class Id, class Geom, class Blocker, class Mark>
+__device__ inline void mark_hidden(const double b[3], double rb, uint32_t total, uint32_t n_near, Kind&& kind,
+                                   Id&& id, Geom&& geom, Blocker&& blocker, Mark&& mark) {
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
-    for (uint32_t q = tid / 8; q < total; q += nt / 8) {
+    for (uint32_t q = tid / 4; q < total; q += nt / 4) {
         const uint32_t kd = kind(q);
         bool covered = false;
         if (kd <= 1u) {
-            const uint32_t k = id(q);
-            gptr<const uint32_t> off = kd == 0u ? A.nb_off : A.cb_off;
-            gptr<const uint32_t> items = kd == 0u ? A.nb_items : A.cb_items;
-            const uint32_t e = off[k + 1];
             dvec4 g0, g1 = dvec4{0.0, 0.0, 0.0, 0.0};
-            if (kd == 0u) {
-                g0 = ((gptr<const dvec4>)A.sph)[k];
-            } else {
-                gptr<const dvec4> cr = (gptr<const dvec4>)A.cir + 2 * (size_t)k;
-                g0 = cr[0];
-                g1 = cr[1];
-            }
-            // Rounds of 8 row items; the group stops at the first round in
-            // which one of its lanes finds a covering blocker.
-            for (uint32_t t0 = off[k]; t0 < e; t0 += 8) {
-                const uint32_t t = t0 + tid % 8;
-                if (t < e) {
-                    const uint32_t it = items[t];
-                    if (hash_get(hk, hv, hmask, it) != 0xffffffffu) {
-                        const dvec4 ks = ((gptr<const dvec4>)A.sph)[it];
-                        covered = kd == 0u ? cap_covered(b, rb, g0, ks) : arc_covered(b, rb, g0, g1, ks);
-                    }
-                }
-                if (((__ballot(covered) >> (lane_id() & ~7u)) & 0xffull) != 0ull) break;
+            geom(kd, id(q), g0, g1);
+            // A blocker farther than this from the centre cannot reach the
+            // sphere's (circle's) points within its limit.
+            const double reach = kd == 0u ? g0.w : g1.z;
+            for (uint32_t k = tid % 4; k < n_near && !covered; k += 4) {
+                const dvec4 ks = blocker(k);
+                const double ex = ks.x - g0.x, ey = ks.y - g0.y, ez = ks.z - g0.z, far = reach + ks.w;
+                if (!(ks.w > 0.0) || ex * ex + ey * ey + ez * ez >= far * far) continue;
+                covered = kd == 0u ? cap_covered(b, rb, g0, ks) : arc_covered(b, rb, g0, g1, ks);
             }
         }
         const unsigned long long bal = __ballot(covered);
         const uint32_t lane = lane_id();
-        if ((lane & 7u) == 0u && ((bal >> lane) & 0xffull) != 0ull) mark(q);
+        if ((lane & 3u) == 0u && ((bal >> lane) & 0xfull) != 0ull) mark(q);
     }
 }
 

@@ -302,8 +302,22 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
     } else if (hide) {
         const double bc[3] = {bx, by, bz};
         mark_hidden(
-            A, bc, rho + COVER_PAD, gathered, s.hk, s.hv, hmask, [&](uint32_t q) { return s.cid[q] >> KIND_SHIFT; },
-            [&](uint32_t q) { return s.cid[q] & ID_MASK; }, [&](uint32_t q) { s.ckey[q] = HIDDEN_KEY; });
+            bc, rho + COVER_PAD, gathered, n[R_NEAR], [&](uint32_t q) { return s.cid[q] >> KIND_SHIFT; },
+            [&](uint32_t q) { return s.cid[q] & ID_MASK; },
+            [&](uint32_t kd, uint32_t k, dvec4& g0, dvec4& g1) {
+                if (kd == 0u) {
+                    g0 = ((gptr<const dvec4>)A.sph)[k];
+                } else {
+                    gptr<const dvec4> cr = (gptr<const dvec4>)A.cir + 2 * (size_t)k;
+                    g0 = cr[0];
+                    g1 = cr[1];
+                }
+            },
+            [&](uint32_t slot) {
+                const NearD k = s.nq[slot];
+                return dvec4{k.x, k.y, k.z, sqrt(k.blk) - COVER_MARGIN};
+            },
+            [&](uint32_t q) { s.ckey[q] = HIDDEN_KEY; });
     }
     __syncthreads();
     auto bucket_of = [&](unsigned long long key, uint32_t id) {

@@ -321,9 +321,30 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
     if (n[C_OVER]) return false;
     const uint32_t gathered = n[C_CAND];
     // Candidates one blocker hides from the whole sub-block leave the lists.
+    // The test runs on the f32 frame-local values the lane kernel uses (the
+    // ball centre taken to that frame); they differ from the f64 records by
+    // at most about 1.2e-4 per coordinate, far inside COVER_MARGIN, so the
+    // refinement's f64 tests find these candidates blocked too.
+    double bl[3];
+    for (int a = 0; a < 3; a++) bl[a] = b[a] - (double)P.frame[a] * P.h;
     mark_hidden(
-        A, b, rho + COVER_PAD, gathered, s.hk, s.hv, hmask, [&](uint32_t q) { return s.cbk[q] < KEY_BUCKETS ? 0u : 1u; },
-        [&](uint32_t q) { return s.cid[q]; }, [&](uint32_t q) { s.cbk[q] |= HIDDEN; });
+        bl, rho + COVER_PAD, gathered, n[C_NEAR], [&](uint32_t q) { return s.cbk[q] < KEY_BUCKETS ? 0u : 1u; },
+        [&](uint32_t q) { return s.cid[q]; },
+        [&](uint32_t kd, uint32_t k, dvec4& g0, dvec4& g1) {
+            if (kd == 0u) {
+                const float4 c = load_f4(Lc.p4, k);
+                g0 = dvec4{c.x, c.y, c.z, c.w};
+            } else {
+                const float4 m = load_f4(Lc.m4, k), u = load_f4(Lc.u4, k);
+                g0 = dvec4{m.x, m.y, m.z, u.x};
+                g1 = dvec4{u.y, u.z, m.w, 0.0};
+            }
+        },
+        [&](uint32_t slot) {
+            const float4 k = s.nd[slot];
+            return dvec4{k.x, k.y, k.z, sqrt((double)k.w) - COVER_MARGIN};
+        },
+        [&](uint32_t q) { s.cbk[q] |= HIDDEN; });
     __syncthreads();
     for (uint32_t q = tid; q < gathered; q += nt)
         if (s.cbk[q] < HIDDEN) atomicAdd(&bfill[s.cbk[q]], 1u);
