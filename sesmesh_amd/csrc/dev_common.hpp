@@ -70,6 +70,7 @@ struct DArr {
     gptr<const CircleRec> cir;
     gptr<const VertexRec> ver;
     DGrid gs, gc, gv;
+    DGrid fs, fc, fv;  // the same items on cells a third as wide, for gather3
     uint32_t n_spheres, n_circles, n_vertices, pad;
     double probe, max_expanded, rho_max;
 };
@@ -499,13 +500,28 @@ template <class F>
 __device__ inline void gather3(const DArr& A, double x, double y, double z, const double r[3], uint32_t* tmp4,
                                F&& f) {
     __shared__ uint32_t rs[ROWS_MAX], rp[ROWS_MAX];
-    Box b0 = grid_box(A.gs, x, y, z, r[0]);
-    Box b1 = grid_box(A.gc, x, y, z, r[1]);
-    Box b2 = grid_box(A.gv, x, y, z, r[2]);
+    // The fine grids visit fewer items outside the ball; the coarse ones
+    // take balls whose fine rows would not fit.
+    Box b0 = grid_box(A.fs, x, y, z, r[0]);
+    Box b1 = grid_box(A.fc, x, y, z, r[1]);
+    Box b2 = grid_box(A.fv, x, y, z, r[2]);
     uint32_t n0 = (b0.z1 - b0.z0 + 1) * (b0.y1 - b0.y0 + 1);
     uint32_t n1 = (b1.z1 - b1.z0 + 1) * (b1.y1 - b1.y0 + 1);
     uint32_t n2 = (b2.z1 - b2.z0 + 1) * (b2.y1 - b2.y0 + 1);
-    const uint32_t tid = threadIdx.x, nt = blockDim.x, nrows = n0 + n1 + n2;
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    const DGrid *g0 = &A.fs, *g1 = &A.fc, *g2 = &A.fv;
+    if (n0 + n1 + n2 > ROWS_MAX || n0 + n1 + n2 > nt) {
+        g0 = &A.gs;
+        g1 = &A.gc;
+        g2 = &A.gv;
+        b0 = grid_box(A.gs, x, y, z, r[0]);
+        b1 = grid_box(A.gc, x, y, z, r[1]);
+        b2 = grid_box(A.gv, x, y, z, r[2]);
+        n0 = (b0.z1 - b0.z0 + 1) * (b0.y1 - b0.y0 + 1);
+        n1 = (b1.z1 - b1.z0 + 1) * (b1.y1 - b1.y0 + 1);
+        n2 = (b2.z1 - b2.z0 + 1) * (b2.y1 - b2.y0 + 1);
+    }
+    const uint32_t nrows = n0 + n1 + n2;
     if (nrows > ROWS_MAX || nrows > nt) {
         for_near(A.gs, x, y, z, r[0], tid, nt, [&](uint32_t k) { f(0u, k); });
         for_near(A.gc, x, y, z, r[1], tid, nt, [&](uint32_t k) { f(1u, k); });
@@ -516,7 +532,7 @@ __device__ inline void gather3(const DArr& A, double x, double y, double z, cons
     if (tid < nrows) {
         uint32_t k = tid < n0 ? 0u : (tid < n0 + n1 ? 1u : 2u);
         uint32_t q = tid - (k == 0 ? 0u : (k == 1 ? n0 : n0 + n1));
-        const DGrid& G = k == 0 ? A.gs : (k == 1 ? A.gc : A.gv);
+        const DGrid& G = k == 0 ? *g0 : (k == 1 ? *g1 : *g2);
         const Box& B = k == 0 ? b0 : (k == 1 ? b1 : b2);
         uint32_t ny = B.y1 - B.y0 + 1;
         uint32_t row = ((B.z0 + q / ny) * G.dy + (B.y0 + q % ny)) * G.dx;
@@ -540,7 +556,7 @@ __device__ inline void gather3(const DArr& A, double x, double y, double z, cons
                 hi = mid - 1;
         }
         uint32_t k = lo < n0 ? 0u : (lo < n0 + n1 ? 1u : 2u);
-        gptr<const uint32_t> items = k == 0 ? A.gs.items : (k == 1 ? A.gc.items : A.gv.items);
+        gptr<const uint32_t> items = k == 0 ? g0->items : (k == 1 ? g1->items : g2->items);
         f(k, items[rs[lo] + (i - rp[lo])]);
     }
     __syncthreads();

@@ -666,6 +666,23 @@ extern "C" int smg_resident_new(smg_device* dev, const smg_arrangement_ref* a, s
     pk.grid(a->sphere_grid, A.gs);
     pk.grid(a->circle_grid, A.gc);
     pk.grid(a->vertex_grid, A.gv);
+    // Finer grids over the same points (a third of the cell edge) for the
+    // sub-block gathers, which may visit items in any order.
+    Grid fine[3];
+    {
+        const double fcell = std::fmax(2.0 * a->max_expanded / 3.0, 1e-6);
+        std::vector<double> cpts(3 * nc);
+        for (size_t c = 0; c < nc; c++)
+            for (int q = 0; q < 3; q++) cpts[3 * c + q] = cf[q][c];
+        build_grid(a->center, n, fcell, fine[0]);
+        build_grid(cpts.data(), nc, fcell, fine[1]);
+        build_grid(a->vertex, nv, fcell, fine[2]);
+        smg_grid_ref fr[3];
+        for (int q = 0; q < 3; q++) grid_ref(fine[q], fr[q]);
+        pk.grid(fr[0], A.fs);
+        pk.grid(fr[1], A.fc);
+        pk.grid(fr[2], A.fv);
+    }
     void* p = nullptr;
     HIP_TRY(hipMalloc(&p, pk.bytes.size()));
     r->buf.ptrs.push_back(p);
