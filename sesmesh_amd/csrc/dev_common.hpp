@@ -70,7 +70,7 @@ struct DArr {
     gptr<const CircleRec> cir;
     gptr<const VertexRec> ver;
     DGrid gs, gc, gv;
-    DGrid fs, fc, fv;  // the same items on cells a third as wide, for gather3
+    DGrid fs, fc, fv;  // the same items on cells 0.4 as wide, for gather3
     uint32_t n_spheres, n_circles, n_vertices, pad;
     double probe, max_expanded, rho_max;
 };

@@ -666,11 +666,12 @@ extern "C" int smg_resident_new(smg_device* dev, const smg_arrangement_ref* a, s
     pk.grid(a->sphere_grid, A.gs);
     pk.grid(a->circle_grid, A.gc);
     pk.grid(a->vertex_grid, A.gv);
-    // Finer grids over the same points (a third of the cell edge) for the
-    // sub-block gathers, which may visit items in any order.
+    // Finer grids over the same points (0.4 of the cell edge; a third and a
+    // quarter measured no faster) for the sub-block gathers, which may
+    // visit items in any order.
     Grid fine[3];
     {
-        const double fcell = std::fmax(2.0 * a->max_expanded / 3.0, 1e-6);
+        const double fcell = std::fmax(0.8 * a->max_expanded, 1e-6);
         std::vector<double> cpts(3 * nc);
         for (size_t c = 0; c < nc; c++)
             for (int q = 0; q < 3; q++) cpts[3 * c + q] = cf[q][c];
