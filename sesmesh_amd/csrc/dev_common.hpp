@@ -451,10 +451,12 @@ __device__ inline bool arc_covered(const double b[3], double rb, const dvec4& a,
 // for class C (circle id(q)), anything else for candidates without
 // blockers; geom(kind, id, g0, g1) gives a candidate's record in the frame
 // of b (S: centre, radius; C: centre, axis x | axis y, axis z, radius);
-// blocker(slot) the near sphere at a near-list slot as (centre, its
-// blocking radius less COVER_MARGIN); mark(q) is called once per hidden
-// candidate, by one lane. 4 lanes per candidate. By the whole workgroup;
-// the caller synchronises afterwards.
+// blocker(slot) the near sphere at a near-list slot as (centre, t) with t
+// the square of its blocking radius; mark(q) is called once per hidden
+// candidate, by one lane. A covering blocker holds the projection of b
+// itself, the cap's (arc's) central point, strictly within its blocking
+// radius: that test, without a square root, comes first. 4 lanes per
+// candidate. By the whole workgroup; the caller synchronises afterwards.
 template <class Kind, class Id, class Geom, class Blocker, class Mark>
 __device__ inline void mark_hidden(const double b[3], double rb, uint32_t total, uint32_t n_near, Kind&& kind,
                                    Id&& id, Geom&& geom, Blocker&& blocker, Mark&& mark) {
@@ -465,14 +467,33 @@ __device__ inline void mark_hidden(const double b[3], double rb, uint32_t total,
         if (kd <= 1u) {
             dvec4 g0, g1 = dvec4{0.0, 0.0, 0.0, 0.0};
             geom(kd, id(q), g0, g1);
-            // A blocker farther than this from the centre cannot reach the
-            // sphere's (circle's) points within its limit.
-            const double reach = kd == 0u ? g0.w : g1.z;
-            for (uint32_t k = tid % 4; k < n_near && !covered; k += 4) {
-                const dvec4 ks = blocker(k);
-                const double ex = ks.x - g0.x, ey = ks.y - g0.y, ez = ks.z - g0.z, far = reach + ks.w;
-                if (!(ks.w > 0.0) || ex * ex + ey * ey + ez * ez >= far * far) continue;
-                covered = kd == 0u ? cap_covered(b, rb, g0, ks) : arc_covered(b, rb, g0, g1, ks);
+            // The projection of b: c + R (b - c) / |b - c|, or m + r w / |w|.
+            double px, py, pz, len2;
+            if (kd == 0u) {
+                px = b[0] - g0.x;
+                py = b[1] - g0.y;
+                pz = b[2] - g0.z;
+            } else {
+                const double yx = b[0] - g0.x, yy = b[1] - g0.y, yz = b[2] - g0.z;
+                const double al = yx * g0.w + yy * g1.x + yz * g1.y;
+                px = yx - g0.w * al;
+                py = yy - g1.x * al;
+                pz = yz - g1.y * al;
+            }
+            len2 = px * px + py * py + pz * pz;
+            // The cap needs the ball off the centre; a circle point needs w != 0.
+            if (kd == 0u ? len2 > rb * rb : len2 > 0.0) {
+                const double sc = (kd == 0u ? g0.w : g1.z) / sqrt(len2);
+                px = g0.x + px * sc;
+                py = g0.y + py * sc;
+                pz = g0.z + pz * sc;
+                for (uint32_t k = tid % 4; k < n_near && !covered; k += 4) {
+                    const dvec4 kt = blocker(k);
+                    const double ex = kt.x - px, ey = kt.y - py, ez = kt.z - pz;
+                    if (!(ex * ex + ey * ey + ez * ez < kt.w)) continue;
+                    const dvec4 ks = dvec4{kt.x, kt.y, kt.z, sqrt(kt.w) - COVER_MARGIN};
+                    covered = kd == 0u ? cap_covered(b, rb, g0, ks) : arc_covered(b, rb, g0, g1, ks);
+                }
             }
         }
         const unsigned long long bal = __ballot(covered);

@@ -315,7 +315,7 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
             },
             [&](uint32_t slot) {
                 const NearD k = s.nq[slot];
-                return dvec4{k.x, k.y, k.z, sqrt(k.blk) - COVER_MARGIN};
+                return dvec4{k.x, k.y, k.z, k.blk};
             },
             [&](uint32_t q) { s.ckey[q] = HIDDEN_KEY; });
     }

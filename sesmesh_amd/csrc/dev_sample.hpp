@@ -342,7 +342,7 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
         },
         [&](uint32_t slot) {
             const float4 k = s.nd[slot];
-            return dvec4{k.x, k.y, k.z, sqrt((double)k.w) - COVER_MARGIN};
+            return dvec4{k.x, k.y, k.z, k.w};
         },
         [&](uint32_t q) { s.cbk[q] |= HIDDEN; });
     __syncthreads();
