@@ -347,10 +347,14 @@ constexpr uint32_t KEY_BUCKETS = 16;  // key buckets per candidate class in the 
 // `slot` of row p is set for each listed sphere the near hash holds at
 // `slot`. Row p's bits are words bits[w * cc + p]. 8 lanes per row, each
 // loading 4 of its items before it probes any, so 4 loads are in flight per
-// lane. By the whole workgroup; the caller synchronises afterwards.
+// lane. keep(p, slot) may leave out a blocker that can never block row p's
+// candidate where it is evaluated. By the whole workgroup; the caller
+// synchronises afterwards.
+template <class Keep>
 __device__ inline void blocker_bits(gptr<const uint32_t> nb_items, gptr<const uint32_t> cb_items, uint32_t ns,
                                     uint32_t rows, const uint32_t* off, const uint32_t* len, const uint32_t* hk,
-                                    const uint32_t* hv, uint32_t hmask, unsigned long long* bits, uint32_t cc) {
+                                    const uint32_t* hv, uint32_t hmask, unsigned long long* bits, uint32_t cc,
+                                    Keep&& keep) {
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     for (uint32_t p = tid / 8; p < rows; p += nt / 8) {
         gptr<const uint32_t> row = (p < ns ? nb_items : cb_items) + off[p];
@@ -361,7 +365,7 @@ __device__ inline void blocker_bits(gptr<const uint32_t> nb_items, gptr<const ui
             for (int u = 0; u < 4; u++) {
                 if (it[u] == 0xffffffffu) continue;
                 const uint32_t slot = hash_get(hk, hv, hmask, it[u]);
-                if (slot != 0xffffffffu) atomicOr(&bits[(slot >> 6) * cc + p], 1ull << (slot & 63));
+                if (slot != 0xffffffffu && keep(p, slot)) atomicOr(&bits[(slot >> 6) * cc + p], 1ull << (slot & 63));
             }
         }
     }
@@ -439,6 +443,48 @@ __device__ inline bool arc_covered(const double b[3], double rb, const dvec4& a,
     if (!(h2 > 0.0) || (wp <= 0.0 && wp * wp >= L2 * h2)) return A0 < 0.0 && A0 * A0 > 4.0 * r * r * L2;
     const double cx = wy * pz - wz * py, cy = wz * px - wx * pz, cz = wx * py - wy * px;
     return A0 * W2 < 2.0 * r * (wp * sqrt(h2) - sqrt(cx * cx + cy * cy + cz * cz) * rb);
+}
+
+// The converse tests: false only if no projection of the ball (b, rb) onto
+// the candidate comes within t of the blocker centre k (t its blocking
+// radius plus COVER_MARGIN), so the blocker can never block there and
+// leaving it out of the candidate's bitset changes no test. The nearest cap
+// point to k is at |d - R| when ck - c lies within the cap's directions
+// (theta <= alpha), else at angle theta - alpha: R^2 + d^2 - 2 R d cos(theta
+// - alpha), with cos(theta - alpha) d = (v.e sqrt(h2) + |v x e| rb) / D2.
+// For the arc likewise with the in-plane angle, the whole circle (h2 <= 0)
+// reaching to r^2 + |ck - m|^2 - 2 r L.
+__device__ inline bool cap_reaches(const double b[3], double rb, const dvec4& c, const dvec4& k, double t) {
+    const double vx = b[0] - c.x, vy = b[1] - c.y, vz = b[2] - c.z;
+    const double D2 = vx * vx + vy * vy + vz * vz, h2 = D2 - rb * rb;
+    const double ex = k.x - c.x, ey = k.y - c.y, ez = k.z - c.z;
+    const double d2 = ex * ex + ey * ey + ez * ez, dot = vx * ex + vy * ey + vz * ez;
+    const double R = c.w;
+    if (!(h2 > 0.0) || (dot >= 0.0 && dot * dot >= d2 * h2)) {  // nearest point |d - R| away
+        const double lo = R - t, hi = R + t;
+        return d2 < hi * hi && !(lo > 0.0 && d2 <= lo * lo);
+    }
+    const double cx = vy * ez - vz * ey, cy = vz * ex - vx * ez, cz = vx * ey - vy * ex;
+    return (R * R + d2 - t * t) * D2 < 2.0 * R * (dot * sqrt(h2) + sqrt(cx * cx + cy * cy + cz * cz) * rb);
+}
+
+// Circle record: a = (mx, my, mz, ax), e = (ay, az, rho, -).
+__device__ inline bool arc_reaches(const double b[3], double rb, const dvec4& a, const dvec4& e, const dvec4& k,
+                                   double t) {
+    const double ux = a.w, uy = e.x, uz = e.y, r = e.z;
+    const double yx = b[0] - a.x, yy = b[1] - a.y, yz = b[2] - a.z;
+    const double al = yx * ux + yy * uy + yz * uz;
+    const double wx = yx - ux * al, wy = yy - uy * al, wz = yz - uz * al;
+    const double W2 = wx * wx + wy * wy + wz * wz, h2 = W2 - rb * rb;
+    const double fx = k.x - a.x, fy = k.y - a.y, fz = k.z - a.z;
+    const double fa = fx * ux + fy * uy + fz * uz;
+    const double px = fx - ux * fa, py = fy - uy * fa, pz = fz - uz * fa;
+    const double L2 = px * px + py * py + pz * pz, wp = wx * px + wy * py + wz * pz;
+    const double A0 = r * r + (fx * fx + fy * fy + fz * fz) - t * t;
+    if (!(h2 > 0.0) || (wp >= 0.0 && wp * wp >= L2 * h2))  // the arc holds the nearest circle point
+        return A0 < 0.0 || A0 * A0 < 4.0 * r * r * L2;
+    const double cx = wy * pz - wz * py, cy = wz * px - wx * pz, cz = wx * py - wy * px;
+    return A0 * W2 < 2.0 * r * (wp * sqrt(h2) + sqrt(cx * cx + cy * cy + cz * cz) * rb);
 }
 
 // Marks the candidates q < total hidden everywhere in the ball (b, rb). A

@@ -403,7 +403,19 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
     }
     __syncthreads();
     // Blocker bitsets of the S and C rows (cpre holds the row length).
-    blocker_bits(A.nb_items, A.cb_items, ns, ns + nc, s.coff, s.cpre, s.hk, s.hv, hmask, s.cbits, cc);
+    // With `hide`, a blocker that no projection of the ball comes near stays
+    // out of the bitset: it could never block there.
+    const double bc[3] = {bx, by, bz};
+    blocker_bits(A.nb_items, A.cb_items, ns, ns + nc, s.coff, s.cpre, s.hk, s.hv, hmask, s.cbits, cc,
+                 [&](uint32_t p, uint32_t slot) {
+                     if (!hide) return true;
+                     const NearD k = s.nq[slot];
+                     const dvec4 kc = dvec4{k.x, k.y, k.z, 0.0};
+                     const double t = sqrt(k.blk) + COVER_MARGIN, rb = rho + COVER_PAD;
+                     const double* d = s.cdat + (size_t)CDAT * p;
+                     if (p < ns) return cap_reaches(bc, rb, dvec4{d[0], d[1], d[2], d[3]}, kc, t);
+                     return arc_reaches(bc, rb, dvec4{d[0], d[1], d[2], d[3]}, dvec4{d[4], d[5], d[6], 0.0}, kc, t);
+                 });
     __syncthreads();
     return true;
 }
