@@ -506,15 +506,15 @@ __device__ inline bool arc_reaches(const double b[3], double rb, const dvec4& a,
 template <class Kind, class Id, class Geom, class Blocker, class Mark>
 __device__ inline void mark_hidden(const double b[3], double rb, uint32_t total, uint32_t n_near, Kind&& kind,
                                    Id&& id, Geom&& geom, Blocker&& blocker, Mark&& mark) {
-    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    const uint32_t tid = threadIdx.x, nt = blockDim.x, lane = lane_id();
     for (uint32_t q = tid / 4; q < total; q += nt / 4) {
         const uint32_t kd = kind(q);
-        bool covered = false;
+        bool covered = false, live = false;
+        dvec4 g0 = dvec4{0.0, 0.0, 0.0, 0.0}, g1 = g0;
+        double px = 0.0, py = 0.0, pz = 0.0;
         if (kd <= 1u) {
-            dvec4 g0, g1 = dvec4{0.0, 0.0, 0.0, 0.0};
             geom(kd, id(q), g0, g1);
             // The projection of b: c + R (b - c) / |b - c|, or m + r w / |w|.
-            double px, py, pz, len2;
             if (kd == 0u) {
                 px = b[0] - g0.x;
                 py = b[1] - g0.y;
@@ -526,24 +526,44 @@ __device__ inline void mark_hidden(const double b[3], double rb, uint32_t total,
                 py = yy - g1.x * al;
                 pz = yz - g1.y * al;
             }
-            len2 = px * px + py * py + pz * pz;
+            const double len2 = px * px + py * py + pz * pz;
             // The cap needs the ball off the centre; a circle point needs w != 0.
             if (kd == 0u ? len2 > rb * rb : len2 > 0.0) {
                 const double sc = (kd == 0u ? g0.w : g1.z) / sqrt(len2);
                 px = g0.x + px * sc;
                 py = g0.y + py * sc;
                 pz = g0.z + pz * sc;
-                for (uint32_t k = tid % 4; k < n_near && !covered; k += 4) {
-                    const dvec4 kt = blocker(k);
-                    const double ex = kt.x - px, ey = kt.y - py, ez = kt.z - pz;
-                    if (!(ex * ex + ey * ey + ez * ez < kt.w)) continue;
-                    const dvec4 ks = dvec4{kt.x, kt.y, kt.z, sqrt(kt.w) - COVER_MARGIN};
-                    covered = kd == 0u ? cap_covered(b, rb, g0, ks) : arc_covered(b, rb, g0, g1, ks);
-                }
+                live = true;
             }
         }
+        // Rounds: each lane moves on to its next blocker holding the
+        // projected centre, then every lane that found one runs its cover
+        // test in the same round, so the wavefront takes about as many
+        // rounds as its busiest lane has tests. A candidate's four lanes
+        // stop once one of them has found a cover.
+        uint32_t k = tid % 4;
+        while (true) {
+            bool pass = false;
+            if (live) {
+                for (; k < n_near; k += 4) {
+                    const dvec4 kt = blocker(k);
+                    const double ex = kt.x - px, ey = kt.y - py, ez = kt.z - pz;
+                    if (ex * ex + ey * ey + ez * ez < kt.w) {
+                        pass = true;
+                        break;
+                    }
+                }
+            }
+            if (__ballot(pass) == 0ull) break;
+            if (pass) {
+                const dvec4 kt = blocker(k);
+                const dvec4 ks = dvec4{kt.x, kt.y, kt.z, sqrt(kt.w) - COVER_MARGIN};
+                covered = kd == 0u ? cap_covered(b, rb, g0, ks) : arc_covered(b, rb, g0, g1, ks);
+                k += 4;
+            }
+            if (((__ballot(covered) >> (lane & ~3u)) & 0xfull) != 0ull) live = false;
+        }
         const unsigned long long bal = __ballot(covered);
-        const uint32_t lane = lane_id();
         if ((lane & 3u) == 0u && ((bal >> lane) & 0xfull) != 0ull) mark(q);
     }
 }
