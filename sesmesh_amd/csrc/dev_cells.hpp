@@ -77,15 +77,23 @@ __device__ inline int64_t dense_index(const DRun& P, int64_t kx, int64_t ky, int
     return (z * (int64_t)P.bdim[1] + y) * (int64_t)P.bdim[0] + x;
 }
 
+// Triangles of a brick that k_cell_emit lays out in LDS before writing
+// them; a brick with more takes the per-thread path.
+constexpr uint32_t TRI_SLOTS = 1024;
+
 // Emits the triangles of every brick at tbase[brick] (mesh/extract.rs:245-260
 // and :629-667), with vertex indices local to the brick's molecule (its
 // first vertex is that of its first brick, fbrick[m]). Neighbour bricks
 // are found in the molecule's dense index grid. A reference that does not
 // resolve sets *err; triangles at or beyond `tcap` are not written (the
 // run's triangle count exceeds the buffer and is reported by its caller).
-// With `staged`, the brick's samples, edge masks and per-word prefix counts
-// are first copied to dynamic LDS ((b+1)^3 floats, then `words` u64 and
-// u32): most triangle corners reference the brick's own edges.
+// With `staged`, the brick's samples, then the edge masks and per-word
+// prefix counts of the brick and of its seven upper neighbours, are first
+// copied to dynamic LDS ((b+1)^3 floats, then 8 `words` u64 and 8 `words`
+// u32), so every triangle corner resolves from LDS; each thread then lists
+// its cells' triangles (cell, table row) at its offset in TRI_SLOTS more
+// u32, and the threads take one listed triangle each, so a wavefront no
+// longer waits for the lane whose cells hold the most triangles.
 __global__ void __launch_bounds__(256) k_cell_emit(const Mol* __restrict__ mols, const uint32_t* bmol,
                                                    const uint32_t* fbrick, const float* values, const int32_t* keys,
                                                    const uint32_t* n_bricks, const uint32_t* tbase,
@@ -104,8 +112,8 @@ __global__ void __launch_bounds__(256) k_cell_emit(const Mol* __restrict__ mols,
         const uint32_t m = bmol[brick];
         const DRun& P = mols[m].P;
         const float* val = values + (size_t)brick * P.spb;
-        const unsigned long long* lmask = mask + (size_t)brick * words;
-        const uint32_t* lpre = wpre + (size_t)brick * words;
+        const unsigned long long* cmask = mask;  // edge masks and prefix counts, by brick
+        const uint32_t* cpre = wpre;
         int64_t key[3] = {keys[3 * brick], keys[3 * brick + 1], keys[3 * brick + 2]};
         if (threadIdx.x < 8) {  // the eight bricks whose edges the cells reference, once per brick
             const uint32_t code = threadIdx.x;
@@ -119,18 +127,22 @@ __global__ void __launch_bounds__(256) k_cell_emit(const Mol* __restrict__ mols,
             target[code] = (int32_t)tb;
             tvbase[code] = tb < 0 ? 0u : vbase[tb] - vbase[fbrick[m]];
         }
+        __syncthreads();
         if (staged) {
             float* sv = (float*)smem;
             unsigned long long* sm = (unsigned long long*)(smem + ((4 * (size_t)P.spb + 7) & ~(size_t)7));
-            uint32_t* sp = (uint32_t*)(sm + words);
+            uint32_t* sp = (uint32_t*)(sm + 8 * words);
             for (uint32_t i = threadIdx.x; i < P.spb; i += blockDim.x) sv[i] = val[i];
-            for (uint32_t w = threadIdx.x; w < words; w += blockDim.x) {
-                sm[w] = lmask[w];
-                sp[w] = lpre[w];
+            for (uint32_t i = threadIdx.x; i < 8 * words; i += blockDim.x) {
+                const int32_t tb = target[i / words];
+                if (tb >= 0) {
+                    sm[i] = mask[(size_t)tb * words + i % words];
+                    sp[i] = wpre[(size_t)tb * words + i % words];
+                }
             }
             val = sv;
-            lmask = sm;
-            lpre = sp;
+            cmask = sm;
+            cpre = sp;
         }
         __syncthreads();
         uint32_t b = P.b, ncell = b * b * b, per = (ncell + blockDim.x - 1) / blockDim.x;
@@ -141,38 +153,58 @@ __global__ void __launch_bounds__(256) k_cell_emit(const Mol* __restrict__ mols,
             mine += tab.off[cs + 1] - tab.off[cs];
         }
         uint32_t total;
-        size_t at = (size_t)tbase[brick] + block_scan_256(mine, tmp, total);
-        for (uint32_t c = c_lo; c < c_hi; c++) {
-            uint32_t x = c % b, y = (c / b) % b, z = c / (b * b);
-            uint32_t cs = cell_case(val, P.side, x, y, z);
-            for (uint32_t t = tab.off[cs]; t < tab.off[cs + 1]; t++) {
+        const uint32_t first = block_scan_256(mine, tmp, total);
+        const size_t base = (size_t)tbase[brick];
+        // The vertex of edge e of table row's cell (x, y, z).
+        auto corner = [&](uint32_t x, uint32_t y, uint32_t z, uint32_t e) {
+            uint32_t axis = e / 4, lo = tab.lower[e];
+            uint32_t l[3] = {x + (lo & 1), y + ((lo >> 1) & 1), z + ((lo >> 2) & 1)};
+            uint32_t code = 0, lc[3] = {0, 0, 0};
+            for (int a = 0; a < 3; a++) {
+                if (l[a] == b)
+                    code |= 1u << a;
+                else
+                    lc[a] = l[a];
+            }
+            uint32_t id = ((lc[2] * b + lc[1]) * b + lc[0]) * 3 + axis;
+            const int32_t tb = target[code];
+            const uint32_t at_b = staged ? code : (uint32_t)tb;  // row of cmask / cpre
+            uint32_t vi = 0;
+            if (tb < 0 || !edge_is_crossing(cmask, words, at_b, id))
+                atomicOr(err, 1u);
+            else
+                vi = tvbase[code] + edge_rank(cmask, cpre, words, at_b, id);
+            return vi;
+        };
+        if (staged && total <= TRI_SLOTS) {
+            uint32_t* slots = (uint32_t*)(cpre + 8 * words);
+            uint32_t k = first;
+            for (uint32_t c = c_lo; c < c_hi; c++) {
+                uint32_t cs = cell_case(val, P.side, c % b, (c / b) % b, c / (b * b));
+                for (uint32_t t = tab.off[cs]; t < tab.off[cs + 1]; t++) slots[k++] = (c << 10) | t;
+            }
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
+                const uint32_t c = slots[i] >> 10, t = slots[i] & 1023u;
+                const uint32_t x = c % b, y = (c / b) % b, z = c / (b * b);
+                const size_t at = base + i;
                 for (int q = 0; q < 3; q++) {
-                    uint32_t e = tab.edges[3 * t + q];
-                    uint32_t axis = e / 4, lo = tab.lower[e];
-                    uint32_t l[3] = {x + (lo & 1), y + ((lo >> 1) & 1), z + ((lo >> 2) & 1)};
-                    uint32_t code = 0, lc[3] = {0, 0, 0};
-                    for (int a = 0; a < 3; a++) {
-                        if (l[a] == b)
-                            code |= 1u << a;
-                        else
-                            lc[a] = l[a];
-                    }
-                    uint32_t id = ((lc[2] * b + lc[1]) * b + lc[0]) * 3 + axis;
-                    const int32_t tb = target[code];
-                    uint32_t vi = 0;
-                    if (code == 0) {  // the brick's own edge (masks staged in LDS)
-                        if (!edge_is_crossing(lmask, words, 0u, id))
-                            atomicOr(err, 1u);
-                        else
-                            vi = tvbase[0] + edge_rank(lmask, lpre, words, 0u, id);
-                    } else if (tb < 0 || !edge_is_crossing(mask, words, (uint32_t)tb, id)) {
-                        atomicOr(err, 1u);
-                    } else {
-                        vi = tvbase[code] + edge_rank(mask, wpre, words, (uint32_t)tb, id);
-                    }
+                    const uint32_t vi = corner(x, y, z, tab.edges[3 * t + q]);
                     if (at < tcap) tris[3 * at + q] = vi;
                 }
-                at++;
+            }
+        } else {
+            size_t at = base + first;
+            for (uint32_t c = c_lo; c < c_hi; c++) {
+                uint32_t x = c % b, y = (c / b) % b, z = c / (b * b);
+                uint32_t cs = cell_case(val, P.side, x, y, z);
+                for (uint32_t t = tab.off[cs]; t < tab.off[cs + 1]; t++) {
+                    for (int q = 0; q < 3; q++) {
+                        const uint32_t vi = corner(x, y, z, tab.edges[3 * t + q]);
+                        if (at < tcap) tris[3 * at + q] = vi;
+                    }
+                    at++;
+                }
             }
         }
         __syncthreads();

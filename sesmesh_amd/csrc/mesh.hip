@@ -1554,7 +1554,7 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     TRY(X.temp(&tbase, (size_t)nb + 1));
     // The cell kernels stage a brick's samples (and its edge masks) in LDS
     // when they fit.
-    const uint32_t cell_lds = ((4 * spb + 7) & ~7u) + 12 * words;
+    const uint32_t cell_lds = ((4 * spb + 7) & ~7u) + 8 * 12 * words + 4 * TRI_SLOTS;
     const uint32_t staged = cell_lds + 1024 <= d->prop.sharedMemPerBlock ? 1u : 0u;
     k_cell_count<<<stride_grid(nb, 1), 256, staged ? 4 * spb : 0, s>>>(b, M->values, &cnt[K_NB], tcount, staged);
     TRY(scan_u32(X, tcount, tbase, &cnt[K_NB], nb, &cnt[K_NT]));
