@@ -503,6 +503,8 @@ __device__ inline bool arc_reaches(const double b[3], double rb, const dvec4& a,
 // itself, the cap's (arc's) central point, strictly within its blocking
 // radius: that test, without a square root, comes first. 4 lanes per
 // candidate. By the whole workgroup; the caller synchronises afterwards.
+constexpr uint32_t HIDE_STEPS = 8;  // prefilter steps per round of mark_hidden
+
 template <class Kind, class Id, class Geom, class Blocker, class Mark>
 __device__ inline void mark_hidden(const double b[3], double rb, uint32_t total, uint32_t n_near, Kind&& kind,
                                    Id&& id, Geom&& geom, Blocker&& blocker, Mark&& mark) {
@@ -536,25 +538,27 @@ __device__ inline void mark_hidden(const double b[3], double rb, uint32_t total,
                 live = true;
             }
         }
-        // Rounds: each lane moves on to its next blocker holding the
-        // projected centre, then every lane that found one runs its cover
-        // test in the same round, so the wavefront takes about as many
-        // rounds as its busiest lane has tests. A candidate's four lanes
+        // Rounds: each lane moves on, by at most HIDE_STEPS blockers, to its
+        // next blocker holding the projected centre, then every lane that
+        // found one runs its cover test in the same round, so a wavefront
+        // does not pay a masked f64 test for every step of its slowest lane.
+        // The step bound keeps a round short when the near list is long and
+        // blockers pass often (coarse lattices). A candidate's four lanes
         // stop once one of them has found a cover.
         uint32_t k = tid % 4;
-        while (true) {
+        while (__ballot(live && k < n_near) != 0ull) {
             bool pass = false;
             if (live) {
-                for (; k < n_near; k += 4) {
+                for (uint32_t st = 0; st < HIDE_STEPS && k < n_near; st++) {
                     const dvec4 kt = blocker(k);
                     const double ex = kt.x - px, ey = kt.y - py, ez = kt.z - pz;
                     if (ex * ex + ey * ey + ez * ez < kt.w) {
                         pass = true;
                         break;
                     }
+                    k += 4;
                 }
             }
-            if (__ballot(pass) == 0ull) break;
             if (pass) {
                 const dvec4 kt = blocker(k);
                 const dvec4 ks = dvec4{kt.x, kt.y, kt.z, sqrt(kt.w) - COVER_MARGIN};
