@@ -190,9 +190,14 @@ def sizing_study(dev, rings, brick):
     seeded globules (100 to 50k atoms, 0.35-1.0 A) labelled by the exact
     pipeline, then the 58k-atom assembly at 0.75 and 1.0 A meshed without
     sizing, with the default placement and with placement on every tier:
-    device times and wasted gathers (retries). Meshes must be identical."""
+    device times and wasted gathers (retries). Meshes must be identical.
+    The runs use the default launch shapes (a tuning chosen for another
+    spacing can hide the effect being measured); the device's tuning is
+    restored afterwards."""
     import sesmesh_amd
 
+    tuning = dev.tuning
+    dev.tuning = None
     model = sesmesh_amd.Model(dev, seed=7)
     t = time.perf_counter()
     spec = [(2000, 0.35), (6000, 0.4), (15000, 0.5), (30000, 0.6), (4000, 0.75), (20000, 0.8), (12000, 1.0),
@@ -228,6 +233,7 @@ def sizing_study(dev, rings, brick):
     finally:
         model.policy()
         sesmesh_amd.lib().smg_device_set_model(dev._h, None)
+        dev.tuning = tuning
     return out
 
 

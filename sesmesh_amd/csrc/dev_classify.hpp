@@ -58,6 +58,7 @@ __device__ inline void try_circle_full(const DArr& A, double x, double y, double
 // Candidate list of one wavefront in LDS (the kernels below run one
 // wavefront per workgroup).
 constexpr uint32_t CL_CAP = 512;
+constexpr uint32_t DIST_STEP = 4;  // candidates ses_distance_wave checks per step
 struct CandList {
     double d[CL_CAP];
     uint32_t ref[CL_CAP];  // id | kind << 30 (0 sphere, 1 circle, 2 vertex)
@@ -126,68 +127,91 @@ __device__ __forceinline__ double ses_distance_wave(const DArr& A, double x, dou
         });
         return wave_min(best);
     }
+    // The DIST_STEP nearest unchecked candidates per step, in (distance,
+    // slot) order, each with WAVE / DIST_STEP lanes scanning its blocker row
+    // (rows are mostly short, so one candidate at a time left most lanes
+    // idle); the first free one in that order is the answer.
+    constexpr uint32_t NC = DIST_STEP, LPC = WAVE / DIST_STEP;
     for (;;) {
-        double bd = INFINITY;
-        uint32_t bo = 0xffffffffu;
-        for (uint32_t o = lane; o < count; o += WAVE)
-            if (L.d[o] < bd) {
-                bd = L.d[o];
-                bo = o;
+        double bd[NC];
+        uint32_t bo[NC];
+        for (uint32_t c = 0; c < NC; c++) {
+            double d = INFINITY;
+            uint32_t oi = 0xffffffffu;
+            for (uint32_t o = lane; o < count; o += WAVE) {
+                bool taken = false;
+                for (uint32_t e = 0; e < c; e++) taken |= o == bo[e];
+                if (!taken && L.d[o] < d) {
+                    d = L.d[o];
+                    oi = o;
+                }
             }
-        for (int m = 32; m > 0; m >>= 1) {
-            double od = __shfl_xor(bd, m, WAVE);
-            uint32_t oo = __shfl_xor(bo, m, WAVE);
-            if (od < bd || (od == bd && oo < bo)) {
-                bd = od;
-                bo = oo;
+            for (int m = 32; m > 0; m >>= 1) {
+                double od = __shfl_xor(d, m, WAVE);
+                uint32_t oo = __shfl_xor(oi, m, WAVE);
+                if (od < d || (od == d && oo < oi)) {
+                    d = od;
+                    oi = oo;
+                }
             }
+            bd[c] = d;
+            bo[c] = oi;
         }
-        if (bo == 0xffffffffu) return cutoff;
-        uint32_t ref = L.ref[bo], kind = ref >> 30, id = ref & 0x3fffffffu;
+        if (bo[0] == 0xffffffffu) return cutoff;
         __syncthreads();
-        if (lane == bo % WAVE) L.d[bo] = INFINITY;
+        for (uint32_t c = 0; c < NC; c++)
+            if (bo[c] != 0xffffffffu && lane == bo[c] % WAVE) L.d[bo[c]] = INFINITY;
         __syncthreads();
-        if (kind == 2) return bd;
-        double qx, qy, qz;
-        gptr<const uint32_t> row, end;
-        if (kind == 0) {
-            double vx = x - A.cx[id], vy = y - A.cy[id], vz = z - A.cz[id];
-            double len = sqrt(vx * vx + vy * vy + vz * vz);
-            double s = A.R[id] / len;
-            qx = A.cx[id] + vx * s;
-            qy = A.cy[id] + vy * s;
-            qz = A.cz[id] + vz * s;
-            row = A.nb_items + A.nb_off[id];
-            end = A.nb_items + A.nb_off[id + 1];
-        } else {
-            double yx = x - A.mx[id], yy = y - A.my[id], yz = z - A.mz[id];
-            double ax = A.ax[id], ay = A.ay[id], az = A.az[id];
-            double along = yx * ax + yy * ay + yz * az;
-            double wx = yx - ax * along, wy = yy - ay * along, wz = yz - az * along;
-            double wl = sqrt(wx * wx + wy * wy + wz * wz);
-            double dx, dy, dz;
-            if (wl > 1e-12) {
-                dx = wx / wl;
-                dy = wy / wl;
-                dz = wz / wl;
-            } else {
-                dx = A.e1x[id];
-                dy = A.e1y[id];
-                dz = A.e1z[id];
-            }
-            double rho = A.crho[id];
-            qx = A.mx[id] + dx * rho;
-            qy = A.my[id] + dy * rho;
-            qz = A.mz[id] + dz * rho;
-            row = A.cb_items + A.cb_off[id];
-            end = A.cb_items + A.cb_off[id + 1];
-        }
+        const uint32_t o = bo[lane / LPC];
         bool blocked = false;
-        for (gptr<const uint32_t> p = row + lane; p < end; p += WAVE) {
-            uint32_t k = *p;
-            blocked |= !(ddist(qx, qy, qz, A.cx[k], A.cy[k], A.cz[k]) >= A.R[k] - EPS);
+        if (o != 0xffffffffu) {
+            const uint32_t ref = L.ref[o], id = ref & 0x3fffffffu, kind = ref >> 30;
+            if (kind != 2) {  // vertices are never blocked
+                double qx, qy, qz;
+                gptr<const uint32_t> row, end;
+                if (kind == 0) {
+                    double vx = x - A.cx[id], vy = y - A.cy[id], vz = z - A.cz[id];
+                    double len = sqrt(vx * vx + vy * vy + vz * vz);
+                    double s = A.R[id] / len;
+                    qx = A.cx[id] + vx * s;
+                    qy = A.cy[id] + vy * s;
+                    qz = A.cz[id] + vz * s;
+                    row = A.nb_items + A.nb_off[id];
+                    end = A.nb_items + A.nb_off[id + 1];
+                } else {
+                    double yx = x - A.mx[id], yy = y - A.my[id], yz = z - A.mz[id];
+                    double ax = A.ax[id], ay = A.ay[id], az = A.az[id];
+                    double along = yx * ax + yy * ay + yz * az;
+                    double wx = yx - ax * along, wy = yy - ay * along, wz = yz - az * along;
+                    double wl = sqrt(wx * wx + wy * wy + wz * wz);
+                    double dx, dy, dz;
+                    if (wl > 1e-12) {
+                        dx = wx / wl;
+                        dy = wy / wl;
+                        dz = wz / wl;
+                    } else {
+                        dx = A.e1x[id];
+                        dy = A.e1y[id];
+                        dz = A.e1z[id];
+                    }
+                    double rho = A.crho[id];
+                    qx = A.mx[id] + dx * rho;
+                    qy = A.my[id] + dy * rho;
+                    qz = A.mz[id] + dz * rho;
+                    row = A.cb_items + A.cb_off[id];
+                    end = A.cb_items + A.cb_off[id + 1];
+                }
+                for (gptr<const uint32_t> p = row + lane % LPC; p < end; p += LPC) {
+                    uint32_t k = *p;
+                    blocked |= !(ddist(qx, qy, qz, A.cx[k], A.cy[k], A.cz[k]) >= A.R[k] - EPS);
+                }
+            }
         }
-        if (!wave_any(blocked)) return bd;
+        const unsigned long long bm = __ballot(blocked);
+        for (uint32_t c = 0; c < NC; c++) {
+            if (bo[c] == 0xffffffffu) return cutoff;
+            if (((bm >> (c * LPC)) & (LPC == 64 ? ~0ull : ((1ull << LPC) - 1ull))) == 0ull) return bd[c];
+        }
     }
 }
 
