@@ -53,6 +53,12 @@ struct alignas(64) CircleRec {
 struct alignas(32) VertexRec {
     double x, y, z, pad;
 };
+// A sphere's squared thresholds of the refinement's distance tests,
+// sqrt_threshold(R - EPS) (blocking) and sqrt_threshold(R) (inside), with R
+// the expanded radius: computed once when the arrangement is uploaded.
+struct alignas(16) SphereThr {
+    double blk, nin;
+};
 
 // The resident f64 arrangement, SoA, plus packed records of the fields the
 // gathers read together.
@@ -69,6 +75,7 @@ struct DArr {
     gptr<const SphereRec> sph;
     gptr<const CircleRec> cir;
     gptr<const VertexRec> ver;
+    gptr<const SphereThr> thr;
     DGrid gs, gc, gv;
     DGrid fs, fc, fv;  // the same items on cells 0.4 as wide, for gather3
     uint32_t n_spheres, n_circles, n_vertices, pad;
@@ -93,6 +100,11 @@ __device__ inline CircleRec load_rec(gptr<const CircleRec> p, uint32_t k) {
     gptr<const dvec4> q = (gptr<const dvec4>)p + 2 * (size_t)k;
     dvec4 a = q[0], b = q[1];
     return CircleRec{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+}
+__device__ inline SphereThr load_rec(gptr<const SphereThr> p, uint32_t k) {
+    typedef double dvec2 __attribute__((ext_vector_type(2)));
+    dvec2 v = ((gptr<const dvec2>)p)[k];
+    return SphereThr{v.x, v.y};
 }
 __device__ inline VertexRec load_rec(gptr<const VertexRec> p, uint32_t k) {
     dvec4 v = ((gptr<const dvec4>)p)[k];
@@ -262,12 +274,14 @@ __device__ inline double key_value(uint64_t k) { return __longlong_as_double((in
 // monotonic, so for every s >= 0: sqrt(s) >= m  <=>  s >= threshold. The
 // distance tests `ddist(p, c) < R` and `!(ddist(p, c) >= R - EPS)` thus
 // become `s < threshold` on the same sum of squares, without the sqrt.
-__device__ inline double sqrt_threshold(double m) {
+// Host and device give the same value: both square roots are correctly
+// rounded.
+__host__ __device__ inline double sqrt_threshold(double m) {
     if (!(m > 0.0)) return 0.0;
     double s = m * m;
     if (sqrt(s) >= m) {
         for (;;) {
-            double p = __longlong_as_double(__double_as_longlong(s) - 1);
+            double p = __builtin_bit_cast(double, __builtin_bit_cast(int64_t, s) - 1);
             if (p >= 0.0 && sqrt(p) >= m)
                 s = p;
             else
@@ -275,7 +289,7 @@ __device__ inline double sqrt_threshold(double m) {
         }
     } else {
         do {
-            s = __longlong_as_double(__double_as_longlong(s) + 1);
+            s = __builtin_bit_cast(double, __builtin_bit_cast(int64_t, s) + 1);
         } while (!(sqrt(s) >= m));
     }
     return s;

@@ -220,9 +220,10 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
                 q.x = cx;
                 q.y = cy;
                 q.z = cz;
-                q.blk = sqrt_threshold(R - EPS);
+                const SphereThr t = load_rec(A.thr, k);
+                q.blk = t.blk;
                 s.nq[at] = q;
-                s.nin[at] = sqrt_threshold(R);
+                s.nin[at] = t.nin;
                 if (cap.radii) {
                     s.nR[at] = R;
                     s.nr[at] = A.r[k];

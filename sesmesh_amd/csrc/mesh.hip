@@ -660,7 +660,11 @@ extern "C" int smg_resident_new(smg_device* dev, const smg_arrangement_ref* a, s
         crec[c] = CircleRec{cf[0][c], cf[1][c], cf[2][c], cf[3][c], cf[4][c], cf[5][c], cf[9][c], 0.0};
     std::vector<VertexRec> vrec(nv);
     for (size_t v = 0; v < nv; v++) vrec[v] = VertexRec{vxyz[0][v], vxyz[1][v], vxyz[2][v], 0.0};
+    std::vector<SphereThr> thr(n);
+    for (size_t i = 0; i < n; i++)
+        thr[i] = SphereThr{sqrt_threshold(a->expanded[i] - EPS), sqrt_threshold(a->expanded[i])};
     pk.add(srec.data(), n, &A.sph);
+    pk.add(thr.data(), n, &A.thr);
     pk.add(crec.data(), nc, &A.cir);
     pk.add(vrec.data(), nv, &A.ver);
     pk.grid(a->sphere_grid, A.gs);
