@@ -36,37 +36,65 @@ __device__ inline uint32_t cell_case(const float* val, uint32_t side, uint32_t x
     return cs;
 }
 
-// Triangles per brick: thread t takes the contiguous cells [t*per, (t+1)*per)
-// in (z, y, x) order. One workgroup per brick, striding over *n_bricks;
-// with `staged`, the brick's samples are first copied to dynamic LDS
-// ((b+1)^3 floats), each being read by up to eight cells.
-__global__ void __launch_bounds__(256) k_cell_count(uint32_t b, const float* values, const uint32_t* n_bricks,
-                                                    uint32_t* tcount, uint32_t staged) {
+// Per brick (one workgroup of 256, striding over the *n_bricks bricks),
+// from one copy of its samples (in dynamic LDS with `staged`, (b+1)^3
+// floats): crossing-edge masks (edge ids ((z*b + y)*b + x)*3 + axis, 64 per
+// word), per-word prefix counts, the brick's vertex total and per-sub-block
+// counts (mesh/extract.rs:176-216), and its triangle count from the cells'
+// Marching Cubes cases (extract.rs:245-260), which sizes the cell pass.
+__global__ void __launch_bounds__(256) k_edges(uint32_t b, const float* values, const uint32_t* n_bricks,
+                                               uint32_t words, unsigned long long* mask, uint32_t* wpre,
+                                               uint32_t* bcount, uint32_t* scount, uint32_t* tcount,
+                                               uint32_t staged) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
+    __shared__ uint32_t wc[2048];
+    __shared__ uint32_t sc[8];
     __shared__ uint32_t tmp[4];
-    __shared__ McTables tab;
-    load_tables(tab);
+    __shared__ uint16_t toff[257];
+    for (uint32_t i = threadIdx.x; i < 257; i += blockDim.x) toff[i] = c_tri_offset[i];
     const uint32_t nb = *n_bricks;
     for (uint32_t brick = blockIdx.x; brick < nb; brick += gridDim.x) {
-        const uint32_t side = b + 1, spb = side * side * side;
-        const float* gval = values + (size_t)brick * spb;
-        const float* val = gval;
+        const uint32_t side = b + 1, sub = b / 2, spb = side * side * side;
+        const float* val = values + (size_t)brick * spb;
         if (staged) {
             float* lv = (float*)smem;
-            for (uint32_t i = threadIdx.x; i < spb; i += blockDim.x) lv[i] = gval[i];
-            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < spb; i += blockDim.x) lv[i] = val[i];
             val = lv;
         }
-        uint32_t ncell = b * b * b, per = (ncell + blockDim.x - 1) / blockDim.x;
+        uint32_t nid = 3 * b * b * b, lane = lane_id(), wv = threadIdx.x / WAVE;
+        if (threadIdx.x < 8) sc[threadIdx.x] = 0;
+        __syncthreads();
+        for (uint32_t w = wv; w < words; w += blockDim.x / WAVE) {
+            uint32_t id = w * 64 + lane, x = 0, y = 0, z = 0, axis = 0;
+            bool c = id < nid && edge_crossing(val, b, side, id, x, y, z, axis);
+            unsigned long long m = __ballot(c);
+            if (c) {
+                uint32_t s = min(x / sub, 1u) + 2 * min(y / sub, 1u) + 4 * min(z / sub, 1u);
+                atomicAdd(&sc[s], 1u);
+            }
+            if (lane == 0) {
+                mask[(size_t)brick * words + w] = m;
+                wc[w] = (uint32_t)__popcll(m);
+            }
+        }
         uint32_t mine = 0;
-        for (uint32_t c = threadIdx.x * per; c < min(ncell, (threadIdx.x + 1) * per); c++) {
-            uint32_t x = c % b, y = (c / b) % b, z = c / (b * b);
-            uint32_t cs = cell_case(val, side, x, y, z);
-            mine += tab.off[cs + 1] - tab.off[cs];
+        for (uint32_t c = threadIdx.x; c < b * b * b; c += blockDim.x) {
+            const uint32_t cs = cell_case(val, side, c % b, (c / b) % b, c / (b * b));
+            mine += toff[cs + 1] - toff[cs];
         }
         uint32_t total;
         block_scan_256(mine, tmp, total);
-        if (threadIdx.x == 0) tcount[brick] = total;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t acc = 0;
+            for (uint32_t w = 0; w < words; w++) {
+                wpre[(size_t)brick * words + w] = acc;
+                acc += wc[w];
+            }
+            bcount[brick] = acc;
+            tcount[brick] = total;
+        }
+        if (threadIdx.x < 8) scount[8 * brick + threadIdx.x] = sc[threadIdx.x];
         __syncthreads();
     }
 }

@@ -39,49 +39,6 @@ __device__ inline bool edge_crossing(const float* val, uint32_t b, uint32_t side
     return (val[i0] < 0.0f) != (val[i0 + st] < 0.0f);
 }
 
-// Per brick (one workgroup of 256, striding over the *n_bricks bricks):
-// crossing-edge masks (edge ids ((z*b + y)*b + x)*3 + axis, 64 per word),
-// per-word prefix counts, brick total and per-sub-block counts
-// (mesh/extract.rs:176-216).
-__global__ void __launch_bounds__(256) k_edges(uint32_t b, const float* values, const uint32_t* n_bricks,
-                                               uint32_t words, unsigned long long* mask, uint32_t* wpre,
-                                               uint32_t* bcount, uint32_t* scount) {
-    __shared__ uint32_t wc[2048];
-    __shared__ uint32_t sc[8];
-    const uint32_t nb = *n_bricks;
-    for (uint32_t brick = blockIdx.x; brick < nb; brick += gridDim.x) {
-        const uint32_t side = b + 1, sub = b / 2;
-        const float* val = values + (size_t)brick * side * side * side;
-        uint32_t nid = 3 * b * b * b, lane = lane_id(), wv = threadIdx.x / WAVE;
-        if (threadIdx.x < 8) sc[threadIdx.x] = 0;
-        __syncthreads();
-        for (uint32_t w = wv; w < words; w += blockDim.x / WAVE) {
-            uint32_t id = w * 64 + lane, x = 0, y = 0, z = 0, axis = 0;
-            bool c = id < nid && edge_crossing(val, b, side, id, x, y, z, axis);
-            unsigned long long m = __ballot(c);
-            if (c) {
-                uint32_t s = min(x / sub, 1u) + 2 * min(y / sub, 1u) + 4 * min(z / sub, 1u);
-                atomicAdd(&sc[s], 1u);
-            }
-            if (lane == 0) {
-                mask[(size_t)brick * words + w] = m;
-                wc[w] = (uint32_t)__popcll(m);
-            }
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t acc = 0;
-            for (uint32_t w = 0; w < words; w++) {
-                wpre[(size_t)brick * words + w] = acc;
-                acc += wc[w];
-            }
-            bcount[brick] = acc;
-        }
-        if (threadIdx.x < 8) scount[8 * brick + threadIdx.x] = sc[threadIdx.x];
-        __syncthreads();
-    }
-}
-
 __device__ inline uint32_t edge_rank(const unsigned long long* mask, const uint32_t* wpre, uint32_t words,
                                      uint32_t brick, uint32_t id) {
     size_t w = (size_t)brick * words + id / 64;

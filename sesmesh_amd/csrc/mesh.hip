@@ -1399,13 +1399,16 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     // Crossing edges and vertex numbering (mesh/extract.rs:176-216, 606-627).
     const uint32_t words = (3 * b * b * b + 63) / 64;
     unsigned long long* mask;
-    uint32_t *wpre, *bcount, *scount, *vbase;
+    uint32_t *wpre, *bcount, *scount, *vbase, *tcount;
     TRY(X.temp(&mask, (size_t)nb * words));
     TRY(X.temp(&wpre, (size_t)nb * words));
     TRY(X.temp(&bcount, nb));
     TRY(X.temp(&scount, nsub));
     TRY(X.temp(&vbase, (size_t)nb + 1));
-    k_edges<<<stride_grid(nb, 1), 256, 0, s>>>(b, M->values, &cnt[K_NB], words, mask, wpre, bcount, scount);
+    TRY(X.temp(&tcount, nb));  // triangles per brick, for the cell pass
+    const uint32_t edges_staged = 4 * spb + 16384 <= d->prop.sharedMemPerBlock ? 1u : 0u;
+    k_edges<<<stride_grid(nb, 1), 256, edges_staged ? 4 * spb : 0, s>>>(b, M->values, &cnt[K_NB], words, mask, wpre,
+                                                                        bcount, scount, tcount, edges_staged);
     TRY(scan_u32(X, bcount, vbase, &cnt[K_NB], nb, &cnt[K_NV]));
     k_clamp<<<1, 64, 0, s>>>(&cnt[K_NV], E.nv, &cnt[K_FLAGS], OVER_VERTICES, nullptr, 0u);
     uint32_t *rflag, *roff, *rwork;
@@ -1553,14 +1556,12 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     TRY(rec(4));
 
     // Cell pass and assembly (mesh/extract.rs:226-265, 606-691).
-    uint32_t *tcount, *tbase;
-    TRY(X.temp(&tcount, nb));
+    uint32_t* tbase;
     TRY(X.temp(&tbase, (size_t)nb + 1));
-    // The cell kernels stage a brick's samples (and its edge masks) in LDS
-    // when they fit.
+    // The emit kernel stages a brick's samples and the edge masks it
+    // references in LDS when they fit.
     const uint32_t cell_lds = ((4 * spb + 7) & ~7u) + 8 * 12 * words + 4 * TRI_SLOTS;
     const uint32_t staged = cell_lds + 1024 <= d->prop.sharedMemPerBlock ? 1u : 0u;
-    k_cell_count<<<stride_grid(nb, 1), 256, staged ? 4 * spb : 0, s>>>(b, M->values, &cnt[K_NB], tcount, staged);
     TRY(scan_u32(X, tcount, tbase, &cnt[K_NB], nb, &cnt[K_NT]));
     k_clamp<<<1, 64, 0, s>>>(&cnt[K_NT], E.nt, &cnt[K_FLAGS], OVER_TRIANGLES, nullptr, 0u);
     HIP_TRY(hipGetLastError());
