@@ -56,6 +56,19 @@ def test_crambin_equals_cpu(device, structures, spacing):
     assert m.n_vertices == {0.5: 13812, 0.25: 55542}[spacing]
 
 
+@pytest.mark.parametrize("brick,spacing", [(18, 0.25), (32, 0.5)])
+def test_large_bricks_equal_cpu(device, structures, brick, spacing):
+    """Brick 18 is the largest whose samples and edge masks the edge and
+    cell kernels stage in LDS; at 0.25 A 51 of crambin's 233 surface bricks
+    list more triangles (up to 1,694) than the cell pass's 1,024 LDS slots
+    and take the per-thread path.
+    Brick 32 fits no staging and reads everything from global memory. The
+    mesh must equal the CPU back end's at the same brick size
+    (crates/sesmesh/src/field/gpu.rs:906-962)."""
+    o, m, g = run_both(device, structures("1CRN"), 1.4, spacing, brick)
+    assert_same_mesh(o, m, g)
+
+
 def test_synthetic_330_atom_globule_equals_cpu(device):
     xyzr = sesmesh_amd.synth_globule(1, 330)
     o, m, g = run_both(device, xyzr, 1.4, 0.5)
