@@ -49,11 +49,13 @@ def test_device_opens_with_selftest(device):
         assert device.selftest(1 << 20, seed) == (0, 0)
 
 
-@pytest.mark.parametrize("spacing", [0.5, 0.25])
+@pytest.mark.parametrize("spacing", [0.5, 0.25, 0.75, 1.0])
 def test_crambin_equals_cpu(device, structures, spacing):
+    # 0.75 and 1.0 A: larger sub-block balls, long near lists and overflow
+    # tiers in the gathers.
     o, m, g = run_both(device, structures("1CRN"), 1.4, spacing)
     assert_same_mesh(o, m, g)
-    assert m.n_vertices == {0.5: 13812, 0.25: 55542}[spacing]
+    assert m.n_vertices == {0.5: 13812, 0.25: 55542, 0.75: 6064, 1.0: 3422}[spacing]
 
 
 @pytest.mark.parametrize("brick,spacing", [(18, 0.25), (32, 0.5)])
