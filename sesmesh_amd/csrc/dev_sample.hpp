@@ -55,12 +55,15 @@ __host__ __device__ inline Layout make_layout(const Caps& c) {
 // Engine::assign's owner of sample (lx, ly, lz) (kernel.rs:495-532): the
 // first non-uniform sub-block containing it, or -1 with `positive` the sign
 // of the last uniform one visited. `st` packs the 8 states, 2 bits each.
+// The coordinates lie in [0, 2 sub], so the sub-block range along an axis
+// is hi = min(v / sub, 1) = (v >= sub) and lo = hi except on the shared
+// face v == sub, where it is 0 (v == 2 sub gives lo = hi = 1).
 __device__ inline int sample_owner(uint32_t st, uint32_t sub, uint32_t lx, uint32_t ly, uint32_t lz,
                                    bool& positive) {
     uint32_t v[3] = {lx, ly, lz}, lo[3], hi[3];
     for (int a = 0; a < 3; a++) {
-        hi[a] = min(v[a] / sub, 1u);
-        lo[a] = (v[a] > 0 && v[a] % sub == 0) ? min(v[a] / sub - 1u, 1u) : hi[a];
+        hi[a] = v[a] >= sub ? 1u : 0u;
+        lo[a] = v[a] == sub ? 0u : hi[a];
     }
     positive = true;
     for (uint32_t sz = lo[2]; sz <= hi[2]; sz++)
@@ -88,8 +91,12 @@ __global__ void k_assign_uniform(const Mol* __restrict__ mols, const uint32_t* b
         const DRun& P = mols[bmol[brick]].P;
         uint32_t s = pack_states(st + 8 * brick);
         float* out = values + (size_t)brick * P.spb;
+        // i / d as __umulhi(i, ceil(2^32 / d)): exact for i and d below 2^16.
+        const uint32_t side = P.side, plane = side * side;
+        const uint32_t m1 = (uint32_t)(((1ull << 32) + side - 1) / side);
+        const uint32_t m2 = (uint32_t)(((1ull << 32) + plane - 1) / plane);
         for (uint32_t i = threadIdx.x; i < P.spb; i += blockDim.x) {
-            uint32_t x = i % P.side, y = (i / P.side) % P.side, z = i / (P.side * P.side);
+            const uint32_t z = __umulhi(i, m2), r = i - z * plane, y = __umulhi(r, m1), x = r - y * side;
             bool positive;
             if (sample_owner(s, P.sub, x, y, z, positive) < 0) out[i] = positive ? P.fband : -P.fband;
         }
