@@ -34,13 +34,11 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "s per SES mesh (58k atoms, 0.25 Å, exact) + batched molecules/s vs plain GPU"
 
-# MI355X peaks (dense, published): FP64 vector 78.6 TFLOP/s, FP32 157.3 TFLOP/s.
+# MI355X peaks (dense, published): FP64 vector 78.6 TFLOP/s, FP32 157.3
+# TFLOP/s, HBM3E 8 TB/s.
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
-# Algorithmic operations per unit on this workload, counted by the oracle
-# over every 8th surface brick (one op per IEEE add/sub/mul/div/sqrt/min/
-# compare): used when the CPU leg is skipped; the CPU leg recounts them.
-OPS_PER_EVAL_F64 = 3161.5
-OPS_PER_SAMPLE_F32 = 2360.7
+PEAK_HBM_GBS = 8000.0
+PROFILE = "mesh_stage_58k_rings_0.25A"
 
 
 def progress(msg):
@@ -67,7 +65,9 @@ def cpu_model():
 def cpu_baseline(xyzr, probe, spacing, brick, threads, sample_stride):
     """The oracle's mesh stage on the host: full brick classification (timed)
     plus sampling and meshing of every `sample_stride`-th surface brick
-    (timed, scaled to all bricks)."""
+    (timed, scaled to all bricks; stride 1 meshes every brick). The
+    algorithmic op counts of the roofline are tallied over every 64th
+    surface brick."""
     import oracle
 
     t0 = time.perf_counter()
@@ -83,30 +83,76 @@ def cpu_baseline(xyzr, probe, spacing, brick, threads, sample_stride):
     t_mesh, verts = s.time_mesh_sample(0, count, sample_stride, threads)
     ops = s.count_ops(3, max(1, n // 64), 64)
     scaled = t_mesh * n / count
+    what = (f"sampling and meshing of all {n} surface bricks ({t_mesh:.2f} s)" if sample_stride == 1 else
+            f"sampling and meshing of every {sample_stride}th of the {n} surface bricks ({count} bricks, "
+            f"{t_mesh:.2f} s, scaled x{n / count:.2f})")
     return {
         "value": t_classify + scaled, "unit": "s/mesh", "cores": threads, "kind": "port",
         "sample": (f"oracle (C++ restatement of the reference CPU back end) on the same workload: brick "
-                   f"classification of all {s.touched} touched bricks ({t_classify:.2f} s) plus sampling and "
-                   f"meshing of every {sample_stride}th of the {n} surface bricks ({count} bricks, "
-                   f"{t_mesh:.2f} s, scaled x{n / count:.2f}); {threads} threads on {cpu_model()}"),
+                   f"classification of all {s.touched} touched bricks ({t_classify:.2f} s) plus {what}; "
+                   f"{verts} vertices; {threads} threads on {cpu_model()}"),
         "classify_s": t_classify, "mesh_sample_s": t_mesh, "sample_bricks": count, "sample_vertices": verts,
         "ops": ops,
     }
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` on this workload: rocprofv3 FETCH_SIZE
-    and WRITE_SIZE passes calibrated by a copy of known size, as committed
-    under profiles/ (None if absent)."""
-    path = ROOT / "profiles" / "mesh_stage_58k_rings_0.25A_hbm_traffic.json"
+def profile_json(what):
+    """A committed rocprofv3 summary of this workload (profiles/), or None."""
+    path = ROOT / "profiles" / f"{PROFILE}_{what}.json"
     try:
-        data = json.loads(path.read_text())
+        return json.loads(path.read_text()), str(path.relative_to(ROOT))
     except (OSError, ValueError):
         return None, None
-    for name, k in data["kernels"].items():
-        if name.split("<")[0] == kernel:
-            return k["hbm_bytes"], str(path.relative_to(ROOT))
-    return None, None
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` (its first-pass variant) on this
+    workload: rocprofv3 FETCH_SIZE and WRITE_SIZE passes calibrated by a copy
+    of known size, as committed under profiles/ (None if absent)."""
+    data, src = profile_json("hbm_traffic")
+    if not data:
+        return None, None
+    k = data["kernels"].get(kernel)
+    return (k["hbm_bytes"], src) if k else (None, None)
+
+
+def hbm_rates(ph):
+    """Achieved HBM GB/s of the classification and gather kernels: calibrated
+    bytes per launch (committed counter profile) over this run's kernel
+    time, against the 8 TB/s peak."""
+    out = {}
+    for kernel, key in (("k_classify_bricks", "k_classify_ms"), ("k_uniform<false>", "k_substates_ms"),
+                        ("k_sample<false, false>", "k_sample_ms"), ("k_refine<false, false>", "k_refine_ms")):
+        b, src = measured_traffic(kernel)
+        if b is None or not ph.get(key):
+            continue
+        gbs = b / (ph[key] * 1e-3) / 1e9
+        out[kernel] = {"bytes": b, "ms": ph[key], "GB/s": gbs, "peak": PEAK_HBM_GBS, "frac": gbs / PEAK_HBM_GBS,
+                       "source": src}
+    return out
+
+
+def valu_issue(ph):
+    """VALU issue of the sampling and refinement kernels from the committed
+    SQ counter profile (wavefront instructions per launch) over this run's
+    kernel time: instructions per CU-cycle (one wave64 instruction per
+    CU-cycle is the non-packed issue peak of a CU's four SIMDs) and lane
+    operations per second against the FP32 peak without FMA (78.65 T/s)."""
+    data, src = profile_json("sq_counters")
+    if not data:
+        return None
+    out = {}
+    cus, ghz = data.get("cus", 256), data.get("ghz", 2.4)
+    for kernel, key in (("k_sample<false, false>", "k_sample_ms"), ("k_refine<false, false>", "k_refine_ms")):
+        k = data["kernels"].get(kernel)
+        if not k or not ph.get(key):
+            continue
+        secs = ph[key] * 1e-3
+        out[kernel] = {"valu_insts": k["SQ_INSTS_VALU"], "ms": ph[key],
+                       "per_cu_cycle": k["SQ_INSTS_VALU"] / (secs * cus * ghz * 1e9),
+                       "lane_ops_per_s": 64 * k["SQ_INSTS_VALU"] / secs, "peak_lane_ops_per_s": 78.65e12,
+                       "source": src}
+    return out
 
 
 def device_copy(nbytes):
@@ -125,6 +171,41 @@ def device_copy(nbytes):
         raise RuntimeError("hipMemcpy failed")
     hip.hipFree(src)
     hip.hipFree(dst)
+
+
+def copy_out(plan, res, spacing, info, steps):
+    """Seconds per mesh of a plan replay followed by the copy of the mesh
+    into pinned host buffers: the mesh as the drop-in hands it to the
+    caller's host Mesh. Pinned buffers are allocated once (hipHostMalloc)."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    nv, nt = int(info.vertices), int(info.triangles)
+    sizes = [12 * nv, 12 * nv, 4 * nv, nv, 12 * nt]
+    ptrs = []
+    try:
+        for n in sizes:
+            p = ctypes.c_void_p()
+            if hip.hipHostMalloc(ctypes.byref(p), ctypes.c_size_t(max(n, 1)), 0) != 0:
+                raise RuntimeError("hipHostMalloc failed")
+            ptrs.append(p.value)
+        times, copies = [], []
+        for i in range(steps + 1):
+            t = time.perf_counter()
+            v = plan.run([res], [spacing])
+            t1 = time.perf_counter()
+            v.copy_to(*ptrs)
+            t2 = time.perf_counter()
+            del v
+            if i:
+                times.append(t2 - t)
+                copies.append(t2 - t1)
+    finally:
+        for p in ptrs:
+            hip.hipHostFree(ctypes.c_void_p(p))
+    c = float(np.median(copies))
+    return {"s_per_mesh": float(np.mean(times)), "copy_s": c, "bytes": int(sum(sizes)), "GB/s": sum(sizes) / c / 1e9,
+            "what": "plan replay + copy of positions, normals, atoms, patches and triangles to pinned host memory"}
 
 
 def batched(dev, count, seed, repeats=3):
@@ -244,7 +325,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch", type=int, default=128, help="molecules in the batched measurement (0: skip)")
-    ap.add_argument("--cpu-stride", type=int, default=4)
+    ap.add_argument("--cpu-stride", type=int, default=1,
+                    help="the CPU baseline meshes every Nth surface brick and scales (1: all bricks)")
     ap.add_argument("--precision", type=int, default=0, metavar="COPIES",
                     help="precision study: exact vs fast refinement on COPIES (60: the 2.4M-atom shell) "
                          "icosahedral copies of a 40k-atom globule at 1.0 A (0: skip)")
@@ -315,6 +397,27 @@ def main():
     stats = plan.stats()
     progress(f"GPU: {s_per_mesh:.4f} s/mesh over {args.steps} steps (plan replays {stats['replays']})")
     ph = {k: float(np.median([p[k] for p in phase])) for k in phase[0]}
+
+    # Untimed checks: the replayed plan never fell back to the exact path,
+    # and its mesh equals the plain path's byte for byte (samples included).
+    if stats["fallbacks"] != 0:
+        raise RuntimeError(f"plan fell back to the exact path: {stats}")
+    v = plan.run([res], [spacing])
+    got = v.arrays(values=True)
+    del v
+    m = res.mesh(spacing, brick)
+    want = m.arrays(values=True)
+    del m
+    for k in want:
+        if not np.array_equal(np.ascontiguousarray(got[k]).view(np.uint8), np.ascontiguousarray(want[k]).view(np.uint8)):
+            raise RuntimeError(f"replayed plan differs from the plain path in {k}")
+    del got, want
+    progress("replayed plan equals the plain path byte for byte")
+
+    # The drop-in's output: the stage plus the copy of the mesh into pinned
+    # host buffers (positions, normals, atoms, patches, triangles).
+    copy = copy_out(plan, res, spacing, info, args.steps)
+    progress(f"stage + copy to pinned host memory: {copy['s_per_mesh']:.4f} s/mesh ({copy['GB/s']:.1f} GB/s copy)")
     del plan
     if args.copy_mib:
         device_copy(args.copy_mib << 20)
@@ -325,30 +428,34 @@ def main():
         progress(f"CPU baseline on {host_threads()} threads")
         cpu = cpu_baseline(xyzr, probe, spacing, brick, host_threads(), args.cpu_stride)
         progress(f"CPU baseline: {cpu['value']:.2f} s/mesh")
+    # Algorithmic op counts come from the oracle's tally in the CPU leg;
+    # without it the roofline's achieved rate is not reported.
     ops = cpu["ops"] if cpu else None
-    f64_per_eval = ops["f64_ops"] / ops["evals"] if ops else OPS_PER_EVAL_F64
-    f32_per_sample = ops["f32_ops"] / ops["samples"] if ops else OPS_PER_SAMPLE_F32
+    f64_per_eval = ops["f64_ops"] / ops["evals"] if ops else None
+    f32_per_sample = ops["f32_ops"] / ops["samples"] if ops else None
 
     kernels = {
-        "k_refine": ("valu-fp64", ph["k_refine_ms"], f64_per_eval * info.refine_evaluations,
-                     "fp64"),
-        "k_sample": ("valu-fp32", ph["k_sample_ms"], f32_per_sample * info.evaluated_samples, "fp32"),
+        "k_refine<false, false>": (ph["k_refine_ms"], f64_per_eval and f64_per_eval * info.refine_evaluations,
+                                   "fp64"),
+        "k_sample<false, false>": (ph["k_sample_ms"], f32_per_sample and f32_per_sample * info.evaluated_samples,
+                                   "fp32"),
     }
-    dom = max(kernels, key=lambda k: kernels[k][1])
-    _, ms, flops, prec = kernels[dom]
-    achieved = flops / (ms * 1e-3) / 1e12
+    dom = max(kernels, key=lambda k: kernels[k][0])
+    ms, flops, prec = kernels[dom]
+    achieved = flops / (ms * 1e-3) / 1e12 if flops else None
     traffic, traffic_src = measured_traffic(dom)
     roofline = {"kernel": dom, "bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS[prec],
-                "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[prec], "traffic": traffic,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[prec] if achieved else None, "traffic": traffic,
                 "traffic_source": traffic_src,
                 "precision": prec, "ms": ms,
-                "units": (f"{f64_per_eval:.1f} algorithmic fp64 ops per evaluation x {info.refine_evaluations} "
-                          f"evaluations" if dom == "k_refine" else
+                "units": (None if not flops else
+                          f"{f64_per_eval:.1f} algorithmic fp64 ops per evaluation x {info.refine_evaluations} "
+                          f"evaluations" if dom.startswith("k_refine") else
                           f"{f32_per_sample:.1f} algorithmic fp32 ops per sample x {info.evaluated_samples} samples")}
     others = {}
-    for k, (_, kms, kfl, kp) in kernels.items():
-        a = kfl / (kms * 1e-3) / 1e12
-        others[k] = {"ms": kms, "achieved_tflops": a, "frac": a / PEAK_TFLOPS[kp], "precision": kp}
+    for k, (kms, kfl, kp) in kernels.items():
+        a = kfl / (kms * 1e-3) / 1e12 if kfl else None
+        others[k] = {"ms": kms, "achieved_tflops": a, "frac": a / PEAK_TFLOPS[kp] if a else None, "precision": kp}
 
     precision = None
     if args.precision:
@@ -395,12 +502,16 @@ def main():
                  "autotuned launch shapes"),
         "plain_path_s": plain_s, "speedup_vs_plain_path": plain_s / s_per_mesh if plain_s else None,
         "tuning": tuned["tuning"], "plan_stats": stats,
-        "phases_ms": ph, "kernels": others, "arrangement_s": t_arr,
+        "phases_ms": ph, "kernels": others, "hbm": hbm_rates(ph), "valu_issue": valu_issue(ph),
+        "stage_with_copy": copy, "arrangement_s": t_arr,
         "batched": batch, "sizing_study": sizing, "precision_study": precision, "device": dev.name,
     }
     if cpu:
         out["speedup_vs_cpu_baseline"] = cpu["value"] / s_per_mesh
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
+    # Ordered teardown: meshes, plans and residents before the device.
+    del res, arr
+    dev.close()
 
 
 if __name__ == "__main__":

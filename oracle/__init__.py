@@ -172,6 +172,15 @@ class Surface:
         s = lib().orc_time_mesh_sample(self.handle, first, count, stride, threads, ctypes.byref(v))
         return s, v.value
 
+    def count_point(self, point, rho=0.01):
+        """Algorithmic op counts at one point: the f32 lane kernel over the
+        sampling lists of the ball (point, rho) and the f64 eval_local over
+        its refinement lists: (f32 ops, f64 ops)."""
+        out = (ctypes.c_uint64 * 2)()
+        lib().orc_count_point(self.handle, (ctypes.c_double * 3)(*[float(v) for v in point]), ctypes.c_double(rho),
+                              out)
+        return int(out[0]), int(out[1])
+
     def count_ops(self, first, count, stride):
         """Algorithmic op counts over bricks first + k*stride: f32 ops and
         evaluated samples of the lane kernel, f64 ops and evaluations of the

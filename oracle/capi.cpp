@@ -253,6 +253,29 @@ double orc_time_mesh_sample(void* p, size_t first, size_t count, size_t stride, 
 // Algorithmic op counts of sampling and refining the bricks first + k *
 // stride (k < count), one thread: [f32 ops, evaluated samples, f64 ops of
 // eval_local, evaluations, vertices].
+// Op counts of one point x: the f32 lane kernel over the sampling lists of
+// the ball (x, rho) and the f64 eval_local over the refinement lists of the
+// same ball. out = [f32 ops, f64 ops].
+void orc_count_point(void* p, const double x[3], double rho, uint64_t out[2]) {
+    Surface* S = static_cast<Surface*>(p);
+    const Engine& E = S->engine;
+    BrickScratch s = E.scratch();
+    V3 c(x[0], x[1], x[2]);
+    E.gather_ses(c, rho, s);
+    V3 l = c - E.lattice.frame_origin();
+    float probe = (float)S->field.probe;
+    float cut = (float)(S->field.probe + E.band);
+    g_ops = OpCounts();
+    g_ops.on = true;
+    (void)ses_lane(s.g, (float)l.x, (float)l.y, (float)l.z, probe, cut * cut);
+    LocalCandidates lc;
+    S->field.gather_local(c, rho, E.band, lc);
+    (void)S->field.eval_local(c, E.band, lc);
+    g_ops.on = false;
+    out[0] = g_ops.f32_ops;
+    out[1] = g_ops.f64_ops;
+}
+
 void orc_count_ops(void* p, size_t first, size_t count, size_t stride, uint64_t out[5]) {
     Surface* S = static_cast<Surface*>(p);
     BrickScratch s = S->engine.scratch();

@@ -13,6 +13,7 @@ device, every call raises.
 """
 import ctypes
 import os
+import sys
 from dataclasses import dataclass
 from pathlib import Path
 
@@ -194,6 +195,19 @@ def lib():
     return _lib
 
 
+def _free(name, handle):
+    """Frees a native object. The device outlives every object made on it
+    (the library counts references), so objects may be freed in any order.
+    Nothing is called while the interpreter shuts down: module globals may
+    be gone by then, and the process's exit reclaims the device memory."""
+    try:
+        if not handle or _lib is None or sys.is_finalizing():
+            return
+        getattr(_lib, name)(handle)
+    except Exception:
+        pass
+
+
 def _check(rc):
     if rc != 0:
         raise BackendError(rc, lib().smg_last_error().decode())
@@ -242,15 +256,15 @@ class Device:
         return m32.value, m64.value
 
     def close(self):
-        if self._h:
-            lib().smg_device_close(self._h)
+        """Drops this handle; the device itself lives on until every object
+        made on it is freed."""
+        h = getattr(self, "_h", None)
+        if h:
             self._h = _vp()
+            _free("smg_device_close", h)
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        self.close()
 
 
 class Arrangement:
@@ -294,8 +308,8 @@ class Arrangement:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            lib().smg_arrangement_free(h)
             self._h = _vp()
+            _free("smg_arrangement_free", h)
 
 
 class Resident:
@@ -326,8 +340,8 @@ class Resident:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            lib().smg_resident_free(h)
             self._h = _vp()
+            _free("smg_resident_free", h)
 
 
 def _mesh_params(spacing, brick, refine, sizing):
@@ -391,11 +405,18 @@ class Mesh:
             out["values"] = vals
         return out
 
+    def copy_to(self, positions, normals, atoms, patches, triangles, part=0):
+        """Copies a part's mesh into caller-owned host buffers given as raw
+        addresses (e.g. pinned memory): positions and normals 3 f32 per
+        vertex, atoms u32, patches u8, triangles 3 u32 each."""
+        _check(lib().smg_mesh_part_copy(self._h, part, *[_vp(a) for a in (positions, normals, atoms, patches,
+                                                                            triangles)], None, None, None, None))
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            lib().smg_mesh_free(h)
             self._h = _vp()
+            _free("smg_mesh_free", h)
 
 
 def mesh_many(residents, spacings, brick=10, refine="exact", sizing=0):
@@ -445,8 +466,8 @@ class Plan:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            lib().smg_plan_free(h)
             self._h = _vp()
+            _free("smg_plan_free", h)
 
 
 class Model:
@@ -525,8 +546,8 @@ class Model:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            lib().smg_model_free(h)
             self._h = _vp()
+            _free("smg_model_free", h)
 
 
 def tune(residents, spacings, brick=10, refine="exact", repeats=2, bricks=None):

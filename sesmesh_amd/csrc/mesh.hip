@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -269,13 +270,51 @@ struct smg_device {
     std::mutex mu;
     smg_tuning tune{0, 0, 64, 1, 0};  // launch shapes of the next runs (smg_tune)
     smg_model* model = nullptr;       // sizing model of runs with sizing 1
+    // Static LDS (the __shared__ arrays) of the kernels whose dynamic LDS
+    // depends on the brick or the list tier, read once (hipFuncGetAttributes).
+    uint32_t static_lds[8] = {};
+    bool static_ready = false;
+    // Holders: the handle itself and every resident, mesh, plan and model
+    // made on the device. smg_device_close drops the handle's reference;
+    // the device is destroyed when the last holder is freed, so objects may
+    // be freed in any order relative to the device.
+    std::atomic<int> refs{1};
+};
+
+static void device_destroy(smg_device* d);
+
+static void device_retain(smg_device* d) {
+    if (d) d->refs.fetch_add(1, std::memory_order_relaxed);
+}
+static void device_release(smg_device* d) {
+    if (d && d->refs.fetch_sub(1, std::memory_order_acq_rel) == 1) device_destroy(d);
+}
+
+// A counted reference to a device, the first member of every object made
+// on one (so it is released after the object's own destructor has freed
+// its device memory).
+struct DevRef {
+    smg_device* p = nullptr;
+    DevRef() = default;
+    DevRef(const DevRef& o) : p(o.p) { device_retain(p); }
+    DevRef& operator=(const DevRef& o) { return *this = o.p; }
+    DevRef& operator=(smg_device* d) {
+        device_retain(d);
+        device_release(p);
+        p = d;
+        return *this;
+    }
+    ~DevRef() { device_release(p); }
+    operator smg_device*() const { return p; }
+    smg_device* operator->() const { return p; }
 };
 
 // The sizing model: FP32 master parameters, Adam moments, BF16 copies of
 // W1 and W2 for the matrix cores, and its labelled training set (features
 // of touched bricks, occupancy, log candidate load).
 struct smg_model {
-    smg_device* dev = nullptr;
+    DevRef dev;
+    uint64_t id = 0;  // unique per model of the process (plans recorded with it check it)
     float *P = nullptr, *Mo = nullptr, *Ve = nullptr;
     uint16_t *W1b = nullptr, *W2b = nullptr;
     uint64_t step = 0;
@@ -310,7 +349,7 @@ struct DevBuf {
 };
 
 struct smg_resident {
-    smg_device* dev = nullptr;
+    DevRef dev;
     DArr A{};
     DevBuf buf;
     // Host copies needed to set up a run: centre bounding box.
@@ -322,7 +361,7 @@ struct smg_resident {
 // device arrays; part i owns bricks [bfirst[i], bfirst[i+1]), vertices
 // [vfirst[i], vfirst[i+1]) and triangles [tfirst[i], tfirst[i+1]).
 struct smg_mesh {
-    smg_device* dev = nullptr;
+    DevRef dev;
     uint32_t spb = 0;  // samples per brick
     std::vector<smg_mesh_info> info;
     std::vector<uint32_t> bfirst, vfirst, tfirst;
@@ -401,6 +440,14 @@ extern "C" int smg_device_open(int ordinal, smg_device** out, smg_device_info* i
     if (info) {
         std::memset(info, 0, sizeof(*info));
         std::snprintf(info->name, sizeof(info->name), "%s", d->prop.name);
+        if (!info->name[0]) {  // some runtimes leave the marketing name empty
+            char nm[256] = {};
+            if (hipDeviceGetName(nm, sizeof(nm) - 1, ordinal) == hipSuccess && nm[0])
+                std::snprintf(info->name, sizeof(info->name), "%s", nm);
+            else
+                std::snprintf(info->name, sizeof(info->name), "AMD %s, %d CUs, %.0f GiB", d->prop.gcnArchName,
+                              d->prop.multiProcessorCount, (double)d->prop.totalGlobalMem / (1u << 30));
+        }
         std::snprintf(info->arch, sizeof(info->arch), "%s", d->prop.gcnArchName);
         info->compute_units = d->prop.multiProcessorCount;
         info->clock_khz = d->prop.clockRate;
@@ -412,8 +459,9 @@ extern "C" int smg_device_open(int ordinal, smg_device** out, smg_device_info* i
     return SMG_OK;
 }
 
-extern "C" void smg_device_close(smg_device* d) {
-    if (!d) return;
+extern "C" void smg_device_close(smg_device* d) { device_release(d); }
+
+static void device_destroy(smg_device* d) {
     (void)hipSetDevice(d->ordinal);
     for (auto& e : d->ev)
         if (e) (void)hipEventDestroy(e);
@@ -817,7 +865,6 @@ constexpr RCaps UNIFORM_CAPS[UNIFORM_TIERS] = {
     {128, 64, 192, 256, 0, 1, 0}, {192, 128, 256, 512, 0, 1, 0}, {384, 256, 640, 1024, 0, 1, 0}};
 constexpr uint32_t LDS_EDGES = 512;      // crossing-edge buffer of the LDS passes
 constexpr uint32_t SCRATCH_CHUNK = 1024;  // default workgroups of a scratch pass (one list slice each)
-constexpr uint32_t STATIC_LDS = 6144;     // static LDS of k_sample / k_refine, bound
 constexpr uint32_t HID_PER_SUB = 64;      // hidden-candidate pool entries per sampled sub-block
 constexpr uint32_t NO_CAP = 0xffffffffu;
 
@@ -840,6 +887,81 @@ constexpr float SAMPLE_LOAD_CAP[TIERS] = {192.0f, 256.0f, 384.0f, 768.0f, 2048.0
 constexpr float REFINE_LOAD_CAP[TIERS] = {192.0f, 256.0f, 384.0f, 704.0f, 2048.0f};
 // K_FLAGS bits: a device count exceeded its capacity.
 enum : uint32_t { OVER_BRICKS = 1, OVER_VERTICES = 2, OVER_TRIANGLES = 4, OVER_TABLE = 8 };
+
+// Kernels whose LDS request depends on the brick or the list tier; a
+// launch needs its dynamic bytes plus the kernel's static bytes within the
+// device's per-workgroup LDS.
+enum : int { LK_EDGES, LK_CELLS, LK_UNIFORM, LK_SAMPLE, LK_REFINE, LK_REFINE_WIDE, LK_REFINE_FAST, LK_REFINE_FAST_WIDE, LK_N };
+static_assert(LK_N == 8, "smg_device::static_lds holds every kind");
+
+// Which LDS variants a run at brick b takes: the edge and cell kernels
+// stage a brick in LDS when it fits, and each LDS list tier runs only when
+// it fits (the global-scratch tier always runs).
+struct LdsPlan {
+    uint32_t edges_dyn, cells_dyn, edges_staged, cells_staged;
+    uint32_t uniform_dyn[UNIFORM_TIERS], uniform_fits[UNIFORM_TIERS];
+    uint32_t sample_dyn[TIERS], sample_fits[TIERS];
+    uint32_t refine_dyn[TIERS], refine_fits[TIERS];
+};
+
+LdsPlan lds_plan(uint32_t b, bool fast, uint64_t lds_max, const uint32_t st[LK_N]) {
+    LdsPlan L{};
+    const uint32_t spb = (b + 1) * (b + 1) * (b + 1);
+    const uint32_t words = (3 * b * b * b + 63) / 64;
+    auto fits = [&](uint64_t dyn, int k) { return (uint32_t)(dyn + st[k] <= lds_max); };
+    L.edges_dyn = 4 * spb;
+    L.edges_staged = fits(L.edges_dyn, LK_EDGES);
+    L.cells_dyn = ((4 * spb + 7) & ~7u) + 8 * 12 * words + 4 * TRI_SLOTS;
+    L.cells_staged = fits(L.cells_dyn, LK_CELLS);
+    for (int u = 0; u < UNIFORM_TIERS; u++) {
+        L.uniform_dyn[u] = make_rlayout(UNIFORM_CAPS[u]).bytes;
+        L.uniform_fits[u] = fits(L.uniform_dyn[u], LK_UNIFORM);
+    }
+    const uint32_t sub_edges = 3 * (b / 2) * (b / 2) * (b / 2);
+    for (int p = 0; p < TIERS; p++) {
+        const bool lds = p < TIERS - 1;
+        L.sample_dyn[p] = lds ? make_layout(SAMPLE_CAPS[p]).bytes : 0;
+        L.sample_fits[p] = lds ? fits(L.sample_dyn[p], LK_SAMPLE) : 1;
+        RCaps cap = REFINE_CAPS[p];
+        cap.edges = lds ? std::min(sub_edges, LDS_EDGES) : sub_edges;
+        RLayout rl = make_rlayout(cap);
+        L.refine_dyn[p] = lds ? (fast ? make_flayout(cap, rl.bytes).bytes : rl.bytes) : 0;
+        const int kind = fast ? (p >= WIDE_FROM ? LK_REFINE_FAST_WIDE : LK_REFINE_FAST)
+                              : (p >= WIDE_FROM ? LK_REFINE_WIDE : LK_REFINE);
+        L.refine_fits[p] = lds ? fits(L.refine_dyn[p], kind) : 1;
+    }
+    return L;
+}
+
+// The static LDS of each kind on this device (the largest of the kind's
+// launched variants).
+int device_statics(smg_device* d, const uint32_t** out) {
+    std::lock_guard<std::mutex> lk(d->mu);
+    if (!d->static_ready) {
+        const void* kinds[LK_N][2] = {
+            {(const void*)k_edges, (const void*)k_edges},
+            {(const void*)k_cell_emit, (const void*)k_cell_emit},
+            {(const void*)k_uniform<false>, (const void*)k_uniform<true>},
+            {(const void*)k_sample<false, false>, (const void*)k_sample<false, true>},
+            {(const void*)k_refine<false, false>, (const void*)k_refine<false, true>},
+            {(const void*)k_refine_wide<false, false>, (const void*)k_refine_wide<false, true>},
+            {(const void*)k_refine_fast<false, false>, (const void*)k_refine_fast<false, true>},
+            {(const void*)k_refine_fast_wide<false, false>, (const void*)k_refine_fast_wide<false, true>}};
+        for (int k = 0; k < LK_N; k++) {
+            uint32_t m = 0;
+            for (const void* f : kinds[k]) {
+                hipFuncAttributes a{};
+                if (hipFuncGetAttributes(&a, f) != hipSuccess)
+                    return fail(SMG_E_CALL, "hipFuncGetAttributes failed for a mesh-stage kernel");
+                m = std::max(m, (uint32_t)a.sharedSizeBytes);
+            }
+            d->static_lds[k] = m;
+        }
+        d->static_ready = true;
+    }
+    *out = d->static_lds;
+    return SMG_OK;
+}
 
 // Run-wide extents the kernels read from the uploaded table.
 struct RunHdr {
@@ -1147,6 +1269,9 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     const smg_tuning tune = d->tune;
     const uint32_t scratch_wgs = tune.scratch_workgroups > 0 ? (uint32_t)tune.scratch_workgroups : SCRATCH_CHUNK;
     M->spb = spb;
+    const uint32_t* statics;
+    TRY(device_statics(d, &statics));
+    const LdsPlan LP = lds_plan(b, fast, d->prop.sharedMemPerBlock, statics);
     auto rec = [&](int i) -> int {
         if (timed) HIP_TRY(hipEventRecord(d->ev[i], s));
         return SMG_OK;
@@ -1251,13 +1376,12 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     // overflow them all.
     TRY(rec(8));
     {
-        const size_t lds_max = d->prop.sharedMemPerBlock;
         const uint32_t* list = nullptr;  // the first pass takes every surface brick
         const uint32_t* count = &cnt[K_NB];
         bool first = true;
         for (int u = 0; u < UNIFORM_TIERS && tune.uniform_shared; u++) {
             RLayout lay = make_rlayout(UNIFORM_CAPS[u]);
-            if (lay.bytes + STATIC_LDS > lds_max) continue;
+            if (!LP.uniform_fits[u]) continue;
             uint32_t* over;
             TRY(X.temp(&over, nb));
             Queue q{list, count, &cnt[K_Q_UNIFORM0 + u]};
@@ -1296,7 +1420,6 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     TRY(scan_u32(X, sflag, soff, &cnt[K_NSUB], nsub, &cnt[K_NSW]));
     k_compact_index<<<stride_grid(nsub), 256, 0, s>>>(sflag, soff, &cnt[K_NSUB], swork);
     HIP_TRY(hipGetLastError());
-    const size_t lds_max = d->prop.sharedMemPerBlock;
     // Passes over the LDS tiers from the tuning's first one that fit, then
     // global scratch; each takes a list: sub-blocks predicted for it (all
     // of them in the first pass without learned sizing) followed by those
@@ -1320,8 +1443,7 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     {
         bool runs[TIERS];
         for (int p = 0; p < TIERS; p++) {
-            Layout lay = make_layout(SAMPLE_CAPS[p]);
-            runs[p] = p == TIERS - 1 || (p >= tune.sample_first_tier && lay.bytes + STATIC_LDS <= lds_max);
+            runs[p] = p == TIERS - 1 || (p >= tune.sample_first_tier && LP.sample_fits[p]);
         }
         int first = 0;
         while (!runs[first]) first++;
@@ -1406,7 +1528,7 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     TRY(X.temp(&scount, nsub));
     TRY(X.temp(&vbase, (size_t)nb + 1));
     TRY(X.temp(&tcount, nb));  // triangles per brick, for the cell pass
-    const uint32_t edges_staged = 4 * spb + 16384 <= d->prop.sharedMemPerBlock ? 1u : 0u;
+    const uint32_t edges_staged = LP.edges_staged;
     k_edges<<<stride_grid(nb, 1), 256, edges_staged ? 4 * spb : 0, s>>>(b, M->values, &cnt[K_NB], words, mask, wpre,
                                                                         bcount, scount, tcount, edges_staged);
     TRY(scan_u32(X, bcount, vbase, &cnt[K_NB], nb, &cnt[K_NV]));
@@ -1448,7 +1570,7 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
             lays[p] = make_rlayout(cap);
             flays[p] = make_flayout(cap, lays[p].bytes);
             bytes_of[p] = fast ? flays[p].bytes : lays[p].bytes;
-            runs[p] = !lds || (p >= tune.refine_first_tier && bytes_of[p] + STATIC_LDS <= lds_max);
+            runs[p] = !lds || (p >= tune.refine_first_tier && LP.refine_fits[p]);
         }
         int first = 0;
         while (!runs[first]) first++;
@@ -1560,8 +1682,8 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     TRY(X.temp(&tbase, (size_t)nb + 1));
     // The emit kernel stages a brick's samples and the edge masks it
     // references in LDS when they fit.
-    const uint32_t cell_lds = ((4 * spb + 7) & ~7u) + 8 * 12 * words + 4 * TRI_SLOTS;
-    const uint32_t staged = cell_lds + 1024 <= d->prop.sharedMemPerBlock ? 1u : 0u;
+    const uint32_t cell_lds = LP.cells_dyn;
+    const uint32_t staged = LP.cells_staged;
     TRY(scan_u32(X, tcount, tbase, &cnt[K_NB], nb, &cnt[K_NT]));
     k_clamp<<<1, 64, 0, s>>>(&cnt[K_NT], E.nt, &cnt[K_FLAGS], OVER_TRIANGLES, nullptr, 0u);
     HIP_TRY(hipGetLastError());
@@ -1930,6 +2052,8 @@ extern "C" int smg_model_new(smg_device* dev, uint64_t seed, smg_model** out) {
     HIP_TRY(hipSetDevice(dev->ordinal));
     std::unique_ptr<smg_model> m(new smg_model());
     m->dev = dev;
+    static std::atomic<uint64_t> next_id{1};
+    m->id = next_id.fetch_add(1);
     HIP_TRY(hipMalloc(&m->P, MLP_PARAMS * 4));
     HIP_TRY(hipMalloc(&m->Mo, MLP_PARAMS * 4));
     HIP_TRY(hipMalloc(&m->Ve, MLP_PARAMS * 4));
@@ -2123,7 +2247,7 @@ extern "C" int smg_device_set_model(smg_device* d, smg_model* m) {
 
 // ------------------------------------------------------------------ plans
 struct smg_plan {
-    smg_device* dev = nullptr;
+    DevRef dev;
     smg_plan_caps caps{};
     std::vector<void*> slots;
     std::vector<size_t> slot_bytes;
@@ -2135,7 +2259,11 @@ struct smg_plan {
     smg_plan_stats stats{};
     std::vector<smg_mesh*> views;  // meshes viewing the plan's buffers
     float* lbuf = nullptr;  // slot of the f32 copies (the table image points into it)
-    smg_model* model = nullptr;  // sizing model the graph was recorded with
+    // Sizing the graph was recorded with (model id, 0 for none, and its
+    // placement policy, which the graph holds as kernel arguments).
+    uint64_t model_id = 0;
+    float model_scale = 0.0f;
+    uint32_t model_max_tier = 0;
     bool timed = false;          // the graph records the phase events
     void* out[9] = {};      // slots of the mesh arrays: keys, values, pos, nrm, atom, patch, t, evals, tris
     void release() {
@@ -2144,7 +2272,9 @@ struct smg_plan {
         exec = nullptr;
         graph = nullptr;
         lbuf = nullptr;
-        model = nullptr;
+        model_id = 0;
+        model_scale = 0.0f;
+        model_max_tier = 0;
         timed = false;
         for (void*& o : out) o = nullptr;
         for (void* p : slots) (void)hipFree(p);
@@ -2307,8 +2437,18 @@ extern "C" int smg_plan_run(smg_plan* p, smg_resident* const* residents, const s
     HIP_TRY(hipSetDevice(d->ordinal));
     Sizing sz;
     TRY(run_sizing(d, params, count, sz));
-    if (p->exec && sz.predict != p->model) return fail(SMG_E_INVALID, "the plan was recorded with another sizing");
-    p->model = sz.predict;
+    {  // a graph recorded with another model or policy is recorded again
+        const uint64_t id = sz.predict ? sz.predict->id : 0;
+        const float scale = sz.predict ? sz.predict->scale : 0.0f;
+        const uint32_t tier = sz.predict ? sz.predict->max_tier : 0;
+        if (p->exec && (id != p->model_id || scale != p->model_scale || tier != p->model_max_tier)) {
+            HIP_TRY(hipStreamSynchronize(d->stream));
+            p->release();
+        }
+        p->model_id = id;
+        p->model_scale = scale;
+        p->model_max_tier = tier;
+    }
     p->stats.runs++;
     Table t;
     uint32_t flags = 0;
@@ -2439,6 +2579,25 @@ extern "C" int smg_mesh_copy(const smg_mesh* m, float* positions, float* normals
 }
 
 // The generated Marching Cubes table (for the tests).
+// Test helpers: the LDS plan of a brick for given static LDS sizes (the
+// LdsPlan fields as 30 u32), and the static sizes this device reads.
+extern "C" int smg_lds_plan(uint32_t brick, int fast, uint64_t lds_max, const uint32_t* statics, uint32_t* out) {
+    static_assert(sizeof(LdsPlan) == 30 * sizeof(uint32_t), "LdsPlan is 30 u32");
+    if (!statics || !out || brick < 4 || brick > 32 || brick % 2) return fail(SMG_E_INVALID, "bad arguments");
+    LdsPlan L = lds_plan(brick, fast != 0, lds_max, statics);
+    std::memcpy(out, &L, sizeof(L));
+    return SMG_OK;
+}
+
+extern "C" int smg_device_static_lds(smg_device* d, uint32_t* out) {
+    if (!d || !out) return fail(SMG_E_INVALID, "null device or output");
+    HIP_TRY(hipSetDevice(d->ordinal));
+    const uint32_t* st;
+    TRY(device_statics(d, &st));
+    std::memcpy(out, st, LK_N * sizeof(uint32_t));
+    return SMG_OK;
+}
+
 extern "C" int smg_mc_table(uint16_t offset[257], uint8_t* edges, size_t cap) {
     std::call_once(g_table_once, [] { make_mc_table(g_tri_offset, g_tri_edges); });
     std::memcpy(offset, g_tri_offset, sizeof(g_tri_offset));

@@ -1,0 +1,67 @@
+"""The headline workload against the oracle, bit for bit, at full size.
+
+The 58k-atom two-ring assembly (14 copies of a seeded 4.1k-atom globule in
+two stacked 7-fold rings), SES probe 1.4 A, spacing 0.25 A, brick 10, exact
+refinement: the configuration the speed is quoted on. Meshed on the device
+through the plain path (smg_mesh_new) and through a capture-ready plan
+replayed as a graph, and compared with the oracle's CPU mesh stage: surface
+brick keys, every sample, vertex positions, normals, atoms, patches, edge
+parameters and evaluation counts, and triangles. These are the reference's
+acceptance checks for its device path (crates/sesmesh/src/field/gpu.rs:
+846-903 samples, 906-962 meshes), applied to every brick rather than a
+sample. The hidden-candidate cover margin and the lower-bound pruning of the
+refinement rest on margin arguments; this is their test at the size the
+bench runs.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import sesmesh_amd
+from conftest import cpu_threads
+
+pytestmark = pytest.mark.gpu
+
+PROBE, SPACING, BRICK = 1.4, 0.25, 10
+
+
+@pytest.fixture(scope="module")
+def assembly():
+    xyzr = sesmesh_amd.synth_rings(seed=2)
+    assert len(xyzr) == 57400
+    print(f"[config2] oracle on {cpu_threads()} threads", flush=True)
+    o = oracle.Surface(xyzr, PROBE, SPACING, BRICK, cpu_threads())
+    keys = o.bricks()
+    ref = {"bricks": keys, "values": o.brick_values(keys), **o.mesh()}
+    print(f"[config2] oracle: {o.n_vertices} vertices, {o.n_triangles} triangles", flush=True)
+    return xyzr, o, ref
+
+
+def assert_equal_to_oracle(o, ref, m, g):
+    assert m.info.touched_bricks == o.touched
+    assert m.n_bricks == o.n_bricks and m.n_vertices == o.n_vertices and m.n_triangles == o.n_triangles
+    assert np.array_equal(g["bricks"], ref["bricks"])
+    assert np.array_equal(g["values"].view(np.uint32), ref["values"].view(np.uint32))
+    for k in ("positions", "normals"):
+        assert np.array_equal(g[k].view(np.uint32), ref[k].view(np.uint32)), k
+    for k in ("atoms", "patches", "triangles", "evals"):
+        assert np.array_equal(g[k], ref[k]), k
+    assert np.array_equal(g["t"].view(np.uint64), ref["t"].view(np.uint64))
+    assert m.info.refine_evaluations == o.evals
+
+
+def test_assembly_plain_and_replayed_plan_equal_cpu(device, assembly):
+    xyzr, o, ref = assembly
+    res = sesmesh_amd.Resident(device, sesmesh_amd.Arrangement(xyzr, PROBE))
+    m = res.mesh(SPACING, BRICK)
+    assert m.n_vertices == 6588892 and m.n_triangles == 13179908
+    assert_equal_to_oracle(o, ref, m, m.arrays(values=True))
+    plan = sesmesh_amd.Plan(device, sesmesh_amd.Plan.caps_for([res], [SPACING], brick=BRICK, mesh=m, margin=0.05))
+    del m
+    for run in range(3):  # the first run records the graph, the later ones replay it
+        v = plan.run([res], [SPACING])
+        if run:
+            assert_equal_to_oracle(o, ref, v, v.arrays(values=True))
+        del v
+    st = plan.stats()
+    assert st["fallbacks"] == 0 and st["replays"] == 2 and st["captures"] == 1

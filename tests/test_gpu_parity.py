@@ -58,14 +58,15 @@ def test_crambin_equals_cpu(device, structures, spacing):
     assert m.n_vertices == {0.5: 13812, 0.25: 55542, 0.75: 6064, 1.0: 3422}[spacing]
 
 
-@pytest.mark.parametrize("brick,spacing", [(18, 0.25), (32, 0.5)])
+@pytest.mark.parametrize("brick,spacing", [(18, 0.25), (24, 0.5), (26, 0.5), (32, 0.5)])
 def test_large_bricks_equal_cpu(device, structures, brick, spacing):
-    """Brick 18 is the largest whose samples and edge masks the edge and
-    cell kernels stage in LDS; at 0.25 A 51 of crambin's 233 surface bricks
-    list more triangles (up to 1,694) than the cell pass's 1,024 LDS slots
-    and take the per-thread path.
-    Brick 32 fits no staging and reads everything from global memory. The
-    mesh must equal the CPU back end's at the same brick size
+    """The cell pass stages a brick's samples and edge masks in LDS up to
+    brick 24 (brick 26 needs 161,936 dynamic bytes, which with the kernel's
+    static tables exceed a workgroup's 160 KiB, so it runs unstaged); the
+    edge pass stages its samples up to brick 32. At brick 18 and 0.25 A, 51
+    of crambin's 233 surface bricks list more triangles (up to 1,694) than
+    the cell pass's 1,024 LDS slots and take the per-thread path. The mesh
+    must equal the CPU back end's at the same brick size
     (crates/sesmesh/src/field/gpu.rs:906-962)."""
     o, m, g = run_both(device, structures("1CRN"), 1.4, spacing, brick)
     assert_same_mesh(o, m, g)
@@ -127,3 +128,48 @@ def test_structures_at_quarter_angstrom_equal_cpu(device, structures, golden, en
     assert_same_mesh(o, m, g)
     r = [x for x in golden["runs"] if x["campaign"] == "acc" and x["entry"] == entry and x["spacing"] == 0.25][0]
     assert m.n_vertices == r["vertices"] and m.n_triangles == r["triangles"]
+
+
+def test_device_reads_static_lds_of_its_kernels(device, tmp_path):
+    """The LDS plan of every run uses the static LDS the runtime reports for
+    the loaded kernels: it equals the code object's kernel descriptors."""
+    import ctypes
+
+    from test_lds_budget import KINDS, kernel_statics
+
+    want = kernel_statics(tmp_path)
+    got = (ctypes.c_uint32 * len(KINDS))()
+    assert sesmesh_amd.lib().smg_device_static_lds(device._h, got) == 0
+    assert list(got) == want
+
+
+def test_objects_outlive_a_closed_device_handle(structures):
+    """The device lives until the last object made on it is freed: closing
+    the handle first, then copying a mesh and freeing in any order works."""
+    xyzr = structures("1CRN")
+    dev = sesmesh_amd.Device(0)
+    res = sesmesh_amd.Resident(dev, sesmesh_amd.Arrangement(xyzr, 1.4))
+    m = res.mesh(0.5, 10)
+    dev.close()
+    g = m.arrays()
+    assert len(g["positions"]) == m.n_vertices == 13812
+    m2 = res.mesh(0.5, 10)
+    del res
+    assert np.array_equal(m2.arrays()["triangles"], g["triangles"])
+    del m, m2
+
+
+def test_interpreter_exit_with_live_objects():
+    """Objects left alive at interpreter exit (device closed first, or not
+    at all) end the process cleanly."""
+    import subprocess
+    import sys
+
+    code = ("import sys; sys.path.insert(0, %r); import sesmesh_amd as s, numpy as np\n"
+            "d = s.Device(0); x = s.synth_globule(1, 330)\n"
+            "r = s.Resident(d, s.Arrangement(x, 1.4)); m = r.mesh(0.5, 10)\n"
+            "p = s.Plan(d, s.Plan.caps_for([r], [0.5], mesh=m)); v = p.run([r], [0.5])\n"
+            "d.close(); print('done', m.n_vertices, v.n_vertices)\n") % str(sesmesh_amd.HERE.parent)
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr[-2000:]
+    assert out.stdout.startswith("done 11900 11900")
