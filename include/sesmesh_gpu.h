@@ -210,6 +210,32 @@ int smg_mesh_part_copy(const smg_mesh* m, size_t part, float* positions, float* 
                        uint8_t* patches, uint32_t* triangles, int32_t* bricks, float* values, double* t,
                        uint32_t* evals);
 
+/* --------------------------------------------------------- no allocation
+ * The mesh stage of one molecule with no allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid it).
+ * The caller passes device buffers for the outputs, with their capacities;
+ * every intermediate buffer comes from a fixed scratch array of
+ * SMG_NOALLOC_SCRATCH_BYTES that each device holds for the life of the
+ * process (set up at smg_device_open). The call makes no hipMalloc or
+ * hipFree. The mesh written is smg_mesh_new's, byte for byte. A molecule
+ * whose run needs more scratch than the array holds, or more vertices or
+ * triangles than the output capacities, returns SMG_E_LENGTH (the outputs
+ * are then unspecified). Calls on one device share its array and are
+ * serialised by the library; calls that overlap on one device from
+ * outside it (another library copy or direct writes) race. */
+#define SMG_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+typedef struct {
+    float* positions;   /* device, 3 per vertex */
+    float* normals;     /* device, 3 per vertex */
+    uint32_t* atoms;    /* device, 1 per vertex */
+    uint8_t* patches;   /* device, 1 per vertex */
+    uint32_t* triangles; /* device, 3 per triangle */
+    uint64_t vertex_capacity, triangle_capacity;
+} smg_mesh_out;
+/* info (may be NULL) receives the counts of the mesh written. */
+int smg_mesh_into(smg_resident* r, const smg_mesh_params* p, const smg_mesh_out* out, smg_mesh_info* info);
+
 /* ----------------------------------------------------------------- tuning
  * Launch shapes of the mesh stage on a device. Every setting gives the same
  * meshes, byte for byte; the settings trade LDS capacity, occupancy and

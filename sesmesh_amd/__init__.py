@@ -98,6 +98,29 @@ class PlanCaps(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
 
 
+class MeshOut(ctypes.Structure):
+    """smg_mesh_out of include/sesmesh_gpu.h: caller-owned device buffers."""
+    _fields_ = [("positions", _vp), ("normals", _vp), ("atoms", _vp), ("patches", _vp), ("triangles", _vp),
+                ("vertex_capacity", ctypes.c_uint64), ("triangle_capacity", ctypes.c_uint64)]
+
+
+NOALLOC_SCRATCH_BYTES = 256 << 20
+
+
+def mesh_into(resident, spacing, positions, normals, atoms, patches, triangles, vertex_capacity, triangle_capacity,
+              brick=10, refine="exact", sizing=0):
+    """The mesh stage of one molecule with no allocation (smg_mesh_into):
+    the mesh is written to caller-owned device buffers given as addresses
+    (positions and normals 3 f32 per vertex, atoms u32, patches u8,
+    triangles 3 u32), intermediates use the device's fixed scratch array.
+    Returns the counts (MeshInfo)."""
+    out = MeshOut(positions, normals, atoms, patches, triangles, int(vertex_capacity), int(triangle_capacity))
+    p = _mesh_params(spacing, brick, refine, sizing)
+    info = MeshInfo()
+    _check(lib().smg_mesh_into(resident._h, ctypes.byref(p), ctypes.byref(out), ctypes.byref(info)))
+    return info
+
+
 class Tuning(ctypes.Structure):
     """smg_tuning of include/sesmesh_gpu.h: launch shapes that leave the
     meshes unchanged."""
@@ -140,7 +163,7 @@ ABI_FUNCTIONS = [
     "smg_plan_run", "smg_plan_stats_get", "smg_plan_free", "smg_device_set_tuning", "smg_device_get_tuning",
     "smg_tune", "smg_model_new", "smg_model_free", "smg_model_param_count", "smg_model_get_params",
     "smg_model_set_params", "smg_model_collect", "smg_model_samples", "smg_model_dataset", "smg_model_forward",
-    "smg_model_gradients", "smg_model_train", "smg_device_set_model", "smg_model_set_policy",
+    "smg_model_gradients", "smg_model_train", "smg_device_set_model", "smg_model_set_policy", "smg_mesh_into",
 ]
 
 _lib = None

@@ -257,6 +257,15 @@ inline uint32_t blocks_for(size_t n, uint32_t bs) { return (uint32_t)((n + bs - 
 
 }  // namespace
 
+// The scratch memory of smg_mesh_into: every intermediate buffer of its
+// runs is carved from this array, so that entry point allocates nothing.
+// Each device holds its own copy for the life of the process; host code
+// uses only the device address hipGetSymbolAddress gives for it.
+static __device__ __align__(256) char noalloc_scratch[SMG_NOALLOC_SCRATCH_BYTES];
+
+struct smg_device;
+static int noalloc_setup(smg_device* d);
+
 // ------------------------------------------------------------------ objects
 struct smg_device {
     int ordinal = 0;
@@ -279,6 +288,14 @@ struct smg_device {
     // the device is destroyed when the last holder is freed, so objects may
     // be freed in any order relative to the device.
     std::atomic<int> refs{1};
+    // smg_mesh_into: this device's copy of the fixed scratch array (its
+    // address from hipGetSymbolAddress at open), pinned images of a
+    // one-molecule table and summary, and the lock that serialises its calls.
+    char* scratch_ptr = nullptr;
+    std::string scratch_error;
+    char* noalloc_stage_h = nullptr;
+    char* noalloc_sum_h = nullptr;
+    std::mutex noalloc_mu;
 };
 
 static void device_destroy(smg_device* d);
@@ -437,6 +454,11 @@ extern "C" int smg_device_open(int ordinal, smg_device** out, smg_device_info* i
         return fail(SMG_E_SELFTEST, "device arithmetic differs from the host's: " + std::to_string(m32) +
                                         " f32 and " + std::to_string(m64) + " f64 results");
     }
+    rc = noalloc_setup(d);
+    if (rc != SMG_OK) {
+        smg_device_close(d);
+        return rc;
+    }
     if (info) {
         std::memset(info, 0, sizeof(*info));
         std::snprintf(info->name, sizeof(info->name), "%s", d->prop.name);
@@ -466,6 +488,8 @@ static void device_destroy(smg_device* d) {
     for (auto& e : d->ev)
         if (e) (void)hipEventDestroy(e);
     if (d->host_u32) (void)hipHostFree(d->host_u32);
+    if (d->noalloc_stage_h) (void)hipHostFree(d->noalloc_stage_h);
+    if (d->noalloc_sum_h) (void)hipHostFree(d->noalloc_sum_h);
     if (d->stream) (void)hipStreamDestroy(d->stream);
     delete d;
 }
@@ -1000,9 +1024,23 @@ struct Run {
     std::vector<void*>* slots = nullptr;      // planned
     std::vector<size_t>* slot_bytes = nullptr;
     size_t next_slot = 0;
+    // Allocation-free (smg_mesh_into): sizes are capacities as in a planned
+    // run, and every buffer is carved from a fixed device array.
+    char* arena = nullptr;
+    size_t arena_size = 0, arena_used = 0;
+    uint32_t scratch_wgs = 0;  // workgroups of the global-scratch passes (0: the tuning's)
     float* lbuf = nullptr;  // the run's f32-copy buffer (its table points into it)
-    bool planned() const { return slots != nullptr; }
+    bool planned() const { return slots != nullptr || arena != nullptr; }
     int slot(void** p, size_t bytes) {
+        if (arena) {
+            size_t at = (arena_used + 255) & ~(size_t)255;
+            if (at > arena_size || bytes > arena_size - at)
+                return fail(SMG_E_LENGTH, "the run needs more than the " + std::to_string(arena_size >> 20) +
+                                              " MiB of scratch smg_mesh_into has (a smaller molecule, or smg_mesh_new)");
+            *p = arena + at;
+            arena_used = at + bytes;
+            return SMG_OK;
+        }
         if (next_slot < slots->size()) {
             if ((*slot_bytes)[next_slot] != bytes) return fail(SMG_E_INTERNAL, "plan buffers changed between runs");
             *p = (*slots)[next_slot++];
@@ -1026,6 +1064,7 @@ struct Run {
     template <class X>
     int output(X** p, size_t count) {
         size_t bytes = std::max<size_t>(count, 1) * sizeof(X);
+        if (arena && *p) return SMG_OK;  // the caller's buffer, sized for the run's capacity
         if (planned()) return slot((void**)p, bytes);
         HIP_TRY(hipMalloc((void**)p, bytes));
         return SMG_OK;
@@ -1267,7 +1306,9 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     const uint32_t rows = (uint32_t)E.rows;
     const bool fast = t.fast;
     const smg_tuning tune = d->tune;
-    const uint32_t scratch_wgs = tune.scratch_workgroups > 0 ? (uint32_t)tune.scratch_workgroups : SCRATCH_CHUNK;
+    const uint32_t scratch_wgs = X.scratch_wgs                    ? X.scratch_wgs
+                                 : tune.scratch_workgroups > 0 ? (uint32_t)tune.scratch_workgroups
+                                                               : SCRATCH_CHUNK;
     M->spb = spb;
     const uint32_t* statics;
     TRY(device_statics(d, &statics));
@@ -1852,6 +1893,76 @@ extern "C" int smg_mesh_new(smg_resident* r, const smg_mesh_params* p, smg_mesh*
 }
 
 // ------------------------------------------------------------------ tuning
+// ------------------------------------------------------ allocation-free
+// Set-up of smg_mesh_into at device open, with the device current: this
+// device's address of the scratch array and the pinned table and summary
+// images of a one-molecule run.
+static int noalloc_setup(smg_device* d) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(noalloc_scratch));
+    if (e != hipSuccess || !ptr) {
+        // The device still opens; smg_mesh_into reports this.
+        d->scratch_error = std::string("hipGetSymbolAddress of the scratch array: ") + hipGetErrorString(e);
+        (void)hipGetLastError();
+    } else {
+        d->scratch_ptr = (char*)ptr;
+    }
+    HIP_TRY(hipHostMalloc((void**)&d->noalloc_stage_h, stage_layout(1).bytes, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void**)&d->noalloc_sum_h, sum_layout(1).bytes, hipHostMallocDefault));
+    return SMG_OK;
+}
+
+extern "C" int smg_mesh_into(smg_resident* r, const smg_mesh_params* p, const smg_mesh_out* out, smg_mesh_info* info) {
+    if (!r || !p || !out) return fail(SMG_E_INVALID, "null resident, parameters or outputs");
+    if (!out->positions || !out->normals || !out->atoms || !out->patches || !out->triangles)
+        return fail(SMG_E_INVALID, "every output buffer is required");
+    if (out->vertex_capacity >= (1ull << 32) || out->triangle_capacity >= (1ull << 32))
+        return fail(SMG_E_LENGTH, "output capacities beyond 32-bit counts");
+    smg_device* d = r->dev;
+    std::lock_guard<std::mutex> lk(d->noalloc_mu);
+    if (!d->scratch_ptr) return fail(SMG_E_CALL, "no scratch array on this device: " + d->scratch_error);
+    HIP_TRY(hipSetDevice(d->ordinal));
+    Sizing sz;
+    TRY(run_sizing(d, p, 1, sz));
+    Table t;
+    smg_resident* const R[1] = {r};
+    TRY(build_table(R, p, 1, 1, t));
+    const uint32_t b = t.b, spb = (b + 1) * (b + 1) * (b + 1);
+    // Surface bricks are at most the molecule's brick-box cells.
+    const uint64_t nb = std::max<uint64_t>(t.dense, 1);
+    if (nb * spb * 4 > SMG_NOALLOC_SCRATCH_BYTES / 2)
+        return fail(SMG_E_LENGTH, "molecule too large for smg_mesh_into: its brick box (" + std::to_string(t.dense) +
+                                      " bricks) needs more than the " + std::to_string(SMG_NOALLOC_SCRATCH_BYTES >> 20) +
+                                      " MiB scratch holds; use smg_mesh_new");
+    Extents E{t.rows, t.locals(), t.spheres(), t.words(), t.dense, (uint32_t)nb, (uint32_t)out->vertex_capacity,
+              (uint32_t)out->triangle_capacity};
+    Run X;
+    X.d = d;
+    X.s = d->stream;
+    X.arena = d->scratch_ptr;
+    X.arena_size = SMG_NOALLOC_SCRATCH_BYTES;
+    X.scratch_wgs = 16;  // the global-scratch passes take ~0.9 MB per workgroup
+    smg_mesh M;
+    M.dev = d;
+    M.detached = true;  // nothing here is the mesh's to free
+    M.pos = out->positions;
+    M.nrm = out->normals;
+    M.atom = out->atoms;
+    M.patch = out->patches;
+    M.tris = out->triangles;
+    int rc = pipeline(X, t, E, d->noalloc_stage_h, d->noalloc_sum_h, &M, false, sz);
+    hipError_t e = hipStreamSynchronize(d->stream);
+    if (rc != SMG_OK) return rc;
+    if (e != hipSuccess) return fail(SMG_E_CALL, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+    uint32_t flags = 0;
+    TRY(finish(&M, t, d->noalloc_sum_h, false, nullptr, flags));
+    if (flags & OVER_VERTICES) return fail(SMG_E_LENGTH, "more vertices than the output's vertex capacity");
+    if (flags & OVER_TRIANGLES) return fail(SMG_E_LENGTH, "more triangles than the output's triangle capacity");
+    if (flags) return fail(SMG_E_INTERNAL, "a device count exceeded its capacity");
+    if (info) *info = M.info[0];
+    return SMG_OK;
+}
+
 static int check_tuning(const smg_tuning& t) {
     if (t.sample_first_tier < 0 || t.sample_first_tier > TIERS - 2 || t.refine_first_tier < 0 ||
         t.refine_first_tier > TIERS - 2)
