@@ -210,6 +210,48 @@ int smg_mesh_part_copy(const smg_mesh* m, size_t part, float* positions, float* 
                        uint8_t* patches, uint32_t* triangles, int32_t* bricks, float* values, double* t,
                        uint32_t* evals);
 
+/* ---------------------------------------------------------- measure stage
+ * The measure stage of every part of a mesh, on the device: validation,
+ * connected components and triangles grouped by component, as the CPU back
+ * end runs them after assembly (crates/sesmesh/src/surface.rs:1081-1095):
+ * mesh::validate_with (mesh/mod.rs:632-705), mesh::components_with
+ * (mod.rs:181-300) and post::group_by_component (mesh/post.rs:14-34). The
+ * f64 areas and volumes are summed in the reference's fixed chunk order, so
+ * every component's sums and the stats are the CPU's bit for bit.
+ * Afterwards the mesh's triangles are grouped by component (smg_mesh_copy
+ * returns them in that order, as SurfaceOutput::mesh holds them). A mesh
+ * with an index out of range is validated only (measured = 0). Measuring
+ * again returns at once. `ms` (may be NULL) receives the device time. */
+typedef struct {
+    uint64_t boundary_edges, non_manifold_edges, misoriented_edges, degenerate_triangles, duplicate_triangles,
+        invalid_indices, bad_vertices;
+} smg_validation; /* mesh::Validation (mod.rs:579-597) */
+typedef struct {
+    uint64_t vertices, triangles;
+    double area, volume; /* A^2; signed A^3, negative for a cavity */
+    int64_t euler;       /* V - E + F */
+    uint64_t first_triangle;
+} smg_component; /* mesh::Component (mod.rs:131-149) */
+typedef struct {
+    smg_validation validation;
+    uint64_t components, cavities;
+    double area, volume; /* SurfaceStats::area / volume: sums over the components */
+    int64_t euler;
+    int32_t measured; /* 1: components measured and triangles grouped */
+    int32_t pad;
+} smg_measure_info;
+int smg_mesh_measure(smg_mesh* m, double* ms);
+int smg_mesh_part_measure(const smg_mesh* m, size_t part, smg_measure_info* out);
+/* The part's components (cap entries at most) and per-vertex component ids
+ * (either may be NULL). */
+int smg_mesh_part_components(const smg_mesh* m, size_t part, smg_component* comps, size_t cap,
+                             uint32_t* vertex_component);
+/* A host mesh (one part) on the device, to measure or write it: n vertices
+ * (positions 3 f32 each; normals, atoms and patches may be NULL: no
+ * normals, zero atoms and patches), t triangles (3 u32 each). */
+int smg_mesh_upload(smg_device* dev, const float* positions, const float* normals, const uint32_t* atoms,
+                    const uint8_t* patches, size_t n, const uint32_t* triangles, size_t t, smg_mesh** out);
+
 /* --------------------------------------------------------- no allocation
  * The mesh stage of one molecule with no allocation. No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code

@@ -114,6 +114,24 @@ class Surface:
         lib().orc_surface_measure(self.handle, m)
         return {"components": int(m[0]), "cavities": int(m[1]), "area": m[2], "volume": m[3]}
 
+    def measure_full(self):
+        """The measure stage of this surface's mesh: validation, components
+        (with Euler characteristics and first triangles), vertex components,
+        triangles grouped by component, and the stats."""
+        L = lib()
+        L.orc_surface_measure_new.restype = _c_p
+        L.orc_measure_free.argtypes = [_c_p]
+        h = L.orc_surface_measure_new(self.handle)
+        try:
+            out = _read_measure(h)
+            vc = np.zeros(self.n_vertices, np.uint32)
+            g = np.zeros((self.n_triangles, 3), np.uint32)
+            L.orc_measure_arrays(_c_p(h), _ptr(vc), _ptr(g))
+            out["vertex_component"], out["grouped"] = vc, g
+        finally:
+            L.orc_measure_free(h)
+        return out
+
     def times(self):
         m = (ctypes.c_double * 4)()
         lib().orc_surface_times(self.handle, m)
@@ -188,6 +206,50 @@ class Surface:
         out = (ctypes.c_uint64 * 5)()
         lib().orc_count_ops(self.handle, first, count, stride, out)
         return dict(zip(("f32_ops", "samples", "f64_ops", "evals", "vertices"), list(out)))
+
+
+def _read_measure(h):
+    L = lib()
+    val = (ctypes.c_uint64 * 7)()
+    st = (ctypes.c_double * 4)()
+    cnt = (ctypes.c_int64 * 2)()
+    L.orc_measure_get(_c_p(h), val, st, cnt)
+    names = ("boundary_edges", "non_manifold_edges", "misoriented_edges", "degenerate_triangles",
+             "duplicate_triangles", "invalid_indices", "bad_vertices")
+    out = {"validation": dict(zip(names, [int(x) for x in val])), "area": st[0], "volume": st[1],
+           "components": int(st[2]), "cavities": int(st[3]), "euler": int(cnt[0]), "indices_valid": bool(cnt[1])}
+    nc = out["components"]
+    vt = np.zeros((nc, 2), np.uint64)
+    av = np.zeros((nc, 2), np.float64)
+    eu = np.zeros(nc, np.int64)
+    first = np.zeros(nc, np.uint64)
+    L.orc_measure_components(_c_p(h), _ptr(vt), _ptr(av), _ptr(eu), _ptr(first))
+    out["comp"] = {"vertices": vt[:, 0], "triangles": vt[:, 1], "area": av[:, 0], "volume": av[:, 1], "euler": eu,
+                   "first_triangle": first}
+    return out
+
+
+def measure(positions, triangles, normals=None):
+    """The measure stage (validation, components, grouping; reference
+    mesh/mod.rs, post.rs) of a mesh given as arrays."""
+    L = lib()
+    L.orc_measure_new.restype = _c_p
+    L.orc_measure_new.argtypes = [_c_p, ctypes.c_size_t, _c_p, ctypes.c_size_t, _c_p, ctypes.c_size_t]
+    L.orc_measure_free.argtypes = [_c_p]
+    p = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(triangles, np.uint32).reshape(-1, 3)
+    nr = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+    h = L.orc_measure_new(_ptr(p), len(p), _ptr(nr), 0 if nr is None else len(nr), _ptr(t), len(t))
+    try:
+        out = _read_measure(h)
+        if out["indices_valid"]:
+            vc = np.zeros(len(p), np.uint32)
+            g = np.zeros((len(t), 3), np.uint32)
+            L.orc_measure_arrays(_c_p(h), _ptr(vc), _ptr(g))
+            out["vertex_component"], out["grouped"] = vc, g
+    finally:
+        L.orc_measure_free(h)
+    return out
 
 
 def mc_table():

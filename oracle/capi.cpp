@@ -124,6 +124,67 @@ void orc_surface_measure(void* p, double out[4]) {
     out[3] = s.volume;
 }
 
+// The measure stage of a mesh given as arrays (normals may be NULL), or of a
+// Surface's mesh (orc_surface_measure_new). Returns a handle to read with
+// orc_measure_* and free with orc_measure_free.
+void* orc_measure_new(const float* pos, size_t n, const float* nrm, size_t n_nrm, const uint32_t* tris, size_t t) {
+    std::vector<std::array<float, 3>> P(n), N(nrm ? n_nrm : 0);
+    std::vector<std::array<u32, 3>> T(t);
+    for (size_t i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) P[i][k] = pos[3 * i + k];
+    for (size_t i = 0; i < N.size(); i++)
+        for (int k = 0; k < 3; k++) N[i][k] = nrm[3 * i + k];
+    for (size_t i = 0; i < t; i++)
+        for (int k = 0; k < 3; k++) T[i][k] = tris[3 * i + k];
+    return new Measured(measure_full(P, N, T));
+}
+
+void* orc_surface_measure_new(void* p) {
+    const Mesh& m = static_cast<Surface*>(p)->mesh;
+    return new Measured(measure_full(m.positions, m.normals, m.triangles));
+}
+
+void orc_measure_free(void* h) { delete static_cast<Measured*>(h); }
+
+// validation[7]: boundary, non-manifold, misoriented, degenerate, duplicate,
+// invalid, bad vertices; stats[4]: area, volume, components, cavities;
+// counts[2]: euler, indices valid.
+void orc_measure_get(void* h, uint64_t validation[7], double stats[4], int64_t counts[2]) {
+    const Measured& M = *static_cast<Measured*>(h);
+    const Validation& v = M.validation;
+    uint64_t vals[7] = {v.boundary_edges, v.non_manifold_edges, v.misoriented_edges, v.degenerate_triangles,
+                        v.duplicate_triangles, v.invalid_indices, v.bad_vertices};
+    std::copy(vals, vals + 7, validation);
+    stats[0] = M.area;
+    stats[1] = M.volume;
+    stats[2] = (double)M.components.size();
+    stats[3] = (double)M.cavities;
+    counts[0] = M.euler;
+    counts[1] = M.indices_valid;
+}
+
+// Per component: vt[2] vertices, triangles; av[2] area, volume; euler; first triangle.
+void orc_measure_components(void* h, uint64_t* vt, double* av, int64_t* euler, uint64_t* first) {
+    const Measured& M = *static_cast<Measured*>(h);
+    for (size_t c = 0; c < M.components.size(); c++) {
+        const Component& k = M.components[c];
+        vt[2 * c] = k.vertices;
+        vt[2 * c + 1] = k.triangles;
+        av[2 * c] = k.area;
+        av[2 * c + 1] = k.volume;
+        euler[c] = k.euler;
+        first[c] = k.first_triangle;
+    }
+}
+
+void orc_measure_arrays(void* h, uint32_t* vertex_component, uint32_t* grouped) {
+    const Measured& M = *static_cast<Measured*>(h);
+    if (vertex_component) std::copy(M.vertex_component.begin(), M.vertex_component.end(), vertex_component);
+    if (grouped)
+        for (size_t i = 0; i < M.grouped.size(); i++)
+            for (int k = 0; k < 3; k++) grouped[3 * i + k] = M.grouped[i][k];
+}
+
 // Samples (field/kernel.rs Engine::evaluate) of `count` bricks.
 void orc_brick_values(void* p, const int32_t* keys, size_t count, float* values) {
     Surface* S = static_cast<Surface*>(p);

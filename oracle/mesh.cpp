@@ -323,88 +323,15 @@ std::string assemble(const std::vector<std::array<int32_t, 3>>& bricks, std::vec
     return "";
 }
 
-// mesh/mod.rs:112-124
-static void triangle_area_volume(const Mesh& m, const std::array<u32, 3>& t, const double origin[3], double& area,
-                                 double& volume) {
-    double p[3][3];
-    for (int i = 0; i < 3; i++)
-        for (int k = 0; k < 3; k++) p[i][k] = (double)m.positions[t[i]][k] - origin[k];
-    V3 a(p[0][0], p[0][1], p[0][2]), b(p[1][0], p[1][1], p[1][2]), c(p[2][0], p[2][1], p[2][2]);
-    V3 e1 = b - a, e2 = c - a;
-    area = 0.5 * e1.cross(e2).norm();
-    volume = a.dot(b.cross(c)) / 6.0;
-}
-
-// mesh/mod.rs:181-300: components by smallest vertex, per fixed chunk of
-// 2^14 triangles partial sums per component, merged in chunk order.
+// mesh/mod.rs:181-300 and surface.rs:1160-1168: the stats of the measure
+// stage (components, cavities, area, volume).
 ComponentStats measure(const Mesh& m) {
-    size_t n = m.positions.size();
-    std::vector<u32> parent(n);
-    for (size_t i = 0; i < n; i++) parent[i] = (u32)i;
-    auto find = [&](u32 x) {
-        while (parent[x] != x) {
-            parent[x] = parent[parent[x]];
-            x = parent[x];
-        }
-        return x;
-    };
-    auto unite = [&](u32 a, u32 b) {
-        u32 ra = find(a), rb = find(b);
-        if (ra == rb) return;
-        if (ra < rb)
-            parent[rb] = ra;
-        else
-            parent[ra] = rb;
-    };
-    for (auto& t : m.triangles) {
-        unite(t[0], t[1]);
-        unite(t[0], t[2]);
-    }
-    std::vector<u32> id(n);
-    std::vector<double> area, volume;
-    for (size_t v = 0; v < n; v++) {
-        u32 r = find((u32)v);
-        if (r == v) {
-            id[v] = (u32)area.size();
-            area.push_back(0);
-            volume.push_back(0);
-        } else {
-            id[v] = id[r];
-        }
-    }
-    double origin[3] = {0, 0, 0};
-    if (n) origin[0] = m.positions[0][0], origin[1] = m.positions[0][1], origin[2] = m.positions[0][2];
-    const size_t CH = 1 << 14;
-    for (size_t c0 = 0; c0 < m.triangles.size(); c0 += CH) {
-        std::vector<std::array<double, 2>> acc;
-        std::vector<u32> comp;
-        size_t end = std::min(c0 + CH, m.triangles.size());
-        for (size_t t = c0; t < end; t++) {
-            u32 cid = id[m.triangles[t][0]];
-            double a, v;
-            triangle_area_volume(m, m.triangles[t], origin, a, v);
-            size_t q = 0;
-            while (q < comp.size() && comp[q] != cid) q++;
-            if (q == comp.size()) {
-                comp.push_back(cid);
-                acc.push_back({a, v});
-            } else {
-                acc[q][0] += a;
-                acc[q][1] += v;
-            }
-        }
-        for (size_t q = 0; q < comp.size(); q++) {
-            area[comp[q]] += acc[q][0];
-            volume[comp[q]] += acc[q][1];
-        }
-    }
+    Measured M = measure_full(m.positions, m.normals, m.triangles);
     ComponentStats s;
-    s.components = area.size();
-    for (size_t c = 0; c < area.size(); c++) {
-        s.area += area[c];
-        s.volume += volume[c];
-        s.cavities += volume[c] < 0.0;
-    }
+    s.components = M.components.size();
+    s.cavities = M.cavities;
+    s.area = M.area;
+    s.volume = M.volume;
     return s;
 }
 

@@ -384,6 +384,30 @@ struct ComponentStats {
 };
 ComponentStats measure(const Mesh& m);
 
+// The measure stage (mesh/mod.rs validate, components; post.rs grouping).
+struct Validation {
+    u64 boundary_edges = 0, non_manifold_edges = 0, misoriented_edges = 0, degenerate_triangles = 0,
+        duplicate_triangles = 0, invalid_indices = 0, bad_vertices = 0;
+};
+struct Component {
+    u64 vertices = 0, triangles = 0;
+    double area = 0, volume = 0;
+    int64_t euler = 0;
+    u64 first_triangle = 0;
+};
+struct Measured {
+    Validation validation;
+    bool indices_valid = false;  // components and grouping are measured only then
+    std::vector<u32> vertex_component;
+    std::vector<Component> components;
+    std::vector<std::array<u32, 3>> grouped;
+    double area = 0, volume = 0;
+    int64_t euler = 0;
+    size_t cavities = 0;
+};
+Measured measure_full(const std::vector<std::array<float, 3>>& pos, const std::vector<std::array<float, 3>>& nrm,
+                      const std::vector<std::array<u32, 3>>& tris);
+
 // The whole pipeline of surface.rs:1001-1079 for the CPU back end, with
 // `threads` worker threads over chunks of par::chunk_len(bricks) bricks.
 struct Surface {

@@ -98,6 +98,26 @@ class PlanCaps(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
 
 
+class Validation(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in ("boundary_edges", "non_manifold_edges", "misoriented_edges",
+                                               "degenerate_triangles", "duplicate_triangles", "invalid_indices",
+                                               "bad_vertices")]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+class Component(ctypes.Structure):
+    _fields_ = [("vertices", ctypes.c_uint64), ("triangles", ctypes.c_uint64), ("area", ctypes.c_double),
+                ("volume", ctypes.c_double), ("euler", ctypes.c_int64), ("first_triangle", ctypes.c_uint64)]
+
+
+class MeasureInfo(ctypes.Structure):
+    _fields_ = [("validation", Validation), ("components", ctypes.c_uint64), ("cavities", ctypes.c_uint64),
+                ("area", ctypes.c_double), ("volume", ctypes.c_double), ("euler", ctypes.c_int64),
+                ("measured", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
 class MeshOut(ctypes.Structure):
     """smg_mesh_out of include/sesmesh_gpu.h: caller-owned device buffers."""
     _fields_ = [("positions", _vp), ("normals", _vp), ("atoms", _vp), ("patches", _vp), ("triangles", _vp),
@@ -164,6 +184,7 @@ ABI_FUNCTIONS = [
     "smg_tune", "smg_model_new", "smg_model_free", "smg_model_param_count", "smg_model_get_params",
     "smg_model_set_params", "smg_model_collect", "smg_model_samples", "smg_model_dataset", "smg_model_forward",
     "smg_model_gradients", "smg_model_train", "smg_device_set_model", "smg_model_set_policy", "smg_mesh_into",
+    "smg_mesh_measure", "smg_mesh_part_measure", "smg_mesh_part_components", "smg_mesh_upload",
 ]
 
 _lib = None
@@ -375,11 +396,15 @@ class Mesh:
     """The mesh stage of one or more molecules on the device (one part per
     molecule); the meshes stay in HBM until copied."""
 
-    def __init__(self, resident, spacing, brick, refine, sizing, _batch=None, _plan=None):
+    def __init__(self, resident, spacing, brick, refine, sizing, _batch=None, _plan=None, _handle=None):
         self._h = _vp()
         self.times = MeshTimes()
         self._plan = None
-        if _plan is not None:
+        self._full = _handle is None  # an uploaded mesh holds its own arrays only
+        if _handle is not None:
+            self._h = _handle
+            self.residents = []
+        elif _plan is not None:
             plan, residents, params = _plan
             handles = (ctypes.c_void_p * len(residents))(*[r._h.value for r in residents])
             arr = (MeshParams * len(params))(*params)
@@ -421,12 +446,41 @@ class Mesh:
                "triangles": np.zeros((nt, 3), np.uint32), "bricks": np.zeros((nb, 3), np.int32),
                "t": np.zeros(nv, np.float64), "evals": np.zeros(nv, np.uint32)}
         vals = np.zeros((nb, (self.brick + 1) ** 3), np.float32) if values else None
-        _check(lib().smg_mesh_part_copy(self._h, part, _ptr(out["positions"]), _ptr(out["normals"]),
+        if not self._full:
+            for k in ("normals", "bricks", "t", "evals"):
+                out.pop(k)
+        _check(lib().smg_mesh_part_copy(self._h, part, _ptr(out["positions"]), _ptr(out.get("normals")),
                                         _ptr(out["atoms"]), _ptr(out["patches"]), _ptr(out["triangles"]),
-                                        _ptr(out["bricks"]), _ptr(vals), _ptr(out["t"]), _ptr(out["evals"])))
+                                        _ptr(out.get("bricks")), _ptr(vals), _ptr(out.get("t")),
+                                        _ptr(out.get("evals"))))
         if values:
             out["values"] = vals
         return out
+
+    def measure(self):
+        """The measure stage on the device (smg_mesh_measure): validation,
+        components, triangles grouped by component (arrays() then returns
+        the grouped order). Returns the device time in ms."""
+        ms = ctypes.c_double()
+        _check(lib().smg_mesh_measure(self._h, ctypes.byref(ms)))
+        return ms.value
+
+    def measure_info(self, part=0):
+        mi = MeasureInfo()
+        _check(lib().smg_mesh_part_measure(self._h, ctypes.c_size_t(part), ctypes.byref(mi)))
+        return {"validation": mi.validation.as_dict(), "components": int(mi.components),
+                "cavities": int(mi.cavities), "area": mi.area, "volume": mi.volume, "euler": int(mi.euler),
+                "measured": bool(mi.measured)}
+
+    def components(self, part=0):
+        """The part's components (arrays of vertices, triangles, area,
+        volume, euler, first_triangle) and per-vertex component ids."""
+        nc = self.measure_info(part)["components"]
+        arr = (Component * max(nc, 1))()
+        vc = np.zeros(self.parts[part].vertices, np.uint32)
+        _check(lib().smg_mesh_part_components(self._h, ctypes.c_size_t(part), arr, ctypes.c_size_t(nc), _ptr(vc)))
+        comp = {k: np.array([getattr(arr[i], k) for i in range(nc)]) for k, _ in Component._fields_}
+        return comp, vc
 
     def copy_to(self, positions, normals, atoms, patches, triangles, part=0):
         """Copies a part's mesh into caller-owned host buffers given as raw
@@ -440,6 +494,22 @@ class Mesh:
         if h:
             self._h = _vp()
             _free("smg_mesh_free", h)
+
+
+def upload_mesh(device, positions, triangles, normals=None, atoms=None, patches=None):
+    """A host mesh (one part) on the device (smg_mesh_upload), to measure
+    or write it."""
+    p = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(triangles, np.uint32).reshape(-1, 3)
+    nr = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+    at = None if atoms is None else np.ascontiguousarray(atoms, np.uint32)
+    pa = None if patches is None else np.ascontiguousarray(patches, np.uint8)
+    h = _vp()
+    L = lib()
+    L.smg_mesh_upload.argtypes = [_vp] * 5 + [ctypes.c_size_t, _vp, ctypes.c_size_t, _vp]
+    _check(L.smg_mesh_upload(device._h, _ptr(p), _ptr(nr), _ptr(at), _ptr(pa), len(p), _ptr(t), len(t),
+                             ctypes.byref(h)))
+    return Mesh(None, None, 10, "exact", 0, _handle=h)
 
 
 def mesh_many(residents, spacings, brick=10, refine="exact", sizing=0):

@@ -2,6 +2,7 @@
 // include/sesmesh_gpu.h: device open and self-test, resident arrangement,
 // and the whole mesh stage as a sequence of kernels on one stream.
 #include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <atomic>
@@ -23,6 +24,7 @@
 #include "dev_uniform.hpp"
 #include "dev_fast.hpp"
 #include "dev_mlp.hpp"
+#include "dev_measure.hpp"
 #include "host.hpp"
 
 namespace smg {
@@ -396,6 +398,14 @@ struct smg_mesh {
     double* t = nullptr;
     uint32_t* vevals = nullptr;
     uint32_t* tris = nullptr;
+    // The measure stage (smg_mesh_measure): per-part results, components of
+    // all parts in order (part p's from cfirst_part[p]), vertex components
+    // (device, owned by the mesh); triangles are then grouped by component.
+    bool measured = false;
+    std::vector<smg_measure_info> minfo;
+    std::vector<smg_component> comps;
+    std::vector<uint32_t> cfirst_part;
+    uint32_t* vcomp = nullptr;
     ~smg_mesh();
 };
 
@@ -2406,6 +2416,7 @@ struct smg_plan {
 };
 
 smg_mesh::~smg_mesh() {
+    if (vcomp) (void)hipFree(vcomp);
     if (plan) {
         auto& v = plan->views;
         v.erase(std::remove(v.begin(), v.end(), this), v.end());
@@ -2672,6 +2683,9 @@ extern "C" int smg_mesh_part_copy(const smg_mesh* m, size_t part, float* positio
     size_t t0 = m->tfirst[part], nt = m->tfirst[part + 1] - t0;
     size_t b0 = m->bfirst[part], nb = m->bfirst[part + 1] - b0;
     if (positions && nv) HIP_TRY(hipMemcpy(positions, m->pos + 3 * v0, 3 * nv * 4, hipMemcpyDeviceToHost));
+    if ((normals && nv && !m->nrm) || (t && nv && !m->t) || (evals && nv && !m->vevals) || (bricks && nb && !m->keys) ||
+        (values && nb && !m->values))
+        return fail(SMG_E_INVALID, "the mesh has no such array (an uploaded mesh holds its own arrays only)");
     if (normals && nv) HIP_TRY(hipMemcpy(normals, m->nrm + 3 * v0, 3 * nv * 4, hipMemcpyDeviceToHost));
     if (atoms && nv) HIP_TRY(hipMemcpy(atoms, m->atom + v0, nv * 4, hipMemcpyDeviceToHost));
     if (patches && nv) HIP_TRY(hipMemcpy(patches, m->patch + v0, nv, hipMemcpyDeviceToHost));
@@ -2690,6 +2704,244 @@ extern "C" int smg_mesh_copy(const smg_mesh* m, float* positions, float* normals
 }
 
 // The generated Marching Cubes table (for the tests).
+// ------------------------------------------------------------ measure stage
+namespace {
+inline uint32_t grid_for(uint64_t n, uint32_t bs = 256) { return g1(std::min<uint64_t>(blocks_for(n, bs), 65536)); }
+}  // namespace
+
+extern "C" int smg_mesh_measure(smg_mesh* M, double* ms) {
+    if (!M) return fail(SMG_E_INVALID, "null mesh");
+    if (M->detached) return fail(SMG_E_INVALID, "the mesh viewed a plan that has been freed");
+    if (M->measured) return SMG_OK;  // its triangles are grouped already
+    smg_device* d = M->dev;
+    HIP_TRY(hipSetDevice(d->ordinal));
+    const size_t P = M->info.size();
+    const uint64_t V = M->vfirst[P], T = M->tfirst[P];
+    if (3 * T >= 0xffffffffull) return fail(SMG_E_UNSUPPORTED, "too many triangles in one mesh to measure");
+    hipStream_t s = d->stream;
+    if (ms) HIP_TRY(hipEventRecord(d->ev[0], s));
+    Temps Tm(s);
+    Run X;
+    X.d = d;
+    X.s = s;
+    X.T = &Tm;
+    uint32_t *vf, *tf, *cnt;
+    unsigned long long* val;
+    TRY(X.temp(&vf, P + 1));
+    TRY(X.temp(&tf, P + 1));
+    TRY(X.temp(&cnt, 8));  // V, T, components, uses, segments
+    TRY(X.temp(&val, MV_N * P));
+    uint32_t hc[2] = {(uint32_t)V, (uint32_t)T};
+    HIP_TRY(hipMemcpyAsync(vf, M->vfirst.data(), (P + 1) * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(tf, M->tfirst.data(), (P + 1) * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(cnt, hc, 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemsetAsync(val, 0, MV_N * P * 8, s));
+    const MeasParts MP{vf, tf, (uint32_t)P};
+    uint32_t *parent, *ucount, *flag, *cid, *comp;
+    TRY(X.temp(&parent, V));
+    TRY(X.temp(&ucount, V));
+    TRY(X.temp(&flag, V));
+    TRY(X.temp(&cid, V));
+    TRY(X.temp(&comp, V));
+    HIP_TRY(hipMemsetAsync(ucount, 0, V * 4, s));
+    k_meas_vertices<<<grid_for(V), 256, 0, s>>>(M->pos, M->nrm, (uint32_t)V, MP, parent, val);
+    k_meas_triangles<<<grid_for(T), 256, 0, s>>>(M->tris, M->pos, (uint32_t)T, MP, parent, ucount, val);
+    HIP_TRY(hipGetLastError());
+    std::vector<unsigned long long> hv(MV_N * P);
+    HIP_TRY(hipMemcpyAsync(hv.data(), val, MV_N * P * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    bool invalid = false;
+    for (size_t p = 0; p < P; p++) invalid |= hv[MV_N * p + MV_INVALID] != 0;
+    M->minfo.assign(P, smg_measure_info{});
+    auto validation = [&](size_t p) {
+        smg_validation& v = M->minfo[p].validation;
+        const unsigned long long* h = &hv[MV_N * p];
+        v.boundary_edges = h[MV_BOUNDARY];
+        v.non_manifold_edges = h[MV_NON_MANIFOLD];
+        v.misoriented_edges = h[MV_MISORIENTED];
+        v.degenerate_triangles = h[MV_DEGENERATE];
+        v.duplicate_triangles = h[MV_DUPLICATE];
+        v.invalid_indices = h[MV_INVALID];
+        v.bad_vertices = h[MV_BAD];
+        if (v.invalid_indices)  // mod.rs:665-667: nothing else is measured
+            v.boundary_edges = v.non_manifold_edges = v.misoriented_edges = v.degenerate_triangles =
+                v.duplicate_triangles = 0;
+    };
+    if (invalid) {
+        // A mesh with an index out of range is validated only: its
+        // components are not defined and its triangles stay as they are.
+        for (size_t p = 0; p < P; p++) validation(p);
+        return SMG_OK;
+    }
+    // Components: root flags, ids in root order.
+    k_meas_roots<<<grid_for(V), 256, 0, s>>>(parent, (uint32_t)V, flag);
+    TRY(scan_u32(X, flag, cid, &cnt[0], V, &cnt[2]));
+    uint32_t ncomp = 0;
+    TRY(read_u32(d, &cnt[2], ncomp));
+    uint32_t *cverts, *cedges, *ctris, *cfirst, *coff;
+    TRY(X.temp(&cverts, ncomp));
+    TRY(X.temp(&cedges, ncomp));
+    TRY(X.temp(&ctris, ncomp));
+    TRY(X.temp(&cfirst, ncomp + 1));
+    TRY(X.temp(&coff, P + 1));
+    HIP_TRY(hipMemsetAsync(cverts, 0, (size_t)ncomp * 4, s));
+    HIP_TRY(hipMemsetAsync(cedges, 0, (size_t)ncomp * 4, s));
+    HIP_TRY(hipMemsetAsync(ctris, 0, (size_t)ncomp * 4, s));
+    k_meas_comp<<<grid_for(V), 256, 0, s>>>(parent, cid, (uint32_t)V, comp, cverts);
+    k_meas_coff<<<grid_for(P + 1), 256, 0, s>>>(cid, &cnt[2], (uint32_t)V, MP, coff);
+    // Edge uses bucketed by their smaller vertex.
+    uint32_t *ustart, *unext;
+    unsigned long long* uses;
+    TRY(X.temp(&ustart, V));
+    TRY(X.temp(&unext, V));
+    TRY(X.temp(&uses, 3 * T));
+    TRY(scan_u32(X, ucount, ustart, &cnt[0], V, &cnt[3]));
+    HIP_TRY(hipMemcpyAsync(unext, ustart, V * 4, hipMemcpyDeviceToDevice, s));
+    k_meas_uses<<<grid_for(T), 256, 0, s>>>(M->tris, (uint32_t)T, MP, unext, uses);
+    k_meas_buckets<<<grid_for(V), 256, 0, s>>>(ustart, ucount, uses, (uint32_t)V, MP, comp, cedges, val);
+    // Triangles grouped by component (stable), and the sums in the
+    // reference's chunk order.
+    uint32_t *tcomp, *idx, *skey, *sidx, *gtris, *head, *segid, *segstart;
+    double *A, *Vol, *segA, *segV;
+    TRY(X.temp(&tcomp, T));
+    TRY(X.temp(&idx, T));
+    TRY(X.temp(&skey, T));
+    TRY(X.temp(&sidx, T));
+    TRY(X.temp(&gtris, 3 * T));
+    TRY(X.temp(&head, T));
+    TRY(X.temp(&segid, T));
+    TRY(X.temp(&segstart, T));
+    TRY(X.temp(&A, T));
+    TRY(X.temp(&Vol, T));
+    TRY(X.temp(&segA, T));
+    TRY(X.temp(&segV, T));
+    k_meas_tcomp<<<grid_for(T), 256, 0, s>>>(M->tris, (uint32_t)T, MP, comp, tcomp, idx, ctris);
+    HIP_TRY(hipGetLastError());
+    TRY(scan_u32(X, ctris, cfirst, &cnt[2], ncomp, &cnt[5]));
+    if (T) {
+        uint32_t bits = 1;
+        while (bits < 32 && (1ull << bits) < ncomp) bits++;
+        size_t tmp_bytes = 0;
+        HIP_TRY(rocprim::radix_sort_pairs(nullptr, tmp_bytes, tcomp, skey, idx, sidx, (size_t)T, 0u, bits, s));
+        void* tmp;
+        TRY(X.temp((char**)&tmp, tmp_bytes));
+        HIP_TRY(rocprim::radix_sort_pairs(tmp, tmp_bytes, tcomp, skey, idx, sidx, (size_t)T, 0u, bits, s));
+        k_meas_group<<<grid_for(T), 256, 0, s>>>(sidx, skey, M->tris, M->pos, (uint32_t)T, MP, gtris, A, Vol, head);
+        TRY(scan_u32(X, head, segid, &cnt[1], T, &cnt[4]));
+        k_meas_segstart<<<grid_for(T), 256, 0, s>>>(head, segid, (uint32_t)T, segstart);
+        k_meas_segsum<<<grid_for(T), 256, 0, s>>>(segstart, &cnt[4], (uint32_t)T, A, Vol, segA, segV);
+        HIP_TRY(hipGetLastError());
+    } else {
+        HIP_TRY(hipMemsetAsync(&cnt[4], 0, 4, s));
+    }
+    DComp* dc;
+    TRY(X.temp(&dc, ncomp));
+    k_meas_comps<<<grid_for(ncomp), 256, 0, s>>>(&cnt[2], cverts, ctris, cedges, cfirst, segid, &cnt[4], (uint32_t)T,
+                                                segA, segV, coff, MP, dc);
+    if (!M->vcomp) HIP_TRY(hipMalloc((void**)&M->vcomp, std::max<size_t>(V, 1) * 4));
+    k_meas_vcomp<<<grid_for(V), 256, 0, s>>>(comp, coff, (uint32_t)V, MP, M->vcomp);
+    HIP_TRY(hipGetLastError());
+    if (T) HIP_TRY(hipMemcpyAsync(M->tris, gtris, 12 * T, hipMemcpyDeviceToDevice, s));
+    std::vector<DComp> hc2(ncomp);
+    M->cfirst_part.assign(P + 1, 0);
+    HIP_TRY(hipMemcpyAsync(hc2.data(), dc, (size_t)ncomp * sizeof(DComp), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(M->cfirst_part.data(), coff, (P + 1) * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(hv.data(), val, MV_N * P * 8, hipMemcpyDeviceToHost, s));
+    if (ms) HIP_TRY(hipEventRecord(d->ev[1], s));
+    HIP_TRY(hipStreamSynchronize(s));
+    M->comps.resize(ncomp);
+    for (uint32_t c = 0; c < ncomp; c++)
+        M->comps[c] = smg_component{hc2[c].vertices, hc2[c].triangles, hc2[c].area, hc2[c].volume,
+                                    (int64_t)hc2[c].euler, hc2[c].first_triangle};
+    for (size_t p = 0; p < P; p++) {
+        validation(p);
+        smg_measure_info& I = M->minfo[p];
+        // surface.rs:1160-1168; Rust's f64 sums fold from -0.0
+        I.area = -0.0;
+        I.volume = -0.0;
+        for (uint32_t c = M->cfirst_part[p]; c < M->cfirst_part[p + 1]; c++) {
+            const smg_component& k = M->comps[c];
+            I.area += k.area;
+            I.volume += k.volume;
+            I.euler += k.euler;
+            I.cavities += k.volume < 0.0;
+        }
+        I.components = M->cfirst_part[p + 1] - M->cfirst_part[p];
+        I.measured = 1;
+    }
+    M->measured = true;
+    if (ms) {
+        float f = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&f, d->ev[0], d->ev[1]));
+        *ms = f;
+    }
+    return SMG_OK;
+}
+
+extern "C" int smg_mesh_part_measure(const smg_mesh* m, size_t part, smg_measure_info* out) {
+    if (!m || !out) return fail(SMG_E_INVALID, "null mesh or output");
+    if (m->minfo.empty()) return fail(SMG_E_INVALID, "the mesh has not been measured (smg_mesh_measure)");
+    if (part >= m->minfo.size()) return fail(SMG_E_INVALID, "no part " + std::to_string(part));
+    *out = m->minfo[part];
+    return SMG_OK;
+}
+
+extern "C" int smg_mesh_part_components(const smg_mesh* m, size_t part, smg_component* comps, size_t cap,
+                                        uint32_t* vertex_component) {
+    if (!m) return fail(SMG_E_INVALID, "null mesh");
+    if (!m->measured) return fail(SMG_E_INVALID, "the mesh has no measured components (smg_mesh_measure)");
+    if (part >= m->minfo.size()) return fail(SMG_E_INVALID, "no part " + std::to_string(part));
+    const size_t c0 = m->cfirst_part[part], nc = m->cfirst_part[part + 1] - c0;
+    if (comps) {
+        if (cap < nc) return fail(SMG_E_LENGTH, "the part has " + std::to_string(nc) + " components");
+        std::copy(m->comps.begin() + c0, m->comps.begin() + c0 + nc, comps);
+    }
+    const size_t v0 = m->vfirst[part], nv = m->vfirst[part + 1] - v0;
+    if (vertex_component && nv) {
+        HIP_TRY(hipSetDevice(m->dev->ordinal));
+        HIP_TRY(hipMemcpy(vertex_component, m->vcomp + v0, nv * 4, hipMemcpyDeviceToHost));
+    }
+    return SMG_OK;
+}
+
+extern "C" int smg_mesh_upload(smg_device* dev, const float* positions, const float* normals, const uint32_t* atoms,
+                               const uint8_t* patches, size_t n, const uint32_t* triangles, size_t t, smg_mesh** out) {
+    if (!out) return fail(SMG_E_INVALID, "null output");
+    *out = nullptr;
+    if (!dev || (n && !positions) || (t && !triangles)) return fail(SMG_E_INVALID, "null device or arrays");
+    if (n >= (1ull << 31) || t >= (1ull << 31)) return fail(SMG_E_LENGTH, "mesh too large");
+    HIP_TRY(hipSetDevice(dev->ordinal));
+    std::unique_ptr<smg_mesh> M(new smg_mesh());
+    M->dev = dev;
+    smg_mesh_info I{};
+    I.vertices = n;
+    I.triangles = t;
+    M->info.assign(1, I);
+    M->bfirst = {0, 0};
+    M->vfirst = {0, (uint32_t)n};
+    M->tfirst = {0, (uint32_t)t};
+    HIP_TRY(hipMalloc((void**)&M->pos, std::max<size_t>(n, 1) * 12));
+    HIP_TRY(hipMalloc((void**)&M->atom, std::max<size_t>(n, 1) * 4));
+    HIP_TRY(hipMalloc((void**)&M->patch, std::max<size_t>(n, 1)));
+    HIP_TRY(hipMalloc((void**)&M->tris, std::max<size_t>(t, 1) * 12));
+    if (normals) HIP_TRY(hipMalloc((void**)&M->nrm, std::max<size_t>(n, 1) * 12));
+    if (n) {
+        HIP_TRY(hipMemcpy(M->pos, positions, n * 12, hipMemcpyHostToDevice));
+        if (normals) HIP_TRY(hipMemcpy(M->nrm, normals, n * 12, hipMemcpyHostToDevice));
+        if (atoms)
+            HIP_TRY(hipMemcpy(M->atom, atoms, n * 4, hipMemcpyHostToDevice));
+        else
+            HIP_TRY(hipMemset(M->atom, 0, n * 4));
+        if (patches)
+            HIP_TRY(hipMemcpy(M->patch, patches, n, hipMemcpyHostToDevice));
+        else
+            HIP_TRY(hipMemset(M->patch, 0, n));
+    }
+    if (t) HIP_TRY(hipMemcpy(M->tris, triangles, t * 12, hipMemcpyHostToDevice));
+    *out = M.release();
+    return SMG_OK;
+}
+
 // Test helpers: the LDS plan of a brick for given static LDS sizes (the
 // LdsPlan fields as 30 u32), and the static sizes this device reads.
 extern "C" int smg_lds_plan(uint32_t brick, int fast, uint64_t lds_max, const uint32_t* statics, uint32_t* out) {

@@ -65,3 +65,18 @@ def test_assembly_plain_and_replayed_plan_equal_cpu(device, assembly):
         del v
     st = plan.stats()
     assert st["fallbacks"] == 0 and st["replays"] == 2 and st["captures"] == 1
+
+
+def test_assembly_measure_equals_cpu(device, assembly):
+    """The device measure stage of the 58k assembly's mesh (validation,
+    components, chunk-ordered f64 sums, grouping) equals the oracle's."""
+    from test_gpu_measure import assert_measure_equal
+
+    xyzr, o, ref = assembly
+    res = sesmesh_amd.Resident(device, sesmesh_amd.Arrangement(xyzr, PROBE))
+    m = res.mesh(SPACING, BRICK)
+    ms = m.measure()
+    print(f"[config2] measure stage on the device: {ms:.2f} ms", flush=True)
+    om = o.measure_full()
+    assert_measure_equal(m, om)
+    assert om["validation"] == {k: 0 for k in om["validation"]}
