@@ -252,6 +252,43 @@ int smg_mesh_part_components(const smg_mesh* m, size_t part, smg_component* comp
 int smg_mesh_upload(smg_device* dev, const float* positions, const float* normals, const uint32_t* atoms,
                     const uint8_t* patches, size_t n, const uint32_t* triangles, size_t t, smg_mesh** out);
 
+/* ---------------------------------------------------------------- writers
+ * Mesh files of one part, formatted on the device, byte for byte as the
+ * reference's writers: write_obj (crates/sesmesh/src/export/obj.rs:
+ * 138-224, numbers by fmt.rs:30-68), write_ply (export/ply.rs:97-182,
+ * binary little-endian) and write_msms (export/msms.rs:70-148, the .vert
+ * and .face files as two formats). A measured mesh (smg_mesh_measure) is
+ * the one compute_surface returns: triangles grouped by component, and the
+ * PLY `component` property and MSMS face numbers from its components;
+ * unmeasured, PLY has no component property and MSMS numbers every vertex
+ * and face 1, as for a reference Mesh without component ids. */
+#define SMG_SESMESH_VERSION "1.3.2" /* the version line of the files: sesmesh's (Cargo.toml:23) */
+#define SMG_FORMAT_OBJ 0
+#define SMG_FORMAT_PLY 1
+#define SMG_FORMAT_MSMS_VERT 2
+#define SMG_FORMAT_MSMS_FACE 3
+typedef struct {
+    int32_t precision;        /* OBJ decimals 0..=9 (ObjOptions::precision, default 4) */
+    int32_t normals;          /* OBJ vn records and f a//a, PLY normals (default 1; only if the mesh has them) */
+    const char* object_name;  /* OBJ `o` record (NULL: "ses") */
+    const char* const* comments; /* OBJ `# ` / PLY `comment ` lines after the version line */
+    size_t n_comments;
+    const float* vertex_colors;  /* OBJ / PLY colours, 3 per vertex of the part (NULL: none) */
+    const uint64_t* group_first; /* OBJ `g` records: first triangle, ascending */
+    const char* const* group_names;
+    size_t n_groups;
+    const char* description;  /* MSMS header (MsmsHeader) */
+    uint64_t spheres;
+    double density, probe_radius;
+} smg_export_options;
+void smg_export_defaults(smg_export_options* o);
+/* Formats part `part` in `format` into buf (cap bytes); *len receives the
+ * file's size. With buf NULL only the size is computed; a buffer smaller
+ * than the size returns SMG_E_LENGTH. */
+int smg_export(const smg_mesh* m, size_t part, int format, const smg_export_options* o, char* buf, size_t cap,
+               size_t* len);
+int smg_export_file(const smg_mesh* m, size_t part, int format, const smg_export_options* o, const char* path);
+
 /* --------------------------------------------------------- no allocation
  * The mesh stage of one molecule with no allocation. No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code

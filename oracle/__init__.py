@@ -252,6 +252,84 @@ def measure(positions, triangles, normals=None):
     return out
 
 
+def _bytes(h):
+    L = lib()
+    L.orc_bytes.restype = ctypes.c_size_t
+    L.orc_bytes.argtypes = [_c_p, _c_p]
+    L.orc_bytes_free.argtypes = [_c_p]
+    try:
+        n = L.orc_bytes(h, None)
+        buf = ctypes.create_string_buffer(max(n, 1))
+        L.orc_bytes(h, buf)
+        return buf.raw[:n]
+    finally:
+        L.orc_bytes_free(h)
+
+
+def _strings(items):
+    return b"".join(str(x).encode() + b"\0" for x in items) or b"\0"
+
+
+def _mesh_args(mesh):
+    p = np.ascontiguousarray(mesh["positions"], np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(mesh["triangles"], np.uint32).reshape(-1, 3)
+    get = lambda k, dt: None if mesh.get(k) is None else np.ascontiguousarray(mesh[k], dt)
+    return p, t, get("normals", np.float32), get("atoms", np.uint32), get("patches", np.uint8), \
+        get("vertex_component", np.uint32)
+
+
+def fixed(values, prec):
+    """push_fixed (export/fmt.rs) of each f32 value."""
+    L = lib()
+    L.orc_fixed.restype = _c_p
+    L.orc_fixed.argtypes = [_c_p, ctypes.c_size_t, ctypes.c_int]
+    x = np.ascontiguousarray(values, np.float32)
+    return _bytes(L.orc_fixed(_ptr(x), len(x), prec)).split(b"\0")[:-1]
+
+
+def write_obj(mesh, precision=4, normals=True, comments=(), object_name="ses", vertex_colors=None, groups=()):
+    """OBJ bytes of a mesh dict (export/obj.rs write_obj)."""
+    L = lib()
+    L.orc_write_obj.restype = _c_p
+    L.orc_write_obj.argtypes = [_c_p, _c_p, ctypes.c_size_t, _c_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_char_p, ctypes.c_size_t, ctypes.c_char_p, _c_p, _c_p, ctypes.c_char_p,
+                                ctypes.c_size_t]
+    p, t, nr, _, _, _ = _mesh_args(mesh)
+    rgb = None if vertex_colors is None else np.ascontiguousarray(vertex_colors, np.float32)
+    gf = np.array([g[0] for g in groups], np.uint64)
+    h = L.orc_write_obj(_ptr(p), _ptr(nr), len(p), _ptr(t), len(t), precision, int(normals), _strings(comments),
+                        len(comments), object_name.encode(), _ptr(rgb), _ptr(gf), _strings([g[1] for g in groups]),
+                        len(groups))
+    return _bytes(h)
+
+
+def write_ply(mesh, normals=True, comments=(), vertex_colors=None):
+    """Binary PLY bytes of a mesh dict (export/ply.rs write_ply)."""
+    L = lib()
+    L.orc_write_ply.restype = _c_p
+    L.orc_write_ply.argtypes = [_c_p] * 5 + [ctypes.c_size_t, _c_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_char_p,
+                                             ctypes.c_size_t, _c_p]
+    p, t, nr, at, pa, vc = _mesh_args(mesh)
+    rgb = None if vertex_colors is None else np.ascontiguousarray(vertex_colors, np.float32)
+    h = L.orc_write_ply(_ptr(p), _ptr(nr), _ptr(at), _ptr(pa), _ptr(vc), len(p), _ptr(t), len(t), int(normals),
+                        _strings(comments), len(comments), _ptr(rgb))
+    return _bytes(h)
+
+
+def write_msms(mesh, description, spheres, density, probe):
+    """MSMS (.vert bytes, .face bytes) of a mesh dict (export/msms.rs)."""
+    L = lib()
+    L.orc_write_msms.restype = _c_p
+    L.orc_write_msms.argtypes = [_c_p] * 5 + [ctypes.c_size_t, _c_p, ctypes.c_size_t, ctypes.c_char_p,
+                                              ctypes.c_size_t, ctypes.c_double, ctypes.c_double, _c_p]
+    p, t, nr, at, pa, vc = _mesh_args(mesh)
+    vl = ctypes.c_size_t()
+    h = L.orc_write_msms(_ptr(p), _ptr(nr), _ptr(at), _ptr(pa), _ptr(vc), len(p), _ptr(t), len(t),
+                         description.encode(), spheres, density, probe, ctypes.byref(vl))
+    b = _bytes(h)
+    return b[:vl.value], b[vl.value:]
+
+
 def mc_table():
     off = np.zeros(257, np.uint16)
     tris = np.zeros(3 * 1024, np.uint8)

@@ -3,6 +3,7 @@
 #include <atomic>
 #include <chrono>
 #include <cstdio>
+#include <cstring>
 #include <thread>
 
 #include "oracle.hpp"
@@ -184,6 +185,106 @@ void orc_measure_arrays(void* h, uint32_t* vertex_component, uint32_t* grouped) 
         for (size_t i = 0; i < M.grouped.size(); i++)
             for (int k = 0; k < 3; k++) grouped[3 * i + k] = M.grouped[i][k];
 }
+
+// Writers. The mesh is given as arrays (normals, atoms, patches, vertex
+// components and colours may be NULL); strings are NUL-separated lists of
+// `count` entries. The result is a handle to read with orc_bytes.
+namespace {
+ExportMesh export_mesh(const float* pos, const float* nrm, const uint32_t* atoms, const uint8_t* patches,
+                       const uint32_t* vcomp, size_t n, const uint32_t* tris, size_t t) {
+    ExportMesh m;
+    m.positions.resize(n);
+    for (size_t i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) m.positions[i][k] = pos[3 * i + k];
+    if (nrm) {
+        m.normals.resize(n);
+        for (size_t i = 0; i < n; i++)
+            for (int k = 0; k < 3; k++) m.normals[i][k] = nrm[3 * i + k];
+    }
+    m.vertex_atom.assign(n, 0);
+    m.vertex_patch.assign(n, 0);
+    if (atoms) m.vertex_atom.assign(atoms, atoms + n);
+    if (patches) m.vertex_patch.assign(patches, patches + n);
+    if (vcomp) m.vertex_component.assign(vcomp, vcomp + n);
+    m.triangles.resize(t);
+    for (size_t i = 0; i < t; i++)
+        for (int k = 0; k < 3; k++) m.triangles[i][k] = tris[3 * i + k];
+    return m;
+}
+std::vector<std::string> split(const char* s, size_t count) {
+    std::vector<std::string> out;
+    for (size_t i = 0; i < count; i++) {
+        out.emplace_back(s);
+        s += out.back().size() + 1;
+    }
+    return out;
+}
+std::vector<std::array<float, 3>> colors(const float* c, size_t n) {
+    std::vector<std::array<float, 3>> out;
+    if (c) {
+        out.resize(n);
+        for (size_t i = 0; i < n; i++)
+            for (int k = 0; k < 3; k++) out[i][k] = c[3 * i + k];
+    }
+    return out;
+}
+}  // namespace
+
+void* orc_write_obj(const float* pos, const float* nrm, size_t n, const uint32_t* tris, size_t t, int precision,
+                    int normals, const char* comments, size_t n_comments, const char* object_name, const float* rgb,
+                    const uint64_t* group_first, const char* group_names, size_t n_groups) {
+    ExportMesh m = export_mesh(pos, nrm, nullptr, nullptr, nullptr, n, tris, t);
+    ObjOptions o;
+    o.precision = precision;
+    o.normals = normals != 0;
+    o.comments = split(comments, n_comments);
+    if (object_name) o.object_name = object_name;
+    o.vertex_colors = colors(rgb, n);
+    std::vector<std::string> names = split(group_names, n_groups);
+    for (size_t i = 0; i < n_groups; i++) o.groups.push_back({group_first[i], names[i]});
+    return new std::string(write_obj(m, o));
+}
+
+void* orc_write_ply(const float* pos, const float* nrm, const uint32_t* atoms, const uint8_t* patches,
+                    const uint32_t* vcomp, size_t n, const uint32_t* tris, size_t t, int normals, const char* comments,
+                    size_t n_comments, const float* rgb) {
+    ExportMesh m = export_mesh(pos, nrm, atoms, patches, vcomp, n, tris, t);
+    PlyOptions o;
+    o.normals = normals != 0;
+    o.comments = split(comments, n_comments);
+    o.vertex_colors = colors(rgb, n);
+    return new std::string(write_ply(m, o));
+}
+
+// The .vert text, then the .face text (vert_len receives the first's length).
+void* orc_write_msms(const float* pos, const float* nrm, const uint32_t* atoms, const uint8_t* patches,
+                     const uint32_t* vcomp, size_t n, const uint32_t* tris, size_t t, const char* description,
+                     size_t spheres, double density, double probe, size_t* vert_len) {
+    ExportMesh m = export_mesh(pos, nrm, atoms, patches, vcomp, n, tris, t);
+    MsmsHeader h{description, spheres, density, probe};
+    std::string v, f;
+    write_msms(m, h, v, f);
+    *vert_len = v.size();
+    return new std::string(v + f);
+}
+
+// push_fixed of n values (NUL after each), for the formatter's tests.
+void* orc_fixed(const float* x, size_t n, int prec) {
+    std::string* s = new std::string();
+    for (size_t i = 0; i < n; i++) {
+        push_fixed(*s, x[i], prec);
+        *s += '\0';
+    }
+    return s;
+}
+
+size_t orc_bytes(void* h, char* out) {
+    const std::string& s = *static_cast<std::string*>(h);
+    if (out) std::memcpy(out, s.data(), s.size());
+    return s.size();
+}
+
+void orc_bytes_free(void* h) { delete static_cast<std::string*>(h); }
 
 // Samples (field/kernel.rs Engine::evaluate) of `count` bricks.
 void orc_brick_values(void* p, const int32_t* keys, size_t count, float* values) {

@@ -408,6 +408,38 @@ struct Measured {
 Measured measure_full(const std::vector<std::array<float, 3>>& pos, const std::vector<std::array<float, 3>>& nrm,
                       const std::vector<std::array<u32, 3>>& tris);
 
+// Writers (export/fmt.rs, obj.rs, ply.rs, msms.rs). The version string is
+// the crate version the files name (Cargo.toml:23).
+constexpr const char* SESMESH_VERSION = "1.3.2";
+struct ExportMesh {
+    std::vector<std::array<float, 3>> positions, normals;
+    std::vector<std::array<u32, 3>> triangles;
+    std::vector<u32> vertex_atom, vertex_component;
+    std::vector<uint8_t> vertex_patch;
+};
+struct ObjOptions {
+    int precision = 4;
+    bool normals = true;
+    std::vector<std::string> comments;
+    std::string object_name = "ses";
+    std::vector<std::array<float, 3>> vertex_colors;
+    std::vector<std::pair<size_t, std::string>> groups;
+};
+struct PlyOptions {
+    std::vector<std::string> comments;
+    bool normals = true;
+    std::vector<std::array<float, 3>> vertex_colors;
+};
+struct MsmsHeader {
+    std::string description;
+    size_t spheres = 0;
+    double density = 0, probe_radius = 0;
+};
+void push_fixed(std::string& b, float x, int prec);
+std::string write_obj(const ExportMesh& m, const ObjOptions& o);
+std::string write_ply(const ExportMesh& m, const PlyOptions& o);
+void write_msms(const ExportMesh& m, const MsmsHeader& h, std::string& vert, std::string& face);
+
 // The whole pipeline of surface.rs:1001-1079 for the CPU back end, with
 // `threads` worker threads over chunks of par::chunk_len(bricks) bricks.
 struct Surface {

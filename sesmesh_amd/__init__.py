@@ -118,6 +118,30 @@ class MeasureInfo(ctypes.Structure):
                 ("measured", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
+class ExportOptions(ctypes.Structure):
+    """smg_export_options of include/sesmesh_gpu.h."""
+    _fields_ = [("precision", ctypes.c_int32), ("normals", ctypes.c_int32), ("object_name", ctypes.c_char_p),
+                ("comments", ctypes.POINTER(ctypes.c_char_p)), ("n_comments", ctypes.c_size_t),
+                ("vertex_colors", _vp), ("group_first", _vp), ("group_names", ctypes.POINTER(ctypes.c_char_p)),
+                ("n_groups", ctypes.c_size_t), ("description", ctypes.c_char_p), ("spheres", ctypes.c_uint64),
+                ("density", ctypes.c_double), ("probe_radius", ctypes.c_double)]
+
+
+def _export_options(precision=4, normals=True, object_name="ses", comments=(), vertex_colors=None, groups=(),
+                    description="", spheres=0, density=0.0, probe_radius=0.0):
+    """The options struct and the objects it points into (keep both alive)."""
+    keep = []
+    com = (ctypes.c_char_p * max(len(comments), 1))(*[str(c).encode() for c in comments])
+    names = (ctypes.c_char_p * max(len(groups), 1))(*[str(g[1]).encode() for g in groups])
+    first = np.array([g[0] for g in groups] or [0], np.uint64)
+    rgb = None if vertex_colors is None else np.ascontiguousarray(vertex_colors, np.float32)
+    keep += [com, names, first, rgb]
+    o = ExportOptions(int(precision), int(bool(normals)), object_name.encode(), com, len(comments), _ptr(rgb),
+                      _ptr(first), names, len(groups), description.encode(), int(spheres), float(density),
+                      float(probe_radius))
+    return o, keep
+
+
 class MeshOut(ctypes.Structure):
     """smg_mesh_out of include/sesmesh_gpu.h: caller-owned device buffers."""
     _fields_ = [("positions", _vp), ("normals", _vp), ("atoms", _vp), ("patches", _vp), ("triangles", _vp),
@@ -185,7 +209,10 @@ ABI_FUNCTIONS = [
     "smg_model_set_params", "smg_model_collect", "smg_model_samples", "smg_model_dataset", "smg_model_forward",
     "smg_model_gradients", "smg_model_train", "smg_device_set_model", "smg_model_set_policy", "smg_mesh_into",
     "smg_mesh_measure", "smg_mesh_part_measure", "smg_mesh_part_components", "smg_mesh_upload",
+    "smg_export_defaults", "smg_export", "smg_export_file",
 ]
+
+FORMATS = {"obj": 0, "ply": 1, "msms_vert": 2, "msms_face": 3}
 
 _lib = None
 
@@ -481,6 +508,29 @@ class Mesh:
         _check(lib().smg_mesh_part_components(self._h, ctypes.c_size_t(part), arr, ctypes.c_size_t(nc), _ptr(vc)))
         comp = {k: np.array([getattr(arr[i], k) for i in range(nc)]) for k, _ in Component._fields_}
         return comp, vc
+
+    def export(self, fmt, part=0, **options):
+        """The file `fmt` ("obj", "ply", "msms_vert", "msms_face") of a part,
+        formatted on the device (smg_export), as bytes. Options: precision,
+        normals, object_name, comments, vertex_colors, groups (first
+        triangle, name), description, spheres, density, probe_radius."""
+        o, keep = _export_options(**options)
+        L = lib()
+        L.smg_export.argtypes = [_vp, ctypes.c_size_t, ctypes.c_int, _vp, _vp, ctypes.c_size_t, _vp]
+        n = ctypes.c_size_t()
+        _check(L.smg_export(self._h, part, FORMATS[fmt], ctypes.byref(o), None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(max(n.value, 1))
+        _check(L.smg_export(self._h, part, FORMATS[fmt], ctypes.byref(o), buf, n.value, ctypes.byref(n)))
+        del keep
+        return buf.raw[:n.value]
+
+    def write(self, path, fmt, part=0, **options):
+        """Writes a part's file `fmt` to `path` (smg_export_file)."""
+        o, keep = _export_options(**options)
+        L = lib()
+        L.smg_export_file.argtypes = [_vp, ctypes.c_size_t, ctypes.c_int, _vp, ctypes.c_char_p]
+        _check(L.smg_export_file(self._h, part, FORMATS[fmt], ctypes.byref(o), str(path).encode()))
+        del keep
 
     def copy_to(self, positions, normals, atoms, patches, triangles, part=0):
         """Copies a part's mesh into caller-owned host buffers given as raw

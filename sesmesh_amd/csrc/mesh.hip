@@ -25,6 +25,7 @@
 #include "dev_fast.hpp"
 #include "dev_mlp.hpp"
 #include "dev_measure.hpp"
+#include "dev_export.hpp"
 #include "host.hpp"
 
 namespace smg {
@@ -2940,6 +2941,218 @@ extern "C" int smg_mesh_upload(smg_device* dev, const float* positions, const fl
     if (t) HIP_TRY(hipMemcpy(M->tris, triangles, t * 12, hipMemcpyHostToDevice));
     *out = M.release();
     return SMG_OK;
+}
+
+// ----------------------------------------------------------------- writers
+extern "C" void smg_export_defaults(smg_export_options* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->precision = 4;
+    o->normals = 1;
+}
+
+namespace {
+std::string clean_line(const char* s) {
+    std::string o = s ? s : "";
+    for (char& c : o)
+        if (c == '\n' || c == '\r') c = ' ';
+    return o;
+}
+
+// One block of text records on the device: lengths, offsets, bytes.
+template <int KIND>
+int text_block(Run& X, const ExportArgs& a, uint32_t count, uint32_t* cnt, char** bytes, uint32_t** off,
+               uint32_t& total) {
+    uint32_t* len;
+    TRY(X.temp(&len, count));
+    TRY(X.temp(off, (size_t)count + 1));
+    HIP_TRY(hipMemcpyAsync(cnt, &count, 4, hipMemcpyHostToDevice, X.s));
+    k_text_len<KIND><<<stride_grid(count), 256, 0, X.s>>>(a, count, len);
+    unsigned long long* sum;
+    TRY(X.temp(&sum, 1));
+    HIP_TRY(hipMemsetAsync(sum, 0, 8, X.s));
+    k_len_total<<<stride_grid(count), 256, 0, X.s>>>(len, count, sum);
+    unsigned long long big = 0;
+    HIP_TRY(hipMemcpyAsync(&big, sum, 8, hipMemcpyDeviceToHost, X.s));
+    HIP_TRY(hipStreamSynchronize(X.s));
+    if (big >= 0xffffffffull) return fail(SMG_E_UNSUPPORTED, "a block of the file exceeds 4 GiB");
+    TRY(scan_u32(X, len, *off, cnt, count, *off + count));
+    TRY(read_u32(X.d, *off + count, total));
+    TRY(X.temp(bytes, total));
+    k_text_write<KIND><<<stride_grid(count), 256, 0, X.s>>>(a, count, *off, *bytes);
+    HIP_TRY(hipGetLastError());
+    return SMG_OK;
+}
+}  // namespace
+
+extern "C" int smg_export(const smg_mesh* m, size_t part, int format, const smg_export_options* opt, char* buf,
+                          size_t cap, size_t* len) {
+    if (!m || !len) return fail(SMG_E_INVALID, "null mesh or length");
+    *len = 0;
+    if (part >= m->info.size()) return fail(SMG_E_INVALID, "no part " + std::to_string(part));
+    if (m->detached) return fail(SMG_E_INVALID, "the mesh viewed a plan that has been freed");
+    smg_export_options o;
+    smg_export_defaults(&o);
+    if (opt) o = *opt;
+    const uint32_t v0 = m->vfirst[part], nv = m->vfirst[part + 1] - v0;
+    const uint32_t t0 = m->tfirst[part], nt = m->tfirst[part + 1] - t0;
+    const bool normals = o.normals && m->nrm;
+    smg_device* d = m->dev;
+    HIP_TRY(hipSetDevice(d->ordinal));
+    Temps Tm(d->stream);
+    Run X;
+    X.d = d;
+    X.s = d->stream;
+    X.T = &Tm;
+    ExportArgs a{};
+    a.pos = m->pos + 3 * (size_t)v0;
+    a.nrm = m->nrm ? m->nrm + 3 * (size_t)v0 : nullptr;
+    a.atom = m->atom + v0;
+    a.patch = m->patch + v0;
+    a.vcomp = m->measured ? m->vcomp + v0 : nullptr;
+    a.tris = m->tris + 3 * (size_t)t0;
+    a.nv = nv;
+    a.nt = nt;
+    a.prec = o.precision;
+    a.normals = normals;
+    if (o.vertex_colors && (format == SMG_FORMAT_OBJ || format == SMG_FORMAT_PLY)) {
+        float* rgb;
+        TRY(X.temp(&rgb, 3 * (size_t)nv));
+        HIP_TRY(hipMemcpyAsync(rgb, o.vertex_colors, 12 * (size_t)nv, hipMemcpyHostToDevice, X.s));
+        a.rgb = rgb;
+    }
+    uint32_t* cnt;
+    unsigned long long* bad;
+    TRY(X.temp(&cnt, 4));
+    TRY(X.temp(&bad, 1));
+    HIP_TRY(hipMemsetAsync(bad, 0, 8, X.s));
+    a.bad = bad;
+    std::string head;
+    // (host text, device bytes, size) pieces of the file, in order
+    struct Piece {
+        std::string text;
+        const char* dev = nullptr;
+        size_t bytes = 0;
+    };
+    std::vector<Piece> pieces;
+    const std::string version = SMG_SESMESH_VERSION;
+    if (format == SMG_FORMAT_OBJ) {  // export/obj.rs:138-224
+        if (o.precision < 0 || o.precision > 9)
+            return fail(SMG_E_INVALID, "OBJ precision must be 0..=9, got " + std::to_string(o.precision));
+        for (size_t g = 1; g < o.n_groups; g++)
+            if (o.group_first[g - 1] > o.group_first[g]) return fail(SMG_E_INVALID, "OBJ groups must be sorted by first triangle");
+        head = "# sesmesh " + version + "\n";
+        for (size_t c = 0; c < o.n_comments; c++) head += "# " + clean_line(o.comments[c]) + "\n";
+        head += "o " + clean_line(o.object_name ? o.object_name : "ses") + "\n";
+        pieces.push_back({head});
+        char *vb, *nb = nullptr, *fb;
+        uint32_t *voff, *noff, *foff, vt = 0, ntot = 0, ft = 0;
+        TRY(text_block<XK_OBJ_V>(X, a, nv, cnt, &vb, &voff, vt));
+        if (normals) TRY(text_block<XK_OBJ_VN>(X, a, nv, cnt, &nb, &noff, ntot));
+        TRY(text_block<XK_OBJ_F>(X, a, nt, cnt, &fb, &foff, ft));
+        pieces.push_back({"", vb, vt});
+        if (normals) pieces.push_back({"", nb, ntot});
+        // Face ranges between group starts (obj.rs:199-222).
+        std::vector<std::pair<uint32_t, const char*>> bounds = {{0u, nullptr}};
+        for (size_t g = 0; g < o.n_groups; g++) {
+            uint32_t first = (uint32_t)std::min<uint64_t>(o.group_first[g], nt);
+            if (bounds.back().first == first) bounds.pop_back();
+            bounds.push_back({first, o.group_names[g] ? o.group_names[g] : ""});
+        }
+        std::vector<uint32_t> at(bounds.size() + 1), byte(bounds.size() + 1);
+        for (size_t k = 0; k < bounds.size(); k++) at[k] = bounds[k].first;
+        at[bounds.size()] = nt;
+        uint32_t *idx, *got;
+        TRY(X.temp(&idx, at.size()));
+        TRY(X.temp(&got, at.size()));
+        HIP_TRY(hipMemcpyAsync(idx, at.data(), at.size() * 4, hipMemcpyHostToDevice, X.s));
+        k_gather_u32<<<1, 256, 0, X.s>>>(foff, idx, (uint32_t)at.size(), got);
+        HIP_TRY(hipMemcpyAsync(byte.data(), got, at.size() * 4, hipMemcpyDeviceToHost, X.s));
+        HIP_TRY(hipStreamSynchronize(X.s));
+        for (size_t k = 0; k < bounds.size(); k++) {
+            if (bounds[k].second) pieces.push_back({"g " + clean_line(bounds[k].second) + "\n"});
+            pieces.push_back({"", fb + byte[k], (size_t)byte[k + 1] - byte[k]});
+        }
+    } else if (format == SMG_FORMAT_PLY) {  // export/ply.rs:97-182
+        const bool colors = a.rgb != nullptr;
+        head = "ply\nformat binary_little_endian 1.0\ncomment sesmesh " + version + "\n";
+        for (size_t c = 0; c < o.n_comments; c++) head += "comment " + clean_line(o.comments[c]) + "\n";
+        head += "element vertex " + std::to_string(nv) + "\n";
+        head += "property float x\nproperty float y\nproperty float z\n";
+        if (normals) head += "property float nx\nproperty float ny\nproperty float nz\n";
+        if (colors) head += "property uchar red\nproperty uchar green\nproperty uchar blue\n";
+        head += "property uint atom\nproperty uchar patch\n";
+        if (a.vcomp) head += "property uint component\n";
+        head += "element face " + std::to_string(nt) + "\nproperty list uchar uint vertex_indices\nend_header\n";
+        const uint32_t vstride = 12 + (normals ? 12 : 0) + (colors ? 3 : 0) + 4 + 1 + (a.vcomp ? 4 : 0);
+        const size_t body = (size_t)nv * vstride + (size_t)nt * 13;
+        char* pb;
+        TRY(X.temp(&pb, body));
+        k_ply_write<<<stride_grid((uint64_t)nv + nt), 256, 0, X.s>>>(a, vstride, pb);
+        HIP_TRY(hipGetLastError());
+        pieces.push_back({head});
+        pieces.push_back({"", pb, body});
+    } else if (format == SMG_FORMAT_MSMS_VERT || format == SMG_FORMAT_MSMS_FACE) {  // export/msms.rs:70-148
+        if (!m->nrm) return fail(SMG_E_INVALID, "MSMS output needs normals, atoms and patches for every vertex");
+        const bool vert = format == SMG_FORMAT_MSMS_VERT;
+        const char* what = vert ? "vertices" : "faces";
+        char line[256];
+        std::snprintf(line, sizeof(line), "%7u %7llu %5.2f %5.2f\n", vert ? nv : nt, (unsigned long long)o.spheres,
+                      o.density, o.probe_radius);
+        head = std::string("# MSMS solvent excluded surface ") + what + " for " + clean_line(o.description) + "\n#" +
+               what + " #sphere density probe_r\n" + line;
+        pieces.push_back({head});
+        char* b;
+        uint32_t* off;
+        uint32_t total = 0;
+        if (vert)
+            TRY(text_block<XK_MSMS_V>(X, a, nv, cnt, &b, &off, total));
+        else
+            TRY(text_block<XK_MSMS_F>(X, a, nt, cnt, &b, &off, total));
+        pieces.push_back({"", b, total});
+    } else {
+        return fail(SMG_E_INVALID, "unknown format " + std::to_string(format));
+    }
+    unsigned long long nbad = 0;
+    HIP_TRY(hipMemcpyAsync(&nbad, bad, 8, hipMemcpyDeviceToHost, X.s));
+    HIP_TRY(hipStreamSynchronize(X.s));
+    if (nbad) return fail(SMG_E_UNSUPPORTED, "non-finite or out-of-range values in the mesh");
+    size_t total = 0;
+    for (const Piece& q : pieces) total += q.dev ? q.bytes : q.text.size();
+    *len = total;
+    if (!buf) return SMG_OK;
+    if (cap < total) return fail(SMG_E_LENGTH, "the output needs " + std::to_string(total) + " bytes");
+    size_t at = 0;
+    for (const Piece& q : pieces) {
+        if (q.dev) {
+            if (q.bytes) HIP_TRY(hipMemcpyAsync(buf + at, q.dev, q.bytes, hipMemcpyDeviceToHost, X.s));
+            at += q.bytes;
+        } else {
+            std::memcpy(buf + at, q.text.data(), q.text.size());
+            at += q.text.size();
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(X.s));
+    return SMG_OK;
+}
+
+extern "C" int smg_export_file(const smg_mesh* m, size_t part, int format, const smg_export_options* o,
+                               const char* path) {
+    if (!path) return fail(SMG_E_INVALID, "null path");
+    size_t n = 0;
+    TRY(smg_export(m, part, format, o, nullptr, 0, &n));
+    char* host = nullptr;
+    HIP_TRY(hipHostMalloc((void**)&host, std::max<size_t>(n, 1), hipHostMallocDefault));
+    int rc = smg_export(m, part, format, o, host, n, &n);
+    if (rc == SMG_OK) {
+        FILE* f = std::fopen(path, "wb");
+        if (!f || std::fwrite(host, 1, n, f) != n || std::fclose(f) != 0) {
+            if (f) std::fclose(f);
+            rc = fail(SMG_E_INVALID, std::string("cannot write ") + path);
+        }
+    }
+    (void)hipHostFree(host);
+    return rc;
 }
 
 // Test helpers: the LDS plan of a brick for given static LDS sizes (the

@@ -80,3 +80,27 @@ def test_assembly_measure_equals_cpu(device, assembly):
     om = o.measure_full()
     assert_measure_equal(m, om)
     assert om["validation"] == {k: 0 for k in om["validation"]}
+
+
+def test_assembly_files_equal_cpu(device, assembly, tmp_path):
+    """OBJ, PLY and MSMS files of the measured 58k-assembly mesh, written on
+    the device as the reference's command line writes them, equal the
+    oracle's files byte for byte."""
+    from test_gpu_export import cli_files
+
+    xyzr, o, ref = assembly
+    om = o.measure_full()
+    omesh = {k: ref[k] for k in ("positions", "normals", "atoms", "patches")}
+    omesh["triangles"], omesh["vertex_component"] = om["grouped"], om["vertex_component"]
+    res = sesmesh_amd.Resident(device, sesmesh_amd.Arrangement(xyzr, PROBE))
+    m = res.mesh(SPACING, BRICK)
+    m.measure()
+    groups, hd = cli_files(omesh, om, len(xyzr), PROBE, "rings.pdb")
+    m.write(tmp_path / "a.obj", "obj", groups=groups)
+    assert (tmp_path / "a.obj").read_bytes() == oracle.write_obj(omesh, 4, True, (), "ses", None, groups)
+    m.write(tmp_path / "a.ply", "ply")
+    assert (tmp_path / "a.ply").read_bytes() == oracle.write_ply(omesh)
+    v, f = oracle.write_msms(omesh, hd["description"], hd["spheres"], hd["density"], hd["probe_radius"])
+    m.write(tmp_path / "a.vert", "msms_vert", **hd)
+    m.write(tmp_path / "a.face", "msms_face", **hd)
+    assert (tmp_path / "a.vert").read_bytes() == v and (tmp_path / "a.face").read_bytes() == f
