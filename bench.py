@@ -208,22 +208,44 @@ def copy_out(plan, res, spacing, info, steps):
             "what": "plan replay + copy of positions, normals, atoms, patches and triangles to pinned host memory"}
 
 
-def batched(dev, count, seed, repeats=3):
+def batch_residents(dev, count, seed):
+    """Seeded molecules of 20-5000 atoms (log-uniform), probe 1.2-1.4 A,
+    spacing 0.2 / 0.25 / 0.5 A: arrangements built on the host's cores,
+    then made resident."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import sesmesh_amd
+
+    atoms, probe, spacing, seeds = sesmesh_amd.synth_batch_spec(seed, count)
+
+    def build(i):
+        xyzr = sesmesh_amd.synth_globule(int(seeds[i]), int(atoms[i]))
+        return sesmesh_amd.Arrangement(xyzr, float(probe[i]), threads=1)
+
+    with ThreadPoolExecutor(max_workers=max(1, min(16, host_threads()))) as ex:
+        arrs = list(ex.map(build, range(count)))
+    residents = [sesmesh_amd.Resident(dev, a) for a in arrs]
+    del arrs
+    return atoms, spacing, residents
+
+
+def batched(dev, count, seed, repeats=2):
     """Batched serving on `count` seeded molecules (atoms log-uniform in
     20-5000, probe 1.2-1.4 A, spacing 0.2/0.25/0.5 A): arrangements built and
     resident first. Timed: per-molecule launches of the mesh stage ("plain")
     against one batched pass over all of them (smg_mesh_many) and against a
     capture-ready plan replaying that pass as a hipGraph; best of `repeats`
-    each. Every part's counts must equal its plain run's."""
+    each. Every part's counts must equal its plain run's, and a sample of
+    parts must equal their plain runs byte for byte (untimed)."""
     import sesmesh_amd
 
-    atoms, probe, spacing, seeds = sesmesh_amd.synth_batch_spec(seed, count)
-    residents = []
-    for n, p, sd in zip(atoms, probe, seeds):
-        xyzr = sesmesh_amd.synth_globule(int(sd), int(n))
-        residents.append(sesmesh_amd.Resident(dev, sesmesh_amd.Arrangement(xyzr, float(p))))
-    plain_s, batch_s, plain_counts = [], [], None
-    for _ in range(repeats):
+    t0 = time.perf_counter()
+    atoms, spacing, residents = batch_residents(dev, count, seed)
+    build_s = time.perf_counter() - t0
+    progress(f"batched: {count} molecules ({int(atoms.sum())} atoms) resident in {build_s:.1f} s")
+    check = sorted(set(np.linspace(0, count - 1, min(count, 12)).astype(int).tolist()))
+    plain_s, batch_s, plain_counts, plain_arrays = [], [], None, {}
+    for rep in range(repeats):
         t = time.perf_counter()
         counts = []
         for r, h in zip(residents, spacing):
@@ -232,14 +254,30 @@ def batched(dev, count, seed, repeats=3):
             del m
         plain_s.append(time.perf_counter() - t)
         plain_counts = counts
+    for i in check:
+        plain_arrays[i] = residents[i].mesh(float(spacing[i]), 10).arrays()
     progress(f"batched: plain {min(plain_s):.3f} s")
-    for _ in range(repeats):
+
+    def same(mb, what):
+        counts = [(p.vertices, p.triangles, p.refine_evaluations) for p in mb.parts]
+        if counts != plain_counts:
+            raise RuntimeError(f"{what} parts differ from their single-molecule runs")
+        for i in check:
+            a = mb.arrays(part=i)
+            for k, v in plain_arrays[i].items():
+                if not np.array_equal(np.ascontiguousarray(a[k]).view(np.uint8), np.ascontiguousarray(v).view(np.uint8)):
+                    raise RuntimeError(f"{what} part {i} differs from its single run in {k}")
+
+    info = None
+    for rep in range(repeats):
         t = time.perf_counter()
         mb = sesmesh_amd.mesh_many(residents, spacing, 10)
         batch_s.append(time.perf_counter() - t)
-        counts = [(p.vertices, p.triangles, p.refine_evaluations) for p in mb.parts]
-        if counts != plain_counts:
-            raise RuntimeError("batched parts differ from their single-molecule runs")
+        if rep == 0:
+            same(mb, "batched")
+            info = {"vertices": int(sum(p.vertices for p in mb.parts)),
+                    "triangles": int(sum(p.triangles for p in mb.parts)),
+                    "device_ms": mb.times.total_ms}
         del mb
     progress(f"batched: one pass {min(batch_s):.3f} s")
     # The same batch through a capture-ready plan: one graph replay per pass.
@@ -251,18 +289,20 @@ def batched(dev, count, seed, repeats=3):
         t = time.perf_counter()
         v = plan.run(residents, spacing)
         dt = time.perf_counter() - t
-        counts = [(p.vertices, p.triangles, p.refine_evaluations) for p in v.parts]
-        if counts != plain_counts:
-            raise RuntimeError("plan parts differ from their single-molecule runs")
+        if i == 1:
+            same(v, "plan")
         del v
         if i:
             plan_s.append(dt)
+    stats = plan.stats()
+    del plan
     progress(f"batched: plan replay {min(plan_s):.3f} s")
-    verts = int(sum(c[0] for c in plain_counts))
-    return {"molecules": count, "atoms": int(atoms.sum()), "vertices": verts,
+    del residents
+    return {"molecules": count, "atoms": int(atoms.sum()), **info, "arrangements_s": build_s,
+            "checked_parts": check,
             "plain": {"seconds": min(plain_s), "molecules_per_s": count / min(plain_s)},
             "batched": {"seconds": min(batch_s), "molecules_per_s": count / min(batch_s)},
-            "plan": {"seconds": min(plan_s), "molecules_per_s": count / min(plan_s), "stats": plan.stats()},
+            "plan": {"seconds": min(plan_s), "molecules_per_s": count / min(plan_s), "stats": stats},
             "speedup": min(plain_s) / min(batch_s), "plan_speedup": min(plain_s) / min(plan_s)}
 
 
@@ -324,10 +364,10 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--batch", type=int, default=128, help="molecules in the batched measurement (0: skip)")
+    ap.add_argument("--batch", type=int, default=4096, help="molecules in the batched measurement (0: skip)")
     ap.add_argument("--cpu-stride", type=int, default=1,
                     help="the CPU baseline meshes every Nth surface brick and scales (1: all bricks)")
-    ap.add_argument("--precision", type=int, default=0, metavar="COPIES",
+    ap.add_argument("--precision", type=int, default=60, metavar="COPIES",
                     help="precision study: exact vs fast refinement on COPIES (60: the 2.4M-atom shell) "
                          "icosahedral copies of a 40k-atom globule at 1.0 A (0: skip)")
     ap.add_argument("--no-plain", dest="plain", action="store_false",

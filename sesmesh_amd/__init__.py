@@ -795,13 +795,16 @@ def precision_report(resident, spacing, brick=10):
         out[f"{mode}_s"] = time.perf_counter() - t
         out[f"{mode}_refine_ms"] = m.times.k_refine_ms
         g = m.arrays()
+        t = time.perf_counter()
         f = np.abs(resident.field_clamped(g["positions"].astype(np.float64), 4.0))
-        av = mesh_area_volume(g["positions"], g["triangles"])
+        out[f"{mode}_field_s"] = time.perf_counter() - t
+        m.measure()  # areas and volumes in the reference's summation order
+        mi = m.measure_info()
         meshes[mode] = g
         out[mode] = {"vertices": int(m.n_vertices), "triangles": int(m.n_triangles),
                      "max_abs_F": float(f.max()) if len(f) else 0.0,
                      "mean_abs_F": float(f.mean()) if len(f) else 0.0,
-                     "area": av[0], "volume": av[1]}
+                     "area": mi["area"], "volume": mi["volume"], "components": mi["components"]}
         del m
     e, f = out["exact"], out["fast"]
     out["rel_area_error"] = abs(f["area"] - e["area"]) / e["area"] if e["area"] else 0.0

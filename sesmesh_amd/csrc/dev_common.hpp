@@ -178,7 +178,9 @@ __device__ __forceinline__ bool queue_take(const Queue& q, uint32_t* ticket, uin
 // Runs body(item) for the workgroup's items. PERSIST: loops over the queue
 // (a residency-sized grid). Otherwise one item per workgroup, blockIdx.x
 // of the queue's items (a grid of at least the count; workgroups past it do
-// nothing): cheaper for short items and for the first, largest pass.
+// nothing): cheaper for short items and for the first, largest pass. A
+// dispatch holds fewer than 2^32 work-items: the host launches a pass with
+// more workgroups than that persistent.
 template <bool PERSIST, class F>
 __device__ __forceinline__ void for_each_item(const Queue& q, uint32_t* ticket, F&& body) {
     if (PERSIST) {

@@ -805,15 +805,16 @@ namespace {
 // wavefront per point: probe - ses_eval(x, probe + band).distance, where
 // the distance is 0 outside the SAS (inside_sas, point.rs:372-381).
 __global__ void k_field_clamped(DArr A, const double* points, uint32_t n, double band, double* out) {
-    uint32_t w = blockIdx.x;
-    if (w >= n) return;
-    double x = points[3 * w], y = points[3 * w + 1], z = points[3 * w + 2];
-    bool inside = false;
-    for_near(A.gs, x, y, z, A.max_expanded, lane_id(), WAVE, [&](uint32_t k) {
-        if (!inside && A.status[k] != S_CONTAINED && ddist(x, y, z, A.cx[k], A.cy[k], A.cz[k]) < A.R[k]) inside = true;
-    });
-    double dist = wave_any(inside) ? ses_distance_wave(A, x, y, z, A.probe + band) : 0.0;
-    if (lane_id() == 0) out[w] = A.probe - dist;
+    for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
+        double x = points[3 * (size_t)w], y = points[3 * (size_t)w + 1], z = points[3 * (size_t)w + 2];
+        bool inside = false;
+        for_near(A.gs, x, y, z, A.max_expanded, lane_id(), WAVE, [&](uint32_t k) {
+            if (!inside && A.status[k] != S_CONTAINED && ddist(x, y, z, A.cx[k], A.cy[k], A.cz[k]) < A.R[k])
+                inside = true;
+        });
+        double dist = wave_any(inside) ? ses_distance_wave(A, x, y, z, A.probe + band) : 0.0;
+        if (lane_id() == 0) out[w] = A.probe - dist;
+    }
 }
 }  // namespace
 
@@ -829,7 +830,7 @@ extern "C" int smg_field_clamped(smg_resident* r, const double* points, size_t n
     HIP_TRY(hipMallocAsync((void**)&pd, n * 24, s));
     HIP_TRY(hipMallocAsync((void**)&vd, n * 8, s));
     HIP_TRY(hipMemcpyAsync(pd, points, n * 24, hipMemcpyHostToDevice, s));
-    k_field_clamped<<<(uint32_t)n, WAVE, 0, s>>>(r->A, pd, (uint32_t)n, band, vd);
+    k_field_clamped<<<std::min<uint32_t>((uint32_t)n, 1u << 24), WAVE, 0, s>>>(r->A, pd, (uint32_t)n, band, vd);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(values, vd, n * 8, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipFreeAsync(pd, s));
@@ -1004,6 +1005,11 @@ struct RunHdr {
 };
 
 inline uint32_t g1(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(n, 1u << 30)); }
+// Workgroups of `block` threads one dispatch may hold (fewer than 2^32
+// work-items): a pass with more items than that runs persistent.
+inline uint64_t dispatch_max(uint32_t block) { return std::min<uint64_t>(1u << 30, 0xffffffffull / block); }
+inline bool fits_dispatch(uint64_t items, uint32_t block) { return items <= dispatch_max(block); }
+inline uint32_t gridn(uint64_t n, uint32_t block) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(n, dispatch_max(block))); }
 inline uint32_t stride_grid(uint64_t n, uint32_t bs = 256) { return g1(std::min<uint64_t>(blocks_for(n, bs), 8192)); }
 
 // Workgroups of `block` threads of `kernel` with `lds` dynamic LDS bytes that
@@ -1375,7 +1381,8 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     TRY(rec(6));
     {
         Queue q{nullptr, &cnt[K_TOUCHED], nullptr};
-        k_classify_bricks<<<g1(E.dense), WAVE, 0, s>>>(mols, wofs_d, rows, touched, q, keep);
+        if (!fits_dispatch(E.dense, WAVE)) return fail(SMG_E_UNSUPPORTED, "brick boxes of the run too large for one dispatch");
+        k_classify_bricks<<<gridn(E.dense, WAVE), WAVE, 0, s>>>(mols, wofs_d, rows, touched, q, keep);
     }
     TRY(rec(7));
     TRY(scan_u32(X, keep, koff, &cnt[K_TOUCHED], E.dense, &cnt[K_NB]));
@@ -1400,7 +1407,7 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
         float* feat;
         TRY(X.temp(&feat, (size_t)nb * MLP_IN));
         TRY(X.temp(&pred, (size_t)nb * MLP_OUT));
-        k_features<<<g1(nb), WAVE, 0, s>>>(mols, wofs_d, rows, nullptr, M->keys, bmol, Queue{nullptr, &cnt[K_NB], nullptr},
+        k_features<<<gridn(nb, WAVE), WAVE, 0, s>>>(mols, wofs_d, rows, nullptr, M->keys, bmol, Queue{nullptr, &cnt[K_NB], nullptr},
                                            feat);
         k_mlp_forward<<<g1(std::min<uint64_t>((nb + MLP_ROWS - 1) / MLP_ROWS, 4096)), 256, 0, s>>>(
             feat, &cnt[K_NB], sz.predict->P, sz.predict->W1b, sz.predict->W2b, pred);
@@ -1418,7 +1425,7 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
         TRY(X.temp(&cocc, ntouched));
         TRY(X.temp(&ctgt, ntouched));
         TRY(X.temp(&cload, nb));
-        k_features<<<g1(ntouched), WAVE, 0, s>>>(mols, wofs_d, rows, touched, nullptr, nullptr,
+        k_features<<<gridn(ntouched, WAVE), WAVE, 0, s>>>(mols, wofs_d, rows, touched, nullptr, nullptr,
                                                  Queue{nullptr, &cnt[K_TOUCHED], nullptr}, cfeat);
         k_fill_u32<<<stride_grid(nb), 256, 0, s>>>(cload, 0u, 0, &cnt[K_NB]);
         HIP_TRY(hipGetLastError());
@@ -1439,8 +1446,8 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
             Queue q{list, count, &cnt[K_Q_UNIFORM0 + u]};
             uint32_t items = nb;
             if (!first && !X.planned()) TRY(X.size(count, 0, items));
-            if (first || !X.planned())
-                k_uniform<false><<<g1(items), 256, lay.bytes, s>>>(mols, bmol, M->keys, UNIFORM_CAPS[u], lay, q, st,
+            if ((first || !X.planned()) && fits_dispatch(items, 256))
+                k_uniform<false><<<gridn(items, 256), 256, lay.bytes, s>>>(mols, bmol, M->keys, UNIFORM_CAPS[u], lay, q, st,
                                                                    over, &cnt[K_UOVER + u]);
             else
                 k_uniform<true><<<g1(std::min<uint64_t>(nb, resident(d, (const void*)k_uniform<true>, 256, lay.bytes))),
@@ -1544,8 +1551,8 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
             // 256 threads in every pass: the lane kernel takes one sample per
             // thread (at most (sub + 1)^3 per sub-block), so wider
             // workgroups would idle (measured: no gain on the large tiers).
-            if (pass < TIERS - 1 && (!X.planned() || pass == first)) {
-                k_sample<false, false><<<g1(items), 256, lay.bytes, s>>>(mols, bmol, cap, lay, M->keys, st, q,
+            if (pass < TIERS - 1 && (!X.planned() || pass == first) && fits_dispatch(items, 256)) {
+                k_sample<false, false><<<gridn(items, 256), 256, lay.bytes, s>>>(mols, bmol, cap, lay, M->keys, st, q,
                                                                          M->values, nullptr, 0, next, nnext, nsamp,
                                                                          cload, hid);
             } else if (pass < TIERS - 1) {
@@ -1670,21 +1677,21 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
             // The large tiers (one or two workgroups per CU) run WIDE
             // threads: more wavefronts per SIMD, more edges in flight.
             const bool wide = pass >= WIDE_FROM;
-            if (pass < TIERS - 1 && (!X.planned() || pass == first)) {
+            if (pass < TIERS - 1 && (!X.planned() || pass == first) && fits_dispatch(items, wide ? WIDE : 256)) {
                 if (fast && wide)
-                    k_refine_fast_wide<false, false><<<g1(items), WIDE, bytes, s>>>(
+                    k_refine_fast_wide<false, false><<<gridn(items, WIDE), WIDE, bytes, s>>>(
                         mols, bmol, cap, lay, flay, M->keys, M->values, q, hid, mask, wpre, words, vbase, vo, nullptr, 0,
                         next, nnext, evd);
                 else if (fast)
-                    k_refine_fast<false, false><<<g1(items), 256, bytes, s>>>(mols, bmol, cap, lay, flay, M->keys,
+                    k_refine_fast<false, false><<<gridn(items, 256), 256, bytes, s>>>(mols, bmol, cap, lay, flay, M->keys,
                                                                               M->values, q, hid, mask, wpre, words, vbase,
                                                                               vo, nullptr, 0, next, nnext, evd);
                 else if (wide)
-                    k_refine_wide<false, false><<<g1(items), WIDE, bytes, s>>>(mols, bmol, cap, lay, M->keys,
+                    k_refine_wide<false, false><<<gridn(items, WIDE), WIDE, bytes, s>>>(mols, bmol, cap, lay, M->keys,
                                                                                M->values, q, hid, mask, wpre, words, vbase,
                                                                                vo, nullptr, 0, next, nnext, evd);
                 else
-                    k_refine<false, false><<<g1(items), 256, bytes, s>>>(mols, bmol, cap, lay, M->keys, M->values, q, hid,
+                    k_refine<false, false><<<gridn(items, 256), 256, bytes, s>>>(mols, bmol, cap, lay, M->keys, M->values, q, hid,
                                                                          mask, wpre, words, vbase, vo, nullptr, 0,
                                                                          next, nnext, evd);
             } else if (pass < TIERS - 1) {
@@ -1832,6 +1839,20 @@ int finish(smg_mesh* M, const Table& t, const char* sum_host, bool timed, smg_me
 
 }  // namespace
 
+// The stream-ordered pool keeps what runs free for the next run. After a
+// run whose temporaries held more than an eighth of the device's memory
+// (a batch of thousands of molecules), all but 8 GiB of it goes back to
+// the device, so a plan or another resident can take it.
+static void trim_pool(smg_device* d) {
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, d->ordinal) != hipSuccess) return;
+    uint64_t reserved = 0;
+    if (hipMemPoolGetAttribute(pool, hipMemPoolAttrReservedMemCurrent, &reserved) != hipSuccess) return;
+    const uint64_t keep = 8ull << 30;
+    if (reserved > d->prop.totalGlobalMem / 8 && reserved > keep) (void)hipMemPoolTrimTo(pool, keep);
+    (void)hipGetLastError();
+}
+
 // The learned sizing a run asks for (params[0].sizing: 0 none, 1 the
 // device's sizing model).
 static int run_sizing(smg_device* d, const smg_mesh_params* prm, size_t nm, Sizing& sz) {
@@ -1866,6 +1887,7 @@ static int mesh_many(smg_resident* const* R, const smg_mesh_params* prm, size_t 
     std::vector<char> stage(stage_layout(t.rows).bytes), sum(sum_layout(t.rows).bytes);
     TRY(pipeline(X, t, E, stage.data(), sum.data(), M, times != nullptr, sz));
     HIP_TRY(hipStreamSynchronize(d->stream));
+    trim_pool(d);
     uint32_t flags = 0;
     TRY(finish(M, t, sum.data(), times != nullptr, times, flags));
     if (flags) return fail(SMG_E_INTERNAL, "device counts exceeded the sizes read back");
