@@ -306,53 +306,103 @@ def batched(dev, count, seed, repeats=2):
             "speedup": min(plain_s) / min(batch_s), "plan_speedup": min(plain_s) / min(plan_s)}
 
 
-def sizing_study(dev, rings, brick):
-    """Learned sizing: the model trained on the device on touched bricks of
-    seeded globules (100 to 50k atoms, 0.35-1.0 A) labelled by the exact
-    pipeline, then the 58k-atom assembly at 0.75 and 1.0 A meshed without
-    sizing, with the default placement and with placement on every tier:
-    device times and wasted gathers (retries). Meshes must be identical.
-    The runs use the default launch shapes (a tuning chosen for another
-    spacing can hide the effect being measured); the device's tuning is
-    restored afterwards."""
+def sizing_study(dev, rings, brick, target=1_000_000, max_molecules=400):
+    """Learned sizing at its stated size. The model is trained on the device
+    on at least `target` touched bricks of seeded globules (atoms
+    log-uniform in 100-60000, spacing 0.25-1.0 A), labelled by the exact
+    classifier and gathers; then applied to the 58k-atom assembly, which it
+    has not seen: (1) plan capacities predicted by the model instead of an
+    exact sizing run, timed to the first mesh, with the fallback count; (2)
+    the assembly at 0.25 A meshed with and without the model's tier
+    placement, byte for byte equal; (3) the same at 1.0 A, where lists
+    overflow the first tier. Default launch shapes (a tuning chosen for one
+    spacing can hide the effect); the device's tuning is restored after."""
+    from concurrent.futures import ThreadPoolExecutor
+
     import sesmesh_amd
 
     tuning = dev.tuning
     dev.tuning = None
     model = sesmesh_amd.Model(dev, seed=7)
+    rng = np.random.default_rng(5)
+    spacings = (0.25, 0.35, 0.5, 0.75, 1.0)
     t = time.perf_counter()
-    spec = [(2000, 0.35), (6000, 0.4), (15000, 0.5), (30000, 0.6), (4000, 0.75), (20000, 0.8), (12000, 1.0),
-            (40000, 1.0), (8000, 0.9), (25000, 0.7), (700, 0.5), (50000, 0.45)]
-    for i, (n, h) in enumerate(spec):
-        model.collect([sesmesh_amd.Resident(dev, sesmesh_amd.Arrangement(sesmesh_amd.synth_globule(100 + i, n), 1.4))],
-                      [h])
+    mols, atoms_total = 0, 0
+    with ThreadPoolExecutor(max_workers=max(1, min(16, host_threads()))) as ex:
+        while model.samples < target and mols < max_molecules:
+            specs = [(int(np.exp(rng.uniform(np.log(100), np.log(60000)))), float(spacings[(mols + k) % 5]),
+                      1000 + mols + k) for k in range(8)]
+            arrs = list(ex.map(lambda sp: sesmesh_amd.Arrangement(sesmesh_amd.synth_globule(sp[2], sp[0]), 1.4,
+                                                                  threads=2), specs))
+            for (n, h, _), a in zip(specs, arrs):
+                model.collect([sesmesh_amd.Resident(dev, a)], [h])
+                atoms_total += n
+            mols += len(specs)
     collect_s = time.perf_counter() - t
+    progress(f"sizing study: {model.samples} labelled bricks from {mols} molecules ({atoms_total} atoms) "
+             f"in {collect_s:.1f} s")
     before = model.train(epochs=0, batch=4096, lr=1e-3)
-    after = model.train(epochs=40, batch=4096, lr=2e-3)
-    model.use()
-    out = {"training": after, "untrained": before, "collect_s": collect_s, "molecules": len(spec), "runs": {}}
+    after = model.train(epochs=30, batch=4096, lr=2e-3)
+    progress(f"sizing study: trained in {after['train_ms']:.0f} ms, accuracy {after['occupancy_accuracy']:.3f}")
+    out = {"training": after, "untrained": before, "collect_s": collect_s, "molecules": mols,
+           "atoms": atoms_total}
     try:
-        for h in (0.75, 1.0):
-            rows = {}
-            ref = None
-            for name, sizing, policy in (("none", 0, None), ("default", 1, (0.3, 1)), ("every_tier", 1, (1.0, 2))):
-                if policy:
-                    model.policy(*policy)
+        # (1) Capacities for the unseen assembly: predicted vs an exact sizing run.
+        h = 0.25
+        t = time.perf_counter()
+        caps, est = sesmesh_amd.Plan.caps_predict([rings], [h], model, brick=brick, margin=0.15)
+        plan = sesmesh_amd.Plan(dev, caps)
+        v = plan.run([rings], [h])
+        first_pred = time.perf_counter() - t
+        got = {"surface_bricks": v.n_bricks, "vertices": v.n_vertices, "triangles": v.n_triangles}
+        pred_arrays = v.arrays(values=True)
+        del v
+        replays = []
+        for _ in range(5):
+            t = time.perf_counter()
+            v = plan.run([rings], [h])
+            replays.append(time.perf_counter() - t)
+            del v
+        pred_stats = plan.stats()
+        del plan
+        t = time.perf_counter()
+        m = rings.mesh(h, brick)
+        plan = sesmesh_amd.Plan(dev, sesmesh_amd.Plan.caps_for([rings], [h], brick=brick, mesh=m, margin=0.05))
+        del m
+        v = plan.run([rings], [h])
+        first_exact = time.perf_counter() - t
+        del v, plan
+        out["capacities"] = {"estimate": est, "actual": got, "predicted_caps": caps.as_dict(),
+                             "first_mesh_s_predicted": first_pred, "first_mesh_s_exact_sizing": first_exact,
+                             "replay_s_per_mesh": float(np.mean(replays)),
+                             "fallbacks": pred_stats["fallbacks"], "margin": 0.15}
+        progress(f"sizing study: predicted {est['surface_bricks']:.0f} surface bricks / {est['vertices']:.0f} "
+                 f"vertices (actual {got['surface_bricks']} / {got['vertices']}); first mesh "
+                 f"{first_pred:.3f} s predicted vs {first_exact:.3f} s exact sizing, fallbacks "
+                 f"{pred_stats['fallbacks']}")
+        # (2), (3) tier placement with and without the model; meshes equal.
+        model.use()
+        out["runs"] = {}
+        for h in (0.25, 1.0):
+            rows, ref = {}, None
+            for name, sizing in (("none", 0), ("model", 1)):
                 best = None
                 for _ in range(3):
                     m = rings.mesh(h, brick, sizing=sizing)
                     if best is None or m.times.total_ms < best[0]:
                         best = (m.times.total_ms, m.times.sample_ms, m.times.refine_ms)
-                    a = m.arrays()
+                    a = m.arrays(values=True)
                     if ref is None:
                         ref = a
-                    elif not all(np.array_equal(ref[k], a[k]) for k in ("positions", "triangles")):
+                    elif not all(np.array_equal(ref[k].view(np.uint8), a[k].view(np.uint8)) for k in ref):
                         raise RuntimeError("sizing changed the mesh")
                 rows[name] = {"total_ms": best[0], "sample_ms": best[1], "refine_ms": best[2],
                               "retries": int(m.info.sample_retries + m.info.refine_retries)}
             out["runs"][str(h)] = rows
+            if h == 0.25 and not all(np.array_equal(pred_arrays[k].view(np.uint8), ref[k].view(np.uint8))
+                                     for k in ref):
+                raise RuntimeError("the plan sized by the model made another mesh")
     finally:
-        model.policy()
         sesmesh_amd.lib().smg_device_set_model(dev._h, None)
         dev.tuning = tuning
     return out

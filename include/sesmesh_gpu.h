@@ -447,6 +447,25 @@ int smg_plan_caps_for(smg_resident* const* residents, const smg_mesh_params* par
                       const smg_mesh* m, double margin, smg_plan_caps* caps);
 int smg_plan_run(smg_plan* p, smg_resident* const* residents, const smg_mesh_params* params, size_t count,
                  smg_mesh** out, smg_mesh_times* times);
+/* Capacities predicted by a sizing model, for molecules no run has seen:
+ * the table side exact, the device side from the model. The device marks
+ * the molecules' touched bricks and runs the model on each: the expected
+ * surface bricks are the sum of the predicted occupancy probabilities;
+ * vertices and triangles follow from the ratios of the meshes the model
+ * was trained on at the nearest spacing (vertices per surface brick per
+ * b^2, triangles per vertex). Each is grown by the fraction `margin`. No mesh stage runs; a
+ * plan run whose counts exceed them falls back to the exact path and grows
+ * (see above), so a low estimate costs time, never a different mesh. `est`
+ * (may be NULL) receives the estimates. */
+typedef struct {
+    uint64_t touched_bricks;
+    double surface_bricks;  /* sum of sigmoid(occupancy logit) */
+    double occupied_bricks; /* bricks with a positive logit */
+    double vertices, triangles;
+    double device_ms;
+} smg_sizing_estimate;
+int smg_plan_caps_predict(smg_resident* const* residents, const smg_mesh_params* params, size_t count,
+                          const smg_model* model, double margin, smg_plan_caps* caps, smg_sizing_estimate* est);
 int smg_plan_stats_get(const smg_plan* p, smg_plan_stats* stats);
 void smg_plan_free(smg_plan* p);
 

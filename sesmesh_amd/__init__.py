@@ -209,7 +209,7 @@ ABI_FUNCTIONS = [
     "smg_model_set_params", "smg_model_collect", "smg_model_samples", "smg_model_dataset", "smg_model_forward",
     "smg_model_gradients", "smg_model_train", "smg_device_set_model", "smg_model_set_policy", "smg_mesh_into",
     "smg_mesh_measure", "smg_mesh_part_measure", "smg_mesh_part_components", "smg_mesh_upload",
-    "smg_export_defaults", "smg_export", "smg_export_file",
+    "smg_export_defaults", "smg_export", "smg_export_file", "smg_plan_caps_predict",
 ]
 
 FORMATS = {"obj": 0, "ply": 1, "msms_vert": 2, "msms_face": 3}
@@ -592,6 +592,25 @@ class Plan:
         _check(lib().smg_plan_caps_for(handles, params, len(residents), mesh._h if mesh is not None else None,
                                        ctypes.c_double(margin), ctypes.byref(caps)))
         return caps
+
+    @staticmethod
+    def caps_predict(residents, spacings, model, brick=10, refine="exact", margin=0.15):
+        """Capacities for molecules no run has seen, from a sizing model
+        (smg_plan_caps_predict): returns (caps, estimates)."""
+
+        class Est(ctypes.Structure):
+            _fields_ = [("touched_bricks", ctypes.c_uint64), ("surface_bricks", ctypes.c_double),
+                        ("occupied_bricks", ctypes.c_double), ("vertices", ctypes.c_double),
+                        ("triangles", ctypes.c_double), ("device_ms", ctypes.c_double)]
+
+        params = (MeshParams * len(residents))(*[_mesh_params(float(h), brick, refine, 0) for h in spacings])
+        handles = (ctypes.c_void_p * len(residents))(*[r._h.value for r in residents])
+        caps, est = PlanCaps(), Est()
+        L = lib()
+        L.smg_plan_caps_predict.argtypes = [_vp, _vp, ctypes.c_size_t, _vp, ctypes.c_double, _vp, _vp]
+        _check(L.smg_plan_caps_predict(handles, params, len(residents), model._h, ctypes.c_double(margin),
+                                       ctypes.byref(caps), ctypes.byref(est)))
+        return caps, {k: getattr(est, k) for k, _ in Est._fields_}
 
     def run(self, residents, spacings):
         """The meshes of `residents` (part i equals smg_mesh_many's part i). A

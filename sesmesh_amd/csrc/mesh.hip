@@ -347,6 +347,16 @@ struct smg_model {
     // past it (measured: never slower than no sizing on 0.5-1.0 A).
     float scale = 0.3f;
     uint32_t max_tier = 1;
+    // Ratios of the collected runs for capacity predictions, per spacing
+    // bucket (a third of an octave from 1/8 A): vertices per surface brick
+    // per b^2 (crossing edges grow with a brick's face area, and with the
+    // surface's roughness on the lattice), triangles per vertex.
+    static constexpr int SIZING_BUCKETS = 16;
+    static int spacing_bucket(double h) {
+        double f = std::floor(3.0 * std::log2(h / 0.125));
+        return (int)std::min<double>(std::max<double>(f, 0.0), SIZING_BUCKETS - 1);
+    }
+    double sum_vertices[SIZING_BUCKETS] = {}, sum_brick_b2[SIZING_BUCKETS] = {}, sum_triangles[SIZING_BUCKETS] = {};
     ~smg_model() {
         void* ps[] = {P, Mo, Ve, W1b, W2b, X, occ, tgt};
         for (void* p : ps)
@@ -2246,6 +2256,14 @@ extern "C" int smg_model_collect(smg_model* m, smg_resident* const* rs, const sm
     M->dev = m->dev;
     TRY(mesh_many(rs, q.data(), count, M.get(), nullptr, m));
     HIP_TRY(hipStreamSynchronize(m->dev->stream));
+    const double b = (double)q[0].brick;
+    for (size_t i = 0; i < M->info.size(); i++) {
+        const smg_mesh_info& I = M->info[i];
+        const int k = smg_model::spacing_bucket(q[i].spacing);
+        m->sum_vertices[k] += (double)I.vertices;
+        m->sum_triangles[k] += (double)I.triangles;
+        m->sum_brick_b2[k] += (double)I.surface_bricks * b * b;
+    }
     return SMG_OK;
 }
 
@@ -2564,6 +2582,130 @@ extern "C" int smg_plan_caps_for(smg_resident* const* residents, const smg_mesh_
         c.triangles = grow(m->tfirst[count]);
     }
     *caps = c;
+    return SMG_OK;
+}
+
+namespace {
+// Per molecule: the sum of sigmoid(occupancy logit) and the count of
+// positive logits of its touched bricks (touched[t]: a bit of the run's
+// bitmaps, molecule m holding words wofs[m] .. wofs[m + 1]).
+__global__ void k_pred_occupancy(const float* Y, const uint32_t* touched, const uint32_t* n_rows,
+                                 const uint64_t* wofs, uint32_t nm, double* out) {
+    const uint32_t n = *n_rows;
+    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+        const uint32_t i = i0 + threadIdx.x;
+        const bool on = i < n;
+        double p = 0.0, c = 0.0;
+        uint32_t m = 0;
+        if (on) {
+            const float z = Y[2 * (size_t)i];
+            p = 1.0 / (1.0 + exp(-(double)z));
+            c = z > 0.0f ? 1.0 : 0.0;
+            m = find_mol(wofs, nm, (uint64_t)(touched[i] / 32));
+        }
+        add_by_key(out, 2 * m, p, on);
+        add_by_key(out, 2 * m + 1, c, on);
+    }
+}
+}  // namespace
+
+extern "C" int smg_plan_caps_predict(smg_resident* const* residents, const smg_mesh_params* params, size_t count,
+                                     const smg_model* model, double margin, smg_plan_caps* caps,
+                                     smg_sizing_estimate* est) {
+    if (!residents || !params || !caps || !model || count == 0) return fail(SMG_E_INVALID, "no molecules or model");
+    if (!(margin >= 0.0 && margin <= 100.0)) return fail(SMG_E_INVALID, "margin out of range");
+    bool calibrated = false;
+    for (int k = 0; k < smg_model::SIZING_BUCKETS; k++) calibrated |= model->sum_brick_b2[k] > 0.0;
+    if (!calibrated)
+        return fail(SMG_E_INVALID, "the model has collected no meshes to calibrate its capacities (smg_model_collect)");
+    TRY(smg_plan_caps_for(residents, params, count, nullptr, margin, caps));
+    smg_device* d = residents[0]->dev;
+    if (model->dev != d) return fail(SMG_E_INVALID, "model and molecules on different devices");
+    HIP_TRY(hipSetDevice(d->ordinal));
+    Table t;
+    TRY(build_table(residents, params, count, count, t));
+    hipStream_t s = d->stream;
+    HIP_TRY(hipEventRecord(d->ev[0], s));
+    Temps Tm(s);
+    Run X;
+    X.d = d;
+    X.s = s;
+    X.T = &Tm;
+    const uint32_t rows = (uint32_t)t.rows;
+    StageLayout SL = stage_layout(rows);
+    char* stage;
+    float* lbuf;
+    uint32_t* cnt;
+    TRY(X.temp(&stage, SL.bytes));
+    TRY(X.temp(&lbuf, 4 * LOC_ARRAYS * std::max<uint64_t>(t.locals(), 1)));
+    TRY(X.temp(&cnt, 2));
+    std::vector<char> host(SL.bytes);
+    write_stage(t, lbuf, host.data());
+    HIP_TRY(hipMemcpyAsync(stage, host.data(), SL.bytes, hipMemcpyHostToDevice, s));
+    const RunHdr* hdr = (const RunHdr*)stage;
+    const Mol* mols = (const Mol*)(stage + SL.mols);
+    const uint64_t* sph_d = (const uint64_t*)(stage + SL.sph);
+    const uint64_t* wofs_d = (const uint64_t*)(stage + SL.wofs);
+    uint32_t *bits, *wcount, *woff, *touched;
+    TRY(X.temp(&bits, t.words()));
+    TRY(X.temp(&wcount, t.words()));
+    TRY(X.temp(&woff, t.words()));
+    TRY(X.temp(&touched, t.dense));
+    k_fill_u32<<<stride_grid(t.words()), 256, 0, s>>>(bits, 0u, 0, &hdr->nwords);
+    k_mark_touched<<<stride_grid(t.spheres(), 128), 128, 0, s>>>(mols, sph_d, rows, bits);
+    k_popcount<<<stride_grid(t.words()), 256, 0, s>>>(bits, wcount, &hdr->nwords);
+    TRY(scan_u32(X, wcount, woff, &hdr->nwords, t.words(), &cnt[0]));
+    k_bits_to_list<<<stride_grid(t.words()), 256, 0, s>>>(bits, woff, &hdr->nwords, touched);
+    uint32_t ntouched = 0;
+    TRY(read_u32(d, &cnt[0], ntouched));
+    float *feat, *pred;
+    double* sums;
+    TRY(X.temp(&feat, (size_t)std::max<uint32_t>(ntouched, 1) * MLP_IN));
+    TRY(X.temp(&pred, (size_t)std::max<uint32_t>(ntouched, 1) * MLP_OUT));
+    TRY(X.temp(&sums, 2 * (size_t)rows));
+    HIP_TRY(hipMemsetAsync(sums, 0, 16 * (size_t)rows, s));
+    if (ntouched) {
+        k_features<<<gridn(ntouched, WAVE), WAVE, 0, s>>>(mols, wofs_d, rows, touched, nullptr, nullptr,
+                                                          Queue{nullptr, &cnt[0], nullptr}, feat);
+        k_mlp_forward<<<g1(std::min<uint64_t>((ntouched + MLP_ROWS - 1) / MLP_ROWS, 4096)), 256, 0, s>>>(
+            feat, &cnt[0], model->P, model->W1b, model->W2b, pred);
+        k_pred_occupancy<<<stride_grid(ntouched), 256, 0, s>>>(pred, touched, &cnt[0], wofs_d, rows, sums);
+        HIP_TRY(hipGetLastError());
+    }
+    std::vector<double> h(2 * (size_t)rows);
+    HIP_TRY(hipMemcpyAsync(h.data(), sums, 16 * (size_t)rows, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipEventRecord(d->ev[1], s));
+    HIP_TRY(hipStreamSynchronize(s));
+    // Each molecule's ratios from the nearest spacing the model has seen.
+    const double b = (double)t.b;
+    double bricks = 0.0, occupied = 0.0, verts = 0.0, tris = 0.0;
+    for (size_t i = 0; i < count; i++) {
+        int k0 = smg_model::spacing_bucket(params[i].spacing), k = -1;
+        for (int dk = 0; dk < smg_model::SIZING_BUCKETS && k < 0; dk++)
+            for (int sgn : {-1, 1}) {
+                int q = k0 + sgn * dk;
+                if (k < 0 && q >= 0 && q < smg_model::SIZING_BUCKETS && model->sum_brick_b2[q] > 0.0) k = q;
+            }
+        const double v = h[2 * i] * b * b * model->sum_vertices[k] / model->sum_brick_b2[k];
+        bricks += h[2 * i];
+        occupied += h[2 * i + 1];
+        verts += v;
+        tris += v * model->sum_triangles[k] / model->sum_vertices[k];
+    }
+    auto grow = [&](double v) { return (uint64_t)std::ceil(v * (1.0 + margin)) + 1; };
+    caps->surface_bricks = std::min<uint64_t>(grow(bricks), t.dense);
+    caps->vertices = grow(verts);
+    caps->triangles = grow(tris);
+    if (est) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, d->ev[0], d->ev[1]));
+        est->touched_bricks = ntouched;
+        est->surface_bricks = bricks;
+        est->occupied_bricks = occupied;
+        est->vertices = verts;
+        est->triangles = tris;
+        est->device_ms = ms;
+    }
     return SMG_OK;
 }
 

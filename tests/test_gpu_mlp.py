@@ -154,3 +154,38 @@ def test_sizing_needs_a_model(device):
     sesmesh_amd.lib().smg_device_set_model(device._h, None)
     with pytest.raises(sesmesh_amd.BackendError):
         res.mesh(0.5, 10, sizing=1)
+
+
+def test_predicted_capacities_size_a_plan(device, trained):
+    """A plan sized by the model (smg_plan_caps_predict) for molecules no
+    run has seen meshes them as smg_mesh_new does; with capacities far too
+    low, the plan falls back to the exact path and grows, with the same
+    mesh."""
+    model, _, _ = trained
+    res = [sesmesh_amd.Resident(device, sesmesh_amd.Arrangement(sesmesh_amd.synth_globule(2000 + i, n), 1.4))
+           for i, n in enumerate((9000, 2500))]
+    hs = [0.35, 0.5]
+    caps, est = sesmesh_amd.Plan.caps_predict(res, hs, model, margin=0.15)
+    want = [r.mesh(h, 10) for r, h in zip(res, hs)]
+    nb = sum(m.n_bricks for m in want)
+    nv = sum(m.n_vertices for m in want)
+    assert est["touched_bricks"] == sum(m.info.touched_bricks for m in want)
+    assert abs(est["surface_bricks"] - nb) < 0.15 * nb and abs(est["vertices"] - nv) < 0.25 * nv, (est, nb, nv)
+    for expect_fallback in (False, True):
+        if expect_fallback:
+            low, _ = sesmesh_amd.Plan.caps_predict(res, hs, model, margin=0.15)
+            low.vertices = 10  # far too few
+            plan = sesmesh_amd.Plan(device, low)
+        else:
+            plan = sesmesh_amd.Plan(device, caps)
+        for run in range(2):
+            v = plan.run(res, hs)
+            for i, m in enumerate(want):
+                a, b = v.arrays(values=True, part=i), m.arrays(values=True)
+                for k in KEYS:
+                    assert same_bits(a[k], b[k]), (run, i, k)
+            del v
+        st = plan.stats()
+        if expect_fallback:
+            assert st["fallbacks"] == 1 and st["last_overflow"] & 2
+        del plan
