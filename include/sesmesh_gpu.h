@@ -181,7 +181,9 @@ int smg_mesh_info_get(const smg_mesh* m, smg_mesh_info* info);
  * 2 concave) per vertex, triangles 3 u32 each, surface brick keys 3 i32
  * each, brick samples ((b+1)^3 f32 per brick, x fastest), per-vertex edge
  * parameter t (f64, unclamped) and evaluations (| 1<<31 if Newton
- * converged). */
+ * converged; | 1<<30 in fast mode for an edge whose f32 search did not
+ * converge and was redone by the exact f64 search, as the reference redoes
+ * such device edges on the CPU, crates/sesmesh/src/mesh/extract.rs:336-358). */
 int smg_mesh_copy(const smg_mesh* m, float* positions, float* normals, uint32_t* atoms, uint8_t* patches,
                   uint32_t* triangles, int32_t* bricks, float* values, double* t, uint32_t* evals);
 

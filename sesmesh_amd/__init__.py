@@ -821,6 +821,7 @@ def precision_report(resident, spacing, brick=10):
         mi = m.measure_info()
         meshes[mode] = g
         out[mode] = {"vertices": int(m.n_vertices), "triangles": int(m.n_triangles),
+                     "redone_exactly": int(((g["evals"] >> 30) & 1).sum()) if mode == "fast" else 0,
                      "max_abs_F": float(f.max()) if len(f) else 0.0,
                      "mean_abs_F": float(f.mean()) if len(f) else 0.0,
                      "area": mi["area"], "volume": mi["volume"], "components": mi["components"]}

@@ -12,7 +12,9 @@
 // Illinois / bisection iteration runs in f32 on the edge parameter t; the
 // vertex is placed, and its normal and atom found, in f64 from that t.
 // Results are not bit-identical to the CPU back end; their distance from
-// the exact surface is what fast mode reports.
+// the exact surface is what fast mode reports. An edge the f32 Newton /
+// Illinois phase does not converge on is redone by the exact f64 search,
+// so it is placed exactly as in exact mode.
 #pragma once
 #include "dev_refine.hpp"
 
@@ -458,17 +460,50 @@ __device__ __forceinline__ void refine_sub_fast(const Mol* __restrict__ mols, co
     uint32_t q = threadIdx.x / g;
     bool active = q < ne;
     EdgeF E;
+    // An edge whose f32 search ends in bisection (not converged) is redone
+    // by the exact f64 search from its start, over the f64 lists the
+    // gather left in place, as the reference redoes unconverged device
+    // edges on the CPU (mesh/extract.rs:336-358).
+    EdgeState X;
+    bool exact = false;
     if (active) edgef_start(P, val, org, s.ebuf[q], E);
     unsigned long long my_evals = 0;
     while (__ballot(active) != 0ull) {
         if (active) {
-            float x, y, z;
-            edgef_point(E, h, c, E.t, x, y, z);
-            EvalF e = eval_group_fast(A, s, f, n, cap.cand, nw, G, x, y, z, probe, cutoff);
-            if (edgef_step(E, e, x, y, z, hf)) {
-                size_t slot = (size_t)vbase[brick] + edge_rank(mask, wpre, words, brick, E.id);
-                edgef_finish(A, P, E, G, c, slot, out);
-                my_evals += E.st >> 16;
+            bool done;
+            if (exact) {
+                double x, y, z;
+                {
+                    EdgeGeom Q = edge_geom(X, h);
+                    x = Q.pa[0] + Q.d[0] * X.t;
+                    y = Q.pa[1] + Q.d[1] * X.t;
+                    z = Q.pa[2] + Q.d[2] * X.t;
+                }
+                EvalD e = eval_group(A, s, n, cap.cand, nw, G, c[0], c[1], c[2], x, y, z, A.probe + P.band);
+                done = edge_step(X, e, h);
+                if (done) {
+                    size_t slot = (size_t)vbase[brick] + edge_rank(mask, wpre, words, brick, X.id);
+                    edge_finish(A, P, X, G, slot, out);
+                    if (G.r == 0 && out.evals && slot < out.cap) out.evals[slot] |= 1u << 30;
+                    my_evals += X.st >> 16;
+                }
+            } else {
+                float x, y, z;
+                edgef_point(E, h, c, E.t, x, y, z);
+                EvalF e = eval_group_fast(A, s, f, n, cap.cand, nw, G, x, y, z, probe, cutoff);
+                done = edgef_step(E, e, x, y, z, hf);
+                if (done && !(E.st & (1u << 11))) {
+                    edge_start(P, val, org, E.id, X);
+                    exact = true;
+                    done = false;
+                } else if (done) {
+                    size_t slot = (size_t)vbase[brick] + edge_rank(mask, wpre, words, brick, E.id);
+                    edgef_finish(A, P, E, G, c, slot, out);
+                    my_evals += E.st >> 16;
+                }
+            }
+            if (done) {
+                exact = false;
                 q += ngroups;
                 active = q < ne;
                 if (active) edgef_start(P, val, org, s.ebuf[q], E);

@@ -59,3 +59,27 @@ def test_fast_mode_needs_one_mode_per_run(device):
     params = [sesmesh_amd._mesh_params(0.5, 10, "exact", 0), sesmesh_amd._mesh_params(0.5, 10, "fast", 0)]
     with pytest.raises(sesmesh_amd.BackendError):
         sesmesh_amd.Mesh(None, None, 10, "exact", 0, _batch=([r, r], params))
+
+
+def test_unconverged_fast_edges_are_redone_exactly(device, structures):
+    """Edges whose f32 Newton / Illinois search does not converge are
+    redone by the exact f64 search (the reference redoes such device edges
+    on the CPU, crates/sesmesh/src/mesh/extract.rs:336-358): those vertices
+    (evaluation word bit 30) equal exact mode's bit for bit."""
+    redone = 0
+    for xyzr, h in ((structures("1UBQ"), 0.25), (sesmesh_amd.synth_globule(9, 3000), 0.5), (structures("2PTN"), 0.35)):
+        res = sesmesh_amd.Resident(device, sesmesh_amd.Arrangement(xyzr, 1.4))
+        e = res.mesh(h, 10, refine="exact").arrays()
+        f = res.mesh(h, 10, refine="fast").arrays()
+        assert np.array_equal(e["triangles"], f["triangles"])
+        sel = ((f["evals"] >> 30) & 1).astype(bool)
+        redone += int(sel.sum())
+        for k in ("positions", "normals"):
+            assert np.array_equal(f[k][sel].view(np.uint32), e[k][sel].view(np.uint32)), k
+        for k in ("atoms", "patches"):
+            assert np.array_equal(f[k][sel], e[k][sel]), k
+        assert np.array_equal(f["t"][sel].view(np.uint64), e["t"][sel].view(np.uint64))
+        assert np.array_equal(f["evals"][sel] & ~np.uint32(1 << 30), e["evals"][sel])
+        # converged fast edges carry bit 31, redone ones never
+        assert not np.any((f["evals"][sel] >> 31) & 1 & ((e["evals"][sel] >> 31) ^ 1))
+    assert redone > 0
