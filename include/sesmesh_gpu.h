@@ -35,6 +35,7 @@ extern "C" {
 #define SMG_E_LENGTH (-6)      /* Error::Length: array sizes inconsistent */
 #define SMG_E_INVALID (-7)     /* invalid input (sesmesh Error::InvalidInput) */
 #define SMG_E_INTERNAL (-8)    /* internal inconsistency (Error::Internal) */
+#define SMG_E_CANCELLED (-9)   /* the caller's cancellation flag was set (Error::Cancelled) */
 
 typedef struct smg_device smg_device;
 typedef struct smg_arrangement smg_arrangement;
@@ -65,6 +66,29 @@ typedef struct {
 
 int smg_device_open(int ordinal, smg_device** out, smg_device_info* info);
 void smg_device_close(smg_device* dev);
+
+/* Cancellation and progress of the device's runs, the reference's Control
+ * (crates/sesmesh/src/control.rs:13-131) passed to compute_surface_with.
+ * With `cancel` set (non-zero), a run returns SMG_E_CANCELLED promptly:
+ * the library polls the flag between passes and while it waits, and once
+ * it is set, raises a flag on the device that stops every running pass
+ * from taking more work. `progress` (may be NULL) is called from the
+ * calling thread with the reference's stages: (0, total) when a stage
+ * starts and (total, total) when it ends; the mesh stage's units are
+ * surface bricks, reported after sampling and after refinement too.
+ * Planned runs report when they end. NULL clears the control. */
+#define SMG_STAGE_GEOMETRY 0
+#define SMG_STAGE_CLASSIFY 1
+#define SMG_STAGE_MESH 2
+#define SMG_STAGE_ASSEMBLE 3
+#define SMG_STAGE_MEASURE 4
+typedef void (*smg_progress_fn)(void* user, int32_t stage, uint64_t done, uint64_t total);
+typedef struct {
+    const volatile int32_t* cancel;
+    smg_progress_fn progress;
+    void* user;
+} smg_control;
+int smg_device_set_control(smg_device* dev, const smg_control* control);
 
 /* Self-test on `n` seeded input pairs (selftest.rs:64-80 input generator):
  * counts of f32 and f64 results whose bits differ from the host's. */

@@ -24,8 +24,11 @@ LIB_PATH = HERE / "libsesmesh_hip.so"
 
 SMG_ERRORS = {
     -1: "NoRuntime", -2: "NoDevice", -3: "Unsupported", -4: "Call", -5: "SelfTest", -6: "Length",
-    -7: "InvalidInput", -8: "Internal",
+    -7: "InvalidInput", -8: "Internal", -9: "Cancelled",
 }
+
+STAGES = {0: "geometry", 1: "classify", 2: "mesh", 3: "assemble", 4: "measure"}
+_PROGRESS = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint64, ctypes.c_uint64)
 
 
 class BackendError(RuntimeError):
@@ -209,7 +212,7 @@ ABI_FUNCTIONS = [
     "smg_model_set_params", "smg_model_collect", "smg_model_samples", "smg_model_dataset", "smg_model_forward",
     "smg_model_gradients", "smg_model_train", "smg_device_set_model", "smg_model_set_policy", "smg_mesh_into",
     "smg_mesh_measure", "smg_mesh_part_measure", "smg_mesh_part_components", "smg_mesh_upload",
-    "smg_export_defaults", "smg_export", "smg_export_file", "smg_plan_caps_predict",
+    "smg_export_defaults", "smg_export", "smg_export_file", "smg_plan_caps_predict", "smg_device_set_control",
 ]
 
 FORMATS = {"obj": 0, "ply": 1, "msms_vert": 2, "msms_face": 3}
@@ -319,6 +322,25 @@ class Device:
             return
         t = Tuning(**{**DEFAULT_TUNING, **dict(values)})
         _check(lib().smg_device_set_tuning(self._h, ctypes.byref(t)))
+
+    def set_control(self, cancel=None, progress=None):
+        """Cancellation and progress of the next runs (smg_device_set_control;
+        reference Control, control.rs): `cancel` a ctypes.c_int32 the caller
+        sets to non-zero to cancel, `progress` called as progress(stage name,
+        done, total). Both None: no control."""
+
+        class Control(ctypes.Structure):
+            _fields_ = [("cancel", _vp), ("progress", _PROGRESS), ("user", _vp)]
+
+        if cancel is None and progress is None:
+            self._control = None
+            _check(lib().smg_device_set_control(self._h, None))
+            return
+        cb = _PROGRESS(lambda user, stage, done, total: progress(STAGES.get(stage, stage), done, total)) \
+            if progress else _PROGRESS()
+        c = Control(ctypes.addressof(cancel) if cancel is not None else None, cb, None)
+        self._control = (cancel, cb, c)  # kept alive while the library may call them
+        _check(lib().smg_device_set_control(self._h, ctypes.byref(c)))
 
     def selftest(self, n=1 << 16, seed=1):
         m32, m64 = ctypes.c_uint64(), ctypes.c_uint64()

@@ -160,10 +160,24 @@ struct Queue {
     uint32_t* next;
 };
 
+// Cancellation of the device's current run. Every run has a generation
+// number, which its first kernel stores in g_run; the host cancels run g by
+// writing g to g_cancel (a copy on another stream, while kernels run), in
+// either order. Work-item kernels stop taking items once they are equal.
+__device__ uint32_t g_cancel = 0u;
+__device__ uint32_t g_run = 0xffffffffu;
+__device__ __forceinline__ bool cancel_raised() {
+    return __hip_atomic_load(&g_cancel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+           __hip_atomic_load(&g_run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__global__ void k_run_begin(const uint32_t* gen) {
+    __hip_atomic_store(&g_run, *gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Takes the workgroup's next item; false once the queue is empty. Every
 // thread of the workgroup calls it (it holds two barriers).
 __device__ __forceinline__ bool queue_take(const Queue& q, uint32_t* ticket, uint32_t& item) {
-    if (threadIdx.x == 0) *ticket = atomicAdd(q.next, 1u);
+    if (threadIdx.x == 0) *ticket = cancel_raised() ? 0xffffffffu : atomicAdd(q.next, 1u);
     __syncthreads();
     // The ticket and item are the workgroup's: reading them as wave-uniform
     // keeps everything derived from them (molecule, lattice, arrangement
@@ -191,7 +205,12 @@ __device__ __forceinline__ void for_each_item(const Queue& q, uint32_t* ticket, 
         }
     } else {
         uint32_t t = blockIdx.x;
-        if (t < *q.count) body(q.list ? q.list[t] : t);
+        if (t >= *q.count) return;
+        // one reading of the flag for the whole workgroup (the body holds barriers)
+        __shared__ uint32_t stop;
+        if (threadIdx.x == 0) stop = cancel_raised();
+        __syncthreads();
+        if (!stop) body(q.list ? q.list[t] : t);
     }
 }
 

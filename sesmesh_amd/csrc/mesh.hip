@@ -13,6 +13,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/sesmesh_gpu.h"
@@ -45,6 +46,14 @@ static int fail(int code, const std::string& msg) {
         hipError_t e_ = (call);                                                                   \
         if (e_ != hipSuccess)                                                                     \
             return fail(SMG_E_CALL, std::string(#call) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+#define TRY(x)                  \
+    do {                        \
+        int rc_ = (x);          \
+        if (rc_ != SMG_OK) {    \
+            return rc_;         \
+        }                       \
     } while (0)
 
 extern "C" const char* smg_last_error(void) { return g_error.c_str(); }
@@ -299,6 +308,14 @@ struct smg_device {
     char* noalloc_stage_h = nullptr;
     char* noalloc_sum_h = nullptr;
     std::mutex noalloc_mu;
+    // Cancellation and progress of the device's runs (smg_device_set_control):
+    // the caller's flag, polled by the host; g_cancel on this device, raised
+    // by a copy on its own stream so running kernels stop taking work.
+    smg_control ctl{};
+    uint32_t* cancel_dev = nullptr;
+    hipStream_t cstream = nullptr;
+    bool raised = false;
+    uint32_t gen = 0;  // generation of the current run
 };
 
 static void device_destroy(smg_device* d);
@@ -456,6 +473,11 @@ extern "C" int smg_device_open(int ordinal, smg_device** out, smg_device_info* i
         }
     }
     (void)hipHostMalloc((void**)&d->host_u32, 64 * sizeof(uint32_t), hipHostMallocDefault);
+    if (!d->host_u32 || hipStreamCreateWithFlags(&d->cstream, hipStreamNonBlocking) != hipSuccess ||
+        hipGetSymbolAddress((void**)&d->cancel_dev, HIP_SYMBOL(g_cancel)) != hipSuccess) {
+        smg_device_close(d);
+        return fail(SMG_E_CALL, "cannot set up cancellation on the device");
+    }
     std::call_once(g_table_once, [] { make_mc_table(g_tri_offset, g_tri_edges); });
     uint8_t lower[12] = {0, 2, 4, 6, 0, 1, 4, 5, 0, 1, 2, 3};
     if (hipMemcpyToSymbol(HIP_SYMBOL(c_tri_offset), g_tri_offset, sizeof(g_tri_offset)) != hipSuccess ||
@@ -512,6 +534,7 @@ static void device_destroy(smg_device* d) {
     if (d->noalloc_stage_h) (void)hipHostFree(d->noalloc_stage_h);
     if (d->noalloc_sum_h) (void)hipHostFree(d->noalloc_sum_h);
     if (d->stream) (void)hipStreamDestroy(d->stream);
+    if (d->cstream) (void)hipStreamDestroy(d->cstream);
     delete d;
 }
 
@@ -867,20 +890,57 @@ struct Temps {
     }
 };
 
+// Cancellation (control.rs:104-118): the caller's flag, and the device flag
+// raised once per run while kernels run.
+bool cancel_requested(const smg_device* d) { return d->ctl.cancel && *d->ctl.cancel != 0; }
+void raise_cancel(smg_device* d) {
+    if (d->raised) return;
+    d->host_u32[63] = d->gen;
+    (void)hipMemcpyAsync(d->cancel_dev, d->host_u32 + 63, 4, hipMemcpyHostToDevice, d->cstream);
+    d->raised = true;
+}
+int check_cancel(smg_device* d) {
+    if (!cancel_requested(d)) return SMG_OK;
+    raise_cancel(d);
+    return fail(SMG_E_CANCELLED, "cancelled");
+}
+// Before a run is enqueued: the caller's flag is clear; the run takes the
+// next generation (its table image carries it to the device).
+int prepare_run(smg_device* d) {
+    TRY(check_cancel(d));
+    d->gen = d->gen == 0xfffffffeu ? 1u : d->gen + 1u;  // never g_run's initial value
+    d->raised = false;
+    return SMG_OK;
+}
+void report(const smg_device* d, int32_t stage, uint64_t done, uint64_t total) {
+    if (d->ctl.progress) d->ctl.progress(d->ctl.user, stage, done, total);
+}
+// Waits for the stream; with a cancellation flag, polls it meanwhile and
+// raises the device flag once it is set (the wait then ends as soon as the
+// kernels stop), returning SMG_E_CANCELLED.
+int wait_stream(smg_device* d, hipStream_t s) {
+    if (!d->ctl.cancel) {
+        HIP_TRY(hipStreamSynchronize(s));
+        return SMG_OK;
+    }
+    for (;;) {
+        hipError_t e = hipStreamQuery(s);
+        if (e == hipSuccess) break;
+        if (e != hipErrorNotReady) return fail(SMG_E_CALL, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+        if (cancel_requested(d)) raise_cancel(d);
+        std::this_thread::yield();
+    }
+    if (d->raised) return fail(SMG_E_CANCELLED, "cancelled");
+    return check_cancel(d);
+}
+
 int read_u32(smg_device* d, const uint32_t* dev_ptr, uint32_t& v) {
     HIP_TRY(hipMemcpyAsync(d->host_u32, dev_ptr, 4, hipMemcpyDeviceToHost, d->stream));
-    HIP_TRY(hipStreamSynchronize(d->stream));
+    TRY(wait_stream(d, d->stream));
     v = d->host_u32[0];
     return SMG_OK;
 }
 
-#define TRY(x)                  \
-    do {                        \
-        int rc_ = (x);          \
-        if (rc_ != SMG_OK) {    \
-            return rc_;         \
-        }                       \
-    } while (0)
 
 // List capacities, tried in turn: LDS variants sized for common sub-blocks
 // (several workgroups per CU) and for those of coarse lattices (up to the
@@ -1012,6 +1072,7 @@ int device_statics(smg_device* d, const uint32_t** out) {
 // Run-wide extents the kernels read from the uploaded table.
 struct RunHdr {
     uint32_t nwords, dense, locals, spheres;
+    uint32_t gen;  // the run's generation (cancellation)
 };
 
 inline uint32_t g1(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(n, 1u << 30)); }
@@ -1263,7 +1324,7 @@ constexpr int LOC_ARRAYS = 5;
 
 // Writes the table into `host` (laid out as stage_layout), pointing each
 // molecule's f32 copies into `lbuf`.
-void write_stage(Table& t, float* lbuf, char* host) {
+void write_stage(Table& t, float* lbuf, char* host, uint32_t gen) {
     StageLayout L = stage_layout(t.rows);
     for (size_t m = 0; m < t.rows; m++) {
         DLoc& Lc = t.H[m].L;
@@ -1272,7 +1333,7 @@ void write_stage(Table& t, float* lbuf, char* host) {
         gptr<float4>* fields[LOC_ARRAYS] = {&Lc.p4, &Lc.m4, &Lc.u4, &Lc.e4, &Lc.v4};
         for (int q = 0; q < LOC_ARRAYS; q++) *fields[q] = to_global(base + q * nmax);
     }
-    RunHdr h{(uint32_t)t.words(), (uint32_t)t.dense, (uint32_t)t.locals(), (uint32_t)t.spheres()};
+    RunHdr h{(uint32_t)t.words(), (uint32_t)t.dense, (uint32_t)t.locals(), (uint32_t)t.spheres(), gen};
     std::memset(host, 0, L.bytes);
     std::memcpy(host, &h, sizeof(h));
     std::memcpy(host + L.mols, t.H.data(), t.rows * sizeof(Mol));
@@ -1353,10 +1414,11 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     TRY(X.temp(&sum, UL.bytes));
     TRY(X.temp(&lbuf, 4 * LOC_ARRAYS * E.locals));
     X.lbuf = lbuf;
-    write_stage(t, lbuf, stage_host);
+    write_stage(t, lbuf, stage_host, d->gen);
     TRY(rec(0));
     HIP_TRY(hipMemcpyAsync(stage, stage_host, SL.bytes, hipMemcpyHostToDevice, s));
     const RunHdr* hdr = (const RunHdr*)stage;
+    k_run_begin<<<1, 1, 0, s>>>(&hdr->gen);
     const Mol* mols = (const Mol*)(stage + SL.mols);
     const uint64_t* sph_d = (const uint64_t*)(stage + SL.sph);
     const uint64_t* loc_d = (const uint64_t*)(stage + SL.loc);
@@ -1369,6 +1431,8 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     uint32_t* sfirst = (uint32_t*)(sum + UL.sfirst);
     uint32_t* vfirst = (uint32_t*)(sum + UL.vfirst);
     uint32_t* trfirst = (uint32_t*)(sum + UL.trfirst);
+    const bool report_phases = !X.planned();  // a planned run reports from smg_plan_run
+    if (report_phases) report(d, SMG_STAGE_CLASSIFY, 0, 1);
     k_fill_u32<<<stride_grid(UL.zero_words), 256, 0, s>>>((uint32_t*)sum, 0u, UL.zero_words, nullptr);
     k_localize<<<stride_grid(E.locals), 256, 0, s>>>(mols, loc_d, rows);
     HIP_TRY(hipGetLastError());
@@ -1400,6 +1464,10 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     HIP_TRY(hipGetLastError());
     uint32_t nb;
     TRY(X.size(&cnt[K_NB], E.nb, nb));
+    if (report_phases) {
+        report(d, SMG_STAGE_CLASSIFY, 1, 1);
+        report(d, SMG_STAGE_MESH, 0, nb);
+    }
     uint32_t* bmol;
     uint8_t* st;
     TRY(X.output(&M->keys, (size_t)nb * 3));
@@ -1613,6 +1681,7 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     uint32_t nverts, nrw;
     TRY(X.size(&cnt[K_NV], E.nv, nverts));
     TRY(X.size(&cnt[K_NRW], (uint32_t)nsub, nrw));
+    if (report_phases) report(d, SMG_STAGE_MESH, nb / 2, nb);  // sampled; edges numbered
     TRY(rec(3));
 
     // Refinement and attributes (mesh/extract.rs:425-597).
@@ -1758,6 +1827,7 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     HIP_TRY(hipGetLastError());
     uint32_t ntris;
     TRY(X.size(&cnt[K_NT], E.nt, ntris));
+    if (report_phases) report(d, SMG_STAGE_MESH, nb - nb / 10, nb);  // refined
     TRY(X.output(&M->tris, 3 * (size_t)ntris));
     k_cell_emit<<<stride_grid(nb, 1), 256, staged ? cell_lds : 0, s>>>(mols, bmol, fbrick, M->values, M->keys,
                                                                         &cnt[K_NB], tbase, dense, vbase, mask, wpre,
@@ -1895,12 +1965,17 @@ static int mesh_many(smg_resident* const* R, const smg_mesh_params* prm, size_t 
     X.T = &T;
     Extents E{t.rows, t.locals(), t.spheres(), t.words(), t.dense, NO_CAP, NO_CAP, NO_CAP};
     std::vector<char> stage(stage_layout(t.rows).bytes), sum(sum_layout(t.rows).bytes);
+    TRY(prepare_run(d));
     TRY(pipeline(X, t, E, stage.data(), sum.data(), M, times != nullptr, sz));
-    HIP_TRY(hipStreamSynchronize(d->stream));
+    TRY(wait_stream(d, d->stream));
     trim_pool(d);
     uint32_t flags = 0;
     TRY(finish(M, t, sum.data(), times != nullptr, times, flags));
     if (flags) return fail(SMG_E_INTERNAL, "device counts exceeded the sizes read back");
+    uint64_t nb = M->bfirst.empty() ? 0 : M->bfirst.back();
+    report(d, SMG_STAGE_MESH, nb, nb);
+    report(d, SMG_STAGE_ASSEMBLE, 0, 1);
+    report(d, SMG_STAGE_ASSEMBLE, 1, 1);
     return SMG_OK;
 }
 
@@ -1993,16 +2068,25 @@ extern "C" int smg_mesh_into(smg_resident* r, const smg_mesh_params* p, const sm
     M.atom = out->atoms;
     M.patch = out->patches;
     M.tris = out->triangles;
+    TRY(prepare_run(d));
     int rc = pipeline(X, t, E, d->noalloc_stage_h, d->noalloc_sum_h, &M, false, sz);
-    hipError_t e = hipStreamSynchronize(d->stream);
-    if (rc != SMG_OK) return rc;
-    if (e != hipSuccess) return fail(SMG_E_CALL, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+    if (rc != SMG_OK) {
+        (void)hipStreamSynchronize(d->stream);
+        return rc;
+    }
+    TRY(wait_stream(d, d->stream));
     uint32_t flags = 0;
     TRY(finish(&M, t, d->noalloc_sum_h, false, nullptr, flags));
     if (flags & OVER_VERTICES) return fail(SMG_E_LENGTH, "more vertices than the output's vertex capacity");
     if (flags & OVER_TRIANGLES) return fail(SMG_E_LENGTH, "more triangles than the output's triangle capacity");
     if (flags) return fail(SMG_E_INTERNAL, "a device count exceeded its capacity");
     if (info) *info = M.info[0];
+    return SMG_OK;
+}
+
+extern "C" int smg_device_set_control(smg_device* d, const smg_control* c) {
+    if (!d) return fail(SMG_E_INVALID, "null device");
+    d->ctl = c ? *c : smg_control{};
     return SMG_OK;
 }
 
@@ -2640,9 +2724,11 @@ extern "C" int smg_plan_caps_predict(smg_resident* const* residents, const smg_m
     TRY(X.temp(&lbuf, 4 * LOC_ARRAYS * std::max<uint64_t>(t.locals(), 1)));
     TRY(X.temp(&cnt, 2));
     std::vector<char> host(SL.bytes);
-    write_stage(t, lbuf, host.data());
+    TRY(prepare_run(d));
+    write_stage(t, lbuf, host.data(), d->gen);
     HIP_TRY(hipMemcpyAsync(stage, host.data(), SL.bytes, hipMemcpyHostToDevice, s));
     const RunHdr* hdr = (const RunHdr*)stage;
+    k_run_begin<<<1, 1, 0, s>>>(&hdr->gen);
     const Mol* mols = (const Mol*)(stage + SL.mols);
     const uint64_t* sph_d = (const uint64_t*)(stage + SL.sph);
     const uint64_t* wofs_d = (const uint64_t*)(stage + SL.wofs);
@@ -2758,11 +2844,14 @@ extern "C" int smg_plan_run(smg_plan* p, smg_resident* const* residents, const s
         V->plan = p;  // a view: the arrays are the plan's
         p->views.push_back(V.get());
         V->spb = (t.b + 1) * (t.b + 1) * (t.b + 1);
+        TRY(prepare_run(d));
+        report(d, SMG_STAGE_CLASSIFY, 0, 1);
+        TRY(check_cancel(d));
         HIP_TRY(hipEventRecord(d->ev[14], d->stream));
         if (p->exec) {
             // The graph uploads the table image and uses the buffers of the
             // capture: refill the image, then replay.
-            write_stage(t, p->lbuf, p->stage_h);
+            write_stage(t, p->lbuf, p->stage_h, d->gen);
             HIP_TRY(hipGraphLaunch(p->exec, d->stream));
             p->stats.replays++;
             V->keys = (int32_t*)p->out[0];
@@ -2779,8 +2868,16 @@ extern "C" int smg_plan_run(smg_plan* p, smg_resident* const* residents, const s
         }
         const bool timed = times != nullptr && (p->exec ? p->timed : true);
         HIP_TRY(hipEventRecord(d->ev[15], d->stream));
-        HIP_TRY(hipStreamSynchronize(d->stream));
+        TRY(wait_stream(d, d->stream));
         TRY(finish(V.get(), t, p->sum_h, timed, times, flags));
+        if (!flags) {
+            const uint64_t nb = V->bfirst.empty() ? 0 : V->bfirst.back();
+            report(d, SMG_STAGE_CLASSIFY, 1, 1);
+            report(d, SMG_STAGE_MESH, 0, nb);
+            report(d, SMG_STAGE_MESH, nb, nb);
+            report(d, SMG_STAGE_ASSEMBLE, 0, 1);
+            report(d, SMG_STAGE_ASSEMBLE, 1, 1);
+        }
         if (!flags) {
             if (!p->exec) {
                 // Record the graph the next runs replay (nothing executes),
@@ -2884,6 +2981,8 @@ extern "C" int smg_mesh_measure(smg_mesh* M, double* ms) {
     const uint64_t V = M->vfirst[P], T = M->tfirst[P];
     if (3 * T >= 0xffffffffull) return fail(SMG_E_UNSUPPORTED, "too many triangles in one mesh to measure");
     hipStream_t s = d->stream;
+    TRY(prepare_run(d));
+    report(d, SMG_STAGE_MEASURE, 0, 1);
     if (ms) HIP_TRY(hipEventRecord(d->ev[0], s));
     Temps Tm(s);
     Run X;
@@ -3035,6 +3134,7 @@ extern "C" int smg_mesh_measure(smg_mesh* M, double* ms) {
         I.measured = 1;
     }
     M->measured = true;
+    report(d, SMG_STAGE_MEASURE, 1, 1);
     if (ms) {
         float f = 0.0f;
         HIP_TRY(hipEventElapsedTime(&f, d->ev[0], d->ev[1]));
