@@ -101,9 +101,23 @@ int smg_selftest(smg_device* dev, size_t n, uint64_t seed, uint64_t* f32_mismatc
  * (crates/sesmesh/src/field/point.rs:213-274). `xyzr` holds n rows of centre
  * x, y, z and radius; radius 0 is ignored. `threads` <= 0 means all cores.
  * The caller of the reference builds this on the CPU before the mesh stage;
- * it is provided here so a user of this library has the whole path. */
+ * it is provided here so a user of this library has the whole path.
+ * smg_arrangement_build is the SES kind; probe 0 gives the vdW kind
+ * (SesField::new_with, point.rs:224-228). */
 int smg_arrangement_build(const double* xyzr, size_t n, double probe, int threads, smg_arrangement** out);
 void smg_arrangement_free(smg_arrangement* a);
+
+/* Surface kinds (SurfaceKind, crates/sesmesh/src/field/mod.rs:33-41). The SAS
+ * and vdW surfaces are unions of spheres: their arrangements carry no
+ * circles or SAS vertices (build_with(.., with_circles = false, ..),
+ * point.rs:229-237); the vdW one is built with probe 0, whatever `probe`
+ * says (SurfaceParams::effective_probe, surface.rs:411-417). */
+#define SMG_KIND_SES 0
+#define SMG_KIND_SAS 1
+#define SMG_KIND_VDW 2
+/* SesField::new(spheres, probe, kind) (point.rs:208-274) on the host. */
+int smg_arrangement_build_kind(const double* xyzr, size_t n, double probe, int kind, int threads,
+                               smg_arrangement** out);
 
 typedef struct {
     const double* origin; /* 3 */
@@ -137,6 +151,7 @@ typedef struct {
     const uint32_t* vertex_atoms; /* 3 per SAS vertex */
     smg_grid_ref sphere_grid, circle_grid, vertex_grid;
     double probe, max_expanded, rho_max;
+    int32_t kind; /* SMG_KIND_* */
 } smg_arrangement_ref;
 
 /* View of a built arrangement (valid while it lives). */
@@ -211,11 +226,13 @@ int smg_mesh_info_get(const smg_mesh* m, smg_mesh_info* info);
 int smg_mesh_copy(const smg_mesh* m, float* positions, float* normals, uint32_t* atoms, uint8_t* patches,
                   uint32_t* triangles, int32_t* bricks, float* values, double* t, uint32_t* evals);
 
-/* The exact clamped SES field at n points (xyz f64 each): values[i] =
- * probe - ses_eval(x, probe + band).distance, the distance being 0 outside
- * the SAS; f64, bit for bit the CPU's SesField::clamped_value
- * (crates/sesmesh/src/field/point.rs:759-765, ses_eval :384-411). Used to
- * measure how far a fast-mode vertex lies from the exact surface. */
+/* The exact clamped field at n points (xyz f64 each): for the SES kind
+ * values[i] = probe - ses_eval(x, probe + band).distance, the distance
+ * being 0 outside the SAS; for the SAS and vdW kinds sphere_eval(x, band),
+ * the least |x - c| - R below band; f64, bit for bit the CPU's
+ * SesField::clamped_value (crates/sesmesh/src/field/point.rs:759-765,
+ * ses_eval :384-411, sphere_eval :742-756). Used to measure how far a
+ * fast-mode vertex lies from the exact surface. */
 int smg_field_clamped(smg_resident* r, const double* points, size_t n, double band, double* values);
 
 /* Batched mesh stage: meshes `count` resident molecules of one device in one

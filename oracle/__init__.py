@@ -33,7 +33,7 @@ def lib():
         L = ctypes.CDLL(str(LIB))
         L.orc_surface_new.restype = _c_p
         L.orc_surface_new.argtypes = [_c_p, ctypes.c_size_t, ctypes.c_double, ctypes.c_double, ctypes.c_size_t,
-                                      ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
         L.orc_surface_free.argtypes = [_c_p]
         L.orc_field_sample.restype = ctypes.c_double
         L.orc_time_mesh_sample.restype = ctypes.c_double
@@ -54,22 +54,26 @@ def _ptr(a):
     return a.ctypes.data_as(_c_p) if a is not None else None
 
 
+KINDS = {"ses": 0, "sas": 1, "vdw": 2}
+
+
 class Surface:
     """The oracle's pipeline on one sphere set (`stage` 1: field and engine,
-    2: plus brick classification, 3: plus meshing and assembly)."""
+    2: plus brick classification, 3: plus meshing and assembly) for surface
+    kind `kind` ("ses", "sas" or "vdw"; the vdW kind takes probe 0)."""
 
-    def __init__(self, xyzr, probe=1.4, spacing=0.5, brick=10, threads=None, stage=3):
+    def __init__(self, xyzr, probe=1.4, spacing=0.5, brick=10, threads=None, stage=3, kind="ses"):
         L = lib()
         a = np.ascontiguousarray(xyzr, dtype=np.float64).reshape(-1, 4)
         self.xyzr = a
         if threads is None:
             threads = min(8, os.cpu_count() or 1)
         err = ctypes.create_string_buffer(512)
-        h = L.orc_surface_new(_ptr(a), len(a), probe, spacing, brick, threads, stage, err, 512)
+        h = L.orc_surface_new(_ptr(a), len(a), probe, spacing, brick, threads, stage, KINDS[kind], err, 512)
         if not h:
             raise ValueError(err.value.decode())
         self._h = h
-        self.probe, self.spacing, self.brick = probe, spacing, brick
+        self.probe, self.spacing, self.brick, self.kind = probe, spacing, brick, kind
         c = (ctypes.c_int64 * 14)()
         L.orc_surface_counts(_c_p(h), c)
         (self.touched, self.n_bricks, self.n_vertices, self.n_triangles, self.evals, self.n_spheres,

@@ -20,10 +20,11 @@ void set_err(char* err, size_t len, const std::string& s) {
 
 extern "C" {
 
-// Builds the field and engine, then (stage >= 2) classifies bricks and
-// (stage >= 3) meshes them with `threads` threads. Returns NULL on error.
+// Builds the field and engine of surface kind `kind` (0 SES, 1 SAS, 2 vdW),
+// then (stage >= 2) classifies bricks and (stage >= 3) meshes them with
+// `threads` threads. Returns NULL on error.
 void* orc_surface_new(const double* xyzr, size_t n, double probe, double spacing, size_t brick, int threads,
-                      int stage, char* err, size_t errlen) {
+                      int stage, int kind, char* err, size_t errlen) {
     std::vector<Sphere> spheres(n);
     for (size_t i = 0; i < n; i++) {
         spheres[i].c[0] = xyzr[4 * i];
@@ -34,9 +35,9 @@ void* orc_surface_new(const double* xyzr, size_t n, double probe, double spacing
     Surface* S = new Surface();
     std::string e;
     if (stage >= 3) {
-        e = compute_surface(spheres, probe, spacing, brick, threads, *S);
+        e = compute_surface(spheres, probe, spacing, brick, threads, *S, kind);
     } else {
-        e = S->field.build(spheres, probe);
+        e = S->field.build(spheres, kind == K_VDW ? 0.0 : probe, kind);
         if (e.empty()) e = S->engine.init(&S->field, spacing, brick);
         if (e.empty() && stage >= 2) e = classify_bricks(*S, threads);
     }

@@ -1,6 +1,6 @@
 // Oracle: lattice, brick classification, per-sub-block gathering and the f32
-// lane kernel (test infrastructure). Restates field/lattice.rs and
-// field/kernel.rs of the reference (crates/sesmesh/src/), SES kind.
+// lane kernels (test infrastructure). Restates field/lattice.rs and
+// field/kernel.rs of the reference (crates/sesmesh/src/), every surface kind.
 #include <algorithm>
 
 #include "oracle.hpp"
@@ -56,7 +56,7 @@ std::string touched_bricks(const SesField& f, const Lattice& l, std::vector<u64>
     return "";
 }
 
-// field/lattice.rs:190-227 (SES)
+// field/lattice.rs:190-227
 uint8_t uniform_sign(const SesField& f, const V3& center, double rho) {
     const Arrangement& a = f.arr;
     double margin = rho + CLASSIFY_EPS;
@@ -68,6 +68,11 @@ uint8_t uniform_sign(const SesField& f, const V3& center, double rho) {
             s_vdw = std::fmin(s_vdw, d - a.radii[k]);
         }
     });
+    if (f.kind != K_SES) {  // expanded is the SAS radius, or the vdW radius (probe 0)
+        if (s_sas > margin) return SS_OUT;
+        if (s_sas < -margin) return SS_IN;
+        return SS_NONE;
+    }
     if (s_sas >= margin) return SS_OUT;
     if (s_vdw <= -margin) return SS_IN;
     double cutoff = f.probe + margin + CLASSIFY_EPS;
@@ -200,8 +205,7 @@ void Engine::assign(const int32_t key[3], const uint8_t st[8], BrickScratch& s) 
                     s.groups[owner].push_back((u32)i);
                 } else {
                     int64_t idx[3] = {origin[0] + x, origin[1] + y, origin[2] + z};
-                    double v = field->probe - field->ses_eval(lattice.point(idx), field->probe + band).distance;
-                    s.values[i] = (float)v;
+                    s.values[i] = (float)field->clamped_value(lattice.point(idx), band);
                 }
                 i++;
             }
@@ -259,7 +263,10 @@ bool Engine::prepare_sub(const int32_t key[3], size_t sb, BrickScratch& s, size_
         s.ys[k] = s.ys[m - 1];
         s.zs[k] = s.zs[m - 1];
     }
-    gather_ses(center, rho, s);
+    if (field->kind == K_SES)
+        gather_ses(center, rho, s);
+    else
+        gather_spheres(center, rho, band, s);
     return true;
 }
 
@@ -290,6 +297,12 @@ void Engine::gather_near(const V3& b, double rho, double reach, BrickScratch& s)
         });
         if (pass == 0) g.n_inside = g.nx.size();
     }
+}
+
+// field/kernel.rs:645-652
+void Engine::gather_spheres(const V3& b, double rho, double band_, BrickScratch& s) const {
+    gather_near(b, rho, band_ + rho, s);
+    for (u32 k : s.touched) s.local_of[k] = UINT32_MAX;
 }
 
 // field/kernel.rs:659-751
@@ -449,6 +462,22 @@ float ses_lane(const Gathered& g, float x, float y, float z, float probe, float 
     return result;
 }
 
+// field/kernel.rs:953-981 for one lane: min(band, min_k(|x - c_k| - R_k))
+// over every gathered sphere, IEEE f32.
+float sphere_lane(const Gathered& g, float x, float y, float z, float band) {
+    float best = band;
+    for (size_t k = 0; k < g.nx.size(); k++) {
+        float dx = x - g.nx[k], dy = y - g.ny[k], dz = z - g.nz[k];
+        float v = std::sqrt(dx * dx + dy * dy + dz * dz) - g.nr[k];
+        best = std::fmin(best, v);
+    }
+    if (g_ops.on) {
+        g_ops.f32_ops += 10 * (uint64_t)g.nx.size();
+        g_ops.samples++;
+    }
+    return best;
+}
+
 // field/kernel.rs:415-449
 void Engine::evaluate(const int32_t key[3], BrickScratch& s) const {
     uint8_t st[8];
@@ -458,12 +487,15 @@ void Engine::evaluate(const int32_t key[3], BrickScratch& s) const {
     float probe = (float)field->probe;
     float cutoff = (float)(field->probe + band);
     float cutoff2 = cutoff * cutoff;
+    float fband = (float)band;
     size_t nsb = sub_blocks();
     for (size_t sb = 0; sb < nsb; sb++) {
         size_t padded;
         if (!prepare_sub(key, sb, s, padded)) continue;
         for (size_t j = 0; j < s.groups[sb].size(); j++) {
-            s.values[s.groups[sb][j]] = ses_lane(s.g, s.xs[j], s.ys[j], s.zs[j], probe, cutoff2);
+            s.values[s.groups[sb][j]] = field->kind == K_SES
+                                            ? ses_lane(s.g, s.xs[j], s.ys[j], s.zs[j], probe, cutoff2)
+                                            : sphere_lane(s.g, s.xs[j], s.ys[j], s.zs[j], fband);
         }
     }
 }

@@ -1,5 +1,6 @@
 // Oracle: exact f64 point field (test infrastructure).
-// Restates field/point.rs of the reference (crates/sesmesh/src/), SES kind.
+// Restates field/point.rs of the reference (crates/sesmesh/src/): the SES
+// kind, and the SAS and vdW kinds (unions of spheres).
 #include <algorithm>
 
 #include "oracle.hpp"
@@ -44,12 +45,14 @@ void LocalCandidates::clear() {
     order.clear();
 }
 
-// field/point.rs:213-274 (SES; probe 0 would be the vdW kind, not supported)
-std::string SesField::build(const std::vector<Sphere>& spheres, double p) {
+// field/point.rs:213-274
+std::string SesField::build(const std::vector<Sphere>& spheres, double p, int k) {
     if (!(std::isfinite(p) && p >= 0.0)) return "probe radius must be finite and >= 0";
-    if (p == 0.0) return "probe radius 0 (vdW surface) is not supported";
-    std::string err = build_arrangement(spheres, p, arr);
+    if (k != K_SES && k != K_SAS && k != K_VDW) return "unknown surface kind";
+    if (k == K_SES && p == 0.0) k = K_VDW;
+    std::string err = build_arrangement(spheres, k == K_VDW ? 0.0 : p, arr, k == K_SES);
     if (!err.empty()) return err;
+    kind = k;
     probe = p;
     // Drop neighbour rows of non-exposed spheres (field/point.rs:243-244).
     {
@@ -167,10 +170,21 @@ void SesField::try_vertex(const V3& x, u32 vi, PointEval& best) const {
     if (d < best.distance) best = PointEval{d, p, Feature{F_VERTEX, vi}};
 }
 
-// field/point.rs:496-565 (SES)
+// field/point.rs:496-581
 void SesField::gather_local(const V3& b, double rho, double band, LocalCandidates& out) const {
     const Arrangement& a = arr;
     out.clear();
+    if (kind != K_SES) {  // point.rs:566-578: every sphere within reach, nearest first
+        double reach = band + rho;
+        a.cells.for_each_near(b, reach + a.max_expanded, [&](u32 k) {
+            if (a.status[k] != CONTAINED && b.dist(a.centers[k]) - a.expanded[k] < reach)
+                out.order.push_back({b.dist(a.centers[k]) - a.expanded[k], k});
+        });
+        sort_nearest(out.order);
+        for (auto& o : out.order) out.inside.push_back(o.second);
+        out.order.clear();
+        return;
+    }
     double reach = probe + band + rho;
     a.cells.for_each_near(b, reach + a.max_expanded, [&](u32 k) {
         if (a.status[k] == CONTAINED) return;
@@ -216,7 +230,7 @@ void SesField::gather_local(const V3& b, double rho, double band, LocalCandidate
     out.order.clear();
 }
 
-// field/point.rs:686-722 (SES)
+// field/point.rs:686-738
 NearEval SesField::eval_local(const V3& x, double band, const LocalCandidates& l) const {
     const Arrangement& a = arr;
     struct Scope {
@@ -224,6 +238,19 @@ NearEval SesField::eval_local(const V3& x, double band, const LocalCandidates& l
         ~Scope() { g_in_eval_local = false; }
     } scope;
     if (g_ops.on) g_ops.evals++;
+    if (kind != K_SES) {  // point.rs:723-736: closest point is x itself
+        double best = band;
+        int64_t bi = -1;
+        for (u32 k : l.inside) {
+            double v = x.dist(a.centers[k]) - a.expanded[k];
+            count64(11);
+            if (v < best) {
+                best = v;
+                bi = k;
+            }
+        }
+        return NearEval{best, bi < 0 ? Feature{F_BEYOND, 0} : Feature{F_SPHERE, (u32)bi}, x};
+    }
     bool inside = false;
     for (u32 k : l.inside) {
         count64(10);
@@ -254,12 +281,33 @@ std::pair<double, int64_t> SesField::sphere_eval(const V3& x, double cutoff) con
     return best;
 }
 
-// field/point.rs:292-339 (SES)
+// field/point.rs:292-369
 double SesField::sample(const V3& x, V3& closest, Feature& f) const {
     if (arr.centers.empty()) {
         closest = x;
         f = Feature{F_SOLVENT, 0};
-        return probe;
+        return kind == K_SES ? probe : INFINITY;
+    }
+    if (kind != K_SES) {
+        double cutoff = 4.0;
+        std::pair<double, int64_t> e;
+        for (;;) {
+            e = sphere_eval(x, cutoff);
+            if (e.second >= 0 || cutoff > extent) break;
+            cutoff *= 2.0;
+        }
+        if (e.second < 0) {
+            closest = x;
+            f = Feature{F_SOLVENT, 0};
+            return e.first;
+        }
+        const V3& c = arr.centers[e.second];
+        V3 dir = x - c;
+        double len = dir.norm();
+        V3 unit = len > 1e-12 ? dir / len : V3(1.0, 0.0, 0.0);
+        closest = c + unit * arr.expanded[e.second];
+        f = Feature{F_SPHERE, (u32)e.second};
+        return e.first;
     }
     double cutoff = probe + 4.0;
     PointEval e;
@@ -271,6 +319,12 @@ double SesField::sample(const V3& x, V3& closest, Feature& f) const {
     closest = e.closest;
     f = e.feature;
     return probe - e.distance;
+}
+
+// field/point.rs:760-765
+double SesField::clamped_value(const V3& x, double band) const {
+    if (kind == K_SES) return probe - ses_eval(x, probe + band).distance;
+    return sphere_eval(x, band).first;
 }
 
 }  // namespace orc

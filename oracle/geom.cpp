@@ -239,7 +239,7 @@ bool exposed_arcs(std::vector<Buried>& iv, std::vector<Merged>& merged) {
 }  // namespace
 
 // geom/arrangement.rs:437-645 and :672-763
-std::string build_arrangement(const std::vector<Sphere>& spheres, double probe, Arrangement& A) {
+std::string build_arrangement(const std::vector<Sphere>& spheres, double probe, Arrangement& A, bool with_circles) {
     if (!(std::isfinite(probe) && probe >= 0.0)) return "probe radius must be finite and >= 0";
     for (size_t i = 0; i < spheres.size(); i++) {
         const Sphere& s = spheres[i];
@@ -325,12 +325,12 @@ std::string build_arrangement(const std::vector<Sphere>& spheres, double probe, 
     for (size_t i = 0; i < n; i++) {
         size_t nr;
         A.neighbors.row(i, nr);
-        exposed[i] = !contained[i] && nr == 0;
+        exposed[i] = !contained[i] && (!with_circles || nr == 0);  // arrangement.rs:566-568
     }
     std::vector<Buried> intervals;
     std::vector<Merged> merged;
     std::vector<u32> blockers;
-    for (size_t i = 0; i < n; i++) {
+    for (size_t i = 0; i < n && with_circles; i++) {
         if (contained[i]) continue;
         size_t ni;
         const u32* rowi = A.neighbors.row(i, ni);

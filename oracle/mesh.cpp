@@ -31,17 +31,23 @@ double edge_clamp(const Engine& e) {
 
 namespace {
 
-// mesh/extract.rs:539-549 (SES)
-bool gradient(const V3& x, const NearEval& e, V3& out) {
+// mesh/extract.rs:539-549
+bool gradient(const SesField& f, const V3& x, const NearEval& e, V3& out) {
     if (e.feature.kind == F_SOLVENT || e.feature.kind == F_BEYOND) return false;
-    V3 w = e.closest - x;
+    V3 w;
+    if (f.kind == K_SES)
+        w = e.closest - x;
+    else if (e.feature.kind == F_SPHERE)
+        w = x - f.arr.centers[e.feature.id];
+    else
+        return false;
     double len = w.norm();
     if (!(len > 1e-12)) return false;
     out = w / len;
     return true;
 }
 
-// mesh/extract.rs:553-597 (SES)
+// mesh/extract.rs:553-597
 void attributes(const Engine& eng, const V3& v, const NearEval& e, const V3& outward, V3& normal,
                 uint8_t& patch, u32& atom) {
     const SesField& field = *eng.field;
@@ -63,7 +69,13 @@ void attributes(const Engine& eng, const V3& v, const NearEval& e, const V3& out
         }
         return a.orig[bk];
     };
-    switch (e.feature.kind) {
+    if (field.kind != K_SES && e.feature.kind == F_SPHERE) {
+        normal = unit(v - a.centers[e.feature.id]);
+        patch = CONVEX;
+        atom = a.orig[e.feature.id];
+        return;
+    }
+    switch (field.kind == K_SES ? e.feature.kind : (u32)F_BEYOND) {
         case F_SPHERE:
             normal = unit(e.closest - v);
             patch = CONVEX;
@@ -128,7 +140,7 @@ void vertex(const Engine& eng, const V3& pa, const V3& pb, double fa, double fb,
             side = 1;
         }
         V3 n;
-        double slope = gradient(pa + d * t, e, n) ? n.dot(d) : 0.0;
+        double slope = gradient(field, pa + d * t, e, n) ? n.dot(d) : 0.0;
         double newton = t - g / slope;
         if (std::fabs(slope) > 1e-3 * d.norm() && newton > t0 && newton < t1) {
             t = newton;
@@ -373,10 +385,11 @@ std::string classify_bricks(Surface& S, int threads) {
 
 // surface.rs:1041-1079 (CPU back end, no memory guard, exact refinement)
 std::string compute_surface(const std::vector<Sphere>& spheres, double probe, double spacing, size_t brick,
-                            int threads, Surface& S) {
+                            int threads, Surface& S, int kind) {
     using clk = std::chrono::steady_clock;
     auto t0 = clk::now();
-    std::string err = S.field.build(spheres, probe);
+    // SurfaceParams::effective_probe (surface.rs:411-417): 0 for vdW.
+    std::string err = S.field.build(spheres, kind == K_VDW ? 0.0 : probe, kind);
     if (!err.empty()) return err;
     err = S.engine.init(&S.field, spacing, brick);
     if (!err.empty()) return err;

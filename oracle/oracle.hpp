@@ -160,9 +160,14 @@ struct Arrangement {
     size_t n_contained = 0, n_exposed = 0;
 };
 
-// geom/arrangement.rs:437-645, with_circles = true, with_arcs = false
-// (field/point.rs:229-237). Returns an error message or "".
-std::string build_arrangement(const std::vector<Sphere>& spheres, double probe, Arrangement& a);
+// geom/arrangement.rs:437-645 with with_arcs = false (field/point.rs:
+// 229-237): with_circles for the SES field; without circles every sphere
+// that is not contained is exposed. Returns an error message or "".
+std::string build_arrangement(const std::vector<Sphere>& spheres, double probe, Arrangement& a,
+                              bool with_circles = true);
+
+// field/mod.rs:33-41
+enum SurfaceKind : int { K_SES = 0, K_SAS = 1, K_VDW = 2 };
 
 // ---------------------------------------------------------------- field
 enum FeatureKind : u32 { F_SOLVENT = 0, F_BEYOND = 1, F_SPHERE = 2, F_CIRCLE = 3, F_VERTEX = 4 };
@@ -193,15 +198,17 @@ struct NearEval {
     V3 closest;
 };
 
-// field/point.rs:189-274 (SES kind)
+// field/point.rs:189-274
 struct SesField {
     Arrangement arr;
+    int kind = K_SES;
     double probe = 0;
     CellGrid circle_cells, vertex_cells;
     double rho_max = 0;
     double extent = 0;
 
-    std::string build(const std::vector<Sphere>& spheres, double probe);
+    // SES with probe 0 is the vdW kind (point.rs:224-228).
+    std::string build(const std::vector<Sphere>& spheres, double probe, int kind = K_SES);
     bool inside_sas(const V3& x) const;
     PointEval ses_eval(const V3& x, double cutoff) const;
     void try_sphere_with(const V3& x, u32 i, const u32* blk, size_t nb, PointEval& best) const;
@@ -211,8 +218,10 @@ struct SesField {
     NearEval eval_local(const V3& x, double band, const LocalCandidates& l) const;
     // (value, index or -1)
     std::pair<double, int64_t> sphere_eval(const V3& x, double cutoff) const;
-    // SesField::sample: value, closest, feature (field/point.rs:292-339)
+    // SesField::sample: value, closest, feature (field/point.rs:292-369)
     double sample(const V3& x, V3& closest, Feature& f) const;
+    // SesField::clamped_value (field/point.rs:760-765)
+    double clamped_value(const V3& x, double band) const;
 };
 
 // Algorithmic operation counts (one per IEEE add, sub, mul, div, sqrt, min
@@ -313,6 +322,7 @@ struct Engine {
     bool prepare_sub(const int32_t key[3], size_t sb, BrickScratch& s, size_t& padded) const;
     void gather_near(const V3& b, double rho, double reach, BrickScratch& s) const;
     void gather_ses(const V3& b, double rho, BrickScratch& s) const;
+    void gather_spheres(const V3& b, double rho, double band, BrickScratch& s) const;
     void evaluate(const int32_t key[3], BrickScratch& s) const;
 };
 
@@ -322,6 +332,8 @@ std::string touched_bricks(const SesField& f, const Lattice& l, std::vector<u64>
 uint8_t uniform_sign(const SesField& f, const V3& center, double rho);
 // field/kernel.rs:837-952, one lane (scalar)
 float ses_lane(const Gathered& g, float x, float y, float z, float probe, float cutoff2);
+// field/kernel.rs:953-981, one lane (SAS and vdW)
+float sphere_lane(const Gathered& g, float x, float y, float z, float band);
 
 // ---------------------------------------------------------------- mesh
 // mesh/extract.rs:28-51
@@ -451,7 +463,7 @@ struct Surface {
     double t_geometry = 0, t_classify = 0, t_mesh = 0, t_assemble = 0;
 };
 std::string compute_surface(const std::vector<Sphere>& spheres, double probe, double spacing, size_t brick,
-                            int threads, Surface& out);
+                            int threads, Surface& out, int kind = K_SES);
 
 size_t chunk_len(size_t n);  // par.rs:66-68
 std::string classify_bricks(Surface& S, int threads);

@@ -56,7 +56,7 @@ class ArrangementRef(ctypes.Structure):
                 ("circle_ij", _vp), ("circle", _vp), ("cb_off", _vp), ("cb_items", _vp),
                 ("n_vertices", ctypes.c_size_t), ("vertex", _vp), ("vertex_atoms", _vp), ("sphere_grid", _GridRef),
                 ("circle_grid", _GridRef), ("vertex_grid", _GridRef), ("probe", ctypes.c_double),
-                ("max_expanded", ctypes.c_double), ("rho_max", ctypes.c_double)]
+                ("max_expanded", ctypes.c_double), ("rho_max", ctypes.c_double), ("kind", ctypes.c_int32)]
 
 
 class DeviceInfo(ctypes.Structure):
@@ -204,6 +204,7 @@ class PlanStats(ctypes.Structure):
 # library exports for tests and synthetic inputs.
 ABI_FUNCTIONS = [
     "smg_last_error", "smg_device_open", "smg_device_close", "smg_selftest", "smg_arrangement_build",
+    "smg_arrangement_build_kind",
     "smg_arrangement_free", "smg_arrangement_view", "smg_resident_new", "smg_resident_free", "smg_resident_bytes",
     "smg_mesh_new", "smg_mesh_free", "smg_mesh_info_get", "smg_mesh_copy", "smg_mesh_many", "smg_mesh_count",
     "smg_mesh_part_info", "smg_mesh_part_copy", "smg_field_clamped", "smg_plan_new", "smg_plan_caps_for",
@@ -360,18 +361,26 @@ class Device:
         self.close()
 
 
+KINDS = {"ses": 0, "sas": 1, "vdw": 2}
+
+
 class Arrangement:
     """The host-built SAS arrangement (reference geom/arrangement.rs:437-645 and
-    field/point.rs:213-274)."""
+    field/point.rs:213-274) of surface kind `kind`: "ses" (probe 0 gives
+    "vdw"), "sas" or "vdw" (built with probe 0)."""
 
-    def __init__(self, xyzr, probe=1.4, threads=0):
+    def __init__(self, xyzr, probe=1.4, threads=0, kind="ses"):
+        if kind not in KINDS:
+            raise ValueError(f"unknown surface kind {kind!r} (ses, sas or vdw)")
         a = np.ascontiguousarray(xyzr, dtype=np.float64).reshape(-1, 4)
         self._h = _vp()
-        _check(lib().smg_arrangement_build(_ptr(a), ctypes.c_size_t(len(a)), ctypes.c_double(probe),
-                                           ctypes.c_int(threads), ctypes.byref(self._h)))
+        _check(lib().smg_arrangement_build_kind(_ptr(a), ctypes.c_size_t(len(a)), ctypes.c_double(probe),
+                                                ctypes.c_int(KINDS[kind]), ctypes.c_int(threads),
+                                                ctypes.byref(self._h)))
         self.ref = ArrangementRef()
         _check(lib().smg_arrangement_view(self._h, ctypes.byref(self.ref)))
-        self.probe = probe
+        self.kind = {v: k for k, v in KINDS.items()}[self.ref.kind]
+        self.probe = self.ref.probe
 
     def arrays(self):
         """Copies of the arrangement arrays (internal order)."""

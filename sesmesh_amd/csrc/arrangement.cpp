@@ -252,8 +252,12 @@ static void grid_near(const Grid& g, const double c[3], double r, F&& f) {
         }
 }
 
-std::string build_arrangement(const double* xyzr, size_t n_in, double probe, int threads, Arrangement& A) {
-    if (!(std::isfinite(probe) && probe > 0.0)) return "probe radius must be finite and > 0 (SES)";
+std::string build_arrangement(const double* xyzr, size_t n_in, double probe, int kind, int threads, Arrangement& A) {
+    if (!(std::isfinite(probe) && probe >= 0.0)) return "probe radius must be finite and >= 0";
+    if (kind < 0 || kind > 2) return "unknown surface kind " + std::to_string(kind);
+    if (kind == 0 && probe == 0.0) kind = 2;  // field/point.rs:224-228
+    if (kind == 2) probe = 0.0;               // SurfaceParams::effective_probe (surface.rs:411-417)
+    const bool with_circles = kind == 0;
     for (size_t i = 0; i < n_in; i++) {
         if (!(std::isfinite(xyzr[4 * i]) && std::isfinite(xyzr[4 * i + 1]) && std::isfinite(xyzr[4 * i + 2])))
             return "sphere " + std::to_string(i) + " has a non-finite centre";
@@ -262,6 +266,7 @@ std::string build_arrangement(const double* xyzr, size_t n_in, double probe, int
     }
     if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
     A = Arrangement();
+    A.kind = kind;
     A.probe = probe;
     std::vector<uint32_t> active;
     for (size_t i = 0; i < n_in; i++)
@@ -359,7 +364,7 @@ std::string build_arrangement(const double* xyzr, size_t n_in, double probe, int
         }
     }
 
-    // Circles, blockers and triple points per sphere (geom/arrangement.rs:672-751).
+    // Circles, blockers and triple points per sphere, SES kind only (geom/arrangement.rs:552-611, 672-751).
     struct Part {
         std::vector<double> circle;  // 10 per circle
         std::vector<uint32_t> ij, cb_len, cb_items;
@@ -373,7 +378,7 @@ std::string build_arrangement(const double* xyzr, size_t n_in, double probe, int
         std::vector<Interval> iv;
         std::vector<Run> runs;
         std::vector<uint32_t> blk;
-        for (size_t i = a; i < b; i++) {
+        for (size_t i = a; i < b && with_circles; i++) {
             if (contained[i]) continue;
             const uint32_t* rowi = nb_all.data() + nb_off[i];
             size_t ni = nb_off[i + 1] - nb_off[i];
@@ -450,7 +455,7 @@ std::string build_arrangement(const double* xyzr, size_t n_in, double probe, int
         }
     });
     std::vector<uint8_t> exposed(n);
-    for (size_t i = 0; i < n; i++) exposed[i] = !contained[i] && nb_off[i + 1] == nb_off[i];
+    for (size_t i = 0; i < n; i++) exposed[i] = !contained[i] && (!with_circles || nb_off[i + 1] == nb_off[i]);
     A.cb_off.assign(1, 0);
     for (auto& P : parts) {
         for (uint32_t s : P.exposed) exposed[s] = 1;

@@ -215,8 +215,10 @@ __device__ __forceinline__ double ses_distance_wave(const DArr& A, double x, dou
     }
 }
 
-// lattice::uniform_sign (SES), one wavefront (the workgroup). Returns
-// ST_NONE/OUT/IN in every lane.
+// lattice::uniform_sign, one wavefront (the workgroup). Returns
+// ST_NONE/OUT/IN in every lane. The SAS and vdW kinds decide by the sign
+// of the least SAS distance alone (expanded radii are the vdW radii for
+// the vdW kind).
 __device__ __forceinline__ uint8_t uniform_sign_wave(const DArr& A, double x, double y, double z, double rho) {
     const uint32_t lane = lane_id();
     double margin = rho + CLASSIFY_EPS;
@@ -230,6 +232,7 @@ __device__ __forceinline__ uint8_t uniform_sign_wave(const DArr& A, double x, do
     });
     s_sas = wave_min(s_sas);
     s_vdw = wave_min(s_vdw);
+    if (A.kind != K_SES) return s_sas > margin ? ST_OUT : (s_sas < -margin ? ST_IN : ST_NONE);
     if (s_sas >= margin) return ST_OUT;
     if (s_vdw <= -margin) return ST_IN;
     double cutoff = A.probe + margin + CLASSIFY_EPS;

@@ -23,6 +23,10 @@ constexpr int64_t KEY_BIAS = int64_t(1) << 20;  // field/lattice.rs:15
 enum : uint8_t { ST_NONE = 0, ST_OUT = 1, ST_IN = 2 };  // sub-block / brick states
 enum : uint32_t { S_EXPOSED = 0, S_BURIED = 1, S_CONTAINED = 2 };
 enum : uint32_t { F_SOLVENT = 0, F_BEYOND = 1, F_SPHERE = 2, F_CIRCLE = 3, F_VERTEX = 4 };
+// Surface kinds (field/mod.rs:33-41). SAS and vdW are unions of spheres:
+// their arrangements have no circles or SAS vertices, and the vdW one is
+// built with probe 0.
+enum : uint32_t { K_SES = 0, K_SAS = 1, K_VDW = 2 };
 
 // Pointers into device global memory (address space 1). The arrangement's
 // pointers are read from the molecule table in memory; typed like this,
@@ -78,7 +82,7 @@ struct DArr {
     gptr<const SphereThr> thr;
     DGrid gs, gc, gv;
     DGrid fs, fc, fv;  // the same items on cells 0.4 as wide, for gather3
-    uint32_t n_spheres, n_circles, n_vertices, pad;
+    uint32_t n_spheres, n_circles, n_vertices, kind;
     double probe, max_expanded, rho_max;
 };
 
@@ -290,6 +294,12 @@ __device__ inline uint64_t total_key(double d) {
 }
 // Inverse of total_key for keys of non-negative values.
 __device__ inline double key_value(uint64_t k) { return __longlong_as_double((int64_t)(k ^ 0x8000000000000000ull)); }
+// Inverse of total_key for keys of any sign.
+__device__ inline double key_value_any(uint64_t k) {
+    int64_t i = (int64_t)(k ^ 0x8000000000000000ull);
+    i ^= (int64_t)(((uint64_t)(i >> 63)) >> 1);
+    return __longlong_as_double(i);
+}
 
 // Smallest s >= 0 with sqrt(s) >= m. A correctly rounded sqrt is
 // monotonic, so for every s >= 0: sqrt(s) >= m  <=>  s >= threshold. The

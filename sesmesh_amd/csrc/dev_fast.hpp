@@ -463,10 +463,19 @@ __device__ __forceinline__ void refine_sub_fast(const Mol* __restrict__ mols, co
     // An edge whose f32 search ends in bisection (not converged) is redone
     // by the exact f64 search from its start, over the f64 lists the
     // gather left in place, as the reference redoes unconverged device
-    // edges on the CPU (mesh/extract.rs:336-358).
+    // edges on the CPU (mesh/extract.rs:336-358). The SAS and vdW kinds
+    // (a handful of spheres per point, no blockers) take the exact search
+    // for every edge.
+    const bool sph = A.kind != K_SES;
     EdgeState X;
-    bool exact = false;
-    if (active) edgef_start(P, val, org, s.ebuf[q], E);
+    bool exact = sph;
+    auto start = [&](uint32_t id) {
+        if (sph)
+            edge_start(P, val, org, id, X);
+        else
+            edgef_start(P, val, org, id, E);
+    };
+    if (active) start(s.ebuf[q]);
     unsigned long long my_evals = 0;
     while (__ballot(active) != 0ull) {
         if (active) {
@@ -479,12 +488,12 @@ __device__ __forceinline__ void refine_sub_fast(const Mol* __restrict__ mols, co
                     y = Q.pa[1] + Q.d[1] * X.t;
                     z = Q.pa[2] + Q.d[2] * X.t;
                 }
-                EvalD e = eval_group(A, s, n, cap.cand, nw, G, c[0], c[1], c[2], x, y, z, A.probe + P.band);
-                done = edge_step(X, e, h);
+                EvalD e = eval_group(A, s, n, cap.cand, nw, G, c[0], c[1], c[2], x, y, z, A.probe + P.band, P.band);
+                done = edge_step(X, e, h, sph);
                 if (done) {
                     size_t slot = (size_t)vbase[brick] + edge_rank(mask, wpre, words, brick, X.id);
                     edge_finish(A, P, X, G, slot, out);
-                    if (G.r == 0 && out.evals && slot < out.cap) out.evals[slot] |= 1u << 30;
+                    if (G.r == 0 && out.evals && slot < out.cap && !sph) out.evals[slot] |= 1u << 30;
                     my_evals += X.st >> 16;
                 }
             } else {
@@ -503,10 +512,10 @@ __device__ __forceinline__ void refine_sub_fast(const Mol* __restrict__ mols, co
                 }
             }
             if (done) {
-                exact = false;
+                exact = sph;
                 q += ngroups;
                 active = q < ne;
-                if (active) edgef_start(P, val, org, s.ebuf[q], E);
+                if (active) start(s.ebuf[q]);
             }
         }
     }

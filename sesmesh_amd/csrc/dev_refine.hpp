@@ -1,10 +1,12 @@
 // Crossing edges, exact f64 vertex refinement and vertex attributes.
 //
 // Restates the edge scan of mesh_brick (reference mesh/extract.rs:146-223),
-// SesField::gather_local / eval_local (field/point.rs:419-489, 496-565,
-// 686-722) and vertex / gradient / attributes (mesh/extract.rs:425-597) in
+// SesField::gather_local / eval_local (field/point.rs:419-489, 496-581,
+// 686-738) and vertex / gradient / attributes (mesh/extract.rs:425-597) in
 // f64 with the CPU's operations and iteration rules, so every vertex,
-// normal, atom and patch equals the CPU back end's bit for bit.
+// normal, atom and patch equals the CPU back end's bit for bit. The SAS and
+// vdW kinds list every sphere within reach as class S candidates, nearest
+// first by their signed distance at the ball centre, without blockers.
 //
 // One workgroup (256 threads) per sub-block with crossing edges. It gathers
 // the sub-block's lists into LDS once: the near spheres, the inside group,
@@ -155,14 +157,37 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
     if (tid < WAVE) bstart[tid] = 0u;
     __syncthreads();
     const uint32_t hmask = cap.hash - 1;
-    const double reach = A.probe + band + rho;
+    // SAS / vdW (point.rs:566-578): reach band + rho, keys d - R (signed,
+    // at least -max_expanded), bucketed from -max_expanded.
+    const bool sph = A.kind != K_SES;
+    hide = hide && !sph;
+    const double reach = (sph ? 0.0 : A.probe) + band + rho;
     const double radii[3] = {reach + A.max_expanded, reach + A.rho_max, reach};
-    const double per_bucket = KEY_BUCKETS / reach;
+    const double key0 = sph ? A.max_expanded : 0.0;
+    const double per_bucket = KEY_BUCKETS / (reach + key0);
     auto bucket = [&](uint32_t cls, double key) {
-        return cls * KEY_BUCKETS + min(KEY_BUCKETS - 1u, (uint32_t)(key * per_bucket));
+        return cls * KEY_BUCKETS + min(KEY_BUCKETS - 1u, (uint32_t)(fmax(key + key0, 0.0) * per_bucket));
     };
     gather3(A, bx, by, bz, radii, tmp4, [&](uint32_t grid, uint32_t k) {
-        if (grid == 0) {
+        if (grid == 0 && sph) {
+            if (A.status[k] == S_CONTAINED) return;
+            const SphereRec S = load_rec(A.sph, k);
+            const double v = ddist(bx, by, bz, S.x, S.y, S.z) - S.R;
+            if (!(v < reach)) return;
+            uint32_t at = atomicAdd(&n[R_NEAR], 1u);
+            uint32_t q = atomicAdd(&n[R_CAND], 1u);
+            if (at >= cap.near || q >= cap.cand) {
+                n[R_OVER] = 1u;
+                return;
+            }
+            s.nq[at] = NearD{S.x, S.y, S.z, 0.0};
+            if (cap.radii) {
+                s.nR[at] = S.R;
+                s.nr[at] = A.r[k];
+            }
+            s.ckey[q] = total_key(v);
+            s.cid[q] = k;
+        } else if (grid == 0) {
             if (A.status[k] == S_CONTAINED) return;
             const SphereRec S = load_rec(A.sph, k);
             double R = S.R, cx = S.x, cy = S.y, cz = S.z;
@@ -279,7 +304,7 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
     }
     __syncthreads();
     auto bucket_of = [&](unsigned long long key, uint32_t id) {
-        return bucket(id >> KIND_SHIFT, key_value(key));
+        return bucket(id >> KIND_SHIFT, sph ? key_value_any(key) : key_value(key));
     };
     for (uint32_t q = tid; q < gathered; q += nt)
         if (s.ckey[q] != HIDDEN_KEY) atomicAdd(&bstart[bucket_of(s.ckey[q], s.cid[q])], 1u);
@@ -339,7 +364,7 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
             d[2] = S.z;
             d[3] = S.R;
             s.coff[p] = A.nb_off[k];
-            s.cpre[p] = A.nb_off[k + 1] - A.nb_off[k];
+            s.cpre[p] = sph ? 0u : A.nb_off[k + 1] - A.nb_off[k];
         } else if (p < ns + nc) {
             const CircleRec C = load_rec(A.cir, k);
             d[0] = C.mx;
@@ -521,12 +546,56 @@ __device__ inline void scan_class(const DArr& A, const RLists& s, uint32_t cc, u
     }
 }
 
+// eval_local of the SAS and vdW kinds (field/point.rs:723-736): the first
+// sphere, in list order, of the least |x - c| - R below band. Each lane
+// scans its share in list order and stops where the key (the value at the
+// ball centre) minus the point's offset exceeds its best: the values are
+// 1-Lipschitz, so no later sphere is below that best. The record keeps the
+// sphere's centre in q, from which the gradient and the normal are taken.
+__device__ inline EvalD eval_spheres(const RLists& s, const uint32_t* n, const Group& G, double cx, double cy,
+                                     double cz, double x, double y, double z, double band) {
+    const uint32_t total = n[R_CAND];
+    double ox = x - cx, oy = y - cy, oz = z - cz;
+    const double delta = sqrt(ox * ox + oy * oy + oz * oz) + 1e-7;
+    double best = band;
+    uint32_t bo = 0xffffffffu;
+    for (uint32_t p = G.r; p < total; p += G.g) {
+        if (key_value_any(s.ckey[p]) > best + delta) break;
+        const double* d = s.cdat + (size_t)CDAT * p;
+        const double v = ddist(x, y, z, d[0], d[1], d[2]) - d[3];
+        if (v < best) {
+            best = v;
+            bo = p;
+        }
+    }
+    group_least(G, best, bo);
+    EvalD e;
+    e.value = best;
+    if (bo == 0xffffffffu) {
+        e.kind = F_BEYOND;
+        e.id = 0;
+        e.qx = x;
+        e.qy = y;
+        e.qz = z;
+    } else {
+        const double* d = s.cdat + (size_t)CDAT * bo;
+        e.kind = F_SPHERE;
+        e.id = s.cid[bo] & ID_MASK;
+        e.qx = d[0];
+        e.qy = d[1];
+        e.qz = d[2];
+    }
+    return e;
+}
+
 // eval_local (field/point.rs:686-722) at (x, y, z), split over a lane group.
 // (cx, cy, cz) is the centre of the gathered ball. The group shares its
-// best after each class, which only tightens later tests.
+// best after each class, which only tightens later tests. The SAS and vdW
+// kinds take eval_spheres with `band`.
 __device__ inline EvalD eval_group(const DArr& A, const RLists& s, const uint32_t* n, uint32_t cc, uint32_t nw,
                                    const Group& G, double cx, double cy, double cz, double x, double y, double z,
-                                   double cutoff) {
+                                   double cutoff, double band) {
+    if (A.kind != K_SES) return eval_spheres(s, n, G, cx, cy, cz, x, y, z, band);
     EvalD e;
     e.qx = x;
     e.qy = y;
@@ -630,8 +699,10 @@ __device__ inline void edge_start(const DRun& P, const float* val, const int64_t
     E.st = 1u << 7;  // side 0
 }
 
-// Applies evaluation e at E.t; true when the search is over.
-__device__ inline bool edge_step(EdgeState& E, const EvalD& e, double h) {
+// Applies evaluation e at E.t; true when the search is over. The gradient
+// (mesh/extract.rs:539-549) points from x to the closest point for the SES
+// kind, and from the sphere's centre (held in e.q) to x for the others.
+__device__ inline bool edge_step(EdgeState& E, const EvalD& e, double h, bool sph) {
     E.st += 1u << 16;  // evals
     E.last = e;
     double g = e.value;
@@ -658,6 +729,11 @@ __device__ inline bool edge_step(EdgeState& E, const EvalD& e, double h) {
         if (e.kind != F_SOLVENT && e.kind != F_BEYOND) {
             double x = Q.pa[0] + Q.d[0] * E.t, y = Q.pa[1] + Q.d[1] * E.t, z = Q.pa[2] + Q.d[2] * E.t;
             double wx = e.qx - x, wy = e.qy - y, wz = e.qz - z;
+            if (sph) {
+                wx = x - e.qx;
+                wy = y - e.qy;
+                wz = z - e.qz;
+            }
             double len = sqrt(wx * wx + wy * wy + wz * wz);
             if (len > 1e-12) {
                 double nx = wx / len, ny = wy / len, nz = wz / len;
@@ -737,7 +813,11 @@ __device__ inline void edge_finish(const DArr& A, const DRun& P, const EdgeState
         }
         return A.orig[bk];
     };
-    if (last.kind == F_SPHERE) {
+    if (last.kind == F_SPHERE && A.kind != K_SES) {  // q: the sphere's centre
+        unit(v[0] - last.qx, v[1] - last.qy, v[2] - last.qz);
+        patch = 0;
+        atom = A.orig[last.id];
+    } else if (last.kind == F_SPHERE) {
         unit(last.qx - v[0], last.qy - v[1], last.qz - v[2]);
         patch = 0;
         atom = A.orig[last.id];
@@ -861,6 +941,7 @@ __device__ __forceinline__ void refine_sub(const Mol* __restrict__ mols, const u
     G.mask = g == WAVE ? ~0ull : ((1ull << g) - 1ull);
     const uint32_t ngroups = blockDim.x / g, nw = cap.near / 64;
     const double cutoff = A.probe + P.band;
+    const bool sph = A.kind != K_SES;
     uint32_t q = threadIdx.x / g;
     bool active = q < ne;
     EdgeState E;
@@ -875,8 +956,8 @@ __device__ __forceinline__ void refine_sub(const Mol* __restrict__ mols, const u
                 y = Q.pa[1] + Q.d[1] * E.t;
                 z = Q.pa[2] + Q.d[2] * E.t;
             }
-            EvalD e = eval_group(A, s, n, cap.cand, nw, G, c[0], c[1], c[2], x, y, z, cutoff);
-            if (edge_step(E, e, h)) {
+            EvalD e = eval_group(A, s, n, cap.cand, nw, G, c[0], c[1], c[2], x, y, z, cutoff, P.band);
+            if (edge_step(E, e, h, sph)) {
                 size_t slot = (size_t)vbase[brick] + edge_rank(mask, wpre, words, brick, E.id);
                 edge_finish(A, P, E, G, slot, out);
                 my_evals += E.st >> 16;

@@ -1,7 +1,8 @@
 // Candidate gathering and field sampling of one sub-block per workgroup.
 //
-// Restates Engine::assign / prepare_sub / gather_near / gather_ses and the
-// SES lane kernel (reference field/kernel.rs:489-751, 837-952). Membership
+// Restates Engine::assign / prepare_sub / gather_near / gather_ses /
+// gather_spheres and the SES and sphere lane kernels (reference
+// field/kernel.rs:489-751, 837-981). Membership
 // decisions are made in f64 exactly as the CPU makes them; the candidate
 // data stored for sampling are the CPU's frame-local f32 copies. The CPU
 // sorts the class S and C candidates nearest-first; the sampled value does
@@ -240,7 +241,21 @@ __device__ inline float ses_value(const Lists& s, const uint32_t* n, uint32_t cc
     return probe - sqrtf(best2);
 }
 
+// One lane of the sphere kernel of the SAS and vdW kinds (kernel.rs:953-981):
+// min(band, min_k(|x - c_k| - R_k)) over the near spheres (nd: x, y, z, R).
+__device__ inline float sphere_value(const Lists& s, const uint32_t* n, float x, float y, float z, float band) {
+    float best = band;
+    for (uint32_t k = 0; k < n[C_NEAR]; k++) {
+        float4 c = s.nd[k];
+        float dx = x - c.x, dy = y - c.y, dz = z - c.z;
+        best = fminf(best, sqrtf(dx * dx + dy * dy + dz * dz) - c.w);
+    }
+    return best;
+}
+
 // Gathers the lists of sub-block `sb` of brick `key` (kernel.rs:553-751).
+// The SAS and vdW kinds (gather_spheres, kernel.rs:645-652) list the near
+// spheres within band + rho, with their radii, and nothing else.
 // Returns false (in every thread) if a capacity was exceeded.
 __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, const Caps& cap, Lists& s,
                                   uint32_t* n, uint32_t* tmp4, const int32_t* key, uint32_t sb) {
@@ -252,7 +267,8 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
     __syncthreads();
     double h = P.h, sub = (double)P.sub, half = sub * 0.5;
     double rho = 0.5 * sqrt(3.0) * sub * h;
-    double reach = P.probe + P.band + rho;
+    const bool sph = A.kind != K_SES;
+    double reach = (sph ? 0.0 : P.probe) + P.band + rho;
     double b[3];
     for (int a = 0; a < 3; a++) {
         double base = (double)((int64_t)key[a] * (int64_t)P.b) * h;
@@ -276,6 +292,10 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
                 return;
             }
             const float4 p = load_f4(Lc.p4, k);
+            if (sph) {
+                s.nd[at] = p;
+                return;
+            }
             s.nd[at] = make_float4(p.x, p.y, p.z, rec_r2tol(p.w));
             hash_put(s.hk, s.hv, hmask, k, at);
             if (d < R + rho) {
@@ -490,7 +510,8 @@ __device__ __forceinline__ void sample_sub(const Mol* __restrict__ mols, const u
             float x = (float)(int32_t)(org[0] + lx) * P.h32;
             float y = (float)(int32_t)(org[1] + ly) * P.h32;
             float z = (float)(int32_t)(org[2] + lz) * P.h32;
-            out[lx + side * (ly + side * lz)] = ses_value(s, n, cc, nw, x, y, z, P.fprobe, P.fcutoff2);
+            out[lx + side * (ly + side * lz)] = A.kind != K_SES ? sphere_value(s, n, x, y, z, P.fband)
+                                                                : ses_value(s, n, cc, nw, x, y, z, P.fprobe, P.fcutoff2);
         }
         if (threadIdx.x == 0 && n_samples) atomicAdd(&n_samples[m], (unsigned long long)cnt);
         __syncthreads();

@@ -82,12 +82,15 @@ __device__ __forceinline__ void uniform_brick(const Mol* __restrict__ mols, cons
         s_vdw = fmin(s_vdw, __shfl_xor(s_vdw, (int)o, WAVE));
     }
     uint8_t state;
-    if (s_sas >= margin) {
+    if (A.kind != K_SES) {
+        state = s_sas > margin ? ST_OUT : (s_sas < -margin ? ST_IN : ST_NONE);
+    } else if (s_sas >= margin) {
         state = ST_OUT;
     } else if (s_vdw <= -margin) {
         state = ST_IN;
     } else {
-        EvalD e = eval_group(A, s, n, cap.cand, cap.near / 64, G, C[0], C[1], C[2], c[0], c[1], c[2], cutoff);
+        EvalD e = eval_group(A, s, n, cap.cand, cap.near / 64, G, C[0], C[1], C[2], c[0], c[1], c[2], cutoff,
+                             margin + CLASSIFY_EPS);
         double v = e.value;
         state = v > margin ? ST_OUT : (v < -margin ? ST_IN : ST_NONE);
     }

@@ -16,6 +16,7 @@ struct Grid {
 
 // The f64 SAS arrangement with the field's cell grids, SoA, internal order.
 struct Arrangement {
+    int kind = 0;  // 0 SES, 1 SAS, 2 vdW
     double probe = 0, max_expanded = 0, rho_max = 0;
     std::vector<double> center, radius, expanded;  // center: 3 per sphere
     std::vector<uint8_t> status;                    // 0 exposed, 1 buried, 2 contained
@@ -30,6 +31,9 @@ struct Arrangement {
 };
 
 void build_grid(const double* pts, size_t n, double cell, Grid& g);
-std::string build_arrangement(const double* xyzr, size_t n, double probe, int threads, Arrangement& a);
+// `kind` 0 (SES) builds the circles and SAS vertices; 1 (SAS) and 2 (vdW,
+// built with probe 0) only containment and neighbours, every sphere that is
+// not contained being exposed.
+std::string build_arrangement(const double* xyzr, size_t n, double probe, int kind, int threads, Arrangement& a);
 
 }  // namespace smg

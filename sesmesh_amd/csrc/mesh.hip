@@ -629,17 +629,23 @@ extern "C" int smg_selftest(smg_device* d, size_t n, uint64_t seed, uint64_t* m3
 }
 
 // ------------------------------------------------------------- arrangement
-extern "C" int smg_arrangement_build(const double* xyzr, size_t n, double probe, int threads, smg_arrangement** out) {
+extern "C" int smg_arrangement_build_kind(const double* xyzr, size_t n, double probe, int kind, int threads,
+                                          smg_arrangement** out) {
+    if (!out) return fail(SMG_E_INVALID, "null output");
     *out = nullptr;
     if (!xyzr && n) return fail(SMG_E_INVALID, "no spheres");
     smg_arrangement* a = new smg_arrangement();
-    std::string e = build_arrangement(xyzr, n, probe, threads, a->a);
+    std::string e = build_arrangement(xyzr, n, probe, kind, threads, a->a);
     if (!e.empty()) {
         delete a;
         return fail(SMG_E_INVALID, e);
     }
     *out = a;
     return SMG_OK;
+}
+
+extern "C" int smg_arrangement_build(const double* xyzr, size_t n, double probe, int threads, smg_arrangement** out) {
+    return smg_arrangement_build_kind(xyzr, n, probe, SMG_KIND_SES, threads, out);
 }
 extern "C" void smg_arrangement_free(smg_arrangement* a) { delete a; }
 
@@ -678,6 +684,7 @@ extern "C" int smg_arrangement_view(const smg_arrangement* p, smg_arrangement_re
     r->probe = a.probe;
     r->max_expanded = a.max_expanded;
     r->rho_max = a.rho_max;
+    r->kind = a.kind;
     return SMG_OK;
 }
 
@@ -716,7 +723,13 @@ extern "C" int smg_resident_new(smg_device* dev, const smg_arrangement_ref* a, s
     if (!dev || !a) return fail(SMG_E_INVALID, "null device or arrangement");
     if (a->n_spheres >= (1u << 30) || a->n_circles >= (1u << 30) || a->n_vertices >= (1u << 30))
         return fail(SMG_E_LENGTH, "arrangement too large for 30-bit candidate indices");
-    if (!(a->probe > 0.0)) return fail(SMG_E_UNSUPPORTED, "the device path computes the SES (probe > 0)");
+    if (a->kind < SMG_KIND_SES || a->kind > SMG_KIND_VDW)
+        return fail(SMG_E_INVALID, "unknown surface kind " + std::to_string(a->kind));
+    if (!(std::isfinite(a->probe) && a->probe >= 0.0)) return fail(SMG_E_INVALID, "probe radius must be finite and >= 0");
+    if (a->kind == SMG_KIND_SES && !(a->probe > 0.0))
+        return fail(SMG_E_INVALID, "an SES arrangement has probe > 0 (probe 0 is the vdW kind)");
+    if (a->kind != SMG_KIND_SES && (a->n_circles || a->n_vertices))
+        return fail(SMG_E_INVALID, "SAS and vdW arrangements have no circles or SAS vertices");
     HIP_TRY(hipSetDevice(dev->ordinal));
     size_t n = a->n_spheres, nc = a->n_circles, nv = a->n_vertices;
     std::unique_ptr<smg_resident> r(new smg_resident());
@@ -816,6 +829,7 @@ extern "C" int smg_resident_new(smg_device* dev, const smg_arrangement_ref* a, s
     A.n_spheres = (uint32_t)n;
     A.n_circles = (uint32_t)nc;
     A.n_vertices = (uint32_t)nv;
+    A.kind = (uint32_t)a->kind;
     A.probe = a->probe;
     A.max_expanded = a->max_expanded;
     A.rho_max = a->rho_max;
@@ -836,10 +850,21 @@ extern "C" size_t smg_resident_bytes(const smg_resident* r) { return r ? r->buf.
 namespace {
 // SesField::clamped_value (field/point.rs:759-765) at points[w], one
 // wavefront per point: probe - ses_eval(x, probe + band).distance, where
-// the distance is 0 outside the SAS (inside_sas, point.rs:372-381).
+// the distance is 0 outside the SAS (inside_sas, point.rs:372-381); for the
+// SAS and vdW kinds sphere_eval(x, band) (point.rs:742-756), the least
+// |x - c| - R below band.
 __global__ void k_field_clamped(DArr A, const double* points, uint32_t n, double band, double* out) {
     for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
         double x = points[3 * (size_t)w], y = points[3 * (size_t)w + 1], z = points[3 * (size_t)w + 2];
+        if (A.kind != K_SES) {
+            double best = band;
+            for_near(A.gs, x, y, z, band + A.max_expanded, lane_id(), WAVE, [&](uint32_t k) {
+                if (A.status[k] != S_CONTAINED) best = fmin(best, ddist(x, y, z, A.cx[k], A.cy[k], A.cz[k]) - A.R[k]);
+            });
+            best = wave_min(best);
+            if (lane_id() == 0) out[w] = best;
+            continue;
+        }
         bool inside = false;
         for_near(A.gs, x, y, z, A.max_expanded, lane_id(), WAVE, [&](uint32_t k) {
             if (!inside && A.status[k] != S_CONTAINED && ddist(x, y, z, A.cx[k], A.cy[k], A.cz[k]) < A.R[k])
