@@ -767,25 +767,28 @@ def tune(residents, spacings, brick=10, refine="exact", repeats=2, bricks=None):
 class SurfaceParams:
     """The parameters of the mesh stage (reference SurfaceParams,
     crates/sesmesh/src/surface.rs:195-300): SES with probe 1.4 A, spacing
-    0.5 A, brick 10, exact refinement."""
+    0.5 A, brick 10, exact refinement. `kind` is "ses", "sas" or "vdw"
+    (SurfaceParams::ses / sas / vdw, surface.rs:245-285)."""
     probe_radius: float = 1.4
     spacing: float = 0.5
     brick_size: int = 10
     refine: str = "exact"
     sizing: int = 0
+    kind: str = "ses"
 
 
 def compute_surface(xyzr, params=None, device=None):
-    """The SES mesh of spheres `xyzr` (n x 4: centre, radius) on the GPU:
-    returns a dict of the mesh arrays (positions, normals, atoms, patches,
-    triangles) and stats, equal to the reference CPU back end's mesh."""
+    """The mesh of spheres `xyzr` (n x 4: centre, radius) on the GPU for the
+    surface kind of `params`: returns a dict of the mesh arrays (positions,
+    normals, atoms, patches, triangles) and stats, equal to the reference
+    CPU back end's mesh."""
     params = params or SurfaceParams()
     dev = device or Device(0)
-    arr = Arrangement(xyzr, params.probe_radius)
+    arr = Arrangement(xyzr, params.probe_radius, kind=params.kind)
     res = Resident(dev, arr)
     m = res.mesh(params.spacing, params.brick_size, params.refine, params.sizing)
     out = m.arrays()
-    out["stats"] = {"touched_bricks": m.info.touched_bricks, "surface_bricks": m.info.surface_bricks,
+    out["stats"] = {"kind": arr.kind, "touched_bricks": m.info.touched_bricks, "surface_bricks": m.info.surface_bricks,
                     "vertices": m.n_vertices, "triangles": m.n_triangles,
                     "refine_evaluations": m.info.refine_evaluations, "timings_ms": m.times.as_dict()}
     return out
