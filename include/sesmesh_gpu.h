@@ -166,6 +166,32 @@ void smg_resident_free(smg_resident* r);
 /* DeviceResident::resident_bytes (batch.rs:165-207). */
 size_t smg_resident_bytes(const smg_resident* r);
 
+/* SesField::new(spheres, probe, kind) and Resident::new in one call, the
+ * arrangement built on the device (geom/arrangement.rs:437-645, with the
+ * field's cell grids, point.rs:213-274): the same arrays, bit for bit and
+ * in the same order, as smg_arrangement_build_kind followed by
+ * smg_resident_new, without the host build. `xyzr` (n rows of centre x, y,
+ * z and radius) is read during the call. The one comparison the device
+ * cannot repeat to the last bit, an exposed-arc angle gap from atan2 within
+ * 1e-13 rad of the test's threshold, sends that sphere set to the host
+ * builder instead (smg_resident_info reports which ran). */
+int smg_resident_build(smg_device* dev, const double* xyzr, size_t n, double probe, int kind, smg_resident** out);
+
+typedef struct {
+    size_t n_spheres, n_circles, n_vertices; /* after containment; circles and SAS vertices of the SES kind */
+    int32_t kind;                            /* SMG_KIND_* */
+    int32_t built_on_device;                 /* 1: smg_resident_build's device path; 0: a host arrangement */
+    double probe;                            /* the arrangement's probe (0 for vdW) */
+    double build_ms;                         /* wall time of smg_resident_build */
+    size_t bytes;                            /* HBM held */
+} smg_resident_info;
+int smg_resident_info_get(const smg_resident* r, smg_resident_info* info);
+
+/* The resident's arrangement copied back to the host, as a new
+ * smg_arrangement (free with smg_arrangement_free; view with
+ * smg_arrangement_view). */
+int smg_resident_arrangement(const smg_resident* r, smg_arrangement** out);
+
 /* ------------------------------------------------------------ mesh stage
  * Parameters of a mesh stage: SurfaceParams' spacing, brick size and
  * refinement mode (crates/sesmesh/src/surface.rs:195-300). */

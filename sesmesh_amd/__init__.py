@@ -204,7 +204,7 @@ class PlanStats(ctypes.Structure):
 # library exports for tests and synthetic inputs.
 ABI_FUNCTIONS = [
     "smg_last_error", "smg_device_open", "smg_device_close", "smg_selftest", "smg_arrangement_build",
-    "smg_arrangement_build_kind",
+    "smg_arrangement_build_kind", "smg_resident_build", "smg_resident_info_get", "smg_resident_arrangement",
     "smg_arrangement_free", "smg_arrangement_view", "smg_resident_new", "smg_resident_free", "smg_resident_bytes",
     "smg_mesh_new", "smg_mesh_free", "smg_mesh_info_get", "smg_mesh_copy", "smg_mesh_many", "smg_mesh_count",
     "smg_mesh_part_info", "smg_mesh_part_copy", "smg_field_clamped", "smg_plan_new", "smg_plan_caps_for",
@@ -382,6 +382,16 @@ class Arrangement:
         self.kind = {v: k for k, v in KINDS.items()}[self.ref.kind]
         self.probe = self.ref.probe
 
+    @classmethod
+    def _from_handle(cls, h):
+        self = cls.__new__(cls)
+        self._h = h
+        self.ref = ArrangementRef()
+        _check(lib().smg_arrangement_view(self._h, ctypes.byref(self.ref)))
+        self.kind = {v: k for k, v in KINDS.items()}[self.ref.kind]
+        self.probe = self.ref.probe
+        return self
+
     def arrays(self):
         """Copies of the arrangement arrays (internal order)."""
         r = self.ref
@@ -405,7 +415,18 @@ class Arrangement:
             "vertices": arr(r.vertex, 3 * nv, np.float64).reshape(-1, 3),
             "vertex_atoms": arr(r.vertex_atoms, 3 * nv, np.uint32).reshape(-1, 3),
             "max_expanded": r.max_expanded, "rho_max": r.rho_max,
+            "grids": {k: self._grid(getattr(r, k)) for k in ("sphere_grid", "circle_grid", "vertex_grid")},
         }
+
+    @staticmethod
+    def _grid(g):
+        dims = tuple(int(x) for x in g.dims)
+        cells = dims[0] * dims[1] * dims[2]
+        origin = np.ctypeslib.as_array(ctypes.cast(g.origin, ctypes.POINTER(ctypes.c_double)), shape=(3,)).copy()
+        starts = np.ctypeslib.as_array(ctypes.cast(g.starts, ctypes.POINTER(ctypes.c_uint32)), shape=(cells + 1,)).copy()
+        items = (np.ctypeslib.as_array(ctypes.cast(g.items, ctypes.POINTER(ctypes.c_uint32)), shape=(g.n_items,)).copy()
+                 if g.n_items else np.zeros(0, np.uint32))
+        return {"origin": origin, "cell": g.cell, "dims": dims, "starts": starts, "items": items}
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -414,14 +435,51 @@ class Arrangement:
             _free("smg_arrangement_free", h)
 
 
+class ResidentInfo(ctypes.Structure):
+    """smg_resident_info of include/sesmesh_gpu.h."""
+    _fields_ = [("n_spheres", ctypes.c_size_t), ("n_circles", ctypes.c_size_t), ("n_vertices", ctypes.c_size_t),
+                ("kind", ctypes.c_int32), ("built_on_device", ctypes.c_int32), ("probe", ctypes.c_double),
+                ("build_ms", ctypes.c_double), ("bytes", ctypes.c_size_t)]
+
+
 class Resident:
     """The arrangement resident in HBM (reference hip::Resident::new,
-    crates/sesmesh-gpu/src/hip/resident.rs:343-371)."""
+    crates/sesmesh-gpu/src/hip/resident.rs:343-371), from a host-built
+    Arrangement, or (Resident.build) built on the device."""
 
-    def __init__(self, device, arrangement):
+    def __init__(self, device, arrangement=None, _handle=None):
         self.device = device
         self._h = _vp()
-        _check(lib().smg_resident_new(device._h, ctypes.byref(arrangement.ref), ctypes.byref(self._h)))
+        if _handle is not None:
+            self._h = _handle
+        else:
+            _check(lib().smg_resident_new(device._h, ctypes.byref(arrangement.ref), ctypes.byref(self._h)))
+
+    @classmethod
+    def build(cls, device, xyzr, probe=1.4, kind="ses"):
+        """SesField::new + Resident::new with the arrangement built on the
+        device (smg_resident_build)."""
+        if kind not in KINDS:
+            raise ValueError(f"unknown surface kind {kind!r} (ses, sas or vdw)")
+        a = np.ascontiguousarray(xyzr, dtype=np.float64).reshape(-1, 4)
+        h = _vp()
+        _check(lib().smg_resident_build(device._h, _ptr(a), ctypes.c_size_t(len(a)), ctypes.c_double(probe),
+                                        ctypes.c_int(KINDS[kind]), ctypes.byref(h)))
+        return cls(device, _handle=h)
+
+    def info(self):
+        i = ResidentInfo()
+        _check(lib().smg_resident_info_get(self._h, ctypes.byref(i)))
+        out = {k: getattr(i, k) for k, _ in ResidentInfo._fields_}
+        out["kind"] = {v: k for k, v in KINDS.items()}[out["kind"]]
+        out["built_on_device"] = bool(out["built_on_device"])
+        return out
+
+    def arrangement(self):
+        """The resident's arrangement copied back to the host (an Arrangement)."""
+        h = _vp()
+        _check(lib().smg_resident_arrangement(self._h, ctypes.byref(h)))
+        return Arrangement._from_handle(h)
 
     @property
     def bytes(self):

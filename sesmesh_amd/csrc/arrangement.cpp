@@ -180,22 +180,11 @@ bool has_exposed_arc(std::vector<Interval>& iv, std::vector<Run>& runs) {
 
 }  // namespace
 
-// Cell grid of geom/cells.rs:27-77.
-void build_grid(const double* pts, size_t n, double cell, Grid& g) {
+// Dimensions of the cell grid of n points with bounds lo..hi
+// (geom/cells.rs:27-60): the cell edge doubles until the cells number at
+// most max(8 n, 2^16).
+void grid_shape(const double lo[3], const double hi[3], size_t n, double cell, Grid& g) {
     g.cell = cell;
-    if (n == 0) {
-        g.origin[0] = g.origin[1] = g.origin[2] = 0.0;
-        g.dims[0] = g.dims[1] = g.dims[2] = 1;
-        g.starts.assign(2, 0);
-        g.items.clear();
-        return;
-    }
-    double lo[3] = {pts[0], pts[1], pts[2]}, hi[3] = {pts[0], pts[1], pts[2]};
-    for (size_t i = 0; i < n; i++)
-        for (int a = 0; a < 3; a++) {
-            lo[a] = std::fmin(lo[a], pts[3 * i + a]);
-            hi[a] = std::fmax(hi[a], pts[3 * i + a]);
-        }
     size_t max_cells = std::max<size_t>(8 * n, size_t(1) << 16);
     for (;;) {
         unsigned __int128 total = 1;
@@ -218,6 +207,25 @@ void build_grid(const double* pts, size_t n, double cell, Grid& g) {
         g.cell *= 2.0;
     }
     for (int a = 0; a < 3; a++) g.origin[a] = lo[a];
+}
+
+// Cell grid of geom/cells.rs:27-77.
+void build_grid(const double* pts, size_t n, double cell, Grid& g) {
+    g.cell = cell;
+    if (n == 0) {
+        g.origin[0] = g.origin[1] = g.origin[2] = 0.0;
+        g.dims[0] = g.dims[1] = g.dims[2] = 1;
+        g.starts.assign(2, 0);
+        g.items.clear();
+        return;
+    }
+    double lo[3] = {pts[0], pts[1], pts[2]}, hi[3] = {pts[0], pts[1], pts[2]};
+    for (size_t i = 0; i < n; i++)
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::fmin(lo[a], pts[3 * i + a]);
+            hi[a] = std::fmax(hi[a], pts[3 * i + a]);
+        }
+    grid_shape(lo, hi, n, cell, g);
     size_t total = (size_t)g.dims[0] * g.dims[1] * g.dims[2];
     std::vector<size_t> key(n);
     for (size_t i = 0; i < n; i++) {

@@ -31,6 +31,9 @@ struct Arrangement {
 };
 
 void build_grid(const double* pts, size_t n, double cell, Grid& g);
+// The cell edge, dimensions and origin of the grid of n points with bounds
+// lo..hi (n > 0); build_grid then files the points into cells.
+void grid_shape(const double lo[3], const double hi[3], size_t n, double cell, Grid& g);
 // `kind` 0 (SES) builds the circles and SAS vertices; 1 (SAS) and 2 (vdW,
 // built with probe 0) only containment and neighbours, every sphere that is
 // not contained being exposed.

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -27,6 +28,7 @@
 #include "dev_mlp.hpp"
 #include "dev_measure.hpp"
 #include "dev_export.hpp"
+#include "dev_arrange.hpp"
 #include "host.hpp"
 
 namespace smg {
@@ -402,6 +404,8 @@ struct smg_resident {
     // Host copies needed to set up a run: centre bounding box.
     double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     size_t n_spheres = 0;
+    bool on_device = false;  // arrangement built by smg_resident_build on the device
+    double build_ms = 0.0;
 };
 
 // The meshes of one run: one part per molecule, concatenated in the
@@ -2001,6 +2005,492 @@ static int mesh_many(smg_resident* const* R, const smg_mesh_params* prm, size_t 
     report(d, SMG_STAGE_MESH, nb, nb);
     report(d, SMG_STAGE_ASSEMBLE, 0, 1);
     report(d, SMG_STAGE_ASSEMBLE, 1, 1);
+    return SMG_OK;
+}
+
+// ------------------------------------------------- arrangement on the device
+namespace {
+
+template <class X>
+int own(DevBuf& b, X** p, size_t count) {
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(X);
+    HIP_TRY(hipMalloc((void**)p, bytes));
+    b.ptrs.push_back(*p);
+    b.bytes += bytes;
+    return SMG_OK;
+}
+
+template <class K>
+int sort_pairs(Run& X, const K* kin, K* kout, const uint32_t* vin, uint32_t* vout, size_t n, uint32_t bits) {
+    if (n == 0) return SMG_OK;
+    size_t tmp_bytes = 0;
+    HIP_TRY(rocprim::radix_sort_pairs(nullptr, tmp_bytes, kin, kout, vin, vout, n, 0u, bits, X.s));
+    char* tmp;
+    TRY(X.temp(&tmp, tmp_bytes));
+    HIP_TRY(rocprim::radix_sort_pairs((void*)tmp, tmp_bytes, kin, kout, vin, vout, n, 0u, bits, X.s));
+    return SMG_OK;
+}
+
+// A cell grid of n interleaved device points with the given shape (CellGrid
+// ::new, geom/cells.rs:27-77): cells in (z, y, x) order, each listing its
+// points in index order. Its starts and items are owned by `keep`.
+int device_grid(Run& X, DevBuf& keep, const double* p3, size_t n, const Grid& shape, DGrid& g) {
+    const uint64_t cells = (uint64_t)shape.dims[0] * shape.dims[1] * shape.dims[2];
+    if (cells >= (1ull << 31)) return fail(SMG_E_LENGTH, "cell grid too large");
+    uint32_t *starts, *items;
+    TRY(own(keep, &starts, cells + 1));
+    TRY(own(keep, &items, n));
+    g.starts = (gptr<const uint32_t>)starts;
+    g.items = (gptr<const uint32_t>)items;
+    g.ox = shape.origin[0];
+    g.oy = shape.origin[1];
+    g.oz = shape.origin[2];
+    g.cell = shape.cell;
+    g.dx = shape.dims[0];
+    g.dy = shape.dims[1];
+    g.dz = shape.dims[2];
+    if (n == 0) {
+        HIP_TRY(hipMemsetAsync(starts, 0, (cells + 1) * 4, X.s));
+        return SMG_OK;
+    }
+    uint32_t *count, *key, *idx, *skey, *cnt;
+    TRY(X.temp(&count, cells));
+    TRY(X.temp(&key, n));
+    TRY(X.temp(&idx, n));
+    TRY(X.temp(&skey, n));
+    TRY(X.temp(&cnt, 1));
+    HIP_TRY(hipMemsetAsync(count, 0, cells * 4, X.s));
+    k_arr_cells<<<stride_grid(n), 256, 0, X.s>>>(p3, (uint32_t)n, g, key, idx, count);
+    HIP_TRY(hipGetLastError());
+    uint32_t bits = 1;
+    while (bits < 32 && (1ull << bits) < cells) bits++;
+    TRY(sort_pairs(X, key, skey, idx, items, n, bits));
+    const uint32_t c32 = (uint32_t)cells;
+    HIP_TRY(hipMemcpyAsync(cnt, &c32, 4, hipMemcpyHostToDevice, X.s));
+    TRY(scan_u32(X, count, starts, cnt, cells, starts + cells));
+    return SMG_OK;
+}
+
+// Bounds of n interleaved device points (n > 0) and, with `vals`, the
+// largest of them.
+int device_bounds(Run& X, const double* p3, size_t n, double lo[3], double hi[3], const double* vals, double* vmax) {
+    const uint32_t blocks = std::min<uint32_t>(blocks_for(n, 256), 512);
+    double* part;
+    unsigned long long* vm;
+    TRY(X.temp(&part, 6 * (size_t)blocks));
+    TRY(X.temp(&vm, 1));
+    HIP_TRY(hipMemsetAsync(vm, 0, 8, X.s));
+    k_arr_bounds<<<blocks, 256, 0, X.s>>>(p3, (uint32_t)n, part, vals, vm);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> h(6 * (size_t)blocks);
+    unsigned long long hv = 0;
+    HIP_TRY(hipMemcpyAsync(h.data(), part, h.size() * 8, hipMemcpyDeviceToHost, X.s));
+    HIP_TRY(hipMemcpyAsync(&hv, vm, 8, hipMemcpyDeviceToHost, X.s));
+    TRY(wait_stream(X.d, X.s));
+    for (int a = 0; a < 3; a++) {
+        lo[a] = h[a];
+        hi[a] = h[3 + a];
+    }
+    for (uint32_t b = 1; b < blocks; b++)
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::fmin(lo[a], h[6 * b + a]);
+            hi[a] = std::fmax(hi[a], h[6 * b + 3 + a]);
+        }
+    if (vmax) std::memcpy(vmax, &hv, 8);
+    return SMG_OK;
+}
+
+// Arrangement::build_with and the field's grids (arrangement.rs:437-645,
+// point.rs:213-274) on the device, into R. `undecided` is set, and nothing
+// of R is usable, when a circle's exposed-arc test was left to the host.
+int arrange_device(smg_device* d, const double* xyzr, size_t n_in, const std::vector<uint32_t>& active, double probe,
+                   int kind, double maxe, const double lo[3], const double hi[3], smg_resident& R, bool& undecided) {
+    undecided = false;
+    Temps T(d->stream);
+    Run X;
+    X.d = d;
+    X.s = d->stream;
+    X.T = &T;
+    const hipStream_t s = X.s;
+    const uint32_t n = (uint32_t)active.size();
+    DevBuf& keep = R.buf;
+    DArr& A = R.A;
+    const double cell = std::fmax(2.0 * maxe, 1e-6), fcell = std::fmax(0.8 * maxe, 1e-6);
+    double* dxyzr;
+    uint32_t *dact, *cnt;
+    TRY(X.temp(&dxyzr, 4 * n_in));
+    TRY(X.temp(&dact, n));
+    TRY(X.temp(&cnt, 8));
+    if (n_in) HIP_TRY(hipMemcpyAsync(dxyzr, xyzr, 32 * n_in, hipMemcpyHostToDevice, s));
+    if (n) HIP_TRY(hipMemcpyAsync(dact, active.data(), 4 * (size_t)n, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(cnt, &n, 4, hipMemcpyHostToDevice, s));
+    // Spheres in Morton order.
+    uint64_t *mkey, *skey;
+    uint32_t *mval, *orig;
+    TRY(X.temp(&mkey, n));
+    TRY(X.temp(&skey, n));
+    TRY(X.temp(&mval, n));
+    TRY(own(keep, &orig, n));
+    k_arr_morton<<<stride_grid(n), 256, 0, s>>>(dxyzr, dact, n, lo[0], lo[1], lo[2], cell, mkey, mval);
+    HIP_TRY(hipGetLastError());
+    TRY(sort_pairs(X, mkey, skey, mval, orig, n, 63));
+    double *cx, *cy, *cz, *c3, *r, *Rx;
+    SphereRec* srec;
+    TRY(own(keep, &cx, n));
+    TRY(own(keep, &cy, n));
+    TRY(own(keep, &cz, n));
+    TRY(own(keep, &r, n));
+    TRY(own(keep, &Rx, n));
+    TRY(own(keep, &srec, n));
+    TRY(X.temp(&c3, 3 * (size_t)n));
+    k_arr_gather<<<stride_grid(n), 256, 0, s>>>(dxyzr, orig, n, probe, cx, cy, cz, c3, r, Rx, srec);
+    HIP_TRY(hipGetLastError());
+    Grid shape;
+    shape.cell = cell;
+    if (n) grid_shape(lo, hi, n, cell, shape);
+    TRY(device_grid(X, keep, c3, n, shape, A.gs));
+    // Containment and the full neighbour rows.
+    uint8_t* contained;
+    uint32_t *nbc, *nboff, *nbitems;
+    TRY(X.temp(&contained, n));
+    TRY(X.temp(&nbc, n));
+    TRY(X.temp(&nboff, (size_t)n + 1));
+    ABuild B{cx, cy, cz, Rx, orig, contained, nboff, nullptr, n, maxe};
+    k_arr_contain<<<stride_grid(n, 64), 64, 0, s>>>(B, A.gs, contained);
+    k_arr_neighbours<false><<<stride_grid(n, 64), 64, 0, s>>>(B, A.gs, nbc, nullptr, nullptr);
+    HIP_TRY(hipGetLastError());
+    TRY(scan_u32(X, nbc, nboff, cnt, n, nboff + n));
+    uint32_t nbt = 0;
+    TRY(read_u32(d, nboff + n, nbt));
+    TRY(X.temp(&nbitems, nbt));
+    k_arr_neighbours<true><<<stride_grid(n, 64), 64, 0, s>>>(B, A.gs, nullptr, nboff, nbitems);
+    HIP_TRY(hipGetLastError());
+    B.nb_items = nbitems;
+    uint32_t* exposed;
+    TRY(X.temp(&exposed, n));
+    HIP_TRY(hipMemsetAsync(exposed, 0, 4 * (size_t)n, s));
+    // Circles, blockers and triple points (SES only).
+    uint32_t nc = 0, ncb = 0, nv = 0;
+    CircleOut o{};
+    double* m3 = nullptr;
+    double* v3 = nullptr;
+    uint32_t npairs = 0;
+    uint32_t *pc = nullptr, *poff = nullptr, *pi = nullptr, *pj = nullptr;
+    if (kind == SMG_KIND_SES && nbt) {
+        TRY(X.temp(&pc, n));
+        TRY(X.temp(&poff, (size_t)n + 1));
+        k_arr_pairs<false><<<stride_grid(n), 256, 0, s>>>(B, pc, nullptr, nullptr, nullptr);
+        HIP_TRY(hipGetLastError());
+        TRY(scan_u32(X, pc, poff, cnt, n, poff + n));
+        TRY(read_u32(d, poff + n, npairs));
+    }
+    if (npairs) {
+        TRY(X.temp(&pi, npairs));
+        TRY(X.temp(&pj, npairs));
+        k_arr_pairs<true><<<stride_grid(n), 256, 0, s>>>(B, nullptr, poff, pi, pj);
+        uint32_t *kp, *nb, *nvt, *coff, *boff, *voff, *flags;
+        TRY(X.temp(&kp, npairs));
+        TRY(X.temp(&nb, npairs));
+        TRY(X.temp(&nvt, npairs));
+        TRY(X.temp(&coff, (size_t)npairs + 1));
+        TRY(X.temp(&boff, (size_t)npairs + 1));
+        TRY(X.temp(&voff, (size_t)npairs + 1));
+        TRY(X.temp(&flags, 1));
+        HIP_TRY(hipMemsetAsync(flags, 0, 4, s));
+        k_arr_circles<false><<<stride_grid(npairs, 256), 256, 0, s>>>(B, pi, pj, npairs, kp, nb, nvt, nullptr,
+                                                                       nullptr, nullptr, o, flags);
+        HIP_TRY(hipGetLastError());
+        uint32_t fl = 0;
+        TRY(read_u32(d, flags, fl));
+        if (fl) {
+            undecided = true;
+            return SMG_OK;
+        }
+        HIP_TRY(hipMemcpyAsync(cnt + 1, &npairs, 4, hipMemcpyHostToDevice, s));
+        TRY(scan_u32(X, kp, coff, cnt + 1, npairs, coff + npairs));
+        TRY(scan_u32(X, nb, boff, cnt + 1, npairs, boff + npairs));
+        TRY(scan_u32(X, nvt, voff, cnt + 1, npairs, voff + npairs));
+        TRY(read_u32(d, coff + npairs, nc));
+        TRY(read_u32(d, boff + npairs, ncb));
+        TRY(read_u32(d, voff + npairs, nv));
+        double** soa[10] = {&o.mx, &o.my, &o.mz, &o.ax, &o.ay, &o.az, &o.e1x, &o.e1y, &o.e1z, &o.rho};
+        for (double** f : soa) TRY(own(keep, f, nc));
+        TRY(X.temp(&m3, 3 * (size_t)nc));
+        o.m3 = m3;
+        TRY(own(keep, &o.rec, nc));
+        TRY(own(keep, &o.ci, nc));
+        TRY(own(keep, &o.cj, nc));
+        TRY(own(keep, &o.cb_off, (size_t)nc + 1));
+        TRY(own(keep, &o.cb_items, ncb));
+        TRY(own(keep, &o.vx, nv));
+        TRY(own(keep, &o.vy, nv));
+        TRY(own(keep, &o.vz, nv));
+        TRY(X.temp(&v3, 3 * (size_t)nv));
+        o.v3 = v3;
+        TRY(own(keep, &o.vrec, nv));
+        TRY(own(keep, &o.vatoms, 3 * (size_t)nv));
+        o.exposed = exposed;
+        k_arr_circles<true><<<stride_grid(npairs, 256), 256, 0, s>>>(B, pi, pj, npairs, kp, nb, nvt, coff, boff, voff,
+                                                                      o, flags);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(o.cb_off + nc, &ncb, 4, hipMemcpyHostToDevice, s));
+    } else {
+        double** soa[10] = {&o.mx, &o.my, &o.mz, &o.ax, &o.ay, &o.az, &o.e1x, &o.e1y, &o.e1z, &o.rho};
+        for (double** f : soa) TRY(own(keep, f, 0));
+        TRY(own(keep, &o.rec, 0));
+        TRY(own(keep, &o.ci, 0));
+        TRY(own(keep, &o.cj, 0));
+        TRY(own(keep, &o.cb_off, 1));
+        TRY(own(keep, &o.cb_items, 0));
+        TRY(own(keep, &o.vx, 0));
+        TRY(own(keep, &o.vy, 0));
+        TRY(own(keep, &o.vz, 0));
+        TRY(own(keep, &o.vrec, 0));
+        TRY(own(keep, &o.vatoms, 0));
+        HIP_TRY(hipMemsetAsync(o.cb_off, 0, 4, s));
+    }
+    // Status, the field's neighbour rows, thresholds.
+    uint8_t* status;
+    uint32_t *fcount, *foff, *fitems;
+    SphereThr* thr;
+    TRY(own(keep, &status, n));
+    TRY(own(keep, &thr, n));
+    TRY(X.temp(&fcount, n));
+    TRY(own(keep, &foff, (size_t)n + 1));
+    k_arr_status<<<stride_grid(n), 256, 0, s>>>(B, exposed, kind == SMG_KIND_SES ? 1 : 0, status, fcount, thr);
+    HIP_TRY(hipGetLastError());
+    TRY(scan_u32(X, fcount, foff, cnt, n, foff + n));
+    uint32_t ft = 0;
+    TRY(read_u32(d, foff + n, ft));
+    TRY(own(keep, &fitems, ft));
+    k_arr_rows<<<stride_grid(n), 256, 0, s>>>(B, status, foff, fitems);
+    HIP_TRY(hipGetLastError());
+    // The field's circle and vertex grids, and the finer gather grids.
+    double rho_max = 0.0;
+    Grid cshape, vshape, fs, fc, fv;
+    cshape.cell = vshape.cell = cell;
+    fs.cell = fc.cell = fv.cell = fcell;
+    if (n) grid_shape(lo, hi, n, fcell, fs);
+    if (nc) {
+        double clo[3], chi[3];
+        TRY(device_bounds(X, m3, nc, clo, chi, o.rho, &rho_max));
+        grid_shape(clo, chi, nc, cell, cshape);
+        grid_shape(clo, chi, nc, fcell, fc);
+    }
+    if (nv) {
+        double vlo[3], vhi[3];
+        TRY(device_bounds(X, v3, nv, vlo, vhi, nullptr, nullptr));
+        grid_shape(vlo, vhi, nv, cell, vshape);
+        grid_shape(vlo, vhi, nv, fcell, fv);
+    }
+    TRY(device_grid(X, keep, m3, nc, cshape, A.gc));
+    TRY(device_grid(X, keep, v3, nv, vshape, A.gv));
+    TRY(device_grid(X, keep, c3, n, fs, A.fs));
+    TRY(device_grid(X, keep, m3, nc, fc, A.fc));
+    TRY(device_grid(X, keep, v3, nv, fv, A.fv));
+    A.cx = (gptr<const double>)cx;
+    A.cy = (gptr<const double>)cy;
+    A.cz = (gptr<const double>)cz;
+    A.R = (gptr<const double>)Rx;
+    A.r = (gptr<const double>)r;
+    A.status = (gptr<const uint8_t>)status;
+    A.orig = (gptr<const uint32_t>)orig;
+    A.nb_off = (gptr<const uint32_t>)foff;
+    A.nb_items = (gptr<const uint32_t>)fitems;
+    A.mx = (gptr<const double>)o.mx;
+    A.my = (gptr<const double>)o.my;
+    A.mz = (gptr<const double>)o.mz;
+    A.ax = (gptr<const double>)o.ax;
+    A.ay = (gptr<const double>)o.ay;
+    A.az = (gptr<const double>)o.az;
+    A.e1x = (gptr<const double>)o.e1x;
+    A.e1y = (gptr<const double>)o.e1y;
+    A.e1z = (gptr<const double>)o.e1z;
+    A.crho = (gptr<const double>)o.rho;
+    A.ci = (gptr<const uint32_t>)o.ci;
+    A.cj = (gptr<const uint32_t>)o.cj;
+    A.cb_off = (gptr<const uint32_t>)o.cb_off;
+    A.cb_items = (gptr<const uint32_t>)o.cb_items;
+    A.vx = (gptr<const double>)o.vx;
+    A.vy = (gptr<const double>)o.vy;
+    A.vz = (gptr<const double>)o.vz;
+    A.vatoms = (gptr<const uint32_t>)o.vatoms;
+    A.sph = (gptr<const SphereRec>)srec;
+    A.cir = (gptr<const CircleRec>)o.rec;
+    A.ver = (gptr<const VertexRec>)o.vrec;
+    A.thr = (gptr<const SphereThr>)thr;
+    A.n_spheres = n;
+    A.n_circles = nc;
+    A.n_vertices = nv;
+    A.kind = (uint32_t)kind;
+    A.probe = probe;
+    A.max_expanded = maxe;
+    A.rho_max = rho_max;
+    TRY(wait_stream(d, s));
+    return SMG_OK;
+}
+
+}  // namespace
+
+extern "C" int smg_resident_build(smg_device* dev, const double* xyzr, size_t n_in, double probe, int kind,
+                                  smg_resident** out) {
+    if (!out) return fail(SMG_E_INVALID, "null output");
+    *out = nullptr;
+    if (!dev) return fail(SMG_E_INVALID, "null device");
+    if (!xyzr && n_in) return fail(SMG_E_INVALID, "no spheres");
+    if (!(std::isfinite(probe) && probe >= 0.0)) return fail(SMG_E_INVALID, "probe radius must be finite and >= 0");
+    if (kind < SMG_KIND_SES || kind > SMG_KIND_VDW) return fail(SMG_E_INVALID, "unknown surface kind " + std::to_string(kind));
+    if (kind == SMG_KIND_SES && probe == 0.0) kind = SMG_KIND_VDW;  // field/point.rs:224-228
+    if (kind == SMG_KIND_VDW) probe = 0.0;                        // SurfaceParams::effective_probe
+    std::vector<uint32_t> active;
+    double maxe = 0.0, lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (size_t i = 0; i < n_in; i++) {
+        const double* s = xyzr + 4 * i;
+        if (!(std::isfinite(s[0]) && std::isfinite(s[1]) && std::isfinite(s[2])))
+            return fail(SMG_E_INVALID, "sphere " + std::to_string(i) + " has a non-finite centre");
+        if (!(std::isfinite(s[3]) && s[3] >= 0.0))
+            return fail(SMG_E_INVALID, "sphere " + std::to_string(i) + " has an invalid radius");
+        if (!(s[3] > 0.0)) continue;
+        if (active.empty())
+            for (int a = 0; a < 3; a++) lo[a] = hi[a] = s[a];
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::fmin(lo[a], s[a]);
+            hi[a] = std::fmax(hi[a], s[a]);
+        }
+        maxe = std::fmax(maxe, s[3] + probe);
+        active.push_back((uint32_t)i);
+    }
+    if (active.size() >= (1u << 30)) return fail(SMG_E_LENGTH, "arrangement too large for 30-bit candidate indices");
+    HIP_TRY(hipSetDevice(dev->ordinal));
+    const auto t0 = std::chrono::steady_clock::now();
+    std::unique_ptr<smg_resident> r(new smg_resident());
+    r->dev = dev;
+    r->n_spheres = active.size();
+    for (int a = 0; a < 3; a++) {
+        r->lo[a] = lo[a];
+        r->hi[a] = hi[a];
+    }
+    bool undecided = false;
+    const int rc = arrange_device(dev, xyzr, n_in, active, probe, kind, maxe, lo, hi, *r, undecided);
+    if (rc != SMG_OK) {
+        r->buf.release();
+        return rc;
+    }
+    if (undecided) {  // the host's atan2 decides: the whole arrangement on the host
+        r->buf.release();
+        r.reset();
+        smg_arrangement* a = nullptr;
+        TRY(smg_arrangement_build_kind(xyzr, n_in, probe, kind, 0, &a));
+        std::unique_ptr<smg_arrangement> ha(a);
+        smg_arrangement_ref ref;
+        TRY(smg_arrangement_view(a, &ref));
+        smg_resident* h = nullptr;
+        TRY(smg_resident_new(dev, &ref, &h));
+        r.reset(h);
+    } else {
+        r->on_device = true;
+    }
+    r->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    *out = r.release();
+    return SMG_OK;
+}
+
+extern "C" int smg_resident_info_get(const smg_resident* r, smg_resident_info* info) {
+    if (!r || !info) return fail(SMG_E_INVALID, "null resident or info");
+    info->n_spheres = r->A.n_spheres;
+    info->n_circles = r->A.n_circles;
+    info->n_vertices = r->A.n_vertices;
+    info->kind = (int32_t)r->A.kind;
+    info->built_on_device = r->on_device ? 1 : 0;
+    info->probe = r->A.probe;
+    info->build_ms = r->build_ms;
+    info->bytes = r->buf.bytes;
+    return SMG_OK;
+}
+
+// The resident's arrangement copied back to the host (both kinds of
+// resident hold the same arrays).
+extern "C" int smg_resident_arrangement(const smg_resident* r, smg_arrangement** out) {
+    if (!out) return fail(SMG_E_INVALID, "null output");
+    *out = nullptr;
+    if (!r) return fail(SMG_E_INVALID, "null resident");
+    HIP_TRY(hipSetDevice(r->dev->ordinal));
+    const DArr& A = r->A;
+    std::unique_ptr<smg_arrangement> h(new smg_arrangement());
+    Arrangement& a = h->a;
+    const size_t n = A.n_spheres, nc = A.n_circles, nv = A.n_vertices;
+    auto get = [&](auto& vec, const void* src, size_t count) -> int {
+        vec.resize(count);
+        if (count) HIP_TRY(hipMemcpy(vec.data(), src, count * sizeof(vec[0]), hipMemcpyDeviceToHost));
+        return SMG_OK;
+    };
+    std::vector<double> x, y, z;
+    TRY(get(x, (const void*)A.cx, n));
+    TRY(get(y, (const void*)A.cy, n));
+    TRY(get(z, (const void*)A.cz, n));
+    a.center.resize(3 * n);
+    for (size_t i = 0; i < n; i++) {
+        a.center[3 * i] = x[i];
+        a.center[3 * i + 1] = y[i];
+        a.center[3 * i + 2] = z[i];
+    }
+    TRY(get(a.radius, (const void*)A.r, n));
+    TRY(get(a.expanded, (const void*)A.R, n));
+    TRY(get(a.status, (const void*)A.status, n));
+    TRY(get(a.orig, (const void*)A.orig, n));
+    TRY(get(a.nb_off, (const void*)A.nb_off, n + 1));
+    TRY(get(a.nb_items, (const void*)A.nb_items, a.nb_off[n]));
+    std::vector<uint32_t> ci, cj;
+    TRY(get(ci, (const void*)A.ci, nc));
+    TRY(get(cj, (const void*)A.cj, nc));
+    a.circle_ij.resize(2 * nc);
+    for (size_t c = 0; c < nc; c++) {
+        a.circle_ij[2 * c] = ci[c];
+        a.circle_ij[2 * c + 1] = cj[c];
+    }
+    const void* cf[10] = {(const void*)A.mx,  (const void*)A.my,  (const void*)A.mz,  (const void*)A.ax,
+                          (const void*)A.ay,  (const void*)A.az,  (const void*)A.e1x, (const void*)A.e1y,
+                          (const void*)A.e1z, (const void*)A.crho};
+    a.circle.resize(10 * nc);
+    for (int q = 0; q < 10; q++) {
+        std::vector<double> f;
+        TRY(get(f, cf[q], nc));
+        for (size_t c = 0; c < nc; c++) a.circle[10 * c + q] = f[c];
+    }
+    TRY(get(a.cb_off, (const void*)A.cb_off, nc + 1));
+    TRY(get(a.cb_items, (const void*)A.cb_items, a.cb_off[nc]));
+    std::vector<double> vx, vy, vz;
+    TRY(get(vx, (const void*)A.vx, nv));
+    TRY(get(vy, (const void*)A.vy, nv));
+    TRY(get(vz, (const void*)A.vz, nv));
+    a.vertex.resize(3 * nv);
+    for (size_t v = 0; v < nv; v++) {
+        a.vertex[3 * v] = vx[v];
+        a.vertex[3 * v + 1] = vy[v];
+        a.vertex[3 * v + 2] = vz[v];
+    }
+    TRY(get(a.vertex_atoms, (const void*)A.vatoms, 3 * nv));
+    auto grid = [&](const DGrid& g, Grid& out) -> int {
+        out.origin[0] = g.ox;
+        out.origin[1] = g.oy;
+        out.origin[2] = g.oz;
+        out.cell = g.cell;
+        out.dims[0] = g.dx;
+        out.dims[1] = g.dy;
+        out.dims[2] = g.dz;
+        const size_t cells = (size_t)g.dx * g.dy * g.dz;
+        TRY(get(out.starts, (const void*)g.starts, cells + 1));
+        TRY(get(out.items, (const void*)g.items, out.starts[cells]));
+        return SMG_OK;
+    };
+    TRY(grid(A.gs, a.sphere_grid));
+    TRY(grid(A.gc, a.circle_grid));
+    TRY(grid(A.gv, a.vertex_grid));
+    a.kind = (int)A.kind;
+    a.probe = A.probe;
+    a.max_expanded = A.max_expanded;
+    a.rho_max = A.rho_max;
+    *out = h.release();
     return SMG_OK;
 }
 
