@@ -842,11 +842,10 @@ def compute_surface(xyzr, params=None, device=None):
     CPU back end's mesh."""
     params = params or SurfaceParams()
     dev = device or Device(0)
-    arr = Arrangement(xyzr, params.probe_radius, kind=params.kind)
-    res = Resident(dev, arr)
+    res = Resident.build(dev, xyzr, params.probe_radius, params.kind)  # the arrangement on the device
     m = res.mesh(params.spacing, params.brick_size, params.refine, params.sizing)
     out = m.arrays()
-    out["stats"] = {"kind": arr.kind, "touched_bricks": m.info.touched_bricks, "surface_bricks": m.info.surface_bricks,
+    out["stats"] = {"kind": res.info()["kind"], "touched_bricks": m.info.touched_bricks, "surface_bricks": m.info.surface_bricks,
                     "vertices": m.n_vertices, "triangles": m.n_triangles,
                     "refine_evaluations": m.info.refine_evaluations, "timings_ms": m.times.as_dict()}
     return out

@@ -214,13 +214,6 @@ __device__ inline double diamond_dev(const double v[2]) {
     return x < 0.0 ? 2.0 - y / (-x - y) : 3.0 + x / (x - y);
 }
 
-// A buried interval of the circle (first == last: its sphere), or, after
-// the merge, a run of them (its first and last sphere).
-struct IvD {
-    double start, end;
-    uint32_t first, last;
-};
-
 __device__ inline double rem_tau_dev(double a) {
     const double r = fmod(a, arrb::TAU);
     return r < 0.0 ? r + arrb::TAU : r;
@@ -229,63 +222,54 @@ __device__ inline double rem_tau_dev(double a) {
 __device__ inline D3v centre_of(const ABuild& B, uint32_t k) { return {B.cx[k], B.cy[k], B.cz[k]}; }
 
 // Whether an arc of the circle stays exposed (arrangement.rs:340-410), from
-// its buried intervals: 1 yes, 0 no, 2 undecided (an angle gap within
-// GAP_MARGIN of the threshold or of a wrap; the host decides those).
-// Runs keep the first and last sphere of their intervals; their enter and
-// leave points are recomputed from those spheres' cuts. The runs are
-// merged in place: run r is written after interval r has been read.
-__device__ inline int exposed_arc_dev(const ABuild& B, const CircleG& C, IvD* iv, uint32_t m) {
+// its buried intervals sorted by (start, end descending, sphere): 1 yes,
+// 0 no, 2 undecided (an angle gap within GAP_MARGIN of the threshold or of
+// a wrap; the host decides those). One lane. The intervals (st, en, and
+// first == last their sphere) are merged into runs in place, run r written
+// after interval r is read; a run keeps its first and last sphere, whose
+// cuts give its enter and leave points.
+__device__ inline int exposed_arc_runs(const ABuild& B, const CircleG& C, double* st, double* en, uint32_t* fi,
+                                       uint32_t* la, uint32_t m) {
     if (m == 0) return 1;
-    for (uint32_t a = 1; a < m; a++) {  // by (start, end descending, sphere), total order
-        const IvD v = iv[a];
-        const uint64_t ks = total_key(v.start), ke = total_key(v.end);
-        uint32_t b = a;
-        while (b > 0) {
-            const IvD u = iv[b - 1];
-            const uint64_t us = total_key(u.start), ue = total_key(u.end);
-            const bool less = ks < us || (ks == us && (ue < ke || (ue == ke && v.first < u.first)));
-            if (!less) break;
-            iv[b] = u;
-            b--;
-        }
-        iv[b] = v;
-    }
-    IvD* runs = iv;
     uint32_t nr = 0;
     for (uint32_t q = 0; q < m; q++) {
-        const IvD b = iv[q];
-        if (nr && b.start <= runs[nr - 1].end) {
-            IvD& l = runs[nr - 1];
-            if (b.end > l.end) {
-                l.end = b.end;
-                l.last = b.last;
+        const double bs = st[q], be = en[q];
+        const uint32_t bf = fi[q], bl = la[q];
+        if (nr && bs <= en[nr - 1]) {
+            if (be > en[nr - 1]) {
+                en[nr - 1] = be;
+                la[nr - 1] = bl;
             }
         } else {
-            runs[nr++] = b;
+            st[nr] = bs;
+            en[nr] = be;
+            fi[nr] = bf;
+            la[nr] = bl;
+            nr++;
         }
     }
     uint32_t head = 0;
     while (nr - head > 1) {
-        const IvD first = runs[head];
-        IvD& l = runs[nr - 1];
-        if (l.end - arrb::PERIOD < first.start) break;
-        if (first.end + arrb::PERIOD > l.end) {
-            l.end = first.end + arrb::PERIOD;
-            l.last = first.last;
+        const double fs = st[head], fe = en[head];
+        const uint32_t fl = la[head], l = nr - 1;
+        if (en[l] - arrb::PERIOD < fs) break;
+        if (fe + arrb::PERIOD > en[l]) {
+            en[l] = fe + arrb::PERIOD;
+            la[l] = fl;
         }
         head++;
     }
     const uint32_t m2 = nr - head;
     bool undecided = false;
     for (uint32_t k = 0; k < m2; k++) {
-        const IvD cur = runs[head + k], nx = runs[head + (k + 1) % m2];
-        const double ns = k + 1 < m2 ? nx.start : nx.start + arrb::PERIOD;
-        if (!(ns - cur.end > arrb::PSEUDO_EPS)) continue;
-        double en[2], lv[2], tmp[2];
-        (void)cut_dev(C, centre_of(B, cur.last), B.R[cur.last], tmp, lv);
-        (void)cut_dev(C, centre_of(B, nx.first), B.R[nx.first], en, tmp);
+        const uint32_t c = head + k, x = head + (k + 1) % m2;
+        const double ns = k + 1 < m2 ? st[x] : st[x] + arrb::PERIOD;
+        if (!(ns - en[c] > arrb::PSEUDO_EPS)) continue;
+        double ent[2], lv[2], tmp[2];
+        (void)cut_dev(C, centre_of(B, la[c]), B.R[la[c]], tmp, lv);
+        (void)cut_dev(C, centre_of(B, fi[x]), B.R[fi[x]], ent, tmp);
         const double a0 = rem_tau_dev(atan2(lv[1], lv[0]));
-        const double a1 = atan2(en[1], en[0]);
+        const double a1 = atan2(ent[1], ent[0]);
         const double g = rem_tau_dev(a1 - a0);
         if (g > arrb::ANG_EPS + arrb::GAP_MARGIN && g < arrb::TAU - arrb::GAP_MARGIN) return 1;
         undecided = true;
@@ -306,10 +290,33 @@ struct CircleOut {
     uint32_t* exposed;
 };
 
-// The circle of pair p = (i, j > i) (arrangement.rs:672-751): kept if no
-// common neighbour buries it and an arc stays exposed; its blockers are the
-// common neighbours cutting it (ascending), its triple points those of
-// blockers k > j (leave, then enter) that no other neighbour of i holds.
+// Orders LDS traffic between the lanes of one wavefront.
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ inline bool row_has(const uint32_t* row, uint32_t n, uint32_t k) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (row[mid] < k)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo < n && row[lo] == k;
+}
+
+constexpr uint32_t ARR_WAVES = 4;  // wavefronts per workgroup of k_arr_circles
+
+// The circle of pair p = (i, j > i) (arrangement.rs:672-751), one wavefront
+// per pair: kept if no common neighbour buries it and an arc stays exposed;
+// its blockers are the common neighbours cutting it (ascending), its triple
+// points those of blockers k > j (leave, then enter) that no other
+// neighbour of i holds. The lanes take the common neighbours 64 at a time;
+// the cutting ones are compacted into LDS in ascending order, ranked there
+// by (start, end descending, sphere), and one lane merges them into runs.
 // COUNT writes keep / blocker / vertex counts per pair; FILL writes the
 // records at the scanned offsets.
 template <bool FILL>
@@ -317,9 +324,18 @@ __global__ void __launch_bounds__(256) k_arr_circles(ABuild B, const uint32_t* p
                                                      uint32_t* keep, uint32_t* nblk, uint32_t* nvert,
                                                      const uint32_t* coff, const uint32_t* boff, const uint32_t* voff,
                                                      CircleOut o, uint32_t* flags) {
-    IvD iv[arrb::MAX_IV];
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += gridDim.x * blockDim.x) {
+    constexpr uint32_t M = arrb::MAX_IV;
+    __shared__ double u_st[ARR_WAVES][M], u_en[ARR_WAVES][M], s_st[ARR_WAVES][M], s_en[ARR_WAVES][M];
+    __shared__ uint32_t u_k[ARR_WAVES][M], s_fi[ARR_WAVES][M], s_la[ARR_WAVES][M];
+    __shared__ int s_res[ARR_WAVES];
+    const uint32_t w = threadIdx.x / WAVE, lane = lane_id();
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    const uint64_t waves = (uint64_t)gridDim.x * ARR_WAVES;
+    for (uint64_t p = (uint64_t)blockIdx.x * ARR_WAVES + w; p < npairs; p += waves) {
         if (FILL && !keep[p]) continue;
+        auto drop = [&]() {
+            if (!FILL && lane == 0) keep[p] = nblk[p] = nvert[p] = 0;
+        };
         const uint32_t i = pi[p], j = pj[p];
         const D3v ci = centre_of(B, i), cj = centre_of(B, j);
         const double Ri = B.R[i], Rj = B.R[j];
@@ -327,14 +343,16 @@ __global__ void __launch_bounds__(256) k_arr_circles(ABuild B, const uint32_t* p
         const D3v axis = ddiv(dsub(cj, ci), d);
         const double t = (d * d + Ri * Ri - Rj * Rj) / (2.0 * d);
         const double rho2 = Ri * Ri - t * t;
-        if (!FILL) keep[p] = nblk[p] = nvert[p] = 0;
-        if (rho2 <= 0.0) continue;
+        if (rho2 <= 0.0) {
+            drop();
+            continue;
+        }
         CircleG C;
         C.axis = axis;
         {  // geom/vec.rs:112-124: helper along the smallest axis component
             const double x = fabs(axis.x), y = fabs(axis.y), z = fabs(axis.z);
             const D3v h = (x <= y && x <= z) ? D3v{1, 0, 0} : (y <= z ? D3v{0, 1, 0} : D3v{0, 0, 1});
-            D3v e1 = dcross(h, axis);
+            const D3v e1 = dcross(h, axis);
             C.e1 = ddiv(e1, dnorm(e1));
         }
         C.e2 = dcross(axis, C.e1);
@@ -344,101 +362,149 @@ __global__ void __launch_bounds__(256) k_arr_circles(ABuild B, const uint32_t* p
         const uint32_t ni = B.nb_off[i + 1] - B.nb_off[i];
         const uint32_t* rowj = B.nb_items + B.nb_off[j];
         const uint32_t nj = B.nb_off[j + 1] - B.nb_off[j];
-        // Common neighbours in ascending order: buried, or the intervals.
-        uint32_t m = 0, jj = 0;
+        // Common neighbours, ascending: a burying one drops the circle; the
+        // cutting ones go to u_* in order (after the last pair's reads).
+        wave_sync();
+        uint32_t m = 0;
         bool buried = false, over = false;
-        for (uint32_t q = 0; q < ni && !buried; q++) {
-            const uint32_t k = rowi[q];
-            while (jj < nj && rowj[jj] < k) jj++;
-            if (!(jj < nj && rowj[jj] == k)) continue;
+        for (uint32_t base = 0; base < ni; base += WAVE) {
+            const uint32_t q = base + lane;
+            const uint32_t k = q < ni ? rowi[q] : 0u;
+            int c = 0;
             double en[2], lv[2];
-            const int c = cut_dev(C, centre_of(B, k), B.R[k], en, lv);
-            if (c == 1) {
+            if (q < ni && row_has(rowj, nj, k)) c = cut_dev(C, centre_of(B, k), B.R[k], en, lv);
+            if (__ballot(c == 1)) {
                 buried = true;
-            } else if (c == 2) {
-                if (m == arrb::MAX_IV) {
-                    over = true;
-                    break;
-                }
-                double s = diamond_dev(en), e = diamond_dev(lv);
-                if (e <= s) e += arrb::PERIOD;
-                iv[m++] = IvD{s, e, k, k};
+                break;
             }
+            const unsigned long long bm = __ballot(c == 2);
+            const uint32_t cnt = (uint32_t)__popcll(bm);
+            if (m + cnt > M) {
+                over = true;
+                break;
+            }
+            if (c == 2) {
+                const uint32_t slot = m + (uint32_t)__popcll(bm & lt);
+                const double s0 = diamond_dev(en);
+                double e0 = diamond_dev(lv);
+                if (e0 <= s0) e0 += arrb::PERIOD;
+                u_st[w][slot] = s0;
+                u_en[w][slot] = e0;
+                u_k[w][slot] = k;
+            }
+            m += cnt;
         }
         if (over) {
-            atomicOr(flags, arrb::F_OVERFLOW);
+            if (lane == 0) atomicOr(flags, arrb::F_OVERFLOW);
+            drop();
             continue;
         }
-        if (buried) continue;
-        const uint32_t nb = m;
-        if (!FILL) {
-            const int ex = exposed_arc_dev(B, C, iv, m);
-            if (ex == 2) atomicOr(flags, arrb::F_UNCERTAIN);
-            if (ex != 1) continue;
+        if (buried) {
+            drop();
+            continue;
         }
-        // Blockers and triple points, re-walking the common neighbours.
-        const uint32_t c = FILL ? coff[p] : 0u;
-        uint32_t bw = FILL ? boff[p] : 0u, vw = FILL ? voff[p] : 0u, nv = 0;
-        jj = 0;
-        for (uint32_t q = 0; q < ni; q++) {
-            const uint32_t k = rowi[q];
-            while (jj < nj && rowj[jj] < k) jj++;
-            if (!(jj < nj && rowj[jj] == k)) continue;
-            double en[2], lv[2];
-            if (cut_dev(C, centre_of(B, k), B.R[k], en, lv) != 2) continue;
-            if (FILL) o.cb_items[bw++] = k;
-            if (k <= j) continue;
-            const double* pts[2] = {lv, en};
-            for (int w = 0; w < 2; w++) {
-                const D3v qv = dadd(C.cc, dmul(dadd(dmul(C.e1, pts[w][0]), dmul(C.e2, pts[w][1])), C.radius));
-                bool free = true;
-                for (uint32_t u = 0; u < ni && free; u++) {
-                    const uint32_t l = rowi[u];
-                    if (l == j || l == k) continue;
-                    if (!(dnorm(dsub(qv, centre_of(B, l))) >= B.R[l] - EPS)) free = false;
+        wave_sync();
+        if (!FILL) {
+            // Rank each interval by (start, end descending, sphere) in the
+            // total order; the spheres differ, so the ranks are a permutation.
+            for (uint32_t e = lane; e < m; e += WAVE) {
+                const uint64_t ks = total_key(u_st[w][e]), ke = total_key(u_en[w][e]);
+                const uint32_t kk = u_k[w][e];
+                uint32_t r = 0;
+                for (uint32_t f = 0; f < m; f++) {
+                    const uint64_t fs = total_key(u_st[w][f]), fe = total_key(u_en[w][f]);
+                    r += (fs < ks || (fs == ks && (ke < fe || (fe == ke && u_k[w][f] < kk)))) ? 1u : 0u;
                 }
-                if (!free) continue;
-                nv++;
-                if (FILL) {
-                    o.vx[vw] = qv.x;
-                    o.vy[vw] = qv.y;
-                    o.vz[vw] = qv.z;
-                    o.v3[3 * (size_t)vw] = qv.x;
-                    o.v3[3 * (size_t)vw + 1] = qv.y;
-                    o.v3[3 * (size_t)vw + 2] = qv.z;
-                    o.vrec[vw] = VertexRec{qv.x, qv.y, qv.z, 0.0};
-                    o.vatoms[3 * (size_t)vw] = i;
-                    o.vatoms[3 * (size_t)vw + 1] = j;
-                    o.vatoms[3 * (size_t)vw + 2] = k;
-                    vw++;
-                }
+                s_st[w][r] = u_st[w][e];
+                s_en[w][r] = u_en[w][e];
+                s_fi[w][r] = kk;
+                s_la[w][r] = kk;
+            }
+            wave_sync();
+            if (lane == 0) s_res[w] = exposed_arc_runs(B, C, s_st[w], s_en[w], s_fi[w], s_la[w], m);
+            wave_sync();
+            const int ex = s_res[w];
+            if (ex == 2 && lane == 0) atomicOr(flags, arrb::F_UNCERTAIN);
+            if (ex != 1) {
+                drop();
+                continue;
             }
         }
+        // Triple points of the cutting neighbours k > j: leave, then enter,
+        // if no other neighbour of i holds them.
+        uint32_t nv = 0;
+        const uint32_t vw = FILL ? voff[p] : 0u;
+        for (uint32_t base = 0; base < m; base += WAVE) {
+            const uint32_t e = base + lane;
+            const uint32_t k = e < m ? u_k[w][e] : 0u;
+            bool f[2] = {false, false};
+            D3v qv[2];
+            if (e < m && k > j) {
+                double en[2], lv[2];
+                (void)cut_dev(C, centre_of(B, k), B.R[k], en, lv);
+                const double* pts[2] = {lv, en};
+                for (int s2 = 0; s2 < 2; s2++) {
+                    qv[s2] = dadd(C.cc, dmul(dadd(dmul(C.e1, pts[s2][0]), dmul(C.e2, pts[s2][1])), C.radius));
+                    bool free = true;
+                    for (uint32_t u = 0; u < ni && free; u++) {
+                        const uint32_t l = rowi[u];
+                        if (l == j || l == k) continue;
+                        if (!(dnorm(dsub(qv[s2], centre_of(B, l))) >= B.R[l] - EPS)) free = false;
+                    }
+                    f[s2] = free;
+                }
+            }
+            const unsigned long long b0 = __ballot(f[0]), b1 = __ballot(f[1]);
+            if (FILL) {
+                uint32_t at = vw + nv + (uint32_t)(__popcll(b0 & lt) + __popcll(b1 & lt));
+                for (int s2 = 0; s2 < 2; s2++) {
+                    if (!f[s2]) continue;
+                    o.vx[at] = qv[s2].x;
+                    o.vy[at] = qv[s2].y;
+                    o.vz[at] = qv[s2].z;
+                    o.v3[3 * (size_t)at] = qv[s2].x;
+                    o.v3[3 * (size_t)at + 1] = qv[s2].y;
+                    o.v3[3 * (size_t)at + 2] = qv[s2].z;
+                    o.vrec[at] = VertexRec{qv[s2].x, qv[s2].y, qv[s2].z, 0.0};
+                    o.vatoms[3 * (size_t)at] = i;
+                    o.vatoms[3 * (size_t)at + 1] = j;
+                    o.vatoms[3 * (size_t)at + 2] = k;
+                    at++;
+                }
+            }
+            nv += (uint32_t)(__popcll(b0) + __popcll(b1));
+        }
         if (!FILL) {
-            keep[p] = 1;
-            nblk[p] = nb;
-            nvert[p] = nv;
+            if (lane == 0) {
+                keep[p] = 1;
+                nblk[p] = m;
+                nvert[p] = nv;
+            }
             continue;
         }
-        o.mx[c] = C.cc.x;
-        o.my[c] = C.cc.y;
-        o.mz[c] = C.cc.z;
-        o.ax[c] = axis.x;
-        o.ay[c] = axis.y;
-        o.az[c] = axis.z;
-        o.e1x[c] = C.e1.x;
-        o.e1y[c] = C.e1.y;
-        o.e1z[c] = C.e1.z;
-        o.rho[c] = C.radius;
-        o.m3[3 * (size_t)c] = C.cc.x;
-        o.m3[3 * (size_t)c + 1] = C.cc.y;
-        o.m3[3 * (size_t)c + 2] = C.cc.z;
-        o.rec[c] = CircleRec{C.cc.x, C.cc.y, C.cc.z, axis.x, axis.y, axis.z, C.radius, 0.0};
-        o.ci[c] = i;
-        o.cj[c] = j;
-        o.cb_off[c] = boff[p];
-        o.exposed[i] = 1u;
-        o.exposed[j] = 1u;
+        for (uint32_t e = lane; e < m; e += WAVE) o.cb_items[boff[p] + e] = u_k[w][e];
+        if (lane == 0) {
+            const uint32_t c = coff[p];
+            o.mx[c] = C.cc.x;
+            o.my[c] = C.cc.y;
+            o.mz[c] = C.cc.z;
+            o.ax[c] = axis.x;
+            o.ay[c] = axis.y;
+            o.az[c] = axis.z;
+            o.e1x[c] = C.e1.x;
+            o.e1y[c] = C.e1.y;
+            o.e1z[c] = C.e1.z;
+            o.rho[c] = C.radius;
+            o.m3[3 * (size_t)c] = C.cc.x;
+            o.m3[3 * (size_t)c + 1] = C.cc.y;
+            o.m3[3 * (size_t)c + 2] = C.cc.z;
+            o.rec[c] = CircleRec{C.cc.x, C.cc.y, C.cc.z, axis.x, axis.y, axis.z, C.radius, 0.0};
+            o.ci[c] = i;
+            o.cj[c] = j;
+            o.cb_off[c] = boff[p];
+            o.exposed[i] = 1u;
+            o.exposed[j] = 1u;
+        }
     }
 }
 

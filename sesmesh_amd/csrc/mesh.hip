@@ -2197,8 +2197,9 @@ int arrange_device(smg_device* d, const double* xyzr, size_t n_in, const std::ve
         TRY(X.temp(&voff, (size_t)npairs + 1));
         TRY(X.temp(&flags, 1));
         HIP_TRY(hipMemsetAsync(flags, 0, 4, s));
-        k_arr_circles<false><<<stride_grid(npairs, 256), 256, 0, s>>>(B, pi, pj, npairs, kp, nb, nvt, nullptr,
-                                                                       nullptr, nullptr, o, flags);
+        const uint32_t cgrid = g1(std::min<uint64_t>(blocks_for(npairs, ARR_WAVES), 65536));
+        k_arr_circles<false><<<cgrid, ARR_WAVES * WAVE, 0, s>>>(B, pi, pj, npairs, kp, nb, nvt, nullptr, nullptr,
+                                                                 nullptr, o, flags);
         HIP_TRY(hipGetLastError());
         uint32_t fl = 0;
         TRY(read_u32(d, flags, fl));
@@ -2230,8 +2231,8 @@ int arrange_device(smg_device* d, const double* xyzr, size_t n_in, const std::ve
         TRY(own(keep, &o.vrec, nv));
         TRY(own(keep, &o.vatoms, 3 * (size_t)nv));
         o.exposed = exposed;
-        k_arr_circles<true><<<stride_grid(npairs, 256), 256, 0, s>>>(B, pi, pj, npairs, kp, nb, nvt, coff, boff, voff,
-                                                                      o, flags);
+        k_arr_circles<true><<<cgrid, ARR_WAVES * WAVE, 0, s>>>(B, pi, pj, npairs, kp, nb, nvt, coff, boff, voff, o,
+                                                                flags);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(o.cb_off + nc, &ncb, 4, hipMemcpyHostToDevice, s));
     } else {
