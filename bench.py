@@ -609,7 +609,7 @@ def main():
         out["speedup_vs_cpu_baseline"] = cpu["value"] / s_per_mesh
     print(json.dumps(out), flush=True)
     # Ordered teardown: meshes, plans and residents before the device.
-    del res, arr
+    del res
     dev.close()
 
 

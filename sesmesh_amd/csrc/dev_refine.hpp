@@ -432,23 +432,36 @@ __device__ inline bool precedes(double d, uint32_t p, double best, uint32_t bo) 
 }
 
 // Distance from (x, y, z) of a class-K candidate (record d) as eval_local
-// compares it (field/point.rs:419-489); +inf for a sphere through the point.
+// compares it (field/point.rs:419-489); +inf for a sphere through the point,
+// and for a candidate whose squared distances already show it farther than
+// `best` by more than 1e-9 A (the square roots are then skipped). That
+// margin is far above the rounding of the squares and roots (about 1e-15
+// A here), so the distance computed in full would exceed `best` too: the
+// candidate could not be taken either way.
 template <int K>
-__device__ inline double cand_dist(const double* d, double x, double y, double z) {
+__device__ inline double cand_dist(const double* d, double x, double y, double z, double best) {
     if (K == 0) {
         double vx = x - d[0], vy = y - d[1], vz = z - d[2];
-        double len = sqrt(vx * vx + vy * vy + vz * vz);
-        return len < 1e-12 ? INFINITY : fabs(len - d[3]);
+        const double s = vx * vx + vy * vy + vz * vz, R = d[3];
+        const double hi = R + best + 1e-9, lo = R - best - 1e-9;
+        if (s >= hi * hi || (lo > 0.0 && s <= lo * lo)) return INFINITY;
+        double len = sqrt(s);
+        return len < 1e-12 ? INFINITY : fabs(len - R);
     }
+    const double b2 = (best + 1e-9) * (best + 1e-9);
     if (K == 1) {
         double yx = x - d[0], yy = y - d[1], yz = z - d[2];
         double ax = d[3], ay = d[4], az = d[5], rho = d[6];
         double along = yx * ax + yy * ay + yz * az;
+        if (along * along >= b2) return INFINITY;  // the sum below is at least along^2
         double wx = yx - ax * along, wy = yy - ay * along, wz = yz - az * along;
         double wl = sqrt(wx * wx + wy * wy + wz * wz);
-        return sqrt(along * along + (wl - rho) * (wl - rho));
+        const double D = along * along + (wl - rho) * (wl - rho);
+        return D >= b2 ? INFINITY : sqrt(D);
     }
-    return ddist(x, y, z, d[0], d[1], d[2]);
+    double dx = x - d[0], dy = y - d[1], dz = z - d[2];
+    const double s = dx * dx + dy * dy + dz * dz;
+    return s >= b2 ? INFINITY : sqrt(s);
 }
 
 // Projection of x onto candidate p (its probe position, point.rs:419-489).
@@ -529,7 +542,7 @@ __device__ inline void scan_class(const DArr& A, const RLists& s, uint32_t cc, u
             uint32_t p = p0 + j * G.g;
             bool use = p < e && !stop && !(key_value(s.ckey[p]) > lim);
             stop |= !use;
-            dist[j] = use ? cand_dist<K>(s.cdat + (size_t)CDAT * p, x, y, z) : INFINITY;
+            dist[j] = use ? cand_dist<K>(s.cdat + (size_t)CDAT * p, x, y, z, best) : INFINITY;
         }
         for (int j = 0; j < W; j++) {
             uint32_t p = p0 + j * G.g;
