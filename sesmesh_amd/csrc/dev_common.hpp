@@ -380,9 +380,20 @@ constexpr uint32_t KIND_SHIFT = 30, ID_MASK = (1u << KIND_SHIFT) - 1u;
 // the sampling ball plus COVER_PAD holds the refinement's ball, its near
 // spheres and candidates are among the refinement's, and the cover limit
 // is the same, so each of them is hidden there too.
+// The refinement's candidate superset of each sampled sub-block j: ids (a
+// sphere id, or a circle / SAS vertex id | class << KIND_SHIFT) of every
+// sphere, circle and vertex the refinement's gather of that sub-block
+// tests positively, found by the sampling's grid walk; the refinement
+// tests just these instead of walking the grids again. cnt[j] NO_LIST: no
+// list (the refinement walks). ids null: none are kept.
+struct SupList {
+    uint32_t *off, *cnt, *ids, *used;
+    uint32_t cap;
+};
 struct HidList {
     uint32_t *off, *cnt, *ids, *used;
     uint32_t cap;
+    SupList sup;  // handed over the same way
 };
 constexpr uint32_t NO_LIST = 0xffffffffu;
 constexpr uint32_t KEY_BUCKETS = 16;  // key buckets per candidate class in the gathers
@@ -662,6 +673,7 @@ __device__ inline void gather3(const DArr& A, double x, double y, double z, cons
         for_near(A.gs, x, y, z, r[0], tid, nt, [&](uint32_t k) { f(0u, k); });
         for_near(A.gc, x, y, z, r[1], tid, nt, [&](uint32_t k) { f(1u, k); });
         for_near(A.gv, x, y, z, r[2], tid, nt, [&](uint32_t k) { f(2u, k); });
+        __syncthreads();  // the caller reads what f gathered
         return;
     }
     uint32_t len = 0, start = 0;

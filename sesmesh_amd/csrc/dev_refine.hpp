@@ -149,7 +149,8 @@ enum { R_NEAR = 0, R_IN, R_S, R_C, R_V, R_CAND, R_E, R_OVER, R_N };
 // the same sub-block listed them, else found with mark_hidden.
 __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s, uint32_t* n, uint32_t* tmp4,
                                      double bx, double by, double bz, double rho, double band, bool hide,
-                                     const uint32_t* hl = nullptr, uint32_t hn = NO_LIST) {
+                                     const uint32_t* hl = nullptr, uint32_t hn = NO_LIST,
+                                     const uint32_t* sl = nullptr, uint32_t sn = NO_LIST) {
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     __shared__ uint32_t bstart[WAVE], bfill[WAVE];  // per key bucket: 16 for each of S, C, V
     for (uint32_t i = tid; i < cap.hash; i += nt) s.hk[i] = 0u;
@@ -168,7 +169,9 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
     auto bucket = [&](uint32_t cls, double key) {
         return cls * KEY_BUCKETS + min(KEY_BUCKETS - 1u, (uint32_t)(fmax(key + key0, 0.0) * per_bucket));
     };
-    gather3(A, bx, by, bz, radii, tmp4, [&](uint32_t grid, uint32_t k) {
+    // The grids' items near the ball, or (sl) the sampling's list of the
+    // items these tests take (SupList); the tests are the same either way.
+    auto visit = [&](uint32_t grid, uint32_t k) {
         if (grid == 0 && sph) {
             if (A.status[k] == S_CONTAINED) return;
             const SphereRec S = load_rec(A.sph, k);
@@ -261,7 +264,16 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
             s.ckey[q] = total_key(d);
             s.cid[q] = k | (2u << KIND_SHIFT);
         }
-    });
+    };
+    if (sl) {
+        for (uint32_t i = tid; i < sn; i += nt) {
+            const uint32_t e = sl[i];
+            visit(e >> KIND_SHIFT, e & ID_MASK);
+        }
+        __syncthreads();
+    } else {
+        gather3(A, bx, by, bz, radii, tmp4, visit);
+    }
     if (n[R_OVER]) return false;
     const uint32_t gathered = n[R_CAND];
     // Candidates one blocker hides from the whole ball leave the lists (their
@@ -913,7 +925,10 @@ __device__ __forceinline__ void refine_sub(const Mol* __restrict__ mols, const u
     const uint32_t* hl = nullptr;  // the sampling's hidden candidates of this sub-block
     uint32_t hn = hid.cnt ? hid.cnt[j] : NO_LIST;
     if (hn != NO_LIST) hl = hid.ids + hid.off[j];
-    bool ok = gather_refine(A, cap, s, n, tmp4, c[0], c[1], c[2], rho, P.band, true, hl, hn);
+    const uint32_t* sl = nullptr;  // and its superset of this gather's items
+    uint32_t sn = hid.sup.cnt ? hid.sup.cnt[j] : NO_LIST;
+    if (sn != NO_LIST) sl = hid.sup.ids + hid.sup.off[j];
+    bool ok = gather_refine(A, cap, s, n, tmp4, c[0], c[1], c[2], rho, P.band, true, hl, hn, sl, sn);
     // Crossing edges whose lower sample lies in this sub-block, ascending.
     uint32_t lo[3], hi[3];
     for (int a = 0; a < 3; a++) {

@@ -133,7 +133,7 @@ __device__ inline Lists carve(char* base, const Layout& L) {
 }
 
 // Counters of the list builder.
-enum { C_NEAR = 0, C_IN, C_S, C_C, C_V, C_CAND, C_OVER, C_N };
+enum { C_NEAR = 0, C_IN, C_S, C_C, C_V, C_CAND, C_OVER, C_SUP, C_SUPOVER, C_N };
 constexpr uint32_t HIDDEN = 0x80000000u;  // a candidate's bucket word when it is hidden
 
 // True if (qx, qy, qz) lies inside a blocker of candidate p (kernel.rs:
@@ -256,9 +256,14 @@ __device__ inline float sphere_value(const Lists& s, const uint32_t* n, float x,
 // Gathers the lists of sub-block `sb` of brick `key` (kernel.rs:553-751).
 // The SAS and vdW kinds (gather_spheres, kernel.rs:645-652) list the near
 // spheres within band + rho, with their radii, and nothing else.
-// Returns false (in every thread) if a capacity was exceeded.
+// Returns false (in every thread) if a capacity was exceeded. The walk
+// covers the refinement's slightly larger ball (rho (1 + 1e-9) + 1e-9,
+// mesh/extract.rs:172) too, and the items the refinement's gather of this
+// sub-block (j = 8 brick + sb) will take go to `sup` (see SupList); the
+// sampling's own tests are unchanged.
 __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, const Caps& cap, Lists& s,
-                                  uint32_t* n, uint32_t* tmp4, const int32_t* key, uint32_t sb) {
+                                  uint32_t* n, uint32_t* tmp4, const int32_t* key, uint32_t sb,
+                                  const SupList& sup, uint32_t j) {
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     __shared__ uint32_t bfill[WAVE];
     for (uint32_t i = tid; i < cap.hash; i += nt) s.hk[i] = 0u;
@@ -269,6 +274,20 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
     double rho = 0.5 * sqrt(3.0) * sub * h;
     const bool sph = A.kind != K_SES;
     double reach = (sph ? 0.0 : P.probe) + P.band + rho;
+    // The refinement's ball and reach (refine_sub / gather_refine), computed
+    // as it computes them.
+    const double rho_r = 0.5 * sqrt(3.0) * sub * h * (1.0 + 1e-9) + 1e-9;
+    const double reach_r = (sph ? 0.0 : A.probe) + P.band + rho_r;
+    uint32_t* stage = (uint32_t*)s.cdat;  // free until the candidate records are stored
+    const uint32_t stage_cap = sup.ids ? 12u * cap.cand : 0u;
+    auto sup_add = [&](uint32_t code) {
+        if (!stage_cap) return;
+        const uint32_t at = atomicAdd(&n[C_SUP], 1u);
+        if (at < stage_cap)
+            stage[at] = code;
+        else
+            n[C_SUPOVER] = 1u;
+    };
     double b[3];
     for (int a = 0; a < 3; a++) {
         double base = (double)((int64_t)key[a] * (int64_t)P.b) * h;
@@ -276,7 +295,7 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
         b[a] = base + off * h;
     }
     const uint32_t hmask = cap.hash - 1;
-    const double radii[3] = {reach + A.max_expanded, reach + A.rho_max, reach};
+    const double radii[3] = {reach_r + A.max_expanded, reach_r + A.rho_max, reach_r};
     const double per_bucket = KEY_BUCKETS / reach;
     auto bucket = [&](double key) { return min(KEY_BUCKETS - 1u, (uint32_t)(key * per_bucket)); };
     gather3(A, b[0], b[1], b[2], radii, tmp4, [&](uint32_t grid, uint32_t k) {
@@ -285,6 +304,11 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
             const SphereRec S = load_rec(A.sph, k);
             double R = S.R;
             double d = ddist(b[0], b[1], b[2], S.x, S.y, S.z);
+            // gather_refine: SES near (d < R + reach) or class S (exposed,
+            // |d - R| <= reach); SAS / vdW d - R < reach.
+            if (sph ? d - R < reach_r
+                    : (d < R + reach_r || (A.status[k] == S_EXPOSED && fabs(d - R) <= reach_r)))
+                sup_add(k);
             if (d >= R + reach) return;
             uint32_t at = atomicAdd(&n[C_NEAR], 1u);
             if (at >= cap.near) {
@@ -325,6 +349,7 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
             double wl = sqrt(wx * wx + wy * wy + wz * wz);
             double rr = C.rho;
             double dist = sqrt(along * along + (wl - rr) * (wl - rr));
+            if (dist <= reach_r) sup_add(k | (1u << KIND_SHIFT));
             if (!(dist <= reach)) return;
             atomicAdd(&n[C_C], 1u);
             uint32_t q = atomicAdd(&n[C_CAND], 1u);
@@ -336,7 +361,9 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
             s.cid[q] = k;
         } else {  // class V (kernel.rs:738-747)
             const VertexRec V = load_rec(A.ver, k);
-            if (!(ddist(b[0], b[1], b[2], V.x, V.y, V.z) <= reach)) return;
+            const double dv = ddist(b[0], b[1], b[2], V.x, V.y, V.z);
+            if (dv <= reach_r) sup_add(k | (2u << KIND_SHIFT));
+            if (!(dv <= reach)) return;
             uint32_t q = atomicAdd(&n[C_V], 1u);
             if (q >= cap.v) {
                 n[C_OVER] = 1u;
@@ -346,6 +373,24 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
         }
     });
     if (n[C_OVER]) return false;
+    if (stage_cap) {  // the superset to the pool (none if it overflowed)
+        __shared__ uint32_t sup_at;
+        const uint32_t m = n[C_SUP];
+        if (threadIdx.x == 0) {
+            uint32_t at = NO_LIST;
+            if (!n[C_SUPOVER]) {
+                at = atomicAdd(sup.used, m);
+                if (at > sup.cap || m > sup.cap - at) at = NO_LIST;
+            }
+            sup_at = at;
+            sup.off[j] = at == NO_LIST ? 0u : at;
+            sup.cnt[j] = at == NO_LIST ? NO_LIST : m;
+        }
+        __syncthreads();
+        if (sup_at != NO_LIST)
+            for (uint32_t i = tid; i < m; i += nt) sup.ids[sup_at + i] = stage[i];
+        __syncthreads();
+    }
     const uint32_t gathered = n[C_CAND];
     // Candidates one blocker hides from the whole sub-block leave the lists.
     // The test runs on the f32 frame-local values the lane kernel uses (the
@@ -458,7 +503,7 @@ __device__ __forceinline__ void sample_sub(const Mol* __restrict__ mols, const u
     Lists s = carve(base, lay);
     const int32_t* key = keys + 3 * brick;
     if (threadIdx.x == 0) n_owned = 0;
-    bool ok = gather_sub(A, Lc, P, cap, s, n, tmp4, key, sb);
+    bool ok = gather_sub(A, Lc, P, cap, s, n, tmp4, key, sb, hid.sup, j);
     if (!ok) {
         if (threadIdx.x == 0) overflow[atomicAdd(n_overflow, 1u)] = j;
         return;

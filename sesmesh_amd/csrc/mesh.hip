@@ -1001,6 +1001,7 @@ constexpr RCaps UNIFORM_CAPS[UNIFORM_TIERS] = {
 constexpr uint32_t LDS_EDGES = 512;      // crossing-edge buffer of the LDS passes
 constexpr uint32_t SCRATCH_CHUNK = 1024;  // default workgroups of a scratch pass (one list slice each)
 constexpr uint32_t HID_PER_SUB = 64;      // hidden-candidate pool entries per sampled sub-block
+constexpr uint32_t SUP_PER_SUB = 160;     // refinement-superset pool entries per sampled sub-block
 constexpr uint32_t NO_CAP = 0xffffffffu;
 
 // Device counters of a run (u32 slots, zero at the start of the run).
@@ -1014,6 +1015,7 @@ enum : uint32_t {
     K_RLIST0, K_RPRED0 = K_RLIST0 + TIERS, K_RLAST = K_RPRED0 + TIERS,
     K_Q_SUBST, K_Q_SAMPLE0, K_Q_REFINE0 = K_Q_SAMPLE0 + TIERS,
     K_HID = K_Q_REFINE0 + TIERS,  // entries taken from the hidden-candidate pool
+    K_SUP,                        // entries taken from the refinement-superset pool
     K_N
 };
 // Capacity of each tier in candidate-load units (max(S + C, 2 near, 4 SAS
@@ -1606,6 +1608,19 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     hid.cap = (uint32_t)hcap;
     k_fill_u32<<<stride_grid(nsub), 256, 0, s>>>(hid.cnt, NO_LIST, nsub, &cnt[K_NSUB]);
     HIP_TRY(hipGetLastError());
+    // The refinement's candidate supersets (SupList), SUP_PER_SUB entries per
+    // sampled sub-block; not kept by allocation-free runs, whose fixed
+    // scratch they would crowd.
+    if (!X.arena) {
+        TRY(X.temp(&hid.sup.off, nsub));
+        TRY(X.temp(&hid.sup.cnt, nsub));
+        const uint64_t scap = std::min<uint64_t>((uint64_t)SUP_PER_SUB * std::max<uint32_t>(nsw, 1u), 0xfffffff0ull);
+        TRY(X.temp(&hid.sup.ids, scap));
+        hid.sup.used = &cnt[K_SUP];
+        hid.sup.cap = (uint32_t)scap;
+        k_fill_u32<<<stride_grid(nsub), 256, 0, s>>>(hid.sup.cnt, NO_LIST, nsub, &cnt[K_NSUB]);
+        HIP_TRY(hipGetLastError());
+    }
     {
         bool runs[TIERS];
         for (int p = 0; p < TIERS; p++) {
