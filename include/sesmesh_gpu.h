@@ -196,7 +196,7 @@ int smg_resident_arrangement(const smg_resident* r, smg_arrangement** out);
  * Parameters of a mesh stage: SurfaceParams' spacing, brick size and
  * refinement mode (crates/sesmesh/src/surface.rs:195-300). */
 #define SMG_REFINE_EXACT 0 /* f64 refinement, CPU-identical bits */
-#define SMG_REFINE_FAST 1  /* f32 compensated refinement (not identical) */
+#define SMG_REFINE_FAST 1  /* f32 refinement (not identical) */
 typedef struct {
     double spacing;
     uint32_t brick;       /* even, 4..=32 (2x2x2 sub-blocks) */

@@ -892,7 +892,7 @@ def mesh_area_volume(positions, triangles):
 
 
 def precision_report(resident, spacing, brick=10):
-    """Fast (f32 compensated) against exact (f64) refinement of one
+    """Fast (f32) against exact (f64) refinement of one
     molecule: |F| at every fast and exact vertex (F: the exact f64 clamped
     field), and the relative area and volume error of the fast mesh."""
     import time

@@ -1,16 +1,13 @@
 // Fast refinement: the exact path's root search and attributes with the
-// candidate evaluation in f32 and compensated arithmetic.
+// candidate evaluation in f32.
 //
 // The sub-block's lists are gathered exactly as in exact mode (f64
 // decisions), then converted once to f32 records relative to the ball
-// centre, where coordinates are a few angstroms and f32 keeps about 1e-6 A.
-// Squared lengths and dot products are Dot2 sums (Ogita, Rump and Oishi,
-// "Accurate sum and dot product", SIAM J. Sci. Comput. 26, 2005): every
-// product's rounding error is recovered with an FMA (TwoProduct) and every
-// addition's with TwoSum (Knuth), so the result is as accurate as if
-// computed in twice the working precision and then rounded. The Newton /
-// Illinois / bisection iteration runs in f32 on the edge parameter t; the
-// vertex is placed, and its normal and atom found, in f64 from that t.
+// centre, where coordinates are a few angstroms and f32 keeps about 1e-6 A
+// (far below the 1e-5 A root tolerance). Squared lengths and dot products
+// are FMA chains. The Newton / Illinois / bisection iteration runs in f32
+// on the edge parameter t; the vertex is placed, and its normal and atom
+// found, in f64 from that t.
 // Results are not bit-identical to the CPU back end; their distance from
 // the exact surface is what fast mode reports. An edge the f32 Newton /
 // Illinois phase does not converge on is redone by the exact f64 search,
@@ -20,26 +17,9 @@
 
 namespace smg {
 
-__device__ inline void two_prod(float a, float b, float& p, float& e) {
-    p = a * b;
-    e = __builtin_fmaf(a, b, -p);
-}
-__device__ inline void two_sum(float a, float b, float& s, float& e) {
-    s = a + b;
-    float bb = s - a;
-    e = (a - (s - bb)) + (b - bb);
-}
-// a . b for 3-vectors, Dot2.
+// a . b for 3-vectors.
 __device__ inline float dot3c(float ax, float ay, float az, float bx, float by, float bz) {
-    float p, s, h, r, q;
-    two_prod(ax, bx, p, s);
-    two_prod(ay, by, h, r);
-    two_sum(p, h, p, q);
-    s = s + (q + r);
-    two_prod(az, bz, h, r);
-    two_sum(p, h, p, q);
-    s = s + (q + r);
-    return p + s;
+    return __builtin_fmaf(ax, bx, __builtin_fmaf(ay, by, az * bz));
 }
 
 // f32 lists, after the f64 ones in the same LDS / scratch slice.
@@ -529,8 +509,11 @@ __device__ __forceinline__ void refine_sub_fast(const Mol* __restrict__ mols, co
     }
 }
 
+// Five wavefronts per SIMD (at most 96 VGPRs), as k_refine: the f32 search
+// and the exact redo share the kernel, and unbounded they take 137 VGPRs
+// (three wavefronts).
 template <bool GLOBAL, bool PERSIST>
-__global__ void __launch_bounds__(256) k_refine_fast(const Mol* __restrict__ mols, const uint32_t* bmol, RCaps cap,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_refine_fast(const Mol* __restrict__ mols, const uint32_t* bmol, RCaps cap,
                                                      RLayout lay, FLayout flay, const int32_t* keys,
                                                      const float* values, Queue q, HidList hid,
                                                      const unsigned long long* mask,
