@@ -321,6 +321,48 @@ __global__ void __launch_bounds__(64) k_classify_bricks(const Mol* __restrict__ 
     });
 }
 
+// Brick classification and sub-block states of the touched bricks whose
+// shared-gather lists overflow (k_classify_touched): one wavefront per
+// brick of the queue q (persistent): keep[w], and for a surface brick its
+// eight sub-block states in stt[8 w + sb].
+__global__ void __launch_bounds__(64) k_classify_touched_wave(const Mol* __restrict__ mols, const uint64_t* wofs,
+                                                              uint32_t nm, const uint32_t* touched, Queue q,
+                                                              uint32_t* keep, uint8_t* stt) {
+    __shared__ uint32_t ticket;
+    for_each_item<true>(q, &ticket, [&](uint32_t w) {
+        const uint32_t bit = touched[w];
+        const uint32_t m = find_mol(wofs, nm, (uint64_t)(bit >> 5));
+        const Mol& M = mols[m];
+        const DRun& P = M.P;
+        int32_t key[3];
+        key_of_lin(P, bit - 32 * wofs[m], key);
+        double c[3];
+        brick_center(P, key, c);
+        const uint8_t bs = uniform_sign_wave(M.A, c[0], c[1], c[2], 0.5 * sqrt(3.0) * (double)P.b * P.h);
+        if (lane_id() == 0) keep[w] = bs == ST_NONE ? 1u : 0u;
+        if (bs != ST_NONE) return;
+        const double sub = (double)P.sub, h = P.h, rho = 0.5 * sqrt(3.0) * sub * h, half = sub * 0.5;
+        for (uint32_t i = 0; i < 8; i++) {
+            for (int a = 0; a < 3; a++) {
+                const double base = (double)((int64_t)key[a] * (int64_t)P.b) * h;
+                c[a] = base + ((double)((i >> a) & 1) * sub + half) * h;
+            }
+            const uint8_t st = uniform_sign_wave(M.A, c[0], c[1], c[2], rho);
+            if (lane_id() == 0) stt[8 * (size_t)w + i] = st;
+        }
+    });
+}
+
+// Sub-block states of a surface brick, gathered from the touched-brick
+// order: st[8 * off[w] + i] = stt[8 * w + i] for every kept w.
+__global__ void k_gather_states(const uint32_t* keep, const uint32_t* off, const uint32_t* n, uint32_t cap,
+                                const uint8_t* stt, uint8_t* st) {
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < *n; w += gridDim.x * blockDim.x) {
+        if (!keep[w] || off[w] >= cap) continue;
+        for (int i = 0; i < 8; i++) st[8 * (size_t)off[w] + i] = stt[8 * (size_t)w + i];
+    }
+}
+
 // Sub-block states of surface bricks (kernel.rs:467-482): one wavefront per
 // brick of the queue q (persistent), its eight sub-blocks in turn; st[8 *
 // brick + sb]. Used for bricks whose shared-gather lists overflow.

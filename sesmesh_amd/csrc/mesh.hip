@@ -1075,10 +1075,11 @@ LdsPlan lds_plan(uint32_t b, bool fast, uint64_t lds_max, const uint32_t st[LK_N
 int device_statics(smg_device* d, const uint32_t** out) {
     std::lock_guard<std::mutex> lk(d->mu);
     if (!d->static_ready) {
-        const void* kinds[LK_N][2] = {
+        const void* kinds[LK_N][4] = {
             {(const void*)k_edges, (const void*)k_edges},
             {(const void*)k_cell_emit, (const void*)k_cell_emit},
-            {(const void*)k_uniform<false>, (const void*)k_uniform<true>},
+            {(const void*)k_uniform<false>, (const void*)k_uniform<true>, (const void*)k_classify_touched<false>,
+             (const void*)k_classify_touched<true>},
             {(const void*)k_sample<false, false>, (const void*)k_sample<false, true>},
             {(const void*)k_refine<false, false>, (const void*)k_refine<false, true>},
             {(const void*)k_refine_wide<false, false>, (const void*)k_refine_wide<false, true>},
@@ -1087,6 +1088,7 @@ int device_statics(smg_device* d, const uint32_t** out) {
         for (int k = 0; k < LK_N; k++) {
             uint32_t m = 0;
             for (const void* f : kinds[k]) {
+                if (!f) continue;
                 hipFuncAttributes a{};
                 if (hipFuncGetAttributes(&a, f) != hipSuccess)
                     return fail(SMG_E_CALL, "hipFuncGetAttributes failed for a mesh-stage kernel");
@@ -1482,9 +1484,43 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     k_popcount<<<stride_grid(E.words), 256, 0, s>>>(bits, wcount, &hdr->nwords);
     TRY(scan_u32(X, wcount, woff, &hdr->nwords, E.words, &cnt[K_TOUCHED]));
     k_bits_to_list<<<stride_grid(E.words), 256, 0, s>>>(bits, woff, &hdr->nwords, touched);
-    // Brick classification (lattice.rs:168-183), one wavefront per brick.
+    // Brick classification (lattice.rs:168-183). With shared gathers, one
+    // workgroup per touched brick also finds a surface brick's sub-block
+    // states (in touched order, stt), in the LDS tiers in turn, one
+    // wavefront per brick for those whose lists overflow them all;
+    // otherwise one wavefront per brick classifies it alone.
     TRY(rec(6));
-    {
+    const bool fused = tune.uniform_shared && LP.uniform_fits[0];
+    uint8_t* stt = nullptr;
+    if (fused) {
+        TRY(X.temp(&stt, 8 * (size_t)E.dense));
+        const uint32_t* list = nullptr;  // the first pass takes every touched brick
+        const uint32_t* count = &cnt[K_TOUCHED];
+        bool first = true;
+        for (int u = 0; u < UNIFORM_TIERS; u++) {
+            if (!LP.uniform_fits[u]) continue;
+            RLayout lay = make_rlayout(UNIFORM_CAPS[u]);
+            uint32_t* over;
+            TRY(X.temp(&over, E.dense));
+            Queue q{list, count, &cnt[K_Q_UNIFORM0 + u]};
+            if (first && fits_dispatch(E.dense, 256))
+                k_classify_touched<false><<<gridn(E.dense, 256), 256, lay.bytes, s>>>(
+                    mols, wofs_d, rows, touched, UNIFORM_CAPS[u], lay, q, keep, stt, over, &cnt[K_UOVER + u]);
+            else
+                k_classify_touched<true><<<g1(std::min<uint64_t>(E.dense, resident(d, (const void*)k_classify_touched<true>,
+                                                                                    256, lay.bytes))),
+                                           256, lay.bytes, s>>>(mols, wofs_d, rows, touched, UNIFORM_CAPS[u], lay, q,
+                                                                keep, stt, over, &cnt[K_UOVER + u]);
+            HIP_TRY(hipGetLastError());
+            first = false;
+            list = over;
+            count = &cnt[K_UOVER + u];
+        }
+        Queue rest{list, count, &cnt[K_Q_SUBST]};
+        uint32_t grid = g1(std::min<uint64_t>(E.dense, resident(d, (const void*)k_classify_touched_wave, WAVE, 0)));
+        k_classify_touched_wave<<<grid, WAVE, 0, s>>>(mols, wofs_d, rows, touched, rest, keep, stt);
+        HIP_TRY(hipGetLastError());
+    } else {
         Queue q{nullptr, &cnt[K_TOUCHED], nullptr};
         if (!fits_dispatch(E.dense, WAVE)) return fail(SMG_E_UNSUPPORTED, "brick boxes of the run too large for one dispatch");
         k_classify_bricks<<<gridn(E.dense, WAVE), WAVE, 0, s>>>(mols, wofs_d, rows, touched, q, keep);
@@ -1539,11 +1575,16 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
         k_fill_u32<<<stride_grid(nb), 256, 0, s>>>(cload, 0u, 0, &cnt[K_NB]);
         HIP_TRY(hipGetLastError());
     }
-    // Sub-block states (kernel.rs:467-482): one shared gather per brick in
-    // the LDS tiers in turn, one wavefront per brick for bricks whose lists
-    // overflow them all.
+    // Sub-block states (kernel.rs:467-482): from the touched-order states of
+    // the fused classification, or one shared gather per brick in the LDS
+    // tiers in turn, one wavefront per brick for bricks whose lists overflow
+    // them all.
     TRY(rec(8));
-    {
+    if (fused) {
+        k_gather_states<<<stride_grid(E.dense), 256, 0, s>>>(keep, koff, &cnt[K_TOUCHED], nb, stt, st);
+        HIP_TRY(hipGetLastError());
+        TRY(rec(9));
+    } else {
         const uint32_t* list = nullptr;  // the first pass takes every surface brick
         const uint32_t* count = &cnt[K_NB];
         bool first = true;

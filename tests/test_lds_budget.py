@@ -23,6 +23,9 @@ KINDS = ["k_edges", "k_cell_emit", "k_uniform", "k_sample", "k_refine", "k_refin
 LAUNCHED = {"k_uniform": ("ILb0E", "ILb1E"), "k_sample": ("ILb0ELb0E", "ILb0ELb1E"),
             "k_refine": ("ILb0ELb0E", "ILb0ELb1E"), "k_refine_wide": ("ILb0ELb0E", "ILb0ELb1E"),
             "k_refine_fast": ("ILb0ELb0E", "ILb0ELb1E"), "k_refine_fast_wide": ("ILb0ELb0E", "ILb0ELb1E")}
+# Other kernels that share a kind's LDS tiers (the touched-brick classification
+# runs the sub-block classification's tiers).
+SHARING = {"k_uniform": (("k_classify_touched", ("ILb0E", "ILb1E")),)}
 TOOLS = ["objcopy", "/opt/rocm/llvm/bin/clang-offload-bundler", "/opt/rocm/llvm/bin/llvm-readelf"]
 
 
@@ -52,6 +55,11 @@ def kernel_statics(tmp_path):
         got = [v for name, v in sizes.items() if name.startswith(prefix) and
                (kind not in LAUNCHED or any(name[len(prefix):].startswith(t) for t in LAUNCHED[kind]))]
         assert got, kind
+        for other, variants in SHARING.get(kind, ()):
+            pre = f"_ZN3smg{len(other)}{other}"
+            shared = [v for name, v in sizes.items() if any(name.startswith(pre + t) for t in variants)]
+            assert shared, other
+            got += shared
         out.append(max(got))
     return out
 
