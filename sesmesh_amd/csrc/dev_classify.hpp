@@ -354,12 +354,17 @@ __global__ void __launch_bounds__(64) k_classify_touched_wave(const Mol* __restr
 }
 
 // Sub-block states of a surface brick, gathered from the touched-brick
-// order: st[8 * off[w] + i] = stt[8 * w + i] for every kept w.
+// order: st[8 * off[w] + i] = stt[8 * w + i] for every kept w; likewise its
+// superset list (bt, touched order, into bs, surface order) when kept.
 __global__ void k_gather_states(const uint32_t* keep, const uint32_t* off, const uint32_t* n, uint32_t cap,
-                                const uint8_t* stt, uint8_t* st) {
+                                const uint8_t* stt, uint8_t* st, SupList bt, SupList bs) {
     for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < *n; w += gridDim.x * blockDim.x) {
         if (!keep[w] || off[w] >= cap) continue;
         for (int i = 0; i < 8; i++) st[8 * (size_t)off[w] + i] = stt[8 * (size_t)w + i];
+        if (bs.cnt) {
+            bs.off[off[w]] = bt.off[w];
+            bs.cnt[off[w]] = bt.cnt[w];
+        }
     }
 }
 

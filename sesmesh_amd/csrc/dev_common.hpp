@@ -393,7 +393,8 @@ struct SupList {
 struct HidList {
     uint32_t *off, *cnt, *ids, *used;
     uint32_t cap;
-    SupList sup;  // handed over the same way
+    SupList sup;   // handed over the same way
+    SupList bsup;  // per surface brick: the classification's superset of its sampling walks (see gather_refine)
 };
 constexpr uint32_t NO_LIST = 0xffffffffu;
 constexpr uint32_t KEY_BUCKETS = 16;  // key buckets per candidate class in the gathers

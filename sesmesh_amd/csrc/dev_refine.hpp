@@ -136,7 +136,7 @@ __device__ inline RLists rcarve(char* base, const RLayout& L) {
     return s;
 }
 
-enum { R_NEAR = 0, R_IN, R_S, R_C, R_V, R_CAND, R_E, R_OVER, R_N };
+enum { R_NEAR = 0, R_IN, R_S, R_C, R_V, R_CAND, R_E, R_OVER, R_X, R_XOVER, R_N };
 
 // gather_local (field/point.rs:496-565) of the ball (b, rho) by the
 // workgroup: near spheres with their thresholds, the inside group as
@@ -146,11 +146,15 @@ enum { R_NEAR = 0, R_IN, R_S, R_C, R_V, R_CAND, R_E, R_OVER, R_N };
 // With `hide`, candidates that one blocker hides from the whole ball are
 // left out: for callers that evaluate many points of the ball. The hidden
 // ones are taken from `hl` (hn entries, see HidList) when the sampling of
-// the same sub-block listed them, else found with mark_hidden.
+// the same sub-block listed them, else found with mark_hidden. With xs,
+// the walk also lists (SupList xs, entry xj) every item within reach_x of
+// the ball centre (spheres: d - R, or for the SES kind d < R + reach_x),
+// whatever the tests decide: a superset for walks of nearby balls.
 __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s, uint32_t* n, uint32_t* tmp4,
                                      double bx, double by, double bz, double rho, double band, bool hide,
                                      const uint32_t* hl = nullptr, uint32_t hn = NO_LIST,
-                                     const uint32_t* sl = nullptr, uint32_t sn = NO_LIST) {
+                                     const uint32_t* sl = nullptr, uint32_t sn = NO_LIST, SupList xs = SupList{},
+                                     uint32_t xj = 0, double reach_x = 0.0) {
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     __shared__ uint32_t bstart[WAVE], bfill[WAVE];  // per key bucket: 16 for each of S, C, V
     for (uint32_t i = tid; i < cap.hash; i += nt) s.hk[i] = 0u;
@@ -163,8 +167,18 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
     const bool sph = A.kind != K_SES;
     hide = hide && !sph;
     const double reach = (sph ? 0.0 : A.probe) + band + rho;
-    const double radii[3] = {reach + A.max_expanded, reach + A.rho_max, reach};
+    const double rw = xs.ids ? fmax(reach, reach_x) : reach;  // the walk's reach
+    const double radii[3] = {rw + A.max_expanded, rw + A.rho_max, rw};
     const double key0 = sph ? A.max_expanded : 0.0;
+    uint32_t* xstage = (uint32_t*)s.cdat;  // free until the walk has ended
+    const uint32_t xcap = xs.ids ? 2u * CDAT * cap.cand : 0u;
+    auto xadd = [&](uint32_t code) {
+        const uint32_t at = atomicAdd(&n[R_X], 1u);
+        if (at < xcap)
+            xstage[at] = code;
+        else
+            n[R_XOVER] = 1u;
+    };
     const double per_bucket = KEY_BUCKETS / (reach + key0);
     auto bucket = [&](uint32_t cls, double key) {
         return cls * KEY_BUCKETS + min(KEY_BUCKETS - 1u, (uint32_t)(fmax(key + key0, 0.0) * per_bucket));
@@ -176,6 +190,7 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
             if (A.status[k] == S_CONTAINED) return;
             const SphereRec S = load_rec(A.sph, k);
             const double v = ddist(bx, by, bz, S.x, S.y, S.z) - S.R;
+            if (xcap && v < reach_x) xadd(k);
             if (!(v < reach)) return;
             uint32_t at = atomicAdd(&n[R_NEAR], 1u);
             uint32_t q = atomicAdd(&n[R_CAND], 1u);
@@ -195,6 +210,7 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
             const SphereRec S = load_rec(A.sph, k);
             double R = S.R, cx = S.x, cy = S.y, cz = S.z;
             double d = ddist(bx, by, bz, cx, cy, cz);
+            if (xcap && d < R + reach_x) xadd(k);
             if (d < R + reach) {
                 uint32_t at = atomicAdd(&n[R_NEAR], 1u);
                 if (at >= cap.near) {
@@ -242,6 +258,7 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
             double wl = sqrt(wx * wx + wy * wy + wz * wz);
             double rr = C.rho;
             double dist = sqrt(along * along + (wl - rr) * (wl - rr));
+            if (xcap && dist <= reach_x) xadd(k | (1u << KIND_SHIFT));
             if (!(dist <= reach)) return;
             atomicAdd(&n[R_C], 1u);
             uint32_t q = atomicAdd(&n[R_CAND], 1u);
@@ -254,6 +271,7 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
         } else {
             const VertexRec V = load_rec(A.ver, k);
             double d = ddist(bx, by, bz, V.x, V.y, V.z);
+            if (xcap && d <= reach_x) xadd(k | (2u << KIND_SHIFT));
             if (!(d <= reach)) return;
             atomicAdd(&n[R_V], 1u);
             uint32_t q = atomicAdd(&n[R_CAND], 1u);
@@ -275,6 +293,24 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
         gather3(A, bx, by, bz, radii, tmp4, visit);
     }
     if (n[R_OVER]) return false;
+    if (xcap) {  // the superset to its pool (none if it overflowed)
+        __shared__ uint32_t x_at;
+        const uint32_t m = n[R_X];
+        if (tid == 0) {
+            uint32_t at = NO_LIST;
+            if (!n[R_XOVER]) {
+                at = atomicAdd(xs.used, m);
+                if (at > xs.cap || m > xs.cap - at) at = NO_LIST;
+            }
+            x_at = at;
+            xs.off[xj] = at == NO_LIST ? 0u : at;
+            xs.cnt[xj] = at == NO_LIST ? NO_LIST : m;
+        }
+        __syncthreads();
+        if (x_at != NO_LIST)
+            for (uint32_t i = tid; i < m; i += nt) xs.ids[x_at + i] = xstage[i];
+        __syncthreads();
+    }
     const uint32_t gathered = n[R_CAND];
     // Candidates one blocker hides from the whole ball leave the lists (their
     // key becomes all ones); the list order of the others is unchanged.

@@ -263,7 +263,8 @@ __device__ inline float sphere_value(const Lists& s, const uint32_t* n, float x,
 // sampling's own tests are unchanged.
 __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, const Caps& cap, Lists& s,
                                   uint32_t* n, uint32_t* tmp4, const int32_t* key, uint32_t sb,
-                                  const SupList& sup, uint32_t j) {
+                                  const SupList& sup, uint32_t j, const uint32_t* xl = nullptr,
+                                  uint32_t xn = NO_LIST) {
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     __shared__ uint32_t bfill[WAVE];
     for (uint32_t i = tid; i < cap.hash; i += nt) s.hk[i] = 0u;
@@ -298,7 +299,9 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
     const double radii[3] = {reach_r + A.max_expanded, reach_r + A.rho_max, reach_r};
     const double per_bucket = KEY_BUCKETS / reach;
     auto bucket = [&](double key) { return min(KEY_BUCKETS - 1u, (uint32_t)(key * per_bucket)); };
-    gather3(A, b[0], b[1], b[2], radii, tmp4, [&](uint32_t grid, uint32_t k) {
+    // The grids' items near the ball, or (xl) the brick classification's
+    // superset of them; the tests are the same either way.
+    auto visit = [&](uint32_t grid, uint32_t k) {
         if (grid == 0) {  // near spheres, the inside group and class S (gather_near, gather_ses)
             if (A.status[k] == S_CONTAINED) return;
             const SphereRec S = load_rec(A.sph, k);
@@ -371,7 +374,16 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
             }
             s.vd[q] = load_f4(Lc.v4, k);
         }
-    });
+    };
+    if (xl) {
+        for (uint32_t i = tid; i < xn; i += nt) {
+            const uint32_t e = xl[i];
+            visit(e >> KIND_SHIFT, e & ID_MASK);
+        }
+        __syncthreads();
+    } else {
+        gather3(A, b[0], b[1], b[2], radii, tmp4, visit);
+    }
     if (n[C_OVER]) return false;
     if (stage_cap) {  // the superset to the pool (none if it overflowed)
         __shared__ uint32_t sup_at;
@@ -503,7 +515,10 @@ __device__ __forceinline__ void sample_sub(const Mol* __restrict__ mols, const u
     Lists s = carve(base, lay);
     const int32_t* key = keys + 3 * brick;
     if (threadIdx.x == 0) n_owned = 0;
-    bool ok = gather_sub(A, Lc, P, cap, s, n, tmp4, key, sb, hid.sup, j);
+    const uint32_t* xl = nullptr;  // the brick's superset from its classification
+    uint32_t xn = hid.bsup.cnt ? hid.bsup.cnt[brick] : NO_LIST;
+    if (xn != NO_LIST) xl = hid.bsup.ids + hid.bsup.off[brick];
+    bool ok = gather_sub(A, Lc, P, cap, s, n, tmp4, key, sb, hid.sup, j, xl, xn);
     if (!ok) {
         if (threadIdx.x == 0) overflow[atomicAdd(n_overflow, 1u)] = j;
         return;

@@ -111,7 +111,7 @@ __device__ __forceinline__ void uniform_brick(const Mol* __restrict__ mols, cons
 __device__ __forceinline__ void classify_touched(const Mol* __restrict__ mols, const uint64_t* wofs, uint32_t nm,
                                                  const uint32_t* touched, const RCaps& cap, const RLayout& lay,
                                                  uint32_t w, uint32_t* keep, uint8_t* stt, uint32_t* overflow,
-                                                 uint32_t* n_overflow) {
+                                                 uint32_t* n_overflow, const SupList& bsup) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     __shared__ uint32_t n[R_N];
     __shared__ uint32_t tmp4[4];
@@ -133,7 +133,14 @@ __device__ __forceinline__ void classify_touched(const Mol* __restrict__ mols, c
     const double margin = rho + CLASSIFY_EPS;
     const double cutoff = A.probe + margin + CLASSIFY_EPS;
     RLists s = rcarve(smem, lay);
-    bool ok = gather_refine(A, cap, s, n, tmp4, C[0], C[1], C[2], off, margin + CLASSIFY_EPS, false);
+    // The walk also lists what the brick's eight sampling walks (and the
+    // refinement supersets they collect) can take: items within the
+    // refinement's reach of a sub-block centre, which lies within the
+    // grown offset `off` of C (the 1e-6 A margin covers the rounding of the
+    // centres and distances).
+    const double reach_r = (A.kind != K_SES ? 0.0 : A.probe) + P.band + off;
+    bool ok = gather_refine(A, cap, s, n, tmp4, C[0], C[1], C[2], off, margin + CLASSIFY_EPS, false, nullptr, NO_LIST,
+                            nullptr, NO_LIST, bsup, w, reach_r + off + 1e-6);
     if (!ok) {
         if (threadIdx.x == 0) overflow[atomicAdd(n_overflow, 1u)] = w;
         return;
@@ -235,10 +242,10 @@ template <bool PERSIST>
 __global__ void __launch_bounds__(256) k_classify_touched(const Mol* __restrict__ mols, const uint64_t* wofs,
                                                           uint32_t nm, const uint32_t* touched, RCaps cap, RLayout lay,
                                                           Queue q, uint32_t* keep, uint8_t* stt, uint32_t* overflow,
-                                                          uint32_t* n_overflow) {
+                                                          uint32_t* n_overflow, SupList bsup) {
     __shared__ uint32_t ticket;
     for_each_item<PERSIST>(q, &ticket, [&](uint32_t w) {
-        classify_touched(mols, wofs, nm, touched, cap, lay, w, keep, stt, overflow, n_overflow);
+        classify_touched(mols, wofs, nm, touched, cap, lay, w, keep, stt, overflow, n_overflow, bsup);
     });
 }
 

@@ -1002,6 +1002,8 @@ constexpr uint32_t LDS_EDGES = 512;      // crossing-edge buffer of the LDS pass
 constexpr uint32_t SCRATCH_CHUNK = 1024;  // default workgroups of a scratch pass (one list slice each)
 constexpr uint32_t HID_PER_SUB = 64;      // hidden-candidate pool entries per sampled sub-block
 constexpr uint32_t SUP_PER_SUB = 160;     // refinement-superset pool entries per sampled sub-block
+constexpr uint32_t BSUP_PER_BRICK = 320;  // brick-superset pool entries per touched brick
+constexpr uint64_t BSUP_MAX = 1ull << 28; // brick-superset pool entries at most
 constexpr uint32_t NO_CAP = 0xffffffffu;
 
 // Device counters of a run (u32 slots, zero at the start of the run).
@@ -1016,6 +1018,7 @@ enum : uint32_t {
     K_Q_SUBST, K_Q_SAMPLE0, K_Q_REFINE0 = K_Q_SAMPLE0 + TIERS,
     K_HID = K_Q_REFINE0 + TIERS,  // entries taken from the hidden-candidate pool
     K_SUP,                        // entries taken from the refinement-superset pool
+    K_BSUP,                       // entries taken from the brick-superset pool
     K_N
 };
 // Capacity of each tier in candidate-load units (max(S + C, 2 near, 4 SAS
@@ -1492,8 +1495,21 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     TRY(rec(6));
     const bool fused = tune.uniform_shared && LP.uniform_fits[0];
     uint8_t* stt = nullptr;
+    SupList bsup_t{};  // per touched brick: the superset of its sampling walks (not in allocation-free runs)
     if (fused) {
         TRY(X.temp(&stt, 8 * (size_t)E.dense));
+        if (!X.arena) {
+            uint32_t ntc;
+            TRY(X.size(&cnt[K_TOUCHED], (uint32_t)E.dense, ntc));
+            TRY(X.temp(&bsup_t.off, E.dense));
+            TRY(X.temp(&bsup_t.cnt, E.dense));
+            const uint64_t bcap = std::min<uint64_t>((uint64_t)BSUP_PER_BRICK * std::max<uint32_t>(ntc, 1u), BSUP_MAX);
+            TRY(X.temp(&bsup_t.ids, bcap));
+            bsup_t.used = &cnt[K_BSUP];
+            bsup_t.cap = (uint32_t)bcap;
+            k_fill_u32<<<stride_grid(E.dense), 256, 0, s>>>(bsup_t.cnt, NO_LIST, 0, &cnt[K_TOUCHED]);
+            HIP_TRY(hipGetLastError());
+        }
         const uint32_t* list = nullptr;  // the first pass takes every touched brick
         const uint32_t* count = &cnt[K_TOUCHED];
         bool first = true;
@@ -1505,12 +1521,12 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
             Queue q{list, count, &cnt[K_Q_UNIFORM0 + u]};
             if (first && fits_dispatch(E.dense, 256))
                 k_classify_touched<false><<<gridn(E.dense, 256), 256, lay.bytes, s>>>(
-                    mols, wofs_d, rows, touched, UNIFORM_CAPS[u], lay, q, keep, stt, over, &cnt[K_UOVER + u]);
+                    mols, wofs_d, rows, touched, UNIFORM_CAPS[u], lay, q, keep, stt, over, &cnt[K_UOVER + u], bsup_t);
             else
                 k_classify_touched<true><<<g1(std::min<uint64_t>(E.dense, resident(d, (const void*)k_classify_touched<true>,
                                                                                     256, lay.bytes))),
                                            256, lay.bytes, s>>>(mols, wofs_d, rows, touched, UNIFORM_CAPS[u], lay, q,
-                                                                keep, stt, over, &cnt[K_UOVER + u]);
+                                                                keep, stt, over, &cnt[K_UOVER + u], bsup_t);
             HIP_TRY(hipGetLastError());
             first = false;
             list = over;
@@ -1580,8 +1596,15 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     // tiers in turn, one wavefront per brick for bricks whose lists overflow
     // them all.
     TRY(rec(8));
+    SupList bsup_s{};  // the brick supersets in surface-brick order, for the sampling
     if (fused) {
-        k_gather_states<<<stride_grid(E.dense), 256, 0, s>>>(keep, koff, &cnt[K_TOUCHED], nb, stt, st);
+        if (bsup_t.ids) {
+            bsup_s = bsup_t;
+            TRY(X.temp(&bsup_s.off, nb));
+            TRY(X.temp(&bsup_s.cnt, nb));
+            k_fill_u32<<<stride_grid(nb), 256, 0, s>>>(bsup_s.cnt, NO_LIST, 0, &cnt[K_NB]);
+        }
+        k_gather_states<<<stride_grid(E.dense), 256, 0, s>>>(keep, koff, &cnt[K_TOUCHED], nb, stt, st, bsup_t, bsup_s);
         HIP_TRY(hipGetLastError());
         TRY(rec(9));
     } else {
@@ -1641,6 +1664,7 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     // refinement (HidList): one pool of HID_PER_SUB entries per sampled
     // sub-block; a sub-block that finds it full gets no list.
     HidList hid{};
+    hid.bsup = bsup_s;
     TRY(X.temp(&hid.off, nsub));
     TRY(X.temp(&hid.cnt, nsub));
     const uint64_t hcap = std::min<uint64_t>((uint64_t)HID_PER_SUB * std::max<uint32_t>(nsw, 1u), 0xfffffff0ull);
