@@ -239,7 +239,7 @@ __device__ __forceinline__ void classify_touched(const Mol* __restrict__ mols, c
 }
 
 template <bool PERSIST>
-__global__ void __launch_bounds__(256) k_classify_touched(const Mol* __restrict__ mols, const uint64_t* wofs,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_classify_touched(const Mol* __restrict__ mols, const uint64_t* wofs,
                                                           uint32_t nm, const uint32_t* touched, RCaps cap, RLayout lay,
                                                           Queue q, uint32_t* keep, uint8_t* stt, uint32_t* overflow,
                                                           uint32_t* n_overflow, SupList bsup) {

@@ -1493,14 +1493,19 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     // wavefront per brick for those whose lists overflow them all;
     // otherwise one wavefront per brick classifies it alone.
     TRY(rec(6));
-    const bool fused = tune.uniform_shared && LP.uniform_fits[0];
+    // Batches of many molecules keep the two passes: their molecules are
+    // small, with a larger share of touched bricks that are not surface
+    // bricks (a full gather each in the fused pass), and their planned runs
+    // would launch a workgroup per brick-box cell (measured: 4096 molecules,
+    // 0.26-0.56 s fused against 0.24 s).
+    const bool fused = tune.uniform_shared && LP.uniform_fits[0] && rows == 1;
     uint8_t* stt = nullptr;
     SupList bsup_t{};  // per touched brick: the superset of its sampling walks (not in allocation-free runs)
     if (fused) {
         TRY(X.temp(&stt, 8 * (size_t)E.dense));
+        uint32_t ntc;  // touched bricks (their capacity in a planned run)
+        TRY(X.size(&cnt[K_TOUCHED], (uint32_t)E.dense, ntc));
         if (!X.arena) {
-            uint32_t ntc;
-            TRY(X.size(&cnt[K_TOUCHED], (uint32_t)E.dense, ntc));
             TRY(X.temp(&bsup_t.off, E.dense));
             TRY(X.temp(&bsup_t.cnt, E.dense));
             const uint64_t bcap = std::min<uint64_t>((uint64_t)BSUP_PER_BRICK * std::max<uint32_t>(ntc, 1u), BSUP_MAX);
@@ -1519,8 +1524,14 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
             uint32_t* over;
             TRY(X.temp(&over, E.dense));
             Queue q{list, count, &cnt[K_Q_UNIFORM0 + u]};
-            if (first && fits_dispatch(E.dense, 256))
-                k_classify_touched<false><<<gridn(E.dense, 256), 256, lay.bytes, s>>>(
+            // The first pass takes one workgroup per touched brick: in an exact
+            // run a grid of their count; in a planned run, whose count is known
+            // only on the device, a grid of the brick boxes' cells (few enough
+            // here that the idle workgroups cost little). The other passes
+            // loop over their queues.
+            const uint64_t first_grid = X.planned() ? E.dense : ntc;
+            if (first && fits_dispatch(first_grid, 256))
+                k_classify_touched<false><<<gridn(first_grid, 256), 256, lay.bytes, s>>>(
                     mols, wofs_d, rows, touched, UNIFORM_CAPS[u], lay, q, keep, stt, over, &cnt[K_UOVER + u], bsup_t);
             else
                 k_classify_touched<true><<<g1(std::min<uint64_t>(E.dense, resident(d, (const void*)k_classify_touched<true>,
