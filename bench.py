@@ -117,12 +117,12 @@ def measured_traffic(kernel):
 
 
 def hbm_rates(ph):
-    """Achieved HBM GB/s of the classification and gather kernels: calibrated
-    bytes per launch (committed counter profile) over this run's kernel
-    time, against the 8 TB/s peak."""
+    """Achieved HBM GB/s of the classification, sampling and refinement
+    kernels: calibrated bytes per launch (committed counter profile) over
+    this run's kernel time, against the 8 TB/s peak."""
     out = {}
-    for kernel, key in (("k_classify_bricks", "k_classify_ms"), ("k_uniform<false>", "k_substates_ms"),
-                        ("k_sample<false, false>", "k_sample_ms"), ("k_refine<false, false>", "k_refine_ms")):
+    for kernel, key in (("k_classify_touched<false>", "k_classify_ms"), ("k_sample<false, false>", "k_sample_ms"),
+                        ("k_refine<false, false>", "k_refine_ms")):
         b, src = measured_traffic(kernel)
         if b is None or not ph.get(key):
             continue
