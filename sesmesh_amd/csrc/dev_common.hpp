@@ -404,25 +404,27 @@ constexpr uint32_t KEY_BUCKETS = 16;  // key buckets per candidate class in the 
 // `slot` of row p is set for each listed sphere the near hash holds at
 // `slot`. Row p's bits are words bits[w * cc + p]. 8 lanes per row, each
 // loading 4 of its items before it probes any, so 4 loads are in flight per
-// lane. keep(p, slot) may leave out a blocker that can never block row p's
-// candidate where it is evaluated. By the whole workgroup; the caller
-// synchronises afterwards.
-template <class Keep>
+// lane. keep(r, p, slot) may leave out a blocker that can never block row
+// p's candidate where it is evaluated; r = prep(p) is computed once per row
+// by each of its lanes. By the whole workgroup; the caller synchronises
+// afterwards.
+template <class Prep, class Keep>
 __device__ inline void blocker_bits(gptr<const uint32_t> nb_items, gptr<const uint32_t> cb_items, uint32_t ns,
                                     uint32_t rows, const uint32_t* off, const uint32_t* len, const uint32_t* hk,
                                     const uint32_t* hv, uint32_t hmask, unsigned long long* bits, uint32_t cc,
-                                    Keep&& keep) {
+                                    Prep&& prep, Keep&& keep) {
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     for (uint32_t p = tid / 8; p < rows; p += nt / 8) {
         gptr<const uint32_t> row = (p < ns ? nb_items : cb_items) + off[p];
         const uint32_t n = len[p];
+        const auto r = prep(p);
         for (uint32_t q0 = tid % 8; q0 < n; q0 += 32) {
             uint32_t it[4];
             for (int u = 0; u < 4; u++) it[u] = q0 + 8 * u < n ? row[q0 + 8 * u] : 0xffffffffu;
             for (int u = 0; u < 4; u++) {
                 if (it[u] == 0xffffffffu) continue;
                 const uint32_t slot = hash_get(hk, hv, hmask, it[u]);
-                if (slot != 0xffffffffu && keep(p, slot)) atomicOr(&bits[(slot >> 6) * cc + p], 1ull << (slot & 63));
+                if (slot != 0xffffffffu && keep(r, p, slot)) atomicOr(&bits[(slot >> 6) * cc + p], 1ull << (slot & 63));
             }
         }
     }
@@ -511,37 +513,90 @@ __device__ inline bool arc_covered(const double b[3], double rb, const dvec4& a,
 // - alpha), with cos(theta - alpha) d = (v.e sqrt(h2) + |v x e| rb) / D2.
 // For the arc likewise with the in-plane angle, the whole circle (h2 <= 0)
 // reaching to r^2 + |ck - m|^2 - 2 r L.
-__device__ inline bool cap_reaches(const double b[3], double rb, const dvec4& c, const dvec4& k, double t) {
-    const double vx = b[0] - c.x, vy = b[1] - c.y, vz = b[2] - c.z;
-    const double D2 = vx * vx + vy * vy + vz * vz, h2 = D2 - rb * rb;
-    const double ex = k.x - c.x, ey = k.y - c.y, ez = k.z - c.z;
-    const double d2 = ex * ex + ey * ey + ez * ez, dot = vx * ex + vy * ey + vz * ez;
-    const double R = c.w;
-    if (!(h2 > 0.0) || (dot >= 0.0 && dot * dot >= d2 * h2)) {  // nearest point |d - R| away
-        const double lo = R - t, hi = R + t;
-        return d2 < hi * hi && !(lo > 0.0 && d2 <= lo * lo);
-    }
-    const double cx = vy * ez - vz * ey, cy = vz * ex - vx * ez, cz = vx * ey - vy * ex;
-    return (R * R + d2 - t * t) * D2 < 2.0 * R * (dot * sqrt(h2) + sqrt(cx * cx + cy * cy + cz * cz) * rb);
+//
+// Each test is split in two: what depends only on the ball and the
+// candidate (its row of the bitsets) is computed once per row, with the
+// operations and order of the whole test, so every decision is unchanged.
+// T is double, or float for callers whose inputs are f32 and whose limits
+// carry a slack far above the f32 rounding of the test.
+template <class T>
+struct ReachRow {
+    T ox, oy, oz;  // sphere centre c, or circle centre m
+    T ux, uy, uz;  // circle axis
+    T vx, vy, vz;  // b - c, or w (b - m in the circle's plane)
+    T R;           // sphere radius, or circle radius r
+    T D2, h2;      // |v|^2 (W2 for the arc) and that less rb^2
+    T sh2;         // sqrt(h2) where h2 > 0
+    T rb;
+};
+
+// Cap row: sphere (cx, cy, cz) of radius R.
+template <class T>
+__device__ inline ReachRow<T> cap_row(const T b[3], T rb, T cx, T cy, T cz, T R) {
+    ReachRow<T> w;
+    w.ox = cx;
+    w.oy = cy;
+    w.oz = cz;
+    w.ux = w.uy = w.uz = T(0);
+    w.vx = b[0] - cx;
+    w.vy = b[1] - cy;
+    w.vz = b[2] - cz;
+    w.R = R;
+    w.D2 = w.vx * w.vx + w.vy * w.vy + w.vz * w.vz;
+    w.h2 = w.D2 - rb * rb;
+    w.sh2 = w.h2 > T(0) ? sqrt(w.h2) : T(0);
+    w.rb = rb;
+    return w;
 }
 
-// Circle record: a = (mx, my, mz, ax), e = (ay, az, rho, -).
-__device__ inline bool arc_reaches(const double b[3], double rb, const dvec4& a, const dvec4& e, const dvec4& k,
-                                   double t) {
-    const double ux = a.w, uy = e.x, uz = e.y, r = e.z;
-    const double yx = b[0] - a.x, yy = b[1] - a.y, yz = b[2] - a.z;
-    const double al = yx * ux + yy * uy + yz * uz;
-    const double wx = yx - ux * al, wy = yy - uy * al, wz = yz - uz * al;
-    const double W2 = wx * wx + wy * wy + wz * wz, h2 = W2 - rb * rb;
-    const double fx = k.x - a.x, fy = k.y - a.y, fz = k.z - a.z;
-    const double fa = fx * ux + fy * uy + fz * uz;
-    const double px = fx - ux * fa, py = fy - uy * fa, pz = fz - uz * fa;
-    const double L2 = px * px + py * py + pz * pz, wp = wx * px + wy * py + wz * pz;
-    const double A0 = r * r + (fx * fx + fy * fy + fz * fz) - t * t;
-    if (!(h2 > 0.0) || (wp >= 0.0 && wp * wp >= L2 * h2))  // the arc holds the nearest circle point
-        return A0 < 0.0 || A0 * A0 < 4.0 * r * r * L2;
-    const double cx = wy * pz - wz * py, cy = wz * px - wx * pz, cz = wx * py - wy * px;
-    return A0 * W2 < 2.0 * r * (wp * sqrt(h2) + sqrt(cx * cx + cy * cy + cz * cz) * rb);
+template <class T>
+__device__ inline bool cap_reaches(const ReachRow<T>& w, T kx, T ky, T kz, T t) {
+    const T ex = kx - w.ox, ey = ky - w.oy, ez = kz - w.oz;
+    const T d2 = ex * ex + ey * ey + ez * ez, dot = w.vx * ex + w.vy * ey + w.vz * ez;
+    const T R = w.R;
+    if (!(w.h2 > T(0)) || (dot >= T(0) && dot * dot >= d2 * w.h2)) {  // nearest point |d - R| away
+        const T lo = R - t, hi = R + t;
+        return d2 < hi * hi && !(lo > T(0) && d2 <= lo * lo);
+    }
+    const T cx = w.vy * ez - w.vz * ey, cy = w.vz * ex - w.vx * ez, cz = w.vx * ey - w.vy * ex;
+    return (R * R + d2 - t * t) * w.D2 < T(2) * R * (dot * w.sh2 + sqrt(cx * cx + cy * cy + cz * cz) * w.rb);
+}
+
+// Arc row: circle centre (mx, my, mz), unit axis (ax, ay, az), radius r.
+template <class T>
+__device__ inline ReachRow<T> arc_row(const T b[3], T rb, T mx, T my, T mz, T ax, T ay, T az, T r) {
+    ReachRow<T> w;
+    w.ox = mx;
+    w.oy = my;
+    w.oz = mz;
+    w.ux = ax;
+    w.uy = ay;
+    w.uz = az;
+    w.R = r;
+    const T yx = b[0] - mx, yy = b[1] - my, yz = b[2] - mz;
+    const T al = yx * w.ux + yy * w.uy + yz * w.uz;
+    w.vx = yx - w.ux * al;
+    w.vy = yy - w.uy * al;
+    w.vz = yz - w.uz * al;
+    w.D2 = w.vx * w.vx + w.vy * w.vy + w.vz * w.vz;
+    w.h2 = w.D2 - rb * rb;
+    w.sh2 = w.h2 > T(0) ? sqrt(w.h2) : T(0);
+    w.rb = rb;
+    return w;
+}
+
+template <class T>
+__device__ inline bool arc_reaches(const ReachRow<T>& w, T kx, T ky, T kz, T t) {
+    const T r = w.R;
+    const T fx = kx - w.ox, fy = ky - w.oy, fz = kz - w.oz;
+    const T fa = fx * w.ux + fy * w.uy + fz * w.uz;
+    const T px = fx - w.ux * fa, py = fy - w.uy * fa, pz = fz - w.uz * fa;
+    const T L2 = px * px + py * py + pz * pz, wp = w.vx * px + w.vy * py + w.vz * pz;
+    const T A0 = r * r + (fx * fx + fy * fy + fz * fz) - t * t;
+    if (!(w.h2 > T(0)) || (wp >= T(0) && wp * wp >= L2 * w.h2))  // the arc holds the nearest circle point
+        return A0 < T(0) || A0 * A0 < T(4) * r * r * L2;
+    const T cx = w.vy * pz - w.vz * py, cy = w.vz * px - w.vx * pz, cz = w.vx * py - w.vy * px;
+    return A0 * w.D2 < T(2) * r * (wp * w.sh2 + sqrt(cx * cx + cy * cy + cz * cz) * w.rb);
 }
 
 // Marks the candidates q < total hidden everywhere in the ball (b, rb). A
@@ -558,13 +613,15 @@ __device__ inline bool arc_reaches(const double b[3], double rb, const dvec4& a,
 // the square of its blocking radius; mark(q) is called once per hidden
 // candidate, by one lane. A covering blocker holds the projection of b
 // itself, the cap's (arc's) central point, strictly within its blocking
-// radius: that test, without a square root, comes first. 4 lanes per
+// radius: that test, without a square root, comes first, as pre(pp, slot)
+// with pp = prep(x, y, z) of the projection. It may pass more blockers than
+// the exact test (their cover tests then fail), never fewer. 4 lanes per
 // candidate. By the whole workgroup; the caller synchronises afterwards.
 constexpr uint32_t HIDE_STEPS = 8;  // prefilter steps per round of mark_hidden
 
-template <class Kind, class Id, class Geom, class Blocker, class Mark>
+template <class Kind, class Id, class Geom, class Blocker, class Prep, class Pre, class Mark>
 __device__ inline void mark_hidden(const double b[3], double rb, uint32_t total, uint32_t n_near, Kind&& kind,
-                                   Id&& id, Geom&& geom, Blocker&& blocker, Mark&& mark) {
+                                   Id&& id, Geom&& geom, Blocker&& blocker, Prep&& prep, Pre&& pre, Mark&& mark) {
     const uint32_t tid = threadIdx.x, nt = blockDim.x, lane = lane_id();
     for (uint32_t q = tid / 4; q < total; q += nt / 4) {
         const uint32_t kd = kind(q);
@@ -603,13 +660,12 @@ __device__ inline void mark_hidden(const double b[3], double rb, uint32_t total,
         // blockers pass often (coarse lattices). A candidate's four lanes
         // stop once one of them has found a cover.
         uint32_t k = tid % 4;
+        const auto pp = prep(px, py, pz);
         while (__ballot(live && k < n_near) != 0ull) {
             bool pass = false;
             if (live) {
                 for (uint32_t st = 0; st < HIDE_STEPS && k < n_near; st++) {
-                    const dvec4 kt = blocker(k);
-                    const double ex = kt.x - px, ey = kt.y - py, ez = kt.z - pz;
-                    if (ex * ex + ey * ey + ez * ez < kt.w) {
+                    if (pre(pp, k)) {
                         pass = true;
                         break;
                     }

@@ -65,7 +65,7 @@ struct RCaps {
 
 // LDS / scratch layout of the refinement lists (byte offsets).
 struct RLayout {
-    uint32_t nq, nin, nR, nr, hk, hv, ins, ckey, cid, cdat, cbits, coff, cpre, ebuf, bytes;
+    uint32_t nq, nin, nR, nr, nt, hk, hv, ins, ckey, cid, cdat, cbits, coff, cpre, ebuf, bytes;
 };
 
 // Near sphere record: centre and the blocking threshold of R - EPS.
@@ -87,6 +87,7 @@ __host__ __device__ inline RLayout make_rlayout(const RCaps& c) {
     L.nin = take(8 * c.near);  // inside threshold of R
     L.nR = take(c.radii ? 8 * c.near : 0);
     L.nr = take(c.radii ? 8 * c.near : 0);
+    L.nt = take(c.radii ? 0 : 8 * c.near);  // reach limit of the near spheres as blockers
     L.hk = take(4 * c.hash);
     L.hv = take(4 * c.hash);
     L.ins = take(2 * c.inside);
@@ -105,6 +106,7 @@ struct RLists {
     NearD* nq;
     double* nin;
     double *nR, *nr;  // radii of near spheres (when gathered)
+    double* nt;       // sqrt(blk) + COVER_MARGIN of near spheres (refinement lists)
     uint32_t *hk, *hv;
     uint16_t* ins;
     unsigned long long* ckey;
@@ -123,6 +125,7 @@ __device__ inline RLists rcarve(char* base, const RLayout& L) {
     s.nin = (double*)(base + L.nin);
     s.nR = (double*)(base + L.nR);
     s.nr = (double*)(base + L.nr);
+    s.nt = (double*)(base + L.nt);
     s.hk = (uint32_t*)(base + L.hk);
     s.hv = (uint32_t*)(base + L.hv);
     s.ins = (uint16_t*)(base + L.ins);
@@ -225,6 +228,7 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
                 q.blk = t.blk;
                 s.nq[at] = q;
                 s.nin[at] = t.nin;
+                if (hide) s.nt[at] = sqrt(t.blk) + COVER_MARGIN;
                 if (cap.radii) {
                     s.nR[at] = R;
                     s.nr[at] = A.r[k];
@@ -348,6 +352,12 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
                 const NearD k = s.nq[slot];
                 return dvec4{k.x, k.y, k.z, k.blk};
             },
+            [](double x, double y, double z) { return dvec4{x, y, z, 0.0}; },
+            [&](const dvec4& p, uint32_t slot) {
+                const NearD k = s.nq[slot];
+                const double ex = k.x - p.x, ey = k.y - p.y, ez = k.z - p.z;
+                return ex * ex + ey * ey + ez * ez < k.blk;
+            },
             [&](uint32_t q) { s.ckey[q] = HIDDEN_KEY; });
     }
     __syncthreads();
@@ -437,16 +447,21 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
     // With `hide`, a blocker that no projection of the ball comes near stays
     // out of the bitset: it could never block there.
     const double bc[3] = {bx, by, bz};
-    blocker_bits(A.nb_items, A.cb_items, ns, ns + nc, s.coff, s.cpre, s.hk, s.hv, hmask, s.cbits, cc,
-                 [&](uint32_t p, uint32_t slot) {
-                     if (!hide) return true;
-                     const NearD k = s.nq[slot];
-                     const dvec4 kc = dvec4{k.x, k.y, k.z, 0.0};
-                     const double t = sqrt(k.blk) + COVER_MARGIN, rb = rho + COVER_PAD;
-                     const double* d = s.cdat + (size_t)CDAT * p;
-                     if (p < ns) return cap_reaches(bc, rb, dvec4{d[0], d[1], d[2], d[3]}, kc, t);
-                     return arc_reaches(bc, rb, dvec4{d[0], d[1], d[2], d[3]}, dvec4{d[4], d[5], d[6], 0.0}, kc, t);
-                 });
+    const double rb = rho + COVER_PAD;
+    blocker_bits(
+        A.nb_items, A.cb_items, ns, ns + nc, s.coff, s.cpre, s.hk, s.hv, hmask, s.cbits, cc,
+        [&](uint32_t p) {
+            const double* d = s.cdat + (size_t)CDAT * p;
+            if (!hide) return ReachRow<double>{};
+            if (p < ns) return cap_row(bc, rb, d[0], d[1], d[2], d[3]);
+            return arc_row(bc, rb, d[0], d[1], d[2], d[3], d[4], d[5], d[6]);
+        },
+        [&](const ReachRow<double>& w, uint32_t p, uint32_t slot) {
+            if (!hide) return true;
+            const NearD k = s.nq[slot];
+            const double t = s.nt[slot];
+            return p < ns ? cap_reaches(w, k.x, k.y, k.z, t) : arc_reaches(w, k.x, k.y, k.z, t);
+        });
     __syncthreads();
     return true;
 }

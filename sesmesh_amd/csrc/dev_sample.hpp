@@ -428,6 +428,17 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
             const float4 k = s.nd[slot];
             return dvec4{k.x, k.y, k.z, k.w};
         },
+        // The prefilter in f32: the projection rounded to f32 moves by at
+        // most 1.1e-4 (coordinates below 2048), which moves the squared
+        // distance of a blocker of radius below 9 by less than 2e-3, and
+        // the f32 arithmetic adds a few ulps; the limit is widened by 2e-3
+        // plus 1e-4 of itself, so every blocker the f64 test passes passes.
+        [](double x, double y, double z) { return make_float4((float)x, (float)y, (float)z, 0.0f); },
+        [&](const float4& p, uint32_t slot) {
+            const float4 k = s.nd[slot];
+            const float ex = k.x - p.x, ey = k.y - p.y, ez = k.z - p.z;
+            return ex * ex + ey * ey + ez * ez < k.w + (2e-3f + 1e-4f * k.w);
+        },
         [&](uint32_t q) { s.cbk[q] |= HIDDEN; });
     __syncthreads();
     for (uint32_t q = tid; q < gathered; q += nt)
@@ -475,12 +486,32 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
         for (uint32_t w = 0; w < nw; w++) s.cbits[w * cc + p] = 0ull;
     }
     __syncthreads();
-    // Blocker bitsets of the S and C rows (cpre holds the row length).
-    // Every near row member stays in the bitset: leaving out the blockers
-    // that cannot reach the ball's projections (as the refinement does)
-    // costs the sampling more in f64 tests than its f32 blocker tests save.
-    blocker_bits(A.nb_items, A.cb_items, ns, total, s.coff, s.cpre, s.hk, s.hv, hmask, s.cbits, cc,
-                 [](uint32_t, uint32_t) { return true; });
+    // Blocker bitsets of the S and C rows (cpre holds the row length). A
+    // blocker that no projection of the ball comes near stays out of the
+    // bitset: it could never block a sample of the sub-block. The test runs
+    // in f32 on the f32 frame-local records the lane kernel uses. Its limits
+    // carry COVER_MARGIN, which holds the f32 rounding of the samples and of
+    // their projections, and REACH_SLACK on both the ball and the reach,
+    // which holds the f32 rounding of the test itself (the ball centre to
+    // f32: at most 1.2e-4 per coordinate; the test's own arithmetic on
+    // lengths of a few A: about 1e-5): a blocker the test leaves out cannot
+    // come within its blocking radius of any projection.
+    constexpr float REACH_SLACK = 1e-3f;
+    const float bf[3] = {(float)bl[0], (float)bl[1], (float)bl[2]};
+    const float rbf = (float)(rho + COVER_PAD) + REACH_SLACK;
+    blocker_bits(
+        A.nb_items, A.cb_items, ns, total, s.coff, s.cpre, s.hk, s.hv, hmask, s.cbits, cc,
+        [&](uint32_t p) {
+            const float4 c = s.cdat[3 * p];
+            if (p < ns) return cap_row(bf, rbf, c.x, c.y, c.z, c.w);
+            const float4 u = s.cdat[3 * p + 1];
+            return arc_row(bf, rbf, c.x, c.y, c.z, u.x, u.y, u.z, c.w);
+        },
+        [&](const ReachRow<float>& w, uint32_t p, uint32_t slot) {
+            const float4 k = s.nd[slot];
+            const float t = sqrtf(k.w) + ((float)COVER_MARGIN + REACH_SLACK);
+            return p < ns ? cap_reaches(w, k.x, k.y, k.z, t) : arc_reaches(w, k.x, k.y, k.z, t);
+        });
     __syncthreads();
     return true;
 }
