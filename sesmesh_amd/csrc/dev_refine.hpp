@@ -501,14 +501,19 @@ __device__ inline bool precedes(double d, uint32_t p, double best, uint32_t bo) 
 // margin is far above the rounding of the squares and roots (about 1e-15
 // A here), so the distance computed in full would exceed `best` too: the
 // candidate could not be taken either way.
+//
+// Where the distance is finite, `pv` keeps what project() needs of it:
+// class S the offset x - c and its length, class C the in-plane part w of
+// x - m and |w|, computed with project()'s operations.
 template <int K>
-__device__ inline double cand_dist(const double* d, double x, double y, double z, double best) {
+__device__ inline double cand_dist(const double* d, double x, double y, double z, double best, dvec4& pv) {
     if (K == 0) {
         double vx = x - d[0], vy = y - d[1], vz = z - d[2];
         const double s = vx * vx + vy * vy + vz * vz, R = d[3];
         const double hi = R + best + 1e-9, lo = R - best - 1e-9;
         if (s >= hi * hi || (lo > 0.0 && s <= lo * lo)) return INFINITY;
         double len = sqrt(s);
+        pv = dvec4{vx, vy, vz, len};
         return len < 1e-12 ? INFINITY : fabs(len - R);
     }
     const double b2 = (best + 1e-9) * (best + 1e-9);
@@ -519,6 +524,7 @@ __device__ inline double cand_dist(const double* d, double x, double y, double z
         if (along * along >= b2) return INFINITY;  // the sum below is at least along^2
         double wx = yx - ax * along, wy = yy - ay * along, wz = yz - az * along;
         double wl = sqrt(wx * wx + wy * wy + wz * wz);
+        pv = dvec4{wx, wy, wz, wl};
         const double D = along * along + (wl - rho) * (wl - rho);
         return D >= b2 ? INFINITY : sqrt(D);
     }
@@ -566,6 +572,37 @@ __device__ inline void project(const DArr& A, const RLists& s, uint32_t ns, uint
     }
 }
 
+// project() of candidate p of class K from cand_dist's `pv` at the same
+// point: the same operations on the same values, so the same bits.
+template <int K>
+__device__ inline void project_kept(const DArr& A, const RLists& s, uint32_t p, const dvec4& pv, double& qx,
+                                    double& qy, double& qz) {
+    const double* d = s.cdat + (size_t)CDAT * p;
+    if (K == 0) {
+        double cx = d[0], cy = d[1], cz = d[2], R = d[3];
+        double sc = R / pv.w;
+        qx = cx + pv.x * sc;
+        qy = cy + pv.y * sc;
+        qz = cz + pv.z * sc;
+    } else {
+        double mx = d[0], my = d[1], mz = d[2], rho = d[6];
+        double dx, dy, dz;
+        if (pv.w > 1e-12) {
+            dx = pv.x / pv.w;
+            dy = pv.y / pv.w;
+            dz = pv.z / pv.w;
+        } else {
+            uint32_t c = s.cid[p] & ID_MASK;
+            dx = A.e1x[c];
+            dy = A.e1y[c];
+            dz = A.e1z[c];
+        }
+        qx = mx + dx * rho;
+        qy = my + dy * rho;
+        qz = mz + dz * rho;
+    }
+}
+
 // Lane group of a workgroup: g lanes (a power of two, at most 64) starting
 // at lane `shift` of the wavefront; r is the lane's rank in the group.
 struct Group {
@@ -600,19 +637,20 @@ __device__ inline void scan_class(const DArr& A, const RLists& s, uint32_t cc, u
     for (uint32_t p0 = a + G.r; p0 < e; p0 += W * G.g) {
         double lim = best + delta;
         double dist[W];
+        dvec4 pv[W];
         bool stop = false;
         for (int j = 0; j < W; j++) {
             uint32_t p = p0 + j * G.g;
             bool use = p < e && !stop && !(key_value(s.ckey[p]) > lim);
             stop |= !use;
-            dist[j] = use ? cand_dist<K>(s.cdat + (size_t)CDAT * p, x, y, z, best) : INFINITY;
+            dist[j] = use ? cand_dist<K>(s.cdat + (size_t)CDAT * p, x, y, z, best, pv[j]) : INFINITY;
         }
         for (int j = 0; j < W; j++) {
             uint32_t p = p0 + j * G.g;
             if (!(dist[j] < cutoff) || !precedes(dist[j], p, best, bo)) continue;
             if (K != 2) {
                 double qx, qy, qz;
-                project(A, s, ns, nc, p, x, y, z, qx, qy, qz);
+                project_kept<K>(A, s, p, pv[j], qx, qy, qz);
                 if (blocked(s, cc, nw, p, qx, qy, qz)) continue;
             }
             best = dist[j];
@@ -812,8 +850,13 @@ __device__ inline bool edge_step(EdgeState& E, const EvalD& e, double h, bool sp
             }
             double len = sqrt(wx * wx + wy * wy + wz * wz);
             if (len > 1e-12) {
-                double nx = wx / len, ny = wy / len, nz = wz / len;
-                slope = nx * Q.d[0] + ny * Q.d[1] + nz * Q.d[2];
+                // n . d with d zero off the edge's axis: the other two
+                // products are +-0 and leave the sum's bits, except the sign
+                // of a zero slope, which never takes the Newton step below.
+                const uint32_t axis = E.id % 3;
+                const double wa = axis == 0 ? wx : (axis == 1 ? wy : wz);
+                const double da = axis == 0 ? Q.d[0] : (axis == 1 ? Q.d[1] : Q.d[2]);
+                slope = (wa / len) * da;
             }
         }
         double newton = E.t - g / slope;
