@@ -612,9 +612,34 @@ struct Group {
 
 __device__ inline bool group_any(const Group& G, bool p) { return ((__ballot(p) >> G.shift) & G.mask) != 0ull; }
 
-// The least (best, bo) over the group's lanes.
+// One step of group_least with a DPP lane pattern (a VALU move, no LDS
+// round trip): each lane takes the least of its pair and the lane CTRL
+// names, which lies in the same group.
+template <int CTRL>
+__device__ inline void least_dpp(double& best, uint32_t& bo) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(best);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, 0xF, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, 0xF, 0xF, false);
+    const double od = __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+    const uint32_t oo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)bo, CTRL, 0xF, 0xF, false);
+    if (precedes(od, oo, best, bo)) {
+        best = od;
+        bo = oo;
+    }
+}
+
+// The least (best, bo) over the group's lanes, in every lane of the group
+// (every lane of the group must be active). precedes() is a total order on
+// (distance, list position), so the least does not depend on how lanes are
+// paired: within 16 lanes by DPP quad permutes (lane ^ 1, lane ^ 2) and the
+// half-row and row mirrors (lane i with 7 - i, then 15 - i, in its 8 and 16
+// lanes), beyond by shuffles.
 __device__ inline void group_least(const Group& G, double& best, uint32_t& bo) {
-    for (uint32_t o = G.g >> 1; o > 0; o >>= 1) {
+    if (G.g >= 2) least_dpp<0xB1>(best, bo);   // quad_perm [1, 0, 3, 2]
+    if (G.g >= 4) least_dpp<0x4E>(best, bo);   // quad_perm [2, 3, 0, 1]
+    if (G.g >= 8) least_dpp<0x141>(best, bo);  // row_half_mirror
+    if (G.g >= 16) least_dpp<0x140>(best, bo); // row_mirror
+    for (uint32_t o = 16; o < G.g; o <<= 1) {
         double od = __shfl_xor(best, (int)o, WAVE);
         uint32_t oo = __shfl_xor(bo, (int)o, WAVE);
         if (precedes(od, oo, best, bo)) {
