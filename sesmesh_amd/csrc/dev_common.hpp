@@ -284,6 +284,16 @@ __device__ inline float wave_minf(float v) {
 }
 __device__ inline bool wave_any(bool p) { return __ballot(p) != 0ull; }
 
+// A wavefront-uniform double moved to scalar registers: values the whole
+// workgroup shares (a ball centre, a radius) then stay out of the vector
+// registers, which the list passes need.
+__device__ inline double uniform_f64(double v) {
+    const unsigned long long x = (unsigned long long)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(x >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 __device__ inline uint32_t lane_id() { return threadIdx.x & (WAVE - 1); }
 
 // f64::total_cmp as an order-preserving integer key.

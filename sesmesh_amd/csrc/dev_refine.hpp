@@ -1032,14 +1032,14 @@ __device__ __forceinline__ void refine_sub(const Mol* __restrict__ mols, const u
     double h = P.h, subd = (double)sub;
     if (threadIdx.x == 0) evals_sum = 0;
     // Ball of the sub-block (mesh/extract.rs:164-205).
-    double rho = 0.5 * sqrt(3.0) * subd * h * (1.0 + 1e-9) + 1e-9;
+    double rho = uniform_f64(0.5 * sqrt(3.0) * subd * h * (1.0 + 1e-9) + 1e-9);
     uint32_t o3[3] = {sb & 1, (sb >> 1) & 1, sb >> 2};
     int64_t org[3];
     double c[3];
     for (int a = 0; a < 3; a++) {
         org[a] = (int64_t)key[a] * (int64_t)b;
         double basep = (double)org[a] * h;
-        c[a] = basep + ((double)(o3[a] * sub) + 0.5 * subd) * h;
+        c[a] = uniform_f64(basep + ((double)(o3[a] * sub) + 0.5 * subd) * h);
     }
     const uint32_t* hl = nullptr;  // the sampling's hidden candidates of this sub-block
     uint32_t hn = hid.cnt ? hid.cnt[j] : NO_LIST;

@@ -272,7 +272,7 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
     if (tid < WAVE) bfill[tid] = 0u;
     __syncthreads();
     double h = P.h, sub = (double)P.sub, half = sub * 0.5;
-    double rho = 0.5 * sqrt(3.0) * sub * h;
+    double rho = uniform_f64(0.5 * sqrt(3.0) * sub * h);
     const bool sph = A.kind != K_SES;
     double reach = (sph ? 0.0 : P.probe) + P.band + rho;
     // The refinement's ball and reach (refine_sub / gather_refine), computed
@@ -293,7 +293,7 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
     for (int a = 0; a < 3; a++) {
         double base = (double)((int64_t)key[a] * (int64_t)P.b) * h;
         double off = (double)((sb >> a) & 1) * sub + half;
-        b[a] = base + off * h;
+        b[a] = uniform_f64(base + off * h);
     }
     const uint32_t hmask = cap.hash - 1;
     const double radii[3] = {reach_r + A.max_expanded, reach_r + A.rho_max, reach_r};
@@ -410,9 +410,9 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
     // at most about 1.2e-4 per coordinate, far inside COVER_MARGIN, so the
     // refinement's f64 tests find these candidates blocked too.
     double bl[3];
-    for (int a = 0; a < 3; a++) bl[a] = b[a] - (double)P.frame[a] * P.h;
+    for (int a = 0; a < 3; a++) bl[a] = uniform_f64(b[a] - (double)P.frame[a] * P.h);
     mark_hidden(
-        bl, rho + COVER_PAD, gathered, n[C_NEAR], [&](uint32_t q) { return s.cbk[q] < KEY_BUCKETS ? 0u : 1u; },
+        bl, uniform_f64(rho + COVER_PAD), gathered, n[C_NEAR], [&](uint32_t q) { return s.cbk[q] < KEY_BUCKETS ? 0u : 1u; },
         [&](uint32_t q) { return s.cid[q]; },
         [&](uint32_t kd, uint32_t k, dvec4& g0, dvec4& g1) {
             if (kd == 0u) {
