@@ -527,8 +527,13 @@ __device__ inline bool arc_covered(const double b[3], double rb, const dvec4& a,
 // Each test is split in two: what depends only on the ball and the
 // candidate (its row of the bitsets) is computed once per row, with the
 // operations and order of the whole test, so every decision is unchanged.
-// T is double, or float for callers whose inputs are f32 and whose limits
-// carry a slack far above the f32 rounding of the test.
+// T is double, or float for callers whose limits carry REACH_SLACK on both
+// the ball radius and the reach: f32 inputs within 1.2e-4 of the values
+// they stand for (coordinates below 2048, or lengths of a few A relative to
+// the ball centre) and the test's own f32 arithmetic on lengths of a few A
+// (about 1e-5) stay far inside it, so a blocker the f32 test leaves out
+// cannot come within its blocking radius of any projection either.
+constexpr float REACH_SLACK = 1e-3f;
 template <class T>
 struct ReachRow {
     T ox, oy, oz;  // sphere centre c, or circle centre m

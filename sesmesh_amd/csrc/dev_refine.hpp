@@ -87,7 +87,7 @@ __host__ __device__ inline RLayout make_rlayout(const RCaps& c) {
     L.nin = take(8 * c.near);  // inside threshold of R
     L.nR = take(c.radii ? 8 * c.near : 0);
     L.nr = take(c.radii ? 8 * c.near : 0);
-    L.nt = take(c.radii ? 0 : 8 * c.near);  // reach limit of the near spheres as blockers
+    L.nt = take(c.radii ? 0 : 4 * c.near);  // reach limit of the near spheres as blockers
     L.hk = take(4 * c.hash);
     L.hv = take(4 * c.hash);
     L.ins = take(2 * c.inside);
@@ -106,7 +106,7 @@ struct RLists {
     NearD* nq;
     double* nin;
     double *nR, *nr;  // radii of near spheres (when gathered)
-    double* nt;       // sqrt(blk) + COVER_MARGIN of near spheres (refinement lists)
+    float* nt;        // sqrt(blk) + COVER_MARGIN + REACH_SLACK of near spheres (refinement lists)
     uint32_t *hk, *hv;
     uint16_t* ins;
     unsigned long long* ckey;
@@ -125,7 +125,7 @@ __device__ inline RLists rcarve(char* base, const RLayout& L) {
     s.nin = (double*)(base + L.nin);
     s.nR = (double*)(base + L.nR);
     s.nr = (double*)(base + L.nr);
-    s.nt = (double*)(base + L.nt);
+    s.nt = (float*)(base + L.nt);
     s.hk = (uint32_t*)(base + L.hk);
     s.hv = (uint32_t*)(base + L.hv);
     s.ins = (uint16_t*)(base + L.ins);
@@ -228,7 +228,7 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
                 q.blk = t.blk;
                 s.nq[at] = q;
                 s.nin[at] = t.nin;
-                if (hide) s.nt[at] = sqrt(t.blk) + COVER_MARGIN;
+                if (hide) s.nt[at] = (float)(sqrt(t.blk) + COVER_MARGIN) + REACH_SLACK;
                 if (cap.radii) {
                     s.nR[at] = R;
                     s.nr[at] = A.r[k];
@@ -445,22 +445,26 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
     __syncthreads();
     // Blocker bitsets of the S and C rows (cpre holds the row length).
     // With `hide`, a blocker that no projection of the ball comes near stays
-    // out of the bitset: it could never block there.
-    const double bc[3] = {bx, by, bz};
-    const double rb = rho + COVER_PAD;
+    // out of the bitset: it could never block there. The test runs in f32 on
+    // the records relative to the ball centre (lengths of a few A), its
+    // limits widened by REACH_SLACK.
+    const float b0[3] = {0.0f, 0.0f, 0.0f};
+    const float rbf = (float)(rho + COVER_PAD) + REACH_SLACK;
     blocker_bits(
         A.nb_items, A.cb_items, ns, ns + nc, s.coff, s.cpre, s.hk, s.hv, hmask, s.cbits, cc,
         [&](uint32_t p) {
             const double* d = s.cdat + (size_t)CDAT * p;
-            if (!hide) return ReachRow<double>{};
-            if (p < ns) return cap_row(bc, rb, d[0], d[1], d[2], d[3]);
-            return arc_row(bc, rb, d[0], d[1], d[2], d[3], d[4], d[5], d[6]);
+            if (!hide) return ReachRow<float>{};
+            const float ox = (float)(d[0] - bx), oy = (float)(d[1] - by), oz = (float)(d[2] - bz);
+            if (p < ns) return cap_row(b0, rbf, ox, oy, oz, (float)d[3]);
+            return arc_row(b0, rbf, ox, oy, oz, (float)d[3], (float)d[4], (float)d[5], (float)d[6]);
         },
-        [&](const ReachRow<double>& w, uint32_t p, uint32_t slot) {
+        [&](const ReachRow<float>& w, uint32_t p, uint32_t slot) {
             if (!hide) return true;
             const NearD k = s.nq[slot];
-            const double t = s.nt[slot];
-            return p < ns ? cap_reaches(w, k.x, k.y, k.z, t) : arc_reaches(w, k.x, k.y, k.z, t);
+            const float kx = (float)(k.x - bx), ky = (float)(k.y - by), kz = (float)(k.z - bz);
+            const float t = s.nt[slot];
+            return p < ns ? cap_reaches(w, kx, ky, kz, t) : arc_reaches(w, kx, ky, kz, t);
         });
     __syncthreads();
     return true;

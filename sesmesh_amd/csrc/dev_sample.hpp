@@ -489,14 +489,9 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
     // Blocker bitsets of the S and C rows (cpre holds the row length). A
     // blocker that no projection of the ball comes near stays out of the
     // bitset: it could never block a sample of the sub-block. The test runs
-    // in f32 on the f32 frame-local records the lane kernel uses. Its limits
-    // carry COVER_MARGIN, which holds the f32 rounding of the samples and of
-    // their projections, and REACH_SLACK on both the ball and the reach,
-    // which holds the f32 rounding of the test itself (the ball centre to
-    // f32: at most 1.2e-4 per coordinate; the test's own arithmetic on
-    // lengths of a few A: about 1e-5): a blocker the test leaves out cannot
-    // come within its blocking radius of any projection.
-    constexpr float REACH_SLACK = 1e-3f;
+    // in f32 on the f32 frame-local records the lane kernel uses (the ball
+    // centre rounded to f32); its limits carry COVER_MARGIN, which holds the
+    // f32 rounding of the samples and of their projections, and REACH_SLACK.
     const float bf[3] = {(float)bl[0], (float)bl[1], (float)bl[2]};
     const float rbf = (float)(rho + COVER_PAD) + REACH_SLACK;
     blocker_bits(
