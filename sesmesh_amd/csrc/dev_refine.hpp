@@ -1107,6 +1107,11 @@ __device__ __forceinline__ void refine_sub(const Mol* __restrict__ mols, const u
                 y = Q.pa[1] + Q.d[1] * E.t;
                 z = Q.pa[2] + Q.d[2] * E.t;
             }
+            // The lane's rank, opaque to the compiler here: the list
+            // addresses derived from it are recomputed in each evaluation
+            // (two ALU operations) instead of being hoisted out of the loop
+            // and spilled for want of registers.
+            asm volatile("" : "+v"(G.r));
             EvalD e = eval_group(A, s, n, cap.cand, nw, G, c[0], c[1], c[2], x, y, z, cutoff, P.band);
             if (edge_step(E, e, h, sph)) {
                 size_t slot = (size_t)vbase[brick] + edge_rank(mask, wpre, words, brick, E.id);
