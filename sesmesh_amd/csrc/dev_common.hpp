@@ -416,18 +416,19 @@ constexpr uint32_t KEY_BUCKETS = 16;  // key buckets per candidate class in the 
 // loading 4 of its items before it probes any, so 4 loads are in flight per
 // lane. keep(r, p, slot) may leave out a blocker that can never block row
 // p's candidate where it is evaluated; r = prep(p) is computed once per row
-// by each of its lanes. By the whole workgroup; the caller synchronises
-// afterwards.
-template <class Prep, class Keep>
+// by each of its lanes, and finish(r, p) runs once the row's items are done,
+// by all 8 of its lanes together (len[p] is no longer read then). By the
+// whole workgroup; the caller synchronises afterwards.
+template <class Prep, class Keep, class Finish>
 __device__ inline void blocker_bits(gptr<const uint32_t> nb_items, gptr<const uint32_t> cb_items, uint32_t ns,
                                     uint32_t rows, const uint32_t* off, const uint32_t* len, const uint32_t* hk,
                                     const uint32_t* hv, uint32_t hmask, unsigned long long* bits, uint32_t cc,
-                                    Prep&& prep, Keep&& keep) {
+                                    Prep&& prep, Keep&& keep, Finish&& finish) {
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     for (uint32_t p = tid / 8; p < rows; p += nt / 8) {
         gptr<const uint32_t> row = (p < ns ? nb_items : cb_items) + off[p];
         const uint32_t n = len[p];
-        const auto r = prep(p);
+        auto r = prep(p);
         for (uint32_t q0 = tid % 8; q0 < n; q0 += 32) {
             uint32_t it[4];
             for (int u = 0; u < 4; u++) it[u] = q0 + 8 * u < n ? row[q0 + 8 * u] : 0xffffffffu;
@@ -437,6 +438,7 @@ __device__ inline void blocker_bits(gptr<const uint32_t> nb_items, gptr<const ui
                 if (slot != 0xffffffffu && keep(r, p, slot)) atomicOr(&bits[(slot >> 6) * cc + p], 1ull << (slot & 63));
             }
         }
+        finish(r, p);
     }
 }
 

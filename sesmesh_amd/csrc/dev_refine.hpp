@@ -115,9 +115,11 @@ struct RLists {
     // V (x, y, z).
     double* cdat;
     unsigned long long* cbits;  // near/64 rows of `cand` words
-    uint32_t *coff, *cpre;
+    uint32_t *coff, *cpre;  // blocker row offset and length; once the bitsets are built, cpre holds the hint
     uint32_t* ebuf;
 };
+
+constexpr uint32_t NO_HINT = 0xffffffffu;
 
 __device__ inline RLists rcarve(char* base, const RLayout& L) {
     RLists s;
@@ -447,24 +449,63 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
     // With `hide`, a blocker that no projection of the ball comes near stays
     // out of the bitset: it could never block there. The test runs in f32 on
     // the records relative to the ball centre (lengths of a few A), its
-    // limits widened by REACH_SLACK.
+    // limits widened by REACH_SLACK. Each row's hint (then in cpre) is the
+    // kept blocker holding the row's central projection deepest (least
+    // power distance |k - p0|^2 - t^2, when negative): projections of nearby
+    // points are likely inside it too, so blocked() tries it first.
+    struct HintRow {
+        ReachRow<float> w;
+        float px, py, pz, best;
+        uint32_t hint;
+    };
     const float b0[3] = {0.0f, 0.0f, 0.0f};
     const float rbf = (float)(rho + COVER_PAD) + REACH_SLACK;
     blocker_bits(
         A.nb_items, A.cb_items, ns, ns + nc, s.coff, s.cpre, s.hk, s.hv, hmask, s.cbits, cc,
         [&](uint32_t p) {
+            HintRow r{};
+            r.best = 0.0f;
+            r.hint = NO_HINT;
+            if (!hide) return r;
             const double* d = s.cdat + (size_t)CDAT * p;
-            if (!hide) return ReachRow<float>{};
             const float ox = (float)(d[0] - bx), oy = (float)(d[1] - by), oz = (float)(d[2] - bz);
-            if (p < ns) return cap_row(b0, rbf, ox, oy, oz, (float)d[3]);
-            return arc_row(b0, rbf, ox, oy, oz, (float)d[3], (float)d[4], (float)d[5], (float)d[6]);
+            r.w = p < ns ? cap_row(b0, rbf, ox, oy, oz, (float)d[3])
+                         : arc_row(b0, rbf, ox, oy, oz, (float)d[3], (float)d[4], (float)d[5], (float)d[6]);
+            // p0 = c + R v / |v| (v = b - c, or the in-plane w for a circle)
+            const float vl = sqrtf(r.w.D2), sc = vl > 0.0f ? r.w.R / vl : 0.0f;
+            r.px = r.w.ox + r.w.vx * sc;
+            r.py = r.w.oy + r.w.vy * sc;
+            r.pz = r.w.oz + r.w.vz * sc;
+            return r;
         },
-        [&](const ReachRow<float>& w, uint32_t p, uint32_t slot) {
+        [&](HintRow& r, uint32_t p, uint32_t slot) {
             if (!hide) return true;
             const NearD k = s.nq[slot];
             const float kx = (float)(k.x - bx), ky = (float)(k.y - by), kz = (float)(k.z - bz);
             const float t = s.nt[slot];
-            return p < ns ? cap_reaches(w, kx, ky, kz, t) : arc_reaches(w, kx, ky, kz, t);
+            const bool reach = p < ns ? cap_reaches(r.w, kx, ky, kz, t) : arc_reaches(r.w, kx, ky, kz, t);
+            if (reach) {
+                const float ex = kx - r.px, ey = ky - r.py, ez = kz - r.pz;
+                const float pw = ex * ex + ey * ey + ez * ez - t * t;
+                if (pw < r.best) {
+                    r.best = pw;
+                    r.hint = slot;
+                }
+            }
+            return reach;
+        },
+        [&](HintRow& r, uint32_t p) {
+            // the least (best, hint) of the row's 8 lanes (total order: power, then slot)
+            auto step = [&](float ob, uint32_t oh) {
+                if (ob < r.best || (ob == r.best && oh < r.hint)) {
+                    r.best = ob;
+                    r.hint = oh;
+                }
+            };
+            step(__shfl_xor(r.best, 1, 8), __shfl_xor(r.hint, 1, 8));
+            step(__shfl_xor(r.best, 2, 8), __shfl_xor(r.hint, 2, 8));
+            step(__shfl_xor(r.best, 4, 8), __shfl_xor(r.hint, 4, 8));
+            if ((threadIdx.x & 7u) == 0u) s.cpre[p] = r.hint;
         });
     __syncthreads();
     return true;
@@ -480,6 +521,12 @@ struct EvalD {
 // !(ddist(q, c) >= R - EPS) for a near row member.
 __device__ inline bool blocked(const RLists& s, uint32_t cc, uint32_t nw, uint32_t p, double qx, double qy,
                                double qz) {
+    const uint32_t hint = s.cpre[p];  // one of the bitset's blockers, tried first
+    if (hint != NO_HINT) {
+        const NearD b = s.nq[hint];
+        const double ex = qx - b.x, ey = qy - b.y, ez = qz - b.z;
+        if (ex * ex + ey * ey + ez * ez < b.blk) return true;
+    }
     for (uint32_t w = 0; w < nw; w++) {
         unsigned long long m = s.cbits[w * cc + p];
         while (m) {

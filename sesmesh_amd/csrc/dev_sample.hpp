@@ -506,7 +506,8 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
             const float4 k = s.nd[slot];
             const float t = sqrtf(k.w) + ((float)COVER_MARGIN + REACH_SLACK);
             return p < ns ? cap_reaches(w, k.x, k.y, k.z, t) : arc_reaches(w, k.x, k.y, k.z, t);
-        });
+        },
+        [](const ReachRow<float>&, uint32_t) {});
     __syncthreads();
     return true;
 }
