@@ -138,7 +138,14 @@ constexpr uint32_t HIDDEN = 0x80000000u;  // a candidate's bucket word when it i
 
 // True if (qx, qy, qz) lies inside a blocker of candidate p (kernel.rs:
 // 898-906, 935-943): !(|q - c|^2 >= (r - tol)^2) for a near row member.
+// The row's hint (in cpre, one of the bitset's blockers) is tried first.
 __device__ inline bool blocked_f(const Lists& s, uint32_t cc, uint32_t nw, uint32_t p, float qx, float qy, float qz) {
+    const uint32_t hint = s.cpre[p];
+    if (hint != 0xffffffffu) {
+        const float4 b = s.nd[hint];
+        const float ex = qx - b.x, ey = qy - b.y, ez = qz - b.z;
+        if (!(ex * ex + ey * ey + ez * ez >= b.w)) return true;
+    }
     for (uint32_t w = 0; w < nw; w++) {
         unsigned long long m = s.cbits[w * cc + p];
         while (m) {
@@ -492,22 +499,61 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
     // in f32 on the f32 frame-local records the lane kernel uses (the ball
     // centre rounded to f32); its limits carry COVER_MARGIN, which holds the
     // f32 rounding of the samples and of their projections, and REACH_SLACK.
+    // Each row's hint (then in cpre) is the kept blocker holding the row's
+    // central projection p0 deepest (least |k - p0|^2 - t^2, when
+    // negative), which blocked_f() tries first.
+    struct HintRow {
+        ReachRow<float> w;
+        float px, py, pz, best;
+        uint32_t hint;
+    };
     const float bf[3] = {(float)bl[0], (float)bl[1], (float)bl[2]};
     const float rbf = (float)(rho + COVER_PAD) + REACH_SLACK;
     blocker_bits(
         A.nb_items, A.cb_items, ns, total, s.coff, s.cpre, s.hk, s.hv, hmask, s.cbits, cc,
         [&](uint32_t p) {
+            HintRow r;
             const float4 c = s.cdat[3 * p];
-            if (p < ns) return cap_row(bf, rbf, c.x, c.y, c.z, c.w);
-            const float4 u = s.cdat[3 * p + 1];
-            return arc_row(bf, rbf, c.x, c.y, c.z, u.x, u.y, u.z, c.w);
+            if (p < ns) {
+                r.w = cap_row(bf, rbf, c.x, c.y, c.z, c.w);
+            } else {
+                const float4 u = s.cdat[3 * p + 1];
+                r.w = arc_row(bf, rbf, c.x, c.y, c.z, u.x, u.y, u.z, c.w);
+            }
+            const float vl = sqrtf(r.w.D2), sc = vl > 0.0f ? r.w.R / vl : 0.0f;
+            r.px = r.w.ox + r.w.vx * sc;
+            r.py = r.w.oy + r.w.vy * sc;
+            r.pz = r.w.oz + r.w.vz * sc;
+            r.best = 0.0f;
+            r.hint = 0xffffffffu;
+            return r;
         },
-        [&](const ReachRow<float>& w, uint32_t p, uint32_t slot) {
+        [&](HintRow& r, uint32_t p, uint32_t slot) {
             const float4 k = s.nd[slot];
             const float t = sqrtf(k.w) + ((float)COVER_MARGIN + REACH_SLACK);
-            return p < ns ? cap_reaches(w, k.x, k.y, k.z, t) : arc_reaches(w, k.x, k.y, k.z, t);
+            const bool reach = p < ns ? cap_reaches(r.w, k.x, k.y, k.z, t) : arc_reaches(r.w, k.x, k.y, k.z, t);
+            if (reach) {
+                const float ex = k.x - r.px, ey = k.y - r.py, ez = k.z - r.pz;
+                const float pw = ex * ex + ey * ey + ez * ez - t * t;
+                if (pw < r.best) {
+                    r.best = pw;
+                    r.hint = slot;
+                }
+            }
+            return reach;
         },
-        [](const ReachRow<float>&, uint32_t) {});
+        [&](HintRow& r, uint32_t p) {
+            auto step = [&](float ob, uint32_t oh) {
+                if (ob < r.best || (ob == r.best && oh < r.hint)) {
+                    r.best = ob;
+                    r.hint = oh;
+                }
+            };
+            step(__shfl_xor(r.best, 1, 8), __shfl_xor(r.hint, 1, 8));
+            step(__shfl_xor(r.best, 2, 8), __shfl_xor(r.hint, 2, 8));
+            step(__shfl_xor(r.best, 4, 8), __shfl_xor(r.hint, 4, 8));
+            if ((threadIdx.x & 7u) == 0u) s.cpre[p] = r.hint;
+        });
     __syncthreads();
     return true;
 }
