@@ -400,12 +400,37 @@ struct SupList {
     uint32_t *off, *cnt, *ids, *used;
     uint32_t cap;
 };
+// Fixed-size slots of the blocker records: sampled sub-block j (dense
+// index slot[j] among the sampled ones, below `slots`) has `words` words at
+// ids + slot[j] * words; cnt[j] is its record's length, NO_LIST where it
+// has none (not sampled, or the record did not fit). ids null: none are
+// kept.
+struct BlkPool {
+    const uint32_t* slot;
+    uint32_t *cnt, *ids;
+    uint32_t words, slots;
+};
 struct HidList {
     uint32_t *off, *cnt, *ids, *used;
     uint32_t cap;
     SupList sup;   // handed over the same way
     SupList bsup;  // per surface brick: the classification's superset of its sampling walks (see gather_refine)
+    BlkPool blk;   // per sampled sub-block: its blocker record (see BlkRecord)
 };
+
+// The blocker record a sampling workgroup leaves for the refinement of the
+// same sub-block, in the u32 pool of HidList::blk: n_near, n_s, n_c, nw
+// (64-bit words per bitset), then the n_near near-sphere ids by near-list
+// slot, then per kept class S / C candidate, in the refinement's list
+// order (class, f64 total_cmp key at the ball centre, index): its id (class
+// in the top bits, as the refinement's lists hold it), its hint slot, and
+// its blocker bitset over near-list slots (nw words, low half first). The
+// blockers are those the sampling's reach test keeps, a superset of those
+// any refinement point can meet (the test's ball and limits are larger),
+// so the refinement may take them instead of walking the rows, once its
+// own near list and candidates are found to be the same.
+constexpr uint32_t BLK_HEAD = 4;
+__host__ __device__ inline uint32_t blk_entry_words(uint32_t nw) { return 2u + 2u * nw; }
 constexpr uint32_t NO_LIST = 0xffffffffu;
 constexpr uint32_t KEY_BUCKETS = 16;  // key buckets per candidate class in the gathers
 

@@ -400,7 +400,10 @@ __device__ __forceinline__ void refine_sub_fast(const Mol* __restrict__ mols, co
     const uint32_t* sl = nullptr;  // and its superset of this gather's items
     uint32_t sn = hid.sup.cnt ? hid.sup.cnt[j] : NO_LIST;
     if (sn != NO_LIST) sl = hid.sup.ids + hid.sup.off[j];
-    bool ok = gather_refine(A, cap, s, n, tmp4, c[0], c[1], c[2], rho, P.band, true, hl, hn, sl, sn);
+    const uint32_t* br = nullptr;  // and its blocker record
+    if (hid.blk.cnt && hid.blk.cnt[j] != NO_LIST) br = hid.blk.ids + (size_t)hid.blk.slot[j] * hid.blk.words;
+    bool ok = gather_refine(A, cap, s, n, tmp4, c[0], c[1], c[2], rho, P.band, true, hl, hn, sl, sn, SupList{}, 0,
+                            0.0, br);
     uint32_t lo[3], hi[3];
     for (int a = 0; a < 3; a++) {
         lo[a] = o3[a] * sub;

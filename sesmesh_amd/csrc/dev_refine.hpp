@@ -159,7 +159,7 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
                                      double bx, double by, double bz, double rho, double band, bool hide,
                                      const uint32_t* hl = nullptr, uint32_t hn = NO_LIST,
                                      const uint32_t* sl = nullptr, uint32_t sn = NO_LIST, SupList xs = SupList{},
-                                     uint32_t xj = 0, double reach_x = 0.0) {
+                                     uint32_t xj = 0, double reach_x = 0.0, const uint32_t* br = nullptr) {
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     __shared__ uint32_t bstart[WAVE], bfill[WAVE];  // per key bucket: 16 for each of S, C, V
     for (uint32_t i = tid; i < cap.hash; i += nt) s.hk[i] = 0u;
@@ -453,6 +453,48 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
     // kept blocker holding the row's central projection deepest (least
     // power distance |k - p0|^2 - t^2, when negative): projections of nearby
     // points are likely inside it too, so blocked() tries it first.
+    //
+    // The sampling of the same sub-block may have left its blocker record
+    // (br, see BlkRecord): when its near spheres and kept candidates are
+    // this list's, in the same order, its bitsets (a superset of what this
+    // reach test keeps) and hints are taken instead, mapped to this list's
+    // near-list slots through the hash.
+    __shared__ uint32_t rec_head, rec_bad;
+    if (tid == 0) {
+        rec_head = br && hide && br[0] == n[R_NEAR] && br[1] == ns && br[2] == nc && br[3] == (n[R_NEAR] + 63) / 64;
+        rec_bad = 0u;
+    }
+    __syncthreads();
+    if (rec_head) {
+        const uint32_t nn = br[0], nwr = br[3], ew = blk_entry_words(nwr);
+        const uint32_t* ent = br + BLK_HEAD + nn;
+        uint16_t* rslot = (uint16_t*)s.nt;  // the reach limits are not needed on this path
+        for (uint32_t i = tid; i < nn; i += nt) {
+            const uint32_t slot = hash_get(s.hk, s.hv, hmask, br[BLK_HEAD + i]);
+            if (slot == 0xffffffffu)
+                rec_bad = 1u;
+            else
+                rslot[i] = (uint16_t)slot;
+        }
+        for (uint32_t p = tid; p < ns + nc; p += nt)
+            if (ent[(size_t)p * ew] != s.cid[p]) rec_bad = 1u;
+        __syncthreads();
+        if (!rec_bad) {
+            for (uint32_t q = tid; q < (ns + nc) * nwr; q += nt) {
+                const uint32_t p = q / nwr, w = q % nwr;
+                const uint32_t* e = ent + (size_t)p * ew;
+                unsigned long long bits = (unsigned long long)e[2 + 2 * w] | ((unsigned long long)e[3 + 2 * w] << 32);
+                while (bits) {
+                    const uint32_t sl = rslot[w * 64 + (uint32_t)__builtin_ctzll(bits)];
+                    bits &= bits - 1ull;
+                    atomicOr(&s.cbits[(sl >> 6) * cc + p], 1ull << (sl & 63));
+                }
+                if (w == 0) s.cpre[p] = e[1] == NO_HINT ? NO_HINT : (uint32_t)rslot[e[1]];
+            }
+            __syncthreads();
+            return true;
+        }
+    }
     struct HintRow {
         ReachRow<float> w;
         float px, py, pz, best;
@@ -1098,7 +1140,10 @@ __device__ __forceinline__ void refine_sub(const Mol* __restrict__ mols, const u
     const uint32_t* sl = nullptr;  // and its superset of this gather's items
     uint32_t sn = hid.sup.cnt ? hid.sup.cnt[j] : NO_LIST;
     if (sn != NO_LIST) sl = hid.sup.ids + hid.sup.off[j];
-    bool ok = gather_refine(A, cap, s, n, tmp4, c[0], c[1], c[2], rho, P.band, true, hl, hn, sl, sn);
+    const uint32_t* br = nullptr;  // and its blocker record
+    if (hid.blk.cnt && hid.blk.cnt[j] != NO_LIST) br = hid.blk.ids + (size_t)hid.blk.slot[j] * hid.blk.words;
+    bool ok = gather_refine(A, cap, s, n, tmp4, c[0], c[1], c[2], rho, P.band, true, hl, hn, sl, sn, SupList{}, 0,
+                            0.0, br);
     // Crossing edges whose lower sample lies in this sub-block, ascending.
     uint32_t lo[3], hi[3];
     for (int a = 0; a < 3; a++) {

@@ -260,6 +260,87 @@ __device__ inline float sphere_value(const Lists& s, const uint32_t* n, float x,
     return best;
 }
 
+// The kept class S / C candidate ids of the lists (see the scatter in
+// gather_sub).
+__device__ inline uint32_t kept_id(const Lists& s, uint32_t ns, uint32_t p) {
+    return p < ns ? __float_as_uint(s.cdat[3 * p + 1].x) : __float_as_uint(s.cdat[3 * p + 2].w);
+}
+
+// Writes the BlkRecord of sub-block j (ball centre b, the lists' ns class S
+// and total class S + C kept candidates) to the pool, or marks it absent if
+// the pool is full. The candidates are ranked by the refinement's order:
+// their f64 keys at the ball centre computed as gather_refine computes them
+// (|d - R| of a sphere, the circle distance), then the id. By the whole
+// workgroup, after the bitsets; it reuses the near hash as scratch.
+__device__ inline void write_blk_record(const DArr& A, const Lists& s, const uint32_t* n, const Caps& cap,
+                                        const BlkPool& blk, uint32_t j, const double b[3], uint32_t ns,
+                                        uint32_t total) {
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    const uint32_t nn = n[C_NEAR], nwr = (nn + 63) / 64, ew = blk_entry_words(nwr), cc = cap.cand;
+    const uint32_t m = BLK_HEAD + nn + total * ew, slot = blk.slot[j];
+    const bool fits = m <= blk.words && slot < blk.slots;  // the same in every thread
+    if (tid == 0) blk.cnt[j] = fits ? m : NO_LIST;
+    if (!fits) return;
+    uint32_t* o = blk.ids + (size_t)slot * blk.words;
+    if (tid == 0) {
+        o[0] = nn;
+        o[1] = ns;
+        o[2] = total - ns;
+        o[3] = nwr;
+    }
+    for (uint32_t i = tid; i < cap.hash; i += nt) {  // near ids by slot, from the near hash
+        const uint32_t key = s.hk[i];
+        if (key) o[BLK_HEAD + s.hv[i]] = key - 1u;
+    }
+    __syncthreads();
+    unsigned long long* kk = (unsigned long long*)s.hk;  // the hash is free now
+    for (uint32_t p = tid; p < total; p += nt) {
+        const uint32_t k = kept_id(s, ns, p);
+        double key;
+        if (p < ns) {
+            const SphereRec S = load_rec(A.sph, k);
+            key = fabs(ddist(b[0], b[1], b[2], S.x, S.y, S.z) - S.R);
+        } else {
+            const CircleRec C = load_rec(A.cir, k);
+            double yx = b[0] - C.mx, yy = b[1] - C.my, yz = b[2] - C.mz;
+            double ax = C.ax, ay = C.ay, az = C.az;
+            double along = yx * ax + yy * ay + yz * az;
+            double wx = yx - ax * along, wy = yy - ay * along, wz = yz - az * along;
+            double wl = sqrt(wx * wx + wy * wy + wz * wz);
+            double rr = C.rho;
+            key = sqrt(along * along + (wl - rr) * (wl - rr));
+        }
+        kk[p] = total_key(key);
+    }
+    __syncthreads();
+    uint32_t* ent = o + BLK_HEAD + nn;
+    // 8 lanes per candidate count the ones of its class ranked before it.
+    for (uint32_t p = tid / 8; p < total; p += nt / 8) {
+        const bool S = p < ns;
+        const uint32_t lo = S ? 0u : ns, hi = S ? ns : total, k = kept_id(s, ns, p);
+        const unsigned long long kp = kk[p];
+        uint32_t rank = 0;
+        for (uint32_t u = lo + (tid & 7u); u < hi; u += 8) {
+            const unsigned long long ku = kk[u];
+            rank += (ku < kp || (ku == kp && kept_id(s, ns, u) < k)) ? 1u : 0u;
+        }
+        rank += __shfl_xor(rank, 1, 8);
+        rank += __shfl_xor(rank, 2, 8);
+        rank += __shfl_xor(rank, 4, 8);
+        uint32_t* e = ent + (size_t)(lo + rank) * ew;
+        const uint32_t r = tid & 7u;
+        if (r == 0) {
+            e[0] = k | (S ? 0u : 1u << KIND_SHIFT);
+            e[1] = s.cpre[p];
+        }
+        for (uint32_t w = r; w < nwr; w += 8) {
+            const unsigned long long bits = s.cbits[w * cc + p];
+            e[2 + 2 * w] = (uint32_t)bits;
+            e[3 + 2 * w] = (uint32_t)(bits >> 32);
+        }
+    }
+}
+
 // Gathers the lists of sub-block `sb` of brick `key` (kernel.rs:553-751).
 // The SAS and vdW kinds (gather_spheres, kernel.rs:645-652) list the near
 // spheres within band + rho, with their radii, and nothing else.
@@ -270,8 +351,8 @@ __device__ inline float sphere_value(const Lists& s, const uint32_t* n, float x,
 // sampling's own tests are unchanged.
 __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, const Caps& cap, Lists& s,
                                   uint32_t* n, uint32_t* tmp4, const int32_t* key, uint32_t sb,
-                                  const SupList& sup, uint32_t j, const uint32_t* xl = nullptr,
-                                  uint32_t xn = NO_LIST) {
+                                  const SupList& sup, uint32_t j, const uint32_t* xl, uint32_t xn,
+                                  const BlkPool& blk) {
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     __shared__ uint32_t bfill[WAVE];
     for (uint32_t i = tid; i < cap.hash; i += nt) s.hk[i] = 0u;
@@ -479,14 +560,19 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
         if (s.cbk[q] >= HIDDEN) continue;
         const uint32_t p = atomicAdd(&bfill[s.cbk[q]], 1u);
         const uint32_t k = s.cid[q];
+        // The candidate's id also goes to a slot of its record the lane
+        // kernel does not read (S: the second float4, C: the third's w).
         if (p < ns) {
             s.cdat[3 * p] = load_f4(Lc.p4, k);
+            s.cdat[3 * p + 1].x = __uint_as_float(k);
             s.coff[p] = A.nb_off[k];
             s.cpre[p] = A.nb_off[k + 1] - A.nb_off[k];
         } else {
             s.cdat[3 * p] = load_f4(Lc.m4, k);
             s.cdat[3 * p + 1] = load_f4(Lc.u4, k);
-            s.cdat[3 * p + 2] = load_f4(Lc.e4, k);
+            float4 e = load_f4(Lc.e4, k);
+            e.w = __uint_as_float(k);
+            s.cdat[3 * p + 2] = e;
             s.coff[p] = A.cb_off[k];
             s.cpre[p] = A.cb_off[k + 1] - A.cb_off[k];
         }
@@ -555,6 +641,7 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
             if ((threadIdx.x & 7u) == 0u) s.cpre[p] = r.hint;
         });
     __syncthreads();
+    if (blk.ids && total) write_blk_record(A, s, n, cap, blk, j, b, ns, total);
     return true;
 }
 
@@ -591,7 +678,7 @@ __device__ __forceinline__ void sample_sub(const Mol* __restrict__ mols, const u
     const uint32_t* xl = nullptr;  // the brick's superset from its classification
     uint32_t xn = hid.bsup.cnt ? hid.bsup.cnt[brick] : NO_LIST;
     if (xn != NO_LIST) xl = hid.bsup.ids + hid.bsup.off[brick];
-    bool ok = gather_sub(A, Lc, P, cap, s, n, tmp4, key, sb, hid.sup, j, xl, xn);
+    bool ok = gather_sub(A, Lc, P, cap, s, n, tmp4, key, sb, hid.sup, j, xl, xn, hid.blk);
     if (!ok) {
         if (threadIdx.x == 0) overflow[atomicAdd(n_overflow, 1u)] = j;
         return;

@@ -1002,6 +1002,7 @@ constexpr uint32_t LDS_EDGES = 512;      // crossing-edge buffer of the LDS pass
 constexpr uint32_t SCRATCH_CHUNK = 1024;  // default workgroups of a scratch pass (one list slice each)
 constexpr uint32_t HID_PER_SUB = 64;      // hidden-candidate pool entries per sampled sub-block
 constexpr uint32_t SUP_PER_SUB = 160;     // refinement-superset pool entries per sampled sub-block
+constexpr uint32_t BLK_PER_SUB = 192;     // blocker-record slot words per sampled sub-block
 constexpr uint32_t BSUP_PER_BRICK = 320;  // brick-superset pool entries per touched brick
 constexpr uint64_t BSUP_MAX = 1ull << 28; // brick-superset pool entries at most
 constexpr uint32_t NO_CAP = 0xffffffffu;
@@ -1695,6 +1696,17 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
         hid.sup.used = &cnt[K_SUP];
         hid.sup.cap = (uint32_t)scap;
         k_fill_u32<<<stride_grid(nsub), 256, 0, s>>>(hid.sup.cnt, NO_LIST, nsub, &cnt[K_NSUB]);
+        HIP_TRY(hipGetLastError());
+        // The sampling's blocker records for the refinement (BlkRecord): a
+        // slot of BLK_PER_SUB words per sampled sub-block, at its dense
+        // index; a record that does not fit is not kept and that
+        // sub-block's refinement builds its own bitsets.
+        hid.blk.slot = soff;
+        hid.blk.words = BLK_PER_SUB;
+        hid.blk.slots = std::max<uint32_t>(nsw, 1u);
+        TRY(X.temp(&hid.blk.cnt, nsub));
+        TRY(X.temp(&hid.blk.ids, (size_t)BLK_PER_SUB * hid.blk.slots));
+        k_fill_u32<<<stride_grid(nsub), 256, 0, s>>>(hid.blk.cnt, NO_LIST, nsub, &cnt[K_NSUB]);
         HIP_TRY(hipGetLastError());
     }
     {
