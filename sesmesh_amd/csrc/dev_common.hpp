@@ -444,18 +444,22 @@ struct HidList {
 };
 
 // The blocker record a sampling workgroup leaves for the refinement of the
-// same sub-block, in the u32 pool of HidList::blk: n_near, n_s, n_c, nw
-// (64-bit words per bitset), then the n_near near-sphere ids by near-list
-// slot, then per kept class S / C candidate, in the refinement's list
-// order (class, f64 total_cmp key at the ball centre, index): its id (class
-// in the top bits, as the refinement's lists hold it), its hint slot, and
-// its blocker bitset over near-list slots (nw words, low half first). The
+// same sub-block, in the pool of HidList::blk (u32 words): n_near, n_s,
+// n_c, nw (64-bit words per bitset), n_v and `same`; then the n_near
+// near-sphere ids by near-list slot; then per kept candidate, classes S, C
+// and V, in the refinement's list order (class, f64 total_cmp key at the
+// ball centre, index): its id (class in the top bits, as the refinement's
+// lists hold it), its hint slot, its key (low half first) and its blocker
+// bitset over near-list slots (nw words, low half first; none for V). The
 // blockers are those the sampling's reach test keeps, a superset of those
-// any refinement point can meet (the test's ball and limits are larger),
-// so the refinement may take them instead of walking the rows, once its
-// own near list and candidates are found to be the same.
-constexpr uint32_t BLK_HEAD = 4;
-__host__ __device__ inline uint32_t blk_entry_words(uint32_t nw) { return 2u + 2u * nw; }
+// any refinement point can meet (the test's ball and limits are larger).
+// `same` is 1 when no item lies between the sampling's reach and the
+// refinement's slightly larger one: the near spheres and candidates are
+// then the refinement's own, so the refinement may take its lists from the
+// record; otherwise it gathers and takes only the bitsets, once its near
+// list and candidates are found to be the record's.
+constexpr uint32_t BLK_HEAD = 6;
+__host__ __device__ inline uint32_t blk_entry_words(uint32_t nw) { return 4u + 2u * nw; }
 constexpr uint32_t NO_LIST = 0xffffffffu;
 constexpr uint32_t KEY_BUCKETS = 16;  // key buckets per candidate class in the gathers
 

@@ -171,6 +171,72 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
     // at least -max_expanded), bucketed from -max_expanded.
     const bool sph = A.kind != K_SES;
     hide = hide && !sph;
+    // A blocker record whose lists are this gather's own (`same`, see
+    // BlkRecord) gives them directly: the near spheres in its slot order
+    // with their records, the inside group tested here, the kept candidates
+    // in list order with their keys, records, bitsets and hints.
+    if (br && hide && br[5] && br[0] <= cap.near && br[1] + br[2] + br[4] <= cap.cand &&
+        br[3] <= cap.near / 64) {
+        const uint32_t nn = br[0], ns = br[1], nc = br[2], nwr = br[3], nv = br[4], total = ns + nc + nv;
+        const uint32_t ew = blk_entry_words(nwr), cc = cap.cand, nw = cap.near / 64;
+        const uint32_t* ent = br + BLK_HEAD + nn;
+        for (uint32_t i = tid; i < nn; i += nt) {
+            const uint32_t k = br[BLK_HEAD + i];
+            const SphereRec S = load_rec(A.sph, k);
+            const SphereThr T = load_rec(A.thr, k);
+            s.nq[i] = NearD{S.x, S.y, S.z, T.blk};
+            s.nin[i] = T.nin;
+            if (ddist(bx, by, bz, S.x, S.y, S.z) < S.R + rho) {
+                const uint32_t q = atomicAdd(&n[R_IN], 1u);
+                if (q >= cap.inside)
+                    n[R_OVER] = 1u;
+                else
+                    s.ins[q] = (uint16_t)i;
+            }
+        }
+        for (uint32_t p = tid; p < total; p += nt) {
+            const uint32_t* e = ent + (size_t)p * ew;
+            const uint32_t k = e[0] & ID_MASK;
+            s.ckey[p] = (unsigned long long)e[2] | ((unsigned long long)e[3] << 32);
+            s.cid[p] = e[0];
+            double* d = s.cdat + (size_t)CDAT * p;
+            if (p < ns) {
+                const SphereRec S = load_rec(A.sph, k);
+                d[0] = S.x;
+                d[1] = S.y;
+                d[2] = S.z;
+                d[3] = S.R;
+                s.cpre[p] = e[1];
+            } else if (p < ns + nc) {
+                const CircleRec C = load_rec(A.cir, k);
+                d[0] = C.mx;
+                d[1] = C.my;
+                d[2] = C.mz;
+                d[3] = C.ax;
+                d[4] = C.ay;
+                d[5] = C.az;
+                d[6] = C.rho;
+                s.cpre[p] = e[1];
+            } else {
+                const VertexRec V = load_rec(A.ver, k);
+                d[0] = V.x;
+                d[1] = V.y;
+                d[2] = V.z;
+            }
+            for (uint32_t w = 0; w < nw; w++)
+                s.cbits[w * cc + p] =
+                    w < nwr ? (unsigned long long)e[4 + 2 * w] | ((unsigned long long)e[5 + 2 * w] << 32) : 0ull;
+        }
+        if (tid == 0) {
+            n[R_NEAR] = nn;
+            n[R_S] = ns;
+            n[R_C] = nc;
+            n[R_V] = nv;
+            n[R_CAND] = total;
+        }
+        __syncthreads();
+        return n[R_OVER] == 0u;
+    }
     const double reach = (sph ? 0.0 : A.probe) + band + rho;
     const double rw = xs.ids ? fmax(reach, reach_x) : reach;  // the walk's reach
     const double radii[3] = {rw + A.max_expanded, rw + A.rho_max, rw};
@@ -480,7 +546,7 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
             for (uint32_t q = tid; q < (ns + nc) * nwr; q += nt) {
                 const uint32_t p = q / nwr, w = q % nwr;
                 const uint32_t* e = ent + (size_t)p * ew;
-                unsigned long long bits = (unsigned long long)e[2 + 2 * w] | ((unsigned long long)e[3 + 2 * w] << 32);
+                unsigned long long bits = (unsigned long long)e[4 + 2 * w] | ((unsigned long long)e[5 + 2 * w] << 32);
                 while (bits) {
                     const uint32_t sl = rslot[w * 64 + (uint32_t)__builtin_ctzll(bits)];
                     bits &= bits - 1ull;

@@ -133,7 +133,9 @@ __device__ inline Lists carve(char* base, const Layout& L) {
 }
 
 // Counters of the list builder.
-enum { C_NEAR = 0, C_IN, C_S, C_C, C_V, C_CAND, C_OVER, C_SUP, C_SUPOVER, C_N };
+// C_DIFF: an item the refinement's gather of the sub-block (its slightly
+// larger ball) would list differently.
+enum { C_NEAR = 0, C_IN, C_S, C_C, C_V, C_CAND, C_OVER, C_SUP, C_SUPOVER, C_DIFF, C_N };
 constexpr uint32_t HIDDEN = 0x80000000u;  // a candidate's bucket word when it is hidden
 
 // True if (qx, qy, qz) lies inside a blocker of candidate p (kernel.rs:
@@ -267,18 +269,19 @@ __device__ inline uint32_t kept_id(const Lists& s, uint32_t ns, uint32_t p) {
 }
 
 // Writes the BlkRecord of sub-block j (ball centre b, the lists' ns class S
-// and total class S + C kept candidates) to the pool, or marks it absent if
-// the pool is full. The candidates are ranked by the refinement's order:
-// their f64 keys at the ball centre computed as gather_refine computes them
-// (|d - R| of a sphere, the circle distance), then the id. By the whole
-// workgroup, after the bitsets; it reuses the near hash as scratch.
+// and total class S + C kept candidates, then the class V ones) to the
+// pool, or marks it absent if it does not fit. The candidates are ranked
+// by the refinement's order: their f64 keys at the ball centre computed as
+// gather_refine computes them (|d - R| of a sphere, the circle distance,
+// the vertex distance), then the id. By the whole workgroup, after the
+// bitsets; it reuses the near hash as scratch.
 __device__ inline void write_blk_record(const DArr& A, const Lists& s, const uint32_t* n, const Caps& cap,
                                         const BlkPool& blk, uint32_t j, const double b[3], uint32_t ns,
                                         uint32_t total) {
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
-    const uint32_t nn = n[C_NEAR], nwr = (nn + 63) / 64, ew = blk_entry_words(nwr), cc = cap.cand;
-    const uint32_t m = BLK_HEAD + nn + total * ew, slot = blk.slot[j];
-    const bool fits = m <= blk.words && slot < blk.slots;  // the same in every thread
+    const uint32_t nn = n[C_NEAR], nv = n[C_V], nwr = (nn + 63) / 64, ew = blk_entry_words(nwr), cc = cap.cand;
+    const uint32_t all = total + nv, m = BLK_HEAD + nn + all * ew, slot = blk.slot[j];
+    const bool fits = m <= blk.words && slot < blk.slots && all <= cap.hash;  // the same in every thread
     if (tid == 0) blk.cnt[j] = fits ? m : NO_LIST;
     if (!fits) return;
     uint32_t* o = blk.ids + (size_t)slot * blk.words;
@@ -287,6 +290,8 @@ __device__ inline void write_blk_record(const DArr& A, const Lists& s, const uin
         o[1] = ns;
         o[2] = total - ns;
         o[3] = nwr;
+        o[4] = nv;
+        o[5] = n[C_DIFF] ? 0u : 1u;
     }
     for (uint32_t i = tid; i < cap.hash; i += nt) {  // near ids by slot, from the near hash
         const uint32_t key = s.hk[i];
@@ -294,10 +299,14 @@ __device__ inline void write_blk_record(const DArr& A, const Lists& s, const uin
     }
     __syncthreads();
     unsigned long long* kk = (unsigned long long*)s.hk;  // the hash is free now
-    for (uint32_t p = tid; p < total; p += nt) {
-        const uint32_t k = kept_id(s, ns, p);
+    auto id_of = [&](uint32_t p) { return p < total ? kept_id(s, ns, p) : __float_as_uint(s.vd[p - total].w); };
+    for (uint32_t p = tid; p < all; p += nt) {
+        const uint32_t k = id_of(p);
         double key;
-        if (p < ns) {
+        if (p >= total) {
+            const VertexRec V = load_rec(A.ver, k);
+            key = ddist(b[0], b[1], b[2], V.x, V.y, V.z);
+        } else if (p < ns) {
             const SphereRec S = load_rec(A.sph, k);
             key = fabs(ddist(b[0], b[1], b[2], S.x, S.y, S.z) - S.R);
         } else {
@@ -315,14 +324,15 @@ __device__ inline void write_blk_record(const DArr& A, const Lists& s, const uin
     __syncthreads();
     uint32_t* ent = o + BLK_HEAD + nn;
     // 8 lanes per candidate count the ones of its class ranked before it.
-    for (uint32_t p = tid / 8; p < total; p += nt / 8) {
-        const bool S = p < ns;
-        const uint32_t lo = S ? 0u : ns, hi = S ? ns : total, k = kept_id(s, ns, p);
+    for (uint32_t p = tid / 8; p < all; p += nt / 8) {
+        const uint32_t cls = p < ns ? 0u : (p < total ? 1u : 2u);
+        const uint32_t lo = cls == 0 ? 0u : (cls == 1 ? ns : total), hi = cls == 0 ? ns : (cls == 1 ? total : all);
+        const uint32_t k = id_of(p);
         const unsigned long long kp = kk[p];
         uint32_t rank = 0;
         for (uint32_t u = lo + (tid & 7u); u < hi; u += 8) {
             const unsigned long long ku = kk[u];
-            rank += (ku < kp || (ku == kp && kept_id(s, ns, u) < k)) ? 1u : 0u;
+            rank += (ku < kp || (ku == kp && id_of(u) < k)) ? 1u : 0u;
         }
         rank += __shfl_xor(rank, 1, 8);
         rank += __shfl_xor(rank, 2, 8);
@@ -330,13 +340,15 @@ __device__ inline void write_blk_record(const DArr& A, const Lists& s, const uin
         uint32_t* e = ent + (size_t)(lo + rank) * ew;
         const uint32_t r = tid & 7u;
         if (r == 0) {
-            e[0] = k | (S ? 0u : 1u << KIND_SHIFT);
-            e[1] = s.cpre[p];
+            e[0] = k | (cls << KIND_SHIFT);
+            e[1] = cls < 2 ? s.cpre[p] : 0xffffffffu;
+            e[2] = (uint32_t)kp;
+            e[3] = (uint32_t)(kp >> 32);
         }
         for (uint32_t w = r; w < nwr; w += 8) {
-            const unsigned long long bits = s.cbits[w * cc + p];
-            e[2 + 2 * w] = (uint32_t)bits;
-            e[3 + 2 * w] = (uint32_t)(bits >> 32);
+            const unsigned long long bits = cls < 2 ? s.cbits[w * cc + p] : 0ull;
+            e[4 + 2 * w] = (uint32_t)bits;
+            e[5 + 2 * w] = (uint32_t)(bits >> 32);
         }
     }
 }
@@ -400,6 +412,11 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
             if (sph ? d - R < reach_r
                     : (d < R + reach_r || (A.status[k] == S_EXPOSED && fabs(d - R) <= reach_r)))
                 sup_add(k);
+            if (!sph) {  // near and class S here and in the refinement's gather
+                const bool ex = A.status[k] == S_EXPOSED, near_s = d < R + reach, near_r = d < R + reach_r;
+                const double lb0 = fabs(d - R);
+                if (near_s != near_r || (near_s && ex && lb0 <= reach) != (ex && lb0 <= reach_r)) n[C_DIFF] = 1u;
+            }
             if (d >= R + reach) return;
             uint32_t at = atomicAdd(&n[C_NEAR], 1u);
             if (at >= cap.near) {
@@ -441,6 +458,7 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
             double rr = C.rho;
             double dist = sqrt(along * along + (wl - rr) * (wl - rr));
             if (dist <= reach_r) sup_add(k | (1u << KIND_SHIFT));
+            if ((dist <= reach) != (dist <= reach_r)) n[C_DIFF] = 1u;
             if (!(dist <= reach)) return;
             atomicAdd(&n[C_C], 1u);
             uint32_t q = atomicAdd(&n[C_CAND], 1u);
@@ -454,13 +472,16 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
             const VertexRec V = load_rec(A.ver, k);
             const double dv = ddist(b[0], b[1], b[2], V.x, V.y, V.z);
             if (dv <= reach_r) sup_add(k | (2u << KIND_SHIFT));
+            if ((dv <= reach) != (dv <= reach_r)) n[C_DIFF] = 1u;
             if (!(dv <= reach)) return;
             uint32_t q = atomicAdd(&n[C_V], 1u);
             if (q >= cap.v) {
                 n[C_OVER] = 1u;
                 return;
             }
-            s.vd[q] = load_f4(Lc.v4, k);
+            float4 v = load_f4(Lc.v4, k);
+            v.w = __uint_as_float(k);  // the lane kernel reads x, y, z
+            s.vd[q] = v;
         }
     };
     if (xl) {

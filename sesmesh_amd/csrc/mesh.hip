@@ -1002,7 +1002,7 @@ constexpr uint32_t LDS_EDGES = 512;      // crossing-edge buffer of the LDS pass
 constexpr uint32_t SCRATCH_CHUNK = 1024;  // default workgroups of a scratch pass (one list slice each)
 constexpr uint32_t HID_PER_SUB = 64;      // hidden-candidate pool entries per sampled sub-block
 constexpr uint32_t SUP_PER_SUB = 160;     // refinement-superset pool entries per sampled sub-block
-constexpr uint32_t BLK_PER_SUB = 192;     // blocker-record slot words per sampled sub-block
+constexpr uint32_t BLK_PER_SUB = 320;     // blocker-record slot words per sampled sub-block
 constexpr uint32_t BSUP_PER_BRICK = 320;  // brick-superset pool entries per touched brick
 constexpr uint64_t BSUP_MAX = 1ull << 28; // brick-superset pool entries at most
 constexpr uint32_t NO_CAP = 0xffffffffu;
