@@ -1081,13 +1081,17 @@ __device__ inline void edge_finish(const DArr& A, const DRun& P, const EdgeState
     EdgeGeom Q = edge_geom(E, P.h);
     double tc = fmin(fmax(t, P.t_min), 1.0 - P.t_min);
     double v[3] = {Q.pa[0] + Q.d[0] * tc, Q.pa[1] + Q.d[1] * tc, Q.pa[2] + Q.d[2] * tc};
-    double ow[3];
-    for (int a = 0; a < 3; a++) ow[a] = (double)E.fa < 0.0 ? Q.d[a] : -Q.d[a];
-    double ol = sqrt(ow[0] * ow[0] + ow[1] * ow[1] + ow[2] * ow[2]);
-    double outward[3] = {ow[0] / ol, ow[1] / ol, ow[2] / ol};
     double nrm[3];
     uint8_t patch = 0;
     uint32_t atom = 0;
+    // The edge's outward direction, the normal where the closest point
+    // gives none (computed only there).
+    auto outward = [&]() {
+        double ow[3];
+        for (int a = 0; a < 3; a++) ow[a] = (double)E.fa < 0.0 ? Q.d[a] : -Q.d[a];
+        double ol = sqrt(ow[0] * ow[0] + ow[1] * ow[1] + ow[2] * ow[2]);
+        for (int a = 0; a < 3; a++) nrm[a] = ow[a] / ol;
+    };
     auto unit = [&](double wx, double wy, double wz) {
         double len = sqrt(wx * wx + wy * wy + wz * wz);
         if (len > 1e-9) {
@@ -1095,9 +1099,7 @@ __device__ inline void edge_finish(const DArr& A, const DRun& P, const EdgeState
             nrm[1] = wy / len;
             nrm[2] = wz / len;
         } else {
-            nrm[0] = outward[0];
-            nrm[1] = outward[1];
-            nrm[2] = outward[2];
+            outward();
         }
     };
     auto nearest = [&](const uint32_t* atoms, int cnt) {
@@ -1105,7 +1107,8 @@ __device__ inline void edge_finish(const DArr& A, const DRun& P, const EdgeState
         uint32_t bk = atoms[0];
         for (int q = 0; q < cnt; q++) {
             uint32_t k = atoms[q];
-            double dd = ddist(v[0], v[1], v[2], A.cx[k], A.cy[k], A.cz[k]) - A.r[k];
+            const SphereRec S = load_rec(A.sph, k);  // the centre's x, y, z in one load
+            double dd = ddist(v[0], v[1], v[2], S.x, S.y, S.z) - A.r[k];
             if (dd < bd) {
                 bd = dd;
                 bk = k;
@@ -1134,9 +1137,7 @@ __device__ inline void edge_finish(const DArr& A, const DRun& P, const EdgeState
     } else {
         // sphere_eval(v, probe + band + 8) over the whole grid (point.rs:742-756),
         // first minimum in scan order (rare).
-        nrm[0] = outward[0];
-        nrm[1] = outward[1];
-        nrm[2] = outward[2];
+        outward();
         patch = 0;
         double cutoff = A.probe + P.band + 8.0;
         double bv = cutoff;

@@ -274,8 +274,9 @@ __device__ inline uint32_t kept_id(const Lists& s, uint32_t ns, uint32_t p) {
 // by the refinement's order: their f64 keys at the ball centre computed as
 // gather_refine computes them (|d - R| of a sphere, the circle distance,
 // the vertex distance), then the id. By the whole workgroup, after the
-// bitsets; it reuses the near hash as scratch.
-__device__ inline void write_blk_record(const DArr& A, const Lists& s, const uint32_t* n, const Caps& cap,
+// bitsets; it reuses the near hash as scratch. True (in every thread) if
+// it wrote a record whose lists are the refinement's own.
+__device__ inline bool write_blk_record(const DArr& A, const Lists& s, const uint32_t* n, const Caps& cap,
                                         const BlkPool& blk, uint32_t j, const double b[3], uint32_t ns,
                                         uint32_t total) {
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
@@ -283,7 +284,7 @@ __device__ inline void write_blk_record(const DArr& A, const Lists& s, const uin
     const uint32_t all = total + nv, m = BLK_HEAD + nn + all * ew, slot = blk.slot[j];
     const bool fits = m <= blk.words && slot < blk.slots && all <= cap.hash;  // the same in every thread
     if (tid == 0) blk.cnt[j] = fits ? m : NO_LIST;
-    if (!fits) return;
+    if (!fits) return false;
     uint32_t* o = blk.ids + (size_t)slot * blk.words;
     if (tid == 0) {
         o[0] = nn;
@@ -351,6 +352,7 @@ __device__ inline void write_blk_record(const DArr& A, const Lists& s, const uin
             e[5 + 2 * w] = (uint32_t)(bits >> 32);
         }
     }
+    return n[C_DIFF] == 0u;
 }
 
 // Gathers the lists of sub-block `sb` of brick `key` (kernel.rs:553-751).
@@ -364,7 +366,8 @@ __device__ inline void write_blk_record(const DArr& A, const Lists& s, const uin
 __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, const Caps& cap, Lists& s,
                                   uint32_t* n, uint32_t* tmp4, const int32_t* key, uint32_t sb,
                                   const SupList& sup, uint32_t j, const uint32_t* xl, uint32_t xn,
-                                  const BlkPool& blk) {
+                                  const BlkPool& blk, bool& same) {
+    same = false;
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     __shared__ uint32_t bfill[WAVE];
     for (uint32_t i = tid; i < cap.hash; i += nt) s.hk[i] = 0u;
@@ -659,7 +662,7 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
             if ((threadIdx.x & 7u) == 0u) s.cpre[p] = r.hint;
         });
     __syncthreads();
-    if (blk.ids && total) write_blk_record(A, s, n, cap, blk, j, b, ns, total);
+    same = blk.ids && total && write_blk_record(A, s, n, cap, blk, j, b, ns, total);
     return true;
 }
 
@@ -696,13 +699,14 @@ __device__ __forceinline__ void sample_sub(const Mol* __restrict__ mols, const u
     const uint32_t* xl = nullptr;  // the brick's superset from its classification
     uint32_t xn = hid.bsup.cnt ? hid.bsup.cnt[brick] : NO_LIST;
     if (xn != NO_LIST) xl = hid.bsup.ids + hid.bsup.off[brick];
-    bool ok = gather_sub(A, Lc, P, cap, s, n, tmp4, key, sb, hid.sup, j, xl, xn, hid.blk);
+    bool same;  // a blocker record that gives the refinement its lists
+    bool ok = gather_sub(A, Lc, P, cap, s, n, tmp4, key, sb, hid.sup, j, xl, xn, hid.blk, same);
     if (!ok) {
         if (threadIdx.x == 0) overflow[atomicAdd(n_overflow, 1u)] = j;
         return;
     }
     if (loads && threadIdx.x == 0) atomicMax(&loads[brick], candidate_load(n[C_CAND], n[C_NEAR], n[C_V], n[C_IN]));
-    if (hid.ids) {  // the hidden candidates, for this sub-block's refinement
+    if (hid.ids && !same) {  // the hidden candidates, for this sub-block's refinement
         __shared__ uint32_t h_at, h_fill;
         const uint32_t gathered = n[C_CAND], nh = gathered - n[C_S] - n[C_C];
         if (threadIdx.x == 0) {
