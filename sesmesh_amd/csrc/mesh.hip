@@ -1009,9 +1009,7 @@ constexpr uint32_t NO_CAP = 0xffffffffu;
 
 // A list pool of `items` fixed slots of `per` entries (PoolSlots) and a
 // shared room after them of `extra` entries per item, within `limit`
-// entries in all; `cap` is the room's size, `total` the pool's. The room is
-// as large as the slots: at coarse spacings most lists outgrow their slot,
-// and the room then holds what one shared pool of `per` per item held.
+// entries in all; `cap` is the room's size, `total` the pool's.
 PoolSlots pool_slots(const uint32_t* slot, uint32_t per, uint64_t items, uint64_t extra, uint64_t limit,
                      uint32_t& cap, uint64_t& total) {
     items = std::max<uint64_t>(items, 1);
@@ -1524,7 +1522,7 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
             TRY(X.temp(&bsup_t.off, E.dense));
             TRY(X.temp(&bsup_t.cnt, E.dense));
             uint64_t btotal;
-            bsup_t.fixed = pool_slots(nullptr, BSUP_PER_BRICK, ntc, BSUP_PER_BRICK, 2 * BSUP_MAX, bsup_t.cap, btotal);
+            bsup_t.fixed = pool_slots(nullptr, BSUP_PER_BRICK, ntc, BSUP_PER_BRICK / 4, BSUP_MAX, bsup_t.cap, btotal);
             TRY(X.temp(&bsup_t.ids, btotal));
             bsup_t.used = &cnt[K_BSUP];
             k_fill_u32<<<stride_grid(E.dense), 256, 0, s>>>(bsup_t.cnt, NO_LIST, 0, &cnt[K_TOUCHED]);
@@ -1694,7 +1692,7 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     TRY(X.temp(&hid.off, nsub));
     TRY(X.temp(&hid.cnt, nsub));
     uint64_t htotal;
-    hid.fixed = pool_slots(soff, HID_PER_SUB, nsw, HID_PER_SUB, 0xfffffff0ull, hid.cap, htotal);
+    hid.fixed = pool_slots(soff, HID_PER_SUB, nsw, HID_PER_SUB / 4, 0xfffffff0ull, hid.cap, htotal);
     TRY(X.temp(&hid.ids, htotal));
     hid.used = &cnt[K_HID];
     k_fill_u32<<<stride_grid(nsub), 256, 0, s>>>(hid.cnt, NO_LIST, nsub, &cnt[K_NSUB]);
@@ -1706,7 +1704,7 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
         TRY(X.temp(&hid.sup.off, nsub));
         TRY(X.temp(&hid.sup.cnt, nsub));
         uint64_t stotal;
-        hid.sup.fixed = pool_slots(soff, SUP_PER_SUB, nsw, SUP_PER_SUB, 0xfffffff0ull, hid.sup.cap, stotal);
+        hid.sup.fixed = pool_slots(soff, SUP_PER_SUB, nsw, SUP_PER_SUB / 4, 0xfffffff0ull, hid.sup.cap, stotal);
         TRY(X.temp(&hid.sup.ids, stotal));
         hid.sup.used = &cnt[K_SUP];
         k_fill_u32<<<stride_grid(nsub), 256, 0, s>>>(hid.sup.cnt, NO_LIST, nsub, &cnt[K_NSUB]);
