@@ -382,29 +382,6 @@ __device__ inline uint32_t hash_get(const uint32_t* hk, const uint32_t* hv, uint
 
 constexpr uint32_t KIND_SHIFT = 30, ID_MASK = (1u << KIND_SHIFT) - 1u;
 
-// Where a list goes in its pool. Item j first tries a fixed slot of
-// `per` entries at slot index slot[j] (j itself when slot is null, below
-// `slots`), which costs no atomic: one counter that every workgroup of a
-// pass increments is contended. A list that does not fit its slot takes
-// room after the slots (slots * per onwards, `cap` entries) through the
-// pool's counter. per 0: only the counter.
-struct PoolSlots {
-    const uint32_t* slot;
-    uint32_t per, slots;
-};
-
-// The pool position of item j's list of m entries, or NO_LIST when the
-// pool is full.
-__device__ inline uint32_t pool_take(const PoolSlots& f, uint32_t* used, uint32_t cap, uint32_t j, uint32_t m) {
-    const uint32_t base = f.per * f.slots;
-    if (f.per) {
-        const uint32_t sl = f.slot ? f.slot[j] : j;
-        if (m <= f.per && sl < f.slots) return sl * f.per;
-    }
-    const uint32_t at = atomicAdd(used, m);
-    return at > cap || m > cap - at ? 0xffffffffu : base + at;
-}
-
 // The candidates the sampling gather dropped as hidden, per sampled
 // sub-block j: cnt[j] entries (candidate id | class << KIND_SHIFT, class 0
 // for S, 1 for C) at ids + off[j]; cnt[j] is NO_LIST where there is no list
@@ -422,7 +399,6 @@ __device__ inline uint32_t pool_take(const PoolSlots& f, uint32_t* used, uint32_
 struct SupList {
     uint32_t *off, *cnt, *ids, *used;
     uint32_t cap;
-    PoolSlots fixed;
 };
 // Fixed-size slots of the blocker records: sampled sub-block j (dense
 // index slot[j] among the sampled ones, below `slots`) has `words` words at
@@ -437,7 +413,6 @@ struct BlkPool {
 struct HidList {
     uint32_t *off, *cnt, *ids, *used;
     uint32_t cap;
-    PoolSlots fixed;
     SupList sup;   // handed over the same way
     SupList bsup;  // per surface brick: the classification's superset of its sampling walks (see gather_refine)
     BlkPool blk;   // per sampled sub-block: its blocker record (see BlkRecord)

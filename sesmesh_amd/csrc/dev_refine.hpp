@@ -370,7 +370,10 @@ __device__ inline bool gather_refine(const DArr& A, const RCaps& cap, RLists& s,
         const uint32_t m = n[R_X];
         if (tid == 0) {
             uint32_t at = NO_LIST;
-            if (!n[R_XOVER]) at = pool_take(xs.fixed, xs.used, xs.cap, xj, m);
+            if (!n[R_XOVER]) {
+                at = atomicAdd(xs.used, m);
+                if (at > xs.cap || m > xs.cap - at) at = NO_LIST;
+            }
             x_at = at;
             xs.off[xj] = at == NO_LIST ? 0u : at;
             xs.cnt[xj] = at == NO_LIST ? NO_LIST : m;

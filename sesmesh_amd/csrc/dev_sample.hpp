@@ -502,7 +502,10 @@ __device__ inline bool gather_sub(const DArr& A, const DLoc& Lc, const DRun& P, 
         const uint32_t m = n[C_SUP];
         if (threadIdx.x == 0) {
             uint32_t at = NO_LIST;
-            if (!n[C_SUPOVER]) at = pool_take(sup.fixed, sup.used, sup.cap, j, m);
+            if (!n[C_SUPOVER]) {
+                at = atomicAdd(sup.used, m);
+                if (at > sup.cap || m > sup.cap - at) at = NO_LIST;
+            }
             sup_at = at;
             sup.off[j] = at == NO_LIST ? 0u : at;
             sup.cnt[j] = at == NO_LIST ? NO_LIST : m;
@@ -710,8 +713,8 @@ __device__ __forceinline__ void sample_sub(const Mol* __restrict__ mols, const u
         __shared__ uint32_t h_at, h_fill;
         const uint32_t gathered = n[C_CAND], nh = gathered - n[C_S] - n[C_C];
         if (threadIdx.x == 0) {
-            const uint32_t at = pool_take(hid.fixed, hid.used, hid.cap, j, nh);
-            h_at = at;
+            const uint32_t at = atomicAdd(hid.used, nh);
+            h_at = at <= hid.cap && nh <= hid.cap - at ? at : NO_LIST;
             h_fill = 0;
             if (h_at != NO_LIST) {
                 hid.off[j] = at;

@@ -1007,18 +1007,6 @@ constexpr uint32_t BSUP_PER_BRICK = 320;  // brick-superset pool entries per tou
 constexpr uint64_t BSUP_MAX = 1ull << 28; // brick-superset pool entries at most
 constexpr uint32_t NO_CAP = 0xffffffffu;
 
-// A list pool of `items` fixed slots of `per` entries (PoolSlots) and a
-// shared room after them of `extra` entries per item, within `limit`
-// entries in all; `cap` is the room's size, `total` the pool's.
-PoolSlots pool_slots(const uint32_t* slot, uint32_t per, uint64_t items, uint64_t extra, uint64_t limit,
-                     uint32_t& cap, uint64_t& total) {
-    items = std::max<uint64_t>(items, 1);
-    const uint64_t slots = std::min<uint64_t>(items, limit / (per + extra));
-    cap = (uint32_t)std::min<uint64_t>(extra * items, limit - slots * per);
-    total = slots * per + cap;
-    return PoolSlots{slot, per, (uint32_t)slots};
-}
-
 // Device counters of a run (u32 slots, zero at the start of the run).
 enum : uint32_t {
     K_TOUCHED, K_NB, K_NSUB, K_NSW, K_NV, K_NRW, K_NT, K_FLAGS, K_ERR,
@@ -1521,10 +1509,10 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
         if (!X.arena) {
             TRY(X.temp(&bsup_t.off, E.dense));
             TRY(X.temp(&bsup_t.cnt, E.dense));
-            uint64_t btotal;
-            bsup_t.fixed = pool_slots(nullptr, BSUP_PER_BRICK, ntc, BSUP_PER_BRICK / 4, BSUP_MAX, bsup_t.cap, btotal);
-            TRY(X.temp(&bsup_t.ids, btotal));
+            const uint64_t bcap = std::min<uint64_t>((uint64_t)BSUP_PER_BRICK * std::max<uint32_t>(ntc, 1u), BSUP_MAX);
+            TRY(X.temp(&bsup_t.ids, bcap));
             bsup_t.used = &cnt[K_BSUP];
+            bsup_t.cap = (uint32_t)bcap;
             k_fill_u32<<<stride_grid(E.dense), 256, 0, s>>>(bsup_t.cnt, NO_LIST, 0, &cnt[K_TOUCHED]);
             HIP_TRY(hipGetLastError());
         }
@@ -1691,10 +1679,10 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     hid.bsup = bsup_s;
     TRY(X.temp(&hid.off, nsub));
     TRY(X.temp(&hid.cnt, nsub));
-    uint64_t htotal;
-    hid.fixed = pool_slots(soff, HID_PER_SUB, nsw, HID_PER_SUB / 4, 0xfffffff0ull, hid.cap, htotal);
-    TRY(X.temp(&hid.ids, htotal));
+    const uint64_t hcap = std::min<uint64_t>((uint64_t)HID_PER_SUB * std::max<uint32_t>(nsw, 1u), 0xfffffff0ull);
+    TRY(X.temp(&hid.ids, hcap));
     hid.used = &cnt[K_HID];
+    hid.cap = (uint32_t)hcap;
     k_fill_u32<<<stride_grid(nsub), 256, 0, s>>>(hid.cnt, NO_LIST, nsub, &cnt[K_NSUB]);
     HIP_TRY(hipGetLastError());
     // The refinement's candidate supersets (SupList), SUP_PER_SUB entries per
@@ -1703,10 +1691,10 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     if (!X.arena) {
         TRY(X.temp(&hid.sup.off, nsub));
         TRY(X.temp(&hid.sup.cnt, nsub));
-        uint64_t stotal;
-        hid.sup.fixed = pool_slots(soff, SUP_PER_SUB, nsw, SUP_PER_SUB / 4, 0xfffffff0ull, hid.sup.cap, stotal);
-        TRY(X.temp(&hid.sup.ids, stotal));
+        const uint64_t scap = std::min<uint64_t>((uint64_t)SUP_PER_SUB * std::max<uint32_t>(nsw, 1u), 0xfffffff0ull);
+        TRY(X.temp(&hid.sup.ids, scap));
         hid.sup.used = &cnt[K_SUP];
+        hid.sup.cap = (uint32_t)scap;
         k_fill_u32<<<stride_grid(nsub), 256, 0, s>>>(hid.sup.cnt, NO_LIST, nsub, &cnt[K_NSUB]);
         HIP_TRY(hipGetLastError());
         // The sampling's blocker records for the refinement (BlkRecord): a
