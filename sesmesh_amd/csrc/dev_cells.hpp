@@ -27,6 +27,23 @@ __device__ inline void load_tables(McTables& t) {
     __syncthreads();
 }
 
+// A brick's cell index c (< b^3) split into (x, y, z), x fastest, without
+// integer division: c / d as __umulhi(c, ceil(2^32 / d)), exact for c and d
+// below 2^16 (b <= 32 gives c < 32768 and d = b^2 <= 1024).
+struct CellSplit {
+    uint32_t b, bb, m1, m2;
+};
+__host__ __device__ inline CellSplit cell_split(uint32_t b) {
+    const uint64_t one = 1ull << 32;
+    return CellSplit{b, b * b, (uint32_t)((one + b - 1) / b), (uint32_t)((one + b * b - 1) / (b * b))};
+}
+__device__ inline void split_cell(const CellSplit& S, uint32_t c, uint32_t& x, uint32_t& y, uint32_t& z) {
+    z = __umulhi(c, S.m2);
+    const uint32_t r = c - z * S.bb;
+    y = __umulhi(r, S.m1);
+    x = r - y * S.b;
+}
+
 __device__ inline uint32_t cell_case(const float* val, uint32_t side, uint32_t x, uint32_t y, uint32_t z) {
     uint32_t i0 = x + side * (y + side * z), cs = 0;
     for (uint32_t c = 0; c < 8; c++) {
@@ -64,12 +81,21 @@ __global__ void __launch_bounds__(256) k_edges(uint32_t b, const float* values, 
         uint32_t nid = 3 * b * b * b, lane = lane_id(), wv = threadIdx.x / WAVE;
         if (threadIdx.x < 8) sc[threadIdx.x] = 0;
         __syncthreads();
+        const CellSplit S = cell_split(b);
         for (uint32_t w = wv; w < words; w += blockDim.x / WAVE) {
-            uint32_t id = w * 64 + lane, x = 0, y = 0, z = 0, axis = 0;
-            bool c = id < nid && edge_crossing(val, b, side, id, x, y, z, axis);
+            const uint32_t id = w * 64 + lane;
+            bool c = false;
+            uint32_t x = 0, y = 0, z = 0;
+            if (id < nid) {  // edge ((z b + y) b + x) 3 + axis (edge_crossing)
+                const uint32_t axis = id % 3u;
+                split_cell(S, id / 3u, x, y, z);
+                const uint32_t i0 = x + side * (y + side * z);
+                const uint32_t st = axis == 0 ? 1u : (axis == 1 ? side : side * side);
+                c = (val[i0] < 0.0f) != (val[i0 + st] < 0.0f);
+            }
             unsigned long long m = __ballot(c);
-            if (c) {
-                uint32_t s = min(x / sub, 1u) + 2 * min(y / sub, 1u) + 4 * min(z / sub, 1u);
+            if (c) {  // x, y, z < b = 2 sub: min(v / sub, 1) is v >= sub
+                uint32_t s = (x >= sub ? 1u : 0u) + (y >= sub ? 2u : 0u) + (z >= sub ? 4u : 0u);
                 atomicAdd(&sc[s], 1u);
             }
             if (lane == 0) {
@@ -79,7 +105,9 @@ __global__ void __launch_bounds__(256) k_edges(uint32_t b, const float* values, 
         }
         uint32_t mine = 0;
         for (uint32_t c = threadIdx.x; c < b * b * b; c += blockDim.x) {
-            const uint32_t cs = cell_case(val, side, c % b, (c / b) % b, c / (b * b));
+            uint32_t x, y, z;
+            split_cell(S, c, x, y, z);
+            const uint32_t cs = cell_case(val, side, x, y, z);
             mine += toff[cs + 1] - toff[cs];
         }
         uint32_t total;
@@ -175,9 +203,12 @@ __global__ void __launch_bounds__(256) k_cell_emit(const Mol* __restrict__ mols,
         __syncthreads();
         uint32_t b = P.b, ncell = b * b * b, per = (ncell + blockDim.x - 1) / blockDim.x;
         uint32_t c_lo = threadIdx.x * per, c_hi = min(ncell, (threadIdx.x + 1) * per);
+        const CellSplit S = cell_split(b);
         uint32_t mine = 0;
         for (uint32_t c = c_lo; c < c_hi; c++) {
-            uint32_t cs = cell_case(val, P.side, c % b, (c / b) % b, c / (b * b));
+            uint32_t x, y, z;
+            split_cell(S, c, x, y, z);
+            uint32_t cs = cell_case(val, P.side, x, y, z);
             mine += tab.off[cs + 1] - tab.off[cs];
         }
         uint32_t total;
@@ -208,13 +239,16 @@ __global__ void __launch_bounds__(256) k_cell_emit(const Mol* __restrict__ mols,
             uint32_t* slots = (uint32_t*)(cpre + 8 * words);
             uint32_t k = first;
             for (uint32_t c = c_lo; c < c_hi; c++) {
-                uint32_t cs = cell_case(val, P.side, c % b, (c / b) % b, c / (b * b));
+                uint32_t x, y, z;
+                split_cell(S, c, x, y, z);
+                uint32_t cs = cell_case(val, P.side, x, y, z);
                 for (uint32_t t = tab.off[cs]; t < tab.off[cs + 1]; t++) slots[k++] = (c << 10) | t;
             }
             __syncthreads();
             for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
                 const uint32_t c = slots[i] >> 10, t = slots[i] & 1023u;
-                const uint32_t x = c % b, y = (c / b) % b, z = c / (b * b);
+                uint32_t x, y, z;
+                split_cell(S, c, x, y, z);
                 const size_t at = base + i;
                 for (int q = 0; q < 3; q++) {
                     const uint32_t vi = corner(x, y, z, tab.edges[3 * t + q]);
@@ -224,7 +258,8 @@ __global__ void __launch_bounds__(256) k_cell_emit(const Mol* __restrict__ mols,
         } else {
             size_t at = base + first;
             for (uint32_t c = c_lo; c < c_hi; c++) {
-                uint32_t x = c % b, y = (c / b) % b, z = c / (b * b);
+                uint32_t x, y, z;
+                split_cell(S, c, x, y, z);
                 uint32_t cs = cell_case(val, P.side, x, y, z);
                 for (uint32_t t = tab.off[cs]; t < tab.off[cs + 1]; t++) {
                     for (int q = 0; q < 3; q++) {

@@ -29,18 +29,6 @@
 
 namespace smg {
 
-__device__ inline bool edge_crossing(const float* val, uint32_t b, uint32_t side, uint32_t id, uint32_t& x,
-                                     uint32_t& y, uint32_t& z, uint32_t& axis) {
-    axis = id % 3u;
-    uint32_t cell = id / 3u;
-    x = cell % b;
-    y = (cell / b) % b;
-    z = cell / (b * b);
-    uint32_t i0 = x + side * (y + side * z);
-    uint32_t st = axis == 0 ? 1u : (axis == 1 ? side : side * side);
-    return (val[i0] < 0.0f) != (val[i0 + st] < 0.0f);
-}
-
 __device__ inline uint32_t edge_rank(const unsigned long long* mask, const uint32_t* wpre, uint32_t words,
                                      uint32_t brick, uint32_t id) {
     size_t w = (size_t)brick * words + id / 64;
