@@ -10,13 +10,15 @@ arrangement resident in HBM (its host build is reported, not timed).
 
 The timed steps replay a capture-ready plan: the whole mesh stage recorded
 once as a hipGraph over buffers sized once (the serving path); every step
-runs every kernel. Also measured: the plain path (one smg_mesh_new per
-mesh), the autotuner's launch shapes, batched throughput on seeded
-molecules of 20-5000 atoms (molecules/s: per-molecule launches, one batched
-pass, a replayed plan), the learned-sizing study, and, unless
---no-cpu-baseline, the CPU oracle (a scalar C++ restatement of the
-reference CPU back end) on a bounded sample of the same workload with all
-host threads.
+runs every kernel. The plan's capacities come from the learned sizing
+model, trained first on 10^6 labelled bricks of other molecules, instead
+of an exact sizing run (with --no-sizing, from an exact run). Also
+measured: the plain path (one smg_mesh_new per mesh), the autotuner's
+launch shapes, batched throughput on seeded molecules of 20-5000 atoms
+(molecules/s: per-molecule launches, one batched pass, a replayed plan),
+the learned-sizing study, and, unless --no-cpu-baseline, the CPU oracle (a
+scalar C++ restatement of the reference CPU back end) on the same workload
+with all host threads.
 
 Prints one JSON line.
 """
@@ -306,23 +308,15 @@ def batched(dev, count, seed, repeats=2):
             "speedup": min(plain_s) / min(batch_s), "plan_speedup": min(plain_s) / min(plan_s)}
 
 
-def sizing_study(dev, rings, brick, target=1_000_000, max_molecules=400):
-    """Learned sizing at its stated size. The model is trained on the device
-    on at least `target` touched bricks of seeded globules (atoms
-    log-uniform in 100-60000, spacing 0.25-1.0 A), labelled by the exact
-    classifier and gathers; then applied to the 58k-atom assembly, which it
-    has not seen: (1) plan capacities predicted by the model instead of an
-    exact sizing run, timed to the first mesh, with the fallback count; (2)
-    the assembly at 0.25 A meshed with and without the model's tier
-    placement, byte for byte equal; (3) the same at 1.0 A, where lists
-    overflow the first tier. Default launch shapes (a tuning chosen for one
-    spacing can hide the effect); the device's tuning is restored after."""
+def train_sizing_model(dev, target=1_000_000, max_molecules=400):
+    """The learned sizing model at its stated size, trained on the device on
+    at least `target` touched bricks of seeded globules (atoms log-uniform in
+    100-60000, spacing 0.25-1.0 A), labelled by the exact classifier and
+    gathers. Returns (model, training record)."""
     from concurrent.futures import ThreadPoolExecutor
 
     import sesmesh_amd
 
-    tuning = dev.tuning
-    dev.tuning = None
     model = sesmesh_amd.Model(dev, seed=7)
     rng = np.random.default_rng(5)
     spacings = (0.25, 0.35, 0.5, 0.75, 1.0)
@@ -344,8 +338,23 @@ def sizing_study(dev, rings, brick, target=1_000_000, max_molecules=400):
     before = model.train(epochs=0, batch=4096, lr=1e-3)
     after = model.train(epochs=30, batch=4096, lr=2e-3)
     progress(f"sizing study: trained in {after['train_ms']:.0f} ms, accuracy {after['occupancy_accuracy']:.3f}")
-    out = {"training": after, "untrained": before, "collect_s": collect_s, "molecules": mols,
-           "atoms": atoms_total}
+    return model, {"training": after, "untrained": before, "collect_s": collect_s, "molecules": mols,
+                   "atoms": atoms_total}
+
+
+def sizing_study(dev, rings, brick, model, trained):
+    """The trained sizing model applied to the 58k-atom assembly, which it
+    has not seen: (1) plan capacities predicted by the model instead of an
+    exact sizing run, timed to the first mesh, with the fallback count; (2)
+    the assembly at 0.25 A meshed with and without the model's tier
+    placement, byte for byte equal; (3) the same at 1.0 A, where lists
+    overflow the first tier. Default launch shapes (a tuning chosen for one
+    spacing can hide the effect); the device's tuning is restored after."""
+    import sesmesh_amd
+
+    tuning = dev.tuning
+    dev.tuning = None
+    out = dict(trained)
     try:
         # (1) Capacities for the unseen assembly: predicted vs an exact sizing run.
         h = 0.25
@@ -425,7 +434,8 @@ def main():
     ap.add_argument("--no-tune", dest="tune", action="store_false",
                     help="skip the autotuner (default launch shapes)")
     ap.add_argument("--no-sizing", dest="sizing", action="store_false",
-                    help="skip the learned-sizing study (training on seeded globules, 0.75 / 1.0 A meshes)")
+                    help="skip the learned sizing (training on seeded globules; the timed plan is then sized by an "
+                         "exact run) and its study")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the meshes, one device-to-device copy of this many MiB (the known-size kernel "
                          "that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -471,12 +481,30 @@ def main():
         tuned = sesmesh_amd.tune([res], [spacing], brick=brick, repeats=2)
         progress(f"autotuner: {tuned['tuning']} ({tuned['ms']:.2f} ms device time)")
 
+    # The learned sizing model (config 5 at its stated size), trained before
+    # the timed steps so that it sizes the headline plan.
+    model, trained = None, None
+    if args.sizing:
+        progress("sizing model: collecting and training")
+        model, trained = train_sizing_model(dev)
+        progress(f"sizing model: accuracy {trained['training']['occupancy_accuracy']:.3f}, "
+                 f"load rmse {trained['training']['load_rmse']:.3f}")
+
     # Timed steps: the capture-ready plan (buffers sized once, the whole
     # pipeline replayed as a hipGraph each step, phase events recorded in
-    # the graph). Warm-up builds the plan and records the graph.
-    m = res.mesh(spacing, brick)
-    plan = sesmesh_amd.Plan(dev, sesmesh_amd.Plan.caps_for([res], [spacing], mesh=m, margin=0.05))
-    del m
+    # the graph). Its capacities come from the sizing model's predictions
+    # for this unseen molecule (no exact sizing run; an overflow would be
+    # redone exactly and grow the plan, and is counted), or without the
+    # model from an exact run. Warm-up builds the plan and records the graph.
+    if model is not None:
+        caps, estimate = sesmesh_amd.Plan.caps_predict([res], [spacing], model, brick=brick, margin=0.15)
+        sizing = {"by": "learned sizing model (smg_plan_caps_predict, margin 0.15)", "estimate": estimate}
+    else:
+        m = res.mesh(spacing, brick)
+        caps = sesmesh_amd.Plan.caps_for([res], [spacing], mesh=m, margin=0.05)
+        del m
+        sizing = {"by": "an exact sizing run (smg_plan_caps_for, margin 0.05)"}
+    plan = sesmesh_amd.Plan(dev, caps)
     for _ in range(max(2, args.warmup)):
         v = plan.run([res], [spacing])
         del v
@@ -493,10 +521,12 @@ def main():
     progress(f"GPU: {s_per_mesh:.4f} s/mesh over {args.steps} steps (plan replays {stats['replays']})")
     ph = {k: float(np.median([p[k] for p in phase])) for k in phase[0]}
 
-    # Untimed checks: the replayed plan never fell back to the exact path,
+    # Untimed checks: the replayed plan never fell back to the exact path
+    # (capacities from an exact run must hold; predicted ones are reported),
     # and its mesh equals the plain path's byte for byte (samples included).
-    if stats["fallbacks"] != 0:
+    if stats["fallbacks"] != 0 and model is None:
         raise RuntimeError(f"plan fell back to the exact path: {stats}")
+    sizing["fallbacks"] = stats["fallbacks"]
     v = plan.run([res], [spacing])
     got = v.arrays(values=True)
     del v
@@ -574,12 +604,10 @@ def main():
                  f"{batch['plan']['molecules_per_s']:.1f} replayed plan; "
                  f"{batch['speedup']:.2f}x / {batch['plan_speedup']:.2f}x the plain per-molecule launches")
 
-    sizing = None
-    if args.sizing:
-        progress("sizing study: collecting and training")
-        sizing = sizing_study(dev, res, brick)
-        progress(f"sizing study: accuracy {sizing['training']['occupancy_accuracy']:.3f}, "
-                 f"load rmse {sizing['training']['load_rmse']:.3f}")
+    study = None
+    if model is not None:
+        progress("sizing study: predicted capacities and tier placement")
+        study = sizing_study(dev, res, brick, model, trained)
 
     out = {
         "metric": METRIC, "value": s_per_mesh, "unit": "s/mesh", "n_gpus": 1, "steps": args.steps,
@@ -594,8 +622,9 @@ def main():
         "roofline": roofline,
         "cpu_baseline": ({k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")} if cpu else None),
         "best_s": min(times), "median_s": float(np.median(times)),
-        "path": ("capture-ready plan: whole mesh stage replayed as a hipGraph per step, buffers sized once, "
-                 "autotuned launch shapes"),
+        "path": ("capture-ready plan: whole mesh stage replayed as a hipGraph per step, buffers sized once by "
+                 + sizing["by"] + ", autotuned launch shapes"),
+        "plan_sizing": sizing,
         "plain_path_s": plain_s, "speedup_vs_plain_path": plain_s / s_per_mesh if plain_s else None,
         "tuning": tuned["tuning"], "plan_stats": stats,
         "phases_ms": ph, "kernels": others, "hbm": hbm_rates(ph), "valu_issue": valu_issue(ph),
@@ -603,7 +632,7 @@ def main():
         "arrangement": {"device_ms": arr_info["build_ms"], "built_on_device": arr_info["built_on_device"],
                         "host_s": t_arr, "host_threads": host_threads(),
                         "what": "SesField::new + upload: smg_resident_build on the device against the host builder"},
-        "batched": batch, "sizing_study": sizing, "precision_study": precision, "device": dev.name,
+        "batched": batch, "sizing_study": study, "precision_study": precision, "device": dev.name,
     }
     if cpu:
         out["speedup_vs_cpu_baseline"] = cpu["value"] / s_per_mesh
