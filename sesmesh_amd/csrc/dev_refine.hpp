@@ -832,6 +832,21 @@ __device__ inline void scan_class(const DArr& A, const RLists& s, uint32_t cc, u
     }
 }
 
+// The scans' bound on the point's offset o from the gathered ball's centre:
+// a scan stops at the first candidate whose key (its value at the centre)
+// exceeds best + bound, which is sound for every bound of at least
+// |o| + 1e-7 (the values are 1-Lipschitz; 1e-7 A covers the rounding). It is
+// taken in f32 with the hardware square root (within 1 ulp) and scaled by
+// 1 + 1.0e-5, far above the f32 roundings of the square, the root and the
+// product (together below 2.2e-7), then 1.01e-7 is added in f64: the result
+// is never below the f64 sqrt(|o|^2) + 1e-7, so a scan stops no earlier
+// than with that bound and takes the same candidate. Where |o|^2 is below
+// the f32 normal range, |o| < 1.1e-19 and the 1.01e-7 alone covers it.
+__device__ inline double offset_bound(double ox, double oy, double oz) {
+    const float s = (float)(ox * ox + oy * oy + oz * oz);
+    return (double)(__builtin_amdgcn_sqrtf(s) * 1.00001f) + 1.01e-7;
+}
+
 // eval_local of the SAS and vdW kinds (field/point.rs:723-736): the first
 // sphere, in list order, of the least |x - c| - R below band. Each lane
 // scans its share in list order and stops where the key (the value at the
@@ -841,8 +856,7 @@ __device__ inline void scan_class(const DArr& A, const RLists& s, uint32_t cc, u
 __device__ inline EvalD eval_spheres(const RLists& s, const uint32_t* n, const Group& G, double cx, double cy,
                                      double cz, double x, double y, double z, double band) {
     const uint32_t total = n[R_CAND];
-    double ox = x - cx, oy = y - cy, oz = z - cz;
-    const double delta = sqrt(ox * ox + oy * oy + oz * oz) + 1e-7;
+    const double delta = offset_bound(x - cx, y - cy, z - cz);
     double best = band;
     uint32_t bo = 0xffffffffu;
     for (uint32_t p = G.r; p < total; p += G.g) {
@@ -901,8 +915,7 @@ __device__ inline EvalD eval_group(const DArr& A, const RLists& s, const uint32_
         return e;
     }
     const uint32_t ns = n[R_S], nc = n[R_C], total = n[R_CAND];
-    double ox = x - cx, oy = y - cy, oz = z - cz;
-    const double delta = sqrt(ox * ox + oy * oy + oz * oz) + 1e-7;
+    const double delta = offset_bound(x - cx, y - cy, z - cz);
     double best = cutoff;
     uint32_t bo = 0xffffffffu;
     scan_class<0, 1>(A, s, cc, nw, ns, nc, 0, ns, G, x, y, z, delta, cutoff, best, bo);
