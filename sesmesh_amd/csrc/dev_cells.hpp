@@ -53,73 +53,131 @@ __device__ inline uint32_t cell_case(const float* val, uint32_t side, uint32_t x
     return cs;
 }
 
-// Per brick (one workgroup of 256, striding over the *n_bricks bricks),
-// from one copy of its samples (in dynamic LDS with `staged`, (b+1)^3
-// floats): crossing-edge masks (edge ids ((z*b + y)*b + x)*3 + axis, 64 per
-// word), per-word prefix counts, the brick's vertex total and per-sub-block
-// counts (mesh/extract.rs:176-216), and its triangle count from the cells'
+// Sign bits of a brick's spb samples: bit i of sb is val[i] < 0 (the
+// comparisons of cell_case and the edge test), one ballot per 64 samples;
+// word (spb + 63) / 64, past the last sample, is written zero.
+__device__ inline void sign_bits(const float* val, uint32_t spb, unsigned long long* sb) {
+    const uint32_t nsw = (spb + 63) / 64, lane = lane_id();
+    for (uint32_t k = threadIdx.x / WAVE; k <= nsw; k += blockDim.x / WAVE) {
+        const uint32_t i = 64 * k + lane;
+        const unsigned long long m = __ballot(i < spb && val[i] < 0.0f);
+        if (lane == 0) sb[k] = m;
+    }
+}
+
+// n <= 33 sign bits from bit i (a row of samples along x); the zero word
+// past the last sample covers the read of word j + 1.
+__device__ inline unsigned long long sign_run(const unsigned long long* sb, uint32_t i, uint32_t n) {
+    const uint32_t j = i >> 6, off = i & 63u;
+    const unsigned long long v = (sb[j] >> off) | ((sb[j + 1] << 1) << (63u - off));
+    return v & ((1ull << n) - 1ull);
+}
+
+// The sample rows a row of cells (y, z) reads: (y, z), (y + 1, z), (y, z + 1)
+// and (y + 1, z + 1), bit x the sample at x.
+struct CellRows {
+    unsigned long long m0, m1, m2, m3;
+};
+__device__ inline CellRows cell_rows(const unsigned long long* sb, uint32_t side, uint32_t y, uint32_t z) {
+    const uint32_t i0 = side * (y + side * z), pl = side * side;
+    return CellRows{sign_run(sb, i0, side), sign_run(sb, i0 + side, side), sign_run(sb, i0 + pl, side),
+                    sign_run(sb, i0 + side + pl, side)};
+}
+// cell_case of cell x of the row: corner c = (dx, dy, dz) is bit dx + x of
+// the row (y + dy, z + dz).
+__device__ inline uint32_t row_case(const CellRows& R, uint32_t x) {
+    return (uint32_t)(((R.m0 >> x) & 3ull) | (((R.m1 >> x) & 3ull) << 2) | (((R.m2 >> x) & 3ull) << 4) |
+                      (((R.m3 >> x) & 3ull) << 6));
+}
+
+// Bit i of x (i < 21) to bit 3 i.
+__device__ inline unsigned long long spread3(unsigned long long x) {
+    x &= 0x1fffffull;
+    x = (x | (x << 32)) & 0x1f00000000ffffull;
+    x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+    x = (x | (x << 8)) & 0x100f00f00f00f00full;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+// Per brick (one workgroup of 256, striding over the *n_bricks bricks):
+// crossing-edge masks (edge ids ((z*b + y)*b + x)*3 + axis, 64 per word),
+// per-word prefix counts, the brick's vertex total and per-sub-block counts
+// (mesh/extract.rs:176-216), and its triangle count from the cells'
 // Marching Cubes cases (extract.rs:245-260), which sizes the cell pass.
+// The samples are read once, into sign bits (sign_bits); one thread per
+// row of cells (y, z) then takes the row's x, y and z crossings as xors of
+// sample rows (bit x: (x, y, z) against (x + 1, y, z), (x, y + 1, z),
+// (x, y, z + 1)), interleaves them into the row's 3 b edge bits (at edge
+// id 3 b (z b + y), 16 cells at a time) and ORs them into the words, and
+// counts its cells' triangles from row_case. Dynamic LDS: the sign bits,
+// (b + 1)^3 / 64 + 1 words, then the `words` mask words.
 __global__ void __launch_bounds__(256) k_edges(uint32_t b, const float* values, const uint32_t* n_bricks,
                                                uint32_t words, unsigned long long* mask, uint32_t* wpre,
-                                               uint32_t* bcount, uint32_t* scount, uint32_t* tcount,
-                                               uint32_t staged) {
+                                               uint32_t* bcount, uint32_t* scount, uint32_t* tcount) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
-    __shared__ uint32_t wc[2048];
     __shared__ uint32_t sc[8];
     __shared__ uint32_t tmp[4];
     __shared__ uint16_t toff[257];
     for (uint32_t i = threadIdx.x; i < 257; i += blockDim.x) toff[i] = c_tri_offset[i];
     const uint32_t nb = *n_bricks;
+    const uint32_t side = b + 1, sub = b / 2, spb = side * side * side, nsw = (spb + 63) / 64;
+    unsigned long long* sb = (unsigned long long*)smem;
+    unsigned long long* wm = sb + nsw + 1;
+    const unsigned long long lowb = (1ull << b) - 1ull, lows = (1ull << sub) - 1ull;
     for (uint32_t brick = blockIdx.x; brick < nb; brick += gridDim.x) {
-        const uint32_t side = b + 1, sub = b / 2, spb = side * side * side;
-        const float* val = values + (size_t)brick * spb;
-        if (staged) {
-            float* lv = (float*)smem;
-            for (uint32_t i = threadIdx.x; i < spb; i += blockDim.x) lv[i] = val[i];
-            val = lv;
-        }
-        uint32_t nid = 3 * b * b * b, lane = lane_id(), wv = threadIdx.x / WAVE;
+        sign_bits(values + (size_t)brick * spb, spb, sb);
+        for (uint32_t w = threadIdx.x; w < words; w += blockDim.x) wm[w] = 0ull;
         if (threadIdx.x < 8) sc[threadIdx.x] = 0;
         __syncthreads();
-        const CellSplit S = cell_split(b);
-        for (uint32_t w = wv; w < words; w += blockDim.x / WAVE) {
-            const uint32_t id = w * 64 + lane;
-            bool c = false;
-            uint32_t x = 0, y = 0, z = 0;
-            if (id < nid) {  // edge ((z b + y) b + x) 3 + axis (edge_crossing)
-                const uint32_t axis = id % 3u;
-                split_cell(S, id / 3u, x, y, z);
-                const uint32_t i0 = x + side * (y + side * z);
-                const uint32_t st = axis == 0 ? 1u : (axis == 1 ? side : side * side);
-                c = (val[i0] < 0.0f) != (val[i0 + st] < 0.0f);
-            }
-            unsigned long long m = __ballot(c);
-            if (c) {  // x, y, z < b = 2 sub: min(v / sub, 1) is v >= sub
-                uint32_t s = (x >= sub ? 1u : 0u) + (y >= sub ? 2u : 0u) + (z >= sub ? 4u : 0u);
-                atomicAdd(&sc[s], 1u);
-            }
-            if (lane == 0) {
-                mask[(size_t)brick * words + w] = m;
-                wc[w] = (uint32_t)__popcll(m);
-            }
-        }
         uint32_t mine = 0;
-        for (uint32_t c = threadIdx.x; c < b * b * b; c += blockDim.x) {
-            uint32_t x, y, z;
-            split_cell(S, c, x, y, z);
-            const uint32_t cs = cell_case(val, side, x, y, z);
-            mine += toff[cs + 1] - toff[cs];
+        for (uint32_t r = threadIdx.x; r < b * b; r += blockDim.x) {
+            const uint32_t y = r % b, z = r / b;
+            const CellRows R = cell_rows(sb, side, y, z);
+            const unsigned long long ex = (R.m0 ^ (R.m0 >> 1)) & lowb, ey = (R.m0 ^ R.m1) & lowb,
+                                     ez = (R.m0 ^ R.m2) & lowb;
+            // sub-blocks of the row's lower (x < sub) and upper cells
+            const uint32_t s0 = (y >= sub ? 2u : 0u) + (z >= sub ? 4u : 0u);
+            const uint32_t nlo = __popcll(ex & lows) + __popcll(ey & lows) + __popcll(ez & lows);
+            const uint32_t nhi = __popcll(ex >> sub) + __popcll(ey >> sub) + __popcll(ez >> sub);
+            if (nlo) atomicAdd(&sc[s0], nlo);
+            if (nhi) atomicAdd(&sc[s0 + 1], nhi);
+            for (uint32_t k = 0; 16 * k < b; k++) {
+                const uint32_t sh = 16 * k;
+                const unsigned long long v = spread3((ex >> sh) & 0xffffull) | (spread3((ey >> sh) & 0xffffull) << 1) |
+                                             (spread3((ez >> sh) & 0xffffull) << 2);
+                if (!v) continue;
+                // at most 48 bits from edge bit P: word j, and j + 1 past bit 64
+                const uint32_t P = 3 * b * r + 48 * k, j = P >> 6, off = P & 63u;
+                atomicOr(&wm[j], v << off);
+                if (off > 16u) {
+                    const unsigned long long hi = v >> (64u - off);
+                    if (hi) atomicOr(&wm[j + 1], hi);
+                }
+            }
+            for (uint32_t x = 0; x < b; x++) {
+                const uint32_t cs = row_case(R, x);
+                mine += toff[cs + 1] - toff[cs];
+            }
         }
+        __syncthreads();
         uint32_t total;
         block_scan_256(mine, tmp, total);
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t acc = 0;
-            for (uint32_t w = 0; w < words; w++) {
-                wpre[(size_t)brick * words + w] = acc;
-                acc += wc[w];
+        uint32_t carry = 0;
+        for (uint32_t base = 0; base < words; base += blockDim.x) {
+            const uint32_t w = base + threadIdx.x;
+            const unsigned long long m = w < words ? wm[w] : 0ull;
+            uint32_t all;
+            const uint32_t before = block_scan_256((uint32_t)__popcll(m), tmp, all);
+            if (w < words) {
+                mask[(size_t)brick * words + w] = m;
+                wpre[(size_t)brick * words + w] = carry + before;
             }
-            bcount[brick] = acc;
+            carry += all;
+        }
+        if (threadIdx.x == 0) {
+            bcount[brick] = carry;
             tcount[brick] = total;
         }
         if (threadIdx.x < 8) scount[8 * brick + threadIdx.x] = sc[threadIdx.x];
@@ -143,10 +201,12 @@ constexpr uint32_t TRI_SLOTS = 1024;
 // are found in the molecule's dense index grid. A reference that does not
 // resolve sets *err; triangles at or beyond `tcap` are not written (the
 // run's triangle count exceeds the buffer and is reported by its caller).
-// With `staged`, the brick's samples, then the edge masks and per-word
-// prefix counts of the brick and of its seven upper neighbours, are first
-// copied to dynamic LDS ((b+1)^3 floats, then 8 `words` u64 and 8 `words`
-// u32), so every triangle corner resolves from LDS; each thread then lists
+// With `staged`, the brick's sample signs (sign_bits, (b+1)^3 / 64 + 1 u64),
+// then the edge masks and per-word prefix counts of the brick and of its
+// seven upper neighbours, are first laid out in dynamic LDS (8 `words` u64
+// and 8 `words` u32), so every cell case and triangle corner resolves from
+// LDS (a thread's cells are consecutive, so it reads each row of sample
+// signs once); each thread then lists
 // its cells' triangles (cell, table row) at its offset in TRI_SLOTS more
 // u32, and the threads take one listed triangle each, so a wavefront no
 // longer waits for the lane whose cells hold the most triangles.
@@ -184,11 +244,11 @@ __global__ void __launch_bounds__(256) k_cell_emit(const Mol* __restrict__ mols,
             tvbase[code] = tb < 0 ? 0u : vbase[tb] - vbase[fbrick[m]];
         }
         __syncthreads();
+        unsigned long long* sb = (unsigned long long*)smem;
         if (staged) {
-            float* sv = (float*)smem;
-            unsigned long long* sm = (unsigned long long*)(smem + ((4 * (size_t)P.spb + 7) & ~(size_t)7));
+            sign_bits(val, P.spb, sb);
+            unsigned long long* sm = sb + (P.spb + 63) / 64 + 1;
             uint32_t* sp = (uint32_t*)(sm + 8 * words);
-            for (uint32_t i = threadIdx.x; i < P.spb; i += blockDim.x) sv[i] = val[i];
             for (uint32_t i = threadIdx.x; i < 8 * words; i += blockDim.x) {
                 const int32_t tb = target[i / words];
                 if (tb >= 0) {
@@ -196,7 +256,6 @@ __global__ void __launch_bounds__(256) k_cell_emit(const Mol* __restrict__ mols,
                     sp[i] = wpre[(size_t)tb * words + i % words];
                 }
             }
-            val = sv;
             cmask = sm;
             cpre = sp;
         }
@@ -204,13 +263,42 @@ __global__ void __launch_bounds__(256) k_cell_emit(const Mol* __restrict__ mols,
         uint32_t b = P.b, ncell = b * b * b, per = (ncell + blockDim.x - 1) / blockDim.x;
         uint32_t c_lo = threadIdx.x * per, c_hi = min(ncell, (threadIdx.x + 1) * per);
         const CellSplit S = cell_split(b);
-        uint32_t mine = 0;
-        for (uint32_t c = c_lo; c < c_hi; c++) {
+        // The case of cell c = (x, y, z): from the sign rows of its row of
+        // cells (y, z), or from the samples in global memory. The first walk
+        // over the thread's cells reads each row once and keeps the cases of
+        // its first 8 cells, a byte each, for the second.
+        uint32_t mine = 0, pk0 = 0u, pk1 = 0u;
+        {
+            CellRows rows{};
+            uint32_t row_at = 0xffffffffu;
+            for (uint32_t c = c_lo; c < c_hi; c++) {
+                uint32_t x, y, z;
+                split_cell(S, c, x, y, z);
+                uint32_t cs;
+                if (staged) {
+                    if (z * b + y != row_at) {
+                        rows = cell_rows(sb, P.side, y, z);
+                        row_at = z * b + y;
+                    }
+                    cs = row_case(rows, x);
+                } else {
+                    cs = cell_case(val, P.side, x, y, z);
+                }
+                const uint32_t q = c - c_lo;
+                if (q < 4u)
+                    pk0 |= cs << (8u * q);
+                else if (q < 8u)
+                    pk1 |= cs << (8u * (q - 4u));
+                mine += tab.off[cs + 1] - tab.off[cs];
+            }
+        }
+        auto case_again = [&](uint32_t c) {
+            const uint32_t q = c - c_lo;
+            if (q < 8u) return ((q < 4u ? pk0 : pk1) >> (8u * (q & 3u))) & 255u;
             uint32_t x, y, z;
             split_cell(S, c, x, y, z);
-            uint32_t cs = cell_case(val, P.side, x, y, z);
-            mine += tab.off[cs + 1] - tab.off[cs];
-        }
+            return staged ? row_case(cell_rows(sb, P.side, y, z), x) : cell_case(val, P.side, x, y, z);
+        };
         uint32_t total;
         const uint32_t first = block_scan_256(mine, tmp, total);
         const size_t base = (size_t)tbase[brick];
@@ -239,9 +327,7 @@ __global__ void __launch_bounds__(256) k_cell_emit(const Mol* __restrict__ mols,
             uint32_t* slots = (uint32_t*)(cpre + 8 * words);
             uint32_t k = first;
             for (uint32_t c = c_lo; c < c_hi; c++) {
-                uint32_t x, y, z;
-                split_cell(S, c, x, y, z);
-                uint32_t cs = cell_case(val, P.side, x, y, z);
+                const uint32_t cs = case_again(c);
                 for (uint32_t t = tab.off[cs]; t < tab.off[cs + 1]; t++) slots[k++] = (c << 10) | t;
             }
             __syncthreads();
@@ -260,7 +346,7 @@ __global__ void __launch_bounds__(256) k_cell_emit(const Mol* __restrict__ mols,
             for (uint32_t c = c_lo; c < c_hi; c++) {
                 uint32_t x, y, z;
                 split_cell(S, c, x, y, z);
-                uint32_t cs = cell_case(val, P.side, x, y, z);
+                const uint32_t cs = case_again(c);
                 for (uint32_t t = tab.off[cs]; t < tab.off[cs + 1]; t++) {
                     for (int q = 0; q < 3; q++) {
                         const uint32_t vi = corner(x, y, z, tab.edges[3 * t + q]);

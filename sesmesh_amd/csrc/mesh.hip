@@ -1050,9 +1050,10 @@ LdsPlan lds_plan(uint32_t b, bool fast, uint64_t lds_max, const uint32_t st[LK_N
     const uint32_t spb = (b + 1) * (b + 1) * (b + 1);
     const uint32_t words = (3 * b * b * b + 63) / 64;
     auto fits = [&](uint64_t dyn, int k) { return (uint32_t)(dyn + st[k] <= lds_max); };
-    L.edges_dyn = 4 * spb;
+    const uint32_t signs = 8 * ((spb + 63) / 64 + 1);  // sign bits of the samples, one zero word past
+    L.edges_dyn = signs + 8 * words;
     L.edges_staged = fits(L.edges_dyn, LK_EDGES);
-    L.cells_dyn = ((4 * spb + 7) & ~7u) + 8 * 12 * words + 4 * TRI_SLOTS;
+    L.cells_dyn = signs + 8 * 12 * words + 4 * TRI_SLOTS;
     L.cells_staged = fits(L.cells_dyn, LK_CELLS);
     for (int u = 0; u < UNIFORM_TIERS; u++) {
         L.uniform_dyn[u] = make_rlayout(UNIFORM_CAPS[u]).bytes;
@@ -1797,9 +1798,9 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     TRY(X.temp(&scount, nsub));
     TRY(X.temp(&vbase, (size_t)nb + 1));
     TRY(X.temp(&tcount, nb));  // triangles per brick, for the cell pass
-    const uint32_t edges_staged = LP.edges_staged;
-    k_edges<<<stride_grid(nb, 1), 256, edges_staged ? 4 * spb : 0, s>>>(b, M->values, &cnt[K_NB], words, mask, wpre,
-                                                                        bcount, scount, tcount, edges_staged);
+    if (!LP.edges_staged) return fail(SMG_E_INTERNAL, "the edge pass's sign bits do not fit in LDS");
+    k_edges<<<stride_grid(nb, 1), 256, LP.edges_dyn, s>>>(b, M->values, &cnt[K_NB], words, mask, wpre, bcount, scount,
+                                                          tcount);
     TRY(scan_u32(X, bcount, vbase, &cnt[K_NB], nb, &cnt[K_NV]));
     k_clamp<<<1, 64, 0, s>>>(&cnt[K_NV], E.nv, &cnt[K_FLAGS], OVER_VERTICES, nullptr, 0u);
     uint32_t *rflag, *roff, *rwork;

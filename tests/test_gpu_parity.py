@@ -60,13 +60,13 @@ def test_crambin_equals_cpu(device, structures, spacing):
 
 @pytest.mark.parametrize("brick,spacing", [(18, 0.25), (24, 0.5), (26, 0.5), (32, 0.5)])
 def test_large_bricks_equal_cpu(device, structures, brick, spacing):
-    """The cell pass stages a brick's samples and edge masks in LDS up to
-    brick 24 (brick 26 needs 161,936 dynamic bytes, which with the kernel's
-    static tables exceed a workgroup's 160 KiB, so it runs unstaged); the
-    edge pass stages its samples up to brick 32. At brick 18 and 0.25 A, 51
-    of crambin's 233 surface bricks list more triangles (up to 1,694) than
-    the cell pass's 1,024 LDS slots and take the per-thread path. The mesh
-    must equal the CPU back end's at the same brick size
+    """The edge and cell passes keep a brick's sample signs and (cell pass)
+    the edge masks of it and its seven upper neighbours in LDS; at brick 32
+    the cell pass asks for 156,056 dynamic bytes, within a workgroup's 160
+    KiB with its static tables. At brick 18 and 0.25 A, 51 of crambin's 233
+    surface bricks list more triangles (up to 1,694) than the cell pass's
+    1,024 LDS slots and take the per-thread path. The mesh must equal the
+    CPU back end's at the same brick size
     (crates/sesmesh/src/field/gpu.rs:906-962)."""
     o, m, g = run_both(device, structures("1CRN"), 1.4, spacing, brick)
     assert_same_mesh(o, m, g)

@@ -98,7 +98,9 @@ def test_every_brick_fits_its_lds(tmp_path):
             assert p["sample_fits"][0] and p["refine_fits"][0] and p["uniform_fits"][0]
         if p["cells_staged"]:
             staged_cells.append(brick)
-    # brick 26 asks for 161,936 dynamic bytes: with the kernel's static
-    # tables it does not fit, so its cell pass runs unstaged.
-    assert 26 not in staged_cells and 24 in staged_cells
+    # the cell pass keeps the samples as sign bits: brick 32 asks for 156,056
+    # dynamic bytes, which with the kernel's static tables fit, so every
+    # brick stages.
+    assert staged_cells == list(range(4, 33, 2))
+    assert lds_plan(32, 0, statics)["cells_dyn"] == 8 * (562 + 1) + 96 * 1536 + 4 * 1024
     assert lds_plan(32, 0, statics)["edges_staged"] == 1
