@@ -84,22 +84,25 @@ __device__ inline uint32_t pack_states(const uint8_t* st) {
 }
 
 // Samples owned by no sub-block get +-band (kernel.rs:533-534); one
-// workgroup per brick, striding over the *n_bricks bricks.
+// workgroup per brick, striding over the *n_bricks bricks of the run's
+// brick size b. A brick whose eight sub-blocks are all sampled owns every
+// sample and is skipped.
 __global__ void k_assign_uniform(const Mol* __restrict__ mols, const uint32_t* bmol, const uint8_t* st,
-                                 const uint32_t* n_bricks, float* values) {
+                                 const uint32_t* n_bricks, uint32_t b, float* values) {
     const uint32_t nb = *n_bricks;
+    const uint32_t side = b + 1, plane = side * side, spb = plane * side, sub = b / 2;
+    // i / d as __umulhi(i, ceil(2^32 / d)): exact for i and d below 2^16.
+    const uint32_t m1 = (uint32_t)(((1ull << 32) + side - 1) / side);
+    const uint32_t m2 = (uint32_t)(((1ull << 32) + plane - 1) / plane);
     for (uint32_t brick = blockIdx.x; brick < nb; brick += gridDim.x) {
-        const DRun& P = mols[bmol[brick]].P;
-        uint32_t s = pack_states(st + 8 * brick);
-        float* out = values + (size_t)brick * P.spb;
-        // i / d as __umulhi(i, ceil(2^32 / d)): exact for i and d below 2^16.
-        const uint32_t side = P.side, plane = side * side;
-        const uint32_t m1 = (uint32_t)(((1ull << 32) + side - 1) / side);
-        const uint32_t m2 = (uint32_t)(((1ull << 32) + plane - 1) / plane);
-        for (uint32_t i = threadIdx.x; i < P.spb; i += blockDim.x) {
+        const uint32_t s = pack_states(st + 8 * brick);
+        if (s == 0u) continue;  // ST_NONE in every sub-block
+        const float band = mols[bmol[brick]].P.fband;
+        float* out = values + (size_t)brick * spb;
+        for (uint32_t i = threadIdx.x; i < spb; i += blockDim.x) {
             const uint32_t z = __umulhi(i, m2), r = i - z * plane, y = __umulhi(r, m1), x = r - y * side;
             bool positive;
-            if (sample_owner(s, P.sub, x, y, z, positive) < 0) out[i] = positive ? P.fband : -P.fband;
+            if (sample_owner(s, sub, x, y, z, positive) < 0) out[i] = positive ? band : -band;
         }
     }
 }

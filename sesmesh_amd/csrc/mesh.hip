@@ -1655,7 +1655,7 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
 
     // Samples (field/kernel.rs:415-449).
     TRY(X.output(&M->values, (size_t)nb * spb));
-    k_assign_uniform<<<stride_grid(nb, 1), 256, 0, s>>>(mols, bmol, st, &cnt[K_NB], M->values);
+    k_assign_uniform<<<stride_grid(nb, 1), 256, 0, s>>>(mols, bmol, st, &cnt[K_NB], b, M->values);
     const uint64_t nsub = 8 * (uint64_t)nb;
     uint32_t *sflag, *soff, *swork;
     TRY(X.temp(&sflag, nsub));
