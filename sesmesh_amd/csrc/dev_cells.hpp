@@ -195,12 +195,37 @@ __device__ inline int64_t dense_index(const DRun& P, int64_t kx, int64_t ky, int
 // them; a brick with more takes the per-thread path.
 constexpr uint32_t TRI_SLOTS = 1024;
 
+// The bricks whose edges a brick's cells reference (mesh/extract.rs:629-667),
+// one thread per (brick, code): tgt[8 brick + code] for code 1..7 is the
+// brick at key + (code bits, one per axis) in the molecule's dense index
+// grid, -1 if that is no surface brick; tgt[8 brick] is the molecule's
+// first brick fbrick[m] (the brick itself is its own code-0 target), whose
+// vertex base the molecule's local vertex indices start from. Resolved
+// here, in parallel over every brick, the cell pass reads them in one load
+// instead of four dependent ones per brick.
+__global__ void k_cell_targets(const Mol* __restrict__ mols, const uint32_t* bmol, const uint32_t* fbrick,
+                               const int32_t* keys, const uint32_t* dense, const uint32_t* n_bricks, int32_t* tgt) {
+    const uint64_t n = 8ull * *n_bricks;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t brick = (uint32_t)(i / 8), code = (uint32_t)(i % 8), m = bmol[brick];
+        int64_t tb = fbrick[m];
+        if (code) {
+            const int64_t di = dense_index(mols[m].P, (int64_t)keys[3 * brick] + (code & 1),
+                                           (int64_t)keys[3 * brick + 1] + ((code >> 1) & 1),
+                                           (int64_t)keys[3 * brick + 2] + ((code >> 2) & 1));
+            const uint32_t* mdense = dense + mols[m].dense_off;
+            tb = (di < 0 || mdense[di] == 0xffffffffu) ? -1 : (int64_t)mdense[di];
+        }
+        tgt[i] = (int32_t)tb;
+    }
+}
+
 // Emits the triangles of every brick at tbase[brick] (mesh/extract.rs:245-260
 // and :629-667), with vertex indices local to the brick's molecule (its
-// first vertex is that of its first brick, fbrick[m]). Neighbour bricks
-// are found in the molecule's dense index grid. A reference that does not
-// resolve sets *err; triangles at or beyond `tcap` are not written (the
-// run's triangle count exceeds the buffer and is reported by its caller).
+// first vertex is that of its first brick), the referenced bricks from
+// k_cell_targets. A reference that does not resolve sets *err; triangles
+// at or beyond `tcap` are not written (the run's triangle count exceeds the
+// buffer and is reported by its caller).
 // With `staged`, the brick's sample signs (sign_bits, (b+1)^3 / 64 + 1 u64),
 // then the edge masks and per-word prefix counts of the brick and of its
 // seven upper neighbours, are first laid out in dynamic LDS (8 `words` u64
@@ -210,13 +235,11 @@ constexpr uint32_t TRI_SLOTS = 1024;
 // its cells' triangles (cell, table row) at its offset in TRI_SLOTS more
 // u32, and the threads take one listed triangle each, so a wavefront no
 // longer waits for the lane whose cells hold the most triangles.
-__global__ void __launch_bounds__(256) k_cell_emit(const Mol* __restrict__ mols, const uint32_t* bmol,
-                                                   const uint32_t* fbrick, const float* values, const int32_t* keys,
+__global__ void __launch_bounds__(256) k_cell_emit(uint32_t brick_size, const int32_t* tgt, const float* values,
                                                    const uint32_t* n_bricks, const uint32_t* tbase,
-                                                   const uint32_t* dense, const uint32_t* vbase,
-                                                   const unsigned long long* mask, const uint32_t* wpre,
-                                                   uint32_t words, uint32_t* tris, size_t tcap, uint32_t* err,
-                                                   uint32_t staged) {
+                                                   const uint32_t* vbase, const unsigned long long* mask,
+                                                   const uint32_t* wpre, uint32_t words, uint32_t* tris, size_t tcap,
+                                                   uint32_t* err, uint32_t staged) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     __shared__ uint32_t tmp[4];
     __shared__ int32_t target[8];  // brick at key + (code bits), -1 if none
@@ -225,29 +248,26 @@ __global__ void __launch_bounds__(256) k_cell_emit(const Mol* __restrict__ mols,
     load_tables(tab);
     const uint32_t nb = *n_bricks;
     for (uint32_t brick = blockIdx.x; brick < nb; brick += gridDim.x) {
-        const uint32_t m = bmol[brick];
-        const DRun& P = mols[m].P;
-        const float* val = values + (size_t)brick * P.spb;
+        // The brick size, opaque to the compiler within each brick: known
+        // across the loop, the thread's cell coordinates and LDS addresses
+        // would be hoisted out of it and held in registers (about 190 VGPRs).
+        uint32_t b = brick_size;
+        asm volatile("" : "+s"(b));
+        const uint32_t side = b + 1, spb = side * side * side;
+        const float* val = values + (size_t)brick * spb;
         const unsigned long long* cmask = mask;  // edge masks and prefix counts, by brick
         const uint32_t* cpre = wpre;
-        int64_t key[3] = {keys[3 * brick], keys[3 * brick + 1], keys[3 * brick + 2]};
         if (threadIdx.x < 8) {  // the eight bricks whose edges the cells reference, once per brick
             const uint32_t code = threadIdx.x;
-            int64_t tb = brick;
-            if (code) {
-                const int64_t di =
-                    dense_index(P, key[0] + (code & 1), key[1] + ((code >> 1) & 1), key[2] + ((code >> 2) & 1));
-                const uint32_t* mdense = dense + mols[m].dense_off;
-                tb = (di < 0 || mdense[di] == 0xffffffffu) ? -1 : (int64_t)mdense[di];
-            }
-            target[code] = (int32_t)tb;
-            tvbase[code] = tb < 0 ? 0u : vbase[tb] - vbase[fbrick[m]];
+            const int32_t first = tgt[8 * (size_t)brick], tb = code ? tgt[8 * (size_t)brick + code] : (int32_t)brick;
+            target[code] = tb;
+            tvbase[code] = tb < 0 ? 0u : vbase[tb] - vbase[first];
         }
-        __syncthreads();
         unsigned long long* sb = (unsigned long long*)smem;
+        if (staged) sign_bits(val, spb, sb);
+        __syncthreads();
         if (staged) {
-            sign_bits(val, P.spb, sb);
-            unsigned long long* sm = sb + (P.spb + 63) / 64 + 1;
+            unsigned long long* sm = sb + (spb + 63) / 64 + 1;
             uint32_t* sp = (uint32_t*)(sm + 8 * words);
             for (uint32_t i = threadIdx.x; i < 8 * words; i += blockDim.x) {
                 const int32_t tb = target[i / words];
@@ -260,7 +280,7 @@ __global__ void __launch_bounds__(256) k_cell_emit(const Mol* __restrict__ mols,
             cpre = sp;
         }
         __syncthreads();
-        uint32_t b = P.b, ncell = b * b * b, per = (ncell + blockDim.x - 1) / blockDim.x;
+        const uint32_t ncell = b * b * b, per = (ncell + blockDim.x - 1) / blockDim.x;
         uint32_t c_lo = threadIdx.x * per, c_hi = min(ncell, (threadIdx.x + 1) * per);
         const CellSplit S = cell_split(b);
         // The case of cell c = (x, y, z): from the sign rows of its row of
@@ -277,12 +297,12 @@ __global__ void __launch_bounds__(256) k_cell_emit(const Mol* __restrict__ mols,
                 uint32_t cs;
                 if (staged) {
                     if (z * b + y != row_at) {
-                        rows = cell_rows(sb, P.side, y, z);
+                        rows = cell_rows(sb, side, y, z);
                         row_at = z * b + y;
                     }
                     cs = row_case(rows, x);
                 } else {
-                    cs = cell_case(val, P.side, x, y, z);
+                    cs = cell_case(val, side, x, y, z);
                 }
                 const uint32_t q = c - c_lo;
                 if (q < 4u)
@@ -297,7 +317,7 @@ __global__ void __launch_bounds__(256) k_cell_emit(const Mol* __restrict__ mols,
             if (q < 8u) return ((q < 4u ? pk0 : pk1) >> (8u * (q & 3u))) & 255u;
             uint32_t x, y, z;
             split_cell(S, c, x, y, z);
-            return staged ? row_case(cell_rows(sb, P.side, y, z), x) : cell_case(val, P.side, x, y, z);
+            return staged ? row_case(cell_rows(sb, side, y, z), x) : cell_case(val, side, x, y, z);
         };
         uint32_t total;
         const uint32_t first = block_scan_256(mine, tmp, total);

@@ -1962,9 +1962,12 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     TRY(X.size(&cnt[K_NT], E.nt, ntris));
     if (report_phases) report(d, SMG_STAGE_MESH, nb - nb / 10, nb);  // refined
     TRY(X.output(&M->tris, 3 * (size_t)ntris));
-    k_cell_emit<<<stride_grid(nb, 1), 256, staged ? cell_lds : 0, s>>>(mols, bmol, fbrick, M->values, M->keys,
-                                                                        &cnt[K_NB], tbase, dense, vbase, mask, wpre,
-                                                                        words, M->tris, ntris, &cnt[K_ERR], staged);
+    int32_t* tgt;
+    TRY(X.temp(&tgt, 8 * (size_t)nb));
+    k_cell_targets<<<stride_grid(8 * (uint64_t)nb), 256, 0, s>>>(mols, bmol, fbrick, M->keys, dense, &cnt[K_NB], tgt);
+    k_cell_emit<<<stride_grid(nb, 1), 256, staged ? cell_lds : 0, s>>>(b, tgt, M->values, &cnt[K_NB], tbase, vbase,
+                                                                        mask, wpre, words, M->tris, ntris, &cnt[K_ERR],
+                                                                        staged);
     HIP_TRY(hipGetLastError());
     TRY(rec(5));
     k_mol_summary<<<stride_grid(rows + 1), 256, 0, s>>>(rows, fbrick, &cnt[K_NB], soff, &cnt[K_NSW], vbase,
