@@ -1000,6 +1000,10 @@ constexpr RCaps UNIFORM_CAPS[UNIFORM_TIERS] = {
     {128, 64, 192, 256, 0, 1, 0}, {192, 128, 256, 512, 0, 1, 0}, {384, 256, 640, 1024, 0, 1, 0}};
 constexpr uint32_t LDS_EDGES = 512;      // crossing-edge buffer of the LDS passes
 constexpr uint32_t SCRATCH_CHUNK = 1024;  // default workgroups of a scratch pass (one list slice each)
+// Workgroups of the wavefront passes that take what every list tier left: few
+// bricks reach them, and a residency-sized grid (8,192 one-wavefront groups)
+// costs tens of microseconds to launch and drain even when none does.
+constexpr uint32_t WAVE_PASS_WGS = 2048;
 constexpr uint32_t HID_PER_SUB = 64;      // hidden-candidate pool entries per sampled sub-block
 constexpr uint32_t SUP_PER_SUB = 160;     // refinement-superset pool entries per sampled sub-block
 constexpr uint32_t BLK_PER_SUB = 320;     // blocker-record slot words per sampled sub-block
@@ -1546,7 +1550,8 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
             count = &cnt[K_UOVER + u];
         }
         Queue rest{list, count, &cnt[K_Q_SUBST]};
-        uint32_t grid = g1(std::min<uint64_t>(E.dense, resident(d, (const void*)k_classify_touched_wave, WAVE, 0)));
+        uint32_t grid = g1(std::min<uint64_t>({E.dense, resident(d, (const void*)k_classify_touched_wave, WAVE, 0),
+                                                 (uint64_t)WAVE_PASS_WGS}));
         k_classify_touched_wave<<<grid, WAVE, 0, s>>>(mols, wofs_d, rows, touched, rest, keep, stt);
         HIP_TRY(hipGetLastError());
     } else {
@@ -1647,7 +1652,7 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
         }
         if (first) TRY(rec(9));
         Queue rest{list, count, &cnt[K_Q_SUBST]};
-        uint32_t grid = g1(std::min<uint64_t>(nb, resident(d, (const void*)k_sub_states, WAVE, 0)));
+        uint32_t grid = g1(std::min<uint64_t>({nb, resident(d, (const void*)k_sub_states, WAVE, 0), (uint64_t)WAVE_PASS_WGS}));
         k_sub_states<<<grid, WAVE, 0, s>>>(mols, bmol, M->keys, rest, st);
         HIP_TRY(hipGetLastError());
     }
