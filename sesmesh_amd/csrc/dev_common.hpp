@@ -166,13 +166,28 @@ struct Queue {
 
 // Cancellation of the device's current run. Every run has a generation
 // number, which its first kernel stores in g_run; the host cancels run g by
-// writing g to g_cancel (a copy on another stream, while kernels run), in
-// either order. Work-item kernels stop taking items once they are equal.
+// writing g to a word of coherent pinned host memory (g_cancel_host, its
+// device address), in either order. Work-item kernels stop taking items once
+// g_cancel equals g_run. The host word is read over the bus, so only every
+// 64th item taken polls it (every launch polls at its item 0) and copies a
+// match into g_cancel, which every other take reads from device memory. A
+// copy into g_cancel on the host's own stream raises it too; such a copy may
+// wait for free compute units behind kernels that fill the device, the
+// polled word does not.
 __device__ uint32_t g_cancel = 0u;
 __device__ uint32_t g_run = 0xffffffffu;
+__device__ const uint32_t* g_cancel_host = nullptr;
 __device__ __forceinline__ bool cancel_raised() {
     return __hip_atomic_load(&g_cancel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
            __hip_atomic_load(&g_run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool poll_host_cancel() {
+    const uint32_t* h = g_cancel_host;
+    if (!h) return false;
+    const uint32_t g = __hip_atomic_load(h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (g != __hip_atomic_load(&g_run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+    __hip_atomic_store(&g_cancel, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
 }
 __global__ void k_run_begin(const uint32_t* gen) {
     __hip_atomic_store(&g_run, *gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -181,7 +196,11 @@ __global__ void k_run_begin(const uint32_t* gen) {
 // Takes the workgroup's next item; false once the queue is empty. Every
 // thread of the workgroup calls it (it holds two barriers).
 __device__ __forceinline__ bool queue_take(const Queue& q, uint32_t* ticket, uint32_t& item) {
-    if (threadIdx.x == 0) *ticket = cancel_raised() ? 0xffffffffu : atomicAdd(q.next, 1u);
+    if (threadIdx.x == 0) {
+        uint32_t t = cancel_raised() ? 0xffffffffu : atomicAdd(q.next, 1u);
+        if ((t & 63u) == 0u && poll_host_cancel()) t = 0xffffffffu;
+        *ticket = t;
+    }
     __syncthreads();
     // The ticket and item are the workgroup's: reading them as wave-uniform
     // keeps everything derived from them (molecule, lattice, arrangement
@@ -212,7 +231,7 @@ __device__ __forceinline__ void for_each_item(const Queue& q, uint32_t* ticket, 
         if (t >= *q.count) return;
         // one reading of the flag for the whole workgroup (the body holds barriers)
         __shared__ uint32_t stop;
-        if (threadIdx.x == 0) stop = cancel_raised();
+        if (threadIdx.x == 0) stop = cancel_raised() || ((t & 63u) == 0u && poll_host_cancel());
         __syncthreads();
         if (!stop) body(q.list ? q.list[t] : t);
     }

@@ -311,10 +311,12 @@ struct smg_device {
     char* noalloc_sum_h = nullptr;
     std::mutex noalloc_mu;
     // Cancellation and progress of the device's runs (smg_device_set_control):
-    // the caller's flag, polled by the host; g_cancel on this device, raised
-    // by a copy on its own stream so running kernels stop taking work.
+    // the caller's flag, polled by the host. Raising a run's cancellation
+    // writes its generation into cancel_host (the ordinal's word, which the
+    // work-item kernels poll) and copies it into g_cancel on cstream.
     smg_control ctl{};
     uint32_t* cancel_dev = nullptr;
+    uint32_t* cancel_host = nullptr;
     hipStream_t cstream = nullptr;
     bool raised = false;
     uint32_t gen = 0;  // generation of the current run
@@ -443,6 +445,28 @@ struct smg_mesh {
 
 // ------------------------------------------------------------------ device
 static std::once_flag g_table_once;
+
+// The cancellation word of each device ordinal: coherent pinned host memory
+// that the device's work-item kernels poll (g_cancel_host holds its device
+// address). Every handle opened on the ordinal shares it and it is never
+// freed, so no handle's close can free memory another handle's kernels read;
+// run generations are drawn from one counter per ordinal, so a raised
+// generation names one run of one handle.
+static std::mutex g_cancel_mu;
+static uint32_t* g_cancel_word[64] = {};
+static std::atomic<uint32_t> g_generation[64];
+static uint32_t* cancel_word(int ordinal) {
+    if (ordinal < 0 || ordinal >= 64) return nullptr;
+    std::lock_guard<std::mutex> lock(g_cancel_mu);
+    if (!g_cancel_word[ordinal]) {
+        uint32_t* w = nullptr;
+        if (hipHostMalloc((void**)&w, 64 * sizeof(uint32_t), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess)
+            return nullptr;
+        w[0] = 0u;  // no generation is 0
+        g_cancel_word[ordinal] = w;
+    }
+    return g_cancel_word[ordinal];
+}
 static uint16_t g_tri_offset[257];
 static std::vector<uint8_t> g_tri_edges;
 
@@ -477,8 +501,15 @@ extern "C" int smg_device_open(int ordinal, smg_device** out, smg_device_info* i
         }
     }
     (void)hipHostMalloc((void**)&d->host_u32, 64 * sizeof(uint32_t), hipHostMallocDefault);
-    if (!d->host_u32 || hipStreamCreateWithFlags(&d->cstream, hipStreamNonBlocking) != hipSuccess ||
-        hipGetSymbolAddress((void**)&d->cancel_dev, HIP_SYMBOL(g_cancel)) != hipSuccess) {
+    d->cancel_host = cancel_word(ordinal);
+    const uint32_t* cancel_host_dev = nullptr;
+    void* cancel_host_sym = nullptr;
+    if (!d->host_u32 || !d->cancel_host ||
+        hipHostGetDevicePointer((void**)&cancel_host_dev, d->cancel_host, 0) != hipSuccess ||
+        hipStreamCreateWithFlags(&d->cstream, hipStreamNonBlocking) != hipSuccess ||
+        hipGetSymbolAddress((void**)&d->cancel_dev, HIP_SYMBOL(g_cancel)) != hipSuccess ||
+        hipGetSymbolAddress(&cancel_host_sym, HIP_SYMBOL(g_cancel_host)) != hipSuccess ||
+        hipMemcpy(cancel_host_sym, &cancel_host_dev, sizeof(cancel_host_dev), hipMemcpyHostToDevice) != hipSuccess) {
         smg_device_close(d);
         return fail(SMG_E_CALL, "cannot set up cancellation on the device");
     }
@@ -924,6 +955,7 @@ struct Temps {
 bool cancel_requested(const smg_device* d) { return d->ctl.cancel && *d->ctl.cancel != 0; }
 void raise_cancel(smg_device* d) {
     if (d->raised) return;
+    __atomic_store_n(d->cancel_host, d->gen, __ATOMIC_SEQ_CST);
     d->host_u32[63] = d->gen;
     (void)hipMemcpyAsync(d->cancel_dev, d->host_u32 + 63, 4, hipMemcpyHostToDevice, d->cstream);
     d->raised = true;
@@ -937,7 +969,10 @@ int check_cancel(smg_device* d) {
 // next generation (its table image carries it to the device).
 int prepare_run(smg_device* d) {
     TRY(check_cancel(d));
-    d->gen = d->gen == 0xfffffffeu ? 1u : d->gen + 1u;  // never g_run's initial value
+    uint32_t g;  // never 0 (the word's initial value) nor g_run's initial value
+    do g = g_generation[d->ordinal].fetch_add(1u, std::memory_order_relaxed) + 1u;
+    while (g == 0u || g == 0xffffffffu);
+    d->gen = g;
     d->raised = false;
     return SMG_OK;
 }
@@ -1184,10 +1219,14 @@ struct Run {
         *p = q;
         return SMG_OK;
     }
+    // Allocations of an unplanned run check the caller's cancellation flag
+    // first: a large allocation holds the host for a while, and the flag is
+    // not polled meanwhile.
     template <class X>
     int temp(X** p, size_t count) {
         size_t bytes = std::max<size_t>(count, 1) * sizeof(X);
         if (planned()) return slot((void**)p, bytes);
+        TRY(check_cancel(d));
         HIP_TRY(T->get((void**)p, bytes));
         return SMG_OK;
     }
@@ -1196,6 +1235,7 @@ struct Run {
         size_t bytes = std::max<size_t>(count, 1) * sizeof(X);
         if (arena && *p) return SMG_OK;  // the caller's buffer, sized for the run's capacity
         if (planned()) return slot((void**)p, bytes);
+        TRY(check_cancel(d));
         HIP_TRY(hipMalloc((void**)p, bytes));
         return SMG_OK;
     }
@@ -1474,7 +1514,10 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     uint32_t* vfirst = (uint32_t*)(sum + UL.vfirst);
     uint32_t* trfirst = (uint32_t*)(sum + UL.trfirst);
     const bool report_phases = !X.planned();  // a planned run reports from smg_plan_run
-    if (report_phases) report(d, SMG_STAGE_CLASSIFY, 0, 1);
+    if (report_phases) {
+        report(d, SMG_STAGE_CLASSIFY, 0, 1);
+        TRY(check_cancel(d));
+    }
     k_fill_u32<<<stride_grid(UL.zero_words), 256, 0, s>>>((uint32_t*)sum, 0u, UL.zero_words, nullptr);
     k_localize<<<stride_grid(E.locals), 256, 0, s>>>(mols, loc_d, rows);
     HIP_TRY(hipGetLastError());
@@ -1568,6 +1611,7 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     if (report_phases) {
         report(d, SMG_STAGE_CLASSIFY, 1, 1);
         report(d, SMG_STAGE_MESH, 0, nb);
+        TRY(check_cancel(d));
     }
     uint32_t* bmol;
     uint8_t* st;
@@ -1819,7 +1863,10 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     uint32_t nverts, nrw;
     TRY(X.size(&cnt[K_NV], E.nv, nverts));
     TRY(X.size(&cnt[K_NRW], (uint32_t)nsub, nrw));
-    if (report_phases) report(d, SMG_STAGE_MESH, nb / 2, nb);  // sampled; edges numbered
+    if (report_phases) {
+        report(d, SMG_STAGE_MESH, nb / 2, nb);  // sampled; edges numbered
+        TRY(check_cancel(d));
+    }
     TRY(rec(3));
 
     // Refinement and attributes (mesh/extract.rs:425-597).
@@ -1965,7 +2012,10 @@ int pipeline(Run& X, Table& t, const Extents& E, char* stage_host, char* sum_hos
     HIP_TRY(hipGetLastError());
     uint32_t ntris;
     TRY(X.size(&cnt[K_NT], E.nt, ntris));
-    if (report_phases) report(d, SMG_STAGE_MESH, nb - nb / 10, nb);  // refined
+    if (report_phases) {
+        report(d, SMG_STAGE_MESH, nb - nb / 10, nb);  // refined
+        TRY(check_cancel(d));
+    }
     TRY(X.output(&M->tris, 3 * (size_t)ntris));
     int32_t* tgt;
     TRY(X.temp(&tgt, 8 * (size_t)nb));

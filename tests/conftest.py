@@ -40,6 +40,16 @@ def structures():
     return get
 
 
+@pytest.fixture(scope="module", autouse=True)
+def free_cycles_after_module():
+    """Device objects a module left in reference cycles (through tracebacks,
+    closures) are freed when the module ends, not by a later collection
+    inside another module's measurement."""
+    yield
+    import gc
+    gc.collect()
+
+
 @pytest.fixture(scope="session")
 def device():
     import sesmesh_amd
