@@ -173,6 +173,23 @@ __device__ inline bool blocked_f(const Lists& s, uint32_t cc, uint32_t nw, uint3
     return false;
 }
 
+// Correctly rounded f32 square root of a finite x >= 0: the hardware root
+// (within 1 ulp for x >= 2^-96) and one correction by the residuals of its
+// two neighbours, the rounding step of the compiler's own sqrtf expansion,
+// without that expansion's scaling of inputs below 2^-96 and its handling of
+// infinities and negatives. Below 2^-96 the result is a tiny value that may
+// differ from sqrtf's; the lane kernel only compares such a root against
+// 1e-6 or subtracts it from a radius, where any value under 2^-40 gives the
+// same outcome.
+__device__ __forceinline__ float sqrt_rn(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float lo = __uint_as_float(__float_as_uint(s) - 1u), hi = __uint_as_float(__float_as_uint(s) + 1u);
+    float r = s;
+    if (__builtin_fmaf(-lo, s, x) <= 0.0f) r = lo;
+    if (__builtin_fmaf(-hi, s, x) > 0.0f) r = hi;
+    return r;
+}
+
 // One lane of the SES kernel (kernel.rs:837-952) at frame-local (x, y, z).
 __device__ inline float ses_value(const Lists& s, const uint32_t* n, uint32_t cc, uint32_t nw, float x, float y,
                                   float z, float probe, float cutoff2) {
@@ -193,75 +210,61 @@ __device__ inline float ses_value(const Lists& s, const uint32_t* n, uint32_t cc
         float d2 = dx * dx + dy * dy + dz * dz;
         best2 = d2 < best2 ? d2 : best2;
     }
-    // Class S and C, four candidates at a time. A candidate is first bounded
-    // from below with the hardware square root (within 1 ulp): l <= the
-    // reference's |dist - R| (or |wl - rho|) by a margin of 1e-6 (dist + R),
-    // which covers that square root's error and the roundings that differ,
-    // so fl(l * l) >= best2 proves the exact test fails. Only candidates the
-    // bound leaves open take the correctly rounded square root, and their
-    // projection and blockers are tested only if the exact value improves on
-    // best2, as before.
+    // Class S and C, four candidates' distances at a time; a candidate's
+    // projection and blockers are tested only if it improves on best2.
     const uint32_t ns = n[C_S], nsc = ns + n[C_C];
     for (uint32_t i0 = 0; i0 < ns; i0 += 4) {
-        float lo2[4];
+        float lb2[4], dist[4];
         for (int j = 0; j < 4; j++) {
             uint32_t i = min(i0 + j, ns - 1);
             float4 c = s.cdat[3 * i];
             float dx = x - c.x, dy = y - c.y, dz = z - c.z;
-            float da = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
-            float l = fmaxf(fabsf(da - c.w) - (da + c.w) * 1e-6f, 0.0f);
-            lo2[j] = i0 + j < ns ? l * l : INFINITY;
+            dist[j] = sqrt_rn(dx * dx + dy * dy + dz * dz);
+            float lb = fabsf(dist[j] - c.w);
+            lb2[j] = i0 + j < ns ? lb * lb : INFINITY;
         }
         for (int j = 0; j < 4; j++) {
-            if (!(lo2[j] < best2)) continue;
+            if (!(lb2[j] < best2 && 1e-6f < dist[j])) continue;
             uint32_t i = i0 + j;
             float4 c = s.cdat[3 * i];
             float dx = x - c.x, dy = y - c.y, dz = z - c.z;
-            float dist = sqrtf(dx * dx + dy * dy + dz * dz);
-            float lb = fabsf(dist - c.w);
-            float lb2 = lb * lb;
-            if (!(lb2 < best2 && 1e-6f < dist)) continue;
-            float sc = c.w / dist;
-            if (!blocked_f(s, cc, nw, i, c.x + dx * sc, c.y + dy * sc, c.z + dz * sc)) best2 = lb2;
+            float sc = c.w / dist[j];
+            if (!blocked_f(s, cc, nw, i, c.x + dx * sc, c.y + dy * sc, c.z + dz * sc)) best2 = lb2[j];
         }
     }
     for (uint32_t c0 = ns; c0 < nsc; c0 += 4) {
-        float lo2[4];
+        float d2[4], wl[4];
         for (int j = 0; j < 4; j++) {
             uint32_t c = min(c0 + j, nsc - 1);
             float4 m = s.cdat[3 * c], u = s.cdat[3 * c + 1];
             float yx = x - m.x, yy = y - m.y, yz = z - m.z;
             float along = yx * u.x + yy * u.y + yz * u.z;
             float wx = yx - u.x * along, wy = yy - u.y * along, wz = yz - u.z * along;
-            float wa = __builtin_amdgcn_sqrtf(wx * wx + wy * wy + wz * wz);
-            float t = fmaxf(fabsf(wa - m.w) - (wa + m.w) * 1e-6f, 0.0f);
-            lo2[j] = c0 + j < nsc ? along * along + t * t : INFINITY;
+            wl[j] = sqrt_rn(wx * wx + wy * wy + wz * wz);
+            float t = wl[j] - m.w;
+            d2[j] = c0 + j < nsc ? along * along + t * t : INFINITY;
         }
         for (int j = 0; j < 4; j++) {
-            if (!(lo2[j] < best2)) continue;
+            if (!(d2[j] < best2)) continue;
             uint32_t c = c0 + j;
             float4 m = s.cdat[3 * c], u = s.cdat[3 * c + 1];
             float rho = m.w;
-            float yx = x - m.x, yy = y - m.y, yz = z - m.z;
-            float along = yx * u.x + yy * u.y + yz * u.z;
-            float wx = yx - u.x * along, wy = yy - u.y * along, wz = yz - u.z * along;
-            float wl = sqrtf(wx * wx + wy * wy + wz * wz);
-            float t = wl - rho;
-            float d2 = along * along + t * t;
-            if (!(d2 < best2)) continue;
             float qx, qy, qz;
-            if (wl < 1e-6f) {
+            if (wl[j] < 1e-6f) {
                 float4 e = s.cdat[3 * c + 2];
                 qx = m.x + e.x * rho;
                 qy = m.y + e.y * rho;
                 qz = m.z + e.z * rho;
             } else {
-                float sc = rho / wl;
+                float yx = x - m.x, yy = y - m.y, yz = z - m.z;
+                float along = yx * u.x + yy * u.y + yz * u.z;
+                float wx = yx - u.x * along, wy = yy - u.y * along, wz = yz - u.z * along;
+                float sc = rho / wl[j];
                 qx = m.x + wx * sc;
                 qy = m.y + wy * sc;
                 qz = m.z + wz * sc;
             }
-            if (!blocked_f(s, cc, nw, c, qx, qy, qz)) best2 = d2;
+            if (!blocked_f(s, cc, nw, c, qx, qy, qz)) best2 = d2[j];
         }
     }
     return probe - sqrtf(best2);
