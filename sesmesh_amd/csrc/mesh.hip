@@ -992,7 +992,10 @@ int wait_stream(smg_device* d, hipStream_t s) {
         if (e == hipSuccess) break;
         if (e != hipErrorNotReady) return fail(SMG_E_CALL, std::string("hipStreamQuery: ") + hipGetErrorString(e));
         if (cancel_requested(d)) raise_cancel(d);
-        std::this_thread::yield();
+        // A short sleep, not a spin: back-to-back stream queries slowed the
+        // kernels they waited for several times over (a 1024-molecule
+        // batch's sampling 0.23 -> 0.7-1.4 s with a control installed).
+        std::this_thread::sleep_for(std::chrono::microseconds(100));
     }
     if (d->raised) return fail(SMG_E_CANCELLED, "cancelled");
     return check_cancel(d);
